@@ -1,0 +1,144 @@
+/*
+ * wfx.h — C-ABI of the MI355X-native explicit updated-Lagrangian step.
+ *
+ * Drop-in boundary for WeldFormFEM's per-step Domain_d API: every per-step
+ * entry point below replaces one MetFEM::Domain_d member function (cited as
+ * reference file:line) and runs as HIP kernels on the given stream.  Plain
+ * pointers and sizes only; every call returns 0 on success or a negative
+ * WFX_ERR_* code, with a message in wfx_last_error().
+ *
+ * State is owned by the context (device structure-of-arrays buffers).  Host
+ * arrays passed to wfx_set_field / wfx_get_field use the reference's own
+ * layout (Domain_d.h:841-1043): nodal vectors [N][3], element tensors
+ * [E][6] as xx,yy,zz,xy,yz,xz, element forces [E][n][3], shape derivatives
+ * [E][n] per direction.
+ *
+ * Threading: one context is used by one host thread at a time; kernels are
+ * asynchronous on the stream passed in (NULL = default stream).
+ */
+#ifndef WFX_H
+#define WFX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wfx_ctx wfx_ctx;
+typedef void *wfx_stream; /* hipStream_t */
+
+enum {
+  WFX_OK = 0,
+  WFX_ERR_ARG = -1,      /* bad argument or size                         */
+  WFX_ERR_STATE = -2,    /* call out of order (no mesh, not initialised) */
+  WFX_ERR_HIP = -3,      /* HIP runtime error                            */
+  WFX_ERR_NONFINITE = -4 /* non-finite value detected                    */
+};
+
+enum { WFX_FP64 = 0, WFX_FP32_MIXED = 1 };
+enum { WFX_BILINEAR = 0, WFX_HOLLOMON = 1 };
+
+const char *wfx_last_error(void);
+int wfx_version(void);
+
+/* ---- lifetime ------------------------------------------------------------ */
+/* new Domain_d (Domain_d.h:238); precision WFX_FP64 or WFX_FP32_MIXED */
+int wfx_create(wfx_ctx **out, int precision);
+/* Domain_d::Free (Domain_d.C:786-900) */
+int wfx_destroy(wfx_ctx *ctx);
+
+/* ---- mesh ---------------------------------------------------------------- */
+/* Domain_d::AddBoxLength (Domain_d.C:1136-1504), 3D, reduced integration:
+ * structured hexa (tritetra=0) or six tets per cube (tritetra=1). */
+int wfx_add_box_length(wfx_ctx *ctx, const double V[3], const double L[3], double r,
+                       int red_int, int tritetra);
+/* Domain_d::CreateFromLSDyna + setNodElem (Domain_d.C:1647-1699, :1508-1611):
+ * x[N][3], elnod[E][nodxelem] (nodxelem 4 or 8); builds the node->element CSR. */
+int wfx_set_mesh(wfx_ctx *ctx, int node_count, int elem_count, int nodxelem,
+                 const double *x, const int *elnod);
+/* getNodeCount / getElemCount / m_nodxelem (Domain_d.h) */
+int wfx_get_sizes(wfx_ctx *ctx, int *node_count, int *elem_count, int *nodxelem);
+
+/* ---- set-up -------------------------------------------------------------- */
+/* Domain_d::setDensity (Domain_d.C:952-958) */
+int wfx_set_density(wfx_ctx *ctx, double rho);
+/* Material_ + Elastic_ + AssignMaterial (Material.cuh:15-104, Domain_d.C:945).
+ * model WFX_BILINEAR: p1 = tangent modulus Et (sigma_y = sy0 + H*ep with
+ * H = E*Et/(E-Et)); model WFX_HOLLOMON: p1 = K, p2 = m (Material.cuh:353-395). */
+int wfx_set_material(wfx_ctx *ctx, int model, double E, double nu, double rho0, double sy0,
+                     double p1, double p2);
+/* per-element initial yield stress (one Material_ per element, mat[e]->sy0) */
+int wfx_set_elem_sy0(wfx_ctx *ctx, const double *sy0, int elem_count);
+/* Domain_d::AddBCVelNode (Domain_d.C:1057-1061) */
+int wfx_add_bc_vel_node(wfx_ctx *ctx, int node, int dim, double val);
+/* Domain_d::AllocateBCs (Domain_d.C:1063-1107): uploads the lists */
+int wfx_allocate_bcs(wfx_ctx *ctx);
+/* Domain_d::SetDT (Domain_d.h) */
+int wfx_set_dt(wfx_ctx *ctx, double dt);
+/* m_alpha, m_beta, m_gamma (Solver_explicit.C:193-197); defaults rho_b=0.8182 */
+int wfx_set_ch(wfx_ctx *ctx, double alpha, double beta, double gamma);
+/* Domain_d::m_stab and m_artifvisc (Domain_d.h:283-294, main.C:84-120) and the
+ * 3D hexa hourglass coefficient ("hg_coeff").  Names: alpha_free, log_factor,
+ * J_min, hg_coeff_free, pspg_scale, p_pspg_bulkfac, av_coeff_div,
+ * av_coeff_bulk, artifvisc0, artifvisc1, hg_coeff. */
+int wfx_set_param(wfx_ctx *ctx, const char *name, double value);
+int wfx_get_param(wfx_ctx *ctx, const char *name, double *value);
+/* nodes whose internal force is summed by wfx_reaction_force (no counterpart
+ * in the reference; it has no reaction output) */
+int wfx_set_reaction_nodes(wfx_ctx *ctx, const int *nodes, int count);
+/* Domain_d::calcMinEdgeLength + getMinLength (Domain_d.C:2224-2469): the
+ * minimum edge of the tet formed by each element's first four nodes,
+ * reduced on the device. */
+int wfx_calc_min_edge_length(wfx_ctx *ctx, double *out, wfx_stream stream);
+
+/* ---- init block of SolveChungHulbert (Solver_explicit.C:101-340) --------- */
+int wfx_init(wfx_ctx *ctx, wfx_stream stream);
+
+/* ---- per-step API (Solver_explicit.C:524-978 call order) ----------------- */
+int wfx_update_prediction(wfx_ctx *ctx, wfx_stream s);          /* Domain_d.C:961-974   */
+int wfx_impose_bcv(wfx_ctx *ctx, int dim, wfx_stream s);        /* Domain_d.C:1109-1122 */
+int wfx_calc_elem_jacobians(wfx_ctx *ctx, wfx_stream s);        /* Domain_d.C:1701-2044 */
+int wfx_calc_elem_vol(wfx_ctx *ctx, wfx_stream s);              /* Mechanical.C:264-293 */
+int wfx_calc_elem_density(wfx_ctx *ctx, wfx_stream s);          /* Mechanical.C:295-307 */
+int wfx_calc_nodal_vol(wfx_ctx *ctx, wfx_stream s);             /* Mechanical.C:1555-1572 */
+int wfx_calc_nodal_mass_from_vol(wfx_ctx *ctx, wfx_stream s);   /* Mechanical.C:1576-1601 */
+int wfx_calc_elem_strain_rates(wfx_ctx *ctx, wfx_stream s);     /* Mechanical.C:41-126  */
+int wfx_calc_elem_pressure(wfx_ctx *ctx, wfx_stream s);         /* Mechanical.C:691-819 */
+int wfx_calc_stress_strain(wfx_ctx *ctx, double dt, wfx_stream s); /* Mechanical.C:1664-1839 */
+int wfx_calc_artificial_viscosity(wfx_ctx *ctx, wfx_stream s);  /* Mechanical.C:1948-1977 */
+int wfx_calc_elem_forces(wfx_ctx *ctx, wfx_stream s);           /* Mechanical.C:375-481 */
+/* Mechanical.C:1842-1943 (3D hexa branch per f90_ver/src/Mechanical.f90:241-303) */
+int wfx_calc_elem_hourglass_forces(wfx_ctx *ctx, wfx_stream s);
+/* Matrices.C:42-87, plus the non-finite scrub of Solver_explicit.C:777-785 */
+int wfx_assembly_forces(wfx_ctx *ctx, wfx_stream s);
+int wfx_calc_accel(wfx_ctx *ctx, wfx_stream s);                 /* Mechanical.C:321-341 */
+int wfx_impose_bca(wfx_ctx *ctx, int dim, wfx_stream s);        /* Domain_d.C:1124-1134 */
+int wfx_update_correction_accvel(wfx_ctx *ctx, wfx_stream s);   /* Domain_d.C:981-997   */
+int wfx_update_correction_pos(wfx_ctx *ctx, wfx_stream s);      /* Domain_d.C:1005-1025 */
+
+/* ---- composed ------------------------------------------------------------ */
+/* One SolveChungHulbert loop body through the per-step entry points above,
+ * in the reference order (Solver_explicit.C:524-978). */
+int wfx_step_api(wfx_ctx *ctx, wfx_stream s);
+/* nsteps loop bodies; reaction-force history (length nsteps, host memory,
+ * may be NULL) is copied back at the end. */
+int wfx_run(wfx_ctx *ctx, int nsteps, double *reaction_hist, wfx_stream s);
+/* Sum of internal force component dim over the reaction nodes (synchronous). */
+int wfx_reaction_force(wfx_ctx *ctx, int dim, double *out, wfx_stream s);
+/* Domain_d::computeEnergies (Mechanical.C:2145-2185), synchronous. */
+int wfx_energies(wfx_ctx *ctx, double *ekin, double *deint, wfx_stream s);
+/* number of non-finite internal-force entries scrubbed so far */
+int wfx_nonfinite_count(wfx_ctx *ctx, long *out);
+
+/* ---- state transfer (reference layout on the host side, synchronous) ----- */
+/* names: x v a u u_dt prev_a fi fe mdiag voln elnod nodel nodel_loc
+ * nodel_offset nodel_count detJ dHdx dHdy dHdz vol vol0 rho rho0 p pl_strain
+ * sigma_y sy0 str_rate rot_rate sigma tau eps strain_pl_incr f_elem f_hg.
+ * Integer fields take int32 buffers, the rest double buffers. */
+int wfx_field_count(wfx_ctx *ctx, const char *name, long *count, int *is_int);
+int wfx_get_field(wfx_ctx *ctx, const char *name, void *host, long count);
+int wfx_set_field(wfx_ctx *ctx, const char *name, const void *host, long count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,63 @@
+"""Build recipe of the wfx HIP library (weldformfem_amd/libwfx.so) for gfx950.
+
+Plain hipcc/g++ invocations, no build system.  The FP64 kernel unit is built
+with -ffp-contract=off so its expressions round like the reference's CPU
+code; the FP32-mixed unit allows contraction.  Every HIP unit is built
+without packed-FP32 instructions.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libwfx.so")
+BUILD = os.path.join(HERE, "_build")
+ARCH = os.environ.get("WFX_OFFLOAD_ARCH", "gfx950")
+
+HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+UNITS = [
+    ("wfx_k64.hip", ["-ffp-contract=off"]),
+    ("wfx_k32.hip", []),
+    ("wfx_api.hip", ["-ffp-contract=off"]),
+]
+
+
+def _hipcc():
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+
+
+def build(verbose=False):
+    hipcc = _hipcc()
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for src, extra in UNITS:
+        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        cmd = [hipcc] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(obj)
+    obj = os.path.join(BUILD, "wfx_mesh.o")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, "wfx_mesh.cpp"), "-o", obj])
+    objs.append(obj)
+    tmp = OUT + ".tmp"
+    _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,715 @@
+// C-ABI of the wfx library: context lifetime, mesh and set-up, the per-step
+// Domain_d API and state transfer.  Kernels live in wfx_k64.hip / wfx_k32.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/wfx.h"
+#include "wfx_internal.hpp"
+#include "wfx_launch.hpp"
+
+namespace wfx {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+int box_length_mesh(const double V[3], const double L[3], double r, bool tritetra,
+                    std::vector<double> &x, std::vector<int> &elnod, int &nodxelem);
+int build_nodel(int N, int E, int n, const int *elnod, std::vector<int> &nodel,
+                std::vector<int> &nodel_loc, std::vector<int> &offset, std::vector<int> &count);
+
+template <typename T>
+static hipError_t dalloc(wfx_ctx *c, T **p, size_t count) {
+  void *q = nullptr;
+  size_t bytes = (count > 0 ? count : 1) * sizeof(T);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return e;
+  c->allocations.push_back(q);
+  e = hipMemset(q, 0, bytes);
+  *p = (T *)q;
+  return e;
+}
+
+static void free_all(wfx_ctx *c) {
+  for (void *p : c->allocations) (void)hipFree(p);
+  c->allocations.clear();
+  c->nd = DevNode();
+  c->ms = DevMesh();
+  c->e64 = DevElem<double>();
+  c->e32 = DevElem<float>();
+  for (int d = 0; d < 3; ++d) { c->bc_nod[d] = nullptr; c->bc_val[d] = nullptr; c->bc_count[d] = 0; }
+  c->react = nullptr;
+  c->scalars = nullptr;
+  c->status = nullptr;
+  c->hist = nullptr;
+  c->hist_cap = 0;
+}
+
+template <typename T>
+static hipError_t alloc_elem(wfx_ctx *c, DevElem<T> &el) {
+  const size_t E = c->E, n = c->n;
+  hipError_t e = hipSuccess;
+#define A(ptr, cnt) \
+  if (e == hipSuccess) e = dalloc(c, &el.ptr, (cnt));
+  A(detJ, E) A(dH, 3 * n * E) A(vol, E) A(vol0, E) A(rho, E) A(rho0, E) A(p, E) A(pl_strain, E)
+  A(sigma_y, E) A(sy0, E) A(str_rate, 6 * E) A(rot_rate, 3 * E) A(sigma, 6 * E) A(tau, 6 * E)
+  A(eps, 6 * E) A(strain_pl_incr, 6 * E) A(f_elem, 3 * n * E) A(f_hg, 3 * n * E)
+#undef A
+  return e;
+}
+
+// Allocates every device buffer for the mesh held on the host and uploads
+// coordinates, connectivity and CSR.
+static int setup_device(wfx_ctx *c) {
+  free_all(c);
+  const size_t N = c->N, E = c->E, n = c->n;
+  hipError_t e = hipSuccess;
+#define A(ptr, cnt) \
+  if (e == hipSuccess) e = dalloc(c, &ptr, (cnt));
+  A(c->nd.x, 3 * N) A(c->nd.v, 3 * N) A(c->nd.a, 3 * N) A(c->nd.u, 3 * N) A(c->nd.u_dt, 3 * N)
+  A(c->nd.prev_a, 3 * N) A(c->nd.fi, 3 * N) A(c->nd.fe, 3 * N) A(c->nd.mdiag, N) A(c->nd.voln, N)
+  A(c->nd.voln_sum, N) A(c->nd.voln0_sum, N)
+  A(c->ms.elnod, n * E) A(c->ms.nodel, n * E) A(c->ms.nodel_loc, n * E) A(c->ms.nodel_offset, N)
+  A(c->ms.nodel_count, N) A(c->scalars, 64) A(c->status, 16)
+#undef A
+  if (e == hipSuccess) e = (c->precision == WFX_FP64) ? alloc_elem(c, c->e64) : alloc_elem(c, c->e32);
+  if (e != hipSuccess) {
+    set_error(std::string("device allocation: ") + hipGetErrorString(e));
+    free_all(c);
+    return WFX_ERR_HIP;
+  }
+  // coordinates [N][3] -> [3][N]; connectivity [E][n] -> [n][E]
+  std::vector<double> xs(3 * N);
+  for (size_t i = 0; i < N; ++i)
+    for (int d = 0; d < 3; ++d) xs[d * N + i] = c->x_h[3 * i + d];
+  std::vector<int> es(n * E);
+  for (size_t k = 0; k < E; ++k)
+    for (size_t a = 0; a < n; ++a) es[a * E + k] = c->elnod_h[k * n + a];
+  WFX_CHECK_HIP(hipMemcpy(c->nd.x, xs.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(c->ms.elnod, es.data(), n * E * sizeof(int), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(c->ms.nodel, c->nodel_h.data(), n * E * sizeof(int), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_loc, c->nodel_loc_h.data(), n * E * sizeof(int), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_offset, c->nodel_offset_h.data(), N * sizeof(int), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_count, c->nodel_count_h.data(), N * sizeof(int), hipMemcpyHostToDevice));
+  c->has_mesh = true;
+  c->initialized = false;
+  return WFX_OK;
+}
+
+static int finish_mesh(wfx_ctx *c) {
+  c->nnz = c->E * c->n;
+  if (build_nodel(c->N, c->E, c->n, c->elnod_h.data(), c->nodel_h, c->nodel_loc_h, c->nodel_offset_h,
+                  c->nodel_count_h)) {
+    set_error("element connectivity references a node outside [0, node_count)");
+    return WFX_ERR_ARG;
+  }
+  c->sy0_h.assign(c->E, c->prm.mat.sy0);
+  int rc = setup_device(c);
+  if (rc) return rc;
+  c->prm.N = c->N;
+  c->prm.E = c->E;
+  c->prm.n = c->n;
+  return WFX_OK;
+}
+
+// host <-> device element-real conversion for the two precisions
+template <typename T>
+static int upload_elem_real(T *dst, const double *src, size_t count) {
+  if constexpr (std::is_same<T, double>::value) {
+    WFX_CHECK_HIP(hipMemcpy(dst, src, count * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> tmp(src, src + count);
+    WFX_CHECK_HIP(hipMemcpy(dst, tmp.data(), count * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return WFX_OK;
+}
+
+static int upload_real(wfx_ctx *c, void *dst, const double *src, size_t count, bool elem_type) {
+  if (!elem_type || c->precision == WFX_FP64) return upload_elem_real<double>((double *)dst, src, count);
+  return upload_elem_real<float>((float *)dst, src, count);
+}
+
+static int download_real(wfx_ctx *c, double *dst, const void *src, size_t count, bool elem_type) {
+  if (!elem_type || c->precision == WFX_FP64) {
+    WFX_CHECK_HIP(hipMemcpy(dst, src, count * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(count);
+    WFX_CHECK_HIP(hipMemcpy(tmp.data(), src, count * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < count; ++i) dst[i] = tmp[i];
+  }
+  return WFX_OK;
+}
+
+// Description of one transferable field.  Host layout is [cnt][rows] (the
+// reference's), device layout is [rows][cnt] starting at dev.
+struct FieldDesc {
+  void *dev = nullptr;
+  long cnt = 0;    // items (nodes, elements, or entries)
+  int rows = 1;    // components per item on the host
+  int is_int = 0;  // int32 field
+  int elem = 0;    // element real type (T) rather than double
+  int kind = 0;    // 0 plain transpose, 1 spin (3 device rows -> 6 host columns)
+};
+
+static bool find_field(wfx_ctx *c, const char *name, FieldDesc &f) {
+  const long N = c->N, E = c->E, n = c->n;
+  const bool d64 = (c->precision == WFX_FP64);
+  auto E64 = [&](void *p64, void *p32) { return d64 ? p64 : p32; };
+  std::string s(name);
+  f = FieldDesc();
+  auto node3 = [&](double *p) { f.dev = p; f.cnt = N; f.rows = 3; };
+  auto node1 = [&](double *p) { f.dev = p; f.cnt = N; f.rows = 1; };
+  auto el = [&](void *p, int rows) { f.dev = p; f.cnt = E; f.rows = rows; f.elem = 1; };
+  if (s == "x") node3(c->nd.x);
+  else if (s == "v") node3(c->nd.v);
+  else if (s == "a") node3(c->nd.a);
+  else if (s == "u") node3(c->nd.u);
+  else if (s == "u_dt") node3(c->nd.u_dt);
+  else if (s == "prev_a") node3(c->nd.prev_a);
+  else if (s == "fi") node3(c->nd.fi);
+  else if (s == "fe") node3(c->nd.fe);
+  else if (s == "mdiag") node1(c->nd.mdiag);
+  else if (s == "voln") node1(c->nd.voln);
+  else if (s == "elnod") { f.dev = c->ms.elnod; f.cnt = E; f.rows = (int)n; f.is_int = 1; }
+  else if (s == "nodel") { f.dev = c->ms.nodel; f.cnt = E * n; f.is_int = 1; }
+  else if (s == "nodel_loc") { f.dev = c->ms.nodel_loc; f.cnt = E * n; f.is_int = 1; }
+  else if (s == "nodel_offset") { f.dev = c->ms.nodel_offset; f.cnt = N; f.is_int = 1; }
+  else if (s == "nodel_count") { f.dev = c->ms.nodel_count; f.cnt = N; f.is_int = 1; }
+  else if (s == "detJ") el(E64(c->e64.detJ, c->e32.detJ), 1);
+  else if (s == "dHdx" || s == "dHdy" || s == "dHdz") {
+    long d = s[3] - 'x';
+    void *base = d64 ? (void *)(c->e64.dH + d * n * E) : (void *)(c->e32.dH + d * n * E);
+    el(base, (int)n);
+  }
+  else if (s == "vol") el(E64(c->e64.vol, c->e32.vol), 1);
+  else if (s == "vol0") el(E64(c->e64.vol0, c->e32.vol0), 1);
+  else if (s == "rho") el(E64(c->e64.rho, c->e32.rho), 1);
+  else if (s == "rho0") el(E64(c->e64.rho0, c->e32.rho0), 1);
+  else if (s == "p") el(E64(c->e64.p, c->e32.p), 1);
+  else if (s == "pl_strain") el(E64(c->e64.pl_strain, c->e32.pl_strain), 1);
+  else if (s == "sigma_y") el(E64(c->e64.sigma_y, c->e32.sigma_y), 1);
+  else if (s == "sy0") el(E64(c->e64.sy0, c->e32.sy0), 1);
+  else if (s == "str_rate") el(E64(c->e64.str_rate, c->e32.str_rate), 6);
+  else if (s == "rot_rate") { el(E64(c->e64.rot_rate, c->e32.rot_rate), 6); f.kind = 1; }
+  else if (s == "sigma") el(E64(c->e64.sigma, c->e32.sigma), 6);
+  else if (s == "tau") el(E64(c->e64.tau, c->e32.tau), 6);
+  else if (s == "eps") el(E64(c->e64.eps, c->e32.eps), 6);
+  else if (s == "strain_pl_incr") el(E64(c->e64.strain_pl_incr, c->e32.strain_pl_incr), 6);
+  else if (s == "f_elem") el(E64(c->e64.f_elem, c->e32.f_elem), (int)(3 * n));
+  else if (s == "f_hg") el(E64(c->e64.f_hg, c->e32.f_hg), (int)(3 * n));
+  else return false;
+  return true;
+}
+
+static hipStream_t as_stream(wfx_stream s) { return (hipStream_t)s; }
+
+}  // namespace wfx
+
+using namespace wfx;
+
+#define REQUIRE(cond, code, msg) \
+  do {                           \
+    if (!(cond)) {               \
+      set_error(msg);            \
+      return code;               \
+    }                            \
+  } while (0)
+#define REQUIRE_CTX(c) REQUIRE((c) != nullptr, WFX_ERR_ARG, "null context")
+#define REQUIRE_MESH(c)  \
+  REQUIRE_CTX(c);        \
+  REQUIRE((c)->has_mesh, WFX_ERR_STATE, "no mesh: call wfx_add_box_length or wfx_set_mesh first")
+#define LAUNCH(expr)                                                        \
+  do {                                                                      \
+    hipError_t _e = (expr);                                                 \
+    if (_e != hipSuccess) {                                                 \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(_e));        \
+      return WFX_ERR_HIP;                                                   \
+    }                                                                       \
+  } while (0)
+#define BY_PRECISION(c, fn, ...) \
+  ((c)->precision == WFX_FP64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+extern "C" {
+
+const char *wfx_last_error(void) { return g_err.c_str(); }
+int wfx_version(void) { return 1; }
+
+int wfx_create(wfx_ctx **out, int precision) {
+  REQUIRE(out != nullptr, WFX_ERR_ARG, "null output pointer");
+  REQUIRE(precision == WFX_FP64 || precision == WFX_FP32_MIXED, WFX_ERR_ARG, "unknown precision");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error("no HIP device available");
+    return WFX_ERR_HIP;
+  }
+  wfx_ctx *c = new wfx_ctx();
+  c->precision = precision;
+  double rho_b = 0.818200;  // Solver_explicit.C:193-197
+  c->prm.alpha = (2.0 * rho_b - 1.0) / (1.0 + rho_b);
+  c->prm.beta = (5.0 - 3.0 * rho_b) / ((1.0 + rho_b) * (1.0 + rho_b) * (2.0 - rho_b));
+  c->prm.gamma = 1.5 - c->prm.alpha;
+  *out = c;
+  return WFX_OK;
+}
+
+int wfx_destroy(wfx_ctx *c) {
+  if (!c) return WFX_OK;
+  free_all(c);
+  delete c;
+  return WFX_OK;
+}
+
+int wfx_add_box_length(wfx_ctx *c, const double V[3], const double L[3], double r, int red_int,
+                       int tritetra) {
+  REQUIRE_CTX(c);
+  REQUIRE(V && L && r > 0.0, WFX_ERR_ARG, "bad box arguments");
+  REQUIRE(L[2] > 0.0, WFX_ERR_ARG, "2D boxes are not supported (L.z must be > 0)");
+  REQUIRE(red_int, WFX_ERR_ARG, "only reduced integration is supported");
+  int nodxelem = 0;
+  if (box_length_mesh(V, L, r, tritetra != 0, c->x_h, c->elnod_h, nodxelem)) {
+    set_error("box mesh: no elements or sizes beyond int32");
+    return WFX_ERR_ARG;
+  }
+  c->n = nodxelem;
+  c->N = (int)(c->x_h.size() / 3);
+  c->E = (int)(c->elnod_h.size() / nodxelem);
+  return finish_mesh(c);
+}
+
+int wfx_set_mesh(wfx_ctx *c, int node_count, int elem_count, int nodxelem, const double *x,
+                 const int *elnod) {
+  REQUIRE_CTX(c);
+  REQUIRE(node_count > 0 && elem_count > 0 && x && elnod, WFX_ERR_ARG, "bad mesh arguments");
+  REQUIRE(nodxelem == 4 || nodxelem == 8, WFX_ERR_ARG, "nodxelem must be 4 (tet) or 8 (hexa)");
+  REQUIRE((long)elem_count * nodxelem < 2147483647L, WFX_ERR_ARG, "mesh too large for int32 CSR");
+  c->n = nodxelem;
+  c->N = node_count;
+  c->E = elem_count;
+  c->x_h.assign(x, x + 3L * node_count);
+  c->elnod_h.assign(elnod, elnod + (long)elem_count * nodxelem);
+  return finish_mesh(c);
+}
+
+int wfx_get_sizes(wfx_ctx *c, int *N, int *E, int *n) {
+  REQUIRE_CTX(c);
+  if (N) *N = c->N;
+  if (E) *E = c->E;
+  if (n) *n = c->n;
+  return WFX_OK;
+}
+
+int wfx_set_density(wfx_ctx *c, double rho) {
+  REQUIRE_MESH(c);
+  std::vector<double> h(c->E, rho);
+  void *dst = c->precision == WFX_FP64 ? (void *)c->e64.rho0 : (void *)c->e32.rho0;
+  return upload_real(c, dst, h.data(), h.size(), true);
+}
+
+int wfx_set_material(wfx_ctx *c, int model, double E, double nu, double rho0, double sy0, double p1,
+                     double p2) {
+  REQUIRE_CTX(c);
+  REQUIRE(model == WFX_BILINEAR || model == WFX_HOLLOMON, WFX_ERR_ARG, "unknown material model");
+  REQUIRE(E > 0.0 && nu > -1.0 && nu < 0.5 && rho0 > 0.0, WFX_ERR_ARG, "bad elastic constants");
+  Material m;
+  m.model = model;
+  m.E = E;
+  m.nu = nu;
+  m.K = E / (3.0 * (1.0 - 2.0 * nu));  // Material.cuh:22-27
+  m.G = E / (2.0 * (1.0 + nu));
+  m.rho0 = rho0;
+  m.cs0 = std::sqrt(m.K / rho0);
+  m.sy0 = sy0;
+  if (model == WFX_BILINEAR) {
+    REQUIRE(p1 >= 0.0 && p1 < E, WFX_ERR_ARG, "bilinear tangent modulus must be in [0, E)");
+    m.Et = p1;
+    m.H = E * p1 / (E - p1);
+  } else {
+    REQUIRE(p1 > 0.0 && p2 > 0.0, WFX_ERR_ARG, "Hollomon K and m must be positive");
+    m.Kh = p1;
+    m.mh = p2;
+    m.eps0 = sy0 / E;  // Material.cuh:95-98
+    m.eps1 = std::pow(sy0 / p1, 1. / p2);
+  }
+  c->prm.mat = m;
+  if (c->has_mesh) {
+    c->sy0_h.assign(c->E, sy0);
+    void *dst = c->precision == WFX_FP64 ? (void *)c->e64.sy0 : (void *)c->e32.sy0;
+    return upload_real(c, dst, c->sy0_h.data(), c->sy0_h.size(), true);
+  }
+  return WFX_OK;
+}
+
+int wfx_set_elem_sy0(wfx_ctx *c, const double *sy0, int count) {
+  REQUIRE_MESH(c);
+  REQUIRE(sy0 && count == c->E, WFX_ERR_ARG, "sy0 needs one value per element");
+  c->sy0_h.assign(sy0, sy0 + count);
+  void *dst = c->precision == WFX_FP64 ? (void *)c->e64.sy0 : (void *)c->e32.sy0;
+  return upload_real(c, dst, sy0, count, true);
+}
+
+int wfx_add_bc_vel_node(wfx_ctx *c, int node, int dim, double val) {
+  REQUIRE_MESH(c);
+  REQUIRE(dim >= 0 && dim < 3, WFX_ERR_ARG, "dim must be 0, 1 or 2");
+  REQUIRE(node >= 0 && node < c->N, WFX_ERR_ARG, "node out of range");
+  c->bc_nod_h[dim].push_back(node);
+  c->bc_val_h[dim].push_back(val);
+  return WFX_OK;
+}
+
+int wfx_allocate_bcs(wfx_ctx *c) {
+  REQUIRE_MESH(c);
+  for (int d = 0; d < 3; ++d) {
+    // the reference applies its list serially, so for a node listed twice the
+    // last value wins; keep exactly that entry (first-occurrence order kept)
+    std::vector<int> last(c->N, -1);
+    const auto &nod = c->bc_nod_h[d];
+    for (size_t k = 0; k < nod.size(); ++k) last[nod[k]] = (int)k;
+    std::vector<int> dn;
+    std::vector<double> dv;
+    for (size_t k = 0; k < nod.size(); ++k)
+      if (last[nod[k]] >= 0) {
+        dn.push_back(nod[k]);
+        dv.push_back(c->bc_val_h[d][last[nod[k]]]);
+        last[nod[k]] = -1;
+      }
+    int cnt = (int)dn.size();
+    if (cnt > 0) {
+      int *pn = nullptr;
+      double *pv = nullptr;
+      WFX_CHECK_HIP(dalloc(c, &pn, cnt));
+      WFX_CHECK_HIP(dalloc(c, &pv, cnt));
+      WFX_CHECK_HIP(hipMemcpy(pn, dn.data(), cnt * sizeof(int), hipMemcpyHostToDevice));
+      WFX_CHECK_HIP(hipMemcpy(pv, dv.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
+      c->bc_nod[d] = pn;
+      c->bc_val[d] = pv;
+    }
+    c->bc_count[d] = cnt;
+  }
+  return WFX_OK;
+}
+
+int wfx_set_dt(wfx_ctx *c, double dt) {
+  REQUIRE_CTX(c);
+  REQUIRE(dt > 0.0 && std::isfinite(dt), WFX_ERR_ARG, "dt must be positive");
+  c->prm.dt = dt;
+  return WFX_OK;
+}
+
+int wfx_set_ch(wfx_ctx *c, double alpha, double beta, double gamma) {
+  REQUIRE_CTX(c);
+  REQUIRE(alpha != 1.0, WFX_ERR_ARG, "alpha must differ from 1");
+  c->prm.alpha = alpha;
+  c->prm.beta = beta;
+  c->prm.gamma = gamma;
+  return WFX_OK;
+}
+
+static double *param_slot(wfx_ctx *c, const char *name) {
+  Stab &s = c->prm.stab;
+  struct { const char *nm; double *p; } t[] = {
+      {"alpha_free", &s.alpha_free}, {"log_factor", &s.log_factor}, {"J_min", &s.J_min},
+      {"hg_coeff_free", &s.hg_coeff_free}, {"pspg_scale", &s.pspg_scale},
+      {"p_pspg_bulkfac", &s.p_pspg_bulkfac}, {"av_coeff_div", &s.av_coeff_div},
+      {"av_coeff_bulk", &s.av_coeff_bulk}, {"artifvisc0", &s.artifvisc0},
+      {"artifvisc1", &s.artifvisc1}, {"hg_coeff", &s.hg_coeff}, {"dt", &c->prm.dt},
+      {"alpha", &c->prm.alpha}, {"beta", &c->prm.beta}, {"gamma", &c->prm.gamma}};
+  for (auto &e : t)
+    if (!std::strcmp(e.nm, name)) return e.p;
+  return nullptr;
+}
+
+int wfx_set_param(wfx_ctx *c, const char *name, double value) {
+  REQUIRE_CTX(c);
+  REQUIRE(name, WFX_ERR_ARG, "null name");
+  double *p = param_slot(c, name);
+  REQUIRE(p, WFX_ERR_ARG, std::string("unknown parameter ") + name);
+  *p = value;
+  return WFX_OK;
+}
+
+int wfx_get_param(wfx_ctx *c, const char *name, double *value) {
+  REQUIRE_CTX(c);
+  REQUIRE(name && value, WFX_ERR_ARG, "null argument");
+  std::string s(name);
+  const Material &m = c->prm.mat;
+  if (s == "K") { *value = m.K; return WFX_OK; }
+  if (s == "G") { *value = m.G; return WFX_OK; }
+  if (s == "cs0") { *value = m.cs0; return WFX_OK; }
+  if (s == "H") { *value = m.H; return WFX_OK; }
+  double *p = param_slot(c, name);
+  REQUIRE(p, WFX_ERR_ARG, std::string("unknown parameter ") + name);
+  *value = *p;
+  return WFX_OK;
+}
+
+int wfx_set_reaction_nodes(wfx_ctx *c, const int *nodes, int count) {
+  REQUIRE_MESH(c);
+  REQUIRE(count >= 0 && (count == 0 || nodes), WFX_ERR_ARG, "bad node list");
+  for (int k = 0; k < count; ++k) REQUIRE(nodes[k] >= 0 && nodes[k] < c->N, WFX_ERR_ARG, "reaction node out of range");
+  c->react_h.assign(nodes, nodes + count);
+  int *p = nullptr;
+  WFX_CHECK_HIP(dalloc(c, &p, count));
+  if (count) WFX_CHECK_HIP(hipMemcpy(p, nodes, count * sizeof(int), hipMemcpyHostToDevice));
+  c->react = p;
+  return WFX_OK;
+}
+
+int wfx_calc_min_edge_length(wfx_ctx *c, double *out, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(out, WFX_ERR_ARG, "null output");
+  double *partial = nullptr;
+  WFX_CHECK_HIP(hipMalloc(&partial, sizeof(double) * (min_edge_partial_count(c) + 1)));
+  hipError_t e = launch_min_edge(c, partial, c->scalars + 8, as_stream(s));
+  if (e == hipSuccess) e = hipMemcpyAsync(out, c->scalars + 8, sizeof(double), hipMemcpyDeviceToHost, as_stream(s));
+  if (e == hipSuccess) e = hipStreamSynchronize(as_stream(s));
+  (void)hipFree(partial);
+  WFX_CHECK_HIP(e);
+  return WFX_OK;
+}
+
+// ---- init block -----------------------------------------------------------
+int wfx_init(wfx_ctx *c, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->prm.mat.E > 0.0, WFX_ERR_STATE, "no material: call wfx_set_material");
+  hipStream_t st = as_stream(s);
+  const size_t N = c->N, E = c->E;
+  // InitValues (Domain_d.C:393-416) plus zero state; every buffer the
+  // reference leaves uninitialised starts at zero here.
+  for (double *p : {c->nd.v, c->nd.a, c->nd.u, c->nd.u_dt, c->nd.prev_a, c->nd.fi})
+    WFX_CHECK_HIP(hipMemsetAsync(p, 0, 3 * N * sizeof(double), st));
+  size_t es = c->precision == WFX_FP64 ? sizeof(double) : sizeof(float);
+  auto zero_e = [&](void *p, size_t rows) { return hipMemsetAsync(p, 0, rows * E * es, st); };
+  if (c->precision == WFX_FP64) {
+    auto &el = c->e64;
+    for (auto pr : {std::make_pair((void *)el.p, 1), {(void *)el.pl_strain, 1}, {(void *)el.tau, 6},
+                    {(void *)el.eps, 6}, {(void *)el.sigma, 6}, {(void *)el.strain_pl_incr, 6},
+                    {(void *)el.f_hg, 3 * c->n}})
+      WFX_CHECK_HIP(zero_e(pr.first, pr.second));
+    WFX_CHECK_HIP(hipMemcpyAsync(el.sigma_y, el.sy0, E * es, hipMemcpyDeviceToDevice, st));
+  } else {
+    auto &el = c->e32;
+    for (auto pr : {std::make_pair((void *)el.p, 1), {(void *)el.pl_strain, 1}, {(void *)el.tau, 6},
+                    {(void *)el.eps, 6}, {(void *)el.sigma, 6}, {(void *)el.strain_pl_incr, 6},
+                    {(void *)el.f_hg, 3 * c->n}})
+      WFX_CHECK_HIP(zero_e(pr.first, pr.second));
+    WFX_CHECK_HIP(hipMemcpyAsync(el.sigma_y, el.sy0, E * es, hipMemcpyDeviceToDevice, st));
+  }
+  for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
+  LAUNCH(BY_PRECISION(c, launch_elem_jacobians, c, st));
+  LAUNCH(BY_PRECISION(c, launch_elem_vol, c, st));
+  void *vol = c->precision == WFX_FP64 ? (void *)c->e64.vol : (void *)c->e32.vol;
+  void *vol0 = c->precision == WFX_FP64 ? (void *)c->e64.vol0 : (void *)c->e32.vol0;
+  WFX_CHECK_HIP(hipMemcpyAsync(vol0, vol, E * es, hipMemcpyDeviceToDevice, st));
+  LAUNCH(BY_PRECISION(c, launch_elem_density, c, st));
+  LAUNCH(BY_PRECISION(c, launch_nodal_vol, c, st));
+  LAUNCH(BY_PRECISION(c, launch_nodal_mass, c, st));
+  WFX_CHECK_HIP(hipMemsetAsync(c->status, 0, 16 * sizeof(unsigned int), st));
+  WFX_CHECK_HIP(hipStreamSynchronize(st));
+  c->initialized = true;
+  return WFX_OK;
+}
+
+// ---- per-step API -----------------------------------------------------------
+#define STEP_FN(name, call)                  \
+  int name(wfx_ctx *c, wfx_stream s) {       \
+    REQUIRE_MESH(c);                         \
+    hipStream_t st = as_stream(s);           \
+    (void)st;                                \
+    LAUNCH(call);                            \
+    return WFX_OK;                           \
+  }
+
+STEP_FN(wfx_update_prediction, launch_update_prediction(c, st))
+STEP_FN(wfx_calc_elem_jacobians, BY_PRECISION(c, launch_elem_jacobians, c, st))
+STEP_FN(wfx_calc_elem_vol, BY_PRECISION(c, launch_elem_vol, c, st))
+STEP_FN(wfx_calc_elem_density, BY_PRECISION(c, launch_elem_density, c, st))
+STEP_FN(wfx_calc_nodal_vol, BY_PRECISION(c, launch_nodal_vol, c, st))
+STEP_FN(wfx_calc_nodal_mass_from_vol, BY_PRECISION(c, launch_nodal_mass, c, st))
+STEP_FN(wfx_calc_elem_strain_rates, BY_PRECISION(c, launch_strain_rates, c, st))
+STEP_FN(wfx_calc_elem_pressure, BY_PRECISION(c, launch_pressure, c, st))
+STEP_FN(wfx_calc_artificial_viscosity, BY_PRECISION(c, launch_artificial_viscosity, c, st))
+STEP_FN(wfx_calc_elem_forces, BY_PRECISION(c, launch_elem_forces, c, st))
+STEP_FN(wfx_calc_elem_hourglass_forces, BY_PRECISION(c, launch_hourglass, c, st))
+STEP_FN(wfx_assembly_forces, BY_PRECISION(c, launch_assembly, c, st))
+STEP_FN(wfx_calc_accel, launch_calc_accel(c, st))
+STEP_FN(wfx_update_correction_accvel, launch_correction_accvel(c, st))
+STEP_FN(wfx_update_correction_pos, launch_correction_pos(c, st))
+
+int wfx_calc_stress_strain(wfx_ctx *c, double dt, wfx_stream s) {
+  REQUIRE_MESH(c);
+  LAUNCH(BY_PRECISION(c, launch_stress_strain, c, dt, as_stream(s)));
+  return WFX_OK;
+}
+
+int wfx_impose_bcv(wfx_ctx *c, int dim, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(dim >= 0 && dim < 3, WFX_ERR_ARG, "dim must be 0, 1 or 2");
+  LAUNCH(launch_impose(c, dim, false, as_stream(s)));
+  return WFX_OK;
+}
+
+int wfx_impose_bca(wfx_ctx *c, int dim, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(dim >= 0 && dim < 3, WFX_ERR_ARG, "dim must be 0, 1 or 2");
+  LAUNCH(launch_impose(c, dim, true, as_stream(s)));
+  return WFX_OK;
+}
+
+// One SolveChungHulbert loop body (Solver_explicit.C:524-978, CPU order).
+// When hist is given, the reaction force after assembly goes to *hist.
+static int step_api(wfx_ctx *c, hipStream_t st, double *hist) {
+  LAUNCH(launch_update_prediction(c, st));
+  for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
+  LAUNCH(BY_PRECISION(c, launch_elem_jacobians, c, st));
+  LAUNCH(BY_PRECISION(c, launch_elem_vol, c, st));
+  LAUNCH(BY_PRECISION(c, launch_nodal_vol, c, st));
+  LAUNCH(BY_PRECISION(c, launch_nodal_mass, c, st));
+  LAUNCH(BY_PRECISION(c, launch_strain_rates, c, st));
+  LAUNCH(BY_PRECISION(c, launch_pressure, c, st));
+  LAUNCH(BY_PRECISION(c, launch_stress_strain, c, c->prm.dt, st));
+  LAUNCH(BY_PRECISION(c, launch_artificial_viscosity, c, st));
+  LAUNCH(BY_PRECISION(c, launch_elem_forces, c, st));
+  LAUNCH(BY_PRECISION(c, launch_hourglass, c, st));
+  LAUNCH(BY_PRECISION(c, launch_assembly, c, st));
+  if (hist && !c->react_h.empty()) LAUNCH(launch_reaction(c, 2, hist, st));
+  LAUNCH(launch_calc_accel(c, st));
+  for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, true, st));
+  LAUNCH(launch_correction_accvel(c, st));
+  for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
+  LAUNCH(launch_correction_pos(c, st));
+  return WFX_OK;
+}
+
+int wfx_step_api(wfx_ctx *c, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->initialized, WFX_ERR_STATE, "call wfx_init first");
+  return step_api(c, as_stream(s), nullptr);
+}
+
+int wfx_run(wfx_ctx *c, int nsteps, double *reaction_hist, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->initialized, WFX_ERR_STATE, "call wfx_init first");
+  REQUIRE(nsteps >= 0, WFX_ERR_ARG, "nsteps must be >= 0");
+  hipStream_t st = as_stream(s);
+  double *dh = nullptr;
+  if (reaction_hist && nsteps > 0) {
+    if (c->hist_cap < nsteps) {
+      WFX_CHECK_HIP(dalloc(c, &c->hist, nsteps));
+      c->hist_cap = nsteps;
+    }
+    dh = c->hist;
+  }
+  for (int k = 0; k < nsteps; ++k) {
+    int rc = step_api(c, st, dh ? dh + k : nullptr);
+    if (rc) return rc;
+  }
+  if (dh) {
+    WFX_CHECK_HIP(hipMemcpyAsync(reaction_hist, dh, nsteps * sizeof(double), hipMemcpyDeviceToHost, st));
+    WFX_CHECK_HIP(hipStreamSynchronize(st));
+  }
+  return WFX_OK;
+}
+
+int wfx_reaction_force(wfx_ctx *c, int dim, double *out, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(out && dim >= 0 && dim < 3, WFX_ERR_ARG, "bad arguments");
+  hipStream_t st = as_stream(s);
+  LAUNCH(launch_reaction(c, dim, c->scalars, st));
+  WFX_CHECK_HIP(hipMemcpyAsync(out, c->scalars, sizeof(double), hipMemcpyDeviceToHost, st));
+  WFX_CHECK_HIP(hipStreamSynchronize(st));
+  return WFX_OK;
+}
+
+int wfx_energies(wfx_ctx *c, double *ekin, double *deint, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(ekin && deint, WFX_ERR_ARG, "null output");
+  hipStream_t st = as_stream(s);
+  int nb = (std::max(c->N, c->E) + 255) / 256;
+  double *partial = nullptr;
+  WFX_CHECK_HIP(hipMalloc(&partial, sizeof(double) * 2 * (nb + 1)));
+  hipError_t e = BY_PRECISION(c, launch_energies, c, c->scalars + 2, partial, st);
+  double h[2] = {0, 0};
+  if (e == hipSuccess) e = hipMemcpyAsync(h, c->scalars + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(partial);
+  WFX_CHECK_HIP(e);
+  *ekin = h[0];
+  *deint = h[1] * c->prm.dt;
+  return WFX_OK;
+}
+
+int wfx_nonfinite_count(wfx_ctx *c, long *out) {
+  REQUIRE_MESH(c);
+  REQUIRE(out, WFX_ERR_ARG, "null output");
+  unsigned int h = 0;
+  WFX_CHECK_HIP(hipMemcpy(&h, c->status, sizeof(unsigned int), hipMemcpyDeviceToHost));
+  *out = h;
+  return WFX_OK;
+}
+
+// ---- state transfer ---------------------------------------------------------
+int wfx_field_count(wfx_ctx *c, const char *name, long *count, int *is_int) {
+  REQUIRE_MESH(c);
+  FieldDesc f;
+  REQUIRE(name && find_field(c, name, f), WFX_ERR_ARG, std::string("unknown field ") + (name ? name : ""));
+  if (count) *count = f.cnt * f.rows;
+  if (is_int) *is_int = f.is_int;
+  return WFX_OK;
+}
+
+int wfx_get_field(wfx_ctx *c, const char *name, void *host, long count) {
+  REQUIRE_MESH(c);
+  FieldDesc f;
+  REQUIRE(name && find_field(c, name, f), WFX_ERR_ARG, std::string("unknown field ") + (name ? name : ""));
+  REQUIRE(host && count == f.cnt * f.rows, WFX_ERR_ARG, std::string("size mismatch for field ") + name);
+  WFX_CHECK_HIP(hipDeviceSynchronize());
+  const long cnt = f.cnt;
+  if (f.is_int) {
+    std::vector<int> tmp(cnt * f.rows);
+    WFX_CHECK_HIP(hipMemcpy(tmp.data(), f.dev, tmp.size() * sizeof(int), hipMemcpyDeviceToHost));
+    int *h = (int *)host;
+    for (long i = 0; i < cnt; ++i)
+      for (int r = 0; r < f.rows; ++r) h[i * f.rows + r] = tmp[(long)r * cnt + i];
+    return WFX_OK;
+  }
+  int drows = f.kind == 1 ? 3 : f.rows;
+  std::vector<double> tmp((size_t)drows * cnt);
+  int rc = download_real(c, tmp.data(), f.dev, tmp.size(), f.elem);
+  if (rc) return rc;
+  double *h = (double *)host;
+  if (f.kind == 1) {
+    for (long i = 0; i < cnt; ++i) {
+      h[i * 6 + 0] = h[i * 6 + 1] = h[i * 6 + 2] = 0.0;
+      for (int r = 0; r < 3; ++r) h[i * 6 + 3 + r] = tmp[(long)r * cnt + i];
+    }
+  } else {
+    for (long i = 0; i < cnt; ++i)
+      for (int r = 0; r < f.rows; ++r) h[i * f.rows + r] = tmp[(long)r * cnt + i];
+  }
+  return WFX_OK;
+}
+
+int wfx_set_field(wfx_ctx *c, const char *name, const void *host, long count) {
+  REQUIRE_MESH(c);
+  FieldDesc f;
+  REQUIRE(name && find_field(c, name, f), WFX_ERR_ARG, std::string("unknown field ") + (name ? name : ""));
+  REQUIRE(host && count == f.cnt * f.rows, WFX_ERR_ARG, std::string("size mismatch for field ") + name);
+  REQUIRE(!f.is_int, WFX_ERR_ARG, "connectivity and CSR are set through the mesh calls");
+  WFX_CHECK_HIP(hipDeviceSynchronize());
+  const long cnt = f.cnt;
+  const double *h = (const double *)host;
+  int drows = f.kind == 1 ? 3 : f.rows;
+  std::vector<double> tmp((size_t)drows * cnt);
+  if (f.kind == 1) {
+    for (long i = 0; i < cnt; ++i)
+      for (int r = 0; r < 3; ++r) tmp[(long)r * cnt + i] = h[i * 6 + 3 + r];
+  } else {
+    for (long i = 0; i < cnt; ++i)
+      for (int r = 0; r < f.rows; ++r) tmp[(long)r * cnt + i] = h[i * f.rows + r];
+  }
+  return upload_real(c, f.dev, tmp.data(), tmp.size(), f.elem);
+}
+
+}  // extern "C"

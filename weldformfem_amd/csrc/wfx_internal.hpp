@@ -1,0 +1,111 @@
+// Internal definitions shared by the wfx host code and the HIP kernels.
+//
+// Device data layout (HBM), structure of arrays throughout:
+//   nodal vectors      double  [3][N]      component-major (x_x..., x_y..., x_z...)
+//   nodal scalars      double  [N]
+//   element scalars    real    [E]         real = double (FP64) or float (FP32-mixed)
+//   element tensors    real    [6][E]      components xx,yy,zz,xy,yz,xz
+//   element spin       real    [3][E]      components xy,yz,xz (diagonal is zero)
+//   shape derivatives  real    [3][n][E]   dH_detJ(d,a) at row d*n+a
+//   element forces     real    [n][3][E]   f(a,d) at row a*3+d
+//   connectivity       int32   [n][E]      elnod(e,a) at a*E+e
+//   node->element CSR  int32   nodel[nnz], nodel_loc[nnz], offset[N], count[N]
+// Element-thread kernels therefore read and write every element field with
+// unit stride across a wavefront; nodal gathers go through L2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#define WFX_CHECK_HIP(expr)                                                   \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess) {                                                   \
+      wfx::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));      \
+      return WFX_ERR_HIP;                                                     \
+    }                                                                         \
+  } while (0)
+
+namespace wfx {
+
+void set_error(const std::string &msg);
+
+enum Model { BILINEAR = 0, HOLLOMON = 1 };
+
+struct Material {
+  int model = BILINEAR;
+  double E = 0, nu = 0, K = 0, G = 0, rho0 = 0, cs0 = 0, sy0 = 1.0e10;
+  double Et = 0, H = 0;                 // bilinear: tangent, plastic modulus
+  double Kh = 0, mh = 1, eps0 = 0, eps1 = 0;  // Hollomon
+};
+
+struct Stab {
+  double alpha_free = 0, log_factor = 0, J_min = 0, hg_coeff_free = 0;
+  double pspg_scale = 0, p_pspg_bulkfac = 0, av_coeff_div = 0, av_coeff_bulk = 0;
+  double artifvisc0 = 0, artifvisc1 = 0;
+  double hg_coeff = 0;  // 3D hexa viscous hourglass coefficient
+};
+
+// Device buffers of one domain.  T is the element-field real type.
+template <typename T>
+struct DevElem {
+  T *detJ = nullptr, *dH = nullptr, *vol = nullptr, *vol0 = nullptr, *rho = nullptr,
+    *rho0 = nullptr, *p = nullptr, *pl_strain = nullptr, *sigma_y = nullptr, *sy0 = nullptr;
+  T *str_rate = nullptr, *rot_rate = nullptr, *sigma = nullptr, *tau = nullptr, *eps = nullptr,
+    *strain_pl_incr = nullptr;
+  T *f_elem = nullptr, *f_hg = nullptr;
+};
+
+struct DevNode {
+  double *x = nullptr, *v = nullptr, *a = nullptr, *u = nullptr, *u_dt = nullptr,
+         *prev_a = nullptr, *fi = nullptr, *fe = nullptr;
+  double *mdiag = nullptr, *voln = nullptr, *voln_sum = nullptr, *voln0_sum = nullptr;
+};
+
+struct DevMesh {
+  int *elnod = nullptr;
+  int *nodel = nullptr, *nodel_loc = nullptr, *nodel_offset = nullptr, *nodel_count = nullptr;
+};
+
+// Kernel launch parameters shared by the precision variants.
+struct StepParams {
+  int N = 0, E = 0, n = 0;
+  double dt = 0, alpha = 0, beta = 0, gamma = 0;
+  Material mat;
+  Stab stab;
+};
+
+}  // namespace wfx
+
+// The opaque context behind the C-ABI handle.
+struct wfx_ctx {
+  int precision = 0;  // 0 FP64, 1 FP32-mixed
+  int N = 0, E = 0, n = 0, nnz = 0;
+  bool has_mesh = false, initialized = false;
+  wfx::StepParams prm;
+  // host copies of the mesh and boundary conditions
+  std::vector<double> x_h;       // reference layout [N][3]
+  std::vector<int> elnod_h;      // [E][n]
+  std::vector<int> nodel_h, nodel_loc_h, nodel_offset_h, nodel_count_h;
+  std::vector<double> sy0_h;     // per element
+  std::vector<int> bc_nod_h[3];
+  std::vector<double> bc_val_h[3];
+  std::vector<int> react_h;
+  // device
+  wfx::DevNode nd;
+  wfx::DevMesh ms;
+  wfx::DevElem<double> e64;
+  wfx::DevElem<float> e32;
+  int *bc_nod[3] = {nullptr, nullptr, nullptr};
+  double *bc_val[3] = {nullptr, nullptr, nullptr};
+  int bc_count[3] = {0, 0, 0};
+  int *react = nullptr;
+  double *scalars = nullptr;      // device scratch for reductions (reaction force)
+  unsigned int *status = nullptr; // [0] non-finite internal forces scrubbed
+  double *hist = nullptr;         // reaction-force history of wfx_run
+  int hist_cap = 0;
+  std::vector<void *> allocations;
+};

@@ -1,0 +1,67 @@
+// FP64 instantiation of the per-step kernels (built with -ffp-contract=off)
+// and the precision-independent nodal launchers.
+#include "wfx_kernels.hpp"
+#include "wfx_launch.hpp"
+
+namespace wfx {
+
+WFX_DEFINE_LAUNCHERS(double)
+
+hipError_t launch_update_prediction(wfx_ctx *c, hipStream_t s) {
+  long n3 = 3L * c->N;
+  const auto &p = c->prm;
+  hipLaunchKernelGGL(k_update_prediction, dim3(nblocks(n3)), dim3(kBlock), 0, s, n3, c->nd.v,
+                     c->nd.prev_a, c->nd.u_dt, p.dt, p.beta, p.gamma);
+  return hipGetLastError();
+}
+
+hipError_t launch_impose(wfx_ctx *c, int dim, bool accel, hipStream_t s) {
+  int cnt = c->bc_count[dim];
+  if (cnt == 0) return hipSuccess;
+  double *comp = (accel ? c->nd.a : c->nd.v) + (long)dim * c->N;
+  hipLaunchKernelGGL(k_impose_value, dim3(nblocks(cnt)), dim3(kBlock), 0, s, cnt, c->bc_nod[dim],
+                     accel ? (const double *)nullptr : (const double *)c->bc_val[dim], comp);
+  return hipGetLastError();
+}
+
+hipError_t launch_calc_accel(wfx_ctx *c, hipStream_t s) {
+  hipLaunchKernelGGL(k_calc_accel, dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N, c->nd.fe, c->nd.fi,
+                     c->nd.mdiag, c->nd.a);
+  return hipGetLastError();
+}
+
+hipError_t launch_correction_accvel(wfx_ctx *c, hipStream_t s) {
+  long n3 = 3L * c->N;
+  const auto &p = c->prm;
+  double f = 1.0 / (1.0 - p.alpha);
+  hipLaunchKernelGGL(k_correction_accvel, dim3(nblocks(n3)), dim3(kBlock), 0, s, n3, c->nd.a,
+                     c->nd.prev_a, c->nd.v, f, p.alpha, p.gamma * p.dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_correction_pos(wfx_ctx *c, hipStream_t s) {
+  long n3 = 3L * c->N;
+  const auto &p = c->prm;
+  hipLaunchKernelGGL(k_correction_pos, dim3(nblocks(n3)), dim3(kBlock), 0, s, n3, c->nd.a,
+                     c->nd.u_dt, c->nd.x, c->nd.prev_a, c->nd.u, p.beta * p.dt * p.dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_reaction(wfx_ctx *c, int dim, double *dev_out, hipStream_t s) {
+  int cnt = (int)c->react_h.size();
+  hipLaunchKernelGGL(k_reaction, dim3(1), dim3(kBlock), 0, s, cnt, c->react,
+                     c->nd.fi + (long)dim * c->N, dev_out);
+  return hipGetLastError();
+}
+
+int min_edge_partial_count(wfx_ctx *c) { return nblocks(c->E); }
+
+hipError_t launch_min_edge(wfx_ctx *c, double *partial, double *dev_out, hipStream_t s) {
+  int nb = nblocks(c->E);
+  hipLaunchKernelGGL(k_min_edge, dim3(nb), dim3(kBlock), 0, s, c->E, c->N, c->n, c->ms.elnod, c->nd.x,
+                     partial);
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kBlock), 0, s, nb, partial, dev_out);
+  return hipGetLastError();
+}
+
+}  // namespace wfx

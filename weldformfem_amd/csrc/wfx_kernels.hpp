@@ -1,0 +1,921 @@
+// HIP kernels of the per-step Domain_d API, one kernel per reference
+// function, templated on the element real type T (double: FP64 path, float:
+// FP32-mixed path).  Nodal kinematics (x, u, u_dt, v, a) stay double in both.
+//
+// This header is included by exactly two translation units:
+//   wfx_k64.hip  instantiates T=double, compiled with -ffp-contract=off so
+//                that every expression rounds like the reference's CPU code;
+//   wfx_k32.hip  instantiates T=float with contraction allowed.
+#pragma once
+
+#include <type_traits>
+
+#include "wfx_internal.hpp"
+
+namespace wfx {
+
+constexpr int kBlock = 256;
+inline int nblocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+// Kernels have internal linkage: each precision translation unit carries its
+// own copy of the precision-independent ones.
+namespace {
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8), and each XCD has its own L2.  Remap so that every XCD walks one
+// contiguous range of logical blocks: neighbouring elements share nodes, so
+// their gathers then hit the same L2.
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int per = nb >> 3, rem = nb & 7, xcd = b & 7, idx = b >> 3;
+  return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+__device__ __forceinline__ int gtid() { return xcd_block() * blockDim.x + threadIdx.x; }
+
+// ---------------------------------------------------------------------------
+// tensor3 with the reference's operators (include/common/Tensor3.C)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct T3 {
+  T xx, xy, xz, yx, yy, yz, zx, zy, zz;
+};
+template <typename T>
+__device__ __forceinline__ T3<T> t3_from_sym(T a0, T a1, T a2, T a3, T a4, T a5) {
+  T3<T> r;
+  r.xx = a0; r.yy = a1; r.zz = a2;
+  r.xy = r.yx = a3; r.yz = r.zy = a4; r.xz = r.zx = a5;
+  return r;
+}
+// FromFlatAntiSym with a zero diagonal (the reference stores zeros there)
+template <typename T>
+__device__ __forceinline__ T3<T> t3_from_antisym(T w01, T w12, T w02) {
+  T3<T> r;
+  r.xx = T(0); r.yy = T(0); r.zz = T(0);
+  r.xy = w01; r.yz = w12; r.xz = w02;
+  r.yx = -w01; r.zy = -w12; r.zx = -w02;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ T t3_trace(const T3<T> &m) { return m.xx + m.yy + m.zz; }
+template <typename T>
+__device__ __forceinline__ T3<T> t3_identity() {
+  T3<T> r;
+  r.xx = r.yy = r.zz = T(1);
+  r.xy = r.xz = r.yx = r.yz = r.zx = r.zy = T(0);
+  return r;
+}
+// Tensor3.C operator*(tensor3,tensor3): x column = A*B, y/z columns = A*B^T.
+template <typename T>
+__device__ __forceinline__ T3<T> t3_mul(const T3<T> &a, const T3<T> &b) {
+  T3<T> r;
+  r.xx = a.xx * b.xx + a.xy * b.yx + a.xz * b.zx;
+  r.xy = a.xx * b.yx + a.xy * b.yy + a.xz * b.yz;
+  r.xz = a.xx * b.zx + a.xy * b.zy + a.xz * b.zz;
+  r.yx = a.yx * b.xx + a.yy * b.yx + a.yz * b.zx;
+  r.yy = a.yx * b.yx + a.yy * b.yy + a.yz * b.yz;
+  r.yz = a.yx * b.zx + a.yy * b.zy + a.yz * b.zz;
+  r.zx = a.zx * b.xx + a.zy * b.yx + a.zz * b.zx;
+  r.zy = a.zx * b.yx + a.zy * b.yy + a.zz * b.yz;
+  r.zz = a.zx * b.zx + a.zy * b.zy + a.zz * b.zz;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ T3<T> t3_trans(const T3<T> &m) {
+  T3<T> r;
+  r.xx = m.xx; r.yy = m.yy; r.zz = m.zz;
+  r.xy = m.yx; r.xz = m.zx; r.yx = m.xy; r.yz = m.zy; r.zx = m.xz; r.zy = m.yz;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ T3<T> t3_scale(const T3<T> &m, T f) {
+  T3<T> r;
+  r.xx = m.xx * f; r.xy = m.xy * f; r.xz = m.xz * f;
+  r.yx = m.yx * f; r.yy = m.yy * f; r.yz = m.yz * f;
+  r.zx = m.zx * f; r.zy = m.zy * f; r.zz = m.zz * f;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ T3<T> t3_add(const T3<T> &a, const T3<T> &b) {
+  T3<T> r;
+  r.xx = a.xx + b.xx; r.xy = a.xy + b.xy; r.xz = a.xz + b.xz;
+  r.yx = a.yx + b.yx; r.yy = a.yy + b.yy; r.yz = a.yz + b.yz;
+  r.zx = a.zx + b.zx; r.zy = a.zy + b.zy; r.zz = a.zz + b.zz;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ T3<T> t3_sub(const T3<T> &a, const T3<T> &b) {
+  T3<T> r;
+  r.xx = a.xx - b.xx; r.xy = a.xy - b.xy; r.xz = a.xz - b.xz;
+  r.yx = a.yx - b.yx; r.yy = a.yy - b.yy; r.yz = a.yz - b.yz;
+  r.zx = a.zx - b.zx; r.zy = a.zy - b.zy; r.zz = a.zz - b.zz;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// nodal update kernels: flat over the [3][N] component-major arrays
+// ---------------------------------------------------------------------------
+
+// UpdatePrediction (Domain_d.C:961-974)
+__global__ __launch_bounds__(kBlock) void k_update_prediction(long n3, double *__restrict__ v,
+                                                              const double *__restrict__ prev_a,
+                                                              double *__restrict__ u_dt, double dt,
+                                                              double beta, double gamma) {
+  long i = gtid();
+  if (i >= n3) return;
+  double pa = prev_a[i], vi = v[i];
+  u_dt[i] = dt * (vi + (0.5 - beta) * dt * pa);
+  v[i] = vi + (1.0 - gamma) * dt * pa;
+}
+
+// ImposeBCV / ImposeBCA (Domain_d.C:1109-1134) on one component array.
+// Lists are de-duplicated on the host (last entry per node wins, as in the
+// reference's serial loop), so no two threads touch the same address.
+__global__ __launch_bounds__(kBlock) void k_impose_value(int count, const int *__restrict__ nod,
+                                                         const double *__restrict__ val,
+                                                         double *__restrict__ comp) {
+  int i = gtid();
+  if (i >= count) return;
+  comp[nod[i]] = val ? val[i] : 0.0;
+}
+
+// calcAccel (Mechanical.C:321-341)
+__global__ __launch_bounds__(kBlock) void k_calc_accel(int N, const double *__restrict__ fe,
+                                                       const double *__restrict__ fi,
+                                                       const double *__restrict__ m,
+                                                       double *__restrict__ a) {
+  int i = gtid();
+  if (i >= N) return;
+  double mi = m[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) a[(long)d * N + i] = (fe[(long)d * N + i] - fi[(long)d * N + i]) / mi;
+}
+
+// UpdateCorrectionAccVel (Domain_d.C:981-997)
+__global__ __launch_bounds__(kBlock) void k_correction_accvel(long n3, double *__restrict__ a,
+                                                              const double *__restrict__ prev_a,
+                                                              double *__restrict__ v, double f,
+                                                              double alpha, double gdt) {
+  long i = gtid();
+  if (i >= n3) return;
+  double ai = f * (a[i] - alpha * prev_a[i]);
+  a[i] = ai;
+  v[i] = v[i] + gdt * ai;
+}
+
+// UpdateCorrectionPos (Domain_d.C:1005-1025); bdt2 = beta*dt*dt
+__global__ __launch_bounds__(kBlock) void k_correction_pos(long n3, const double *__restrict__ a,
+                                                           double *__restrict__ u_dt,
+                                                           double *__restrict__ x,
+                                                           double *__restrict__ prev_a,
+                                                           double *__restrict__ u, double bdt2) {
+  long i = gtid();
+  if (i >= n3) return;
+  double ai = a[i];
+  double ud = u_dt[i] + bdt2 * ai;
+  u_dt[i] = ud;
+  x[i] = x[i] + ud;
+  prev_a[i] = ai;
+  u[i] = u[i] + ud;
+}
+
+// ---------------------------------------------------------------------------
+// element kernels
+// ---------------------------------------------------------------------------
+
+// calcElemJAndDerivatives (Domain_d.C:1701-2044): hexa :1822-1850, tet
+// :1855-1887, determinant Matrix.h:593-618, adjugate Matrix.h:694-727.
+template <typename T, int NN>
+__global__ __launch_bounds__(kBlock) void k_elem_jacobians(int E, int N, const int *__restrict__ elnod,
+                                                           const double *__restrict__ x,
+                                                           T *__restrict__ detJ, T *__restrict__ dH) {
+  int e = gtid();
+  if (e >= E) return;
+  T X[NN][3];
+  if constexpr (std::is_same<T, double>::value) {
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      int g = elnod[(long)a * E + e];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) X[a][c] = x[(long)c * N + g];
+    }
+  } else {
+    // FP32 bulk: coordinates relative to local node 0, differenced in FP64
+    int g0 = elnod[e];
+    double x0[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x0[c] = x[(long)c * N + g0];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) X[0][c] = T(0);
+#pragma unroll
+    for (int a = 1; a < NN; ++a) {
+      int g = elnod[(long)a * E + e];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) X[a][c] = T(x[(long)c * N + g] - x0[c]);
+    }
+  }
+  T J[3][3];
+  if constexpr (NN == 8) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      J[0][c] = T(0.125) * (-X[0][c] + X[1][c] + X[2][c] - X[3][c] - X[4][c] + X[5][c] + X[6][c] - X[7][c]);
+      J[1][c] = T(0.125) * (-X[0][c] - X[1][c] + X[2][c] + X[3][c] - X[4][c] - X[5][c] + X[6][c] + X[7][c]);
+      J[2][c] = T(0.125) * (-X[0][c] - X[1][c] - X[2][c] - X[3][c] + X[4][c] + X[5][c] + X[6][c] + X[7][c]);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      J[0][c] = X[1][c] - X[0][c];
+      J[1][c] = X[2][c] - X[0][c];
+      J[2][c] = X[3][c] - X[0][c];
+    }
+  }
+  T C[3][3];
+  C[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
+  C[0][1] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]);
+  C[0][2] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  C[1][0] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]);
+  C[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+  C[1][2] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]);
+  C[2][0] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+  C[2][1] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]);
+  C[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+  detJ[e] = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[1][2] * J[2][1] - J[0][1] * J[1][0] * J[2][2] +
+            J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] - J[0][2] * J[1][1] * J[2][0];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    // adj(d,i) = C(i,d)
+    T a0 = C[0][d], a1 = C[1][d], a2 = C[2][d];
+    T *o = dH + (long)d * NN * E + e;
+    if constexpr (NN == 8) {
+      const T q = T(0.125);
+      o[0 * (long)E] = q * (-a0 - a1 - a2);
+      o[1 * (long)E] = q * (a0 - a1 - a2);
+      o[2 * (long)E] = q * (a0 + a1 - a2);
+      o[3 * (long)E] = q * (-a0 + a1 - a2);
+      o[4 * (long)E] = q * (-a0 - a1 + a2);
+      o[5 * (long)E] = q * (a0 - a1 + a2);
+      o[6 * (long)E] = q * (a0 + a1 + a2);
+      o[7 * (long)E] = q * (-a0 + a1 + a2);
+    } else {
+      o[0 * (long)E] = -a0 - a1 - a2;
+      o[1 * (long)E] = a0;
+      o[2 * (long)E] = a1;
+      o[3 * (long)E] = a2;
+    }
+  }
+}
+
+// CalcElemVol (Mechanical.C:264-293): vol = detJ*w*1.0, w = 8 or 1/6
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_elem_vol(int E, const T *__restrict__ detJ,
+                                                     T *__restrict__ vol, T w) {
+  int e = gtid();
+  if (e >= E) return;
+  T v = T(0);
+  v += detJ[e] * w * T(1);
+  vol[e] = v;
+}
+
+// calcElemDensity (Mechanical.C:295-307)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_elem_density(int E, const T *__restrict__ rho0,
+                                                         const T *__restrict__ vol0,
+                                                         const T *__restrict__ vol,
+                                                         T *__restrict__ rho) {
+  int e = gtid();
+  if (e >= E) return;
+  rho[e] = rho0[e] * vol0[e] / vol[e];
+}
+
+// CalcNodalVol (Mechanical.C:1555-1572): deterministic CSR gather
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_nodal_vol(int N, const int *__restrict__ off,
+                                                      const int *__restrict__ cnt,
+                                                      const int *__restrict__ nodel,
+                                                      const T *__restrict__ vol,
+                                                      double *__restrict__ voln, int nodxelem) {
+  int i = gtid();
+  if (i >= N) return;
+  int o = off[i], c = cnt[i];
+  double s = 0.0;
+  for (int k = 0; k < c; ++k) s += (double)vol[nodel[o + k]];
+  s /= nodxelem;
+  voln[i] = s;
+}
+
+// CalcNodalMassFromVol (Mechanical.C:1576-1601)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_nodal_mass(int N, const int *__restrict__ off,
+                                                       const int *__restrict__ cnt,
+                                                       const int *__restrict__ nodel,
+                                                       const T *__restrict__ rho,
+                                                       const double *__restrict__ voln,
+                                                       double *__restrict__ mdiag) {
+  int i = gtid();
+  if (i >= N) return;
+  int o = off[i], c = cnt[i];
+  double vn = voln[i], dc = (double)c, mass = 0.0;
+  for (int k = 0; k < c; ++k) mass += 1.0 * (double)rho[nodel[o + k]] * vn / dc;
+  mdiag[i] = mass;
+}
+
+// calcElemStrainRates (Mechanical.C:41-126)
+template <typename T, int NN>
+__global__ __launch_bounds__(kBlock) void k_strain_rates(int E, int N, const int *__restrict__ elnod,
+                                                         const double *__restrict__ v,
+                                                         const T *__restrict__ dH,
+                                                         const T *__restrict__ detJ,
+                                                         T *__restrict__ sr, T *__restrict__ rr) {
+  int e = gtid();
+  if (e >= E) return;
+  T s00 = 0, s11 = 0, s22 = 0, s01 = 0, s12 = 0, s02 = 0, w01 = 0, w12 = 0, w02 = 0;
+  T f = T(1) / detJ[e];
+#pragma unroll
+  for (int a = 0; a < NN; ++a) {
+    int g = elnod[(long)a * E + e];
+    T vx = (T)v[g], vy = (T)v[(long)N + g], vz = (T)v[2L * N + g];
+    T h0 = dH[(long)(0 * NN + a) * E + e], h1 = dH[(long)(1 * NN + a) * E + e],
+      h2 = dH[(long)(2 * NN + a) * E + e];
+    s00 = s00 + h0 * f * vx;
+    s11 = s11 + h1 * f * vy;
+    s22 = s22 + h2 * f * vz;
+    s01 = s01 + f * (h1 * vx + h0 * vy);
+    w01 = w01 + f * (h1 * vx - h0 * vy);
+    s12 = s12 + f * (h2 * vy + h1 * vz);
+    s02 = s02 + f * (h2 * vx + h0 * vz);
+    w12 = w12 + f * (h2 * vy - h1 * vz);
+    w02 = w02 + f * (h2 * vx - h0 * vz);
+  }
+  const long Es = E;
+  sr[0 * Es + e] = s00; sr[1 * Es + e] = s11; sr[2 * Es + e] = s22;
+  sr[3 * Es + e] = s01 * T(0.5); sr[4 * Es + e] = s12 * T(0.5); sr[5 * Es + e] = s02 * T(0.5);
+  rr[0 * Es + e] = w01 * T(0.5); rr[1 * Es + e] = w12 * T(0.5); rr[2 * Es + e] = w02 * T(0.5);
+}
+
+// calcElemPressure node part (Mechanical.C:698-705): CSR sums of vol, vol0
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pressure_nodes(int N, const int *__restrict__ off,
+                                                           const int *__restrict__ cnt,
+                                                           const int *__restrict__ nodel,
+                                                           const T *__restrict__ vol,
+                                                           const T *__restrict__ vol0,
+                                                           double *__restrict__ vs,
+                                                           double *__restrict__ v0s) {
+  int i = gtid();
+  if (i >= N) return;
+  int o = off[i], c = cnt[i];
+  double s0 = 0.0, s = 0.0;
+  for (int k = 0; k < c; ++k) {
+    int e = nodel[o + k];
+    s0 += (double)vol0[e];
+    s += (double)vol[e];
+  }
+  v0s[i] = s0;
+  vs[i] = s;
+}
+
+struct PressParams {
+  double K, alpha_free, log_factor, J_min, hg_coeff_free, pspg_scale, p_pspg_bulkfac,
+      av_coeff_div, av_coeff_bulk;
+  int full;  // 0: all stabilisation terms that need h, c, div v are zero
+};
+
+// calcElemPressure element part (Mechanical.C:720-815), no contact.  With
+// pspg/artificial-viscosity coefficients at zero the terms that need h, c and
+// div v are exactly +-0 and are skipped (full == 0).
+template <typename T, int NN>
+__global__ __launch_bounds__(kBlock) void k_pressure_elem(int E, int N, const int *__restrict__ elnod,
+                                                          const double *__restrict__ vs,
+                                                          const double *__restrict__ v0s,
+                                                          const T *__restrict__ vol,
+                                                          const T *__restrict__ vol0,
+                                                          const T *__restrict__ rho,
+                                                          const T *__restrict__ dH,
+                                                          const double *__restrict__ v,
+                                                          T *__restrict__ p, PressParams pp) {
+  int e = gtid();
+  if (e >= E) return;
+  double J_local = (double)vol[e] / (double)vol0[e];
+  double J_avg = 0.0;
+#pragma unroll
+  for (int a = 0; a < NN; ++a) {
+    int nid = elnod[(long)a * E + e];
+    J_avg += vs[nid] / v0s[nid];
+  }
+  J_avg /= NN;
+  double al = pp.alpha_free;
+  double J_bar = al * J_local + (1 - al) * J_avg;
+  if (J_bar < pp.J_min) J_bar = 0.2;
+  double lf = pp.log_factor;
+  double lg = (lf != 0.0) ? log(J_bar) : 0.0;
+  double p_physical = -pp.K * (lf * lg + (1.0 - lf) * (J_bar - 1.0));
+  double p_pspg = 0.0, p_q = 0.0;
+  double p_hg = pp.hg_coeff_free * pp.K * fabs(J_local - J_avg);
+  if (pp.full) {
+    double rho_e = (double)rho[e];
+    double h = pow((double)vol[e], 1.0 / 3.0);
+    double c = sqrt(pp.K / rho_e);
+    double tau = h / (2.0 * c);
+    double div_v = 0.0;
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      int nid = elnod[(long)a * E + e];
+      div_v += (double)dH[(long)(0 * NN + a) * E + e] * v[nid] +
+               (double)dH[(long)(1 * NN + a) * E + e] * v[(long)N + nid] +
+               (double)dH[(long)(2 * NN + a) * E + e] * v[2L * N + nid];
+    }
+    if (div_v < 0.0) {
+      double x1 = pp.pspg_scale * tau * div_v * pp.K, x2 = pp.p_pspg_bulkfac * pp.K;
+      p_pspg = (x2 < x1) ? x2 : x1;
+      double q1 = pp.av_coeff_div * rho_e * h * c * (-div_v);
+      double q2 = pp.av_coeff_bulk * pp.K * (1.0 - J_local);
+      p_q = (q1 < q2) ? q2 : q1;
+    }
+  }
+  p[e] = (T)(p_physical + p_pspg + p_hg + p_q);
+}
+
+struct MatParams {
+  int model;
+  double G, H, sy0, Kh, mh, eps0, eps1;
+};
+
+// Calc*YieldStress / Calc*TangentModulus (Material.cuh:353-364, :389-395);
+// bilinear as sigma_y = sy0_e + H*ep (the reference leaves it undefined).
+template <typename T>
+__device__ __forceinline__ T yield_stress(const MatParams &m, T sy0e, T ep) {
+  if (m.model == HOLLOMON) {
+    double s = (double)ep + m.eps0;
+    return (s > m.eps1) ? (T)(m.Kh * pow(s, m.mh)) : (T)m.sy0;
+  }
+  return sy0e + (T)m.H * ep;
+}
+template <typename T>
+__device__ __forceinline__ T tangent_modulus(const MatParams &m, T ep) {
+  if (m.model == HOLLOMON) {
+    double s = (double)ep + m.eps0;
+    return (s > m.eps1) ? (T)(m.Kh * m.mh * pow(s, (m.mh - 1.0))) : T(0);
+  }
+  return (T)m.H;
+}
+
+// CalcStressStrain (Mechanical.C:1664-1839)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_stress_strain(
+    int E, T dt, MatParams mp, const T *__restrict__ sr, const T *__restrict__ rr,
+    const T *__restrict__ p, const T *__restrict__ sy0, T *__restrict__ tau_io,
+    T *__restrict__ eps_io, T *__restrict__ sigma, T *__restrict__ plincr,
+    T *__restrict__ pl_strain, T *__restrict__ sigma_y) {
+  int e = gtid();
+  if (e >= E) return;
+  const long Es = E;
+  T3<T> tau = t3_from_sym(tau_io[0 * Es + e], tau_io[1 * Es + e], tau_io[2 * Es + e], tau_io[3 * Es + e],
+                          tau_io[4 * Es + e], tau_io[5 * Es + e]);
+  T3<T> D = t3_from_sym(sr[0 * Es + e], sr[1 * Es + e], sr[2 * Es + e], sr[3 * Es + e], sr[4 * Es + e],
+                        sr[5 * Es + e]);
+  T3<T> W = t3_from_antisym(rr[0 * Es + e], rr[1 * Es + e], rr[2 * Es + e]);
+  T3<T> I = t3_identity<T>();
+  T G = (T)mp.G;
+  T3<T> Wt = t3_trans(W);
+  T3<T> SRT = t3_mul(tau, Wt);
+  T3<T> RS = t3_mul(W, tau);
+  T3<T> cI = t3_scale(I, T(1.0 / 3.0) * t3_trace(D));
+  T3<T> devD = t3_sub(D, cI);
+  T3<T> A = t3_scale(devD, T(2) * G);
+  T3<T> B = t3_add(A, SRT);
+  T3<T> C = t3_add(B, RS);
+  tau = t3_add(tau, t3_scale(C, dt));
+
+  T pe = p[e];
+  T3<T> pI = t3_scale(I, -pe);
+  T3<T> st = t3_add(pI, tau);
+  T3<T> s = t3_sub(st, t3_scale(I, T(1.0 / 3.0) * t3_trace(st)));
+  T J2 = T(0.5) * (s.xx * s.xx + T(2) * s.xy * s.xy + T(2) * s.xz * s.xz + s.yy * s.yy +
+                   T(2) * s.yz * s.yz + s.zz * s.zz);
+  T sig_trial = sqrt(T(3) * J2);
+  T ep = pl_strain[e];
+  T sy = yield_stress<T>(mp, sy0[e], ep);
+  sigma_y[e] = sy;
+  T dep = T(0);
+  if (sy < sig_trial) {
+    T H = tangent_modulus<T>(mp, ep);
+    T dgamma = (sig_trial - sy) / (T(3) * G + H);
+    T factor = T(1) - (T(3) * G * dgamma) / sig_trial;
+    tau = t3_scale(s, factor);
+    pl_strain[e] = ep + dgamma;
+    dep = dgamma;
+  }
+  T3<T> sig = t3_add(pI, tau);
+  if (plincr) {
+    T i0 = 0, i1 = 0, i2 = 0, i3 = 0, i4 = 0, i5 = 0;
+    if (dep > T(0)) {
+      T f = dep / sy;
+      i0 = f * (sig.xx - T(0.5) * (sig.yy + sig.zz));
+      i1 = f * (sig.yy - T(0.5) * (sig.xx + sig.zz));
+      i2 = f * (sig.zz - T(0.5) * (sig.xx + sig.yy));
+      i3 = T(1.5) * f * (sig.xy);
+      i4 = T(1.5) * f * (sig.yz);
+      i5 = T(1.5) * f * (sig.xz);
+    }
+    plincr[0 * Es + e] = i0; plincr[1 * Es + e] = i1; plincr[2 * Es + e] = i2;
+    plincr[3 * Es + e] = i3; plincr[4 * Es + e] = i4; plincr[5 * Es + e] = i5;
+  }
+  eps_io[0 * Es + e] = eps_io[0 * Es + e] + D.xx * dt;
+  eps_io[1 * Es + e] = eps_io[1 * Es + e] + D.yy * dt;
+  eps_io[2 * Es + e] = eps_io[2 * Es + e] + D.zz * dt;
+  eps_io[3 * Es + e] = eps_io[3 * Es + e] + D.xy * dt;
+  eps_io[4 * Es + e] = eps_io[4 * Es + e] + D.yz * dt;
+  eps_io[5 * Es + e] = eps_io[5 * Es + e] + D.xz * dt;
+  sigma[0 * Es + e] = sig.xx; sigma[1 * Es + e] = sig.yy; sigma[2 * Es + e] = sig.zz;
+  sigma[3 * Es + e] = sig.xy; sigma[4 * Es + e] = sig.yz; sigma[5 * Es + e] = sig.xz;
+  tau_io[0 * Es + e] = tau.xx; tau_io[1 * Es + e] = tau.yy; tau_io[2 * Es + e] = tau.zz;
+  tau_io[3 * Es + e] = tau.xy; tau_io[4 * Es + e] = tau.yz; tau_io[5 * Es + e] = tau.xz;
+}
+
+// calcArtificialViscosity (Mechanical.C:1948-1977)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_artificial_viscosity(int E, double K, double al,
+                                                                 double be,
+                                                                 const T *__restrict__ rho,
+                                                                 const T *__restrict__ vol,
+                                                                 const T *__restrict__ sr,
+                                                                 T *__restrict__ sigma) {
+  int e = gtid();
+  if (e >= E) return;
+  const long Es = E;
+  double c = sqrt(K / (double)rho[e]);
+  double eps_v = (double)sr[e] + (double)sr[Es + e] + (double)sr[2 * Es + e];
+  if (fabs(eps_v) > 1e-12) {
+    double l = pow((double)vol[e], 1.0 / 3.0);
+    double q = al * c * fabs(eps_v) * l + be * pow(eps_v * l, 2);
+    q = (1e9 < q) ? 1e9 : q;
+    double qs = (eps_v > 0) ? -q : q;
+    for (int k = 0; k < 3; ++k) sigma[k * Es + e] = (T)((double)sigma[k * Es + e] + qs);
+  }
+}
+
+// calcElemForces (Mechanical.C:375-481), 3D branch :448-457
+template <typename T, int NN>
+__global__ __launch_bounds__(kBlock) void k_elem_forces(int E, const T *__restrict__ dH,
+                                                        const T *__restrict__ sigma,
+                                                        T *__restrict__ f, T w) {
+  int e = gtid();
+  if (e >= E) return;
+  const long Es = E;
+  T s0 = sigma[e], s1 = sigma[Es + e], s2 = sigma[2 * Es + e], s3 = sigma[3 * Es + e],
+    s4 = sigma[4 * Es + e], s5 = sigma[5 * Es + e];
+#pragma unroll
+  for (int a = 0; a < NN; ++a) {
+    T h0 = dH[(long)(0 * NN + a) * Es + e], h1 = dH[(long)(1 * NN + a) * Es + e],
+      h2 = dH[(long)(2 * NN + a) * Es + e];
+    T f0 = T(0), f1 = T(0), f2 = T(0);
+    f0 += h0 * s0 * T(1);
+    f1 += h1 * s1 * T(1);
+    f2 += h2 * s2 * T(1);
+    f0 += h1 * s3 + h2 * s5;
+    f1 += h0 * s3 + h2 * s4;
+    f2 += h1 * s4 + h0 * s5;
+    f[(long)(a * 3 + 0) * Es + e] = f0 * w;
+    f[(long)(a * 3 + 1) * Es + e] = f1 * w;
+    f[(long)(a * 3 + 2) * Es + e] = f2 * w;
+  }
+}
+
+// 3D hexa viscous hourglass (f90_ver/src/Mechanical.f90:241-303): the
+// reference C++ branch for 3D is empty (Mechanical.C:1845, :1939-1942).
+__constant__ const signed char kHgGamma[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1},
+                                                       {1, -1, -1, 1, -1, 1, 1, -1},
+                                                       {1, -1, 1, -1, 1, -1, 1, -1},
+                                                       {-1, 1, -1, 1, 1, -1, 1, -1}};
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hourglass(int E, int N, const int *__restrict__ elnod,
+                                                      const double *__restrict__ v,
+                                                      const T *__restrict__ vol,
+                                                      const T *__restrict__ rho,
+                                                      T *__restrict__ fhg, double coeff, double cs0) {
+  int e = gtid();
+  if (e >= E) return;
+  const long Es = E;
+  T V[8][3];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    int g = elnod[(long)a * Es + e];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) V[a][c] = (T)v[(long)c * N + g];
+  }
+  T hm[3][4];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) hm[c][j] = T(0);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) hm[c][j] = hm[c][j] + V[a][c] * (T)kHgGamma[j][a];
+  T c_h = (T)(coeff * pow((double)vol[e], 0.6666666) * (double)rho[e] * 0.25 * cs0);
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      T f = T(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) f = f - hm[c][j] * (T)kHgGamma[j][a];
+      fhg[(long)(a * 3 + c) * Es + e] = f * c_h;
+    }
+}
+
+// assemblyForces (Matrices.C:42-87): per node, CSR-ordered sum of the element
+// forces, then CSR-ordered subtraction of the hourglass forces.  Non-finite
+// results are set to zero and counted (Solver_explicit.C:777-785).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_assembly(int N, int E, const int *__restrict__ off,
+                                                     const int *__restrict__ cnt,
+                                                     const int *__restrict__ nodel,
+                                                     const int *__restrict__ nloc,
+                                                     const T *__restrict__ fel,
+                                                     const T *__restrict__ fhg,
+                                                     double *__restrict__ fi,
+                                                     unsigned int *__restrict__ status) {
+  int i = gtid();
+  if (i >= N) return;
+  const long Es = E;
+  int o = off[i], c = cnt[i];
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  for (int k = 0; k < c; ++k) {
+    long e = nodel[o + k], b = (long)nloc[o + k] * 3;
+    f0 += (double)fel[(b + 0) * Es + e];
+    f1 += (double)fel[(b + 1) * Es + e];
+    f2 += (double)fel[(b + 2) * Es + e];
+  }
+  if (fhg) {
+    for (int k = 0; k < c; ++k) {
+      long e = nodel[o + k], b = (long)nloc[o + k] * 3;
+      f0 -= (double)fhg[(b + 0) * Es + e];
+      f1 -= (double)fhg[(b + 1) * Es + e];
+      f2 -= (double)fhg[(b + 2) * Es + e];
+    }
+  }
+  unsigned bad = 0;
+  if (!isfinite(f0)) { f0 = 0.0; ++bad; }
+  if (!isfinite(f1)) { f1 = 0.0; ++bad; }
+  if (!isfinite(f2)) { f2 = 0.0; ++bad; }
+  if (bad) atomicAdd(status, bad);
+  fi[i] = f0;
+  fi[(long)N + i] = f1;
+  fi[2L * N + i] = f2;
+}
+
+// ---------------------------------------------------------------------------
+// reductions (single work-group, fixed order: deterministic)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// reaction force: sum of fi component over the node set, written to out[slot]
+__global__ __launch_bounds__(kBlock) void k_reaction(int count, const int *__restrict__ nodes,
+                                                     const double *__restrict__ fcomp,
+                                                     double *__restrict__ out) {
+  __shared__ double sh[kBlock];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < count; k += blockDim.x) s += fcomp[nodes[k]];
+  double r = block_sum(s, sh);
+  if (threadIdx.x == 0) *out = r;
+}
+
+// calcMinEdgeLength (Domain_d.C:2224-2260): per-block minimum, then one block
+__global__ __launch_bounds__(kBlock) void k_min_edge(int E, int N, int n, const int *__restrict__ elnod,
+                                                     const double *__restrict__ x,
+                                                     double *__restrict__ partial) {
+  __shared__ double sh[kBlock];
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double m = 1.0e6;
+  if (e < E) {
+    double P[4][3];
+    for (int k = 0; k < 4; ++k) {
+      int g = elnod[(long)k * E + e];
+      for (int c = 0; c < 3; ++c) P[k][c] = x[(long)c * N + g];
+    }
+    const int ep[6][2] = {{1, 0}, {2, 0}, {3, 0}, {2, 1}, {3, 1}, {3, 2}};
+    for (int i = 0; i < 6; ++i) {
+      double ex = P[ep[i][0]][0] - P[ep[i][1]][0], ey = P[ep[i][0]][1] - P[ep[i][1]][1],
+             ez = P[ep[i][0]][2] - P[ep[i][1]][2];
+      double len = sqrt(ex * ex + ey * ey + ez * ez);
+      if (len < m) m = len;
+    }
+  }
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+__global__ __launch_bounds__(kBlock) void k_min_final(int nb, const double *__restrict__ partial,
+                                                      double *__restrict__ out) {
+  __shared__ double sh[kBlock];
+  double m = 1.0e6;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) m = fmin(m, partial[k]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
+// computeEnergies (Mechanical.C:2145-2185): per-block partial sums, then one
+// block; out[0] = Ekin, out[1] = sum sigma:D*vol
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_energy_partial(int N, int E, const double *__restrict__ v,
+                                                           const double *__restrict__ m,
+                                                           const T *__restrict__ sigma,
+                                                           const T *__restrict__ sr,
+                                                           const T *__restrict__ vol,
+                                                           double *__restrict__ partial) {
+  __shared__ double sh[kBlock];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ek = 0.0, ei = 0.0;
+  if (i < N) {
+    double vx = v[i], vy = v[(long)N + i], vz = v[2L * N + i];
+    ek = 0.5 * m[i] * (vx * vx + vy * vy + vz * vz);
+  }
+  if (i < E) {
+    const long Es = E;
+    double sd = 0.0;
+    for (int k = 0; k < 6; ++k) sd += (double)sigma[k * Es + i] * (double)sr[k * Es + i];
+    ei = sd * (double)vol[i];
+  }
+  double a = block_sum(ek, sh);
+  double b = block_sum(ei, sh);
+  if (threadIdx.x == 0) { partial[2 * blockIdx.x] = a; partial[2 * blockIdx.x + 1] = b; }
+}
+__global__ __launch_bounds__(kBlock) void k_energy_final(int nb, const double *__restrict__ partial,
+                                                         double *__restrict__ out) {
+  __shared__ double sh[kBlock];
+  double a = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) { a += partial[2 * k]; b += partial[2 * k + 1]; }
+  double ra = block_sum(a, sh);
+  double rb = block_sum(b, sh);
+  if (threadIdx.x == 0) { out[0] = ra; out[1] = rb; }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers (explicitly instantiated per precision)
+// ---------------------------------------------------------------------------
+template <typename T> DevElem<T> &elem(wfx_ctx *c);
+template <> inline DevElem<double> &elem<double>(wfx_ctx *c) { return c->e64; }
+template <> inline DevElem<float> &elem<float>(wfx_ctx *c) { return c->e32; }
+
+template <typename T> hipError_t launch_elem_jacobians(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_elem_vol(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_elem_density(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_nodal_vol(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_nodal_mass(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_strain_rates(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_pressure(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_stress_strain(wfx_ctx *c, double dt, hipStream_t s);
+template <typename T> hipError_t launch_artificial_viscosity(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_elem_forces(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_hourglass(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_assembly(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, double *partial, hipStream_t s);
+
+#define WFX_DEFINE_LAUNCHERS(T)                                                                    \
+  template <> hipError_t launch_elem_jacobians<T>(wfx_ctx * c, hipStream_t s) {                    \
+    auto &el = elem<T>(c);                                                                         \
+    if (c->n == 8)                                                                                 \
+      hipLaunchKernelGGL((k_elem_jacobians<T, 8>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,  \
+                         c->N, c->ms.elnod, c->nd.x, el.detJ, el.dH);                              \
+    else                                                                                           \
+      hipLaunchKernelGGL((k_elem_jacobians<T, 4>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,  \
+                         c->N, c->ms.elnod, c->nd.x, el.detJ, el.dH);                              \
+    return hipGetLastError();                                                                      \
+  }                                                                                                \
+  template <> hipError_t launch_elem_vol<T>(wfx_ctx * c, hipStream_t s) {                          \
+    auto &el = elem<T>(c);                                                                         \
+    T w = (c->n == 4) ? T(1.0 / 6.0) : T(8.0);                                                     \
+    hipLaunchKernelGGL((k_elem_vol<T>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, el.detJ,    \
+                       el.vol, w);                                                                 \
+    return hipGetLastError();                                                                      \
+  }                                                                                                \
+  template <> hipError_t launch_elem_density<T>(wfx_ctx * c, hipStream_t s) {                      \
+    auto &el = elem<T>(c);                                                                         \
+    hipLaunchKernelGGL((k_elem_density<T>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,         \
+                       el.rho0, el.vol0, el.vol, el.rho);                                          \
+    return hipGetLastError();                                                                      \
+  }                                                                                                \
+  template <> hipError_t launch_nodal_vol<T>(wfx_ctx * c, hipStream_t s) {                         \
+    auto &el = elem<T>(c);                                                                         \
+    hipLaunchKernelGGL((k_nodal_vol<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N,            \
+                       c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel, el.vol, c->nd.voln,     \
+                       c->n);                                                                      \
+    return hipGetLastError();                                                                      \
+  }                                                                                                \
+  template <> hipError_t launch_nodal_mass<T>(wfx_ctx * c, hipStream_t s) {                        \
+    auto &el = elem<T>(c);                                                                         \
+    hipLaunchKernelGGL((k_nodal_mass<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N,           \
+                       c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel, el.rho, c->nd.voln,     \
+                       c->nd.mdiag);                                                               \
+    return hipGetLastError();                                                                      \
+  }                                                                                                \
+  template <> hipError_t launch_strain_rates<T>(wfx_ctx * c, hipStream_t s) {                      \
+    auto &el = elem<T>(c);                                                                         \
+    if (c->n == 8)                                                                                 \
+      hipLaunchKernelGGL((k_strain_rates<T, 8>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,    \
+                         c->N, c->ms.elnod, c->nd.v, el.dH, el.detJ, el.str_rate, el.rot_rate);    \
+    else                                                                                           \
+      hipLaunchKernelGGL((k_strain_rates<T, 4>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,    \
+                         c->N, c->ms.elnod, c->nd.v, el.dH, el.detJ, el.str_rate, el.rot_rate);    \
+    return hipGetLastError();                                                                      \
+  }                                                                                                \
+  template <> hipError_t launch_pressure<T>(wfx_ctx * c, hipStream_t s) {                          \
+    auto &el = elem<T>(c);                                                                         \
+    hipLaunchKernelGGL((k_pressure_nodes<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N,       \
+                       c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel, el.vol, el.vol0,        \
+                       c->nd.voln_sum, c->nd.voln0_sum);                                           \
+    const auto &st = c->prm.stab;                                                                  \
+    PressParams pp{c->prm.mat.K,       st.alpha_free, st.log_factor,     st.J_min,                 \
+                   st.hg_coeff_free,   st.pspg_scale, st.p_pspg_bulkfac, st.av_coeff_div,          \
+                   st.av_coeff_bulk,   0};                                                         \
+    pp.full = (st.pspg_scale != 0.0 || st.p_pspg_bulkfac != 0.0 || st.av_coeff_div != 0.0 ||     \
+               st.av_coeff_bulk != 0.0);                                                           \
+    if (c->n == 8)                                                                                 \
+      hipLaunchKernelGGL((k_pressure_elem<T, 8>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,   \
+                         c->N, c->ms.elnod, c->nd.voln_sum, c->nd.voln0_sum, el.vol, el.vol0,      \
+                         el.rho, el.dH, c->nd.v, el.p, pp);                                        \
+    else                                                                                           \
+      hipLaunchKernelGGL((k_pressure_elem<T, 4>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,   \
+                         c->N, c->ms.elnod, c->nd.voln_sum, c->nd.voln0_sum, el.vol, el.vol0,      \
+                         el.rho, el.dH, c->nd.v, el.p, pp);                                        \
+    return hipGetLastError();                                                                      \
+  }                                                                                                \
+  template <> hipError_t launch_stress_strain<T>(wfx_ctx * c, double dt, hipStream_t s) {          \
+    auto &el = elem<T>(c);                                                                         \
+    const auto &m = c->prm.mat;                                                                    \
+    MatParams mp{m.model, m.G, m.H, m.sy0, m.Kh, m.mh, m.eps0, m.eps1};                            \
+    hipLaunchKernelGGL((k_stress_strain<T>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, (T)dt, \
+                       mp, el.str_rate, el.rot_rate, el.p, el.sy0, el.tau, el.eps, el.sigma,       \
+                       el.strain_pl_incr, el.pl_strain, el.sigma_y);                               \
+    return hipGetLastError();                                                                      \
+  }                                                                                                \
+  template <> hipError_t launch_artificial_viscosity<T>(wfx_ctx * c, hipStream_t s) {              \
+    const auto &st = c->prm.stab;                                                                  \
+    if (st.artifvisc0 == 0.0 && st.artifvisc1 == 0.0) return hipSuccess;                           \
+    auto &el = elem<T>(c);                                                                         \
+    hipLaunchKernelGGL((k_artificial_viscosity<T>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, \
+                       c->prm.mat.K, st.artifvisc0, st.artifvisc1, el.rho, el.vol, el.str_rate,    \
+                       el.sigma);                                                                  \
+    return hipGetLastError();                                                                      \
+  }                                                                                                \
+  template <> hipError_t launch_elem_forces<T>(wfx_ctx * c, hipStream_t s) {                       \
+    auto &el = elem<T>(c);                                                                         \
+    if (c->n == 8)                                                                                 \
+      hipLaunchKernelGGL((k_elem_forces<T, 8>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,     \
+                         el.dH, el.sigma, el.f_elem, T(8.0));                                      \
+    else                                                                                           \
+      hipLaunchKernelGGL((k_elem_forces<T, 4>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,     \
+                         el.dH, el.sigma, el.f_elem, T(1.0 / 6.0));                                \
+    return hipGetLastError();                                                                      \
+  }                                                                                                \
+  template <> hipError_t launch_hourglass<T>(wfx_ctx * c, hipStream_t s) {                         \
+    if (c->n != 8 || c->prm.stab.hg_coeff == 0.0) return hipSuccess;                               \
+    auto &el = elem<T>(c);                                                                         \
+    hipLaunchKernelGGL((k_hourglass<T>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, c->N,      \
+                       c->ms.elnod, c->nd.v, el.vol, el.rho, el.f_hg, c->prm.stab.hg_coeff,        \
+                       c->prm.mat.cs0);                                                            \
+    return hipGetLastError();                                                                      \
+  }                                                                                                \
+  template <> hipError_t launch_assembly<T>(wfx_ctx * c, hipStream_t s) {                          \
+    auto &el = elem<T>(c);                                                                         \
+    const T *fhg = (c->n == 8 && c->prm.stab.hg_coeff != 0.0) ? el.f_hg : nullptr;                 \
+    hipLaunchKernelGGL((k_assembly<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N, c->E,       \
+                       c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel, c->ms.nodel_loc,        \
+                       el.f_elem, fhg, c->nd.fi, c->status);                                       \
+    return hipGetLastError();                                                                      \
+  }                                                                                                \
+  template <> hipError_t launch_energies<T>(wfx_ctx * c, double *out2, double *partial,            \
+                                            hipStream_t s) {                                       \
+    auto &el = elem<T>(c);                                                                         \
+    int nb = nblocks(c->N > c->E ? c->N : c->E);                                                   \
+    hipLaunchKernelGGL((k_energy_partial<T>), dim3(nb), dim3(kBlock), 0, s, c->N, c->E, c->nd.v,   \
+                       c->nd.mdiag, el.sigma, el.str_rate, el.vol, partial);                       \
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, s, nb, partial, out2);            \
+    return hipGetLastError();                                                                      \
+  }
+
+}  // namespace wfx

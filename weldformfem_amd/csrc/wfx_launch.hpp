@@ -1,0 +1,50 @@
+// Declarations of the kernel launchers.  The precision-dependent ones are
+// defined (explicit specialisations) in wfx_k64.hip (double) and wfx_k32.hip
+// (float); the nodal ones, which are precision independent, in wfx_k64.hip.
+#pragma once
+
+#include "wfx_internal.hpp"
+
+namespace wfx {
+
+template <typename T> hipError_t launch_elem_jacobians(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_elem_vol(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_elem_density(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_nodal_vol(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_nodal_mass(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_strain_rates(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_pressure(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_stress_strain(wfx_ctx *c, double dt, hipStream_t s);
+template <typename T> hipError_t launch_artificial_viscosity(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_elem_forces(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_hourglass(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_assembly(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, double *partial, hipStream_t s);
+
+#define WFX_DECLARE_SPECIALISATIONS(T)                                                          \
+  template <> hipError_t launch_elem_jacobians<T>(wfx_ctx *, hipStream_t);                       \
+  template <> hipError_t launch_elem_vol<T>(wfx_ctx *, hipStream_t);                             \
+  template <> hipError_t launch_elem_density<T>(wfx_ctx *, hipStream_t);                         \
+  template <> hipError_t launch_nodal_vol<T>(wfx_ctx *, hipStream_t);                            \
+  template <> hipError_t launch_nodal_mass<T>(wfx_ctx *, hipStream_t);                           \
+  template <> hipError_t launch_strain_rates<T>(wfx_ctx *, hipStream_t);                         \
+  template <> hipError_t launch_pressure<T>(wfx_ctx *, hipStream_t);                             \
+  template <> hipError_t launch_stress_strain<T>(wfx_ctx *, double, hipStream_t);                \
+  template <> hipError_t launch_artificial_viscosity<T>(wfx_ctx *, hipStream_t);                 \
+  template <> hipError_t launch_elem_forces<T>(wfx_ctx *, hipStream_t);                          \
+  template <> hipError_t launch_hourglass<T>(wfx_ctx *, hipStream_t);                            \
+  template <> hipError_t launch_assembly<T>(wfx_ctx *, hipStream_t);                             \
+  template <> hipError_t launch_energies<T>(wfx_ctx *, double *, double *, hipStream_t);
+WFX_DECLARE_SPECIALISATIONS(double)
+WFX_DECLARE_SPECIALISATIONS(float)
+
+hipError_t launch_update_prediction(wfx_ctx *c, hipStream_t s);
+hipError_t launch_impose(wfx_ctx *c, int dim, bool accel, hipStream_t s);
+hipError_t launch_calc_accel(wfx_ctx *c, hipStream_t s);
+hipError_t launch_correction_accvel(wfx_ctx *c, hipStream_t s);
+hipError_t launch_correction_pos(wfx_ctx *c, hipStream_t s);
+hipError_t launch_reaction(wfx_ctx *c, int dim, double *dev_out, hipStream_t s);
+hipError_t launch_min_edge(wfx_ctx *c, double *partial, double *dev_out, hipStream_t s);
+int min_edge_partial_count(wfx_ctx *c);
+
+}  // namespace wfx

@@ -1,0 +1,173 @@
+"""Python mirror of MetFEM::Domain_d over the wfx C-ABI.
+
+Method names, argument meaning and call order follow the reference
+(include/common/Domain_d.h:238-830, src/explicit/Solver_explicit.C) so a
+reference solve loop reads the same here.  Every call goes to the HIP
+library; there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import WFX_BILINEAR, WFX_FP32_MIXED, WFX_FP64, WFX_HOLLOMON, check
+
+__all__ = ["Domain", "WFX_FP64", "WFX_FP32_MIXED", "WFX_BILINEAR", "WFX_HOLLOMON"]
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Domain:
+    """One explicit domain on the GPU.  ``stream`` is a hipStream_t handle
+    (int, e.g. torch.cuda.current_stream().cuda_stream); 0 = default stream."""
+
+    def __init__(self, precision=WFX_FP64, stream=0):
+        self._L = _lib.load()
+        h = ctypes.c_void_p()
+        check(self._L.wfx_create(ctypes.byref(h), int(precision)))
+        self._h = h
+        self.precision = precision
+        self.stream = ctypes.c_void_p(stream or None)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.wfx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- mesh --------------------------------------------------------------
+    def AddBoxLength(self, V, L, r, red_int=True, tritetra=False):
+        Va = np.ascontiguousarray(V, dtype=np.float64)
+        La = np.ascontiguousarray(L, dtype=np.float64)
+        check(self._L.wfx_add_box_length(self._h, _dptr(Va), _dptr(La), float(r), int(red_int), int(tritetra)))
+
+    def CreateFromMesh(self, x, elnod, nodxelem):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        el = np.ascontiguousarray(elnod, dtype=np.int32).reshape(-1)
+        check(self._L.wfx_set_mesh(self._h, x.size // 3, el.size // nodxelem, int(nodxelem),
+                                   x.ctypes.data, el.ctypes.data))
+
+    def sizes(self):
+        N, E, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(self._L.wfx_get_sizes(self._h, ctypes.byref(N), ctypes.byref(E), ctypes.byref(n)))
+        return N.value, E.value, n.value
+
+    @property
+    def node_count(self):
+        return self.sizes()[0]
+
+    @property
+    def elem_count(self):
+        return self.sizes()[1]
+
+    @property
+    def nodxelem(self):
+        return self.sizes()[2]
+
+    # ---- set-up ------------------------------------------------------------
+    def setDensity(self, rho):
+        check(self._L.wfx_set_density(self._h, float(rho)))
+
+    def setMaterial(self, model, E, nu, rho0, sy0, p1=0.0, p2=0.0):
+        check(self._L.wfx_set_material(self._h, int(model), E, nu, rho0, sy0, p1, p2))
+
+    def setElemSy0(self, sy0):
+        a = np.ascontiguousarray(sy0, dtype=np.float64)
+        check(self._L.wfx_set_elem_sy0(self._h, a.ctypes.data, a.size))
+
+    def AddBCVelNode(self, node, dim, val):
+        check(self._L.wfx_add_bc_vel_node(self._h, int(node), int(dim), float(val)))
+
+    def AllocateBCs(self):
+        check(self._L.wfx_allocate_bcs(self._h))
+
+    def SetDT(self, dt):
+        check(self._L.wfx_set_dt(self._h, float(dt)))
+
+    def setCH(self, alpha, beta, gamma):
+        check(self._L.wfx_set_ch(self._h, alpha, beta, gamma))
+
+    def set(self, name, value):
+        check(self._L.wfx_set_param(self._h, name.encode(), float(value)))
+
+    def get(self, name):
+        v = ctypes.c_double()
+        check(self._L.wfx_get_param(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def setReactionNodes(self, nodes):
+        a = np.ascontiguousarray(nodes, dtype=np.int32)
+        check(self._L.wfx_set_reaction_nodes(self._h, a.ctypes.data, a.size))
+
+    def calcMinEdgeLength(self):
+        v = ctypes.c_double()
+        check(self._L.wfx_calc_min_edge_length(self._h, ctypes.byref(v), self.stream))
+        return v.value
+
+    # ---- solver ------------------------------------------------------------
+    def init(self):
+        check(self._L.wfx_init(self._h, self.stream))
+
+    def _call(self, fn, *args):
+        check(fn(self._h, *args, self.stream))
+
+    def UpdatePrediction(self): self._call(self._L.wfx_update_prediction)
+    def ImposeBCV(self, dim): self._call(self._L.wfx_impose_bcv, int(dim))
+    def ImposeBCA(self, dim): self._call(self._L.wfx_impose_bca, int(dim))
+    def calcElemJAndDerivatives(self): self._call(self._L.wfx_calc_elem_jacobians)
+    def CalcElemVol(self): self._call(self._L.wfx_calc_elem_vol)
+    def calcElemDensity(self): self._call(self._L.wfx_calc_elem_density)
+    def CalcNodalVol(self): self._call(self._L.wfx_calc_nodal_vol)
+    def CalcNodalMassFromVol(self): self._call(self._L.wfx_calc_nodal_mass_from_vol)
+    def calcElemStrainRates(self): self._call(self._L.wfx_calc_elem_strain_rates)
+    def calcElemPressure(self): self._call(self._L.wfx_calc_elem_pressure)
+    def CalcStressStrain(self, dt): self._call(self._L.wfx_calc_stress_strain, float(dt))
+    def calcArtificialViscosity(self): self._call(self._L.wfx_calc_artificial_viscosity)
+    def calcElemForces(self): self._call(self._L.wfx_calc_elem_forces)
+    def calcElemHourglassForces(self): self._call(self._L.wfx_calc_elem_hourglass_forces)
+    def assemblyForces(self): self._call(self._L.wfx_assembly_forces)
+    def calcAccel(self): self._call(self._L.wfx_calc_accel)
+    def UpdateCorrectionAccVel(self): self._call(self._L.wfx_update_correction_accvel)
+    def UpdateCorrectionPos(self): self._call(self._L.wfx_update_correction_pos)
+
+    def step(self):
+        check(self._L.wfx_step_api(self._h, self.stream))
+
+    def run(self, nsteps, reaction=True):
+        hist = np.zeros(max(int(nsteps), 1), dtype=np.float64)
+        check(self._L.wfx_run(self._h, int(nsteps), hist.ctypes.data if reaction else None, self.stream))
+        return hist[:nsteps]
+
+    def reactionForce(self, dim=2):
+        v = ctypes.c_double()
+        check(self._L.wfx_reaction_force(self._h, int(dim), ctypes.byref(v), self.stream))
+        return v.value
+
+    def energies(self):
+        ek, de = ctypes.c_double(), ctypes.c_double()
+        check(self._L.wfx_energies(self._h, ctypes.byref(ek), ctypes.byref(de), self.stream))
+        return ek.value, de.value
+
+    def nonfiniteCount(self):
+        v = ctypes.c_long()
+        check(self._L.wfx_nonfinite_count(self._h, ctypes.byref(v)))
+        return v.value
+
+    # ---- fields (reference layout) ------------------------------------------
+    def field(self, name):
+        cnt, isint = ctypes.c_long(), ctypes.c_int()
+        check(self._L.wfx_field_count(self._h, name.encode(), ctypes.byref(cnt), ctypes.byref(isint)))
+        out = np.empty(cnt.value, dtype=np.int32 if isint.value else np.float64)
+        check(self._L.wfx_get_field(self._h, name.encode(), out.ctypes.data, out.size))
+        return out
+
+    def setField(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        check(self._L.wfx_set_field(self._h, name.encode(), a.ctypes.data, a.size))

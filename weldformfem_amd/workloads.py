@@ -1,0 +1,135 @@
+"""Synthetic workloads: the compression specimens the benchmark and the tests
+run, with a fixed pseudo-random generator so every run is reproducible.
+
+PRNG: SplitMix64.  Draw k (k = 1, 2, ...) of seed s is the SplitMix64 output
+for state s + k * 0x9E3779B97F4A7C15; a uniform double is (z >> 11) * 2^-53;
+a standard normal is Box-Muller on two consecutive uniforms,
+sqrt(-2 ln(1 - u1)) * cos(2 pi u2).
+
+Cube compression: E = 200 GPa, nu = 0.3, rho = 7850 kg/m^3, sigma_y = 300 MPa,
+bilinear hardening with tangent modulus 1 GPa, bottom face fixed in z, top
+face moving down at a prescribed speed, dt = 0.8 x the critical step (see
+cfl_dt).
+"""
+import math
+
+import numpy as np
+
+GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+STEEL = dict(E=200.0e9, nu=0.3, rho=7850.0, sy0=300.0e6, Et=1.0e9)
+
+
+def splitmix64(seed, n, start=1):
+    """n consecutive SplitMix64 outputs (draws start .. start+n-1)."""
+    with np.errstate(over="ignore"):
+        k = np.arange(start, start + n, dtype=np.uint64)
+        z = np.uint64(seed) + k * GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed, n, start=1):
+    return (splitmix64(seed, n, start) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def normal(seed, n, start=1):
+    u = uniform(seed, 2 * n, start)
+    u1, u2 = u[0::2], u[1::2]
+    return np.sqrt(-2.0 * np.log(1.0 - u1)) * np.cos(2.0 * math.pi * u2)
+
+
+# Critical Courant numbers of the Chung-Hulbert scheme (rho_b = 0.8182) on these
+# meshes, in units of min edge / dilatational wave speed, measured with the
+# oracle (elastic, 2000 steps): 1-point hexa stable at 0.75 and unstable at
+# 0.8; six-tet boxes stable at 0.5.  The reference's own rule,
+# 0.8 * min edge / sqrt(K/rho) (src/explicit/main.C:876), is 1.02 in these
+# units and diverges.
+CRITICAL_COURANT = {8: 0.75, 4: 0.4}
+
+
+def cfl_dt(min_edge, E=STEEL["E"], nu=STEEL["nu"], rho=STEEL["rho"], nodxelem=8, factor=0.8):
+    """dt = factor * critical Courant number * min edge / c_d"""
+    K = E / (3.0 * (1.0 - 2.0 * nu))
+    G = E / (2.0 * (1.0 + nu))
+    cd = math.sqrt((K + 4.0 * G / 3.0) / rho)
+    return factor * CRITICAL_COURANT[nodxelem] * min_edge / cd
+
+
+def box_layers(nx, ny, nz):
+    """Node ids of the bottom (z index 0) and top (z index nz) layers of a
+    structured (nx+1)(ny+1)(nz+1) node box numbered x fastest."""
+    layer = (nx + 1) * (ny + 1)
+    bottom = np.arange(layer, dtype=np.int32)
+    top = np.arange(nz * layer, (nz + 1) * layer, dtype=np.int32)
+    return bottom, top
+
+
+def cylinder_tets(cells=64, radius=0.5, height=1.0, jitter=0.05, seed=1):
+    """Tetrahedral cylinder: a cells^3 grid on [-R,R]^2 x [0,H]; a cell is kept
+    when its centre satisfies x^2 + y^2 < R^2; each kept cell is split into six
+    tets with the reference's hexa-corner pattern (Domain_d.C:1383-1388).
+    Only nodes used by kept cells are numbered (x fastest, then y, then z).
+    Every node is moved by U(-jitter*h, +jitter*h) per axis (draws in node
+    order, x then y then z).  Returns x[N,3], elnod[E,4], bottom and top
+    node ids (grid layers 0 and cells)."""
+    n = cells
+    hx = 2.0 * radius / n
+    hz = height / n
+    ci = (np.arange(n) + 0.5) * hx - radius
+    cz = np.arange(n)
+    X, Y = np.meshgrid(ci, ci, indexing="xy")  # [j, i]
+    inside = (X * X + Y * Y) < radius * radius  # [j, i]
+    keep = np.broadcast_to(inside, (n, n, n))  # [k, j, i]
+    kk, jj, ii = np.nonzero(keep)
+    p = n + 1
+    gid = lambda i, j, k: (k * p + j) * p + i
+    b1 = gid(ii, jj, kk)
+    b2 = gid(ii, jj + 1, kk)
+    lay = p * p
+    corners = np.stack([b1, b1 + 1, b2 + 1, b2, b1 + lay, b1 + lay + 1, b2 + lay + 1, b2 + lay], axis=1)
+    pattern = np.array([[0, 1, 3, 5], [1, 2, 3, 5], [0, 5, 3, 4], [4, 5, 3, 7], [5, 6, 3, 7], [5, 2, 3, 6]])
+    tets = corners[:, pattern].reshape(-1, 4)
+    used = np.zeros(p * p * p, dtype=bool)
+    used[tets.reshape(-1)] = True
+    old_ids = np.nonzero(used)[0]
+    new_id = -np.ones(p * p * p, dtype=np.int64)
+    new_id[old_ids] = np.arange(old_ids.size)
+    elnod = new_id[tets].astype(np.int32)
+    gi = old_ids % p
+    gj = (old_ids // p) % p
+    gk = old_ids // (p * p)
+    x = np.stack([gi * hx - radius, gj * hx - radius, gk * hz], axis=1)
+    h = min(hx, hz)
+    r = uniform(seed, 3 * old_ids.size).reshape(-1, 3)
+    x = x + (2.0 * r - 1.0) * (jitter * h)
+    bottom = np.nonzero(gk == 0)[0].astype(np.int32)
+    top = np.nonzero(gk == n)[0].astype(np.int32)
+    return np.ascontiguousarray(x), np.ascontiguousarray(elnod), bottom, top
+
+
+def lognormal_sy0(n, median=300.0e6, sigma=0.1, seed=2):
+    return median * np.exp(sigma * normal(seed, n))
+
+
+def setup_compression(dom, bottom, top, v_top, mat=STEEL, sy0=None, hg_coeff=0.06, dt=None):
+    """Material, boundary conditions, reaction set and time step for a
+    compression specimen on a Domain (wfx) or RefDomain (oracle)."""
+    dom.setDensity(mat["rho"])
+    dom.setMaterial(0, mat["E"], mat["nu"], mat["rho"], mat["sy0"], mat["Et"], 0.0)
+    if sy0 is not None:
+        dom.setElemSy0(sy0)
+    for nd in bottom:
+        dom.AddBCVelNode(int(nd), 2, 0.0)
+    for nd in top:
+        dom.AddBCVelNode(int(nd), 2, float(v_top))
+    if hasattr(dom, "AllocateBCs"):
+        dom.AllocateBCs()
+    dom.setReactionNodes(top)
+    dom.set("hg_coeff", hg_coeff)
+    if dt is None:
+        dt = cfl_dt(dom.calcMinEdgeLength(), mat["E"], mat["nu"], mat["rho"], dom.nodxelem)
+    dom.SetDT(dt)
+    return dt
