@@ -1,0 +1,310 @@
+"""Benchmark: element-steps/s of the explicit updated-Lagrangian step on one
+MI355X, FP64 and FP32-mixed, with the error of the mixed path, the HBM
+roofline fraction of the dominant kernel and the oracle timed on the host.
+
+Workload: 2 x 2 x 1 m block of 200 x 200 x 100 reduced-integration hexa
+(4,000,000 elements, 4,080,501 nodes), steel, bilinear hardening with a
+per-element lognormal yield stress (median 300 MPa, sigma 0.1, seed 2), bottom
+face fixed in z, top face at vz = -2 m/s, viscous hourglass 0.06.  A step is
+one Chung-Hulbert time step over the whole mesh with all state resident in
+HBM.  The headline value is the fused four-kernel step (wfx_step_fused) in
+FP64; the per-step Domain_d API path (wfx_run) is timed beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+I4 = 4
+
+
+def fused_bytes(n, s, hg=True):
+    """Algorithmic bytes per element and per node of the fused kernels: each
+    array a kernel touches is moved once, gathered nodal data once per node.
+    s = element real size (8 FP64, 4 FP32-mixed); nodal arrays are double."""
+    f3_reals = 12 + 12 + 1 + 2 + 1 + 1 + (1 if hg else 0) + 3 * n  # tau, eps rw; p; ep rw; sy0; sigma_y; rho; f
+    return {
+        "f1_volume": (n * I4 + s, 3 * 8),
+        "f2_nodal": (n * I4 + 2 * s, 2 * I4 + 8 + 3 * 8),
+        "f3_element": (n * I4 + f3_reals * s, 3 * 8 + 3 * 8 + 8),
+        "f4_node": (n * I4 + 3 * n * s, 2 * I4 + 8 + I4 + 3 * 8 + 5 * 2 * 3 * 8),
+    }
+
+
+def api_bytes(n, s, hg=True):
+    """the same for the per-step API kernels (one kernel per Domain_d call)"""
+    return {
+        "update_prediction": (0, 12 * 8),
+        "elem_jacobians": (n * I4 + (1 + 3 * n) * s, 3 * 8),
+        "elem_vol": (2 * s, 0),
+        "nodal_vol": (n * I4 + s, 8 + 2 * I4),
+        "nodal_mass": (n * I4 + s, 2 * 8 + 2 * I4),
+        "strain_rates": (n * I4 + (1 + 3 * n + 9) * s, 3 * 8),
+        "pressure": (2 * n * I4 + 5 * s, 4 * 8 + 2 * I4),
+        "stress_strain": (50 * s, 0),
+        "elem_forces": ((6 * n + 6) * s, 0),
+        "hourglass": ((n * I4 + (2 + 3 * n) * s) if hg else 0, 3 * 8 if hg else 0),
+        "assembly": (2 * n * I4 + (3 * n * (2 if hg else 1)) * s, 3 * 8 + 2 * I4),
+        "calc_accel": (0, 10 * 8),
+        "correction_accvel": (0, 15 * 8),
+        "correction_pos": (0, 24 * 8),
+    }
+
+
+def total(bytes_model, E, N):
+    return sum(be * E + bn * N for be, bn in bytes_model.values())
+
+
+def build_domain(precision, nx, ny, nz, L, v_top, stream):
+    from weldformfem_amd import Domain, workloads
+
+    d = Domain(precision, stream=stream)
+    d.AddBoxLength([0.0, 0.0, 0.0], L, 0.5 * L[0] / nx)
+    N, E, n = d.sizes()
+    assert (N, E, n) == ((nx + 1) * (ny + 1) * (nz + 1), nx * ny * nz, 8), (N, E, n)
+    bottom, top = workloads.box_layers(nx, ny, nz)
+    sy0 = workloads.lognormal_sy0(E, seed=2)
+    dt = workloads.setup_compression(d, bottom, top, v_top, sy0=sy0, hg_coeff=0.06)
+    d.init()
+    return d, dt
+
+
+def timed(fn, torch):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def event_times(calls, reps, torch):
+    """average duration (s) of each call, HIP events recorded on the stream
+    the kernels are launched on (torch's current stream)"""
+    out = {}
+    stream = torch.cuda.current_stream()
+    for name, fn in calls.items():
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        out[name] = e0.elapsed_time(e1) * 1e-3 / reps
+    return out
+
+
+def fused_calls(d):
+    return {
+        "f1_volume": lambda: d.fusedPhase(1),
+        "f2_nodal": lambda: d.fusedPhase(2),
+        "f3_element": lambda: d.fusedPhase(3),
+        "f4_node": lambda: d.fusedPhase(4, True),
+    }
+
+
+def api_calls(d):
+    return {
+        "update_prediction": d.UpdatePrediction,
+        "elem_jacobians": d.calcElemJAndDerivatives,
+        "elem_vol": d.CalcElemVol,
+        "nodal_vol": d.CalcNodalVol,
+        "nodal_mass": d.CalcNodalMassFromVol,
+        "strain_rates": d.calcElemStrainRates,
+        "pressure": d.calcElemPressure,
+        "stress_strain": lambda: d.CalcStressStrain(0.0),
+        "elem_forces": d.calcElemForces,
+        "hourglass": d.calcElemHourglassForces,
+        "assembly": d.assemblyForces,
+        "calc_accel": d.calcAccel,
+        "correction_accvel": d.UpdateCorrectionAccVel,
+        "correction_pos": d.UpdateCorrectionPos,
+    }
+
+
+def per_kernel_table(times, model, E, N):
+    out = {}
+    for k, t in times.items():
+        be, bn = model[k]
+        b = be * E + bn * N
+        out[k] = {"us": round(t * 1e6, 2), "GB/s": round(b / t / 1e9, 1), "frac": round(b / t / HBM_PEAK, 3)}
+    return out
+
+
+def cpu_baseline(seconds_target=12.0):
+    """oracle (reference-faithful C restatement, OpenMP) on a slab of the same
+    workload: 200 x 200 x 10 hexa at the same element size"""
+    from oracle import wfref
+    from weldformfem_amd import workloads
+
+    nx, ny, nz = 200, 200, 10
+    r = wfref.RefDomain()
+    r.AddBoxLength([0.0, 0.0, 0.0], [2.0, 2.0, 0.1], 0.005)
+    E = r.elem_count
+    bottom, top = workloads.box_layers(nx, ny, nz)
+    workloads.setup_compression(r, bottom, top, -2.0, sy0=workloads.lognormal_sy0(E, seed=2), hg_coeff=0.06)
+    r.init()
+    t0 = time.perf_counter()
+    r.run(2)
+    per = (time.perf_counter() - t0) / 2
+    n = max(2, min(400, int(seconds_target / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    r.run(n)
+    per = (time.perf_counter() - t0) / n
+    return {
+        "value": E / per,
+        "unit": "element-steps/s",
+        "cores": wfref.num_threads(),
+        "kind": "port",
+        "sample": f"oracle (C restatement, OpenMP) on 200x200x10 hexa ({E} elements) of the same workload, "
+                  f"{n} steps after init, in-loop timing",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--err-steps", type=int, default=2000, help="steps after which the mixed error is taken")
+    ap.add_argument("--quick", action="store_true", help="smaller mesh (100x100x50), no CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-api", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(sharding across GPUs is a design only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from weldformfem_amd import WFX_FP32_MIXED, WFX_FP64
+
+    torch.cuda.init()
+    stream = torch.cuda.current_stream().cuda_stream
+    nx, ny, nz, L = (200, 200, 100, [2.0, 2.0, 1.0]) if not args.quick else (100, 100, 50, [2.0, 2.0, 1.0])
+
+    def log(msg):
+        print("bench: " + msg, file=sys.stderr, flush=True)
+
+    log(f"building {nx}x{ny}x{nz} hexa FP64")
+    d64, dt = build_domain(WFX_FP64, nx, ny, nz, L, -2.0, stream)
+    N, E, n = d64.sizes()
+    steps64 = 0
+    d64.runFused(args.warmup, reaction=False)
+    steps64 += args.warmup
+    s64 = timed(lambda: d64.runFused(args.steps, reaction=False), torch) / args.steps
+    steps64 += args.steps
+    log(f"FP64 fused {s64 * 1e3:.3f} ms/step")
+
+    api = None
+    if not args.no_api:
+        s64a = timed(lambda: d64.run(args.steps, reaction=False), torch) / args.steps
+        steps64 += args.steps
+        log(f"FP64 per-step API {s64a * 1e3:.3f} ms/step")
+        api = {"value": E / s64a, "ms_per_step": s64a * 1e3,
+               "step_algorithmic_bytes_per_element": total(api_bytes(n, 8), E, N) / E,
+               "step_hbm_frac": total(api_bytes(n, 8), E, N) / s64a / HBM_PEAK}
+
+    mixed = None
+    if not args.no_mixed:
+        log("building FP32-mixed")
+        d32, _ = build_domain(WFX_FP32_MIXED, nx, ny, nz, L, -2.0, stream)
+        d32.runFused(args.warmup, reaction=False)
+        s32 = timed(lambda: d32.runFused(args.steps, reaction=False), torch) / args.steps
+        steps32 = args.warmup + args.steps
+        log(f"FP32-mixed fused {s32 * 1e3:.3f} ms/step")
+        # accuracy: both paths to err_steps, then one more step with the reaction
+        target = max(args.err_steps, steps64, steps32)
+        d64.runFused(target - steps64, reaction=False)
+        d32.runFused(target - steps32, reaction=False)
+        r64 = d64.runFused(1)[0]
+        r32 = d32.runFused(1)[0]
+
+        def rel(a, b):
+            return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+        mixed = {
+            "value": E / s32,
+            "ms_per_step": s32 * 1e3,
+            "speedup_vs_fp64": s64 / s32,
+            "dtype": "f32 element arithmetic, f64 nodal kinematics",
+            "rel_err_vs_fp64": {"displacement": rel(d32.field("u"), d64.field("u")),
+                                "reaction_force": abs(r32 - r64) / abs(r64),
+                                "eq_plastic_strain": rel(d32.field("pl_strain"), d64.field("pl_strain")),
+                                "max_eq_plastic_strain": float(d64.field("pl_strain").max()),
+                                "after_steps": target + 1,
+                                "reference": "FP64 GPU path (parity-tested against the oracle)"},
+            "step_algorithmic_bytes_per_element": total(fused_bytes(n, 4), E, N) / E,
+            "step_hbm_frac": total(fused_bytes(n, 4), E, N) / s32 / HBM_PEAK,
+        }
+        kt32 = event_times(fused_calls(d32), args.kernel_reps, torch)
+        mixed["per_kernel"] = per_kernel_table(kt32, fused_bytes(n, 4), E, N)
+        log(f"mixed error: {mixed['rel_err_vs_fp64']}")
+
+    kt64 = event_times(fused_calls(d64), args.kernel_reps, torch)
+    fb64 = fused_bytes(n, 8)
+    dom = max(kt64, key=kt64.get)
+    be, bn = fb64[dom]
+    dom_bytes = be * E + bn * N
+    achieved = dom_bytes / kt64[dom] / 1e9
+    step_bytes = total(fb64, E, N)
+    if api is not None:
+        api["per_kernel"] = per_kernel_table(event_times(api_calls(d64), args.kernel_reps, torch),
+                                             api_bytes(n, 8), E, N)
+
+    cpu = None
+    if not args.no_cpu and not args.quick:
+        log("CPU baseline (oracle)")
+        cpu = cpu_baseline()
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(f"{nx}x{ny}x{nz}", {}).get(dom)
+
+    out = {
+        "metric": "element-steps/s at 1 GPU (FP64 vs FP32-mixed, stated err); % HBM peak",
+        "value": E / s64,
+        "unit": "element-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": s64 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"hexa compression block {nx}x{ny}x{nz} ({E} elements, {N} nodes), "
+                               "fused four-kernel FP64 step",
+                   "elements": E, "nodes": N, "dt": dt, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic,
+                     "algorithmic_bytes": dom_bytes, "avg_launch_us": kt64[dom] * 1e6},
+        "step_algorithmic_bytes_per_element": step_bytes / E,
+        "step_hbm_frac": step_bytes / s64 / HBM_PEAK,
+        "per_kernel": per_kernel_table(kt64, fb64, E, N),
+        "fp32_mixed": mixed,
+        "api_path_fp64": api,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

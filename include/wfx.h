@@ -122,6 +122,17 @@ int wfx_step_api(wfx_ctx *ctx, wfx_stream s);
 /* nsteps loop bodies; reaction-force history (length nsteps, host memory,
  * may be NULL) is copied back at the end. */
 int wfx_run(wfx_ctx *ctx, int nsteps, double *reaction_hist, wfx_stream s);
+/* nsteps loop bodies of the same sequence regrouped into four fused kernels
+ * per step (two element passes, two node passes).  Same state on return as
+ * wfx_run, except that the per-step intermediates detJ, dH*, str_rate,
+ * rot_rate, strain_pl_incr, f_elem and f_hg are not kept (reading them fails
+ * with WFX_ERR_STATE until a per-step API step has run). */
+int wfx_step_fused(wfx_ctx *ctx, int nsteps, double *reaction_hist, wfx_stream s);
+/* One kernel of the fused step on the current state (no counterpart in the
+ * reference; for per-kernel timing and profiling): phase 1 element volumes,
+ * 2 nodal sums, 3 element chain, 4 node update (predict: also run the next
+ * prediction), 5 sigma = -p I + tau. */
+int wfx_fused_phase(wfx_ctx *ctx, int phase, int predict, wfx_stream s);
 /* Sum of internal force component dim over the reaction nodes (synchronous). */
 int wfx_reaction_force(wfx_ctx *ctx, int dim, double *out, wfx_stream s);
 /* Domain_d::computeEnergies (Mechanical.C:2145-2185), synchronous. */

@@ -936,3 +936,15 @@ int wfref_set_double(wfref_domain *d, const char *name, double v) {
     if (strcmp(t[k].nm, name) == 0) { *t[k].p = v; return 0; }
   return -1;
 }
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+/* OpenMP threads the oracle's parallel loops use */
+int wfref_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

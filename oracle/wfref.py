@@ -55,6 +55,7 @@ def lib():
             "wfref_calc_stress_strain": (None, [vp, d]),
             "wfref_impose_bcv": (None, [vp, i]),
             "wfref_impose_bca": (None, [vp, i]),
+            "wfref_num_threads": (i, []),
         }
         for name in ["update_prediction", "calc_elem_jacobians", "calc_elem_vol",
                      "calc_elem_density", "calc_nodal_vol", "calc_nodal_mass_from_vol",
@@ -70,6 +71,10 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+def num_threads():
+    return lib().wfref_num_threads()
 
 
 def _ptr(a):

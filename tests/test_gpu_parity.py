@@ -253,3 +253,64 @@ def test_min_edge_length_and_energies():
     x, el, n = jittered_box(6, 5, 4, 0.02, True, seed=13)
     g, r = pair(x, el, n)
     assert g.calcMinEdgeLength() == r.calcMinEdgeLength()
+
+
+def _trajectory_fused(make_mesh, v_top, nsteps, precision=WFX_FP64, hg=0.06, chunks=(1,)):
+    x, el, n, bottom, top = make_mesh()
+    g = Domain(precision)
+    g.CreateFromMesh(x, el, n)
+    dt = workloads.setup_compression(g, bottom, top, v_top, hg_coeff=hg)
+    r = RefDomain()
+    r.CreateFromMesh(x, el, n)
+    workloads.setup_compression(r, bottom, top, v_top, hg_coeff=hg, dt=dt)
+    g.init()
+    r.init()
+    hist = []
+    done = 0
+    for k in chunks:  # several calls: the prediction hand-over between calls
+        hist.append(g.runFused(k))
+        done += k
+    hist.append(g.runFused(nsteps - done))
+    rr = r.run(nsteps)
+    return g, r, np.concatenate(hist), rr
+
+
+@pytest.mark.parametrize("mesh", [_hexa20, _cyl16])
+def test_fused_trajectory_100_steps_fp64(mesh):
+    say(f"case: fused 100-step FP64 trajectory {mesh.__name__}")
+    g, r, rg, rr = _trajectory_fused(mesh, -20.0, 100, chunks=(1, 30))
+    assert np.max(r.field("pl_strain")) > 0
+    for f in ["u", "v", "x", "a", "prev_a", "u_dt", "fi", "mdiag", "voln"]:
+        e = rel(g.field(f), r.field(f))
+        assert e <= 1e-10, f"{f}: {e:.3e}"
+    for f in ["pl_strain", "sigma", "tau", "eps", "p", "sigma_y", "vol"]:
+        e = rel(g.field(f), r.field(f))
+        assert e <= 1e-10, f"{f}: {e:.3e}"
+    assert rel(rg, rr) <= 1e-10
+    assert g.nonfiniteCount() == 0
+
+
+def test_fused_without_hourglass_is_bitexact():
+    say("case: fused FP64 without hourglass, bit-exact vs oracle")
+    g, r, rg, rr = _trajectory_fused(_hexa20, -20.0, 60, hg=0.0)
+    for f in ["u", "v", "x", "pl_strain", "tau", "p"]:
+        np.testing.assert_array_equal(g.field(f), r.field(f), err_msg=f)
+
+
+@pytest.mark.parametrize("mesh", [_hexa20, _cyl16])
+def test_fused_trajectory_fp32_mixed(mesh):
+    say(f"case: fused 100-step FP32-mixed vs FP64 oracle {mesh.__name__}")
+    g, r, rg, rr = _trajectory_fused(mesh, -20.0, 100, precision=WFX_FP32_MIXED)
+    eu, er, ep = rel(g.field("u"), r.field("u")), rel(rg, rr), rel(g.field("pl_strain"), r.field("pl_strain"))
+    say(f"  fused FP32-mixed rel err: u {eu:.3e} reaction {er:.3e} pl_strain {ep:.3e}")
+    assert eu <= 1e-4 and er <= 1e-3 and ep <= 1e-3
+
+
+def test_fused_intermediates_report_state_error():
+    say("case: fused path intermediates raise until a per-step API step")
+    from weldformfem_amd._lib import WfxError
+    g, r, rg, rr = _trajectory_fused(_hexa20, -1.0, 3)
+    with pytest.raises(WfxError):
+        g.field("str_rate")
+    g.step()
+    assert g.field("str_rate").size == 6 * g.elem_count

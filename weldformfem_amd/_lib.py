@@ -59,6 +59,8 @@ SIGNATURES = {
     "wfx_update_correction_pos": (_i, [_vp, _vp]),
     "wfx_step_api": (_i, [_vp, _vp]),
     "wfx_run": (_i, [_vp, _i, _vp, _vp]),
+    "wfx_step_fused": (_i, [_vp, _i, _vp, _vp]),
+    "wfx_fused_phase": (_i, [_vp, _i, _i, _vp]),
     "wfx_reaction_force": (_i, [_vp, _i, _pd, _vp]),
     "wfx_energies": (_i, [_vp, _pd, _pd, _vp]),
     "wfx_nonfinite_count": (_i, [_vp, ctypes.POINTER(ctypes.c_long)]),

@@ -41,6 +41,10 @@ static void free_all(wfx_ctx *c) {
   c->e32 = DevElem<float>();
   for (int d = 0; d < 3; ++d) { c->bc_nod[d] = nullptr; c->bc_val[d] = nullptr; c->bc_count[d] = 0; }
   c->react = nullptr;
+  c->bc_slot = c->bc_mask = nullptr;
+  c->bc_tab = nullptr;
+  c->has_fe = 0;
+  c->fused_stale = false;
   c->scalars = nullptr;
   c->status = nullptr;
   c->hist = nullptr;
@@ -70,9 +74,9 @@ static int setup_device(wfx_ctx *c) {
   if (e == hipSuccess) e = dalloc(c, &ptr, (cnt));
   A(c->nd.x, 3 * N) A(c->nd.v, 3 * N) A(c->nd.a, 3 * N) A(c->nd.u, 3 * N) A(c->nd.u_dt, 3 * N)
   A(c->nd.prev_a, 3 * N) A(c->nd.fi, 3 * N) A(c->nd.fe, 3 * N) A(c->nd.mdiag, N) A(c->nd.voln, N)
-  A(c->nd.voln_sum, N) A(c->nd.voln0_sum, N)
+  A(c->nd.voln_sum, N) A(c->nd.voln0_sum, N) A(c->nd.anp_ratio, N)
   A(c->ms.elnod, n * E) A(c->ms.nodel, n * E) A(c->ms.nodel_loc, n * E) A(c->ms.nodel_offset, N)
-  A(c->ms.nodel_count, N) A(c->scalars, 64) A(c->status, 16)
+  A(c->ms.nodel_count, N) A(c->ms.nodel_pk, n * E) A(c->bc_slot, N) A(c->scalars, 64) A(c->status, 16)
 #undef A
   if (e == hipSuccess) e = (c->precision == WFX_FP64) ? alloc_elem(c, c->e64) : alloc_elem(c, c->e32);
   if (e != hipSuccess) {
@@ -93,6 +97,10 @@ static int setup_device(wfx_ctx *c) {
   WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_loc, c->nodel_loc_h.data(), n * E * sizeof(int), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_offset, c->nodel_offset_h.data(), N * sizeof(int), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_count, c->nodel_count_h.data(), N * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<int> pk(n * E);
+  for (size_t k = 0; k < n * E; ++k) pk[k] = (c->nodel_h[k] << 3) | c->nodel_loc_h[k];
+  WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_pk, pk.data(), n * E * sizeof(int), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemset(c->bc_slot, 0xff, N * sizeof(int)));  // -1: no boundary condition
   c->has_mesh = true;
   c->initialized = false;
   return WFX_OK;
@@ -387,6 +395,33 @@ int wfx_allocate_bcs(wfx_ctx *c) {
     }
     c->bc_count[d] = cnt;
   }
+  // per-node view of the same data for the fused node pass
+  std::vector<int> slot(c->N, -1), mask;
+  std::vector<double> tab;
+  for (int d = 0; d < 3; ++d) {
+    const auto &nod = c->bc_nod_h[d];
+    for (size_t k = 0; k < nod.size(); ++k) {
+      int g = nod[k];
+      if (slot[g] < 0) {
+        slot[g] = (int)mask.size();
+        mask.push_back(0);
+        tab.insert(tab.end(), {0.0, 0.0, 0.0});
+      }
+      mask[slot[g]] |= 1 << d;
+      tab[3 * slot[g] + d] = c->bc_val_h[d][k];  // last entry wins
+    }
+  }
+  if (!mask.empty()) {
+    int *pm = nullptr;
+    double *pt = nullptr;
+    WFX_CHECK_HIP(dalloc(c, &pm, mask.size()));
+    WFX_CHECK_HIP(dalloc(c, &pt, tab.size()));
+    WFX_CHECK_HIP(hipMemcpy(pm, mask.data(), mask.size() * sizeof(int), hipMemcpyHostToDevice));
+    WFX_CHECK_HIP(hipMemcpy(pt, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->bc_mask = pm;
+    c->bc_tab = pt;
+  }
+  WFX_CHECK_HIP(hipMemcpy(c->bc_slot, slot.data(), c->N * sizeof(int), hipMemcpyHostToDevice));
   return WFX_OK;
 }
 
@@ -579,6 +614,7 @@ static int step_api(wfx_ctx *c, hipStream_t st, double *hist) {
   LAUNCH(launch_correction_accvel(c, st));
   for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
   LAUNCH(launch_correction_pos(c, st));
+  c->fused_stale = false;  // every per-step intermediate is current again
   return WFX_OK;
 }
 
@@ -612,6 +648,45 @@ int wfx_run(wfx_ctx *c, int nsteps, double *reaction_hist, wfx_stream s) {
   return WFX_OK;
 }
 
+int wfx_step_fused(wfx_ctx *c, int nsteps, double *reaction_hist, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->initialized, WFX_ERR_STATE, "call wfx_init first");
+  REQUIRE(nsteps >= 0, WFX_ERR_ARG, "nsteps must be >= 0");
+  if (nsteps == 0) return WFX_OK;
+  hipStream_t st = as_stream(s);
+  double *dh = nullptr;
+  if (reaction_hist) {
+    if (c->hist_cap < nsteps) {
+      WFX_CHECK_HIP(dalloc(c, &c->hist, nsteps));
+      c->hist_cap = nsteps;
+    }
+    dh = c->hist;
+  }
+  // V0_n (vol0 may have been replaced through wfx_set_field), then the first
+  // prediction; each fused step predicts the next one in its node pass.
+  LAUNCH(BY_PRECISION(c, launch_vsum0, c, st));
+  LAUNCH(launch_update_prediction(c, st));
+  for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
+  LAUNCH(BY_PRECISION(c, launch_fused_steps, c, nsteps, dh, st));
+  // the reference leaves a == prev_a after UpdateCorrectionPos
+  WFX_CHECK_HIP(hipMemcpyAsync(c->nd.a, c->nd.prev_a, 3L * c->N * sizeof(double), hipMemcpyDeviceToDevice, st));
+  c->fused_stale = true;
+  if (dh) {
+    WFX_CHECK_HIP(hipMemcpyAsync(reaction_hist, dh, nsteps * sizeof(double), hipMemcpyDeviceToHost, st));
+    WFX_CHECK_HIP(hipStreamSynchronize(st));
+  }
+  return WFX_OK;
+}
+
+int wfx_fused_phase(wfx_ctx *c, int phase, int predict, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->initialized, WFX_ERR_STATE, "call wfx_init first");
+  REQUIRE(phase >= 1 && phase <= 5, WFX_ERR_ARG, "phase must be 1..5");
+  LAUNCH(BY_PRECISION(c, launch_fused_phase, c, phase, predict, as_stream(s)));
+  c->fused_stale = true;
+  return WFX_OK;
+}
+
 int wfx_reaction_force(wfx_ctx *c, int dim, double *out, wfx_stream s) {
   REQUIRE_MESH(c);
   REQUIRE(out && dim >= 0 && dim < 3, WFX_ERR_ARG, "bad arguments");
@@ -625,6 +700,7 @@ int wfx_reaction_force(wfx_ctx *c, int dim, double *out, wfx_stream s) {
 int wfx_energies(wfx_ctx *c, double *ekin, double *deint, wfx_stream s) {
   REQUIRE_MESH(c);
   REQUIRE(ekin && deint, WFX_ERR_ARG, "null output");
+  REQUIRE(!c->fused_stale, WFX_ERR_STATE, "energies need the strain rates of a per-step API step");
   hipStream_t st = as_stream(s);
   int nb = (std::max(c->N, c->E) + 255) / 256;
   double *partial = nullptr;
@@ -659,11 +735,23 @@ int wfx_field_count(wfx_ctx *c, const char *name, long *count, int *is_int) {
   return WFX_OK;
 }
 
+// Per-step intermediates that the fused path keeps in registers only.
+static bool fused_intermediate(const char *name) {
+  static const char *k[] = {"detJ", "dHdx", "dHdy", "dHdz", "str_rate", "rot_rate",
+                            "strain_pl_incr", "f_elem", "f_hg"};
+  for (const char *s : k)
+    if (!std::strcmp(s, name)) return true;
+  return false;
+}
+
 int wfx_get_field(wfx_ctx *c, const char *name, void *host, long count) {
   REQUIRE_MESH(c);
   FieldDesc f;
   REQUIRE(name && find_field(c, name, f), WFX_ERR_ARG, std::string("unknown field ") + (name ? name : ""));
   REQUIRE(host && count == f.cnt * f.rows, WFX_ERR_ARG, std::string("size mismatch for field ") + name);
+  REQUIRE(!(c->fused_stale && fused_intermediate(name)), WFX_ERR_STATE,
+          std::string(name) + " is a per-step intermediate not kept by wfx_step_fused; "
+          "call the per-step entry point that produces it first");
   WFX_CHECK_HIP(hipDeviceSynchronize());
   const long cnt = f.cnt;
   if (f.is_int) {
@@ -708,6 +796,11 @@ int wfx_set_field(wfx_ctx *c, const char *name, const void *host, long count) {
   } else {
     for (long i = 0; i < cnt; ++i)
       for (int r = 0; r < f.rows; ++r) tmp[(long)r * cnt + i] = h[i * f.rows + r];
+  }
+  if (!std::strcmp(name, "fe")) {
+    c->has_fe = 0;
+    for (double v : tmp)
+      if (v != 0.0) { c->has_fe = 1; break; }
   }
   return upload_real(c, f.dev, tmp.data(), tmp.size(), f.elem);
 }

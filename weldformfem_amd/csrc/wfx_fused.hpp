@@ -1,0 +1,552 @@
+// Fused explicit step: the per-step Domain_d sequence regrouped into four
+// kernels with one grid-wide dependency between each pair.
+//
+//   F1 element pass   x gather -> J -> vol                      (CalcElemVol)
+//   F2 node pass      CSR sum of vol -> V_n, ANP ratio V_n/V0_n, nodal mass
+//                                                  (CalcNodalVol,
+//                                                   CalcNodalMassFromVol,
+//                                                   calcElemPressure nodes)
+//   F3 element pass   J and dH recomputed from x, strain rate and spin from
+//                     the predicted v, ANP pressure, Jaumann + J2 return,
+//                     internal and hourglass forces -> f_elem - f_hg
+//                                                  (calcElemJAndDerivatives,
+//                                                   calcElemStrainRates,
+//                                                   calcElemPressure,
+//                                                   CalcStressStrain,
+//                                                   calcElemForces,
+//                                                   calcElemHourglassForces)
+//   F4 node pass      CSR gather of element forces, acceleration, BCs,
+//                     Chung-Hulbert correction, position update and the
+//                     next step's prediction   (assemblyForces, calcAccel,
+//                                                   ImposeBCA/BCV,
+//                                                   UpdateCorrection*,
+//                                                   UpdatePrediction)
+//
+// Every expression is the one of the per-step kernels, so the FP64 fused path
+// differs from the per-step path only where the hourglass force is
+// subtracted per element instead of after the element-force sum.
+//
+// Device inputs beyond the per-step state:
+//   nodel_pk[nnz]  packed CSR entry (element << 3) | local node
+//   bc_slot[N]     -1 or index into bc_mask / bc_val (per-node Dirichlet data)
+//   anp_ratio[N]   V_n / V0_n,   vsum0[N] = V0_n (constant after init)
+#pragma once
+
+#include "wfx_kernels.hpp"
+
+namespace wfx {
+namespace {
+
+// V0_n: CSR-ordered sum of the initial volumes (calcElemPressure :698-705)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void f0_vsum0(int N, const int *__restrict__ off,
+                                                   const int *__restrict__ cnt,
+                                                   const int *__restrict__ nodel_pk,
+                                                   const T *__restrict__ vol0,
+                                                   double *__restrict__ vsum0) {
+  int i = gtid();
+  if (i >= N) return;
+  int o = off[i], c = cnt[i];
+  double s0 = 0.0;
+  for (int k = 0; k < c; ++k) s0 += (double)vol0[nodel_pk[o + k] >> 3];
+  vsum0[i] = s0;
+}
+
+// F1: vol = detJ * w (calcElemJAndDerivatives determinant + CalcElemVol)
+template <typename T, int NN>
+__global__ __launch_bounds__(kBlock) void f1_volume(int E, int N, const int *__restrict__ elnod,
+                                                    const double *__restrict__ x,
+                                                    T *__restrict__ vol) {
+  int e = gtid();
+  if (e >= E) return;
+  T X[NN][3];
+  if constexpr (std::is_same<T, double>::value) {
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      int g = elnod[(long)a * E + e];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) X[a][c] = x[(long)c * N + g];
+    }
+  } else {
+    int g0 = elnod[e];
+    double x0[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { x0[c] = x[(long)c * N + g0]; X[0][c] = T(0); }
+#pragma unroll
+    for (int a = 1; a < NN; ++a) {
+      int g = elnod[(long)a * E + e];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) X[a][c] = T(x[(long)c * N + g] - x0[c]);
+    }
+  }
+  T J[3][3];
+  if constexpr (NN == 8) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      J[0][c] = T(0.125) * (-X[0][c] + X[1][c] + X[2][c] - X[3][c] - X[4][c] + X[5][c] + X[6][c] - X[7][c]);
+      J[1][c] = T(0.125) * (-X[0][c] - X[1][c] + X[2][c] + X[3][c] - X[4][c] - X[5][c] + X[6][c] + X[7][c]);
+      J[2][c] = T(0.125) * (-X[0][c] - X[1][c] - X[2][c] - X[3][c] + X[4][c] + X[5][c] + X[6][c] + X[7][c]);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      J[0][c] = X[1][c] - X[0][c];
+      J[1][c] = X[2][c] - X[0][c];
+      J[2][c] = X[3][c] - X[0][c];
+    }
+  }
+  T det = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[1][2] * J[2][1] - J[0][1] * J[1][0] * J[2][2] +
+          J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] - J[0][2] * J[1][1] * J[2][0];
+  const T w = (NN == 4) ? T(1.0 / 6.0) : T(8.0);
+  T v = T(0);
+  v += det * w * T(1);
+  vol[e] = v;
+}
+
+// F2: nodal volume sum in CSR order; ratio for the ANP pressure, nodal
+// volume and nodal mass.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void f2_nodal(int N, int nodxelem, const int *__restrict__ off,
+                                                   const int *__restrict__ cnt,
+                                                   const int *__restrict__ nodel_pk,
+                                                   const T *__restrict__ vol,
+                                                   const T *__restrict__ rho,
+                                                   const double *__restrict__ vsum0,
+                                                   double *__restrict__ ratio,
+                                                   double *__restrict__ voln,
+                                                   double *__restrict__ mdiag) {
+  int i = gtid();
+  if (i >= N) return;
+  int o = off[i], c = cnt[i];
+  double s = 0.0;
+  for (int k = 0; k < c; ++k) s += (double)vol[nodel_pk[o + k] >> 3];
+  ratio[i] = s / vsum0[i];
+  double vn = s;
+  vn /= nodxelem;
+  voln[i] = vn;
+  double dc = (double)c, mass = 0.0;
+  for (int k = 0; k < c; ++k) mass += 1.0 * (double)rho[nodel_pk[o + k] >> 3] * vn / dc;
+  mdiag[i] = mass;
+}
+
+struct FusedParams {
+  double dt;
+  MatParams mp;
+  PressParams pp;     // pp.full handled as in k_pressure_elem
+  int need_jlocal;    // alpha_free or hg_coeff_free nonzero: J_local needed
+  double hg_coeff, cs0;
+};
+
+// F3: the element chain of one step
+template <typename T, int NN>
+__global__ __launch_bounds__(kBlock) void f3_element(
+    int E, int N, FusedParams fp, const int *__restrict__ elnod, const double *__restrict__ x,
+    const double *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
+    const T *__restrict__ rho, const T *__restrict__ sy0, T *__restrict__ tau_io,
+    T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
+    T *__restrict__ sigma_y, T *__restrict__ fout) {
+  int e = gtid();
+  if (e >= E) return;
+  const long Es = E;
+  int g[NN];
+#pragma unroll
+  for (int a = 0; a < NN; ++a) g[a] = elnod[(long)a * Es + e];
+
+  // ---- Jacobian, determinant, dH (calcElemJAndDerivatives) --------------
+  T X[NN][3];
+  if constexpr (std::is_same<T, double>::value) {
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) X[a][c] = x[(long)c * N + g[a]];
+  } else {
+    double x0[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { x0[c] = x[(long)c * N + g[0]]; X[0][c] = T(0); }
+#pragma unroll
+    for (int a = 1; a < NN; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) X[a][c] = T(x[(long)c * N + g[a]] - x0[c]);
+  }
+  T J[3][3];
+  if constexpr (NN == 8) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      J[0][c] = T(0.125) * (-X[0][c] + X[1][c] + X[2][c] - X[3][c] - X[4][c] + X[5][c] + X[6][c] - X[7][c]);
+      J[1][c] = T(0.125) * (-X[0][c] - X[1][c] + X[2][c] + X[3][c] - X[4][c] - X[5][c] + X[6][c] + X[7][c]);
+      J[2][c] = T(0.125) * (-X[0][c] - X[1][c] - X[2][c] - X[3][c] + X[4][c] + X[5][c] + X[6][c] + X[7][c]);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      J[0][c] = X[1][c] - X[0][c];
+      J[1][c] = X[2][c] - X[0][c];
+      J[2][c] = X[3][c] - X[0][c];
+    }
+  }
+  T C[3][3];
+  C[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
+  C[0][1] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]);
+  C[0][2] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  C[1][0] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]);
+  C[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+  C[1][2] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]);
+  C[2][0] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+  C[2][1] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]);
+  C[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+  T detJ = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[1][2] * J[2][1] - J[0][1] * J[1][0] * J[2][2] +
+           J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] - J[0][2] * J[1][1] * J[2][0];
+  T dH[3][NN];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    T a0 = C[0][d], a1 = C[1][d], a2 = C[2][d];
+    if constexpr (NN == 8) {
+      const T q = T(0.125);
+      dH[d][0] = q * (-a0 - a1 - a2);
+      dH[d][1] = q * (a0 - a1 - a2);
+      dH[d][2] = q * (a0 + a1 - a2);
+      dH[d][3] = q * (-a0 + a1 - a2);
+      dH[d][4] = q * (-a0 - a1 + a2);
+      dH[d][5] = q * (a0 - a1 + a2);
+      dH[d][6] = q * (a0 + a1 + a2);
+      dH[d][7] = q * (-a0 + a1 + a2);
+    } else {
+      dH[d][0] = -a0 - a1 - a2;
+      dH[d][1] = a0;
+      dH[d][2] = a1;
+      dH[d][3] = a2;
+    }
+  }
+  const T w = (NN == 4) ? T(1.0 / 6.0) : T(8.0);
+  T volv = T(0);
+  volv += detJ * w * T(1);
+
+  // ---- strain rate and spin (calcElemStrainRates) -----------------------
+  T V[NN][3];
+#pragma unroll
+  for (int a = 0; a < NN; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) V[a][c] = (T)v[(long)c * N + g[a]];
+  T s00 = 0, s11 = 0, s22 = 0, s01 = 0, s12 = 0, s02 = 0, w01 = 0, w12 = 0, w02 = 0;
+  T f = T(1) / detJ;
+#pragma unroll
+  for (int a = 0; a < NN; ++a) {
+    T vx = V[a][0], vy = V[a][1], vz = V[a][2];
+    T h0 = dH[0][a], h1 = dH[1][a], h2 = dH[2][a];
+    s00 = s00 + h0 * f * vx;
+    s11 = s11 + h1 * f * vy;
+    s22 = s22 + h2 * f * vz;
+    s01 = s01 + f * (h1 * vx + h0 * vy);
+    w01 = w01 + f * (h1 * vx - h0 * vy);
+    s12 = s12 + f * (h2 * vy + h1 * vz);
+    s02 = s02 + f * (h2 * vx + h0 * vz);
+    w12 = w12 + f * (h2 * vy - h1 * vz);
+    w02 = w02 + f * (h2 * vx - h0 * vz);
+  }
+  s01 = s01 * T(0.5); s12 = s12 * T(0.5); s02 = s02 * T(0.5);
+  w01 = w01 * T(0.5); w12 = w12 * T(0.5); w02 = w02 * T(0.5);
+
+  // ---- ANP pressure (calcElemPressure element part) ----------------------
+  const PressParams &pp = fp.pp;
+  double J_avg = 0.0;
+#pragma unroll
+  for (int a = 0; a < NN; ++a) J_avg += ratio[g[a]];
+  J_avg /= NN;
+  double J_local = 0.0;
+  if (fp.need_jlocal || pp.full) J_local = (double)volv / (double)vol0[e];
+  double al = pp.alpha_free;
+  double J_bar = al * J_local + (1 - al) * J_avg;
+  if (J_bar < pp.J_min) J_bar = 0.2;
+  double lf = pp.log_factor;
+  double lg = (lf != 0.0) ? log(J_bar) : 0.0;
+  double p_physical = -pp.K * (lf * lg + (1.0 - lf) * (J_bar - 1.0));
+  double p_pspg = 0.0, p_q = 0.0;
+  double p_hg = pp.hg_coeff_free * pp.K * fabs(J_local - J_avg);
+  if (pp.full) {
+    double rho_e = (double)rho[e];
+    double h = pow((double)volv, 1.0 / 3.0);
+    double c = sqrt(pp.K / rho_e);
+    double tau = h / (2.0 * c);
+    double div_v = 0.0;
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+      div_v += (double)dH[0][a] * v[g[a]] + (double)dH[1][a] * v[(long)N + g[a]] +
+               (double)dH[2][a] * v[2L * N + g[a]];
+    if (div_v < 0.0) {
+      double x1 = pp.pspg_scale * tau * div_v * pp.K, x2 = pp.p_pspg_bulkfac * pp.K;
+      p_pspg = (x2 < x1) ? x2 : x1;
+      double q1 = pp.av_coeff_div * rho_e * h * c * (-div_v);
+      double q2 = pp.av_coeff_bulk * pp.K * (1.0 - J_local);
+      p_q = (q1 < q2) ? q2 : q1;
+    }
+  }
+  T pe = (T)(p_physical + p_pspg + p_hg + p_q);
+  p_out[e] = pe;
+
+  // ---- Jaumann update and J2 return (CalcStressStrain) -------------------
+  T dt = (T)fp.dt;
+  const MatParams &mp = fp.mp;
+  T3<T> tau = t3_from_sym(tau_io[0 * Es + e], tau_io[1 * Es + e], tau_io[2 * Es + e], tau_io[3 * Es + e],
+                          tau_io[4 * Es + e], tau_io[5 * Es + e]);
+  T3<T> D = t3_from_sym(s00, s11, s22, s01, s12, s02);
+  T3<T> W = t3_from_antisym(w01, w12, w02);
+  T3<T> I = t3_identity<T>();
+  T G = (T)mp.G;
+  T3<T> Wt = t3_trans(W);
+  T3<T> SRT = t3_mul(tau, Wt);
+  T3<T> RS = t3_mul(W, tau);
+  T3<T> cI = t3_scale(I, T(1.0 / 3.0) * t3_trace(D));
+  T3<T> devD = t3_sub(D, cI);
+  T3<T> A = t3_scale(devD, T(2) * G);
+  T3<T> B = t3_add(A, SRT);
+  T3<T> Cc = t3_add(B, RS);
+  tau = t3_add(tau, t3_scale(Cc, dt));
+  T3<T> pI = t3_scale(I, -pe);
+  T3<T> st = t3_add(pI, tau);
+  T3<T> s = t3_sub(st, t3_scale(I, T(1.0 / 3.0) * t3_trace(st)));
+  T J2 = T(0.5) * (s.xx * s.xx + T(2) * s.xy * s.xy + T(2) * s.xz * s.xz + s.yy * s.yy +
+                   T(2) * s.yz * s.yz + s.zz * s.zz);
+  T sig_trial = sqrt(T(3) * J2);
+  T ep = pl_strain[e];
+  T sy = yield_stress<T>(mp, sy0[e], ep);
+  sigma_y[e] = sy;
+  if (sy < sig_trial) {
+    T H = tangent_modulus<T>(mp, ep);
+    T dgamma = (sig_trial - sy) / (T(3) * G + H);
+    T factor = T(1) - (T(3) * G * dgamma) / sig_trial;
+    tau = t3_scale(s, factor);
+    pl_strain[e] = ep + dgamma;
+  }
+  T3<T> sig = t3_add(pI, tau);
+  eps_io[0 * Es + e] = eps_io[0 * Es + e] + D.xx * dt;
+  eps_io[1 * Es + e] = eps_io[1 * Es + e] + D.yy * dt;
+  eps_io[2 * Es + e] = eps_io[2 * Es + e] + D.zz * dt;
+  eps_io[3 * Es + e] = eps_io[3 * Es + e] + D.xy * dt;
+  eps_io[4 * Es + e] = eps_io[4 * Es + e] + D.yz * dt;
+  eps_io[5 * Es + e] = eps_io[5 * Es + e] + D.xz * dt;
+  tau_io[0 * Es + e] = tau.xx; tau_io[1 * Es + e] = tau.yy; tau_io[2 * Es + e] = tau.zz;
+  tau_io[3 * Es + e] = tau.xy; tau_io[4 * Es + e] = tau.yz; tau_io[5 * Es + e] = tau.xz;
+
+  // ---- internal forces (calcElemForces) and hourglass ---------------------
+  T hm[3][4];
+  T c_h = T(0);
+  if constexpr (NN == 8) {
+    if (fp.hg_coeff != 0.0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) hm[c][j] = T(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) hm[c][j] = hm[c][j] + V[a][c] * (T)kHgGamma[j][a];
+      c_h = (T)(fp.hg_coeff * pow((double)volv, 0.6666666) * (double)rho[e] * 0.25 * fp.cs0);
+    }
+  }
+  T sg0 = sig.xx, sg1 = sig.yy, sg2 = sig.zz, sg3 = sig.xy, sg4 = sig.yz, sg5 = sig.xz;
+#pragma unroll
+  for (int a = 0; a < NN; ++a) {
+    T h0 = dH[0][a], h1 = dH[1][a], h2 = dH[2][a];
+    T f0 = T(0), f1 = T(0), f2 = T(0);
+    f0 += h0 * sg0 * T(1);
+    f1 += h1 * sg1 * T(1);
+    f2 += h2 * sg2 * T(1);
+    f0 += h1 * sg3 + h2 * sg5;
+    f1 += h0 * sg3 + h2 * sg4;
+    f2 += h1 * sg4 + h0 * sg5;
+    f0 = f0 * w;
+    f1 = f1 * w;
+    f2 = f2 * w;
+    if constexpr (NN == 8) {
+      if (fp.hg_coeff != 0.0) {
+        T q[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          T hf = T(0);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) hf = hf - hm[c][j] * (T)kHgGamma[j][a];
+          q[c] = hf * c_h;
+        }
+        f0 = f0 - q[0];
+        f1 = f1 - q[1];
+        f2 = f2 - q[2];
+      }
+    }
+    fout[(long)(a * 3 + 0) * Es + e] = f0;
+    fout[(long)(a * 3 + 1) * Es + e] = f1;
+    fout[(long)(a * 3 + 2) * Es + e] = f2;
+  }
+}
+
+struct NodeParams {
+  double dt, alpha, beta, gamma;
+  double f;        // 1/(1-alpha)
+  double gdt;      // gamma*dt
+  double bdt2;     // beta*dt*dt
+  double pred_u;   // (0.5-beta)*dt   (next prediction)
+  double pred_v;   // (1-gamma)*dt
+  int predict;     // run the next step's prediction
+  int has_fe;      // external forces present
+};
+
+// F4: assembly and the whole nodal update of one step
+template <typename T>
+__global__ __launch_bounds__(kBlock) void f4_node(
+    int N, int E, NodeParams np, const int *__restrict__ off, const int *__restrict__ cnt,
+    const int *__restrict__ nodel_pk, const T *__restrict__ fel, const double *__restrict__ fe,
+    const double *__restrict__ mdiag, const int *__restrict__ bc_slot,
+    const int *__restrict__ bc_mask, const double *__restrict__ bc_val, double *__restrict__ fi,
+    double *__restrict__ v, double *__restrict__ prev_a, double *__restrict__ u_dt,
+    double *__restrict__ x, double *__restrict__ u, unsigned int *__restrict__ status) {
+  int i = gtid();
+  if (i >= N) return;
+  const long Es = E, Ns = N;
+  int o = off[i], c = cnt[i];
+  double f[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < c; ++k) {
+    int pk = nodel_pk[o + k];
+    long e = pk >> 3, b = (long)(pk & 7) * 3;
+    f[0] += (double)fel[(b + 0) * Es + e];
+    f[1] += (double)fel[(b + 1) * Es + e];
+    f[2] += (double)fel[(b + 2) * Es + e];
+  }
+  unsigned bad = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (!isfinite(f[d])) { f[d] = 0.0; ++bad; }
+  if (bad) atomicAdd(status, bad);
+  int slot = bc_slot[i];
+  int mask = slot >= 0 ? bc_mask[slot] : 0;
+  double m = mdiag[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const long id = d * Ns + i;
+    fi[id] = f[d];
+    double fed = np.has_fe ? fe[id] : 0.0;
+    double a = (fed - f[d]) / m;                       // calcAccel
+    if (mask & (1 << d)) a = 0.0;                       // ImposeBCA
+    double pa = prev_a[id];
+    a = np.f * (a - np.alpha * pa);                     // UpdateCorrectionAccVel
+    double vv = v[id] + np.gdt * a;
+    if (mask & (1 << d)) vv = bc_val[3L * slot + d];    // ImposeBCV
+    double ud = u_dt[id] + np.bdt2 * a;                 // UpdateCorrectionPos
+    x[id] = x[id] + ud;
+    u[id] = u[id] + ud;
+    if (np.predict) {                                   // UpdatePrediction
+      ud = np.dt * (vv + np.pred_u * a);
+      vv = vv + np.pred_v * a;
+      if (mask & (1 << d)) vv = bc_val[3L * slot + d];  // ImposeBCV
+    }
+    u_dt[id] = ud;
+    prev_a[id] = a;
+    v[id] = vv;
+  }
+}
+
+// sigma = -p I + tau (CalcStressStrain :1766), for the field after a fused run
+template <typename T>
+__global__ __launch_bounds__(kBlock) void f_sigma(int E, const T *__restrict__ p,
+                                                  const T *__restrict__ tau, T *__restrict__ sigma) {
+  int e = gtid();
+  if (e >= E) return;
+  const long Es = E;
+  T pe = p[e];
+  T3<T> I = t3_identity<T>();
+  T3<T> pI = t3_scale(I, -pe);
+  T3<T> t = t3_from_sym(tau[0 * Es + e], tau[1 * Es + e], tau[2 * Es + e], tau[3 * Es + e], tau[4 * Es + e],
+                        tau[5 * Es + e]);
+  T3<T> s = t3_add(pI, t);
+  sigma[0 * Es + e] = s.xx; sigma[1 * Es + e] = s.yy; sigma[2 * Es + e] = s.zz;
+  sigma[3 * Es + e] = s.xy; sigma[4 * Es + e] = s.yz; sigma[5 * Es + e] = s.xz;
+}
+
+}  // namespace
+
+hipError_t launch_reaction(wfx_ctx *c, int dim, double *dev_out, hipStream_t s);
+
+template <typename T>
+hipError_t launch_vsum0(wfx_ctx *c, hipStream_t s) {
+  hipLaunchKernelGGL((f0_vsum0<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N, c->ms.nodel_offset,
+                     c->ms.nodel_count, c->ms.nodel_pk, elem<T>(c).vol0, c->nd.voln0_sum);
+  return hipGetLastError();
+}
+
+// Launch one phase (1..4) of a fused step on the current state.
+template <typename T>
+hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s) {
+  auto &el = elem<T>(c);
+  const auto &P = c->prm;
+  const auto &st = P.stab;
+  const int N = c->N, E = c->E;
+  const int gE = nblocks(E), gN = nblocks(N);
+  if (phase == 1) {
+    if (c->n == 8)
+      hipLaunchKernelGGL((f1_volume<T, 8>), dim3(gE), dim3(kBlock), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol);
+    else
+      hipLaunchKernelGGL((f1_volume<T, 4>), dim3(gE), dim3(kBlock), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol);
+  } else if (phase == 2) {
+    hipLaunchKernelGGL((f2_nodal<T>), dim3(gN), dim3(kBlock), 0, s, N, c->n, c->ms.nodel_offset,
+                       c->ms.nodel_count, c->ms.nodel_pk, el.vol, el.rho, c->nd.voln0_sum, c->nd.anp_ratio,
+                       c->nd.voln, c->nd.mdiag);
+  } else if (phase == 3) {
+    FusedParams fp;
+    fp.dt = P.dt;
+    fp.mp = MatParams{P.mat.model, P.mat.G, P.mat.H, P.mat.sy0, P.mat.Kh, P.mat.mh, P.mat.eps0, P.mat.eps1};
+    fp.pp = PressParams{P.mat.K,        st.alpha_free, st.log_factor,     st.J_min,        st.hg_coeff_free,
+                        st.pspg_scale,  st.p_pspg_bulkfac, st.av_coeff_div, st.av_coeff_bulk, 0};
+    fp.pp.full = (st.pspg_scale != 0.0 || st.p_pspg_bulkfac != 0.0 || st.av_coeff_div != 0.0 ||
+                  st.av_coeff_bulk != 0.0);
+    fp.need_jlocal = (st.alpha_free != 0.0 || st.hg_coeff_free != 0.0);
+    fp.hg_coeff = (c->n == 8) ? st.hg_coeff : 0.0;
+    fp.cs0 = P.mat.cs0;
+    if (c->n == 8)
+      hipLaunchKernelGGL((f3_element<T, 8>), dim3(gE), dim3(kBlock), 0, s, E, N, fp, c->ms.elnod, c->nd.x,
+                         c->nd.v, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.tau, el.eps, el.p,
+                         el.pl_strain, el.sigma_y, el.f_elem);
+    else
+      hipLaunchKernelGGL((f3_element<T, 4>), dim3(gE), dim3(kBlock), 0, s, E, N, fp, c->ms.elnod, c->nd.x,
+                         c->nd.v, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.tau, el.eps, el.p,
+                         el.pl_strain, el.sigma_y, el.f_elem);
+  } else if (phase == 4) {
+    NodeParams np;
+    np.dt = P.dt;
+    np.alpha = P.alpha;
+    np.beta = P.beta;
+    np.gamma = P.gamma;
+    np.f = 1.0 / (1.0 - P.alpha);
+    np.gdt = P.gamma * P.dt;
+    np.bdt2 = P.beta * P.dt * P.dt;
+    np.pred_u = (0.5 - P.beta) * P.dt;
+    np.pred_v = (1.0 - P.gamma) * P.dt;
+    np.predict = predict;
+    np.has_fe = c->has_fe;
+    hipLaunchKernelGGL((f4_node<T>), dim3(gN), dim3(kBlock), 0, s, N, E, np, c->ms.nodel_offset,
+                       c->ms.nodel_count, c->ms.nodel_pk, el.f_elem, c->nd.fe, c->nd.mdiag, c->bc_slot,
+                       c->bc_mask, c->bc_tab, c->nd.fi, c->nd.v, c->nd.prev_a, c->nd.u_dt, c->nd.x, c->nd.u,
+                       c->status);
+  } else if (phase == 5) {
+    hipLaunchKernelGGL((f_sigma<T>), dim3(gE), dim3(kBlock), 0, s, E, el.p, el.tau, el.sigma);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t s) {
+  for (int k = 0; k < nsteps; ++k) {
+    for (int ph = 1; ph <= 4; ++ph) {
+      hipError_t e = launch_fused_phase<T>(c, ph, k + 1 < nsteps, s);
+      if (e != hipSuccess) return e;
+    }
+    if (hist) {
+      hipError_t e = launch_reaction(c, 2, hist + k, s);
+      if (e != hipSuccess) return e;
+    }
+  }
+  return launch_fused_phase<T>(c, 5, 0, s);
+}
+
+}  // namespace wfx

@@ -63,11 +63,13 @@ struct DevNode {
   double *x = nullptr, *v = nullptr, *a = nullptr, *u = nullptr, *u_dt = nullptr,
          *prev_a = nullptr, *fi = nullptr, *fe = nullptr;
   double *mdiag = nullptr, *voln = nullptr, *voln_sum = nullptr, *voln0_sum = nullptr;
+  double *anp_ratio = nullptr;  // fused path: V_n / V0_n
 };
 
 struct DevMesh {
   int *elnod = nullptr;
   int *nodel = nullptr, *nodel_loc = nullptr, *nodel_offset = nullptr, *nodel_count = nullptr;
+  int *nodel_pk = nullptr;  // (element << 3) | local node, CSR order
 };
 
 // Kernel launch parameters shared by the precision variants.
@@ -103,6 +105,12 @@ struct wfx_ctx {
   double *bc_val[3] = {nullptr, nullptr, nullptr};
   int bc_count[3] = {0, 0, 0};
   int *react = nullptr;
+  // per-node Dirichlet data for the fused node pass: slot or -1, bit mask of
+  // constrained directions and the three prescribed values per slot
+  int *bc_slot = nullptr, *bc_mask = nullptr;
+  double *bc_tab = nullptr;
+  int has_fe = 0;            // external nodal forces set (non-zero)
+  bool fused_stale = false;  // per-step intermediates not maintained by the fused path
   double *scalars = nullptr;      // device scratch for reductions (reaction force)
   unsigned int *status = nullptr; // [0] non-finite internal forces scrubbed
   double *hist = nullptr;         // reaction-force history of wfx_run

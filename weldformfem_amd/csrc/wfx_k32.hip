@@ -1,7 +1,11 @@
 // FP32-mixed instantiation of the per-step element kernels.
 #include "wfx_kernels.hpp"
+#include "wfx_fused.hpp"
 #include "wfx_launch.hpp"
 
 namespace wfx {
 WFX_DEFINE_LAUNCHERS(float)
+template hipError_t launch_vsum0<float>(wfx_ctx *, hipStream_t);
+template hipError_t launch_fused_phase<float>(wfx_ctx *, int, int, hipStream_t);
+template hipError_t launch_fused_steps<float>(wfx_ctx *, int, double *, hipStream_t);
 }  // namespace wfx

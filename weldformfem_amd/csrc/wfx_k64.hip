@@ -1,11 +1,15 @@
 // FP64 instantiation of the per-step kernels (built with -ffp-contract=off)
 // and the precision-independent nodal launchers.
 #include "wfx_kernels.hpp"
+#include "wfx_fused.hpp"
 #include "wfx_launch.hpp"
 
 namespace wfx {
 
 WFX_DEFINE_LAUNCHERS(double)
+template hipError_t launch_vsum0<double>(wfx_ctx *, hipStream_t);
+template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipStream_t);
+template hipError_t launch_fused_steps<double>(wfx_ctx *, int, double *, hipStream_t);
 
 hipError_t launch_update_prediction(wfx_ctx *c, hipStream_t s) {
   long n3 = 3L * c->N;

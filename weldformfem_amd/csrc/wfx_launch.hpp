@@ -38,6 +38,16 @@ template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, d
 WFX_DECLARE_SPECIALISATIONS(double)
 WFX_DECLARE_SPECIALISATIONS(float)
 
+template <typename T> hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t s);
+template <typename T> hipError_t launch_vsum0(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s);
+extern template hipError_t launch_fused_steps<double>(wfx_ctx *, int, double *, hipStream_t);
+extern template hipError_t launch_fused_steps<float>(wfx_ctx *, int, double *, hipStream_t);
+extern template hipError_t launch_vsum0<double>(wfx_ctx *, hipStream_t);
+extern template hipError_t launch_vsum0<float>(wfx_ctx *, hipStream_t);
+extern template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipStream_t);
+extern template hipError_t launch_fused_phase<float>(wfx_ctx *, int, int, hipStream_t);
+
 hipError_t launch_update_prediction(wfx_ctx *c, hipStream_t s);
 hipError_t launch_impose(wfx_ctx *c, int dim, bool accel, hipStream_t s);
 hipError_t launch_calc_accel(wfx_ctx *c, hipStream_t s);

@@ -145,6 +145,17 @@ class Domain:
         check(self._L.wfx_run(self._h, int(nsteps), hist.ctypes.data if reaction else None, self.stream))
         return hist[:nsteps]
 
+    def runFused(self, nsteps, reaction=True):
+        """nsteps steps through the four-kernel fused path (wfx_step_fused)"""
+        hist = np.zeros(max(int(nsteps), 1), dtype=np.float64)
+        check(self._L.wfx_step_fused(self._h, int(nsteps), hist.ctypes.data if reaction else None,
+                                     self.stream))
+        return hist[:nsteps]
+
+    def fusedPhase(self, phase, predict=True):
+        """launch one kernel of the fused step (timing and profiling)"""
+        check(self._L.wfx_fused_phase(self._h, int(phase), int(predict), self.stream))
+
     def reactionForce(self, dim=2):
         v = ctypes.c_double()
         check(self._L.wfx_reaction_force(self._h, int(dim), ctypes.byref(v), self.stream))
