@@ -172,6 +172,33 @@ def cpu_baseline(seconds_target=12.0):
     }
 
 
+def pmc_probe(args):
+    """Kernels for hardware-counter runs (rocprofv3 --pmc): a plain 1 GiB
+    device-to-device copy whose FETCH_SIZE/WRITE_SIZE calibrate the counters,
+    then a few steps of each path on the benchmark mesh."""
+    import torch
+
+    from weldformfem_amd import WFX_FP32_MIXED, WFX_FP64
+
+    torch.cuda.init()
+    stream = torch.cuda.current_stream().cuda_stream
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    nx, ny, nz, L = (200, 200, 100, [2.0, 2.0, 1.0]) if not args.quick else (100, 100, 50, [2.0, 2.0, 1.0])
+    for prec in (WFX_FP64, WFX_FP32_MIXED):
+        d, _ = build_domain(prec, nx, ny, nz, L, -2.0, stream)
+        d.runFused(3, reaction=False)
+        if prec == WFX_FP64:
+            d.run(1, reaction=False)
+        torch.cuda.synchronize()
+        del d
+    print("pmc probe done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,7 +210,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-api", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="short run for counter collection: a 1 GiB device copy (calibration), "
+                         "then 3 fused steps FP64 and FP32-mixed and one per-step API step")
     args = ap.parse_args()
+    if args.pmc_probe:
+        return pmc_probe(args)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(sharding across GPUs is a design only)", file=sys.stderr)

@@ -37,6 +37,16 @@
 namespace wfx {
 namespace {
 
+// CSR entries fetched per batch of independent loads in the node passes
+constexpr int kChunk = 8;
+// element pass F3: per-thread LDS slots for values parked across the stress
+// update (12 hourglass amplitudes + 9 cofactors)
+#ifndef WFX_F3_PARK
+#define WFX_F3_PARK 0
+#endif
+constexpr bool kParkInLds = WFX_F3_PARK;
+constexpr int kParkSlots = 21;
+
 // V0_n: CSR-ordered sum of the initial volumes (calcElemPressure :698-705)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void f0_vsum0(int N, const int *__restrict__ off,
@@ -118,14 +128,36 @@ __global__ __launch_bounds__(kBlock) void f2_nodal(int N, int nodxelem, const in
   int i = gtid();
   if (i >= N) return;
   int o = off[i], c = cnt[i];
+  // CSR entries are taken eight at a time with independent loads, then
+  // summed in CSR order (the order of the reference's serial loop).
   double s = 0.0;
-  for (int k = 0; k < c; ++k) s += (double)vol[nodel_pk[o + k] >> 3];
+  for (int k = 0; k < c; k += kChunk) {
+    int e[kChunk];
+    double vv[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) e[j] = (k + j < c) ? (nodel_pk[o + k + j] >> 3) : -1;
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) vv[j] = (e[j] >= 0) ? (double)vol[e[j]] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j)
+      if (k + j < c) s += vv[j];
+  }
   ratio[i] = s / vsum0[i];
   double vn = s;
   vn /= nodxelem;
   voln[i] = vn;
   double dc = (double)c, mass = 0.0;
-  for (int k = 0; k < c; ++k) mass += 1.0 * (double)rho[nodel_pk[o + k] >> 3] * vn / dc;
+  for (int k = 0; k < c; k += kChunk) {
+    int e[kChunk];
+    double rr[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) e[j] = (k + j < c) ? (nodel_pk[o + k + j] >> 3) : -1;
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) rr[j] = (e[j] >= 0) ? (double)rho[e[j]] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j)
+      if (k + j < c) mass += 1.0 * rr[j] * vn / dc;
+  }
   mdiag[i] = mass;
 }
 
@@ -136,6 +168,34 @@ struct FusedParams {
   int need_jlocal;    // alpha_free or hg_coeff_free nonzero: J_local needed
   double hg_coeff, cs0;
 };
+
+// dH(d,a) = adj(J)(d,:) . dN_a/dxi from the cofactor matrix C (adj = C^T),
+// written exactly as calcElemJAndDerivatives does (Domain_d.C:1843-1850,
+// :1866-1887).  a is a compile-time constant after unrolling.
+template <typename T, int NN>
+__device__ __forceinline__ T shape_dH(const T (&C)[3][3], int d, int a) {
+  const T a0 = C[0][d], a1 = C[1][d], a2 = C[2][d];
+  if constexpr (NN == 8) {
+    const T q = T(0.125);
+    switch (a) {
+      case 0: return q * (-a0 - a1 - a2);
+      case 1: return q * (a0 - a1 - a2);
+      case 2: return q * (a0 + a1 - a2);
+      case 3: return q * (-a0 + a1 - a2);
+      case 4: return q * (-a0 - a1 + a2);
+      case 5: return q * (a0 - a1 + a2);
+      case 6: return q * (a0 + a1 + a2);
+      default: return q * (-a0 + a1 + a2);
+    }
+  } else {
+    switch (a) {
+      case 0: return -a0 - a1 - a2;
+      case 1: return a0;
+      case 2: return a1;
+      default: return a2;
+    }
+  }
+}
 
 // F3: the element chain of one step
 template <typename T, int NN>
@@ -196,27 +256,8 @@ __global__ __launch_bounds__(kBlock) void f3_element(
   C[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
   T detJ = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[1][2] * J[2][1] - J[0][1] * J[1][0] * J[2][2] +
            J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] - J[0][2] * J[1][1] * J[2][0];
-  T dH[3][NN];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    T a0 = C[0][d], a1 = C[1][d], a2 = C[2][d];
-    if constexpr (NN == 8) {
-      const T q = T(0.125);
-      dH[d][0] = q * (-a0 - a1 - a2);
-      dH[d][1] = q * (a0 - a1 - a2);
-      dH[d][2] = q * (a0 + a1 - a2);
-      dH[d][3] = q * (-a0 + a1 - a2);
-      dH[d][4] = q * (-a0 - a1 + a2);
-      dH[d][5] = q * (a0 - a1 + a2);
-      dH[d][6] = q * (a0 + a1 + a2);
-      dH[d][7] = q * (-a0 + a1 + a2);
-    } else {
-      dH[d][0] = -a0 - a1 - a2;
-      dH[d][1] = a0;
-      dH[d][2] = a1;
-      dH[d][3] = a2;
-    }
-  }
+  // dH(d,a) is re-evaluated from the 3x3 cofactor matrix where it is used
+  // (same expression each time), so 9 values stay live instead of 3n.
   const T w = (NN == 4) ? T(1.0 / 6.0) : T(8.0);
   T volv = T(0);
   volv += detJ * w * T(1);
@@ -232,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void f3_element(
 #pragma unroll
   for (int a = 0; a < NN; ++a) {
     T vx = V[a][0], vy = V[a][1], vz = V[a][2];
-    T h0 = dH[0][a], h1 = dH[1][a], h2 = dH[2][a];
+    T h0 = shape_dH<T, NN>(C, 0, a), h1 = shape_dH<T, NN>(C, 1, a), h2 = shape_dH<T, NN>(C, 2, a);
     s00 = s00 + h0 * f * vx;
     s11 = s11 + h1 * f * vy;
     s22 = s22 + h2 * f * vz;
@@ -245,6 +286,42 @@ __global__ __launch_bounds__(kBlock) void f3_element(
   }
   s01 = s01 * T(0.5); s12 = s12 * T(0.5); s02 = s02 * T(0.5);
   w01 = w01 * T(0.5); w12 = w12 * T(0.5); w02 = w02 * T(0.5);
+
+  // hourglass mode amplitudes now, so the nodal velocities die here and do
+  // not stay live through the stress update (register pressure)
+  T hm[3][4];
+  T c_h = T(0);
+  if constexpr (NN == 8) {
+    if (fp.hg_coeff != 0.0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) hm[c][j] = T(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) hm[c][j] = hm[c][j] + V[a][c] * (T)kHgGamma[j][a];
+      c_h = (T)(fp.hg_coeff * pow((double)volv, 0.6666666) * (double)rho[e] * 0.25 * fp.cs0);
+    }
+  }
+  // Park the hourglass amplitudes and the cofactor matrix in LDS (one slot
+  // per thread, conflict-free) while the stress update runs: they are only
+  // needed again for the forces, and keeping them in VGPRs halves occupancy.
+  __shared__ T park[kParkSlots][kBlock];
+  const int tx = threadIdx.x;
+  if constexpr (kParkInLds) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) park[c * 4 + j][tx] = (NN == 8) ? hm[c][j] : T(0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) park[12 + i * 3 + j][tx] = C[i][j];
+    asm volatile("" ::: "memory");
+  }
 
   // ---- ANP pressure (calcElemPressure element part) ----------------------
   const PressParams &pp = fp.pp;
@@ -270,8 +347,8 @@ __global__ __launch_bounds__(kBlock) void f3_element(
     double div_v = 0.0;
 #pragma unroll
     for (int a = 0; a < NN; ++a)
-      div_v += (double)dH[0][a] * v[g[a]] + (double)dH[1][a] * v[(long)N + g[a]] +
-               (double)dH[2][a] * v[2L * N + g[a]];
+      div_v += (double)shape_dH<T, NN>(C, 0, a) * v[g[a]] + (double)shape_dH<T, NN>(C, 1, a) * v[(long)N + g[a]] +
+               (double)shape_dH<T, NN>(C, 2, a) * v[2L * N + g[a]];
     if (div_v < 0.0) {
       double x1 = pp.pspg_scale * tau * div_v * pp.K, x2 = pp.p_pspg_bulkfac * pp.K;
       p_pspg = (x2 < x1) ? x2 : x1;
@@ -328,27 +405,21 @@ __global__ __launch_bounds__(kBlock) void f3_element(
   tau_io[3 * Es + e] = tau.xy; tau_io[4 * Es + e] = tau.yz; tau_io[5 * Es + e] = tau.xz;
 
   // ---- internal forces (calcElemForces) and hourglass ---------------------
-  T hm[3][4];
-  T c_h = T(0);
-  if constexpr (NN == 8) {
-    if (fp.hg_coeff != 0.0) {
+  if constexpr (kParkInLds) {
+    asm volatile("" ::: "memory");
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) hm[c][j] = T(0);
+      for (int j = 0; j < 4; ++j) hm[c][j] = park[c * 4 + j][tx];
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+    for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int a = 0; a < 8; ++a)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) hm[c][j] = hm[c][j] + V[a][c] * (T)kHgGamma[j][a];
-      c_h = (T)(fp.hg_coeff * pow((double)volv, 0.6666666) * (double)rho[e] * 0.25 * fp.cs0);
-    }
+      for (int j = 0; j < 3; ++j) C[i][j] = park[12 + i * 3 + j][tx];
   }
   T sg0 = sig.xx, sg1 = sig.yy, sg2 = sig.zz, sg3 = sig.xy, sg4 = sig.yz, sg5 = sig.xz;
 #pragma unroll
   for (int a = 0; a < NN; ++a) {
-    T h0 = dH[0][a], h1 = dH[1][a], h2 = dH[2][a];
+    T h0 = shape_dH<T, NN>(C, 0, a), h1 = shape_dH<T, NN>(C, 1, a), h2 = shape_dH<T, NN>(C, 2, a);
     T f0 = T(0), f1 = T(0), f2 = T(0);
     f0 += h0 * sg0 * T(1);
     f1 += h1 * sg1 * T(1);
@@ -405,12 +476,24 @@ __global__ __launch_bounds__(kBlock) void f4_node(
   const long Es = E, Ns = N;
   int o = off[i], c = cnt[i];
   double f[3] = {0.0, 0.0, 0.0};
-  for (int k = 0; k < c; ++k) {
-    int pk = nodel_pk[o + k];
-    long e = pk >> 3, b = (long)(pk & 7) * 3;
-    f[0] += (double)fel[(b + 0) * Es + e];
-    f[1] += (double)fel[(b + 1) * Es + e];
-    f[2] += (double)fel[(b + 2) * Es + e];
+  for (int k = 0; k < c; k += kChunk) {
+    int pk[kChunk];
+    double q[kChunk][3];
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) pk[j] = (k + j < c) ? nodel_pk[o + k + j] : -1;
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      long e = pk[j] >> 3, b = (long)(pk[j] & 7) * 3;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) q[j][d] = (pk[j] >= 0) ? (double)fel[(b + d) * Es + e] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j)
+      if (k + j < c) {
+        f[0] += q[j][0];
+        f[1] += q[j][1];
+        f[2] += q[j][2];
+      }
   }
   unsigned bad = 0;
 #pragma unroll
