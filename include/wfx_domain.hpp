@@ -1,0 +1,155 @@
+// wfx_domain.hpp — header-only C++ mirror of MetFEM::Domain_d over the wfx
+// C-ABI (include/wfx.h).  Method names, argument meaning and call order are
+// the reference's (include/common/Domain_d.h:238-830), so the body of
+// Domain_d::SolveChungHulbert (src/explicit/Solver_explicit.C:346-1170) can
+// call these methods unchanged.  Every call runs HIP kernels on the stream
+// given at construction; a failing call throws wfx::Error with the library's
+// message.
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "wfx.h"
+
+namespace wfx {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc) {
+  if (rc != WFX_OK) throw Error(rc, std::string("wfx: ") + wfx_last_error());
+}
+
+struct double3 {
+  double x, y, z;
+};
+inline double3 make_double3(double x, double y, double z) { return {x, y, z}; }
+
+class Domain_d {
+ public:
+  explicit Domain_d(int precision = WFX_FP64, wfx_stream stream = nullptr) : s_(stream) {
+    check(wfx_create(&c_, precision));
+  }
+  ~Domain_d() { wfx_destroy(c_); }
+  Domain_d(const Domain_d &) = delete;
+  Domain_d &operator=(const Domain_d &) = delete;
+
+  wfx_ctx *handle() { return c_; }
+
+  // ---- mesh (Domain_d.C:1136-1504, :1508-1699) ----
+  void AddBoxLength(const double3 &V, const double3 &L, const double &r, const bool &red_int = true,
+                    const bool &tritetra = false) {
+    const double v[3] = {V.x, V.y, V.z}, l[3] = {L.x, L.y, L.z};
+    check(wfx_add_box_length(c_, v, l, r, red_int, tritetra));
+  }
+  void CreateFromMesh(const std::vector<double> &x, const std::vector<int> &elnod, int nodxelem) {
+    check(wfx_set_mesh(c_, (int)(x.size() / 3), (int)(elnod.size() / nodxelem), nodxelem, x.data(),
+                       elnod.data()));
+  }
+  int getNodeCount() { int n, e, k; check(wfx_get_sizes(c_, &n, &e, &k)); return n; }
+  int getElemCount() { int n, e, k; check(wfx_get_sizes(c_, &n, &e, &k)); return e; }
+
+  // ---- set-up ----
+  void setDensity(const double &r) { check(wfx_set_density(c_, r)); }
+  // Material_ + AssignMaterial: model WFX_BILINEAR (p1 = Et) or WFX_HOLLOMON (p1 = K, p2 = m)
+  void AssignMaterial(int model, double E, double nu, double rho0, double sy0, double p1 = 0.0,
+                      double p2 = 0.0) {
+    check(wfx_set_material(c_, model, E, nu, rho0, sy0, p1, p2));
+  }
+  void AddBCVelNode(const int &node, const int &dim, const double &val) {
+    check(wfx_add_bc_vel_node(c_, node, dim, val));
+  }
+  void AllocateBCs() { check(wfx_allocate_bcs(c_)); }
+  void SetDT(const double &dt) { dt_ = dt; check(wfx_set_dt(c_, dt)); }
+  void SetEndTime(const double &t) { end_t_ = t; }
+  void setParam(const char *name, double v) { check(wfx_set_param(c_, name, v)); }
+  void setReactionNodes(const std::vector<int> &nodes) {
+    check(wfx_set_reaction_nodes(c_, nodes.data(), (int)nodes.size()));
+  }
+  double calcMinEdgeLength() { double v; check(wfx_calc_min_edge_length(c_, &v, s_)); return v; }
+  double getDT() const { return dt_; }
+
+  // ---- init block and per-step API (Solver_explicit.C order) ----
+  void InitSolve() { check(wfx_init(c_, s_)); }
+  void UpdatePrediction() { check(wfx_update_prediction(c_, s_)); }
+  void ImposeBCV(const int dim) { check(wfx_impose_bcv(c_, dim, s_)); }
+  void ImposeBCA(const int dim) { check(wfx_impose_bca(c_, dim, s_)); }
+  void ImposeBCVAllDim() { for (int d = 0; d < 3; ++d) ImposeBCV(d); }
+  void ImposeBCAAllDim() { for (int d = 0; d < 3; ++d) ImposeBCA(d); }
+  void calcElemJAndDerivatives() { check(wfx_calc_elem_jacobians(c_, s_)); }
+  void CalcElemVol() { check(wfx_calc_elem_vol(c_, s_)); }
+  void CalcNodalVol() { check(wfx_calc_nodal_vol(c_, s_)); }
+  void CalcNodalMassFromVol() { check(wfx_calc_nodal_mass_from_vol(c_, s_)); }
+  void calcElemStrainRates() { check(wfx_calc_elem_strain_rates(c_, s_)); }
+  void calcElemPressure() { check(wfx_calc_elem_pressure(c_, s_)); }
+  void CalcStressStrain(double dt) { check(wfx_calc_stress_strain(c_, dt, s_)); }
+  void calcArtificialViscosity() { check(wfx_calc_artificial_viscosity(c_, s_)); }
+  void calcElemForces() { check(wfx_calc_elem_forces(c_, s_)); }
+  void calcElemHourglassForces() { check(wfx_calc_elem_hourglass_forces(c_, s_)); }
+  void assemblyForces() { check(wfx_assembly_forces(c_, s_)); }
+  void calcAccel() { check(wfx_calc_accel(c_, s_)); }
+  void UpdateCorrectionAccVel() { check(wfx_update_correction_accvel(c_, s_)); }
+  void UpdateCorrectionPos() { check(wfx_update_correction_pos(c_, s_)); }
+  double reactionForce(int dim = 2) { double r; check(wfx_reaction_force(c_, dim, &r, s_)); return r; }
+
+  // The CPU-branch loop body of SolveChungHulbert written with the per-step
+  // calls (remeshing, contact, thermal and output left out).
+  void StepChungHulbert() {
+    UpdatePrediction();
+    ImposeBCVAllDim();
+    calcElemJAndDerivatives();
+    CalcElemVol();
+    CalcNodalVol();
+    CalcNodalMassFromVol();
+    calcElemStrainRates();
+    calcElemPressure();
+    CalcStressStrain(dt_);
+    calcArtificialViscosity();
+    calcElemForces();
+    calcElemHourglassForces();
+    assemblyForces();
+    calcAccel();
+    ImposeBCAAllDim();
+    UpdateCorrectionAccVel();
+    ImposeBCVAllDim();
+    UpdateCorrectionPos();
+  }
+
+  // SolveChungHulbert without output: init, then loop while Time < end_t.
+  // fused = true runs the same sequence through wfx_step_fused.
+  int SolveChungHulbert(bool fused = false) {
+    InitSolve();
+    double Time = 0.0;
+    int steps = 0;
+    while (Time < end_t_) {
+      if (fused)
+        check(wfx_step_fused(c_, 1, nullptr, s_));
+      else
+        StepChungHulbert();
+      Time += dt_;
+      ++steps;
+    }
+    return steps;
+  }
+
+  std::vector<double> field(const char *name) {
+    long n = 0;
+    int is_int = 0;
+    check(wfx_field_count(c_, name, &n, &is_int));
+    if (is_int) throw Error(WFX_ERR_ARG, std::string("integer field ") + name);
+    std::vector<double> out(n);
+    check(wfx_get_field(c_, name, out.data(), n));
+    return out;
+  }
+
+ private:
+  wfx_ctx *c_ = nullptr;
+  wfx_stream s_ = nullptr;
+  double dt_ = 0.0, end_t_ = 0.0;
+};
+
+}  // namespace wfx

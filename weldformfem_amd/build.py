@@ -56,6 +56,10 @@ def build(verbose=False):
     tmp = OUT + ".tmp"
     _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, OUT)
+    # C++ driver over the header-only Domain_d mirror (host code only)
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    _run(["g++", "-O2", "-std=c++17", "-I" + inc, os.path.join(HERE, "cpp", "cube_compression.cpp"),
+          "-L" + HERE, "-lwfx", "-Wl,-rpath,$ORIGIN", "-o", os.path.join(HERE, "wfx_cube")])
     return OUT
 
 
