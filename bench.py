@@ -32,11 +32,13 @@ def fused_bytes(n, s, hg=True):
     array a kernel touches is moved once, gathered nodal data once per node.
     s = element real size (8 FP64, 4 FP32-mixed); nodal arrays are double."""
     f3_reals = 12 + 12 + 1 + 2 + 1 + 1 + (1 if hg else 0) + 3 * n  # tau, eps rw; p; ep rw; sy0; sigma_y; rho; f
+    sn = s  # v, prev_a, u_dt, fi between steps: double (FP64) or float (FP32-mixed)
+    u_rw = 2 * 3 * 8 if s == 8 else 0  # FP32-mixed recovers u from x at the end of a run
     return {
         "f1_volume": (n * I4 + s, 3 * 8),
         "f2_nodal": (n * I4 + 2 * s, 2 * I4 + 8 + 3 * 8),
-        "f3_element": (n * I4 + f3_reals * s, 3 * 8 + 3 * 8 + 8),
-        "f4_node": (n * I4 + 3 * n * s, 2 * I4 + 8 + I4 + 3 * 8 + 5 * 2 * 3 * 8),
+        "f3_element": (n * I4 + f3_reals * s, 3 * 8 + 3 * sn + 8),
+        "f4_node": (n * I4 + 3 * n * s, 2 * I4 + 8 + I4 + 3 * sn + 2 * 3 * 8 + 3 * 2 * 3 * sn + u_rw),
     }
 
 
@@ -103,6 +105,25 @@ def event_times(calls, reps, torch):
         e1.synchronize()
         out[name] = e0.elapsed_time(e1) * 1e-3 / reps
     return out
+
+
+def copy_bandwidth(torch, gib=2, reps=10):
+    """device-to-device copy bandwidth (read + write bytes / time), HIP events"""
+    n = (gib << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record(stream)
+    e1.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3 / reps
+    del src, dst
+    return 2 * n * 8 / t
 
 
 def fused_calls(d):
@@ -228,6 +249,7 @@ def main():
     torch.cuda.init()
     stream = torch.cuda.current_stream().cuda_stream
     nx, ny, nz, L = (200, 200, 100, [2.0, 2.0, 1.0]) if not args.quick else (100, 100, 50, [2.0, 2.0, 1.0])
+    copy_bw = copy_bandwidth(torch)
 
     def log(msg):
         print("bench: " + msg, file=sys.stderr, flush=True)
@@ -327,7 +349,8 @@ def main():
                    "elements": E, "nodes": N, "dt": dt, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic,
-                     "algorithmic_bytes": dom_bytes, "avg_launch_us": kt64[dom] * 1e6},
+                     "algorithmic_bytes": dom_bytes, "avg_launch_us": kt64[dom] * 1e6,
+                     "measured_copy_GBs": copy_bw / 1e9, "frac_of_measured_copy": achieved * 1e9 / copy_bw},
         "step_algorithmic_bytes_per_element": step_bytes / E,
         "step_hbm_frac": step_bytes / s64 / HBM_PEAK,
         "per_kernel": per_kernel_table(kt64, fb64, E, N),

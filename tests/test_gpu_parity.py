@@ -306,6 +306,31 @@ def test_fused_trajectory_fp32_mixed(mesh):
     assert eu <= 1e-4 and er <= 1e-3 and ep <= 1e-3
 
 
+@pytest.mark.parametrize("mesh", [_hexa20, _cyl16])
+def test_fused_nonuniform_density(mesh):
+    # per-element densities: the nodal pass loads them (tet nodes have up to
+    # 24 elements: several load chunks)
+    say(f"case: fused FP64 with non-uniform density {mesh.__name__}")
+    x, el, n, bottom, top = mesh()
+    g = Domain()
+    g.CreateFromMesh(x, el, n)
+    dt = workloads.setup_compression(g, bottom, top, -20.0)
+    r = RefDomain()
+    r.CreateFromMesh(x, el, n)
+    workloads.setup_compression(r, bottom, top, -20.0, dt=dt)
+    g.init()
+    r.init()
+    rho = 7850.0 * (1.0 + 0.2 * workloads.uniform(9, g.elem_count))
+    for d in (g, r):
+        d.setField("rho", rho)
+    rg = g.runFused(40)
+    rr = r.run(40)
+    for f in ["u", "v", "mdiag"]:
+        e = rel(g.field(f), r.field(f))
+        assert e <= 1e-10, f"{f}: {e:.3e}"
+    assert rel(rg, rr) <= 1e-10
+
+
 def test_fused_intermediates_report_state_error():
     say("case: fused path intermediates raise until a per-step API step")
     from weldformfem_amd._lib import WfxError

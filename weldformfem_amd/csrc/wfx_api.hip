@@ -43,6 +43,8 @@ static void free_all(wfx_ctx *c) {
   c->react = nullptr;
   c->bc_slot = c->bc_mask = nullptr;
   c->bc_tab = nullptr;
+  c->v32 = c->pa32 = c->ud32 = c->fi32 = nullptr;
+  c->x_start = nullptr;
   c->has_fe = 0;
   c->fused_stale = false;
   c->scalars = nullptr;
@@ -59,7 +61,8 @@ static hipError_t alloc_elem(wfx_ctx *c, DevElem<T> &el) {
   if (e == hipSuccess) e = dalloc(c, &el.ptr, (cnt));
   A(detJ, E) A(dH, 3 * n * E) A(vol, E) A(vol0, E) A(rho, E) A(rho0, E) A(p, E) A(pl_strain, E)
   A(sigma_y, E) A(sy0, E) A(str_rate, 6 * E) A(rot_rate, 3 * E) A(sigma, 6 * E) A(tau, 6 * E)
-  A(eps, 6 * E) A(strain_pl_incr, 6 * E) A(f_elem, 3 * n * E) A(f_hg, 3 * n * E)
+  // f_elem is padded to whole 64-element tiles (the fused path may tile it)
+  A(eps, 6 * E) A(strain_pl_incr, 6 * E) A(f_elem, 3 * n * ((E + 63) / 64 * 64)) A(f_hg, 3 * n * E)
 #undef A
   return e;
 }
@@ -77,8 +80,11 @@ static int setup_device(wfx_ctx *c) {
   A(c->nd.voln_sum, N) A(c->nd.voln0_sum, N) A(c->nd.anp_ratio, N)
   A(c->ms.elnod, n * E) A(c->ms.nodel, n * E) A(c->ms.nodel_loc, n * E) A(c->ms.nodel_offset, N)
   A(c->ms.nodel_count, N) A(c->ms.nodel_pk, n * E) A(c->bc_slot, N) A(c->scalars, 64) A(c->status, 16)
-#undef A
   if (e == hipSuccess) e = (c->precision == WFX_FP64) ? alloc_elem(c, c->e64) : alloc_elem(c, c->e32);
+  if (c->precision == WFX_FP32_MIXED) {
+    A(c->v32, 3 * N) A(c->pa32, 3 * N) A(c->ud32, 3 * N) A(c->fi32, 3 * N) A(c->x_start, 3 * N)
+  }
+#undef A
   if (e != hipSuccess) {
     set_error(std::string("device allocation: ") + hipGetErrorString(e));
     free_all(c);
@@ -212,6 +218,20 @@ static bool find_field(wfx_ctx *c, const char *name, FieldDesc &f) {
 }
 
 static hipStream_t as_stream(wfx_stream s) { return (hipStream_t)s; }
+
+// The fused nodal pass skips the density loads when every element has the
+// same density (the stored value, so the mass sums are unchanged).
+static int detect_uniform_rho(wfx_ctx *c) {
+  std::vector<double> r(c->E);
+  void *src = c->precision == WFX_FP64 ? (void *)c->e64.rho : (void *)c->e32.rho;
+  int rc = download_real(c, r.data(), src, r.size(), true);
+  if (rc) return rc;
+  c->rho_uniform = 0.0;
+  if (!r.empty() && r[0] > 0.0 &&
+      std::all_of(r.begin(), r.end(), [&](double v) { return v == r[0]; }))
+    c->rho_uniform = r[0];
+  return WFX_OK;
+}
 
 }  // namespace wfx
 
@@ -543,7 +563,7 @@ int wfx_init(wfx_ctx *c, wfx_stream s) {
   WFX_CHECK_HIP(hipMemsetAsync(c->status, 0, 16 * sizeof(unsigned int), st));
   WFX_CHECK_HIP(hipStreamSynchronize(st));
   c->initialized = true;
-  return WFX_OK;
+  return detect_uniform_rho(c);
 }
 
 // ---- per-step API -----------------------------------------------------------
@@ -559,7 +579,12 @@ int wfx_init(wfx_ctx *c, wfx_stream s) {
 STEP_FN(wfx_update_prediction, launch_update_prediction(c, st))
 STEP_FN(wfx_calc_elem_jacobians, BY_PRECISION(c, launch_elem_jacobians, c, st))
 STEP_FN(wfx_calc_elem_vol, BY_PRECISION(c, launch_elem_vol, c, st))
-STEP_FN(wfx_calc_elem_density, BY_PRECISION(c, launch_elem_density, c, st))
+int wfx_calc_elem_density(wfx_ctx *c, wfx_stream s) {
+  REQUIRE_MESH(c);
+  LAUNCH(BY_PRECISION(c, launch_elem_density, c, as_stream(s)));
+  c->rho_uniform = 0.0;  // unknown until the next check: the fused pass loads densities
+  return WFX_OK;
+}
 STEP_FN(wfx_calc_nodal_vol, BY_PRECISION(c, launch_nodal_vol, c, st))
 STEP_FN(wfx_calc_nodal_mass_from_vol, BY_PRECISION(c, launch_nodal_mass, c, st))
 STEP_FN(wfx_calc_elem_strain_rates, BY_PRECISION(c, launch_strain_rates, c, st))
@@ -802,7 +827,9 @@ int wfx_set_field(wfx_ctx *c, const char *name, const void *host, long count) {
     for (double v : tmp)
       if (v != 0.0) { c->has_fe = 1; break; }
   }
-  return upload_real(c, f.dev, tmp.data(), tmp.size(), f.elem);
+  int rc = upload_real(c, f.dev, tmp.data(), tmp.size(), f.elem);
+  if (rc == WFX_OK && !std::strcmp(name, "rho")) rc = detect_uniform_rho(c);
+  return rc;
 }
 
 }  // extern "C"

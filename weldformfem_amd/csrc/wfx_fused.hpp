@@ -47,6 +47,26 @@ constexpr int kChunk = 8;
 constexpr bool kParkInLds = WFX_F3_PARK;
 constexpr int kParkSlots = 21;
 
+// Layout of the fused path's element-force buffer (written by F3, gathered by
+// F4, private to the fused path): row-major [3n][E], or tiled in blocks of 64
+// elements [E/64][3n][64] so that a wavefront's force rows are contiguous.
+#ifndef WFX_F_TILED
+#define WFX_F_TILED 1
+#endif
+// Non-temporal stores for the element streams written once per step.
+#ifndef WFX_NT_STORES
+#define WFX_NT_STORES 1
+#endif
+__device__ __forceinline__ long fidx(int row, long e, long Es, int nrow) {
+  if constexpr (WFX_F_TILED) return (e >> 6) * ((long)nrow * 64) + (long)row * 64 + (e & 63);
+  else return (long)row * Es + e;
+}
+template <typename T>
+__device__ __forceinline__ void st_stream(T *p, T v) {
+  if constexpr (WFX_NT_STORES) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // V0_n: CSR-ordered sum of the initial volumes (calcElemPressure :698-705)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void f0_vsum0(int N, const int *__restrict__ off,
@@ -124,13 +144,17 @@ __global__ __launch_bounds__(kBlock) void f2_nodal(int N, int nodxelem, const in
                                                    const double *__restrict__ vsum0,
                                                    double *__restrict__ ratio,
                                                    double *__restrict__ voln,
-                                                   double *__restrict__ mdiag) {
+                                                   double *__restrict__ mdiag, double rho_u) {
   int i = gtid();
   if (i >= N) return;
   int o = off[i], c = cnt[i];
   // CSR entries are taken eight at a time with independent loads, then
   // summed in CSR order (the order of the reference's serial loop).
+  // The densities of the first chunk are loaded together with the volumes and
+  // kept in registers (every hexa node and most tet nodes have <= 8
+  // elements); with a uniform density (rho_u > 0) no density is loaded.
   double s = 0.0;
+  double r0[kChunk];
   for (int k = 0; k < c; k += kChunk) {
     int e[kChunk];
     double vv[kChunk];
@@ -138,6 +162,10 @@ __global__ __launch_bounds__(kBlock) void f2_nodal(int N, int nodxelem, const in
     for (int j = 0; j < kChunk; ++j) e[j] = (k + j < c) ? (nodel_pk[o + k + j] >> 3) : -1;
 #pragma unroll
     for (int j = 0; j < kChunk; ++j) vv[j] = (e[j] >= 0) ? (double)vol[e[j]] : 0.0;
+    if (k == 0 && rho_u <= 0.0) {
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) r0[j] = (e[j] >= 0) ? (double)rho[e[j]] : 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < kChunk; ++j)
       if (k + j < c) s += vv[j];
@@ -148,12 +176,17 @@ __global__ __launch_bounds__(kBlock) void f2_nodal(int N, int nodxelem, const in
   voln[i] = vn;
   double dc = (double)c, mass = 0.0;
   for (int k = 0; k < c; k += kChunk) {
-    int e[kChunk];
     double rr[kChunk];
+    if (rho_u > 0.0) {
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) e[j] = (k + j < c) ? (nodel_pk[o + k + j] >> 3) : -1;
+      for (int j = 0; j < kChunk; ++j) rr[j] = rho_u;
+    } else if (k == 0) {
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) rr[j] = (e[j] >= 0) ? (double)rho[e[j]] : 0.0;
+      for (int j = 0; j < kChunk; ++j) rr[j] = r0[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) rr[j] = (k + j < c) ? (double)rho[nodel_pk[o + k + j] >> 3] : 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < kChunk; ++j)
       if (k + j < c) mass += 1.0 * rr[j] * vn / dc;
@@ -201,7 +234,7 @@ __device__ __forceinline__ T shape_dH(const T (&C)[3][3], int d, int a) {
 template <typename T, int NN>
 __global__ __launch_bounds__(kBlock) void f3_element(
     int E, int N, FusedParams fp, const int *__restrict__ elnod, const double *__restrict__ x,
-    const double *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
+    const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
     const T *__restrict__ rho, const T *__restrict__ sy0, T *__restrict__ tau_io,
     T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
     T *__restrict__ sigma_y, T *__restrict__ fout) {
@@ -395,14 +428,15 @@ __global__ __launch_bounds__(kBlock) void f3_element(
     pl_strain[e] = ep + dgamma;
   }
   T3<T> sig = t3_add(pI, tau);
-  eps_io[0 * Es + e] = eps_io[0 * Es + e] + D.xx * dt;
-  eps_io[1 * Es + e] = eps_io[1 * Es + e] + D.yy * dt;
-  eps_io[2 * Es + e] = eps_io[2 * Es + e] + D.zz * dt;
-  eps_io[3 * Es + e] = eps_io[3 * Es + e] + D.xy * dt;
-  eps_io[4 * Es + e] = eps_io[4 * Es + e] + D.yz * dt;
-  eps_io[5 * Es + e] = eps_io[5 * Es + e] + D.xz * dt;
-  tau_io[0 * Es + e] = tau.xx; tau_io[1 * Es + e] = tau.yy; tau_io[2 * Es + e] = tau.zz;
-  tau_io[3 * Es + e] = tau.xy; tau_io[4 * Es + e] = tau.yz; tau_io[5 * Es + e] = tau.xz;
+  st_stream(&eps_io[0 * Es + e], eps_io[0 * Es + e] + D.xx * dt);
+  st_stream(&eps_io[1 * Es + e], eps_io[1 * Es + e] + D.yy * dt);
+  st_stream(&eps_io[2 * Es + e], eps_io[2 * Es + e] + D.zz * dt);
+  st_stream(&eps_io[3 * Es + e], eps_io[3 * Es + e] + D.xy * dt);
+  st_stream(&eps_io[4 * Es + e], eps_io[4 * Es + e] + D.yz * dt);
+  st_stream(&eps_io[5 * Es + e], eps_io[5 * Es + e] + D.xz * dt);
+  st_stream(&tau_io[0 * Es + e], tau.xx); st_stream(&tau_io[1 * Es + e], tau.yy);
+  st_stream(&tau_io[2 * Es + e], tau.zz); st_stream(&tau_io[3 * Es + e], tau.xy);
+  st_stream(&tau_io[4 * Es + e], tau.yz); st_stream(&tau_io[5 * Es + e], tau.xz);
 
   // ---- internal forces (calcElemForces) and hourglass ---------------------
   if constexpr (kParkInLds) {
@@ -445,9 +479,9 @@ __global__ __launch_bounds__(kBlock) void f3_element(
         f2 = f2 - q[2];
       }
     }
-    fout[(long)(a * 3 + 0) * Es + e] = f0;
-    fout[(long)(a * 3 + 1) * Es + e] = f1;
-    fout[(long)(a * 3 + 2) * Es + e] = f2;
+    st_stream(&fout[fidx(a * 3 + 0, e, Es, 3 * NN)], f0);
+    st_stream(&fout[fidx(a * 3 + 1, e, Es, 3 * NN)], f1);
+    st_stream(&fout[fidx(a * 3 + 2, e, Es, 3 * NN)], f2);
   }
 }
 
@@ -460,6 +494,7 @@ struct NodeParams {
   double pred_v;   // (1-gamma)*dt
   int predict;     // run the next step's prediction
   int has_fe;      // external forces present
+  int nrow;        // rows of the element-force buffer (3n)
 };
 
 // F4: assembly and the whole nodal update of one step
@@ -468,9 +503,13 @@ __global__ __launch_bounds__(kBlock) void f4_node(
     int N, int E, NodeParams np, const int *__restrict__ off, const int *__restrict__ cnt,
     const int *__restrict__ nodel_pk, const T *__restrict__ fel, const double *__restrict__ fe,
     const double *__restrict__ mdiag, const int *__restrict__ bc_slot,
-    const int *__restrict__ bc_mask, const double *__restrict__ bc_val, double *__restrict__ fi,
-    double *__restrict__ v, double *__restrict__ prev_a, double *__restrict__ u_dt,
+    const int *__restrict__ bc_mask, const double *__restrict__ bc_val, T *__restrict__ fi,
+    T *__restrict__ v, T *__restrict__ prev_a, T *__restrict__ u_dt,
     double *__restrict__ x, double *__restrict__ u, unsigned int *__restrict__ status) {
+  // FP64: every nodal array is double.  FP32-mixed: velocity, previous
+  // acceleration, increment and internal force are stored in float (the
+  // update itself runs in double), positions stay double, and u is not
+  // touched per step (recovered as x - x_start when the run ends).
   int i = gtid();
   if (i >= N) return;
   const long Es = E, Ns = N;
@@ -485,7 +524,7 @@ __global__ __launch_bounds__(kBlock) void f4_node(
     for (int j = 0; j < kChunk; ++j) {
       long e = pk[j] >> 3, b = (long)(pk[j] & 7) * 3;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) q[j][d] = (pk[j] >= 0) ? (double)fel[(b + d) * Es + e] : 0.0;
+      for (int d = 0; d < 3; ++d) q[j][d] = (pk[j] >= 0) ? (double)fel[fidx((int)(b + d), e, Es, np.nrow)] : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < kChunk; ++j)
@@ -506,26 +545,72 @@ __global__ __launch_bounds__(kBlock) void f4_node(
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const long id = d * Ns + i;
-    fi[id] = f[d];
+    fi[id] = (T)f[d];
     double fed = np.has_fe ? fe[id] : 0.0;
     double a = (fed - f[d]) / m;                       // calcAccel
     if (mask & (1 << d)) a = 0.0;                       // ImposeBCA
-    double pa = prev_a[id];
+    double pa = (double)prev_a[id];
     a = np.f * (a - np.alpha * pa);                     // UpdateCorrectionAccVel
-    double vv = v[id] + np.gdt * a;
+    double vv = (double)v[id] + np.gdt * a;
     if (mask & (1 << d)) vv = bc_val[3L * slot + d];    // ImposeBCV
-    double ud = u_dt[id] + np.bdt2 * a;                 // UpdateCorrectionPos
+    double ud = (double)u_dt[id] + np.bdt2 * a;         // UpdateCorrectionPos
     x[id] = x[id] + ud;
-    u[id] = u[id] + ud;
+    if constexpr (std::is_same<T, double>::value) u[id] = u[id] + ud;
     if (np.predict) {                                   // UpdatePrediction
       ud = np.dt * (vv + np.pred_u * a);
       vv = vv + np.pred_v * a;
       if (mask & (1 << d)) vv = bc_val[3L * slot + d];  // ImposeBCV
     }
-    u_dt[id] = ud;
-    prev_a[id] = a;
-    v[id] = vv;
+    u_dt[id] = (T)ud;
+    prev_a[id] = (T)a;
+    v[id] = (T)vv;
   }
+}
+
+// FP32-mixed fused runs keep v, prev_a, u_dt and fi in float between steps:
+// narrowed once when the run starts, widened back (and u advanced by the
+// position change) when it ends.
+__global__ __launch_bounds__(kBlock) void f_pack(long n3, const double *__restrict__ v,
+                                                 const double *__restrict__ pa,
+                                                 const double *__restrict__ ud,
+                                                 const double *__restrict__ x, float *__restrict__ v32,
+                                                 float *__restrict__ pa32, float *__restrict__ ud32,
+                                                 double *__restrict__ x0) {
+  long i = gtid();
+  if (i >= n3) return;
+  v32[i] = (float)v[i];
+  pa32[i] = (float)pa[i];
+  ud32[i] = (float)ud[i];
+  x0[i] = x[i];
+}
+__global__ __launch_bounds__(kBlock) void f_unpack(long n3, const float *__restrict__ v32,
+                                                   const float *__restrict__ pa32,
+                                                   const float *__restrict__ ud32,
+                                                   const float *__restrict__ fi32,
+                                                   const double *__restrict__ x,
+                                                   const double *__restrict__ x0, double *__restrict__ v,
+                                                   double *__restrict__ pa, double *__restrict__ a,
+                                                   double *__restrict__ ud, double *__restrict__ fi,
+                                                   double *__restrict__ u) {
+  long i = gtid();
+  if (i >= n3) return;
+  v[i] = v32[i];
+  pa[i] = a[i] = pa32[i];
+  ud[i] = ud32[i];
+  fi[i] = fi32[i];
+  u[i] = u[i] + (x[i] - x0[i]);
+}
+
+// reaction force from the internal forces of the fused node pass
+template <typename T>
+__global__ __launch_bounds__(kBlock) void f_reaction(int count, const int *__restrict__ nodes,
+                                                     const T *__restrict__ fcomp,
+                                                     double *__restrict__ out) {
+  __shared__ double sh[kBlock];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < count; k += blockDim.x) s += (double)fcomp[nodes[k]];
+  double r = block_sum(s, sh);
+  if (threadIdx.x == 0) *out = r;
 }
 
 // sigma = -p I + tau (CalcStressStrain :1766), for the field after a fused run
@@ -564,6 +649,13 @@ hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s)
   const auto &st = P.stab;
   const int N = c->N, E = c->E;
   const int gE = nblocks(E), gN = nblocks(N);
+  // nodal arrays of the fused node pass in the element real type
+  T *vN, *paN, *udN, *fiN;
+  if constexpr (std::is_same<T, double>::value) {
+    vN = c->nd.v; paN = c->nd.prev_a; udN = c->nd.u_dt; fiN = c->nd.fi;
+  } else {
+    vN = c->v32; paN = c->pa32; udN = c->ud32; fiN = c->fi32;
+  }
   if (phase == 1) {
     if (c->n == 8)
       hipLaunchKernelGGL((f1_volume<T, 8>), dim3(gE), dim3(kBlock), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol);
@@ -572,7 +664,7 @@ hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s)
   } else if (phase == 2) {
     hipLaunchKernelGGL((f2_nodal<T>), dim3(gN), dim3(kBlock), 0, s, N, c->n, c->ms.nodel_offset,
                        c->ms.nodel_count, c->ms.nodel_pk, el.vol, el.rho, c->nd.voln0_sum, c->nd.anp_ratio,
-                       c->nd.voln, c->nd.mdiag);
+                       c->nd.voln, c->nd.mdiag, c->rho_uniform);
   } else if (phase == 3) {
     FusedParams fp;
     fp.dt = P.dt;
@@ -586,11 +678,11 @@ hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s)
     fp.cs0 = P.mat.cs0;
     if (c->n == 8)
       hipLaunchKernelGGL((f3_element<T, 8>), dim3(gE), dim3(kBlock), 0, s, E, N, fp, c->ms.elnod, c->nd.x,
-                         c->nd.v, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.tau, el.eps, el.p,
+                         vN, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.tau, el.eps, el.p,
                          el.pl_strain, el.sigma_y, el.f_elem);
     else
       hipLaunchKernelGGL((f3_element<T, 4>), dim3(gE), dim3(kBlock), 0, s, E, N, fp, c->ms.elnod, c->nd.x,
-                         c->nd.v, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.tau, el.eps, el.p,
+                         vN, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.tau, el.eps, el.p,
                          el.pl_strain, el.sigma_y, el.f_elem);
   } else if (phase == 4) {
     NodeParams np;
@@ -605,10 +697,10 @@ hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s)
     np.pred_v = (1.0 - P.gamma) * P.dt;
     np.predict = predict;
     np.has_fe = c->has_fe;
+    np.nrow = 3 * c->n;
     hipLaunchKernelGGL((f4_node<T>), dim3(gN), dim3(kBlock), 0, s, N, E, np, c->ms.nodel_offset,
                        c->ms.nodel_count, c->ms.nodel_pk, el.f_elem, c->nd.fe, c->nd.mdiag, c->bc_slot,
-                       c->bc_mask, c->bc_tab, c->nd.fi, c->nd.v, c->nd.prev_a, c->nd.u_dt, c->nd.x, c->nd.u,
-                       c->status);
+                       c->bc_mask, c->bc_tab, fiN, vN, paN, udN, c->nd.x, c->nd.u, c->status);
   } else if (phase == 5) {
     hipLaunchKernelGGL((f_sigma<T>), dim3(gE), dim3(kBlock), 0, s, E, el.p, el.tau, el.sigma);
   } else {
@@ -619,16 +711,29 @@ hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s)
 
 template <typename T>
 hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t s) {
+  constexpr bool mixed = !std::is_same<T, double>::value;
+  const long n3 = 3L * c->N;
+  if constexpr (mixed)
+    hipLaunchKernelGGL(f_pack, dim3(nblocks(n3)), dim3(kBlock), 0, s, n3, c->nd.v, c->nd.prev_a, c->nd.u_dt,
+                       c->nd.x, c->v32, c->pa32, c->ud32, c->x_start);
   for (int k = 0; k < nsteps; ++k) {
     for (int ph = 1; ph <= 4; ++ph) {
       hipError_t e = launch_fused_phase<T>(c, ph, k + 1 < nsteps, s);
       if (e != hipSuccess) return e;
     }
     if (hist) {
-      hipError_t e = launch_reaction(c, 2, hist + k, s);
-      if (e != hipSuccess) return e;
+      const int cnt = (int)c->react_h.size();
+      if constexpr (mixed)
+        hipLaunchKernelGGL((f_reaction<float>), dim3(1), dim3(kBlock), 0, s, cnt, c->react,
+                           c->fi32 + 2L * c->N, hist + k);
+      else
+        hipLaunchKernelGGL((f_reaction<double>), dim3(1), dim3(kBlock), 0, s, cnt, c->react,
+                           c->nd.fi + 2L * c->N, hist + k);
     }
   }
+  if constexpr (mixed)
+    hipLaunchKernelGGL(f_unpack, dim3(nblocks(n3)), dim3(kBlock), 0, s, n3, c->v32, c->pa32, c->ud32, c->fi32,
+                       c->nd.x, c->x_start, c->nd.v, c->nd.prev_a, c->nd.a, c->nd.u_dt, c->nd.fi, c->nd.u);
   return launch_fused_phase<T>(c, 5, 0, s);
 }
 

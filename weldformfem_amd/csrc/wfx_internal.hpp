@@ -109,7 +109,11 @@ struct wfx_ctx {
   // constrained directions and the three prescribed values per slot
   int *bc_slot = nullptr, *bc_mask = nullptr;
   double *bc_tab = nullptr;
+  // FP32-mixed fused path: narrowed nodal state during a run, start positions
+  float *v32 = nullptr, *pa32 = nullptr, *ud32 = nullptr, *fi32 = nullptr;
+  double *x_start = nullptr;
   int has_fe = 0;            // external nodal forces set (non-zero)
+  double rho_uniform = 0.0;  // > 0: every element density equals this value
   bool fused_stale = false;  // per-step intermediates not maintained by the fused path
   double *scalars = nullptr;      // device scratch for reductions (reaction force)
   unsigned int *status = nullptr; // [0] non-finite internal forces scrubbed
