@@ -193,6 +193,45 @@ def cpu_baseline(seconds_target=12.0):
     }
 
 
+def batch_leg(torch, steps, log):
+    """256 uneven hexa specimens packed into one FP32-mixed domain: block
+    sizes from the autotuner, then `steps` steps as one captured HIP graph"""
+    from weldformfem_amd import WFX_FP32_MIXED, Domain, workloads
+
+    b = workloads.SpecimenBatch()
+    s = torch.cuda.Stream()  # graph capture needs a created stream
+    d = Domain(WFX_FP32_MIXED, stream=s.cuda_stream)
+    b.setup(d)
+    d.init()
+    N, E, _ = d.sizes()
+    probe = min(100, steps)
+    d.runFused(10, reaction=False)
+    t_default = timed(lambda: d.runFused(probe, reaction=False), torch) / probe
+    chosen = d.autotune(reps=5)
+    t_tuned = timed(lambda: d.runFused(probe, reaction=False), torch) / probe
+    log(f"batch: {E} elements, default block sizes {t_default * 1e3:.3f} ms/step, "
+        f"autotuned {chosen} {t_tuned * 1e3:.3f} ms/step")
+    d.graphCapture(steps, reaction=True)
+    d.graphLaunch(reaction=False)  # first replay uploads the graph
+    t_graph = timed(lambda: d.graphLaunch(reaction=False), torch) / steps
+    hist = d.graphLaunch(reaction=True)
+    log(f"batch graph {t_graph * 1e3:.3f} ms/step")
+    return {
+        "value": E / t_graph,
+        "unit": "element-steps/s",
+        "workload": f"{b.count} hexa compression specimens, edges U{{6..40}} per axis (seed 3), "
+                    f"{E} elements, {N} nodes, one FP32-mixed domain, {steps} steps in one HIP graph",
+        "specimen_elements_min_max": [int(min(np.diff(b.elem_off))), int(max(np.diff(b.elem_off)))],
+        "steps": steps,
+        "ms_per_step_graph": t_graph * 1e3,
+        "ms_per_step_eager_default_blocks": t_default * 1e3,
+        "ms_per_step_eager_autotuned": t_tuned * 1e3,
+        "block_sizes": chosen,
+        "reaction_groups": int(hist.shape[1]),
+        "reaction_all_finite": bool(np.all(np.isfinite(hist))),
+    }
+
+
 def pmc_probe(args):
     """Kernels for hardware-counter runs (rocprofv3 --pmc): a plain 1 GiB
     device-to-device copy whose FETCH_SIZE/WRITE_SIZE calibrate the counters,
@@ -231,6 +270,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-api", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short run for counter collection: a 1 GiB device copy (calibration), "
                          "then 3 fused steps FP64 and FP32-mixed and one per-step API step")
@@ -320,6 +361,10 @@ def main():
         api["per_kernel"] = per_kernel_table(event_times(api_calls(d64), args.kernel_reps, torch),
                                              api_bytes(n, 8), E, N)
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_leg(torch, args.batch_steps, log)
+
     cpu = None
     if not args.no_cpu and not args.quick:
         log("CPU baseline (oracle)")
@@ -356,6 +401,7 @@ def main():
         "per_kernel": per_kernel_table(kt64, fb64, E, N),
         "fp32_mixed": mixed,
         "api_path_fp64": api,
+        "batched_specimens": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -83,8 +83,17 @@ int wfx_set_ch(wfx_ctx *ctx, double alpha, double beta, double gamma);
 int wfx_set_param(wfx_ctx *ctx, const char *name, double value);
 int wfx_get_param(wfx_ctx *ctx, const char *name, double *value);
 /* nodes whose internal force is summed by wfx_reaction_force (no counterpart
- * in the reference; it has no reaction output) */
+ * in the reference; it has no reaction output): one group */
 int wfx_set_reaction_nodes(wfx_ctx *ctx, const int *nodes, int count);
+/* several groups (e.g. one per specimen of a batch): group g is
+ * nodes[group_offsets[g] .. group_offsets[g+1]); reaction outputs and
+ * histories then hold ngroups values per step (no reference counterpart) */
+int wfx_set_reaction_groups(wfx_ctx *ctx, const int *nodes, int count, const int *group_offsets,
+                            int ngroups);
+int wfx_reaction_group_count(wfx_ctx *ctx, int *ngroups);
+/* per-element bilinear tangent modulus (one Material_ per element, as the
+ * reference's mat[e] pointers allow; used by specimen batches) */
+int wfx_set_elem_tangent(wfx_ctx *ctx, const double *Et, int elem_count);
 /* Domain_d::calcMinEdgeLength + getMinLength (Domain_d.C:2224-2469): the
  * minimum edge of the tet formed by each element's first four nodes,
  * reduced on the device. */
@@ -133,8 +142,23 @@ int wfx_step_fused(wfx_ctx *ctx, int nsteps, double *reaction_hist, wfx_stream s
  * 2 nodal sums, 3 element chain, 4 node update (predict: also run the next
  * prediction), 5 sigma = -p I + tau. */
 int wfx_fused_phase(wfx_ctx *ctx, int phase, int predict, wfx_stream s);
-/* Sum of internal force component dim over the reaction nodes (synchronous). */
+/* Sum of internal force component dim over each reaction group; out holds
+ * one value per group (synchronous).  Histories of wfx_run,
+ * wfx_step_fused and wfx_graph_launch are [nsteps][ngroups]. */
 int wfx_reaction_force(wfx_ctx *ctx, int dim, double *out, wfx_stream s);
+
+/* ---- launch tuning and graphs (no reference counterpart) ----------------- */
+/* Times each fused phase with block sizes 64/128/256/512 on the current mesh
+ * (reps launches each), keeps the fastest (chosen[4], may be NULL); the state
+ * is saved and restored around the measurement. */
+int wfx_autotune(wfx_ctx *ctx, int reps, int *chosen, wfx_stream s);
+int wfx_set_block_sizes(wfx_ctx *ctx, const int *block_sizes4);
+/* Capture nsteps fused steps (what wfx_step_fused(nsteps) enqueues) into a
+ * HIP graph on stream s (a created stream, not NULL); wfx_graph_launch
+ * replays it.  A launch fails with WFX_ERR_STATE when parameters, boundary
+ * conditions or block sizes changed after the capture. */
+int wfx_graph_capture(wfx_ctx *ctx, int nsteps, int with_reaction, wfx_stream s);
+int wfx_graph_launch(wfx_ctx *ctx, double *reaction_hist, wfx_stream s);
 /* Domain_d::computeEnergies (Mechanical.C:2145-2185), synchronous. */
 int wfx_energies(wfx_ctx *ctx, double *ekin, double *deint, wfx_stream s);
 /* number of non-finite internal-force entries scrubbed so far */

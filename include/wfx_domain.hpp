@@ -94,7 +94,31 @@ class Domain_d {
   void calcAccel() { check(wfx_calc_accel(c_, s_)); }
   void UpdateCorrectionAccVel() { check(wfx_update_correction_accvel(c_, s_)); }
   void UpdateCorrectionPos() { check(wfx_update_correction_pos(c_, s_)); }
-  double reactionForce(int dim = 2) { double r; check(wfx_reaction_force(c_, dim, &r, s_)); return r; }
+  // sum over the first reaction group; reactionForces gives every group
+  double reactionForce(int dim = 2) { return reactionForces(dim)[0]; }
+  std::vector<double> reactionForces(int dim = 2) {
+    int ng = 0;
+    check(wfx_reaction_group_count(c_, &ng));
+    std::vector<double> r(ng > 0 ? ng : 1);
+    check(wfx_reaction_force(c_, dim, r.data(), s_));
+    return r;
+  }
+
+  // ---- specimen batches, launch tuning, graphs (no reference counterpart) ----
+  void setReactionGroups(const std::vector<int> &nodes, const std::vector<int> &offsets) {
+    check(wfx_set_reaction_groups(c_, nodes.data(), (int)nodes.size(), offsets.data(),
+                                  (int)offsets.size() - 1));
+  }
+  void setElemTangent(const std::vector<double> &Et) {
+    check(wfx_set_elem_tangent(c_, Et.data(), (int)Et.size()));
+  }
+  std::vector<int> autotune(int reps = 5) {
+    std::vector<int> bs(4);
+    check(wfx_autotune(c_, reps, bs.data(), s_));
+    return bs;
+  }
+  void graphCapture(int nsteps, bool reaction = true) { check(wfx_graph_capture(c_, nsteps, reaction, s_)); }
+  void graphLaunch(double *reaction_hist = nullptr) { check(wfx_graph_launch(c_, reaction_hist, s_)); }
 
   // The CPU-branch loop body of SolveChungHulbert written with the per-step
   // calls (remeshing, contact, thermal and output left out).

@@ -61,6 +61,13 @@ SIGNATURES = {
     "wfx_run": (_i, [_vp, _i, _vp, _vp]),
     "wfx_step_fused": (_i, [_vp, _i, _vp, _vp]),
     "wfx_fused_phase": (_i, [_vp, _i, _i, _vp]),
+    "wfx_set_reaction_groups": (_i, [_vp, _vp, _i, _vp, _i]),
+    "wfx_reaction_group_count": (_i, [_vp, _pi]),
+    "wfx_set_elem_tangent": (_i, [_vp, _vp, _i]),
+    "wfx_autotune": (_i, [_vp, _i, _vp, _vp]),
+    "wfx_set_block_sizes": (_i, [_vp, _vp]),
+    "wfx_graph_capture": (_i, [_vp, _i, _i, _vp]),
+    "wfx_graph_launch": (_i, [_vp, _vp, _vp]),
     "wfx_reaction_force": (_i, [_vp, _i, _pd, _vp]),
     "wfx_energies": (_i, [_vp, _pd, _pd, _vp]),
     "wfx_nonfinite_count": (_i, [_vp, ctypes.POINTER(ctypes.c_long)]),
@@ -77,11 +84,20 @@ class WfxError(RuntimeError):
 
 
 def load():
-    """Load libwfx.so (raises if it has not been built)."""
+    """Load libwfx.so (raises if it has not been built).
+
+    PyTorch-ROCm ships its own copy of the HIP runtime.  torch is imported
+    first so that libwfx binds to that same copy: with two runtimes in one
+    process, stream handles made by torch are not valid in the library and
+    whichever runtime starts second may find no device."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise WfxError(f"wfx library not built: {LIB_PATH} (run weldformfem_amd/build.py)")
+        try:
+            import torch  # noqa: F401  (one HIP runtime per process)
+        except ImportError:
+            pass
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

@@ -32,7 +32,16 @@ static hipError_t dalloc(wfx_ctx *c, T **p, size_t count) {
   return e;
 }
 
+static void drop_graph(wfx_ctx *c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_steps = 0;
+}
+
 static void free_all(wfx_ctx *c) {
+  drop_graph(c);
   for (void *p : c->allocations) (void)hipFree(p);
   c->allocations.clear();
   c->nd = DevNode();
@@ -499,15 +508,39 @@ int wfx_get_param(wfx_ctx *c, const char *name, double *value) {
   return WFX_OK;
 }
 
-int wfx_set_reaction_nodes(wfx_ctx *c, const int *nodes, int count) {
+int wfx_set_reaction_groups(wfx_ctx *c, const int *nodes, int count, const int *group_offsets,
+                            int ngroups) {
   REQUIRE_MESH(c);
   REQUIRE(count >= 0 && (count == 0 || nodes), WFX_ERR_ARG, "bad node list");
+  REQUIRE(ngroups >= 1 && group_offsets, WFX_ERR_ARG, "need at least one group");
+  REQUIRE(group_offsets[0] == 0 && group_offsets[ngroups] == count, WFX_ERR_ARG,
+          "group offsets must start at 0 and end at count");
+  for (int g = 0; g < ngroups; ++g)
+    REQUIRE(group_offsets[g + 1] >= group_offsets[g], WFX_ERR_ARG, "group offsets must not decrease");
   for (int k = 0; k < count; ++k) REQUIRE(nodes[k] >= 0 && nodes[k] < c->N, WFX_ERR_ARG, "reaction node out of range");
   c->react_h.assign(nodes, nodes + count);
-  int *p = nullptr;
+  c->react_off_h.assign(group_offsets, group_offsets + ngroups + 1);
+  int *p = nullptr, *q = nullptr;
   WFX_CHECK_HIP(dalloc(c, &p, count));
+  WFX_CHECK_HIP(dalloc(c, &q, ngroups + 1));
   if (count) WFX_CHECK_HIP(hipMemcpy(p, nodes, count * sizeof(int), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(q, group_offsets, (ngroups + 1) * sizeof(int), hipMemcpyHostToDevice));
   c->react = p;
+  c->react_off = q;
+  c->ngroups = ngroups;
+  c->hist_cap = 0;  // history layout changed
+  return WFX_OK;
+}
+
+int wfx_set_reaction_nodes(wfx_ctx *c, const int *nodes, int count) {
+  const int off[2] = {0, count};
+  return wfx_set_reaction_groups(c, nodes, count, off, 1);
+}
+
+int wfx_reaction_group_count(wfx_ctx *c, int *ngroups) {
+  REQUIRE_CTX(c);
+  REQUIRE(ngroups, WFX_ERR_ARG, "null output");
+  *ngroups = c->ngroups;
   return WFX_OK;
 }
 
@@ -633,13 +666,24 @@ static int step_api(wfx_ctx *c, hipStream_t st, double *hist) {
   LAUNCH(BY_PRECISION(c, launch_elem_forces, c, st));
   LAUNCH(BY_PRECISION(c, launch_hourglass, c, st));
   LAUNCH(BY_PRECISION(c, launch_assembly, c, st));
-  if (hist && !c->react_h.empty()) LAUNCH(launch_reaction(c, 2, hist, st));
+  if (hist && c->ngroups) LAUNCH(launch_reaction(c, 2, hist, st));
   LAUNCH(launch_calc_accel(c, st));
   for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, true, st));
   LAUNCH(launch_correction_accvel(c, st));
   for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
   LAUNCH(launch_correction_pos(c, st));
   c->fused_stale = false;  // every per-step intermediate is current again
+  return WFX_OK;
+}
+
+// device reaction history for nsteps steps of every reaction group
+static int ensure_hist(wfx_ctx *c, int nsteps, double **out) {
+  const long need = (long)nsteps * (c->ngroups > 0 ? c->ngroups : 1);
+  if (c->hist_cap < need) {
+    WFX_CHECK_HIP(dalloc(c, &c->hist, need));
+    c->hist_cap = (int)need;
+  }
+  *out = c->hist;
   return WFX_OK;
 }
 
@@ -656,22 +700,22 @@ int wfx_run(wfx_ctx *c, int nsteps, double *reaction_hist, wfx_stream s) {
   hipStream_t st = as_stream(s);
   double *dh = nullptr;
   if (reaction_hist && nsteps > 0) {
-    if (c->hist_cap < nsteps) {
-      WFX_CHECK_HIP(dalloc(c, &c->hist, nsteps));
-      c->hist_cap = nsteps;
-    }
-    dh = c->hist;
+    int rc = ensure_hist(c, nsteps, &dh);
+    if (rc) return rc;
   }
+  const int ng = c->ngroups > 0 ? c->ngroups : 1;
   for (int k = 0; k < nsteps; ++k) {
-    int rc = step_api(c, st, dh ? dh + k : nullptr);
+    int rc = step_api(c, st, dh ? dh + (long)k * ng : nullptr);
     if (rc) return rc;
   }
   if (dh) {
-    WFX_CHECK_HIP(hipMemcpyAsync(reaction_hist, dh, nsteps * sizeof(double), hipMemcpyDeviceToHost, st));
+    WFX_CHECK_HIP(hipMemcpyAsync(reaction_hist, dh, (size_t)nsteps * ng * sizeof(double), hipMemcpyDeviceToHost, st));
     WFX_CHECK_HIP(hipStreamSynchronize(st));
   }
   return WFX_OK;
 }
+
+static int enqueue_fused(wfx_ctx *c, int nsteps, double *dev_hist, hipStream_t st);
 
 int wfx_step_fused(wfx_ctx *c, int nsteps, double *reaction_hist, wfx_stream s) {
   REQUIRE_MESH(c);
@@ -681,23 +725,191 @@ int wfx_step_fused(wfx_ctx *c, int nsteps, double *reaction_hist, wfx_stream s) 
   hipStream_t st = as_stream(s);
   double *dh = nullptr;
   if (reaction_hist) {
-    if (c->hist_cap < nsteps) {
-      WFX_CHECK_HIP(dalloc(c, &c->hist, nsteps));
-      c->hist_cap = nsteps;
-    }
-    dh = c->hist;
+    int rc = ensure_hist(c, nsteps, &dh);
+    if (rc) return rc;
   }
   // V0_n (vol0 may have been replaced through wfx_set_field), then the first
   // prediction; each fused step predicts the next one in its node pass.
+  int rc = enqueue_fused(c, nsteps, dh, st);
+  if (rc) return rc;
+  c->fused_stale = true;
+  if (dh) {
+    const int ng = c->ngroups > 0 ? c->ngroups : 1;
+    WFX_CHECK_HIP(hipMemcpyAsync(reaction_hist, dh, (size_t)nsteps * ng * sizeof(double), hipMemcpyDeviceToHost, st));
+    WFX_CHECK_HIP(hipStreamSynchronize(st));
+  }
+  return WFX_OK;
+}
+
+// ---- specimen batches, autotuning, graph capture --------------------------
+int wfx_set_elem_tangent(wfx_ctx *c, const double *Et, int count) {
+  REQUIRE_MESH(c);
+  REQUIRE(Et && count == c->E, WFX_ERR_ARG, "Et needs one value per element");
+  REQUIRE(c->prm.mat.model == WFX_BILINEAR && c->prm.mat.E > 0.0, WFX_ERR_STATE,
+          "per-element tangent moduli need a bilinear material (wfx_set_material first)");
+  const double E = c->prm.mat.E;
+  std::vector<double> H(count);
+  for (int k = 0; k < count; ++k) {
+    REQUIRE(Et[k] >= 0.0 && Et[k] < E, WFX_ERR_ARG, "tangent modulus must be in [0, E)");
+    H[k] = E * Et[k] / (E - Et[k]);
+  }
+  void **slot = c->precision == WFX_FP64 ? (void **)&c->e64.He : (void **)&c->e32.He;
+  if (!*slot) {
+    if (c->precision == WFX_FP64) WFX_CHECK_HIP(dalloc(c, &c->e64.He, c->E));
+    else WFX_CHECK_HIP(dalloc(c, &c->e32.He, c->E));
+  }
+  drop_graph(c);  // kernel arguments changed
+  return upload_real(c, *slot, H.data(), H.size(), true);
+}
+
+// Buffers the fused phases write; saved and restored around autotuning.
+static std::vector<std::pair<void *, size_t>> fused_state(wfx_ctx *c) {
+  const size_t N3 = 3UL * c->N * sizeof(double), N1 = (size_t)c->N * sizeof(double);
+  const size_t es = c->precision == WFX_FP64 ? sizeof(double) : sizeof(float), E = c->E;
+  std::vector<std::pair<void *, size_t>> v = {
+      {c->nd.x, N3}, {c->nd.v, N3}, {c->nd.u, N3}, {c->nd.u_dt, N3}, {c->nd.prev_a, N3}, {c->nd.fi, N3},
+      {c->nd.mdiag, N1}, {c->nd.voln, N1}, {c->nd.anp_ratio, N1}, {c->status, 16 * sizeof(unsigned)}};
+  auto add = [&](void *p, size_t rows) { v.push_back({p, rows * E * es}); };
+  if (c->precision == WFX_FP64) {
+    auto &el = c->e64;
+    add(el.vol, 1); add(el.tau, 6); add(el.eps, 6); add(el.p, 1); add(el.pl_strain, 1); add(el.sigma_y, 1);
+  } else {
+    auto &el = c->e32;
+    add(el.vol, 1); add(el.tau, 6); add(el.eps, 6); add(el.p, 1); add(el.pl_strain, 1); add(el.sigma_y, 1);
+  }
+  return v;
+}
+
+int wfx_autotune(wfx_ctx *c, int reps, int *chosen, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->initialized, WFX_ERR_STATE, "call wfx_init first");
+  REQUIRE(reps > 0, WFX_ERR_ARG, "reps must be positive");
+  hipStream_t st = as_stream(s);
+  auto state = fused_state(c);
+  std::vector<void *> saved(state.size(), nullptr);
+  for (size_t k = 0; k < state.size(); ++k) {
+    WFX_CHECK_HIP(hipMalloc(&saved[k], state[k].second));
+    WFX_CHECK_HIP(hipMemcpyAsync(saved[k], state[k].first, state[k].second, hipMemcpyDeviceToDevice, st));
+  }
+  hipEvent_t e0, e1;
+  WFX_CHECK_HIP(hipEventCreate(&e0));
+  WFX_CHECK_HIP(hipEventCreate(&e1));
+  static const int sizes[4] = {64, 128, 256, 512};
+  hipError_t err = hipSuccess;
+  for (int ph = 1; ph <= 4 && err == hipSuccess; ++ph) {
+    float best = 1e30f;
+    int best_bs = 256;
+    for (int bs : sizes) {
+      c->fused_bs[ph] = bs;
+      err = BY_PRECISION(c, launch_fused_phase, c, ph, 1, st);  // warm-up
+      if (err == hipSuccess) err = hipEventRecord(e0, st);
+      for (int r = 0; r < reps && err == hipSuccess; ++r) err = BY_PRECISION(c, launch_fused_phase, c, ph, 1, st);
+      if (err == hipSuccess) err = hipEventRecord(e1, st);
+      if (err == hipSuccess) err = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+      if (err == hipSuccess && ms < best) { best = ms; best_bs = bs; }
+    }
+    c->fused_bs[ph] = best_bs;
+    if (chosen) chosen[ph - 1] = best_bs;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (size_t k = 0; k < state.size(); ++k) {
+    if (err == hipSuccess)
+      err = hipMemcpyAsync(state[k].first, saved[k], state[k].second, hipMemcpyDeviceToDevice, st);
+  }
+  if (err == hipSuccess) err = hipStreamSynchronize(st);
+  for (void *p : saved) (void)hipFree(p);
+  drop_graph(c);
+  WFX_CHECK_HIP(err);
+  return WFX_OK;
+}
+
+int wfx_set_block_sizes(wfx_ctx *c, const int *bs4) {
+  REQUIRE_CTX(c);
+  REQUIRE(bs4, WFX_ERR_ARG, "null block sizes");
+  for (int k = 0; k < 4; ++k)
+    REQUIRE(bs4[k] == 64 || bs4[k] == 128 || bs4[k] == 256 || bs4[k] == 512, WFX_ERR_ARG,
+            "block sizes must be 64, 128, 256 or 512");
+  for (int k = 0; k < 4; ++k) c->fused_bs[k + 1] = bs4[k];
+  drop_graph(c);
+  return WFX_OK;
+}
+
+// Everything a captured graph bakes into its kernel arguments; a launch
+// refuses to run when it changed since the capture.
+static uint64_t graph_signature(wfx_ctx *c) {
+  uint64_t h = 1469598103934665603ULL;
+  auto mix = [&](const void *p, size_t n) {
+    const unsigned char *b = (const unsigned char *)p;
+    for (size_t k = 0; k < n; ++k) h = (h ^ b[k]) * 1099511628211ULL;
+  };
+  mix(&c->prm.dt, sizeof(double) * 4);
+  mix(&c->prm.mat, sizeof(c->prm.mat));
+  mix(&c->prm.stab, sizeof(c->prm.stab));
+  const void *ptrs[] = {c->bc_nod[0], c->bc_nod[1], c->bc_nod[2], c->bc_slot, c->bc_mask, c->bc_tab,
+                        c->react, c->hist, c->e64.He, c->e32.He};
+  mix(ptrs, sizeof(ptrs));
+  int ints[] = {c->N, c->E, c->n, c->has_fe, c->bc_count[0], c->bc_count[1], c->bc_count[2],
+                (int)c->react_h.size(), c->fused_bs[1], c->fused_bs[2], c->fused_bs[3], c->fused_bs[4]};
+  mix(ints, sizeof(ints));
+  mix(&c->rho_uniform, sizeof(double));
+  return h;
+}
+
+// Enqueue the body of wfx_step_fused (kernels only: it is what a graph holds).
+static int enqueue_fused(wfx_ctx *c, int nsteps, double *dev_hist, hipStream_t st) {
   LAUNCH(BY_PRECISION(c, launch_vsum0, c, st));
   LAUNCH(launch_update_prediction(c, st));
   for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
-  LAUNCH(BY_PRECISION(c, launch_fused_steps, c, nsteps, dh, st));
-  // the reference leaves a == prev_a after UpdateCorrectionPos
-  WFX_CHECK_HIP(hipMemcpyAsync(c->nd.a, c->nd.prev_a, 3L * c->N * sizeof(double), hipMemcpyDeviceToDevice, st));
+  LAUNCH(BY_PRECISION(c, launch_fused_steps, c, nsteps, dev_hist, st));
+  LAUNCH(launch_copy(c->nd.prev_a, c->nd.a, 3L * c->N, st));  // a == prev_a after the step
+  return WFX_OK;
+}
+
+int wfx_graph_capture(wfx_ctx *c, int nsteps, int with_reaction, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->initialized, WFX_ERR_STATE, "call wfx_init first");
+  REQUIRE(nsteps > 0, WFX_ERR_ARG, "nsteps must be positive");
+  REQUIRE(s != nullptr, WFX_ERR_ARG, "graph capture needs a created stream (not the null stream)");
+  hipStream_t st = as_stream(s);
+  drop_graph(c);
+  double *dh = nullptr;
+  if (with_reaction) {
+    int rc = ensure_hist(c, nsteps, &dh);
+    if (rc) return rc;
+  }
+  WFX_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_fused(c, nsteps, dh, st);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  WFX_CHECK_HIP(e);
+  c->graph = g;
+  WFX_CHECK_HIP(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+  c->graph_steps = nsteps;
+  c->graph_hist = dh != nullptr;
+  c->graph_sig = graph_signature(c);
+  return WFX_OK;
+}
+
+int wfx_graph_launch(wfx_ctx *c, double *reaction_hist, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->graph_exec, WFX_ERR_STATE, "no captured graph: call wfx_graph_capture");
+  REQUIRE(c->graph_sig == graph_signature(c), WFX_ERR_STATE,
+          "parameters, boundary conditions or block sizes changed since wfx_graph_capture: capture again");
+  hipStream_t st = as_stream(s);
+  WFX_CHECK_HIP(hipGraphLaunch(c->graph_exec, st));
   c->fused_stale = true;
-  if (dh) {
-    WFX_CHECK_HIP(hipMemcpyAsync(reaction_hist, dh, nsteps * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (reaction_hist) {
+    REQUIRE(c->graph_hist, WFX_ERR_STATE, "graph was captured without the reaction history");
+    const int ng = c->ngroups > 0 ? c->ngroups : 1;
+    WFX_CHECK_HIP(hipMemcpyAsync(reaction_hist, c->hist, (size_t)c->graph_steps * ng * sizeof(double),
+                                 hipMemcpyDeviceToHost, st));
     WFX_CHECK_HIP(hipStreamSynchronize(st));
   }
   return WFX_OK;
@@ -715,9 +927,13 @@ int wfx_fused_phase(wfx_ctx *c, int phase, int predict, wfx_stream s) {
 int wfx_reaction_force(wfx_ctx *c, int dim, double *out, wfx_stream s) {
   REQUIRE_MESH(c);
   REQUIRE(out && dim >= 0 && dim < 3, WFX_ERR_ARG, "bad arguments");
+  REQUIRE(c->ngroups > 0, WFX_ERR_STATE, "no reaction nodes: call wfx_set_reaction_nodes");
   hipStream_t st = as_stream(s);
-  LAUNCH(launch_reaction(c, dim, c->scalars, st));
-  WFX_CHECK_HIP(hipMemcpyAsync(out, c->scalars, sizeof(double), hipMemcpyDeviceToHost, st));
+  double *dh = nullptr;
+  int rc = ensure_hist(c, 1, &dh);
+  if (rc) return rc;
+  LAUNCH(launch_reaction(c, dim, dh, st));
+  WFX_CHECK_HIP(hipMemcpyAsync(out, dh, c->ngroups * sizeof(double), hipMemcpyDeviceToHost, st));
   WFX_CHECK_HIP(hipStreamSynchronize(st));
   return WFX_OK;
 }

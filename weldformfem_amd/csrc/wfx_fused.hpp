@@ -83,8 +83,8 @@ __global__ __launch_bounds__(kBlock) void f0_vsum0(int N, const int *__restrict_
 }
 
 // F1: vol = detJ * w (calcElemJAndDerivatives determinant + CalcElemVol)
-template <typename T, int NN>
-__global__ __launch_bounds__(kBlock) void f1_volume(int E, int N, const int *__restrict__ elnod,
+template <typename T, int NN, int BS>
+__global__ __launch_bounds__(BS) void f1_volume(int E, int N, const int *__restrict__ elnod,
                                                     const double *__restrict__ x,
                                                     T *__restrict__ vol) {
   int e = gtid();
@@ -135,8 +135,8 @@ __global__ __launch_bounds__(kBlock) void f1_volume(int E, int N, const int *__r
 
 // F2: nodal volume sum in CSR order; ratio for the ANP pressure, nodal
 // volume and nodal mass.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void f2_nodal(int N, int nodxelem, const int *__restrict__ off,
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void f2_nodal(int N, int nodxelem, const int *__restrict__ off,
                                                    const int *__restrict__ cnt,
                                                    const int *__restrict__ nodel_pk,
                                                    const T *__restrict__ vol,
@@ -231,12 +231,12 @@ __device__ __forceinline__ T shape_dH(const T (&C)[3][3], int d, int a) {
 }
 
 // F3: the element chain of one step
-template <typename T, int NN>
-__global__ __launch_bounds__(kBlock) void f3_element(
+template <typename T, int NN, int BS>
+__global__ __launch_bounds__(BS) void f3_element(
     int E, int N, FusedParams fp, const int *__restrict__ elnod, const double *__restrict__ x,
     const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
-    const T *__restrict__ rho, const T *__restrict__ sy0, T *__restrict__ tau_io,
-    T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
+    const T *__restrict__ rho, const T *__restrict__ sy0, const T *__restrict__ He,
+    T *__restrict__ tau_io, T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
     T *__restrict__ sigma_y, T *__restrict__ fout) {
   int e = gtid();
   if (e >= E) return;
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(kBlock) void f3_element(
   // Park the hourglass amplitudes and the cofactor matrix in LDS (one slot
   // per thread, conflict-free) while the stress update runs: they are only
   // needed again for the forces, and keeping them in VGPRs halves occupancy.
-  __shared__ T park[kParkSlots][kBlock];
+  __shared__ T park[kParkInLds ? kParkSlots : 1][BS];
   const int tx = threadIdx.x;
   if constexpr (kParkInLds) {
 #pragma unroll
@@ -418,10 +418,17 @@ __global__ __launch_bounds__(kBlock) void f3_element(
                    T(2) * s.yz * s.yz + s.zz * s.zz);
   T sig_trial = sqrt(T(3) * J2);
   T ep = pl_strain[e];
-  T sy = yield_stress<T>(mp, sy0[e], ep);
+  // bilinear plastic modulus: per element when given (specimen batches)
+  T sy, Hb = (T)mp.H;
+  if (mp.model != HOLLOMON && He) {
+    Hb = He[e];
+    sy = sy0[e] + Hb * ep;
+  } else {
+    sy = yield_stress<T>(mp, sy0[e], ep);
+  }
   sigma_y[e] = sy;
   if (sy < sig_trial) {
-    T H = tangent_modulus<T>(mp, ep);
+    T H = (mp.model != HOLLOMON && He) ? Hb : tangent_modulus<T>(mp, ep);
     T dgamma = (sig_trial - sy) / (T(3) * G + H);
     T factor = T(1) - (T(3) * G * dgamma) / sig_trial;
     tau = t3_scale(s, factor);
@@ -498,8 +505,8 @@ struct NodeParams {
 };
 
 // F4: assembly and the whole nodal update of one step
-template <typename T>
-__global__ __launch_bounds__(kBlock) void f4_node(
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void f4_node(
     int N, int E, NodeParams np, const int *__restrict__ off, const int *__restrict__ cnt,
     const int *__restrict__ nodel_pk, const T *__restrict__ fel, const double *__restrict__ fe,
     const double *__restrict__ mdiag, const int *__restrict__ bc_slot,
@@ -601,18 +608,6 @@ __global__ __launch_bounds__(kBlock) void f_unpack(long n3, const float *__restr
   u[i] = u[i] + (x[i] - x0[i]);
 }
 
-// reaction force from the internal forces of the fused node pass
-template <typename T>
-__global__ __launch_bounds__(kBlock) void f_reaction(int count, const int *__restrict__ nodes,
-                                                     const T *__restrict__ fcomp,
-                                                     double *__restrict__ out) {
-  __shared__ double sh[kBlock];
-  double s = 0.0;
-  for (int k = threadIdx.x; k < count; k += blockDim.x) s += (double)fcomp[nodes[k]];
-  double r = block_sum(s, sh);
-  if (threadIdx.x == 0) *out = r;
-}
-
 // sigma = -p I + tau (CalcStressStrain :1766), for the field after a fused run
 template <typename T>
 __global__ __launch_bounds__(kBlock) void f_sigma(int E, const T *__restrict__ p,
@@ -633,6 +628,7 @@ __global__ __launch_bounds__(kBlock) void f_sigma(int E, const T *__restrict__ p
 }  // namespace
 
 hipError_t launch_reaction(wfx_ctx *c, int dim, double *dev_out, hipStream_t s);
+hipError_t launch_reaction_f32(wfx_ctx *c, const float *fi32, double *dev_out, hipStream_t s);
 
 template <typename T>
 hipError_t launch_vsum0(wfx_ctx *c, hipStream_t s) {
@@ -641,14 +637,15 @@ hipError_t launch_vsum0(wfx_ctx *c, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Launch one phase (1..4) of a fused step on the current state.
-template <typename T>
-hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s) {
+// Launch one phase (1..4) of a fused step on the current state with block
+// size BS; launch_fused_phase picks BS from the context (autotuned or 256).
+template <typename T, int BS>
+hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, hipStream_t s) {
   auto &el = elem<T>(c);
   const auto &P = c->prm;
   const auto &st = P.stab;
   const int N = c->N, E = c->E;
-  const int gE = nblocks(E), gN = nblocks(N);
+  const int gE = (E + BS - 1) / BS, gN = (N + BS - 1) / BS;
   // nodal arrays of the fused node pass in the element real type
   T *vN, *paN, *udN, *fiN;
   if constexpr (std::is_same<T, double>::value) {
@@ -658,11 +655,11 @@ hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s)
   }
   if (phase == 1) {
     if (c->n == 8)
-      hipLaunchKernelGGL((f1_volume<T, 8>), dim3(gE), dim3(kBlock), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol);
+      hipLaunchKernelGGL((f1_volume<T, 8, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol);
     else
-      hipLaunchKernelGGL((f1_volume<T, 4>), dim3(gE), dim3(kBlock), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol);
+      hipLaunchKernelGGL((f1_volume<T, 4, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol);
   } else if (phase == 2) {
-    hipLaunchKernelGGL((f2_nodal<T>), dim3(gN), dim3(kBlock), 0, s, N, c->n, c->ms.nodel_offset,
+    hipLaunchKernelGGL((f2_nodal<T, BS>), dim3(gN), dim3(BS), 0, s, N, c->n, c->ms.nodel_offset,
                        c->ms.nodel_count, c->ms.nodel_pk, el.vol, el.rho, c->nd.voln0_sum, c->nd.anp_ratio,
                        c->nd.voln, c->nd.mdiag, c->rho_uniform);
   } else if (phase == 3) {
@@ -677,12 +674,12 @@ hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s)
     fp.hg_coeff = (c->n == 8) ? st.hg_coeff : 0.0;
     fp.cs0 = P.mat.cs0;
     if (c->n == 8)
-      hipLaunchKernelGGL((f3_element<T, 8>), dim3(gE), dim3(kBlock), 0, s, E, N, fp, c->ms.elnod, c->nd.x,
-                         vN, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.tau, el.eps, el.p,
+      hipLaunchKernelGGL((f3_element<T, 8, BS>), dim3(gE), dim3(BS), 0, s, E, N, fp, c->ms.elnod, c->nd.x,
+                         vN, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.He, el.tau, el.eps, el.p,
                          el.pl_strain, el.sigma_y, el.f_elem);
     else
-      hipLaunchKernelGGL((f3_element<T, 4>), dim3(gE), dim3(kBlock), 0, s, E, N, fp, c->ms.elnod, c->nd.x,
-                         vN, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.tau, el.eps, el.p,
+      hipLaunchKernelGGL((f3_element<T, 4, BS>), dim3(gE), dim3(BS), 0, s, E, N, fp, c->ms.elnod, c->nd.x,
+                         vN, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.He, el.tau, el.eps, el.p,
                          el.pl_strain, el.sigma_y, el.f_elem);
   } else if (phase == 4) {
     NodeParams np;
@@ -698,15 +695,26 @@ hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s)
     np.predict = predict;
     np.has_fe = c->has_fe;
     np.nrow = 3 * c->n;
-    hipLaunchKernelGGL((f4_node<T>), dim3(gN), dim3(kBlock), 0, s, N, E, np, c->ms.nodel_offset,
+    hipLaunchKernelGGL((f4_node<T, BS>), dim3(gN), dim3(BS), 0, s, N, E, np, c->ms.nodel_offset,
                        c->ms.nodel_count, c->ms.nodel_pk, el.f_elem, c->nd.fe, c->nd.mdiag, c->bc_slot,
                        c->bc_mask, c->bc_tab, fiN, vN, paN, udN, c->nd.x, c->nd.u, c->status);
   } else if (phase == 5) {
-    hipLaunchKernelGGL((f_sigma<T>), dim3(gE), dim3(kBlock), 0, s, E, el.p, el.tau, el.sigma);
+    hipLaunchKernelGGL((f_sigma<T>), dim3(nblocks(E)), dim3(kBlock), 0, s, E, el.p, el.tau, el.sigma);
   } else {
     return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s) {
+  const int bs = (phase >= 1 && phase <= 4) ? c->fused_bs[phase] : 256;
+  switch (bs) {
+    case 64: return launch_fused_phase_bs<T, 64>(c, phase, predict, s);
+    case 128: return launch_fused_phase_bs<T, 128>(c, phase, predict, s);
+    case 512: return launch_fused_phase_bs<T, 512>(c, phase, predict, s);
+    default: return launch_fused_phase_bs<T, 256>(c, phase, predict, s);
+  }
 }
 
 template <typename T>
@@ -722,13 +730,9 @@ hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t 
       if (e != hipSuccess) return e;
     }
     if (hist) {
-      const int cnt = (int)c->react_h.size();
-      if constexpr (mixed)
-        hipLaunchKernelGGL((f_reaction<float>), dim3(1), dim3(kBlock), 0, s, cnt, c->react,
-                           c->fi32 + 2L * c->N, hist + k);
-      else
-        hipLaunchKernelGGL((f_reaction<double>), dim3(1), dim3(kBlock), 0, s, cnt, c->react,
-                           c->nd.fi + 2L * c->N, hist + k);
+      hipError_t e = mixed ? launch_reaction_f32(c, c->fi32, hist + (long)k * c->ngroups, s)
+                           : launch_reaction(c, 2, hist + (long)k * c->ngroups, s);
+      if (e != hipSuccess) return e;
     }
   }
   if constexpr (mixed)

@@ -57,6 +57,7 @@ struct DevElem {
   T *str_rate = nullptr, *rot_rate = nullptr, *sigma = nullptr, *tau = nullptr, *eps = nullptr,
     *strain_pl_incr = nullptr;
   T *f_elem = nullptr, *f_hg = nullptr;
+  T *He = nullptr;  // optional per-element bilinear plastic modulus (specimen batches)
 };
 
 struct DevNode {
@@ -104,7 +105,10 @@ struct wfx_ctx {
   int *bc_nod[3] = {nullptr, nullptr, nullptr};
   double *bc_val[3] = {nullptr, nullptr, nullptr};
   int bc_count[3] = {0, 0, 0};
-  int *react = nullptr;
+  int *react = nullptr;             // reaction nodes, grouped
+  int *react_off = nullptr;         // group g = react[react_off[g] .. react_off[g+1])
+  std::vector<int> react_off_h;
+  int ngroups = 0;
   // per-node Dirichlet data for the fused node pass: slot or -1, bit mask of
   // constrained directions and the three prescribed values per slot
   int *bc_slot = nullptr, *bc_mask = nullptr;
@@ -114,6 +118,13 @@ struct wfx_ctx {
   double *x_start = nullptr;
   int has_fe = 0;            // external nodal forces set (non-zero)
   double rho_uniform = 0.0;  // > 0: every element density equals this value
+  int fused_bs[5] = {256, 256, 256, 256, 256};  // block size of fused phases 1..4
+  // captured fused-step graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_steps = 0;
+  bool graph_hist = false;
+  uint64_t graph_sig = 0;
   bool fused_stale = false;  // per-step intermediates not maintained by the fused path
   double *scalars = nullptr;      // device scratch for reductions (reaction force)
   unsigned int *status = nullptr; // [0] non-finite internal forces scrubbed

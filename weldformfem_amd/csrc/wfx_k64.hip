@@ -19,6 +19,20 @@ hipError_t launch_update_prediction(wfx_ctx *c, hipStream_t s) {
   return hipGetLastError();
 }
 
+namespace {
+// plain device copy as a kernel (usable inside captured graphs)
+__global__ __launch_bounds__(kBlock) void k_copy(long n, const double *__restrict__ src,
+                                                 double *__restrict__ dst) {
+  long i = gtid();
+  if (i < n) dst[i] = src[i];
+}
+}  // namespace
+
+hipError_t launch_copy(const double *src, double *dst, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy, dim3(nblocks(n)), dim3(kBlock), 0, s, n, src, dst);
+  return hipGetLastError();
+}
+
 hipError_t launch_impose(wfx_ctx *c, int dim, bool accel, hipStream_t s) {
   int cnt = c->bc_count[dim];
   if (cnt == 0) return hipSuccess;
@@ -52,9 +66,16 @@ hipError_t launch_correction_pos(wfx_ctx *c, hipStream_t s) {
 }
 
 hipError_t launch_reaction(wfx_ctx *c, int dim, double *dev_out, hipStream_t s) {
-  int cnt = (int)c->react_h.size();
-  hipLaunchKernelGGL(k_reaction, dim3(1), dim3(kBlock), 0, s, cnt, c->react,
+  if (c->ngroups == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_reaction<double>), dim3(c->ngroups), dim3(kBlock), 0, s, c->react_off, c->react,
                      c->nd.fi + (long)dim * c->N, dev_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_reaction_f32(wfx_ctx *c, const float *fi32, double *dev_out, hipStream_t s) {
+  if (c->ngroups == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_reaction<float>), dim3(c->ngroups), dim3(kBlock), 0, s, c->react_off, c->react,
+                     fi32 + 2L * c->N, dev_out);
   return hipGetLastError();
 }
 

@@ -463,7 +463,7 @@ __device__ __forceinline__ T tangent_modulus(const MatParams &m, T ep) {
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_stress_strain(
     int E, T dt, MatParams mp, const T *__restrict__ sr, const T *__restrict__ rr,
-    const T *__restrict__ p, const T *__restrict__ sy0, T *__restrict__ tau_io,
+    const T *__restrict__ p, const T *__restrict__ sy0, const T *__restrict__ He, T *__restrict__ tau_io,
     T *__restrict__ eps_io, T *__restrict__ sigma, T *__restrict__ plincr,
     T *__restrict__ pl_strain, T *__restrict__ sigma_y) {
   int e = gtid();
@@ -494,11 +494,13 @@ __global__ __launch_bounds__(kBlock) void k_stress_strain(
                    T(2) * s.yz * s.yz + s.zz * s.zz);
   T sig_trial = sqrt(T(3) * J2);
   T ep = pl_strain[e];
-  T sy = yield_stress<T>(mp, sy0[e], ep);
+  // bilinear plastic modulus: per element when given (specimen batches)
+  const bool per_elem = mp.model != HOLLOMON && He;
+  T sy = per_elem ? sy0[e] + He[e] * ep : yield_stress<T>(mp, sy0[e], ep);
   sigma_y[e] = sy;
   T dep = T(0);
   if (sy < sig_trial) {
-    T H = tangent_modulus<T>(mp, ep);
+    T H = per_elem ? He[e] : tangent_modulus<T>(mp, ep);
     T dgamma = (sig_trial - sy) / (T(3) * G + H);
     T factor = T(1) - (T(3) * G * dgamma) / sig_trial;
     tau = t3_scale(s, factor);
@@ -682,15 +684,19 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return r;
 }
 
-// reaction force: sum of fi component over the node set, written to out[slot]
-__global__ __launch_bounds__(kBlock) void k_reaction(int count, const int *__restrict__ nodes,
-                                                     const double *__restrict__ fcomp,
+// reaction forces: one work-group per node group sums the fi component over
+// the group's nodes (fixed order), out[g]
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_reaction(const int *__restrict__ goff,
+                                                     const int *__restrict__ nodes,
+                                                     const T *__restrict__ fcomp,
                                                      double *__restrict__ out) {
   __shared__ double sh[kBlock];
+  const int g = blockIdx.x, k0 = goff[g], k1 = goff[g + 1];
   double s = 0.0;
-  for (int k = threadIdx.x; k < count; k += blockDim.x) s += fcomp[nodes[k]];
+  for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) s += (double)fcomp[nodes[k]];
   double r = block_sum(s, sh);
-  if (threadIdx.x == 0) *out = r;
+  if (threadIdx.x == 0) out[g] = r;
 }
 
 // calcMinEdgeLength (Domain_d.C:2224-2260): per-block minimum, then one block
@@ -869,7 +875,7 @@ template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, d
     const auto &m = c->prm.mat;                                                                    \
     MatParams mp{m.model, m.G, m.H, m.sy0, m.Kh, m.mh, m.eps0, m.eps1};                            \
     hipLaunchKernelGGL((k_stress_strain<T>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, (T)dt, \
-                       mp, el.str_rate, el.rot_rate, el.p, el.sy0, el.tau, el.eps, el.sigma,       \
+                       mp, el.str_rate, el.rot_rate, el.p, el.sy0, el.He, el.tau, el.eps, el.sigma, \
                        el.strain_pl_incr, el.pl_strain, el.sigma_y);                               \
     return hipGetLastError();                                                                      \
   }                                                                                                \

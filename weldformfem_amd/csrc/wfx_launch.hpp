@@ -49,6 +49,7 @@ extern template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipSt
 extern template hipError_t launch_fused_phase<float>(wfx_ctx *, int, int, hipStream_t);
 
 hipError_t launch_update_prediction(wfx_ctx *c, hipStream_t s);
+hipError_t launch_copy(const double *src, double *dst, long n, hipStream_t s);
 hipError_t launch_impose(wfx_ctx *c, int dim, bool accel, hipStream_t s);
 hipError_t launch_calc_accel(wfx_ctx *c, hipStream_t s);
 hipError_t launch_correction_accvel(wfx_ctx *c, hipStream_t s);

@@ -140,26 +140,78 @@ class Domain:
     def step(self):
         check(self._L.wfx_step_api(self._h, self.stream))
 
+    def setReactionGroups(self, groups):
+        """one reaction node list per group (e.g. per specimen of a batch)"""
+        arrs = [np.ascontiguousarray(g, dtype=np.int32).reshape(-1) for g in groups]
+        off = np.zeros(len(arrs) + 1, dtype=np.int32)
+        off[1:] = np.cumsum([a.size for a in arrs])
+        nodes = np.concatenate(arrs) if arrs else np.zeros(0, np.int32)
+        check(self._L.wfx_set_reaction_groups(self._h, nodes.ctypes.data, nodes.size, off.ctypes.data,
+                                              len(arrs)))
+
+    def reactionGroupCount(self):
+        v = ctypes.c_int()
+        check(self._L.wfx_reaction_group_count(self._h, ctypes.byref(v)))
+        return v.value
+
+    def setElemTangent(self, Et):
+        a = np.ascontiguousarray(Et, dtype=np.float64)
+        check(self._L.wfx_set_elem_tangent(self._h, a.ctypes.data, a.size))
+
+    def _hist(self, nsteps, raw):
+        """reaction history shaped (nsteps,) for one group, (nsteps, groups) else"""
+        ng = max(self.reactionGroupCount(), 1)
+        if ng == 1:
+            return raw[:nsteps]
+        return raw[: nsteps * ng].reshape(nsteps, ng)
+
     def run(self, nsteps, reaction=True):
-        hist = np.zeros(max(int(nsteps), 1), dtype=np.float64)
+        ng = max(self.reactionGroupCount(), 1)
+        hist = np.zeros(max(int(nsteps), 1) * ng, dtype=np.float64)
         check(self._L.wfx_run(self._h, int(nsteps), hist.ctypes.data if reaction else None, self.stream))
-        return hist[:nsteps]
+        return self._hist(nsteps, hist)
 
     def runFused(self, nsteps, reaction=True):
         """nsteps steps through the four-kernel fused path (wfx_step_fused)"""
-        hist = np.zeros(max(int(nsteps), 1), dtype=np.float64)
+        ng = max(self.reactionGroupCount(), 1)
+        hist = np.zeros(max(int(nsteps), 1) * ng, dtype=np.float64)
         check(self._L.wfx_step_fused(self._h, int(nsteps), hist.ctypes.data if reaction else None,
                                      self.stream))
-        return hist[:nsteps]
+        return self._hist(nsteps, hist)
 
     def fusedPhase(self, phase, predict=True):
         """launch one kernel of the fused step (timing and profiling)"""
         check(self._L.wfx_fused_phase(self._h, int(phase), int(predict), self.stream))
 
+    def autotune(self, reps=5):
+        """pick the block size of each fused kernel by timing; returns the four sizes"""
+        out = (ctypes.c_int * 4)()
+        check(self._L.wfx_autotune(self._h, int(reps), out, self.stream))
+        return list(out)
+
+    def setBlockSizes(self, sizes):
+        a = (ctypes.c_int * 4)(*[int(s) for s in sizes])
+        check(self._L.wfx_set_block_sizes(self._h, a))
+
+    def graphCapture(self, nsteps, reaction=True):
+        """capture nsteps fused steps into a HIP graph on this domain's stream"""
+        check(self._L.wfx_graph_capture(self._h, int(nsteps), int(bool(reaction)), self.stream))
+        self._graph_steps = int(nsteps)
+
+    def graphLaunch(self, reaction=True):
+        if not reaction:
+            check(self._L.wfx_graph_launch(self._h, None, self.stream))
+            return None
+        ng = max(self.reactionGroupCount(), 1)
+        hist = np.zeros(self._graph_steps * ng, dtype=np.float64)
+        check(self._L.wfx_graph_launch(self._h, hist.ctypes.data, self.stream))
+        return self._hist(self._graph_steps, hist)
+
     def reactionForce(self, dim=2):
-        v = ctypes.c_double()
-        check(self._L.wfx_reaction_force(self._h, int(dim), ctypes.byref(v), self.stream))
-        return v.value
+        ng = max(self.reactionGroupCount(), 1)
+        v = np.zeros(ng, dtype=np.float64)
+        check(self._L.wfx_reaction_force(self._h, int(dim), _dptr(v), self.stream))
+        return float(v[0]) if ng == 1 else v
 
     def energies(self):
         ek, de = ctypes.c_double(), ctypes.c_double()

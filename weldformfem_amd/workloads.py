@@ -114,6 +114,107 @@ def lognormal_sy0(n, median=300.0e6, sigma=0.1, seed=2):
     return median * np.exp(sigma * normal(seed, n))
 
 
+def hexa_box(nx, ny, nz, h, origin=(0.0, 0.0, 0.0)):
+    """Structured hexa box of nx*ny*nz cubes of edge h: nodes x fastest, then
+    y, then z, at origin + index*h; local node order of AddBoxLength
+    (Domain_d.C:1324-1332): b1, b1+1, b2+1, b2 on the lower layer, then the
+    same one layer up.  Returns x[N,3], elnod[E,8]."""
+    px, py, pz = nx + 1, ny + 1, nz + 1
+    k, j, i = np.meshgrid(np.arange(pz), np.arange(py), np.arange(px), indexing="ij")
+    x = np.stack([origin[0] + i.reshape(-1) * h, origin[1] + j.reshape(-1) * h,
+                  origin[2] + k.reshape(-1) * h], axis=1)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    layer = px * py
+    b1 = (layer * k + px * j + i).reshape(-1)
+    b2 = (layer * k + px * (j + 1) + i).reshape(-1)
+    elnod = np.stack([b1, b1 + 1, b2 + 1, b2, b1 + layer, b1 + layer + 1, b2 + layer + 1, b2 + layer], axis=1)
+    return np.ascontiguousarray(x, dtype=np.float64), np.ascontiguousarray(elnod, dtype=np.int32)
+
+
+class SpecimenBatch:
+    """Independent hexa compression specimens packed into one mesh.
+
+    Specimen s has edge counts (nx, ny, nz) each drawn uniformly from
+    lo..hi, cubes of a common edge h (so every specimen has the same stable
+    time step), sigma_y ~ U(sy_lo, sy_hi), tangent modulus ~ U(et_lo, et_hi)
+    and top speed ~ U(v_lo, v_hi) (moving down).  Draws are taken six per
+    specimen in that order from SplitMix64(seed).  Specimens sit side by side
+    along x with a gap and share no node; their nodes and elements are
+    contiguous, in specimen order."""
+
+    def __init__(self, count=256, lo=6, hi=40, h=0.025, seed=3, sy=(200.0e6, 500.0e6),
+                 et=(0.5e9, 5.0e9), v=(0.5, 3.0)):
+        u = uniform(seed, 6 * count).reshape(count, 6)
+        self.edges = (lo + np.floor(u[:, :3] * (hi - lo + 1))).astype(np.int64)
+        self.sy0 = sy[0] + (sy[1] - sy[0]) * u[:, 3]
+        self.Et = et[0] + (et[1] - et[0]) * u[:, 4]
+        self.v_top = -(v[0] + (v[1] - v[0]) * u[:, 5])
+        self.h = h
+        xs, els, self.node_off, self.elem_off = [], [], [0], [0]
+        self.bottom, self.top = [], []
+        x0 = 0.0
+        for s in range(count):
+            nx, ny, nz = (int(q) for q in self.edges[s])
+            x, el = hexa_box(nx, ny, nz, h, (x0, 0.0, 0.0))
+            base = self.node_off[-1]
+            b, t = box_layers(nx, ny, nz)
+            self.bottom.append(b + base)
+            self.top.append(t + base)
+            xs.append(x)
+            els.append(el + base)
+            self.node_off.append(base + x.shape[0])
+            self.elem_off.append(self.elem_off[-1] + el.shape[0])
+            x0 += (nx + 4) * h
+        self.x = np.ascontiguousarray(np.concatenate(xs))
+        self.elnod = np.ascontiguousarray(np.concatenate(els))
+        self.count = count
+
+    @property
+    def elem_count(self):
+        return self.elem_off[-1]
+
+    def specimen(self, s):
+        """mesh of specimen s alone: x, elnod (local ids), bottom, top"""
+        n0, n1 = self.node_off[s], self.node_off[s + 1]
+        e0, e1 = self.elem_off[s], self.elem_off[s + 1]
+        return (self.x[n0:n1].copy(), self.elnod[e0:e1] - n0, self.bottom[s] - n0, self.top[s] - n0)
+
+    def per_element(self, values):
+        return np.repeat(np.asarray(values, dtype=np.float64), np.diff(self.elem_off))
+
+    def dt(self, mat=STEEL):
+        return cfl_dt(self.h, mat["E"], mat["nu"], mat["rho"], 8)
+
+    def setup(self, dom, mat=STEEL, hg_coeff=0.06):
+        """mesh, per-element sigma_y and tangent modulus, the velocity
+        boundary conditions of every specimen and one reaction group per
+        specimen (top face) on a Domain"""
+        dom.CreateFromMesh(self.x, self.elnod, 8)
+        dom.setDensity(mat["rho"])
+        dom.setMaterial(0, mat["E"], mat["nu"], mat["rho"], mat["sy0"], mat["Et"], 0.0)
+        dom.setElemSy0(self.per_element(self.sy0))
+        dom.setElemTangent(self.per_element(self.Et))
+        for s in range(self.count):
+            for nd in self.bottom[s]:
+                dom.AddBCVelNode(int(nd), 2, 0.0)
+            for nd in self.top[s]:
+                dom.AddBCVelNode(int(nd), 2, float(self.v_top[s]))
+        dom.AllocateBCs()
+        dom.setReactionGroups(self.top)
+        dom.set("hg_coeff", hg_coeff)
+        dt = self.dt(mat)
+        dom.SetDT(dt)
+        return dt
+
+    def setup_single(self, dom, s, mat=STEEL, hg_coeff=0.06):
+        """specimen s alone on a Domain or oracle RefDomain, with its own
+        material constants: the comparison case of the batch"""
+        x, el, bottom, top = self.specimen(s)
+        dom.CreateFromMesh(x, el, 8)
+        m = dict(mat, sy0=float(self.sy0[s]), Et=float(self.Et[s]))
+        return setup_compression(dom, bottom, top, float(self.v_top[s]), m, hg_coeff=hg_coeff, dt=self.dt(mat))
+
+
 def setup_compression(dom, bottom, top, v_top, mat=STEEL, sy0=None, hg_coeff=0.06, dt=None):
     """Material, boundary conditions, reaction set and time step for a
     compression specimen on a Domain (wfx) or RefDomain (oracle)."""
