@@ -159,6 +159,24 @@ int wfx_set_block_sizes(wfx_ctx *ctx, const int *block_sizes4);
  * conditions or block sizes changed after the capture. */
 int wfx_graph_capture(wfx_ctx *ctx, int nsteps, int with_reaction, wfx_stream s);
 int wfx_graph_launch(wfx_ctx *ctx, double *reaction_hist, wfx_stream s);
+
+/* ---- adjoint (no reference counterpart; parameter identification) -------- */
+/* Forward: nsteps fused steps from the current state (after wfx_init), as
+ * wfx_step_fused, storing the state every checkpoint_every steps; writes the
+ * reaction history [nsteps][ngroups] (may be NULL).  Bilinear material and
+ * the ANP pressure without stabilisation terms only.
+ * Backward: given dL/dR for that history, recomputes each checkpoint segment
+ * and runs the hand-written reverse step over it; grad[3] receives
+ * dL/d(sigma_y0) (summed over elements: a uniform yield stress), dL/dEt
+ * (summed: a uniform tangent modulus) and dL/d(hg_coeff).  Optional per-element
+ * arrays (length E, may be NULL): dL/d(sy0_e), dL/d(Et_e), dL/d(hg) per element.
+ * The state after the backward call is the forward run's final state.
+ * Adjoint arithmetic is FP64 in both precisions; in FP32-mixed the forward and
+ * recomputation run in FP32-mixed. */
+int wfx_adjoint_forward(wfx_ctx *ctx, int nsteps, int checkpoint_every, double *reaction_hist, wfx_stream s);
+int wfx_adjoint_backward(wfx_ctx *ctx, const double *reaction_hist_bar, double *grad, double *grad_sy0_elem,
+                         double *grad_Et_elem, double *grad_hg_elem, wfx_stream s);
+int wfx_adjoint_release(wfx_ctx *ctx);
 /* Domain_d::computeEnergies (Mechanical.C:2145-2185), synchronous. */
 int wfx_energies(wfx_ctx *ctx, double *ekin, double *deint, wfx_stream s);
 /* number of non-finite internal-force entries scrubbed so far */

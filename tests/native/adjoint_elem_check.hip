@@ -1,0 +1,180 @@
+// Host-only checker of the element forward/reverse functions of the adjoint
+// (weldformfem_amd/csrc/wfx_adjoint.hpp).  Built with hipcc --offload-host-only
+// by tests/test_adjoint_math.py; runs on the CPU.
+//
+//   adjoint_elem_check forward   < elements.txt   element forward, one line per element
+//   adjoint_elem_check vjp NN SEED COUNT PLASTIC  reverse vs central differences
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../weldformfem_amd/csrc/wfx_adjoint.hpp"
+
+using namespace wfx::adj;
+
+static unsigned long long g_state = 1;
+static double urand() {  // SplitMix64 -> [0, 1)
+  unsigned long long z = (g_state += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+static double sym() { return 2.0 * urand() - 1.0; }
+
+template <int NN>
+static int forward_mode() {
+  ElemConst k;
+  int M;
+  if (scanf("%lf %lf %lf %lf %lf %d %d", &k.dt, &k.G, &k.K, &k.cs0, &k.hg, &k.hourglass, &M) != 7) return 2;
+  for (int e = 0; e < M; ++e) {
+    ElemIn<NN> in;
+    for (int a = 0; a < NN; ++a)
+      for (int c = 0; c < 3; ++c) scanf("%lf", &in.X[a][c]);
+    for (int a = 0; a < NN; ++a)
+      for (int c = 0; c < 3; ++c) scanf("%lf", &in.V[a][c]);
+    for (int a = 0; a < NN; ++a) scanf("%lf", &in.r[a]);
+    for (int q = 0; q < 6; ++q) scanf("%lf", &in.tau[q]);
+    if (scanf("%lf %lf %lf %lf", &in.ep, &in.rho, &in.sy0, &in.H) != 4) return 2;
+    ElemFwd<NN> F;
+    elem_forward<NN>(k, in, F);
+    for (int a = 0; a < NN; ++a)
+      for (int c = 0; c < 3; ++c) printf("%.17g ", F.fnet[a][c]);
+    for (int q = 0; q < 6; ++q) printf("%.17g ", F.t2[q]);
+    printf("%.17g %.17g %.17g %d\n", F.ep2, F.p, F.vol, F.plastic);
+  }
+  return 0;
+}
+
+// scalar functional of the element outputs for the difference check
+template <int NN>
+static double phi(const ElemConst &k, const ElemIn<NN> &in, const double fb[NN][3], const double tb[6],
+                  double epb, int *plastic) {
+  ElemFwd<NN> F;
+  elem_forward<NN>(k, in, F);
+  double s = epb * F.ep2;
+  for (int a = 0; a < NN; ++a)
+    for (int c = 0; c < 3; ++c) s += fb[a][c] * F.fnet[a][c];
+  for (int q = 0; q < 6; ++q) s += tb[q] * F.t2[q];
+  if (plastic) *plastic = F.plastic;
+  return s;
+}
+
+template <int NN>
+static int vjp_mode(int count, int want_plastic) {
+  double worst = 0.0;
+  int used = 0;
+  for (int t = 0; t < count * 20 && used < count; ++t) {
+    ElemConst k{1.0e-6, 76.9e9, 166.7e9, 4600.0, 0.06, NN == 8};
+    ElemIn<NN> in;
+    const double h = 0.02;
+    for (int a = 0; a < NN; ++a) {
+      const double base[3] = {NN == 8 ? (double)((((a + 1) >> 1) & 1)) : (a == 1 ? 1.0 : 0.0),
+                              NN == 8 ? (double)((a >> 1) & 1) : (a == 2 ? 1.0 : 0.0),
+                              NN == 8 ? (double)((a >> 2) & 1) : (a == 3 ? 1.0 : 0.0)};
+      for (int c = 0; c < 3; ++c) {
+        in.X[a][c] = h * (base[c] + 0.12 * sym());
+        in.V[a][c] = 3.0 * sym();
+      }
+      in.r[a] = 1.0 + 2.0e-3 * sym();
+    }
+    for (int q = 0; q < 6; ++q) in.tau[q] = 2.0e8 * sym();
+    in.ep = 0.02 * urand();
+    in.rho = 7850.0;
+    in.H = 1.0e9 * (0.5 + urand());
+    // yield stress well below or above the trial stress
+    {
+      ElemIn<NN> probe = in;
+      probe.sy0 = 1.0e30;
+      ElemFwd<NN> F;
+      elem_forward<NN>(k, probe, F);
+      in.sy0 = (want_plastic ? 0.6 : 1.6) * F.st - in.H * in.ep;
+      if (in.sy0 <= 0.0) continue;
+    }
+    double fb[NN][3], tb[6], epb = 1.0e3 * sym();
+    for (int a = 0; a < NN; ++a)
+      for (int c = 0; c < 3; ++c) fb[a][c] = sym();
+    for (int q = 0; q < 6; ++q) tb[q] = 1.0e-8 * sym();
+    ElemFwd<NN> F;
+    elem_forward<NN>(k, in, F);
+    if (F.plastic != want_plastic) continue;
+    ElemBar<NN> B;
+    elem_reverse<NN>(k, in, F, fb, tb, epb, B);
+    double Xv[NN][3];
+    vol_reverse<NN>(in.X, B.volb, Xv);
+    // directions: all inputs at once, then every parameter alone
+    for (int dir = 0; dir < 6; ++dir) {
+      ElemIn<NN> d{};
+      double dhg = 0.0;
+      if (dir < 3) {
+        for (int a = 0; a < NN; ++a) {
+          for (int c = 0; c < 3; ++c) {
+            d.X[a][c] = (dir == 0 || dir == 2) ? 0.05 * h * sym() : 0.0;
+            d.V[a][c] = (dir == 1 || dir == 2) ? sym() : 0.0;
+          }
+          d.r[a] = (dir == 2) ? 1.0e-3 * sym() : 0.0;
+        }
+        for (int q = 0; q < 6; ++q) d.tau[q] = (dir == 2) ? 1.0e8 * sym() : 0.0;
+        d.ep = (dir == 2) ? 0.01 * sym() : 0.0;
+      } else if (dir == 3) {
+        d.sy0 = 1.0e8;
+      } else if (dir == 4) {
+        d.H = 1.0e9;
+      } else {
+        dhg = 0.05;
+      }
+      double ad = B.ep * d.ep + B.sy0 * d.sy0 + B.H * d.H + B.hg * dhg;
+      for (int a = 0; a < NN; ++a) {
+        for (int c = 0; c < 3; ++c) ad += (B.X[a][c] + Xv[a][c]) * d.X[a][c] + B.V[a][c] * d.V[a][c];
+        ad += B.r[a] * d.r[a];
+      }
+      for (int q = 0; q < 6; ++q) ad += B.tau[q] * d.tau[q];
+      // central differences: truncation ~eps^2 and rounding ~1/eps balance
+      // near 1e-4 for these input scales (error 2e-8 there)
+      const double eps = 1.0e-4;
+      auto shifted = [&](double s, int *pl) {
+        ElemIn<NN> p = in;
+        ElemConst kk = k;
+        for (int a = 0; a < NN; ++a) {
+          for (int c = 0; c < 3; ++c) {
+            p.X[a][c] += s * d.X[a][c];
+            p.V[a][c] += s * d.V[a][c];
+          }
+          p.r[a] += s * d.r[a];
+        }
+        for (int q = 0; q < 6; ++q) p.tau[q] += s * d.tau[q];
+        p.ep += s * d.ep;
+        p.sy0 += s * d.sy0;
+        p.H += s * d.H;
+        kk.hg += s * dhg;
+        return phi<NN>(kk, p, fb, tb, epb, pl);
+      };
+      int p1 = 0, p2 = 0;
+      const double fd = (shifted(eps, &p1) - shifted(-eps, &p2)) / (2.0 * eps);
+      if (p1 != want_plastic || p2 != want_plastic) continue;  // crossed the yield surface
+      const double scale = fmax(fabs(fd), fabs(ad));
+      const double err = scale > 0 ? fabs(fd - ad) / scale : 0.0;
+      if (scale > 0 && err > worst) worst = err;
+      if (dir == 0) ++used;
+    }
+  }
+  printf("%d %.3e\n", used, worst);
+  return used == count ? 0 : 3;
+}
+
+int main(int argc, char **argv) {
+  if (argc >= 3 && !strcmp(argv[1], "forward")) {
+    int nn = atoi(argv[2]);
+    return nn == 8 ? forward_mode<8>() : forward_mode<4>();
+  }
+  if (argc >= 6 && !strcmp(argv[1], "vjp")) {
+    int nn = atoi(argv[2]);
+    g_state = strtoull(argv[3], nullptr, 10);
+    int count = atoi(argv[4]), pl = atoi(argv[5]);
+    return nn == 8 ? vjp_mode<8>(count, pl) : vjp_mode<4>(count, pl);
+  }
+  fprintf(stderr, "usage: forward NN < input | vjp NN SEED COUNT PLASTIC\n");
+  return 1;
+}

@@ -1,0 +1,210 @@
+"""The hand-written adjoint (wfx_adjoint_forward / wfx_adjoint_backward).
+
+Loss: L = sum_k (R_k - R*_k)^2 over the top-face reaction history, the
+target R* from a forward solve at hidden parameters.  Gradients with respect
+to (sigma_y0, Et, hg_coeff) are checked against central finite differences
+of the GPU forward and of the oracle (the reference's CPU algorithm), the bar
+being 1e-5 relative.  Steps: 1e-6 relative for a parameter the loss is
+sensitive to (s = |theta dL/dtheta| / L >= 0.1: sigma_y0), 1e-4 for the
+others (Et, hg_coeff: s ~ 1e-3 .. 1e-1).  L itself carries rounding noise of
+~3e-14 relative, so central differences at 1e-6 on a parameter with s ~ 1e-3
+carry an error of ~3e-5 of their own (measured with the oracle on the hexa
+case for hg_coeff: steps 1e-4 and 1e-3 agree to 1e-7, 1e-5 and 1e-6 scatter
+by 1e-5).  The recorded forward must equal the plain fused run bit for bit,
+and the gradient must not depend on the checkpoint interval.
+"""
+import numpy as np
+import pytest
+
+from oracle.wfref import RefDomain
+from weldformfem_amd import Domain, WFX_FP32_MIXED, WFX_FP64, workloads
+from weldformfem_amd.adjoint import CompressionProblem, identify, reaction_history
+
+pytestmark = pytest.mark.gpu
+
+HIDDEN = np.array([320.0e6, 1.5e9, 0.05])
+THETA0 = np.array([290.0e6, 1.1e9, 0.07])
+CELLS, STEPS, EVERY = 6, 200, 30
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def problem(precision=WFX_FP64, every=EVERY, mesh=None):
+    return CompressionProblem(cells=CELLS, size=1.0, v_top=-1.0, nsteps=STEPS, every=every, precision=precision,
+                              mesh=mesh)
+
+
+def gpu_loss(pb, theta, target):
+    h = pb.forward(theta)
+    r = h - target
+    return float(np.dot(r, r))
+
+
+def oracle_hist(theta, mesh=None):
+    r = RefDomain()
+    if mesh is None:
+        r.AddBoxLength([0, 0, 0], [1.0, 1.0, 1.0], 0.5 / CELLS)
+        bottom, top = workloads.box_layers(CELLS, CELLS, CELLS)
+    else:
+        x, el, n, bottom, top = mesh
+        r.CreateFromMesh(x, el, n)
+    m = dict(workloads.STEEL, sy0=float(theta[0]), Et=float(theta[1]))
+    workloads.setup_compression(r, bottom, top, -1.0, m, hg_coeff=float(theta[2]))
+    r.init()
+    return r.run(STEPS)
+
+
+def fd_steps(theta, g, L):
+    """relative step per parameter from the loss sensitivity"""
+    return [1e-6 if abs(t * gi) >= 0.1 * L else 1e-4 for t, gi in zip(theta, g)]
+
+
+def central_fd(loss, theta, rel, which=(0, 1, 2)):
+    g = np.zeros(3)
+    for i in which:
+        h = rel[i] * theta[i]
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        g[i] = (loss(tp) - loss(tm)) / (2.0 * h)
+    return g
+
+
+def test_recorded_forward_equals_fused_run():
+    say("case: adjoint forward vs wfx_step_fused, FP64 and FP32-mixed")
+    for prec in (WFX_FP64, WFX_FP32_MIXED):
+        pb = problem(prec)
+        h = pb.forward(HIDDEN)
+        plain = problem(prec)
+        plain.set_params(HIDDEN)
+        plain.domain.init()
+        hp = plain.domain.runFused(STEPS)
+        np.testing.assert_array_equal(h, hp)
+        fields = ["x", "v", "u", "tau", "pl_strain", "prev_a", "fi"]
+        for f in fields:
+            np.testing.assert_array_equal(pb.domain.field(f), plain.domain.field(f), err_msg=f)
+        # the backward pass leaves the forward's final state in place
+        pb.backward(np.ones_like(h))
+        for f in fields:
+            np.testing.assert_array_equal(pb.domain.field(f), plain.domain.field(f), err_msg=f"after backward {f}")
+
+
+@pytest.fixture(scope="module")
+def fp64_case():
+    pb = problem()
+    target = pb.forward(HIDDEN)
+    L, g = pb.loss_and_grad(THETA0, target)
+    return pb, target, L, g
+
+
+def test_gradient_matches_gpu_finite_differences(fp64_case):
+    pb, target, L, g = fp64_case
+    say(f"case: adjoint gradient {g} vs central differences of the GPU forward")
+    rel = fd_steps(THETA0, g, L)
+    fd = central_fd(lambda t: gpu_loss(pb, t, target), THETA0, rel)
+    err = np.abs(g - fd) / np.abs(fd)
+    say(f"  steps {rel} fd {fd} rel err {err}")
+    assert np.all(err <= 1e-5)
+    assert np.all(np.abs(g) > 0)
+
+
+def test_gradient_matches_oracle_finite_differences(fp64_case):
+    pb, target, L, g = fp64_case
+    say("case: adjoint gradient vs central differences of the oracle")
+    tgt = oracle_hist(HIDDEN)
+    assert np.max(np.abs(tgt - target)) <= 1e-9 * np.max(np.abs(tgt))
+
+    def loss(t):
+        r = oracle_hist(t) - tgt
+        return float(np.dot(r, r))
+
+    fd = central_fd(loss, THETA0, fd_steps(THETA0, g, L))
+    err = np.abs(g - fd) / np.abs(fd)
+    say(f"  oracle fd {fd} rel err {err}")
+    assert np.all(err <= 1e-5)
+
+
+def test_gradient_independent_of_checkpoint_interval(fp64_case):
+    pb, target, L, g = fp64_case
+    say("case: checkpoint intervals 1, 7 and the whole run give the same gradient")
+    for every in (1, 7, STEPS):
+        q = problem(every=every)
+        L2, g2 = q.loss_and_grad(THETA0, target)
+        assert L2 == L
+        np.testing.assert_array_equal(g2, g)
+
+
+def test_per_element_gradients(fp64_case):
+    pb, target, L, g = fp64_case
+    say("case: per-element gradients sum to the uniform ones; one element by differences")
+    r = pb.forward(THETA0) - target
+    ge = pb.domain.adjointBackward(2.0 * r, per_element=True)
+    for k, key in enumerate(("sy0", "Et", "hg_coeff")):
+        s = np.sum(ge[key + "_elem"])
+        assert abs(s - g[k]) <= 1e-9 * np.sum(np.abs(ge[key + "_elem"]))
+    # yield stress of the most sensitive element alone
+    e = int(np.argmax(np.abs(ge["sy0_elem"])))
+    E = pb.domain.elem_count
+    h = 1e-6 * THETA0[0]
+
+    def loss_e(delta):
+        pb.set_params(THETA0)
+        sy = np.full(E, THETA0[0])
+        sy[e] += delta
+        pb.domain.setElemSy0(sy)
+        pb.domain.setField("x", pb.x0)
+        pb.domain.init()
+        rr = pb.domain.runFused(STEPS) - target
+        return float(np.dot(rr, rr))
+
+    fd = (loss_e(h) - loss_e(-h)) / (2 * h)
+    say(f"  element {e}: adjoint {ge['sy0_elem'][e]:.6e} fd {fd:.6e}")
+    assert abs(fd - ge["sy0_elem"][e]) <= 1e-4 * abs(fd)
+
+
+def test_tet_anp_gradient():
+    say("case: tet cylinder (ANP pressure), gradient vs GPU differences")
+    x, el, bottom, top = workloads.cylinder_tets(cells=6)
+    mesh = (x, el, 4, bottom, top)
+    pb = problem(mesh=mesh)
+    target = pb.forward(HIDDEN)
+    L, g = pb.loss_and_grad(THETA0, target)
+    fd = central_fd(lambda t: gpu_loss(pb, t, target), THETA0, fd_steps(THETA0, g, L), which=(0, 1))
+    err = np.abs(g[:2] - fd[:2]) / np.abs(fd[:2])
+    say(f"  grad {g} fd {fd} rel err {err}")
+    assert np.all(err <= 1e-5)
+    assert g[2] == 0.0  # no hourglass control on tets
+
+
+def test_mixed_precision_gradient(fp64_case):
+    pb, target, L, g = fp64_case
+    say("case: FP32-mixed forward with the FP64 adjoint vs the FP64 gradient")
+    q = problem(WFX_FP32_MIXED)
+    L2, g2 = q.loss_and_grad(THETA0, target)
+    err = np.abs(g2 - g) / np.abs(g)
+    say(f"  mixed grad {g2} rel err {err}, loss {L2:.6e} vs {L:.6e}")
+    assert np.all(err <= 1e-2)
+
+
+def test_torch_autograd_function(fp64_case):
+    import torch
+
+    pb, target, L, g = fp64_case
+    say("case: torch.autograd through the solve")
+    theta = torch.tensor(THETA0, dtype=torch.float64, requires_grad=True)
+    hist = reaction_history(theta, pb)
+    loss = torch.sum((hist - torch.from_numpy(target)) ** 2)
+    loss.backward()
+    assert abs(loss.item() - L) <= 1e-12 * L
+    np.testing.assert_allclose(theta.grad.numpy(), g, rtol=1e-12)
+
+
+def test_parameter_identification(fp64_case):
+    pb, target, L, g = fp64_case
+    say("case: identification of (sigma_y0, Et, hg_coeff) from the reaction history")
+    theta, loss, nit, losses = identify(pb, target, THETA0, scale=np.array([300e6, 1e9, 0.06]), max_iter=40)
+    say(f"  theta {theta} hidden {HIDDEN} loss {loss:.3e} after {nit} iterations")
+    assert loss < 1e-4 * losses[0]
+    assert abs(theta[0] / HIDDEN[0] - 1.0) < 1e-2

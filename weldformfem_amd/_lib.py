@@ -68,6 +68,9 @@ SIGNATURES = {
     "wfx_set_block_sizes": (_i, [_vp, _vp]),
     "wfx_graph_capture": (_i, [_vp, _i, _i, _vp]),
     "wfx_graph_launch": (_i, [_vp, _vp, _vp]),
+    "wfx_adjoint_forward": (_i, [_vp, _i, _i, _vp, _vp]),
+    "wfx_adjoint_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "wfx_adjoint_release": (_i, [_vp]),
     "wfx_reaction_force": (_i, [_vp, _i, _pd, _vp]),
     "wfx_energies": (_i, [_vp, _pd, _pd, _vp]),
     "wfx_nonfinite_count": (_i, [_vp, ctypes.POINTER(ctypes.c_long)]),

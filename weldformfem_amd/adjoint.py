@@ -1,0 +1,116 @@
+"""Material-parameter identification from reaction-force histories.
+
+The gradient comes from the library's hand-written adjoint
+(wfx_adjoint_forward / wfx_adjoint_backward: checkpoint every K steps,
+recompute each segment, reverse step kernels).  ``ReactionHistory`` exposes
+the forward solve as a torch.autograd.Function of the parameter vector
+theta = (sigma_y0, Et, hg_coeff), so a loss written in torch differentiates
+through the whole explicit solve.
+"""
+import numpy as np
+
+from . import workloads
+from .domain import WFX_FP64, Domain
+
+PARAMS = ("sy0", "Et", "hg_coeff")
+
+
+class CompressionProblem:
+    """Compression with the bottom face fixed in z and the top face moving
+    down: by default a (cells)^3 reduced-integration hexa cube of edge
+    `size` (or any mesh given as (x, elnod, nodxelem, bottom, top)), nsteps
+    steps at dt = 0.8 x the critical step.  The reaction force of the top
+    face is the observed history."""
+
+    def __init__(self, cells=30, size=1.0, v_top=-1.0, nsteps=2000, every=50, precision=WFX_FP64,
+                 mat=workloads.STEEL, stream=0, mesh=None):
+        d = Domain(precision, stream=stream)
+        if mesh is None:
+            d.AddBoxLength([0.0, 0.0, 0.0], [size, size, size], 0.5 * size / cells)
+            bottom, top = workloads.box_layers(cells, cells, cells)
+        else:  # (x, elnod, nodxelem, bottom, top), e.g. workloads.cylinder_tets
+            x, elnod, n, bottom, top = mesh
+            d.CreateFromMesh(x, elnod, n)
+        self.dt = workloads.setup_compression(d, bottom, top, v_top, mat, hg_coeff=0.06)
+        self.domain, self.mat, self.nsteps, self.every = d, dict(mat), int(nsteps), int(every)
+        self.x0 = d.field("x")
+        self.theta = None
+
+    def set_params(self, theta):
+        sy0, Et, hg = (float(t) for t in theta)
+        m = self.mat
+        d = self.domain
+        d.setMaterial(0, m["E"], m["nu"], m["rho"], sy0, Et, 0.0)
+        d.set("hg_coeff", hg)
+        d.SetDT(self.dt)
+        self.theta = (sy0, Et, hg)
+
+    def forward(self, theta):
+        """reaction history for parameters theta, recorded for backward()"""
+        self.set_params(theta)
+        d = self.domain
+        d.setField("x", self.x0)
+        d.init()
+        return d.adjointForward(self.nsteps, self.every)
+
+    def backward(self, hist_bar):
+        """dL/dtheta for dL/dR = hist_bar of the last forward()"""
+        g = self.domain.adjointBackward(hist_bar)
+        return np.array([g[k] for k in PARAMS])
+
+    def loss_and_grad(self, theta, target):
+        """L = sum_k (R_k - R*_k)^2 and its gradient"""
+        h = self.forward(theta)
+        r = h - target
+        return float(np.dot(r, r)), self.backward(2.0 * r)
+
+
+def _torch_function():
+    import torch
+
+    class ReactionHistory(torch.autograd.Function):
+        """theta (3,) float64 tensor -> reaction history (nsteps,) tensor"""
+
+        @staticmethod
+        def forward(ctx, theta, problem):
+            hist = problem.forward(theta.detach().cpu().numpy())
+            ctx.problem = problem
+            return torch.from_numpy(np.array(hist, dtype=np.float64)).to(theta.device)
+
+        @staticmethod
+        def backward(ctx, grad_hist):
+            g = ctx.problem.backward(grad_hist.detach().cpu().numpy().astype(np.float64))
+            return torch.from_numpy(g).to(grad_hist.device), None
+
+    return ReactionHistory
+
+
+_RH = None
+
+
+def reaction_history(theta, problem):
+    """torch entry point: problem.forward as a differentiable function of theta"""
+    global _RH
+    if _RH is None:
+        _RH = _torch_function()
+    return _RH.apply(theta, problem)
+
+
+def identify(problem, target, theta0, scale=None, max_iter=30, tol=1e-12):
+    """Least-squares fit of theta to a target reaction history with L-BFGS-B
+    on theta/scale; returns (theta, loss, iterations, history of losses)."""
+    from scipy.optimize import minimize
+
+    theta0 = np.asarray(theta0, dtype=np.float64)
+    scale = np.asarray(scale if scale is not None else np.abs(theta0), dtype=np.float64)
+    norm = float(np.dot(target, target))
+    losses = []
+
+    def fun(z):
+        L, g = problem.loss_and_grad(z * scale, target)
+        losses.append(L / norm)
+        return L / norm, g * scale / norm
+
+    res = minimize(fun, theta0 / scale, jac=True, method="L-BFGS-B",
+                   bounds=[(1e-3, None)] * len(theta0), options={"maxiter": max_iter, "ftol": tol, "gtol": 1e-14})
+    return res.x * scale, float(res.fun), int(res.nit), losses

@@ -22,6 +22,7 @@ UNITS = [
     ("wfx_k64.hip", ["-ffp-contract=off"]),
     ("wfx_k32.hip", []),
     ("wfx_api.hip", ["-ffp-contract=off"]),
+    ("wfx_adjoint.hip", ["-ffp-contract=off"]),
 ]
 
 
@@ -39,20 +40,34 @@ def _run(cmd):
         raise RuntimeError("build step failed: " + " ".join(cmd))
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
     hipcc = _hipcc()
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src, extra in UNITS:
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-        cmd = [hipcc] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd))
-        _run(cmd)
+        cmds.append([hipcc] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     obj = os.path.join(BUILD, "wfx_mesh.o")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, "wfx_mesh.cpp"), "-o", obj])
+    cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, "wfx_mesh.cpp"), "-o", obj])
     objs.append(obj)
+    # the units are independent: compile them concurrently
+    jobs = jobs or min(len(cmds), os.cpu_count() or 1, 8)
+    pending, running = list(cmds), []
+    while pending or running:
+        while pending and len(running) < jobs:
+            cmd = pending.pop(0)
+            if verbose:
+                print(" ".join(cmd))
+            running.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                                  text=True)))
+        cmd, proc = running.pop(0)
+        out, _ = proc.communicate()
+        if proc.returncode != 0:
+            for _, p in running:
+                p.wait()
+            sys.stderr.write(out)
+            raise RuntimeError("build step failed: " + " ".join(cmd))
     tmp = OUT + ".tmp"
     _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, OUT)

@@ -1,0 +1,402 @@
+// Element-level forward and reverse (vector-Jacobian product) of the explicit
+// step, for the hand-written adjoint.  One element, FP64, no memory traffic:
+// the kernels in wfx_adjoint.hip gather the inputs, call these functions and
+// scatter the results.  The functions are __host__ __device__ so the same
+// source can be checked on the CPU.
+//
+// Forward map of one element for one step (the per-step sequence of
+// Domain_d, see Solver_explicit.C:524-978):
+//   J       = js * sum_a S(k,a) X(a,c)                 calcElemJAndDerivatives
+//             (hexa: Domain_d.C:1822-1831, js = 1/8; tet: :1855-1887, js = 1)
+//   A       = adj(J) (transposed cofactor), detJ = det J (Matrix.h:593-727)
+//   dH(c,a) = js * sum_k A(c,k) S(k,a)                 (Domain_d.C:1843-1850)
+//   vol     = w * detJ, w = 8 (hexa) or 1/6 (tet)      CalcElemVol (Mechanical.C:264-293)
+//   L(c,d)  = (1/detJ) sum_a dH(d,a) V(a,c)            calcElemStrainRates (Mechanical.C:41-126)
+//   D = sym L, W = skew L (xy, yz, xz)
+//   p       = -K (mean_a r_a - 1), r = nodal V/V0      calcElemPressure, algorithm 0 with
+//                                                      zero stabilisation (Mechanical.C:691-819)
+//   tau1    = tau + dt (2G dev D + R(tau, W))          CalcStressStrain (Mechanical.C:1690-1698)
+//             R is tau*W^T + W*tau with the reference's tensor3 product (x column
+//             A*B, y and z columns A*B^T, Tensor3.C), which reduces to
+//             R_xx = 2(Wxy txy + Wxz txz), R_yy = R_zz = 0,
+//             R_xy = txx Wxy - txz Wyz + Wxy tyy + Wxz tyz,
+//             R_yz = txy Wxz + tyy Wyz - Wxy txz + Wyz tzz,
+//             R_xz = txx Wxz + txy Wyz + Wxy tyz + Wxz tzz
+//   s       = dev(tau1 - p I), st = sqrt(3 J2(s))      (Mechanical.C:1710-1721)
+//   sy      = sy0 + H ep (bilinear)
+//   plastic (sy < st): dg = (st - sy)/(3G + H), tau2 = (1 - 3G dg/st) s, ep2 = ep + dg
+//   elastic:           tau2 = tau1, ep2 = ep           (Mechanical.C:1723-1762)
+//   sigma   = tau2 - p I                               (:1766)
+//   f(a,c)  = w sum_d dH(d,a) sigma(c,d)               calcElemForces (Mechanical.C:448-457)
+//   hexa hourglass: hm(c,j) = sum_a V(a,c) G(j,a); fhg(a,c) = -c_h sum_j hm(c,j) G(j,a),
+//             c_h = hg vol^0.6666666 rho 0.25 cs0      (f90_ver/src/Mechanical.f90:241-303)
+//   element contribution to the internal force: fnet = f - fhg (assemblyForces,
+//   Matrices.C:42-87)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#define WFX_HD __host__ __device__ __forceinline__
+
+namespace wfx {
+namespace adj {
+
+struct ElemConst {
+  double dt, G, K, cs0, hg;
+  int hourglass;  // hexa viscous hourglass on
+};
+
+template <int NN>
+struct ElemIn {
+  double X[NN][3], V[NN][3], r[NN];
+  double tau[6];  // xx yy zz xy yz xz
+  double ep, rho, sy0, H;
+};
+
+template <int NN>
+struct ElemFwd {
+  // geometry
+  double J[3][3], C[3][3], detJ, dH[3][NN], vol;
+  // kinematics
+  double Gm[3][3];  // sum_a dH(d,a) V(a,c), stored Gm[c][d]
+  double f, D[6], W[3];
+  // stress
+  double p, t1[6], s[6], st, sy, dg, fac, t2[6], ep2, sig[6];
+  int plastic;
+  // forces
+  double hm[3][4], ch0, ch, fnet[NN][3];
+};
+
+template <int NN> struct Shape;
+template <> struct Shape<8> {
+  static constexpr double js = 0.125, w = 8.0;
+  WFX_HD static double S(int k, int a) {
+    // natural-derivative signs of node a: xi, eta, zeta rows
+    const int sx = ((a + 1) >> 1) & 1 ? 1 : -1;    // -1 1 1 -1 -1 1 1 -1
+    const int sy = (a & 2) ? 1 : -1;                // -1 -1 1 1 -1 -1 1 1
+    const int sz = (a & 4) ? 1 : -1;                // -1 -1 -1 -1 1 1 1 1
+    return (double)(k == 0 ? sx : (k == 1 ? sy : sz));
+  }
+};
+template <> struct Shape<4> {
+  static constexpr double js = 1.0, w = 1.0 / 6.0;
+  WFX_HD static double S(int k, int a) { return a == 0 ? -1.0 : (a == k + 1 ? 1.0 : 0.0); }
+};
+
+// hexa hourglass base vectors (Flanagan-Belytschko, scaled to +-1)
+WFX_HD double hg_gamma(int j, int a) {
+  // rows: {1,1,-1,-1,-1,-1,1,1}, {1,-1,-1,1,-1,1,1,-1}, {1,-1,1,-1,1,-1,1,-1}, {-1,1,-1,1,1,-1,1,-1}
+  // as bit masks, bit a set for +1
+  const unsigned rows[4] = {0xC3u, 0x69u, 0x55u, 0x5Au};
+  return ((rows[j] >> a) & 1u) ? 1.0 : -1.0;
+}
+
+WFX_HD void cofactor(const double J[3][3], double C[3][3]) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      C[r][c] = J[r1][c1] * J[r2][c2] - J[r1][c2] * J[r2][c1];
+    }
+}
+
+template <int NN>
+WFX_HD void elem_forward(const ElemConst &k, const ElemIn<NN> &in, ElemFwd<NN> &F) {
+  using Sh = Shape<NN>;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+      for (int a = 0; a < NN; ++a) s += Sh::S(r, a) * in.X[a][c];
+      F.J[r][c] = Sh::js * s;
+    }
+  cofactor(F.J, F.C);
+  F.detJ = F.J[0][0] * F.C[0][0] + F.J[0][1] * F.C[0][1] + F.J[0][2] * F.C[0][2];
+  for (int c = 0; c < 3; ++c)
+    for (int a = 0; a < NN; ++a) {
+      double s = 0.0;
+      for (int q = 0; q < 3; ++q) s += F.C[q][c] * Sh::S(q, a);  // A(c,q) = C(q,c)
+      F.dH[c][a] = Sh::js * s;
+    }
+  F.vol = Sh::w * F.detJ;
+  // velocity gradient
+  for (int c = 0; c < 3; ++c)
+    for (int d = 0; d < 3; ++d) {
+      double s = 0.0;
+      for (int a = 0; a < NN; ++a) s += F.dH[d][a] * in.V[a][c];
+      F.Gm[c][d] = s;
+    }
+  F.f = 1.0 / F.detJ;
+  double L[3][3];
+  for (int c = 0; c < 3; ++c)
+    for (int d = 0; d < 3; ++d) L[c][d] = F.f * F.Gm[c][d];
+  F.D[0] = L[0][0]; F.D[1] = L[1][1]; F.D[2] = L[2][2];
+  F.D[3] = 0.5 * (L[0][1] + L[1][0]);
+  F.D[4] = 0.5 * (L[1][2] + L[2][1]);
+  F.D[5] = 0.5 * (L[0][2] + L[2][0]);
+  F.W[0] = 0.5 * (L[0][1] - L[1][0]);
+  F.W[1] = 0.5 * (L[1][2] - L[2][1]);
+  F.W[2] = 0.5 * (L[0][2] - L[2][0]);
+  // pressure
+  double rs = 0.0;
+  for (int a = 0; a < NN; ++a) rs += in.r[a];
+  F.p = -k.K * (rs / NN - 1.0);
+  // Jaumann update
+  const double *t = in.tau;
+  const double Wxy = F.W[0], Wyz = F.W[1], Wxz = F.W[2];
+  const double R[6] = {2.0 * (Wxy * t[3] + Wxz * t[5]), 0.0, 0.0,
+                       t[0] * Wxy - t[5] * Wyz + Wxy * t[1] + Wxz * t[4],
+                       t[3] * Wxz + t[1] * Wyz - Wxy * t[5] + Wyz * t[2],
+                       t[0] * Wxz + t[3] * Wyz + Wxy * t[4] + Wxz * t[2]};
+  const double trD = (F.D[0] + F.D[1] + F.D[2]) / 3.0;
+  for (int q = 0; q < 6; ++q) {
+    const double dd = q < 3 ? F.D[q] - trD : F.D[q];
+    F.t1[q] = t[q] + k.dt * (2.0 * k.G * dd + R[q]);
+  }
+  // trial state and radial return
+  const double m = ((F.t1[0] - F.p) + (F.t1[1] - F.p) + (F.t1[2] - F.p)) / 3.0;
+  for (int q = 0; q < 3; ++q) F.s[q] = (F.t1[q] - F.p) - m;
+  for (int q = 3; q < 6; ++q) F.s[q] = F.t1[q];
+  const double J2 = 0.5 * (F.s[0] * F.s[0] + F.s[1] * F.s[1] + F.s[2] * F.s[2]) +
+                    (F.s[3] * F.s[3] + F.s[4] * F.s[4] + F.s[5] * F.s[5]);
+  F.st = sqrt(3.0 * J2);
+  F.sy = in.sy0 + in.H * in.ep;
+  F.plastic = F.sy < F.st;
+  if (F.plastic) {
+    F.dg = (F.st - F.sy) / (3.0 * k.G + in.H);
+    F.fac = 1.0 - (3.0 * k.G * F.dg) / F.st;
+    for (int q = 0; q < 6; ++q) F.t2[q] = F.fac * F.s[q];
+    F.ep2 = in.ep + F.dg;
+  } else {
+    F.dg = 0.0;
+    F.fac = 1.0;
+    for (int q = 0; q < 6; ++q) F.t2[q] = F.t1[q];
+    F.ep2 = in.ep;
+  }
+  for (int q = 0; q < 6; ++q) F.sig[q] = q < 3 ? F.t2[q] - F.p : F.t2[q];
+  // element forces
+  const double *sg = F.sig;
+  for (int a = 0; a < NN; ++a) {
+    const double h0 = F.dH[0][a], h1 = F.dH[1][a], h2 = F.dH[2][a];
+    F.fnet[a][0] = Sh::w * (h0 * sg[0] + h1 * sg[3] + h2 * sg[5]);
+    F.fnet[a][1] = Sh::w * (h1 * sg[1] + h0 * sg[3] + h2 * sg[4]);
+    F.fnet[a][2] = Sh::w * (h2 * sg[2] + h1 * sg[4] + h0 * sg[5]);
+  }
+  F.ch0 = 0.0;
+  F.ch = 0.0;
+  if (NN == 8 && k.hourglass) {
+    for (int c = 0; c < 3; ++c)
+      for (int j = 0; j < 4; ++j) {
+        double s = 0.0;
+        for (int a = 0; a < NN; ++a) s += in.V[a][c] * hg_gamma(j, a);
+        F.hm[c][j] = s;
+      }
+    F.ch0 = pow(F.vol, 0.6666666) * in.rho * 0.25 * k.cs0;
+    F.ch = k.hg * F.ch0;
+    for (int a = 0; a < NN; ++a)
+      for (int c = 0; c < 3; ++c) {
+        double g = 0.0;
+        for (int j = 0; j < 4; ++j) g -= F.hm[c][j] * hg_gamma(j, a);
+        F.fnet[a][c] -= F.ch * g;  // assembly subtracts the hourglass force
+      }
+  }
+}
+
+// Reverse of elem_forward.  Inputs: the forward record F, the cotangents of
+// the element's net nodal forces (fb, equal to the internal-force cotangent
+// of each node), of the stored deviatoric stress (taub) and of the plastic
+// strain (epb).  Outputs: cotangents of X, V, the nodal ratios r, tau, ep,
+// the element volume through the hourglass coefficient (volb, the volume
+// paths through the nodal volumes are added by the caller) and of the
+// parameters sy0, H and hg.
+template <int NN>
+struct ElemBar {
+  double X[NN][3], V[NN][3], r[NN], tau[6], ep, volb, sy0, H, hg;
+};
+
+template <int NN>
+WFX_HD void elem_reverse(const ElemConst &k, const ElemIn<NN> &in, const ElemFwd<NN> &F,
+                         const double fb[NN][3], const double taub[6], double epb, ElemBar<NN> &B) {
+  using Sh = Shape<NN>;
+  double dHb[3][NN];
+  for (int a = 0; a < NN; ++a)
+    for (int c = 0; c < 3; ++c) {
+      B.X[a][c] = 0.0;
+      B.V[a][c] = 0.0;
+      dHb[c][a] = 0.0;
+    }
+  B.volb = 0.0;
+  B.hg = 0.0;
+  // hourglass: fnet -= ch * g(V)
+  if (NN == 8 && k.hourglass) {
+    double chb = 0.0, hmb[3][4];
+    for (int c = 0; c < 3; ++c)
+      for (int j = 0; j < 4; ++j) hmb[c][j] = 0.0;
+    for (int a = 0; a < NN; ++a)
+      for (int c = 0; c < 3; ++c) {
+        double g = 0.0;
+        for (int j = 0; j < 4; ++j) g -= F.hm[c][j] * hg_gamma(j, a);
+        chb -= fb[a][c] * g;
+        const double gb = -F.ch * fb[a][c];
+        for (int j = 0; j < 4; ++j) hmb[c][j] -= gb * hg_gamma(j, a);
+      }
+    for (int a = 0; a < NN; ++a)
+      for (int c = 0; c < 3; ++c) {
+        double s = 0.0;
+        for (int j = 0; j < 4; ++j) s += hmb[c][j] * hg_gamma(j, a);
+        B.V[a][c] += s;
+      }
+    B.hg = chb * F.ch0;
+    B.volb = chb * k.hg * F.ch0 * 0.6666666 / F.vol;
+  }
+  // element forces
+  double sigb[6] = {0, 0, 0, 0, 0, 0};
+  const double *sg = F.sig;
+  for (int a = 0; a < NN; ++a) {
+    const double h0 = F.dH[0][a], h1 = F.dH[1][a], h2 = F.dH[2][a];
+    const double b0 = Sh::w * fb[a][0], b1 = Sh::w * fb[a][1], b2 = Sh::w * fb[a][2];
+    sigb[0] += b0 * h0;
+    sigb[1] += b1 * h1;
+    sigb[2] += b2 * h2;
+    sigb[3] += b0 * h1 + b1 * h0;
+    sigb[4] += b1 * h2 + b2 * h1;
+    sigb[5] += b0 * h2 + b2 * h0;
+    dHb[0][a] += b0 * sg[0] + b1 * sg[3] + b2 * sg[5];
+    dHb[1][a] += b0 * sg[3] + b1 * sg[1] + b2 * sg[4];
+    dHb[2][a] += b0 * sg[5] + b1 * sg[4] + b2 * sg[2];
+  }
+  // sigma = tau2 - p I
+  double t2b[6];
+  for (int q = 0; q < 6; ++q) t2b[q] = taub[q] + sigb[q];
+  const double pb = -(sigb[0] + sigb[1] + sigb[2]);
+  // radial return
+  double t1b[6];
+  B.ep = epb;
+  B.sy0 = 0.0;
+  B.H = 0.0;
+  if (F.plastic) {
+    const double G3 = 3.0 * k.G, den = G3 + in.H;
+    double sb[6], facb = 0.0;
+    for (int q = 0; q < 6; ++q) {
+      sb[q] = F.fac * t2b[q];
+      facb += t2b[q] * F.s[q];
+    }
+    const double dgb = epb - G3 / F.st * facb;
+    double stb = G3 * F.dg / (F.st * F.st) * facb + dgb / den;
+    const double syb = -dgb / den;
+    B.H = -dgb * F.dg / den + syb * in.ep;
+    B.sy0 = syb;
+    B.ep += syb * in.H;
+    // st = sqrt(3 J2)
+    const double J2b = 1.5 * stb / F.st;
+    for (int q = 0; q < 3; ++q) sb[q] += J2b * F.s[q];
+    for (int q = 3; q < 6; ++q) sb[q] += 2.0 * J2b * F.s[q];
+    // s = dev(tau1 - p I): the pressure cancels
+    const double mb = (sb[0] + sb[1] + sb[2]) / 3.0;
+    for (int q = 0; q < 3; ++q) t1b[q] = sb[q] - mb;
+    for (int q = 3; q < 6; ++q) t1b[q] = sb[q];
+  } else {
+    for (int q = 0; q < 6; ++q) t1b[q] = t2b[q];
+  }
+  // Jaumann update
+  const double *t = in.tau;
+  const double Wxy = F.W[0], Wyz = F.W[1], Wxz = F.W[2];
+  double tb[6], Db[6], Wb[3] = {0, 0, 0};
+  for (int q = 0; q < 6; ++q) tb[q] = t1b[q];
+  {
+    const double c2 = 2.0 * k.G * k.dt;
+    const double mb = (t1b[0] + t1b[1] + t1b[2]) / 3.0;
+    for (int q = 0; q < 3; ++q) Db[q] = c2 * (t1b[q] - mb);
+    for (int q = 3; q < 6; ++q) Db[q] = c2 * t1b[q];
+    const double Rxx = k.dt * t1b[0], Rxy = k.dt * t1b[3], Ryz = k.dt * t1b[4], Rxz = k.dt * t1b[5];
+    // R_xx = 2(Wxy txy + Wxz txz)
+    Wb[0] += 2.0 * Rxx * t[3]; tb[3] += 2.0 * Rxx * Wxy;
+    Wb[2] += 2.0 * Rxx * t[5]; tb[5] += 2.0 * Rxx * Wxz;
+    // R_xy = txx Wxy - txz Wyz + Wxy tyy + Wxz tyz
+    tb[0] += Rxy * Wxy; Wb[0] += Rxy * (t[0] + t[1]);
+    tb[5] -= Rxy * Wyz; Wb[1] -= Rxy * t[5];
+    tb[1] += Rxy * Wxy;
+    Wb[2] += Rxy * t[4]; tb[4] += Rxy * Wxz;
+    // R_yz = txy Wxz + tyy Wyz - Wxy txz + Wyz tzz
+    tb[3] += Ryz * Wxz; Wb[2] += Ryz * t[3];
+    tb[1] += Ryz * Wyz; Wb[1] += Ryz * (t[1] + t[2]);
+    Wb[0] -= Ryz * t[5]; tb[5] -= Ryz * Wxy;
+    tb[2] += Ryz * Wyz;
+    // R_xz = txx Wxz + txy Wyz + Wxy tyz + Wxz tzz
+    tb[0] += Rxz * Wxz; Wb[2] += Rxz * (t[0] + t[2]);
+    tb[3] += Rxz * Wyz; Wb[1] += Rxz * t[3];
+    Wb[0] += Rxz * t[4]; tb[4] += Rxz * Wxy;
+    tb[2] += Rxz * Wxz;
+  }
+  for (int q = 0; q < 6; ++q) B.tau[q] = tb[q];
+  // pressure
+  for (int a = 0; a < NN; ++a) B.r[a] = -k.K / NN * pb;
+  // velocity gradient
+  double Lb[3][3];
+  Lb[0][0] = Db[0]; Lb[1][1] = Db[1]; Lb[2][2] = Db[2];
+  Lb[0][1] = 0.5 * (Db[3] + Wb[0]); Lb[1][0] = 0.5 * (Db[3] - Wb[0]);
+  Lb[1][2] = 0.5 * (Db[4] + Wb[1]); Lb[2][1] = 0.5 * (Db[4] - Wb[1]);
+  Lb[0][2] = 0.5 * (Db[5] + Wb[2]); Lb[2][0] = 0.5 * (Db[5] - Wb[2]);
+  double fsb = 0.0;
+  for (int c = 0; c < 3; ++c)
+    for (int d = 0; d < 3; ++d) {
+      fsb += Lb[c][d] * F.Gm[c][d];
+      const double gb = F.f * Lb[c][d];
+      for (int a = 0; a < NN; ++a) {
+        B.V[a][c] += gb * F.dH[d][a];
+        dHb[d][a] += gb * in.V[a][c];
+      }
+    }
+  double detJb = -fsb * F.f * F.f;
+  // dH = js * A S, A(c,q) = C(q,c)
+  double Cb[3][3];
+  for (int q = 0; q < 3; ++q)
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+      for (int a = 0; a < NN; ++a) s += dHb[c][a] * Sh::S(q, a);
+      Cb[q][c] = Sh::js * s;
+    }
+  // detJ = sum_c J(0,c) C(0,c) (= det J for every row)
+  double Jb[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) Jb[r][c] = detJb * F.C[r][c];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      const double b = Cb[r][c];
+      Jb[r1][c1] += b * F.J[r2][c2];
+      Jb[r2][c2] += b * F.J[r1][c1];
+      Jb[r1][c2] -= b * F.J[r2][c1];
+      Jb[r2][c1] -= b * F.J[r1][c2];
+    }
+  for (int a = 0; a < NN; ++a)
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+      for (int r = 0; r < 3; ++r) s += Jb[r][c] * Sh::S(r, a);
+      B.X[a][c] += Sh::js * s;
+    }
+}
+
+// Cotangent of X through the element volume alone: vol = w det J(X).
+template <int NN>
+WFX_HD void vol_reverse(const double X[NN][3], double volb, double Xb[NN][3]) {
+  using Sh = Shape<NN>;
+  double J[3][3], C[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+      for (int a = 0; a < NN; ++a) s += Sh::S(r, a) * X[a][c];
+      J[r][c] = Sh::js * s;
+    }
+  cofactor(J, C);
+  const double detJb = Sh::w * volb;
+  for (int a = 0; a < NN; ++a)
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+      for (int r = 0; r < 3; ++r) s += C[r][c] * Sh::S(r, a);
+      Xb[a][c] = Sh::js * detJb * s;
+    }
+}
+
+}  // namespace adj
+}  // namespace wfx

@@ -42,6 +42,7 @@ static void drop_graph(wfx_ctx *c) {
 
 static void free_all(wfx_ctx *c) {
   drop_graph(c);
+  adjoint_free(c);
   for (void *p : c->allocations) (void)hipFree(p);
   c->allocations.clear();
   c->nd = DevNode();
@@ -762,8 +763,10 @@ int wfx_set_elem_tangent(wfx_ctx *c, const double *Et, int count) {
   return upload_real(c, *slot, H.data(), H.size(), true);
 }
 
+}  // extern "C"
+
 // Buffers the fused phases write; saved and restored around autotuning.
-static std::vector<std::pair<void *, size_t>> fused_state(wfx_ctx *c) {
+std::vector<std::pair<void *, size_t>> wfx::fused_state_buffers(wfx_ctx *c) {
   const size_t N3 = 3UL * c->N * sizeof(double), N1 = (size_t)c->N * sizeof(double);
   const size_t es = c->precision == WFX_FP64 ? sizeof(double) : sizeof(float), E = c->E;
   std::vector<std::pair<void *, size_t>> v = {
@@ -780,12 +783,14 @@ static std::vector<std::pair<void *, size_t>> fused_state(wfx_ctx *c) {
   return v;
 }
 
+extern "C" {
+
 int wfx_autotune(wfx_ctx *c, int reps, int *chosen, wfx_stream s) {
   REQUIRE_MESH(c);
   REQUIRE(c->initialized, WFX_ERR_STATE, "call wfx_init first");
   REQUIRE(reps > 0, WFX_ERR_ARG, "reps must be positive");
   hipStream_t st = as_stream(s);
-  auto state = fused_state(c);
+  auto state = fused_state_buffers(c);
   std::vector<void *> saved(state.size(), nullptr);
   for (size_t k = 0; k < state.size(); ++k) {
     WFX_CHECK_HIP(hipMalloc(&saved[k], state[k].second));

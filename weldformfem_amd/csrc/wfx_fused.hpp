@@ -717,13 +717,34 @@ hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s)
   }
 }
 
+// FP32-mixed runs keep the nodal velocity, previous acceleration and
+// increment in float between the two; no-ops for FP64.
+template <typename T>
+hipError_t launch_fused_pack(wfx_ctx *c, hipStream_t s) {
+  if constexpr (!std::is_same<T, double>::value) {
+    const long n3 = 3L * c->N;
+    hipLaunchKernelGGL(f_pack, dim3(nblocks(n3)), dim3(kBlock), 0, s, n3, c->nd.v, c->nd.prev_a, c->nd.u_dt,
+                       c->nd.x, c->v32, c->pa32, c->ud32, c->x_start);
+  }
+  return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_fused_unpack(wfx_ctx *c, hipStream_t s) {
+  if constexpr (!std::is_same<T, double>::value) {
+    const long n3 = 3L * c->N;
+    hipLaunchKernelGGL(f_unpack, dim3(nblocks(n3)), dim3(kBlock), 0, s, n3, c->v32, c->pa32, c->ud32, c->fi32,
+                       c->nd.x, c->x_start, c->nd.v, c->nd.prev_a, c->nd.a, c->nd.u_dt, c->nd.fi, c->nd.u);
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t s) {
   constexpr bool mixed = !std::is_same<T, double>::value;
-  const long n3 = 3L * c->N;
-  if constexpr (mixed)
-    hipLaunchKernelGGL(f_pack, dim3(nblocks(n3)), dim3(kBlock), 0, s, n3, c->nd.v, c->nd.prev_a, c->nd.u_dt,
-                       c->nd.x, c->v32, c->pa32, c->ud32, c->x_start);
+  {
+    hipError_t e = launch_fused_pack<T>(c, s);
+    if (e != hipSuccess) return e;
+  }
   for (int k = 0; k < nsteps; ++k) {
     for (int ph = 1; ph <= 4; ++ph) {
       hipError_t e = launch_fused_phase<T>(c, ph, k + 1 < nsteps, s);
@@ -735,9 +756,10 @@ hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t 
       if (e != hipSuccess) return e;
     }
   }
-  if constexpr (mixed)
-    hipLaunchKernelGGL(f_unpack, dim3(nblocks(n3)), dim3(kBlock), 0, s, n3, c->v32, c->pa32, c->ud32, c->fi32,
-                       c->nd.x, c->x_start, c->nd.v, c->nd.prev_a, c->nd.a, c->nd.u_dt, c->nd.fi, c->nd.u);
+  {
+    hipError_t e = launch_fused_unpack<T>(c, s);
+    if (e != hipSuccess) return e;
+  }
   return launch_fused_phase<T>(c, 5, 0, s);
 }
 

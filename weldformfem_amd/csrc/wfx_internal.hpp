@@ -32,6 +32,7 @@
 namespace wfx {
 
 void set_error(const std::string &msg);
+struct Adjoint;  // checkpointed adjoint run (wfx_adjoint.hip)
 
 enum Model { BILINEAR = 0, HOLLOMON = 1 };
 
@@ -131,4 +132,5 @@ struct wfx_ctx {
   double *hist = nullptr;         // reaction-force history of wfx_run
   int hist_cap = 0;
   std::vector<void *> allocations;
+  wfx::Adjoint *adj = nullptr;  // owned, freed with the context or a new mesh
 };

@@ -8,4 +8,6 @@ WFX_DEFINE_LAUNCHERS(float)
 template hipError_t launch_vsum0<float>(wfx_ctx *, hipStream_t);
 template hipError_t launch_fused_phase<float>(wfx_ctx *, int, int, hipStream_t);
 template hipError_t launch_fused_steps<float>(wfx_ctx *, int, double *, hipStream_t);
+template hipError_t launch_fused_pack<float>(wfx_ctx *, hipStream_t);
+template hipError_t launch_fused_unpack<float>(wfx_ctx *, hipStream_t);
 }  // namespace wfx

@@ -10,6 +10,8 @@ WFX_DEFINE_LAUNCHERS(double)
 template hipError_t launch_vsum0<double>(wfx_ctx *, hipStream_t);
 template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipStream_t);
 template hipError_t launch_fused_steps<double>(wfx_ctx *, int, double *, hipStream_t);
+template hipError_t launch_fused_pack<double>(wfx_ctx *, hipStream_t);
+template hipError_t launch_fused_unpack<double>(wfx_ctx *, hipStream_t);
 
 hipError_t launch_update_prediction(wfx_ctx *c, hipStream_t s) {
   long n3 = 3L * c->N;

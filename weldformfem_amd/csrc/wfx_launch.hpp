@@ -47,6 +47,12 @@ extern template hipError_t launch_vsum0<double>(wfx_ctx *, hipStream_t);
 extern template hipError_t launch_vsum0<float>(wfx_ctx *, hipStream_t);
 extern template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipStream_t);
 extern template hipError_t launch_fused_phase<float>(wfx_ctx *, int, int, hipStream_t);
+template <typename T> hipError_t launch_fused_pack(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_fused_unpack(wfx_ctx *c, hipStream_t s);
+extern template hipError_t launch_fused_pack<double>(wfx_ctx *, hipStream_t);
+extern template hipError_t launch_fused_pack<float>(wfx_ctx *, hipStream_t);
+extern template hipError_t launch_fused_unpack<double>(wfx_ctx *, hipStream_t);
+extern template hipError_t launch_fused_unpack<float>(wfx_ctx *, hipStream_t);
 
 hipError_t launch_update_prediction(wfx_ctx *c, hipStream_t s);
 hipError_t launch_copy(const double *src, double *dst, long n, hipStream_t s);
@@ -57,5 +63,11 @@ hipError_t launch_correction_pos(wfx_ctx *c, hipStream_t s);
 hipError_t launch_reaction(wfx_ctx *c, int dim, double *dev_out, hipStream_t s);
 hipError_t launch_min_edge(wfx_ctx *c, double *partial, double *dev_out, hipStream_t s);
 int min_edge_partial_count(wfx_ctx *c);
+hipError_t launch_reaction_f32(wfx_ctx *c, const float *fi32, double *dev_out, hipStream_t s);
+
+// device buffers a fused run writes (saved and restored around autotuning
+// and around the adjoint's recomputation)
+std::vector<std::pair<void *, size_t>> fused_state_buffers(wfx_ctx *c);
+void adjoint_free(wfx_ctx *c);
 
 }  // namespace wfx

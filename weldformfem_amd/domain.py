@@ -207,6 +207,37 @@ class Domain:
         check(self._L.wfx_graph_launch(self._h, hist.ctypes.data, self.stream))
         return self._hist(self._graph_steps, hist)
 
+    # ---- adjoint ---------------------------------------------------------
+    def adjointForward(self, nsteps, checkpoint_every=50):
+        """forward run recorded for adjointBackward; returns the reaction history"""
+        ng = max(self.reactionGroupCount(), 1)
+        hist = np.zeros(int(nsteps) * ng, dtype=np.float64)
+        check(self._L.wfx_adjoint_forward(self._h, int(nsteps), int(checkpoint_every), hist.ctypes.data,
+                                          self.stream))
+        self._adj_steps = int(nsteps)
+        return self._hist(nsteps, hist)
+
+    def adjointBackward(self, hist_bar, per_element=False):
+        """dL/d(sigma_y0, Et, hg_coeff) for the cotangent hist_bar = dL/dR of
+        the recorded history (shape as adjointForward returned)"""
+        hb = np.ascontiguousarray(hist_bar, dtype=np.float64).reshape(-1)
+        ng = max(self.reactionGroupCount(), 1)
+        if hb.size != self._adj_steps * ng:
+            raise ValueError(f"hist_bar needs {self._adj_steps * ng} values, got {hb.size}")
+        g = np.zeros(3)
+        E = self.elem_count
+        el = [np.zeros(E) for _ in range(3)] if per_element else [None, None, None]
+        check(self._L.wfx_adjoint_backward(self._h, hb.ctypes.data, g.ctypes.data,
+                                           *[(a.ctypes.data if a is not None else None) for a in el],
+                                           self.stream))
+        out = {"sy0": g[0], "Et": g[1], "hg_coeff": g[2]}
+        if per_element:
+            out.update({"sy0_elem": el[0], "Et_elem": el[1], "hg_coeff_elem": el[2]})
+        return out
+
+    def adjointRelease(self):
+        check(self._L.wfx_adjoint_release(self._h))
+
     def reactionForce(self, dim=2):
         ng = max(self.reactionGroupCount(), 1)
         v = np.zeros(ng, dtype=np.float64)
