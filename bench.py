@@ -232,6 +232,134 @@ def batch_leg(torch, steps, log):
     }
 
 
+def config_legs(torch, log):
+    """The two other single-domain workloads: the 20^3 hexa cube (8,000
+    elements, 1000 steps: launch-bound, so eager and HIP-graph launches are
+    both timed) and the 64^3-cell tet cylinder (1,239,552 tets, 500 steps)."""
+    from weldformfem_amd import WFX_FP32_MIXED, WFX_FP64, Domain, workloads
+
+    out = {}
+    s = torch.cuda.Stream()
+    for prec, tag in ((WFX_FP64, "fp64"), (WFX_FP32_MIXED, "fp32_mixed")):
+        d = Domain(prec, stream=s.cuda_stream)
+        d.AddBoxLength([0.0, 0.0, 0.0], [1.0, 1.0, 1.0], 0.025)
+        bottom, top = workloads.box_layers(20, 20, 20)
+        workloads.setup_compression(d, bottom, top, -1.0, hg_coeff=0.06)
+        d.init()
+        N, E, _ = d.sizes()
+        d.runFused(50, reaction=False)
+        t_eager = timed(lambda: d.runFused(1000, reaction=False), torch) / 1000
+        d.graphCapture(1000, reaction=False)
+        d.graphLaunch(reaction=False)
+        t_graph = timed(lambda: d.graphLaunch(reaction=False), torch) / 1000
+        out[f"cube20_{tag}"] = {"elements": E, "nodes": N, "us_per_step_eager": t_eager * 1e6,
+                                "us_per_step_graph": t_graph * 1e6, "value_graph": E / t_graph}
+        log(f"cube 20^3 {tag}: eager {t_eager * 1e6:.1f} us/step, graph {t_graph * 1e6:.1f} us/step")
+        del d
+    x, el, bottom, top = workloads.cylinder_tets(cells=64)
+    for prec, tag in ((WFX_FP64, "fp64"), (WFX_FP32_MIXED, "fp32_mixed")):
+        d = Domain(prec, stream=s.cuda_stream)
+        d.CreateFromMesh(x, el, 4)
+        workloads.setup_compression(d, bottom, top, -1.0, hg_coeff=0.0)
+        d.init()
+        N, E, _ = d.sizes()
+        d.runFused(20, reaction=False)
+        t = timed(lambda: d.runFused(500, reaction=False), torch) / 500
+        out[f"cylinder_tets_{tag}"] = {"elements": E, "nodes": N, "ms_per_step": t * 1e3, "value": E / t,
+                                       "step_algorithmic_bytes_per_element": total(fused_bytes(4, 8 if prec ==
+                                                                                               WFX_FP64 else 4,
+                                                                                               hg=False), E, N) / E}
+        log(f"tet cylinder {tag}: {t * 1e3:.3f} ms/step, {E / t:.3e} element-steps/s")
+        del d
+    return out
+
+
+def adjoint_leg(torch, log, steps=2000, cells=30, with_oracle=True):
+    """Parameter identification workload: 30^3 hexa compression, reaction
+    history loss against a target solved at hidden parameters; adjoint
+    gradient (checkpoint every 50 steps) timed and checked against central
+    differences of the GPU forward and of the oracle."""
+    from weldformfem_amd import WFX_FP32_MIXED, WFX_FP64, workloads
+    from weldformfem_amd.adjoint import CompressionProblem, identify
+
+    hidden = workloads.hidden_params(seed=4)        # nominal x (1 + 0.2 U(-1, 1))
+    theta0 = np.array(workloads.NOMINAL_PARAMS)     # identification starts at the nominal values
+    pb = CompressionProblem(cells=cells, nsteps=steps, every=50)
+    target = pb.forward(hidden)
+    E = pb.domain.elem_count
+    pb.forward(theta0)
+    torch.cuda.synchronize()
+    t_fwd = timed(lambda: pb.forward(theta0), torch)
+    r = pb.forward(theta0) - target
+    L = float(np.dot(r, r))
+    t_bwd = timed(lambda: pb.backward(2.0 * r), torch)
+    g = pb.backward(2.0 * r)
+
+    def gpu_loss(t):
+        d = pb.forward(t) - target
+        return float(np.dot(d, d))
+
+    steps_rel = [1e-6 if abs(t * gi) >= 0.1 * L else 1e-4 for t, gi in zip(theta0, g)]
+
+    def cfd(loss):
+        out = np.zeros(3)
+        for i in range(3):
+            h = steps_rel[i] * theta0[i]
+            tp, tm = theta0.copy(), theta0.copy()
+            tp[i] += h
+            tm[i] -= h
+            out[i] = (loss(tp) - loss(tm)) / (2 * h)
+        return out
+
+    fd = cfd(gpu_loss)
+    log(f"adjoint: grad {g}, GPU fd {fd}")
+    out = {
+        "workload": f"{cells}^3 hexa compression ({E} elements), {steps} steps, loss = sum (R - R*)^2 over the "
+                    "top-face reaction history, theta = (sigma_y0, Et, hg_coeff), checkpoint every 50 steps",
+        "forward_record_s": t_fwd, "backward_s": t_bwd,
+        "backward_over_forward": t_bwd / t_fwd,
+        "grad": g.tolist(), "fd_rel_steps": steps_rel,
+        "fd_gpu": fd.tolist(), "rel_err_vs_fd_gpu": (np.abs(g - fd) / np.abs(fd)).tolist(),
+    }
+    if with_oracle:
+        from oracle import wfref
+
+        def oracle_hist(t):
+            ref = wfref.RefDomain()
+            ref.AddBoxLength([0, 0, 0], [1.0, 1.0, 1.0], 0.5 / cells)
+            bottom, top = workloads.box_layers(cells, cells, cells)
+            m = dict(workloads.STEEL, sy0=float(t[0]), Et=float(t[1]))
+            workloads.setup_compression(ref, bottom, top, -1.0, m, hg_coeff=float(t[2]))
+            ref.init()
+            return ref.run(steps)
+
+        t0 = time.perf_counter()
+        tgt = oracle_hist(hidden)
+
+        def oracle_loss(t):
+            d = oracle_hist(t) - tgt
+            return float(np.dot(d, d))
+
+        fdo = cfd(oracle_loss)
+        out["fd_oracle"] = fdo.tolist()
+        out["rel_err_vs_fd_oracle"] = (np.abs(g - fdo) / np.abs(fdo)).tolist()
+        out["oracle_seconds"] = time.perf_counter() - t0
+        log(f"adjoint: oracle fd {fdo}")
+    # FP32-mixed forward with the FP64 adjoint
+    q = CompressionProblem(cells=cells, nsteps=steps, every=50, precision=WFX_FP32_MIXED)
+    L32, g32 = q.loss_and_grad(theta0, target)
+    out["fp32_mixed"] = {"grad": g32.tolist(), "rel_err_vs_fp64": (np.abs(g32 - g) / np.abs(g)).tolist(),
+                         "loss_rel_err": abs(L32 - L) / L}
+    # identification from theta0
+    t0 = time.perf_counter()
+    theta, loss, nit, losses = identify(pb, target, theta0, scale=np.array([300e6, 1e9, 0.06]), max_iter=30)
+    out["identification"] = {"theta0": theta0.tolist(), "hidden": hidden.tolist(), "found": theta.tolist(),
+                             "rel_err": (np.abs(theta - hidden) / hidden).tolist(), "iterations": nit,
+                             "loss_ratio": loss / losses[0], "seconds": time.perf_counter() - t0}
+    log(f"adjoint: identification {out['identification']}")
+    return out
+
+
 def pmc_probe(args):
     """Kernels for hardware-counter runs (rocprofv3 --pmc): a plain 1 GiB
     device-to-device copy whose FETCH_SIZE/WRITE_SIZE calibrate the counters,
@@ -271,6 +399,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-api", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short run for counter collection: a 1 GiB device copy (calibration), "
@@ -365,6 +495,14 @@ def main():
     if not args.no_batch:
         batch = batch_leg(torch, args.batch_steps, log)
 
+    others = None
+    if not args.no_configs:
+        others = config_legs(torch, log)
+
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_leg(torch, log, with_oracle=not (args.no_cpu or args.quick))
+
     cpu = None
     if not args.no_cpu and not args.quick:
         log("CPU baseline (oracle)")
@@ -398,10 +536,19 @@ def main():
                      "measured_copy_GBs": copy_bw / 1e9, "frac_of_measured_copy": achieved * 1e9 / copy_bw},
         "step_algorithmic_bytes_per_element": step_bytes / E,
         "step_hbm_frac": step_bytes / s64 / HBM_PEAK,
+        # the per-step API's compulsory bytes (one pass per Domain_d call) at
+        # the fused path's rate: the fraction a kernel-per-call port would need
+        "b_api_equivalent": {"bytes_per_element_step_fp64": total(api_bytes(n, 8), E, N) / E,
+                             "bytes_per_element_step_mixed": total(api_bytes(n, 4), E, N) / E,
+                             "frac_fp64": E / s64 * total(api_bytes(n, 8), E, N) / E / HBM_PEAK,
+                             "frac_mixed": (E / mixed["ms_per_step"] * 1e3 * total(api_bytes(n, 4), E, N) / E
+                                            / HBM_PEAK) if mixed else None},
         "per_kernel": per_kernel_table(kt64, fb64, E, N),
         "fp32_mixed": mixed,
         "api_path_fp64": api,
+        "other_configs": others,
         "batched_specimens": batch,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -234,3 +234,14 @@ def setup_compression(dom, bottom, top, v_top, mat=STEEL, sy0=None, hg_coeff=0.0
         dt = cfl_dt(dom.calcMinEdgeLength(), mat["E"], mat["nu"], mat["rho"], dom.nodxelem)
     dom.SetDT(dt)
     return dt
+
+
+# parameter identification: nominal (sigma_y0, Et, hg_coeff) and the hidden
+# values the target history is solved at
+NOMINAL_PARAMS = (300.0e6, 1.0e9, 0.06)
+
+
+def hidden_params(seed=4, spread=0.2):
+    """nominal parameters scaled by 1 + spread * U(-1, 1), three draws of seed"""
+    u = uniform(seed, 3)
+    return np.array(NOMINAL_PARAMS) * (1.0 + spread * (2.0 * u - 1.0))
