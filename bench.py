@@ -27,11 +27,13 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 I4 = 4
 
 
-def fused_bytes(n, s, hg=True):
+def fused_bytes(n, s, hg=True, rho_uniform=True):
     """Algorithmic bytes per element and per node of the fused kernels: each
     array a kernel touches is moved once, gathered nodal data once per node.
-    s = element real size (8 FP64, 4 FP32-mixed); nodal arrays are double."""
-    f3_reals = 12 + 12 + 1 + 2 + 1 + 1 + (1 if hg else 0) + 3 * n  # tau, eps rw; p; ep rw; sy0; sigma_y; rho; f
+    s = element real size (8 FP64, 4 FP32-mixed); nodal arrays are double.
+    With a uniform density (the benchmark's) no density is read."""
+    rho = 1 if (hg and not rho_uniform) else 0
+    f3_reals = 12 + 12 + 1 + 2 + 1 + 1 + rho + 3 * n  # tau, eps rw; p; ep rw; sy0; sigma_y; [rho]; f
     sn = s  # v, prev_a, u_dt, fi between steps: double (FP64) or float (FP32-mixed)
     u_rw = 2 * 3 * 8 if s == 8 else 0  # FP32-mixed recovers u from x at the end of a run
     return {
@@ -428,6 +430,8 @@ def main():
     log(f"building {nx}x{ny}x{nz} hexa FP64")
     d64, dt = build_domain(WFX_FP64, nx, ny, nz, L, -2.0, stream)
     N, E, n = d64.sizes()
+    bs64 = d64.autotune(reps=5)
+    log(f"FP64 block sizes (autotuned) {bs64}")
     steps64 = 0
     d64.runFused(args.warmup, reaction=False)
     steps64 += args.warmup
@@ -448,6 +452,8 @@ def main():
     if not args.no_mixed:
         log("building FP32-mixed")
         d32, _ = build_domain(WFX_FP32_MIXED, nx, ny, nz, L, -2.0, stream)
+        bs32 = d32.autotune(reps=5)
+        log(f"FP32-mixed block sizes (autotuned) {bs32}")
         d32.runFused(args.warmup, reaction=False)
         s32 = timed(lambda: d32.runFused(args.steps, reaction=False), torch) / args.steps
         steps32 = args.warmup + args.steps
@@ -466,6 +472,7 @@ def main():
             "value": E / s32,
             "ms_per_step": s32 * 1e3,
             "speedup_vs_fp64": s64 / s32,
+            "block_sizes": bs32,
             "dtype": "f32 element arithmetic, f64 nodal kinematics",
             "rel_err_vs_fp64": {"displacement": rel(d32.field("u"), d64.field("u")),
                                 "reaction_force": abs(r32 - r64) / abs(r64),
@@ -529,7 +536,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"hexa compression block {nx}x{ny}x{nz} ({E} elements, {N} nodes), "
                                "fused four-kernel FP64 step",
-                   "elements": E, "nodes": N, "dt": dt, "parallelism": "single GPU"},
+                   "elements": E, "nodes": N, "dt": dt, "parallelism": "single GPU",
+                   "block_sizes": bs64},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic,
                      "algorithmic_bytes": dom_bytes, "avg_launch_us": kt64[dom] * 1e6,

@@ -182,6 +182,19 @@ int wfx_energies(wfx_ctx *ctx, double *ekin, double *deint, wfx_stream s);
 /* number of non-finite internal-force entries scrubbed so far */
 int wfx_nonfinite_count(wfx_ctx *ctx, long *out);
 
+/* ---- file formats (host side) -------------------------------------------- */
+/* LS-DYNA keyword mesh: *NODE and *ELEMENT_SOLID (CreateFromLSDyna,
+ * Domain_d.C:1647-1699; its reader is an un-vendored submodule, so this
+ * parser is this build's own).  Node ids map to indices in their order under
+ * *NODE; solids whose nodes 4..8 repeat node 4 are tetrahedra.  The parse
+ * call fills sizes first (x, elnod NULL), then x[3N] and elnod[E*n]. */
+int wfx_parse_lsdyna_k(const char *path, int *node_count, int *elem_count, int *nodxelem, double *x, int *elnod);
+int wfx_read_lsdyna_k(wfx_ctx *ctx, const char *path);
+/* Legacy VTK frame of the current state (VTKWriter.C:365-625 field names:
+ * DISP, Velocity, Acceleration, nod_mass, pressure, pl_strain, Vol; plus
+ * sigma_y and the cell stress tensor).  Synchronous. */
+int wfx_write_vtk(wfx_ctx *ctx, const char *path, wfx_stream s);
+
 /* ---- state transfer (reference layout on the host side, synchronous) ----- */
 /* names: x v a u u_dt prev_a fi fe mdiag voln elnod nodel nodel_loc
  * nodel_offset nodel_count detJ dHdx dHdy dHdz vol vol0 rho rho0 p pl_strain

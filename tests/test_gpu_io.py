@@ -1,0 +1,74 @@
+"""Mesh in from an LS-DYNA keyword file, frames out as legacy VTK (GPU)."""
+import numpy as np
+import pytest
+
+from oracle.wfref import RefDomain
+from weldformfem_amd import Domain, WFX_FP64, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def write_k(path, x, el):
+    with open(path, "w") as f:
+        f.write("*KEYWORD\n*NODE\n")
+        for i, p in enumerate(x):
+            f.write(f"{i + 1:8d}{p[0]:16.9e}{p[1]:16.9e}{p[2]:16.9e}       0       0\n")
+        f.write("*ELEMENT_SOLID\n")
+        for e, nd in enumerate(el):
+            f.write(f"{e + 1:8d}       1" + "".join(f"{q + 1:8d}" for q in nd) + "\n")
+        f.write("*END\n")
+
+
+def read_vtk(path):
+    """sections of a legacy ASCII VTK file as float arrays"""
+    toks = open(path).read().split("\n")
+    out, k = {}, 0
+    while k < len(toks):
+        t = toks[k].split()
+        if t and t[0] in ("POINTS", "VECTORS", "SCALARS", "TENSORS", "CELLS", "CELL_TYPES"):
+            name = t[0] if t[0] in ("POINTS", "CELLS", "CELL_TYPES") else t[1]
+            k += 1
+            if t[0] == "SCALARS":
+                k += 1  # LOOKUP_TABLE
+            vals = []
+            while k < len(toks) and toks[k] and (toks[k][0].isdigit() or toks[k][0] in "-+."):
+                vals.extend(float(v) for v in toks[k].split())
+                k += 1
+            out[name] = np.array(vals)
+        else:
+            k += 1
+    return out
+
+
+def test_keyword_mesh_run_and_vtk_frame(tmp_path):
+    print("case: 4x3x2 hexa block through a .k file, 30 steps, VTK frame read back", flush=True)
+    ref = RefDomain()
+    ref.AddBoxLength([0, 0, 0], [0.4, 0.3, 0.2], 0.05)
+    x, el = ref.field("x").reshape(-1, 3), ref.field("elnod").reshape(-1, 8)
+    kpath = tmp_path / "block.k"
+    write_k(kpath, x, el)
+    g = Domain(WFX_FP64)
+    g.readLSDyna(kpath)
+    np.testing.assert_array_equal(g.field("elnod"), el.reshape(-1))
+    np.testing.assert_allclose(g.field("x"), x.reshape(-1), rtol=1e-9, atol=1e-12)
+    bottom, top = workloads.box_layers(4, 3, 2)
+    workloads.setup_compression(g, bottom, top, -5.0, hg_coeff=0.06)
+    g.init()
+    g.runFused(30)
+    vpath = tmp_path / "frame.vtk"
+    g.writeVTK(vpath)
+    v = read_vtk(vpath)
+    N, E = g.node_count, g.elem_count
+    np.testing.assert_allclose(v["POINTS"], g.field("x"), rtol=1e-8, atol=1e-14)
+    np.testing.assert_allclose(v["DISP"], g.field("u"), rtol=1e-8, atol=1e-14)
+    np.testing.assert_allclose(v["Velocity"], g.field("v"), rtol=1e-8, atol=1e-14)
+    np.testing.assert_allclose(v["pl_strain"], g.field("pl_strain"), rtol=1e-8, atol=1e-14)
+    np.testing.assert_allclose(v["pressure"], g.field("p"), rtol=1e-8, atol=1e-3)
+    assert v["CELL_TYPES"].size == E and np.all(v["CELL_TYPES"] == 12)
+    cells = v["CELLS"].reshape(E, 9)
+    np.testing.assert_array_equal(cells[:, 1:].astype(int), el)
+    sig = v["SIGMA"].reshape(E, 3, 3)
+    s6 = g.field("sigma").reshape(E, 6)
+    np.testing.assert_allclose(sig[:, 0, 1], s6[:, 3], rtol=1e-8, atol=1e-3)
+    np.testing.assert_allclose(sig[:, 1, 0], s6[:, 3], rtol=1e-8, atol=1e-3)
+    assert v["DISP"].size == 3 * N

@@ -48,9 +48,10 @@ def build(verbose=False, jobs=None):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         cmds.append([hipcc] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    obj = os.path.join(BUILD, "wfx_mesh.o")
-    cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, "wfx_mesh.cpp"), "-o", obj])
-    objs.append(obj)
+    for src in ("wfx_mesh.cpp", "wfx_io.cpp"):
+        obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
+        cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     # the units are independent: compile them concurrently
     jobs = jobs or min(len(cmds), os.cpu_count() or 1, 8)
     pending, running = list(cmds), []

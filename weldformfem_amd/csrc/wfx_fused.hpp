@@ -200,6 +200,7 @@ struct FusedParams {
   PressParams pp;     // pp.full handled as in k_pressure_elem
   int need_jlocal;    // alpha_free or hg_coeff_free nonzero: J_local needed
   double hg_coeff, cs0;
+  double rho_u;       // > 0: every element density equals it (no density loads)
 };
 
 // dH(d,a) = adj(J)(d,:) . dN_a/dxi from the cofactor matrix C (adj = C^T),
@@ -336,7 +337,8 @@ __global__ __launch_bounds__(BS) void f3_element(
         for (int a = 0; a < 8; ++a)
 #pragma unroll
           for (int c = 0; c < 3; ++c) hm[c][j] = hm[c][j] + V[a][c] * (T)kHgGamma[j][a];
-      c_h = (T)(fp.hg_coeff * pow((double)volv, 0.6666666) * (double)rho[e] * 0.25 * fp.cs0);
+      const double rho_e = fp.rho_u > 0.0 ? fp.rho_u : (double)rho[e];
+      c_h = (T)(fp.hg_coeff * pow((double)volv, 0.6666666) * rho_e * 0.25 * fp.cs0);
     }
   }
   // Park the hourglass amplitudes and the cofactor matrix in LDS (one slot
@@ -673,6 +675,7 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, hipStream_t
     fp.need_jlocal = (st.alpha_free != 0.0 || st.hg_coeff_free != 0.0);
     fp.hg_coeff = (c->n == 8) ? st.hg_coeff : 0.0;
     fp.cs0 = P.mat.cs0;
+    fp.rho_u = c->rho_uniform;
     if (c->n == 8)
       hipLaunchKernelGGL((f3_element<T, 8, BS>), dim3(gE), dim3(BS), 0, s, E, N, fp, c->ms.elnod, c->nd.x,
                          vN, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.He, el.tau, el.eps, el.p,

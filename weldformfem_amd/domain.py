@@ -54,6 +54,14 @@ class Domain:
         check(self._L.wfx_set_mesh(self._h, x.size // 3, el.size // nodxelem, int(nodxelem),
                                    x.ctypes.data, el.ctypes.data))
 
+    def readLSDyna(self, path):
+        """mesh from an LS-DYNA keyword file (*NODE, *ELEMENT_SOLID)"""
+        check(self._L.wfx_read_lsdyna_k(self._h, str(path).encode()))
+
+    def writeVTK(self, path):
+        """legacy VTK frame of the current state"""
+        check(self._L.wfx_write_vtk(self._h, str(path).encode(), self.stream))
+
     def sizes(self):
         N, E, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(self._L.wfx_get_sizes(self._h, ctypes.byref(N), ctypes.byref(E), ctypes.byref(n)))
@@ -265,3 +273,15 @@ class Domain:
     def setField(self, name, values):
         a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
         check(self._L.wfx_set_field(self._h, name.encode(), a.ctypes.data, a.size))
+
+
+def parse_lsdyna_k(path):
+    """(x[N,3], elnod[E,n], n) of an LS-DYNA keyword mesh, host only"""
+    L = _lib.load()
+    N, E, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(L.wfx_parse_lsdyna_k(str(path).encode(), ctypes.byref(N), ctypes.byref(E), ctypes.byref(n), None, None))
+    x = np.zeros((N.value, 3))
+    el = np.zeros((E.value, n.value), dtype=np.int32)
+    check(L.wfx_parse_lsdyna_k(str(path).encode(), ctypes.byref(N), ctypes.byref(E), ctypes.byref(n),
+                               x.ctypes.data, el.ctypes.data))
+    return x, el, n.value
