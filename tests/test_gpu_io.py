@@ -43,7 +43,7 @@ def read_vtk(path):
 def test_keyword_mesh_run_and_vtk_frame(tmp_path):
     print("case: 4x3x2 hexa block through a .k file, 30 steps, VTK frame read back", flush=True)
     ref = RefDomain()
-    ref.AddBoxLength([0, 0, 0], [0.4, 0.3, 0.2], 0.05)
+    ref.AddBoxLength([0, 0, 0], [1.0, 0.75, 0.5], 0.125)  # 4 x 3 x 2 cubes of edge 0.25
     x, el = ref.field("x").reshape(-1, 3), ref.field("elnod").reshape(-1, 8)
     kpath = tmp_path / "block.k"
     write_k(kpath, x, el)
@@ -52,6 +52,7 @@ def test_keyword_mesh_run_and_vtk_frame(tmp_path):
     np.testing.assert_array_equal(g.field("elnod"), el.reshape(-1))
     np.testing.assert_allclose(g.field("x"), x.reshape(-1), rtol=1e-9, atol=1e-12)
     bottom, top = workloads.box_layers(4, 3, 2)
+    assert g.sizes() == (5 * 4 * 3, 24, 8)
     workloads.setup_compression(g, bottom, top, -5.0, hg_coeff=0.06)
     g.init()
     g.runFused(30)

@@ -89,7 +89,8 @@ static int setup_device(wfx_ctx *c) {
   A(c->nd.prev_a, 3 * N) A(c->nd.fi, 3 * N) A(c->nd.fe, 3 * N) A(c->nd.mdiag, N) A(c->nd.voln, N)
   A(c->nd.voln_sum, N) A(c->nd.voln0_sum, N) A(c->nd.anp_ratio, N)
   A(c->ms.elnod, n * E) A(c->ms.nodel, n * E) A(c->ms.nodel_loc, n * E) A(c->ms.nodel_offset, N)
-  A(c->ms.nodel_count, N) A(c->ms.nodel_pk, n * E) A(c->bc_slot, N) A(c->scalars, 64) A(c->status, 16)
+  A(c->ms.nodel_count, N) A(c->ms.nodel_pk, n * E) A(c->ms.nodel_pad, 8 * N) A(c->bc_slot, N)
+  A(c->scalars, 64) A(c->status, 16)
   if (e == hipSuccess) e = (c->precision == WFX_FP64) ? alloc_elem(c, c->e64) : alloc_elem(c, c->e32);
   if (c->precision == WFX_FP32_MIXED) {
     A(c->v32, 3 * N) A(c->pa32, 3 * N) A(c->ud32, 3 * N) A(c->fi32, 3 * N) A(c->x_start, 3 * N)
@@ -116,6 +117,12 @@ static int setup_device(wfx_ctx *c) {
   std::vector<int> pk(n * E);
   for (size_t k = 0; k < n * E; ++k) pk[k] = (c->nodel_h[k] << 3) | c->nodel_loc_h[k];
   WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_pk, pk.data(), n * E * sizeof(int), hipMemcpyHostToDevice));
+  // entry j of node i at j*N + i: the node passes load their first eight
+  // entries with unit stride across a wavefront and need no offset for them
+  std::vector<int> pad(8 * N, -1);
+  for (size_t i = 0; i < N; ++i)
+    for (int j = 0; j < std::min(8, c->nodel_count_h[i]); ++j) pad[j * N + i] = pk[c->nodel_offset_h[i] + j];
+  WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_pad, pad.data(), 8 * N * sizeof(int), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemset(c->bc_slot, 0xff, N * sizeof(int)));  // -1: no boundary condition
   c->has_mesh = true;
   c->initialized = false;

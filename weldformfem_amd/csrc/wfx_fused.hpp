@@ -38,7 +38,7 @@ namespace wfx {
 namespace {
 
 // CSR entries fetched per batch of independent loads in the node passes
-constexpr int kChunk = 8;
+constexpr int kChunk = 8;  // also the width of the padded first-entries table
 // element pass F3: per-thread LDS slots for values parked across the stress
 // update (12 hourglass amplitudes + 9 cofactors)
 #ifndef WFX_F3_PARK
@@ -139,6 +139,7 @@ template <typename T, int BS>
 __global__ __launch_bounds__(BS) void f2_nodal(int N, int nodxelem, const int *__restrict__ off,
                                                    const int *__restrict__ cnt,
                                                    const int *__restrict__ nodel_pk,
+                                                   const int *__restrict__ pad,
                                                    const T *__restrict__ vol,
                                                    const T *__restrict__ rho,
                                                    const double *__restrict__ vsum0,
@@ -147,9 +148,10 @@ __global__ __launch_bounds__(BS) void f2_nodal(int N, int nodxelem, const int *_
                                                    double *__restrict__ mdiag, double rho_u) {
   int i = gtid();
   if (i >= N) return;
-  int o = off[i], c = cnt[i];
+  const int c = cnt[i], o = c > kChunk ? off[i] : 0;
   // CSR entries are taken eight at a time with independent loads, then
-  // summed in CSR order (the order of the reference's serial loop).
+  // summed in CSR order (the order of the reference's serial loop); the
+  // first eight come from the padded [8][N] table.
   // The densities of the first chunk are loaded together with the volumes and
   // kept in registers (every hexa node and most tet nodes have <= 8
   // elements); with a uniform density (rho_u > 0) no density is loaded.
@@ -159,7 +161,8 @@ __global__ __launch_bounds__(BS) void f2_nodal(int N, int nodxelem, const int *_
     int e[kChunk];
     double vv[kChunk];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) e[j] = (k + j < c) ? (nodel_pk[o + k + j] >> 3) : -1;
+    for (int j = 0; j < kChunk; ++j)
+      e[j] = (k == 0) ? (pad[(long)j * N + i] >> 3) : ((k + j < c) ? (nodel_pk[o + k + j] >> 3) : -1);
 #pragma unroll
     for (int j = 0; j < kChunk; ++j) vv[j] = (e[j] >= 0) ? (double)vol[e[j]] : 0.0;
     if (k == 0 && rho_u <= 0.0) {
@@ -510,7 +513,8 @@ struct NodeParams {
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void f4_node(
     int N, int E, NodeParams np, const int *__restrict__ off, const int *__restrict__ cnt,
-    const int *__restrict__ nodel_pk, const T *__restrict__ fel, const double *__restrict__ fe,
+    const int *__restrict__ nodel_pk, const int *__restrict__ pad, const T *__restrict__ fel,
+    const double *__restrict__ fe,
     const double *__restrict__ mdiag, const int *__restrict__ bc_slot,
     const int *__restrict__ bc_mask, const double *__restrict__ bc_val, T *__restrict__ fi,
     T *__restrict__ v, T *__restrict__ prev_a, T *__restrict__ u_dt,
@@ -522,13 +526,14 @@ __global__ __launch_bounds__(BS) void f4_node(
   int i = gtid();
   if (i >= N) return;
   const long Es = E, Ns = N;
-  int o = off[i], c = cnt[i];
+  const int c = cnt[i], o = c > kChunk ? off[i] : 0;
   double f[3] = {0.0, 0.0, 0.0};
   for (int k = 0; k < c; k += kChunk) {
     int pk[kChunk];
     double q[kChunk][3];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) pk[j] = (k + j < c) ? nodel_pk[o + k + j] : -1;
+    for (int j = 0; j < kChunk; ++j)
+      pk[j] = (k == 0) ? pad[(long)j * Ns + i] : ((k + j < c) ? nodel_pk[o + k + j] : -1);
 #pragma unroll
     for (int j = 0; j < kChunk; ++j) {
       long e = pk[j] >> 3, b = (long)(pk[j] & 7) * 3;
@@ -662,7 +667,8 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, hipStream_t
       hipLaunchKernelGGL((f1_volume<T, 4, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol);
   } else if (phase == 2) {
     hipLaunchKernelGGL((f2_nodal<T, BS>), dim3(gN), dim3(BS), 0, s, N, c->n, c->ms.nodel_offset,
-                       c->ms.nodel_count, c->ms.nodel_pk, el.vol, el.rho, c->nd.voln0_sum, c->nd.anp_ratio,
+                       c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.vol, el.rho, c->nd.voln0_sum,
+                       c->nd.anp_ratio,
                        c->nd.voln, c->nd.mdiag, c->rho_uniform);
   } else if (phase == 3) {
     FusedParams fp;
@@ -699,7 +705,8 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, hipStream_t
     np.has_fe = c->has_fe;
     np.nrow = 3 * c->n;
     hipLaunchKernelGGL((f4_node<T, BS>), dim3(gN), dim3(BS), 0, s, N, E, np, c->ms.nodel_offset,
-                       c->ms.nodel_count, c->ms.nodel_pk, el.f_elem, c->nd.fe, c->nd.mdiag, c->bc_slot,
+                       c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.f_elem, c->nd.fe, c->nd.mdiag,
+                       c->bc_slot,
                        c->bc_mask, c->bc_tab, fiN, vN, paN, udN, c->nd.x, c->nd.u, c->status);
   } else if (phase == 5) {
     hipLaunchKernelGGL((f_sigma<T>), dim3(nblocks(E)), dim3(kBlock), 0, s, E, el.p, el.tau, el.sigma);

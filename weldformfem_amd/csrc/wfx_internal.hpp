@@ -72,6 +72,7 @@ struct DevMesh {
   int *elnod = nullptr;
   int *nodel = nullptr, *nodel_loc = nullptr, *nodel_offset = nullptr, *nodel_count = nullptr;
   int *nodel_pk = nullptr;  // (element << 3) | local node, CSR order
+  int *nodel_pad = nullptr; // the first 8 packed entries of every node, [8][N], -1 past its count
 };
 
 // Kernel launch parameters shared by the precision variants.
