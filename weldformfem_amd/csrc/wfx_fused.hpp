@@ -54,6 +54,9 @@ constexpr int kParkSlots = 21;
 #define WFX_F_TILED 1
 #endif
 // Non-temporal stores for the element streams written once per step.
+#ifndef WFX_F4_HOIST
+#define WFX_F4_HOIST 0
+#endif
 #ifndef WFX_NT_STORES
 #define WFX_NT_STORES 1
 #endif
@@ -527,6 +530,26 @@ __global__ __launch_bounds__(BS) void f4_node(
   if (i >= N) return;
   const long Es = E, Ns = N;
   const int c = cnt[i], o = c > kChunk ? off[i] : 0;
+  // The node's own state does not depend on the force gather: with
+  // WFX_F4_HOIST its loads are issued before the gather so that their
+  // latency overlaps it.
+  double h_pa[3], h_v[3], h_ud[3], h_x[3], h_u[3];
+  int h_slot = 0, h_mask = 0;
+  double h_m = 0.0;
+  if constexpr (WFX_F4_HOIST) {
+    h_slot = bc_slot[i];
+    h_m = mdiag[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const long id = d * Ns + i;
+      h_pa[d] = (double)prev_a[id];
+      h_v[d] = (double)v[id];
+      h_ud[d] = (double)u_dt[id];
+      h_x[d] = x[id];
+      if constexpr (std::is_same<T, double>::value) h_u[d] = u[id];
+    }
+    h_mask = h_slot >= 0 ? bc_mask[h_slot] : 0;
+  }
   double f[3] = {0.0, 0.0, 0.0};
   for (int k = 0; k < c; k += kChunk) {
     int pk[kChunk];
@@ -552,10 +575,12 @@ __global__ __launch_bounds__(BS) void f4_node(
 #pragma unroll
   for (int d = 0; d < 3; ++d)
     if (!isfinite(f[d])) { f[d] = 0.0; ++bad; }
-  if (bad) atomicAdd(status, bad);
-  int slot = bc_slot[i];
-  int mask = slot >= 0 ? bc_mask[slot] : 0;
-  double m = mdiag[i];
+  if constexpr (!WFX_F4_HOIST) {
+    if (bad) atomicAdd(status, bad);
+  }
+  const int slot = WFX_F4_HOIST ? h_slot : bc_slot[i];
+  const int mask = WFX_F4_HOIST ? h_mask : (slot >= 0 ? bc_mask[slot] : 0);
+  const double m = WFX_F4_HOIST ? h_m : mdiag[i];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const long id = d * Ns + i;
@@ -563,13 +588,13 @@ __global__ __launch_bounds__(BS) void f4_node(
     double fed = np.has_fe ? fe[id] : 0.0;
     double a = (fed - f[d]) / m;                       // calcAccel
     if (mask & (1 << d)) a = 0.0;                       // ImposeBCA
-    double pa = (double)prev_a[id];
+    double pa = WFX_F4_HOIST ? h_pa[d] : (double)prev_a[id];
     a = np.f * (a - np.alpha * pa);                     // UpdateCorrectionAccVel
-    double vv = (double)v[id] + np.gdt * a;
+    double vv = (WFX_F4_HOIST ? h_v[d] : (double)v[id]) + np.gdt * a;
     if (mask & (1 << d)) vv = bc_val[3L * slot + d];    // ImposeBCV
-    double ud = (double)u_dt[id] + np.bdt2 * a;         // UpdateCorrectionPos
-    x[id] = x[id] + ud;
-    if constexpr (std::is_same<T, double>::value) u[id] = u[id] + ud;
+    double ud = (WFX_F4_HOIST ? h_ud[d] : (double)u_dt[id]) + np.bdt2 * a;  // UpdateCorrectionPos
+    x[id] = (WFX_F4_HOIST ? h_x[d] : x[id]) + ud;
+    if constexpr (std::is_same<T, double>::value) u[id] = (WFX_F4_HOIST ? h_u[d] : u[id]) + ud;
     if (np.predict) {                                   // UpdatePrediction
       ud = np.dt * (vv + np.pred_u * a);
       vv = vv + np.pred_v * a;
@@ -578,6 +603,9 @@ __global__ __launch_bounds__(BS) void f4_node(
     u_dt[id] = (T)ud;
     prev_a[id] = (T)a;
     v[id] = (T)vv;
+  }
+  if constexpr (WFX_F4_HOIST) {
+    if (bad) atomicAdd(status, bad);
   }
 }
 
