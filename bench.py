@@ -403,6 +403,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--bs64", default="", help="fused block sizes for FP64 (f1,f2,f3,f4) instead of autotuning")
+    ap.add_argument("--bs32", default="", help="the same for FP32-mixed")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short run for counter collection: a 1 GiB device copy (calibration), "
@@ -430,8 +432,12 @@ def main():
     log(f"building {nx}x{ny}x{nz} hexa FP64")
     d64, dt = build_domain(WFX_FP64, nx, ny, nz, L, -2.0, stream)
     N, E, n = d64.sizes()
-    bs64 = d64.autotune(reps=5)
-    log(f"FP64 block sizes (autotuned) {bs64}")
+    if args.bs64:
+        bs64 = [int(q) for q in args.bs64.split(",")]
+        d64.setBlockSizes(bs64)
+    else:
+        bs64 = d64.autotune(reps=5)
+    log(f"FP64 block sizes {'(given)' if args.bs64 else '(autotuned)'} {bs64}")
     steps64 = 0
     d64.runFused(args.warmup, reaction=False)
     steps64 += args.warmup
@@ -452,8 +458,12 @@ def main():
     if not args.no_mixed:
         log("building FP32-mixed")
         d32, _ = build_domain(WFX_FP32_MIXED, nx, ny, nz, L, -2.0, stream)
-        bs32 = d32.autotune(reps=5)
-        log(f"FP32-mixed block sizes (autotuned) {bs32}")
+        if args.bs32:
+            bs32 = [int(q) for q in args.bs32.split(",")]
+            d32.setBlockSizes(bs32)
+        else:
+            bs32 = d32.autotune(reps=5)
+        log(f"FP32-mixed block sizes {'(given)' if args.bs32 else '(autotuned)'} {bs32}")
         d32.runFused(args.warmup, reaction=False)
         s32 = timed(lambda: d32.runFused(args.steps, reaction=False), torch) / args.steps
         steps32 = args.warmup + args.steps
