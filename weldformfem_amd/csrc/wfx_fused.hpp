@@ -39,14 +39,6 @@ namespace {
 
 // CSR entries fetched per batch of independent loads in the node passes
 constexpr int kChunk = 8;  // also the width of the padded first-entries table
-// element pass F3: per-thread LDS slots for values parked across the stress
-// update (12 hourglass amplitudes + 9 cofactors)
-#ifndef WFX_F3_PARK
-#define WFX_F3_PARK 0
-#endif
-constexpr bool kParkInLds = WFX_F3_PARK;
-constexpr int kParkSlots = 21;
-
 // Layout of the fused path's element-force buffer (written by F3, gathered by
 // F4, private to the fused path): row-major [3n][E], or tiled in blocks of 64
 // elements [E/64][3n][64] so that a wavefront's force rows are contiguous.
@@ -54,9 +46,6 @@ constexpr int kParkSlots = 21;
 #define WFX_F_TILED 1
 #endif
 // Non-temporal stores for the element streams written once per step.
-#ifndef WFX_F4_HOIST
-#define WFX_F4_HOIST 0
-#endif
 #ifndef WFX_NT_STORES
 #define WFX_NT_STORES 1
 #endif
@@ -249,16 +238,15 @@ __device__ __forceinline__ T shape_dH(const T (&C)[3][3], int d, int a) {
   }
 }
 
-// F3: the element chain of one step
-template <typename T, int NN, int BS>
-__global__ __launch_bounds__(BS) void f3_element(
-    int E, int N, FusedParams fp, const int *__restrict__ elnod, const double *__restrict__ x,
+// F3: the element chain of one step for element e.  The nodal forces of the
+// element (f - f_hg, what the assembly adds to each node) go to sink(a, fx, fy, fz).
+template <typename T, int NN, class Sink>
+__device__ __forceinline__ void element_chain(
+    int e, int E, int N, const FusedParams &fp, const int *__restrict__ elnod, const double *__restrict__ x,
     const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
     const T *__restrict__ rho, const T *__restrict__ sy0, const T *__restrict__ He,
     T *__restrict__ tau_io, T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
-    T *__restrict__ sigma_y, T *__restrict__ fout) {
-  int e = gtid();
-  if (e >= E) return;
+    T *__restrict__ sigma_y, Sink &&sink) {
   const long Es = E;
   int g[NN];
 #pragma unroll
@@ -359,23 +347,6 @@ __global__ __launch_bounds__(BS) void f3_element(
       c_h = (T)(fp.hg_coeff * pow((double)volv, 0.6666666) * rho_e * 0.25 * fp.cs0);
     }
   }
-  // Park the hourglass amplitudes and the cofactor matrix in LDS (one slot
-  // per thread, conflict-free) while the stress update runs: they are only
-  // needed again for the forces, and keeping them in VGPRs halves occupancy.
-  __shared__ T park[kParkInLds ? kParkSlots : 1][BS];
-  const int tx = threadIdx.x;
-  if constexpr (kParkInLds) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) park[c * 4 + j][tx] = (NN == 8) ? hm[c][j] : T(0);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) park[12 + i * 3 + j][tx] = C[i][j];
-    asm volatile("" ::: "memory");
-  }
-
   // ---- ANP pressure (calcElemPressure element part) ----------------------
   const PressParams &pp = fp.pp;
   double J_avg = 0.0;
@@ -467,17 +438,6 @@ __global__ __launch_bounds__(BS) void f3_element(
   st_stream(&tau_io[4 * Es + e], tau.yz); st_stream(&tau_io[5 * Es + e], tau.xz);
 
   // ---- internal forces (calcElemForces) and hourglass ---------------------
-  if constexpr (kParkInLds) {
-    asm volatile("" ::: "memory");
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) hm[c][j] = park[c * 4 + j][tx];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) C[i][j] = park[12 + i * 3 + j][tx];
-  }
   T sg0 = sig.xx, sg1 = sig.yy, sg2 = sig.zz, sg3 = sig.xy, sg4 = sig.yz, sg5 = sig.xz;
 #pragma unroll
   for (int a = 0; a < NN; ++a) {
@@ -507,10 +467,26 @@ __global__ __launch_bounds__(BS) void f3_element(
         f2 = f2 - q[2];
       }
     }
-    st_stream(&fout[fidx(a * 3 + 0, e, Es, 3 * NN)], f0);
-    st_stream(&fout[fidx(a * 3 + 1, e, Es, 3 * NN)], f1);
-    st_stream(&fout[fidx(a * 3 + 2, e, Es, 3 * NN)], f2);
+    sink(a, f0, f1, f2);
   }
+}
+
+template <typename T, int NN, int BS>
+__global__ __launch_bounds__(BS) void f3_element(
+    int E, int N, FusedParams fp, const int *__restrict__ elnod, const double *__restrict__ x,
+    const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
+    const T *__restrict__ rho, const T *__restrict__ sy0, const T *__restrict__ He,
+    T *__restrict__ tau_io, T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
+    T *__restrict__ sigma_y, T *__restrict__ fout) {
+  int e = gtid();
+  if (e >= E) return;
+  const long Es = E;
+  element_chain<T, NN>(e, E, N, fp, elnod, x, v, ratio, vol0, rho, sy0, He, tau_io, eps_io, p_out, pl_strain,
+                       sigma_y, [&](int a, T f0, T f1, T f2) {
+                         st_stream(&fout[fidx(a * 3 + 0, e, Es, 3 * NN)], f0);
+                         st_stream(&fout[fidx(a * 3 + 1, e, Es, 3 * NN)], f1);
+                         st_stream(&fout[fidx(a * 3 + 2, e, Es, 3 * NN)], f2);
+                       });
 }
 
 struct NodeParams {
@@ -525,45 +501,63 @@ struct NodeParams {
   int nrow;        // rows of the element-force buffer (3n)
 };
 
-// F4: assembly and the whole nodal update of one step
-template <typename T, int BS>
-__global__ __launch_bounds__(BS) void f4_node(
-    int N, int E, NodeParams np, const int *__restrict__ off, const int *__restrict__ cnt,
-    const int *__restrict__ nodel_pk, const int *__restrict__ pad, const T *__restrict__ fel,
-    const double *__restrict__ fe,
-    const double *__restrict__ mdiag, const int *__restrict__ bc_slot,
-    const int *__restrict__ bc_mask, const double *__restrict__ bc_val, T *__restrict__ fi,
-    T *__restrict__ v, T *__restrict__ prev_a, T *__restrict__ u_dt,
-    double *__restrict__ x, double *__restrict__ u, unsigned int *__restrict__ status) {
-  // FP64: every nodal array is double.  FP32-mixed: velocity, previous
-  // acceleration, increment and internal force are stored in float (the
-  // update itself runs in double), positions stay double, and u is not
-  // touched per step (recovered as x - x_start when the run ends).
-  int i = gtid();
-  if (i >= N) return;
-  const long Es = E, Ns = N;
-  const int c = cnt[i], o = c > kChunk ? off[i] : 0;
-  // The node's own state does not depend on the force gather: with
-  // WFX_F4_HOIST its loads are issued before the gather so that their
-  // latency overlaps it.
-  double h_pa[3], h_v[3], h_ud[3], h_x[3], h_u[3];
-  int h_slot = 0, h_mask = 0;
-  double h_m = 0.0;
-  if constexpr (WFX_F4_HOIST) {
-    h_slot = bc_slot[i];
-    h_m = mdiag[i];
+// The nodal update of one step for node i given its assembled internal
+// force f: non-finite scrub (Solver_explicit.C:777-785), calcAccel, ImposeBCA,
+// UpdateCorrectionAccVel, ImposeBCV, UpdateCorrectionPos and the next step's
+// UpdatePrediction + ImposeBCV.  FP64: every nodal array is double.
+// FP32-mixed: velocity, previous acceleration, increment and internal force
+// are stored in float (the update itself runs in double), positions stay
+// double, and u is not touched per step (recovered as x - x_start when the
+// run ends).
+template <typename T>
+__device__ __forceinline__ void node_update(
+    int i, long Ns, const NodeParams &np, double (&f)[3], const double *__restrict__ fe,
+    const double *__restrict__ mdiag, const int *__restrict__ bc_slot, const int *__restrict__ bc_mask,
+    const double *__restrict__ bc_val, T *__restrict__ fi, T *__restrict__ v, T *__restrict__ prev_a,
+    T *__restrict__ u_dt, double *__restrict__ x, double *__restrict__ u, unsigned int *__restrict__ status) {
+  unsigned bad = 0;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const long id = d * Ns + i;
-      h_pa[d] = (double)prev_a[id];
-      h_v[d] = (double)v[id];
-      h_ud[d] = (double)u_dt[id];
-      h_x[d] = x[id];
-      if constexpr (std::is_same<T, double>::value) h_u[d] = u[id];
+  for (int d = 0; d < 3; ++d)
+    if (!isfinite(f[d])) { f[d] = 0.0; ++bad; }
+  if (bad) atomicAdd(status, bad);
+  const int slot = bc_slot[i];
+  const int mask = slot >= 0 ? bc_mask[slot] : 0;
+  const double m = mdiag[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const long id = d * Ns + i;
+    fi[id] = (T)f[d];
+    double fed = np.has_fe ? fe[id] : 0.0;
+    double a = (fed - f[d]) / m;                       // calcAccel
+    if (mask & (1 << d)) a = 0.0;                       // ImposeBCA
+    double pa = (double)prev_a[id];
+    a = np.f * (a - np.alpha * pa);                     // UpdateCorrectionAccVel
+    double vv = (double)v[id] + np.gdt * a;
+    if (mask & (1 << d)) vv = bc_val[3L * slot + d];    // ImposeBCV
+    double ud = (double)u_dt[id] + np.bdt2 * a;         // UpdateCorrectionPos
+    x[id] = x[id] + ud;
+    if constexpr (std::is_same<T, double>::value) u[id] = u[id] + ud;
+    if (np.predict) {                                   // UpdatePrediction
+      ud = np.dt * (vv + np.pred_u * a);
+      vv = vv + np.pred_v * a;
+      if (mask & (1 << d)) vv = bc_val[3L * slot + d];  // ImposeBCV
     }
-    h_mask = h_slot >= 0 ? bc_mask[h_slot] : 0;
+    u_dt[id] = (T)ud;
+    prev_a[id] = (T)a;
+    v[id] = (T)vv;
   }
-  double f[3] = {0.0, 0.0, 0.0};
+}
+
+// Sum of the element forces of node i over its CSR entries, in CSR order
+// (assemblyForces, Matrices.C:42-87); the first eight entries come from the
+// padded [8][N] table.
+template <typename T>
+__device__ __forceinline__ void gather_force(int i, long Ns, long Es, int nrow, const int *__restrict__ off,
+                                             const int *__restrict__ cnt, const int *__restrict__ nodel_pk,
+                                             const int *__restrict__ pad, const T *__restrict__ fel,
+                                             double (&f)[3]) {
+  const int c = cnt[i], o = c > kChunk ? off[i] : 0;
+  f[0] = f[1] = f[2] = 0.0;
   for (int k = 0; k < c; k += kChunk) {
     int pk[kChunk];
     double q[kChunk][3];
@@ -574,7 +568,7 @@ __global__ __launch_bounds__(BS) void f4_node(
     for (int j = 0; j < kChunk; ++j) {
       long e = pk[j] >> 3, b = (long)(pk[j] & 7) * 3;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) q[j][d] = (pk[j] >= 0) ? (double)fel[fidx((int)(b + d), e, Es, np.nrow)] : 0.0;
+      for (int d = 0; d < 3; ++d) q[j][d] = (pk[j] >= 0) ? (double)fel[fidx((int)(b + d), e, Es, nrow)] : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < kChunk; ++j)
@@ -584,43 +578,25 @@ __global__ __launch_bounds__(BS) void f4_node(
         f[2] += q[j][2];
       }
   }
-  unsigned bad = 0;
-#pragma unroll
-  for (int d = 0; d < 3; ++d)
-    if (!isfinite(f[d])) { f[d] = 0.0; ++bad; }
-  if constexpr (!WFX_F4_HOIST) {
-    if (bad) atomicAdd(status, bad);
-  }
-  const int slot = WFX_F4_HOIST ? h_slot : bc_slot[i];
-  const int mask = WFX_F4_HOIST ? h_mask : (slot >= 0 ? bc_mask[slot] : 0);
-  const double m = WFX_F4_HOIST ? h_m : mdiag[i];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const long id = d * Ns + i;
-    fi[id] = (T)f[d];
-    double fed = np.has_fe ? fe[id] : 0.0;
-    double a = (fed - f[d]) / m;                       // calcAccel
-    if (mask & (1 << d)) a = 0.0;                       // ImposeBCA
-    double pa = WFX_F4_HOIST ? h_pa[d] : (double)prev_a[id];
-    a = np.f * (a - np.alpha * pa);                     // UpdateCorrectionAccVel
-    double vv = (WFX_F4_HOIST ? h_v[d] : (double)v[id]) + np.gdt * a;
-    if (mask & (1 << d)) vv = bc_val[3L * slot + d];    // ImposeBCV
-    double ud = (WFX_F4_HOIST ? h_ud[d] : (double)u_dt[id]) + np.bdt2 * a;  // UpdateCorrectionPos
-    x[id] = (WFX_F4_HOIST ? h_x[d] : x[id]) + ud;
-    if constexpr (std::is_same<T, double>::value) u[id] = (WFX_F4_HOIST ? h_u[d] : u[id]) + ud;
-    if (np.predict) {                                   // UpdatePrediction
-      ud = np.dt * (vv + np.pred_u * a);
-      vv = vv + np.pred_v * a;
-      if (mask & (1 << d)) vv = bc_val[3L * slot + d];  // ImposeBCV
-    }
-    u_dt[id] = (T)ud;
-    prev_a[id] = (T)a;
-    v[id] = (T)vv;
-  }
-  if constexpr (WFX_F4_HOIST) {
-    if (bad) atomicAdd(status, bad);
-  }
 }
+
+// F4: assembly and the whole nodal update of one step
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void f4_node(
+    int N, int E, NodeParams np, const int *__restrict__ off, const int *__restrict__ cnt,
+    const int *__restrict__ nodel_pk, const int *__restrict__ pad, const T *__restrict__ fel,
+    const double *__restrict__ fe, const double *__restrict__ mdiag, const int *__restrict__ bc_slot,
+    const int *__restrict__ bc_mask, const double *__restrict__ bc_val, T *__restrict__ fi,
+    T *__restrict__ v, T *__restrict__ prev_a, T *__restrict__ u_dt,
+    double *__restrict__ x, double *__restrict__ u, unsigned int *__restrict__ status) {
+  int i = gtid();
+  if (i >= N) return;
+  double f[3];
+  gather_force<T>(i, N, E, np.nrow, off, cnt, nodel_pk, pad, fel, f);
+  node_update<T>(i, N, np, f, fe, mdiag, bc_slot, bc_mask, bc_val, fi, v, prev_a, u_dt, x, u, status);
+}
+
+
 
 // FP32-mixed fused runs keep v, prev_a, u_dt and fi in float between steps:
 // narrowed once when the run starts, widened back (and u advanced by the
@@ -685,6 +661,24 @@ hipError_t launch_vsum0(wfx_ctx *c, hipStream_t s) {
   return hipGetLastError();
 }
 
+inline NodeParams node_params(wfx_ctx *c, int predict) {
+  const auto &P = c->prm;
+  NodeParams np;
+  np.dt = P.dt;
+  np.alpha = P.alpha;
+  np.beta = P.beta;
+  np.gamma = P.gamma;
+  np.f = 1.0 / (1.0 - P.alpha);
+  np.gdt = P.gamma * P.dt;
+  np.bdt2 = P.beta * P.dt * P.dt;
+  np.pred_u = (0.5 - P.beta) * P.dt;
+  np.pred_v = (1.0 - P.gamma) * P.dt;
+  np.predict = predict;
+  np.has_fe = c->has_fe;
+  np.nrow = 3 * c->n;
+  return np;
+}
+
 // Launch one phase (1..4) of a fused step on the current state with block
 // size BS; launch_fused_phase picks BS from the context (autotuned or 256).
 template <typename T, int BS>
@@ -723,32 +717,20 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, hipStream_t
     fp.hg_coeff = (c->n == 8) ? st.hg_coeff : 0.0;
     fp.cs0 = P.mat.cs0;
     fp.rho_u = c->rho_uniform;
-    if (c->n == 8)
+    if (c->n == 8) {
       hipLaunchKernelGGL((f3_element<T, 8, BS>), dim3(gE), dim3(BS), 0, s, E, N, fp, c->ms.elnod, c->nd.x,
                          vN, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.He, el.tau, el.eps, el.p,
                          el.pl_strain, el.sigma_y, el.f_elem);
-    else
+    } else {
       hipLaunchKernelGGL((f3_element<T, 4, BS>), dim3(gE), dim3(BS), 0, s, E, N, fp, c->ms.elnod, c->nd.x,
                          vN, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.He, el.tau, el.eps, el.p,
                          el.pl_strain, el.sigma_y, el.f_elem);
+    }
   } else if (phase == 4) {
-    NodeParams np;
-    np.dt = P.dt;
-    np.alpha = P.alpha;
-    np.beta = P.beta;
-    np.gamma = P.gamma;
-    np.f = 1.0 / (1.0 - P.alpha);
-    np.gdt = P.gamma * P.dt;
-    np.bdt2 = P.beta * P.dt * P.dt;
-    np.pred_u = (0.5 - P.beta) * P.dt;
-    np.pred_v = (1.0 - P.gamma) * P.dt;
-    np.predict = predict;
-    np.has_fe = c->has_fe;
-    np.nrow = 3 * c->n;
+    const NodeParams np = node_params(c, predict);
     hipLaunchKernelGGL((f4_node<T, BS>), dim3(gN), dim3(BS), 0, s, N, E, np, c->ms.nodel_offset,
                        c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.f_elem, c->nd.fe, c->nd.mdiag,
-                       c->bc_slot,
-                       c->bc_mask, c->bc_tab, fiN, vN, paN, udN, c->nd.x, c->nd.u, c->status);
+                       c->bc_slot, c->bc_mask, c->bc_tab, fiN, vN, paN, udN, c->nd.x, c->nd.u, c->status);
   } else if (phase == 5) {
     hipLaunchKernelGGL((f_sigma<T>), dim3(nblocks(E)), dim3(kBlock), 0, s, E, el.p, el.tau, el.sigma);
   } else {
