@@ -35,7 +35,7 @@ enum {
 };
 
 enum { WFX_FP64 = 0, WFX_FP32_MIXED = 1 };
-enum { WFX_BILINEAR = 0, WFX_HOLLOMON = 1 };
+enum { WFX_BILINEAR = 0, WFX_HOLLOMON = 1, WFX_JOHNSON_COOK = 2, WFX_GMT = 3 };
 
 const char *wfx_last_error(void);
 int wfx_version(void);
@@ -66,6 +66,30 @@ int wfx_set_density(wfx_ctx *ctx, double rho);
  * H = E*Et/(E-Et)); model WFX_HOLLOMON: p1 = K, p2 = m (Material.cuh:353-395). */
 int wfx_set_material(wfx_ctx *ctx, int model, double E, double nu, double rho0, double sy0,
                      double p1, double p2);
+/* JohnsonCook(el, A, B, n, C, eps_0, m, T_m, T_t) + AssignMaterial
+ * (Material.cuh:191-203, main.C:535-541; yield and tangent
+ * Material.cuh:378-411): sigma_y = (A + B ep^n)(1 + C ln(edot/eps_0))
+ * (1 - T_h^m), T_h = (T - T_t)/(T_m - T_t).  A is the per-element initial
+ * yield (wfx_set_elem_sy0 overrides it element by element). */
+int wfx_set_material_jc(wfx_ctx *ctx, double E, double nu, double rho0, double A, double B,
+                        double n, double C, double eps_0, double m, double T_m, double T_t);
+/* GMT(el, n1, n2, C1, C2, m1, m2, I1, I2, e_min, e_max, er_min, er_max, T_min,
+ * T_max) + AssignMaterial (Material.cuh:271-291, main.C:548-553; yield and
+ * tangent Material.cuh:412-473).  c = {n1, n2, C1, C2, m1, m2, I1, I2} in the
+ * reference's input order, range = {e_min, e_max, er_min, er_max, T_min,
+ * T_max}; sy0 is the initial sigma_y (yieldStress0). */
+int wfx_set_material_gmt(wfx_ctx *ctx, double E, double nu, double rho0, double sy0,
+                         const double c[8], const double range[6]);
+/* Domain_d::setTemp (Domain_d.h:678-683, IniTemp main.C:424-440, default 20):
+ * uniform temperature read by the Johnson-Cook and GMT laws.  The thermal
+ * solver is not part of this library, so the temperature stays as set. */
+int wfx_set_temperature(wfx_ctx *ctx, double T);
+/* per-element temperatures (the reference's T[e] of CalcStressStrain,
+ * Mechanical.C:1725-1729); NULL returns to the uniform value */
+int wfx_set_elem_temperature(wfx_ctx *ctx, const double *T, int elem_count);
+/* Domain_d::m_max_edot ("maxStrRate", main.C:228; default 1e6): cap of the
+ * effective strain rate passed to the tangent modulus (Mechanical.C:1737) */
+int wfx_set_max_strain_rate(wfx_ctx *ctx, double edot);
 /* per-element initial yield stress (one Material_ per element, mat[e]->sy0) */
 int wfx_set_elem_sy0(wfx_ctx *ctx, const double *sy0, int elem_count);
 /* Domain_d::AddBCVelNode (Domain_d.C:1057-1061) */

@@ -60,6 +60,20 @@ class Domain_d {
                       double p2 = 0.0) {
     check(wfx_set_material(c_, model, E, nu, rho0, sy0, p1, p2));
   }
+  // JohnsonCook(el, A, B, n, C, eps_0, m, T_m, T_t) + AssignMaterial
+  void AssignJohnsonCook(double E, double nu, double rho0, double A, double B, double n, double C,
+                         double eps_0, double m, double T_m, double T_t) {
+    check(wfx_set_material_jc(c_, E, nu, rho0, A, B, n, C, eps_0, m, T_m, T_t));
+  }
+  // GMT(el, n1, n2, C1, C2, m1, m2, I1, I2, ranges) + AssignMaterial
+  void AssignGMT(double E, double nu, double rho0, double sy0, const double c[8], const double range[6]) {
+    check(wfx_set_material_gmt(c_, E, nu, rho0, sy0, c, range));
+  }
+  void setTemp(const double &T) { check(wfx_set_temperature(c_, T)); }
+  void setElemTemp(const std::vector<double> &T) {
+    check(wfx_set_elem_temperature(c_, T.data(), (int)T.size()));
+  }
+  void setMaxStrainRate(const double &edot) { check(wfx_set_max_strain_rate(c_, edot)); }
   void AddBCVelNode(const int &node, const int &dim, const double &val) {
     check(wfx_add_bc_vel_node(c_, node, dim, val));
   }

@@ -102,6 +102,7 @@ static t3 t3_sub(const t3 *a, const t3 *b) {
 wfref_domain *wfref_create(void) {
   wfref_domain *d = (wfref_domain *)calloc(1, sizeof(wfref_domain));
   wfref_set_ch_default(d);
+  d->max_edot = 1.0e6; /* Domain_d.h:824 */
   /* m_artifvisc is left uninitialised by the C++ API drivers; zero here */
   return d;
 }
@@ -112,7 +113,7 @@ static void free_fields(wfref_domain *d) {
                    &d->dHdz, &d->vol, &d->vol0, &d->rho, &d->rho0, &d->p,
                    &d->pl_strain, &d->sigma_y, &d->sy0, &d->str_rate,
                    &d->rot_rate, &d->sigma, &d->tau, &d->eps,
-                   &d->strain_pl_incr, &d->f_elem, &d->f_hg};
+                   &d->strain_pl_incr, &d->f_elem, &d->f_hg, &d->temp};
   int **ip[] = {&d->elnod, &d->nodel, &d->nodel_loc, &d->nodel_offset,
                 &d->nodel_count};
   for (size_t i = 0; i < sizeof(dp) / sizeof(dp[0]); ++i) { free(*dp[i]); *dp[i] = NULL; }
@@ -143,8 +144,10 @@ static void set_dimension(wfref_domain *d, int node_count, int elem_count) {
   RD(d->pl_strain, E); RD(d->sigma_y, E); RD(d->sy0, E);
   RD(d->str_rate, 6 * E); RD(d->rot_rate, 6 * E); RD(d->sigma, 6 * E);
   RD(d->tau, 6 * E); RD(d->eps, 6 * E); RD(d->strain_pl_incr, 6 * E);
-  RD(d->f_elem, 3 * n * E); RD(d->f_hg, 3 * n * E);
+  RD(d->f_elem, 3 * n * E); RD(d->f_hg, 3 * n * E); RD(d->temp, E);
   for (int e = 0; e < E; ++e) d->sy0[e] = d->mat.sy0;
+  /* uniform initial temperature, IniTemp default (main.C:424-440) */
+  for (int e = 0; e < E; ++e) d->temp[e] = 20.0;
 }
 
 /* setNodElem (Domain_d.C:1508-1611): node -> element CSR.  Counts, an
@@ -285,6 +288,36 @@ void wfref_set_material(wfref_domain *d, int model, double E, double nu,
     m->eps1 = pow(sy0 / p1, 1. / p2);
   }
   for (int e = 0; e < d->elem_count; ++e) d->sy0[e] = sy0;
+}
+
+/* JohnsonCook(el, A, B, n, C, eps_0, m, T_m, T_t) (Material.cuh:191-203,
+ * main.C:535-541: A is yieldStress0, the other seven come from "const").
+ * A is kept per element in sy0[] like the bilinear initial yield. */
+void wfref_set_material_jc(wfref_domain *d, double E, double nu, double rho0,
+                           double A, double B, double n, double C, double eps_0,
+                           double m, double T_m, double T_t) {
+  wfref_set_material(d, WFREF_BILINEAR, E, nu, rho0, A, 0.0, 0.0);
+  wfref_material *mt = &d->mat;
+  mt->model = WFREF_JOHNSON_COOK;
+  mt->Et = mt->H = 0.0;
+  mt->B = B; mt->n = n; mt->C = C; mt->eps_0 = eps_0;
+  mt->m = m; mt->T_m = T_m; mt->T_t = T_t;
+}
+
+/* GMT(el, n1, n2, C1, C2, m1, m2, I1, I2, e_min, e_max, er_min, er_max,
+ * T_min, T_max) (Material.cuh:271-291, main.C:548-553); c[] in that input
+ * order, range[] = strain, strain-rate and temperature ranges. */
+void wfref_set_material_gmt(wfref_domain *d, double E, double nu, double rho0,
+                            double sy0, const double c[8], const double range[6]) {
+  wfref_set_material(d, WFREF_BILINEAR, E, nu, rho0, sy0, 0.0, 0.0);
+  wfref_material *mt = &d->mat;
+  mt->model = WFREF_GMT;
+  mt->Et = mt->H = 0.0;
+  mt->n1 = c[0]; mt->n2 = c[1]; mt->C1 = c[2]; mt->C2 = c[3];
+  mt->m1 = c[4]; mt->m2 = c[5]; mt->I1 = c[6]; mt->I2 = c[7];
+  mt->e_min = range[0]; mt->e_max = range[1];
+  mt->er_min = range[2]; mt->er_max = range[3];
+  mt->T_min = range[4]; mt->T_max = range[5];
 }
 
 void wfref_set_elem_sy0(wfref_domain *d, const double *sy0) {
@@ -579,21 +612,70 @@ void wfref_calc_elem_pressure(wfref_domain *d) {
   free(voln);
 }
 
-/* Calc*YieldStress / Calc*TangentModulus (Material.cuh:353-364, :389-395);
- * BILINEAR is this build's definition (the reference leaves it undefined). */
-static double yield_stress(const wfref_domain *d, int e, double ep) {
+/* effective strain rate of CalcStressStrain (Mechanical.C:1700-1705) */
+double wfref_eff_strain_rate(const double D[6]) {
+  double xx = D[0], yy = D[1], zz = D[2], xy = D[3], yz = D[4], zx = D[5];
+  return sqrt(0.5 * ((xx - yy) * (xx - yy) + (yy - zz) * (yy - zz) + (zz - xx) * (zz - xx)) +
+              3.0 * (xy * xy + yz * yz + zx * zx));
+}
+
+/* the clamps of CalcGMTYieldStress / CalcGMTTangentModulus
+ * (Material.cuh:427-435, :451-459) */
+static double clampr(double v, double lo, double hi) {
+  if (v < lo) return lo;
+  if (v > hi) return hi;
+  return v;
+}
+
+/* Calc*YieldStress (Material.cuh:353-364, :378-388, :412-441) of element e
+ * at plastic strain ep and effective strain rate edot; the element
+ * temperature is temp[e] (the reference reads T[e], :1725-1729 of
+ * Mechanical.C).  BILINEAR is this build's definition (the reference leaves
+ * it undefined). */
+double wfref_yield_stress(const wfref_domain *d, int e, double ep, double edot) {
   const wfref_material *m = &d->mat;
   if (m->model == WFREF_HOLLOMON) {
     if (ep + m->eps0 > m->eps1) return m->Kh * pow(ep + m->eps0, m->mh);
     return m->sy0;
   }
+  if (m->model == WFREF_JOHNSON_COOK) {
+    double T_h = (d->temp[e] - m->T_t) / (m->T_m - m->T_t);
+    double sr = edot;
+    if (edot == 0.0) sr = 1.e-5;
+    return (d->sy0[e] + m->B * pow(ep, m->n)) * (1.0 + m->C * log(sr / m->eps_0)) *
+           (1.0 - pow(T_h, m->m));
+  }
+  if (m->model == WFREF_GMT) {
+    double es = clampr(ep, m->e_min, m->e_max);
+    double er = clampr(edot, m->er_min, m->er_max);
+    double T = clampr(d->temp[e], m->T_min, m->T_max);
+    return m->C1 * exp(m->C2 * T) * pow(es, m->n1 * T + m->n2) * exp((m->I1 * T + m->I2) / es) *
+           pow(er, m->m1 * T + m->m2);
+  }
   return d->sy0[e] + m->H * ep;
 }
-static double tangent_modulus(const wfref_domain *d, double ep) {
+
+/* Calc*TangentModulus (Material.cuh:389-395, :397-411, :444-473), with the
+ * strain rate already capped by max_edot (Mechanical.C:1737). */
+double wfref_tangent_modulus(const wfref_domain *d, int e, double ep, double edot) {
   const wfref_material *m = &d->mat;
   if (m->model == WFREF_HOLLOMON) {
     if (ep + m->eps0 > m->eps1) return m->Kh * m->mh * pow(ep + m->eps0, (m->mh - 1.0));
     return 0.;
+  }
+  if (m->model == WFREF_JOHNSON_COOK) {
+    double T_h = (d->temp[e] - m->T_t) / (m->T_m - m->T_t);
+    if (ep > 0.)
+      return m->n * m->B * pow(ep, m->n - 1.) * (1.0 + m->C * log(edot / m->eps_0)) *
+             (1.0 - pow(T_h, m->m));
+    return m->E * 0.1;
+  }
+  if (m->model == WFREF_GMT) {
+    double es = clampr(ep, m->e_min, m->e_max);
+    double er = clampr(edot, m->er_min, m->er_max);
+    double T = clampr(d->temp[e], m->T_min, m->T_max);
+    return m->C1 * exp(m->C2 * T) * pow(er, m->m1 * T + m->m2) * pow(es, T * m->n1 + m->n2 - 2.0) *
+           (-m->I1 * T - m->I2 + es * (m->n1 * T + m->n2)) * exp((m->I1 * T + m->I2) / es);
   }
   return m->H;
 }
@@ -630,10 +712,12 @@ void wfref_calc_stress_strain(wfref_domain *d, double dt) {
     double J2 = 0.5 * (s.xx * s.xx + 2.0 * s.xy * s.xy + 2.0 * s.xz * s.xz + s.yy * s.yy +
                        2.0 * s.yz * s.yz + s.zz * s.zz);
     double sig_trial = sqrt(3.0 * J2);
-    d->sigma_y[e] = yield_stress(d, e, d->pl_strain[e]);
+    double edot = wfref_eff_strain_rate(d->str_rate + 6 * e);
+    d->sigma_y[e] = wfref_yield_stress(d, e, d->pl_strain[e], edot);
+    edot = dmin(edot, d->max_edot);
     double dep = 0.0;
     if (d->sigma_y[e] < sig_trial) {
-      double H = tangent_modulus(d, d->pl_strain[e]);
+      double H = wfref_tangent_modulus(d, e, d->pl_strain[e], edot);
       double dgamma = (sig_trial - d->sigma_y[e]) / (3.0 * G + H);
       double factor = 1.0 - (3.0 * G * dgamma) / sig_trial;
       tau = t3_scale(&s, factor);
@@ -888,7 +972,7 @@ void *wfref_field(wfref_domain *d, const char *name, long *count, int *is_int) {
       {"dHdx", d->dHdx, n * E, 0}, {"dHdy", d->dHdy, n * E, 0}, {"dHdz", d->dHdz, n * E, 0},
       {"vol", d->vol, E, 0}, {"vol0", d->vol0, E, 0}, {"rho", d->rho, E, 0},
       {"rho0", d->rho0, E, 0}, {"p", d->p, E, 0}, {"pl_strain", d->pl_strain, E, 0},
-      {"sigma_y", d->sigma_y, E, 0}, {"sy0", d->sy0, E, 0},
+      {"sigma_y", d->sigma_y, E, 0}, {"sy0", d->sy0, E, 0}, {"temp", d->temp, E, 0},
       {"str_rate", d->str_rate, 6 * E, 0}, {"rot_rate", d->rot_rate, 6 * E, 0},
       {"sigma", d->sigma, 6 * E, 0}, {"tau", d->tau, 6 * E, 0}, {"eps", d->eps, 6 * E, 0},
       {"strain_pl_incr", d->strain_pl_incr, 6 * E, 0}, {"f_elem", d->f_elem, 3 * n * E, 0},
@@ -921,6 +1005,7 @@ double wfref_get_double(wfref_domain *d, const char *name) {
   if (!strcmp(name, "G")) return d->mat.G;
   if (!strcmp(name, "cs0")) return d->mat.cs0;
   if (!strcmp(name, "H")) return d->mat.H;
+  if (!strcmp(name, "max_edot")) return d->max_edot;
   return NAN;
 }
 
@@ -931,7 +1016,7 @@ int wfref_set_double(wfref_domain *d, const char *name, double v) {
       {"pspg_scale", &d->pspg_scale}, {"p_pspg_bulkfac", &d->p_pspg_bulkfac},
       {"av_coeff_div", &d->av_coeff_div}, {"av_coeff_bulk", &d->av_coeff_bulk},
       {"artifvisc0", &d->artifvisc[0]}, {"artifvisc1", &d->artifvisc[1]},
-      {"hg_coeff", &d->hg_coeff}};
+      {"hg_coeff", &d->hg_coeff}, {"max_edot", &d->max_edot}};
   for (size_t k = 0; k < sizeof(t) / sizeof(t[0]); ++k)
     if (strcmp(t[k].nm, name) == 0) { *t[k].p = v; return 0; }
   return -1;

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-enum { WFREF_BILINEAR = 0, WFREF_HOLLOMON = 1 };
+enum { WFREF_BILINEAR = 0, WFREF_HOLLOMON = 1, WFREF_JOHNSON_COOK = 2, WFREF_GMT = 3 };
 
 typedef struct {
   int model;            /* WFREF_BILINEAR / WFREF_HOLLOMON                  */
@@ -40,6 +40,11 @@ typedef struct {
   double sy0;           /* initial yield                                     */
   double Et, H;         /* bilinear tangent and plastic modulus E*Et/(E-Et)  */
   double Kh, mh, eps0, eps1; /* Hollomon (Material.cuh:90-104)               */
+  /* Johnson-Cook (Material.cuh:105-118): A is the per-element sy0[] */
+  double B, n, C, eps_0, m, T_m, T_t;
+  /* GMT (Material.cuh:271-291) and its clamping ranges */
+  double C1, C2, n1, n2, m1, m2, I1, I2;
+  double e_min, e_max, er_min, er_max, T_min, T_max;
 } wfref_material;
 
 typedef struct {
@@ -56,6 +61,8 @@ typedef struct {
   double *vol, *vol0, *rho, *rho0, *p, *pl_strain, *sigma_y, *sy0; /* E */
   double *str_rate, *rot_rate, *sigma, *tau, *eps, *strain_pl_incr; /* 6E */
   double *f_elem, *f_hg;                             /* 3nE, [e][a][d] */
+  double *temp;       /* element temperature, E (IniTemp 20, main.C:424) */
+  double max_edot;    /* strain-rate cap of the tangent (Domain_d.h:824) */
   /* material (one shared law; per-element initial yield in sy0[]) */
   wfref_material mat;
   /* Dirichlet velocity lists per dimension (AddBCVelNode order) */
@@ -89,7 +96,16 @@ void wfref_set_nod_elem(wfref_domain *d, const int *elnod);
 void wfref_set_density(wfref_domain *d, double rho);
 void wfref_set_material(wfref_domain *d, int model, double E, double nu,
                         double rho0, double sy0, double p1, double p2);
+void wfref_set_material_jc(wfref_domain *d, double E, double nu, double rho0,
+                           double A, double B, double n, double C, double eps_0,
+                           double m, double T_m, double T_t);
+void wfref_set_material_gmt(wfref_domain *d, double E, double nu, double rho0,
+                            double sy0, const double c[8], const double range[6]);
 void wfref_set_elem_sy0(wfref_domain *d, const double *sy0);
+/* the hardening law of one element, exposed for the pin tests */
+double wfref_eff_strain_rate(const double D[6]);
+double wfref_yield_stress(const wfref_domain *d, int e, double ep, double edot);
+double wfref_tangent_modulus(const wfref_domain *d, int e, double ep, double edot);
 void wfref_add_bc_vel_node(wfref_domain *d, int node, int dim, double val);
 void wfref_set_reaction_nodes(wfref_domain *d, int count, const int *nodes);
 void wfref_set_ch_default(wfref_domain *d);

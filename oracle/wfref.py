@@ -16,6 +16,8 @@ _lib = None
 
 BILINEAR = 0
 HOLLOMON = 1
+JOHNSON_COOK = 2
+GMT = 3
 
 
 def build():
@@ -38,6 +40,11 @@ def lib():
             "wfref_set_density": (None, [vp, d]),
             "wfref_set_material": (None, [vp, i, d, d, d, d, d, d]),
             "wfref_set_elem_sy0": (None, [vp, dp]),
+            "wfref_set_material_jc": (None, [vp, d, d, d, d, d, d, d, d, d, d, d]),
+            "wfref_set_material_gmt": (None, [vp, d, d, d, d, dp, dp]),
+            "wfref_eff_strain_rate": (d, [dp]),
+            "wfref_yield_stress": (d, [vp, i, d, d]),
+            "wfref_tangent_modulus": (d, [vp, i, d, d]),
             "wfref_add_bc_vel_node": (None, [vp, i, i, d]),
             "wfref_set_reaction_nodes": (None, [vp, i, ip]),
             "wfref_set_ch": (None, [vp, d, d, d]),
@@ -119,6 +126,34 @@ class RefDomain:
 
     def setMaterial(self, model, E, nu, rho0, sy0, p1=0.0, p2=0.0):
         self._L.wfref_set_material(self._d, int(model), E, nu, rho0, sy0, p1, p2)
+
+    def setMaterialJC(self, E, nu, rho0, A, B, n, C, eps_0, m, T_m, T_t):
+        self._L.wfref_set_material_jc(self._d, E, nu, rho0, A, B, n, C, eps_0, m, T_m, T_t)
+
+    def setMaterialGMT(self, E, nu, rho0, sy0, c, ranges):
+        ca = np.ascontiguousarray(c, dtype=np.float64)
+        ra = np.ascontiguousarray(ranges, dtype=np.float64)
+        assert ca.size == 8 and ra.size == 6
+        self._L.wfref_set_material_gmt(self._d, E, nu, rho0, sy0, _ptr(ca), _ptr(ra))
+
+    def setTemperature(self, T):
+        """uniform element temperature, or one value per element"""
+        t = np.broadcast_to(np.asarray(T, dtype=np.float64), (self.elem_count,))
+        self.setField("temp", np.ascontiguousarray(t))
+
+    def setMaxStrainRate(self, edot):
+        self.set("max_edot", edot)
+
+    def yieldStress(self, e, ep, edot):
+        return self._L.wfref_yield_stress(self._d, int(e), float(ep), float(edot))
+
+    def tangentModulus(self, e, ep, edot):
+        return self._L.wfref_tangent_modulus(self._d, int(e), float(ep), float(edot))
+
+    @staticmethod
+    def effStrainRate(D6):
+        a = np.ascontiguousarray(D6, dtype=np.float64)
+        return lib().wfref_eff_strain_rate(_ptr(a))
 
     def setElemSy0(self, sy0):
         a = np.ascontiguousarray(sy0, dtype=np.float64)

@@ -13,6 +13,8 @@ WFX_FP64 = 0
 WFX_FP32_MIXED = 1
 WFX_BILINEAR = 0
 WFX_HOLLOMON = 1
+WFX_JOHNSON_COOK = 2
+WFX_GMT = 3
 
 _vp, _i, _d, _cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_char_p
 _pd = ctypes.POINTER(ctypes.c_double)
@@ -30,6 +32,11 @@ SIGNATURES = {
     "wfx_set_density": (_i, [_vp, _d]),
     "wfx_set_material": (_i, [_vp, _i, _d, _d, _d, _d, _d, _d]),
     "wfx_set_elem_sy0": (_i, [_vp, _vp, _i]),
+    "wfx_set_material_jc": (_i, [_vp, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d]),
+    "wfx_set_material_gmt": (_i, [_vp, _d, _d, _d, _d, _vp, _vp]),
+    "wfx_set_temperature": (_i, [_vp, _d]),
+    "wfx_set_elem_temperature": (_i, [_vp, _vp, _i]),
+    "wfx_set_max_strain_rate": (_i, [_vp, _d]),
     "wfx_add_bc_vel_node": (_i, [_vp, _i, _i, _d]),
     "wfx_allocate_bcs": (_i, [_vp]),
     "wfx_set_dt": (_i, [_vp, _d]),

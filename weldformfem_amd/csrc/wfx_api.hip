@@ -55,6 +55,7 @@ static void free_all(wfx_ctx *c) {
   c->bc_tab = nullptr;
   c->v32 = c->pa32 = c->ud32 = c->fi32 = nullptr;
   c->x_start = nullptr;
+  c->temp_e = c->temp_buf = nullptr;
   c->has_fe = 0;
   c->fused_stale = false;
   c->scalars = nullptr;
@@ -352,20 +353,46 @@ int wfx_set_density(wfx_ctx *c, double rho) {
   return upload_real(c, dst, h.data(), h.size(), true);
 }
 
-int wfx_set_material(wfx_ctx *c, int model, double E, double nu, double rho0, double sy0, double p1,
-                     double p2) {
-  REQUIRE_CTX(c);
-  REQUIRE(model == WFX_BILINEAR || model == WFX_HOLLOMON, WFX_ERR_ARG, "unknown material model");
-  REQUIRE(E > 0.0 && nu > -1.0 && nu < 0.5 && rho0 > 0.0, WFX_ERR_ARG, "bad elastic constants");
+// Elastic_ (Material.cuh:22-27) and the fields every law shares.  The
+// temperature and the strain-rate cap are domain state (Domain_d::T,
+// m_max_edot) and survive a change of material.
+static Material elastic_material(const wfx_ctx *c, int model, double E, double nu, double rho0,
+                                 double sy0) {
   Material m;
   m.model = model;
   m.E = E;
   m.nu = nu;
-  m.K = E / (3.0 * (1.0 - 2.0 * nu));  // Material.cuh:22-27
+  m.K = E / (3.0 * (1.0 - 2.0 * nu));
   m.G = E / (2.0 * (1.0 + nu));
   m.rho0 = rho0;
   m.cs0 = std::sqrt(m.K / rho0);
   m.sy0 = sy0;
+  m.temp = c->prm.mat.temp;
+  m.max_edot = c->prm.mat.max_edot;
+  return m;
+}
+
+// AssignMaterial (Domain_d.C:945): every element starts at the law's sy0
+static int assign_material(wfx_ctx *c, const Material &m) {
+  c->prm.mat = m;
+  if (c->has_mesh) {
+    c->sy0_h.assign(c->E, m.sy0);
+    void *dst = c->precision == WFX_FP64 ? (void *)c->e64.sy0 : (void *)c->e32.sy0;
+    return upload_real(c, dst, c->sy0_h.data(), c->sy0_h.size(), true);
+  }
+  return WFX_OK;
+}
+
+#define REQUIRE_ELASTIC(E, nu, rho0) \
+  REQUIRE((E) > 0.0 && (nu) > -1.0 && (nu) < 0.5 && (rho0) > 0.0, WFX_ERR_ARG, "bad elastic constants")
+
+int wfx_set_material(wfx_ctx *c, int model, double E, double nu, double rho0, double sy0, double p1,
+                     double p2) {
+  REQUIRE_CTX(c);
+  REQUIRE(model == WFX_BILINEAR || model == WFX_HOLLOMON, WFX_ERR_ARG,
+          "unknown material model (Johnson-Cook and GMT have setters of their own)");
+  REQUIRE_ELASTIC(E, nu, rho0);
+  Material m = elastic_material(c, model, E, nu, rho0, sy0);
   if (model == WFX_BILINEAR) {
     REQUIRE(p1 >= 0.0 && p1 < E, WFX_ERR_ARG, "bilinear tangent modulus must be in [0, E)");
     m.Et = p1;
@@ -377,12 +404,73 @@ int wfx_set_material(wfx_ctx *c, int model, double E, double nu, double rho0, do
     m.eps0 = sy0 / E;  // Material.cuh:95-98
     m.eps1 = std::pow(sy0 / p1, 1. / p2);
   }
-  c->prm.mat = m;
-  if (c->has_mesh) {
-    c->sy0_h.assign(c->E, sy0);
-    void *dst = c->precision == WFX_FP64 ? (void *)c->e64.sy0 : (void *)c->e32.sy0;
-    return upload_real(c, dst, c->sy0_h.data(), c->sy0_h.size(), true);
+  return assign_material(c, m);
+}
+
+int wfx_set_material_jc(wfx_ctx *c, double E, double nu, double rho0, double A, double B, double n,
+                        double C, double eps_0, double m_, double T_m, double T_t) {
+  REQUIRE_CTX(c);
+  REQUIRE_ELASTIC(E, nu, rho0);
+  REQUIRE(eps_0 > 0.0 && T_m != T_t, WFX_ERR_ARG,
+          "Johnson-Cook needs eps_0 > 0 and T_m != T_t");
+  Material m = elastic_material(c, WFX_JOHNSON_COOK, E, nu, rho0, A);
+  m.B = B;
+  m.n = n;
+  m.C = C;
+  m.eps_0 = eps_0;
+  m.m = m_;
+  m.T_m = T_m;
+  m.T_t = T_t;
+  return assign_material(c, m);
+}
+
+int wfx_set_material_gmt(wfx_ctx *c, double E, double nu, double rho0, double sy0, const double k[8],
+                         const double r[6]) {
+  REQUIRE_CTX(c);
+  REQUIRE_ELASTIC(E, nu, rho0);
+  REQUIRE(k && r, WFX_ERR_ARG, "GMT needs 8 constants and 6 range bounds");
+  Material m = elastic_material(c, WFX_GMT, E, nu, rho0, sy0);
+  m.n1 = k[0];
+  m.n2 = k[1];
+  m.C1 = k[2];
+  m.C2 = k[3];
+  m.m1 = k[4];
+  m.m2 = k[5];
+  m.I1 = k[6];
+  m.I2 = k[7];
+  m.e_min = r[0];
+  m.e_max = r[1];
+  m.er_min = r[2];
+  m.er_max = r[3];
+  m.T_min = r[4];
+  m.T_max = r[5];
+  return assign_material(c, m);
+}
+
+int wfx_set_temperature(wfx_ctx *c, double T) {
+  REQUIRE_CTX(c);
+  REQUIRE(std::isfinite(T), WFX_ERR_ARG, "temperature must be finite");
+  c->prm.mat.temp = T;
+  return WFX_OK;
+}
+
+int wfx_set_elem_temperature(wfx_ctx *c, const double *T, int count) {
+  REQUIRE_MESH(c);
+  if (!T) {
+    c->temp_e = nullptr;  // the buffer stays allocated until the mesh changes
+    return WFX_OK;
   }
+  REQUIRE(count == c->E, WFX_ERR_ARG, "temperature needs one value per element");
+  if (!c->temp_buf) WFX_CHECK_HIP(dalloc(c, &c->temp_buf, c->E));
+  WFX_CHECK_HIP(hipMemcpy(c->temp_buf, T, sizeof(double) * (size_t)count, hipMemcpyHostToDevice));
+  c->temp_e = c->temp_buf;
+  return WFX_OK;
+}
+
+int wfx_set_max_strain_rate(wfx_ctx *c, double edot) {
+  REQUIRE_CTX(c);
+  REQUIRE(edot >= 0.0, WFX_ERR_ARG, "strain-rate cap must be non-negative");
+  c->prm.mat.max_edot = edot;
   return WFX_OK;
 }
 
@@ -867,7 +955,7 @@ static uint64_t graph_signature(wfx_ctx *c) {
   mix(&c->prm.mat, sizeof(c->prm.mat));
   mix(&c->prm.stab, sizeof(c->prm.stab));
   const void *ptrs[] = {c->bc_nod[0], c->bc_nod[1], c->bc_nod[2], c->bc_slot, c->bc_mask, c->bc_tab,
-                        c->react, c->hist, c->e64.He, c->e32.He};
+                        c->react, c->hist, c->e64.He, c->e32.He, c->temp_e};
   mix(ptrs, sizeof(ptrs));
   int ints[] = {c->N, c->E, c->n, c->has_fe, c->bc_count[0], c->bc_count[1], c->bc_count[2],
                 (int)c->react_h.size(), c->fused_bs[1], c->fused_bs[2], c->fused_bs[3], c->fused_bs[4]};

@@ -412,15 +412,17 @@ __device__ __forceinline__ void element_chain(
   T ep = ld_stream(&pl_strain[e]);
   // bilinear plastic modulus: per element when given (specimen batches)
   T sy, Hb = (T)mp.H;
-  if (mp.model != HOLLOMON && He) {
+  const bool per_elem = mp.model == BILINEAR && He;
+  const double edot = (mp.model >= JOHNSON_COOK) ? eff_strain_rate(D) : 0.0;
+  if (per_elem) {
     Hb = ld_stream(&He[e]);
     sy = ld_stream(&sy0[e]) + Hb * ep;
   } else {
-    sy = yield_stress<T>(mp, ld_stream(&sy0[e]), ep);
+    sy = yield_stress<T>(mp, e, ld_stream(&sy0[e]), ep, edot);
   }
   sigma_y[e] = sy;
   if (sy < sig_trial) {
-    T H = (mp.model != HOLLOMON && He) ? Hb : tangent_modulus<T>(mp, ep);
+    T H = per_elem ? Hb : tangent_modulus<T>(mp, e, ep, cap_rate(mp, edot));
     T dgamma = (sig_trial - sy) / (T(3) * G + H);
     T factor = T(1) - (T(3) * G * dgamma) / sig_trial;
     tau = t3_scale(s, factor);
@@ -708,7 +710,7 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, hipStream_t
   } else if (phase == 3) {
     FusedParams fp;
     fp.dt = P.dt;
-    fp.mp = MatParams{P.mat.model, P.mat.G, P.mat.H, P.mat.sy0, P.mat.Kh, P.mat.mh, P.mat.eps0, P.mat.eps1};
+    fp.mp = mat_params(c);
     fp.pp = PressParams{P.mat.K,        st.alpha_free, st.log_factor,     st.J_min,        st.hg_coeff_free,
                         st.pspg_scale,  st.p_pspg_bulkfac, st.av_coeff_div, st.av_coeff_bulk, 0};
     fp.pp.full = (st.pspg_scale != 0.0 || st.p_pspg_bulkfac != 0.0 || st.av_coeff_div != 0.0 ||

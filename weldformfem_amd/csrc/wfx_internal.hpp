@@ -34,13 +34,18 @@ namespace wfx {
 void set_error(const std::string &msg);
 struct Adjoint;  // checkpointed adjoint run (wfx_adjoint.hip)
 
-enum Model { BILINEAR = 0, HOLLOMON = 1 };
+enum Model { BILINEAR = 0, HOLLOMON = 1, JOHNSON_COOK = 2, GMT = 3 };
 
 struct Material {
   int model = BILINEAR;
   double E = 0, nu = 0, K = 0, G = 0, rho0 = 0, cs0 = 0, sy0 = 1.0e10;
   double Et = 0, H = 0;                 // bilinear: tangent, plastic modulus
   double Kh = 0, mh = 1, eps0 = 0, eps1 = 0;  // Hollomon
+  double B = 0, n = 1, C = 0, eps_0 = 1, m = 1, T_m = 1e10, T_t = 0;  // Johnson-Cook (A = sy0)
+  double C1 = 0, C2 = 0, n1 = 0, n2 = 0, m1 = 0, m2 = 0, I1 = 0, I2 = 0;  // GMT
+  double e_min = 0, e_max = 1e10, er_min = 0, er_max = 1e10, T_min = 0, T_max = 1e10;
+  double temp = 20.0;      // uniform element temperature (IniTemp, main.C:424)
+  double max_edot = 1e6;   // strain-rate cap of the tangent (Domain_d.h:824)
 };
 
 struct Stab {
@@ -120,6 +125,8 @@ struct wfx_ctx {
   double *x_start = nullptr;
   int has_fe = 0;            // external nodal forces set (non-zero)
   double rho_uniform = 0.0;  // > 0: every element density equals this value
+  double *temp_e = nullptr;  // per-element temperature (else prm.mat.temp for all)
+  double *temp_buf = nullptr;  // device buffer behind temp_e, kept once allocated
   int fused_bs[5] = {256, 256, 256, 256, 256};  // block size of fused phases 1..4
   // captured fused-step graph
   hipGraph_t graph = nullptr;

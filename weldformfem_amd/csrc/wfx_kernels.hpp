@@ -438,23 +438,91 @@ __global__ __launch_bounds__(kBlock) void k_pressure_elem(int E, int N, const in
 struct MatParams {
   int model;
   double G, H, sy0, Kh, mh, eps0, eps1;
+  double E;                                // Johnson-Cook tangent at zero plastic strain
+  double B, n, C, eps_0, m, T_m, T_t;      // Johnson-Cook (A is the element's sy0)
+  double C1, C2, n1, n2, m1, m2, I1, I2;   // GMT
+  double e_min, e_max, er_min, er_max, T_min, T_max;
+  double temp, max_edot;                   // uniform temperature, tangent strain-rate cap
+  const double *Te;                        // per-element temperature, or null
 };
 
-// Calc*YieldStress / Calc*TangentModulus (Material.cuh:353-364, :389-395);
-// bilinear as sigma_y = sy0_e + H*ep (the reference leaves it undefined).
+inline MatParams mat_params(const wfx_ctx *c) {
+  const Material &m = c->prm.mat;
+  return MatParams{m.model, m.G,     m.H,     m.sy0,   m.Kh,     m.mh,     m.eps0, m.eps1, m.E,
+                   m.B,     m.n,     m.C,     m.eps_0, m.m,      m.T_m,    m.T_t,  m.C1,   m.C2,
+                   m.n1,    m.n2,    m.m1,    m.m2,    m.I1,     m.I2,     m.e_min, m.e_max,
+                   m.er_min, m.er_max, m.T_min, m.T_max, m.temp, m.max_edot, c->temp_e};
+}
+
+// effective strain rate of CalcStressStrain (Mechanical.C:1700-1705)
 template <typename T>
-__device__ __forceinline__ T yield_stress(const MatParams &m, T sy0e, T ep) {
+__device__ __forceinline__ double eff_strain_rate(const T3<T> &D) {
+  const double xx = D.xx, yy = D.yy, zz = D.zz, xy = D.xy, yz = D.yz, zx = D.xz;
+  return sqrt(0.5 * ((xx - yy) * (xx - yy) + (yy - zz) * (yy - zz) + (zz - xx) * (zz - xx)) +
+              3.0 * (xy * xy + yz * yz + zx * zx));
+}
+
+// std::min(edot, cap) (Mechanical.C:1737)
+__device__ __forceinline__ double cap_rate(const MatParams &m, double edot) {
+  return (m.max_edot < edot) ? m.max_edot : edot;
+}
+
+// the clamps of the GMT law (Material.cuh:427-435, :451-459)
+__device__ __forceinline__ double clamp_range(double v, double lo, double hi) {
+  if (v < lo) return lo;
+  if (v > hi) return hi;
+  return v;
+}
+
+// Calc*YieldStress (Material.cuh:353-364, :378-388, :412-441) of element e
+// at plastic strain ep and effective strain rate edot; bilinear as
+// sigma_y = sy0_e + H*ep (the reference leaves it undefined).  The
+// rate-dependent laws evaluate in double in both precisions.
+template <typename T>
+__device__ __forceinline__ T yield_stress(const MatParams &m, int e, T sy0e, T ep, double edot) {
   if (m.model == HOLLOMON) {
     double s = (double)ep + m.eps0;
     return (s > m.eps1) ? (T)(m.Kh * pow(s, m.mh)) : (T)m.sy0;
   }
+  if (m.model == JOHNSON_COOK) {
+    const double temp = m.Te ? m.Te[e] : m.temp;
+    const double T_h = (temp - m.T_t) / (m.T_m - m.T_t);
+    const double sr = (edot == 0.0) ? 1.e-5 : edot;
+    return (T)(((double)sy0e + m.B * pow((double)ep, m.n)) * (1.0 + m.C * log(sr / m.eps_0)) *
+               (1.0 - pow(T_h, m.m)));
+  }
+  if (m.model == GMT) {
+    const double es = clamp_range((double)ep, m.e_min, m.e_max);
+    const double er = clamp_range(edot, m.er_min, m.er_max);
+    const double Tc = clamp_range(m.Te ? m.Te[e] : m.temp, m.T_min, m.T_max);
+    return (T)(m.C1 * exp(m.C2 * Tc) * pow(es, m.n1 * Tc + m.n2) * exp((m.I1 * Tc + m.I2) / es) *
+               pow(er, m.m1 * Tc + m.m2));
+  }
   return sy0e + (T)m.H * ep;
 }
+
+// Calc*TangentModulus (Material.cuh:389-395, :397-411, :444-473); edot is
+// the capped rate.
 template <typename T>
-__device__ __forceinline__ T tangent_modulus(const MatParams &m, T ep) {
+__device__ __forceinline__ T tangent_modulus(const MatParams &m, int e, T ep, double edot) {
   if (m.model == HOLLOMON) {
     double s = (double)ep + m.eps0;
     return (s > m.eps1) ? (T)(m.Kh * m.mh * pow(s, (m.mh - 1.0))) : T(0);
+  }
+  if (m.model == JOHNSON_COOK) {
+    const double temp = m.Te ? m.Te[e] : m.temp;
+    const double T_h = (temp - m.T_t) / (m.T_m - m.T_t);
+    if (ep > T(0))
+      return (T)(m.n * m.B * pow((double)ep, m.n - 1.) * (1.0 + m.C * log(edot / m.eps_0)) *
+                 (1.0 - pow(T_h, m.m)));
+    return (T)(m.E * 0.1);
+  }
+  if (m.model == GMT) {
+    const double es = clamp_range((double)ep, m.e_min, m.e_max);
+    const double er = clamp_range(edot, m.er_min, m.er_max);
+    const double Tc = clamp_range(m.Te ? m.Te[e] : m.temp, m.T_min, m.T_max);
+    return (T)(m.C1 * exp(m.C2 * Tc) * pow(er, m.m1 * Tc + m.m2) * pow(es, Tc * m.n1 + m.n2 - 2.0) *
+               (-m.I1 * Tc - m.I2 + es * (m.n1 * Tc + m.n2)) * exp((m.I1 * Tc + m.I2) / es));
   }
   return (T)m.H;
 }
@@ -495,12 +563,13 @@ __global__ __launch_bounds__(kBlock) void k_stress_strain(
   T sig_trial = sqrt(T(3) * J2);
   T ep = pl_strain[e];
   // bilinear plastic modulus: per element when given (specimen batches)
-  const bool per_elem = mp.model != HOLLOMON && He;
-  T sy = per_elem ? sy0[e] + He[e] * ep : yield_stress<T>(mp, sy0[e], ep);
+  const bool per_elem = mp.model == BILINEAR && He;
+  const double edot = (mp.model >= JOHNSON_COOK) ? eff_strain_rate(D) : 0.0;
+  T sy = per_elem ? sy0[e] + He[e] * ep : yield_stress<T>(mp, e, sy0[e], ep, edot);
   sigma_y[e] = sy;
   T dep = T(0);
   if (sy < sig_trial) {
-    T H = per_elem ? He[e] : tangent_modulus<T>(mp, ep);
+    T H = per_elem ? He[e] : tangent_modulus<T>(mp, e, ep, cap_rate(mp, edot));
     T dgamma = (sig_trial - sy) / (T(3) * G + H);
     T factor = T(1) - (T(3) * G * dgamma) / sig_trial;
     tau = t3_scale(s, factor);
@@ -872,8 +941,7 @@ template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, d
   }                                                                                                \
   template <> hipError_t launch_stress_strain<T>(wfx_ctx * c, double dt, hipStream_t s) {          \
     auto &el = elem<T>(c);                                                                         \
-    const auto &m = c->prm.mat;                                                                    \
-    MatParams mp{m.model, m.G, m.H, m.sy0, m.Kh, m.mh, m.eps0, m.eps1};                            \
+    const MatParams mp = mat_params(c);                                                            \
     hipLaunchKernelGGL((k_stress_strain<T>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, (T)dt, \
                        mp, el.str_rate, el.rot_rate, el.p, el.sy0, el.He, el.tau, el.eps, el.sigma, \
                        el.strain_pl_incr, el.pl_strain, el.sigma_y);                               \

@@ -10,9 +10,9 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import WFX_BILINEAR, WFX_FP32_MIXED, WFX_FP64, WFX_HOLLOMON, check
+from ._lib import WFX_BILINEAR, WFX_FP32_MIXED, WFX_FP64, WFX_GMT, WFX_HOLLOMON, WFX_JOHNSON_COOK, check
 
-__all__ = ["Domain", "WFX_FP64", "WFX_FP32_MIXED", "WFX_BILINEAR", "WFX_HOLLOMON"]
+__all__ = ["Domain", "WFX_FP64", "WFX_FP32_MIXED", "WFX_BILINEAR", "WFX_HOLLOMON", "WFX_JOHNSON_COOK", "WFX_GMT"]
 
 
 def _dptr(a):
@@ -85,6 +85,34 @@ class Domain:
 
     def setMaterial(self, model, E, nu, rho0, sy0, p1=0.0, p2=0.0):
         check(self._L.wfx_set_material(self._h, int(model), E, nu, rho0, sy0, p1, p2))
+
+    def setMaterialJC(self, E, nu, rho0, A, B, n, C, eps_0, m, T_m, T_t):
+        """Johnson-Cook law (Material.cuh:191-203, :378-411); A is the
+        per-element initial yield"""
+        check(self._L.wfx_set_material_jc(self._h, E, nu, rho0, A, B, n, C, eps_0, m, T_m, T_t))
+
+    def setMaterialGMT(self, E, nu, rho0, sy0, c, ranges):
+        """GMT law (Material.cuh:271-291, :412-473); c = (n1, n2, C1, C2, m1,
+        m2, I1, I2), ranges = (e_min, e_max, er_min, er_max, T_min, T_max)"""
+        ca = np.ascontiguousarray(c, dtype=np.float64)
+        ra = np.ascontiguousarray(ranges, dtype=np.float64)
+        if ca.size != 8 or ra.size != 6:
+            raise ValueError("GMT needs 8 constants and 6 range bounds")
+        check(self._L.wfx_set_material_gmt(self._h, E, nu, rho0, sy0, ca.ctypes.data, ra.ctypes.data))
+
+    def setTemperature(self, T):
+        """uniform temperature (Domain_d::setTemp), or one value per element"""
+        a = np.asarray(T, dtype=np.float64)
+        if a.ndim == 0:
+            check(self._L.wfx_set_elem_temperature(self._h, None, 0))
+            check(self._L.wfx_set_temperature(self._h, float(a)))
+        else:
+            a = np.ascontiguousarray(a.reshape(-1))
+            check(self._L.wfx_set_elem_temperature(self._h, a.ctypes.data, a.size))
+
+    def setMaxStrainRate(self, edot):
+        """Domain_d::m_max_edot: cap of the rate the tangent modulus sees"""
+        check(self._L.wfx_set_max_strain_rate(self._h, float(edot)))
 
     def setElemSy0(self, sy0):
         a = np.ascontiguousarray(sy0, dtype=np.float64)
