@@ -98,6 +98,25 @@ int wfx_add_bc_vel_node(wfx_ctx *ctx, int node, int dim, double val);
 int wfx_allocate_bcs(wfx_ctx *ctx);
 /* Domain_d::SetDT (Domain_d.h) */
 int wfx_set_dt(wfx_ctx *ctx, double dt);
+/* Domain_d::setCFL + setFixedDt(false) (main.C:263-266): cfl_factor > 0
+ * recomputes dt every step as in the BUILD_REMESH CPU solver
+ * (Solver_explicit.C:576-600): after the prediction and the Jacobians,
+ * dt = cfl_factor * min edge / (sqrt(K/rho[0]) + max |v|); the step's stress
+ * update and correction and the next prediction use it.  The dt of
+ * wfx_set_dt is then the first prediction's.  0 (default) keeps dt fixed.
+ * Runs synchronise at their end to bring dt back to the host. */
+int wfx_set_cfl(wfx_ctx *ctx, double cfl_factor);
+/* the per-step CFL time step of the current state, as SolveChungHulbert
+ * computes it after calcElemJAndDerivatives (Solver_explicit.C:576-600);
+ * becomes the dt of the following per-step calls (wfx_run does this itself).
+ * Synchronises. */
+int wfx_calc_cfl_dt(wfx_ctx *ctx, double *dt, wfx_stream stream);
+/* Domain_d::getDT: the current dt (with the per-step CFL time step, the last
+ * step's) */
+int wfx_get_dt(wfx_ctx *ctx, double *dt);
+/* the dt of each step of the last wfx_run / wfx_step_fused / wfx_graph_launch
+ * with the per-step CFL time step (count <= its steps) */
+int wfx_dt_history(wfx_ctx *ctx, double *out, int count);
 /* m_alpha, m_beta, m_gamma (Solver_explicit.C:193-197); defaults rho_b=0.8182 */
 int wfx_set_ch(wfx_ctx *ctx, double alpha, double beta, double gamma);
 /* Domain_d::m_stab and m_artifvisc (Domain_d.h:283-294, main.C:84-120) and the

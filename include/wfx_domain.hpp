@@ -79,6 +79,9 @@ class Domain_d {
   }
   void AllocateBCs() { check(wfx_allocate_bcs(c_)); }
   void SetDT(const double &dt) { dt_ = dt; check(wfx_set_dt(c_, dt)); }
+  void setCFL(const double &cfl) { cfl_ = cfl; check(wfx_set_cfl(c_, cfl)); }
+  // the per-step CFL time step of the current state (Solver_explicit.C:576-600)
+  double calcCflDt() { check(wfx_calc_cfl_dt(c_, &dt_, s_)); return dt_; }
   void SetEndTime(const double &t) { end_t_ = t; }
   void setParam(const char *name, double v) { check(wfx_set_param(c_, name, v)); }
   void setReactionNodes(const std::vector<int> &nodes) {
@@ -140,6 +143,7 @@ class Domain_d {
     UpdatePrediction();
     ImposeBCVAllDim();
     calcElemJAndDerivatives();
+    if (cfl_ > 0.0) calcCflDt();
     CalcElemVol();
     CalcNodalVol();
     CalcNodalMassFromVol();
@@ -164,10 +168,12 @@ class Domain_d {
     double Time = 0.0;
     int steps = 0;
     while (Time < end_t_) {
-      if (fused)
+      if (fused) {
         check(wfx_step_fused(c_, 1, nullptr, s_));
-      else
+        check(wfx_get_dt(c_, &dt_));
+      } else {
         StepChungHulbert();
+      }
       Time += dt_;
       ++steps;
     }
@@ -187,7 +193,7 @@ class Domain_d {
  private:
   wfx_ctx *c_ = nullptr;
   wfx_stream s_ = nullptr;
-  double dt_ = 0.0, end_t_ = 0.0;
+  double dt_ = 0.0, end_t_ = 0.0, cfl_ = 0.0;
 };
 
 }  // namespace wfx

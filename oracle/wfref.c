@@ -915,10 +915,27 @@ double wfref_reaction_force(wfref_domain *d, int dim) {
 
 /* One pass of the SolveChungHulbert loop body, CPU branch, in its order
  * (Solver_explicit.C:524-978; remesh, contact, thermal and I/O left out). */
+/* Per-step time step of the BUILD_REMESH CPU solver (Solver_explicit.C:576-600):
+ * after the prediction and the Jacobians, dt = cfl * min edge / (c + max |v|)
+ * with c = sqrt(K / rho[0]) and |v| the predicted nodal speed; the step's
+ * stress update and correction use it, the next step's prediction too. */
+static void cfl_time_step(wfref_domain *d) {
+  double mat_cs = sqrt(d->mat.K / d->rho[0]);
+  double minl = wfref_calc_min_edge_length(d);
+  double max_vel = 0.0;
+  for (int i = 0; i < d->node_count; ++i) {
+    const double *v = d->v + 3 * i;
+    double nv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (nv > max_vel) max_vel = nv;
+  }
+  d->dt = d->cfl * minl / (mat_cs + max_vel);
+}
+
 void wfref_step(wfref_domain *d, double *reaction_z) {
   wfref_update_prediction(d);
   for (int c = 0; c < 3; ++c) wfref_impose_bcv(d, c);
   wfref_calc_elem_jacobians(d);
+  if (d->cfl > 0.0) cfl_time_step(d);
   wfref_calc_elem_vol(d);
   wfref_calc_nodal_vol(d);
   wfref_calc_nodal_mass_from_vol(d);
@@ -1016,7 +1033,7 @@ int wfref_set_double(wfref_domain *d, const char *name, double v) {
       {"pspg_scale", &d->pspg_scale}, {"p_pspg_bulkfac", &d->p_pspg_bulkfac},
       {"av_coeff_div", &d->av_coeff_div}, {"av_coeff_bulk", &d->av_coeff_bulk},
       {"artifvisc0", &d->artifvisc[0]}, {"artifvisc1", &d->artifvisc[1]},
-      {"hg_coeff", &d->hg_coeff}, {"max_edot", &d->max_edot}};
+      {"hg_coeff", &d->hg_coeff}, {"max_edot", &d->max_edot}, {"cfl", &d->cfl}};
   for (size_t k = 0; k < sizeof(t) / sizeof(t[0]); ++k)
     if (strcmp(t[k].nm, name) == 0) { *t[k].p = v; return 0; }
   return -1;

@@ -63,6 +63,7 @@ typedef struct {
   double *f_elem, *f_hg;                             /* 3nE, [e][a][d] */
   double *temp;       /* element temperature, E (IniTemp 20, main.C:424) */
   double max_edot;    /* strain-rate cap of the tangent (Domain_d.h:824) */
+  double cfl;         /* > 0: per-step dt = cfl*minl/(cs+max|v|); 0: fixed */
   /* material (one shared law; per-element initial yield in sy0[]) */
   wfref_material mat;
   /* Dirichlet velocity lists per dimension (AddBCVelNode order) */

@@ -141,6 +141,13 @@ class RefDomain:
         t = np.broadcast_to(np.asarray(T, dtype=np.float64), (self.elem_count,))
         self.setField("temp", np.ascontiguousarray(t))
 
+    def setCFL(self, cfl_factor):
+        """per-step CFL time step (0: fixed dt)"""
+        self.set("cfl", cfl_factor)
+
+    def getDT(self):
+        return self._L.wfref_get_double(self._d, b"dt")
+
     def setMaxStrainRate(self, edot):
         self.set("max_edot", edot)
 

@@ -1,0 +1,164 @@
+"""Per-step CFL time step on the GPU against the oracle.
+
+The reference recomputes dt every step in its CPU solver
+(Solver_explicit.C:576-600).  The per-step path computes it on the device and
+brings it back to the host; the fused path keeps it on the device (minimum
+edge length in the volume pass, largest predicted speed in the node pass).
+FP64: the dt sequence equals the oracle's (relative 1e-14: the device sqrt
+and min are exact, the operation order is the oracle's) and the state stays
+within 1e-10 after 100 steps.  FP32-mixed: dt within 1e-6.
+"""
+import numpy as np
+import pytest
+
+from oracle.wfref import RefDomain
+from weldformfem_amd import Domain, WFX_FP32_MIXED, WFX_FP64, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def _hexa20():
+    r = RefDomain()
+    r.AddBoxLength([0, 0, 0], [1.0, 1.0, 1.0], 0.025)
+    bottom, top = workloads.box_layers(20, 20, 20)
+    return r.field("x"), r.field("elnod"), 8, bottom, top
+
+
+def _cyl16():
+    x, el, bottom, top = workloads.cylinder_tets(cells=16)
+    return x, el, 4, bottom, top
+
+
+def rel(a, b):
+    s = np.max(np.abs(b))
+    return np.max(np.abs(a - b)) / (s if s > 0 else 1.0)
+
+
+def _pair(mesh, precision=WFX_FP64, cfl=0.25, v_top=-20.0, stream=0):
+    x, el, n, bottom, top = mesh()
+    g = Domain(precision, stream=stream)
+    g.CreateFromMesh(x, el, n)
+    dt = workloads.setup_compression(g, bottom, top, v_top)
+    r = RefDomain()
+    r.CreateFromMesh(x, el, n)
+    workloads.setup_compression(r, bottom, top, v_top, dt=dt)
+    for d in (g, r):
+        d.setCFL(cfl)
+        d.init()
+    return g, r
+
+
+def oracle_steps(r, nsteps):
+    hist, dts = [], []
+    for _ in range(nsteps):
+        hist.append(r.run(1)[0])
+        dts.append(r.getDT())
+    return np.array(hist), np.array(dts)
+
+
+@pytest.mark.parametrize("mesh", [_hexa20, _cyl16])
+def test_per_step_path_fp64(mesh):
+    say(f"case: per-step API with per-step CFL dt, FP64, {mesh.__name__}")
+    g, r = _pair(mesh)
+    rg = g.run(100)
+    dg = g.dtHistory(100)
+    rr, dr = oracle_steps(r, 100)
+    np.testing.assert_allclose(dg, dr, rtol=1e-14, atol=0)
+    assert g.getDT() == pytest.approx(dr[-1], rel=1e-14)
+    for f in ["u", "v", "x", "pl_strain"]:
+        e = rel(g.field(f), r.field(f))
+        assert e <= 1e-10, f"{f}: {e:.3e}"
+    assert rel(rg, rr) <= 1e-10
+
+
+@pytest.mark.parametrize("mesh", [_hexa20, _cyl16])
+def test_fused_path_fp64(mesh):
+    say(f"case: fused path with per-step CFL dt, FP64, {mesh.__name__}")
+    g, r = _pair(mesh)
+    dts, hist = [], []
+    for k in (1, 30, 69):  # the dt of a run's last step carries into the next run
+        hist.append(g.runFused(k))
+        dts.append(g.dtHistory(k))
+    rg, dg = np.concatenate(hist), np.concatenate(dts)
+    rr, dr = oracle_steps(r, 100)
+    np.testing.assert_allclose(dg, dr, rtol=1e-14, atol=0)
+    assert g.getDT() == pytest.approx(dr[-1], rel=1e-14)
+    for f in ["u", "v", "x", "pl_strain", "p", "tau"]:
+        e = rel(g.field(f), r.field(f))
+        assert e <= 1e-10, f"{f}: {e:.3e}"
+    assert rel(rg, rr) <= 1e-10
+    assert g.nonfiniteCount() == 0
+
+
+def test_fused_mixed_with_per_step_api():
+    # fused runs and per-step API steps alternate on one dt sequence
+    say("case: per-step CFL dt, fused and per-step API steps interleaved")
+    g, r = _pair(_hexa20)
+    g.runFused(20)
+    g.run(10)
+    g.runFused(15)
+    _, dr = oracle_steps(r, 45)
+    assert g.getDT() == pytest.approx(dr[-1], rel=1e-14)
+    for f in ["u", "v", "pl_strain"]:
+        e = rel(g.field(f), r.field(f))
+        assert e <= 1e-10, f"{f}: {e:.3e}"
+
+
+def test_graph_replays_with_device_dt():
+    say("case: captured fused graph with per-step CFL dt, two replays vs eager")
+    import torch
+
+    stream = torch.cuda.Stream()
+    g, r = _pair(_hexa20, stream=stream.cuda_stream)
+    e, _ = _pair(_hexa20)
+    g.graphCapture(25)
+    h1 = g.graphLaunch()
+    d1 = g.dtHistory(25)
+    torch.cuda.synchronize()
+    h2 = g.graphLaunch()  # dt changed between the replays: the graph reads it on the device
+    d2 = g.dtHistory(25)
+    torch.cuda.synchronize()
+    he = np.concatenate([e.runFused(25), e.runFused(25)])
+    np.testing.assert_array_equal(np.concatenate([h1, h2]), he)
+    np.testing.assert_array_equal(g.field("u"), e.field("u"))
+    _, dr = oracle_steps(r, 50)
+    np.testing.assert_allclose(np.concatenate([d1, d2]), dr, rtol=1e-14, atol=0)
+
+
+def test_fused_fp32_mixed():
+    say("case: fused FP32-mixed with per-step CFL dt vs FP64 oracle")
+    g, r = _pair(_hexa20, precision=WFX_FP32_MIXED)
+    rg = g.runFused(100)
+    dg = g.dtHistory(100)
+    rr, dr = oracle_steps(r, 100)
+    ed, eu, er = rel(dg, dr), rel(g.field("u"), r.field("u")), rel(rg, rr)
+    say(f"  FP32-mixed rel err: dt {ed:.3e} u {eu:.3e} reaction {er:.3e}")
+    assert ed <= 1e-6 and eu <= 1e-4 and er <= 1e-3
+
+
+def test_calc_cfl_dt_entry_point():
+    say("case: wfx_calc_cfl_dt after a prediction")
+    g, r = _pair(_cyl16)
+    g.run(5)
+    oracle_steps(r, 5)
+    for d in (g, r):
+        d.UpdatePrediction()
+        for k in range(3):
+            d.ImposeBCV(k)
+    dt = g.calcCflDt()
+    want = 0.25 * r.calcMinEdgeLength() / (np.sqrt(r.get("K") / r.field("rho")[0]) +
+                                          np.max(np.linalg.norm(r.field("v").reshape(-1, 3), axis=1)))
+    assert dt == pytest.approx(want, rel=1e-14)
+    assert g.getDT() == dt
+
+
+def test_adjoint_rejects_per_step_dt():
+    from weldformfem_amd._lib import WfxError
+
+    g, _ = _pair(_hexa20)
+    with pytest.raises(WfxError, match="fixed time step"):
+        g.adjointForward(10, checkpoint_every=5)

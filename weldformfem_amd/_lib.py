@@ -37,6 +37,10 @@ SIGNATURES = {
     "wfx_set_temperature": (_i, [_vp, _d]),
     "wfx_set_elem_temperature": (_i, [_vp, _vp, _i]),
     "wfx_set_max_strain_rate": (_i, [_vp, _d]),
+    "wfx_set_cfl": (_i, [_vp, _d]),
+    "wfx_get_dt": (_i, [_vp, _vp]),
+    "wfx_calc_cfl_dt": (_i, [_vp, _vp, _vp]),
+    "wfx_dt_history": (_i, [_vp, _vp, _i]),
     "wfx_add_bc_vel_node": (_i, [_vp, _i, _i, _d]),
     "wfx_allocate_bcs": (_i, [_vp]),
     "wfx_set_dt": (_i, [_vp, _d]),

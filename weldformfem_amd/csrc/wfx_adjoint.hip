@@ -340,6 +340,7 @@ template <> DevElem<float> &el_of<float>(wfx_ctx *c) { return c->e32; }
 static int check_supported(wfx_ctx *c) {
   const Stab &s = c->prm.stab;
   REQUIRE(c->prm.mat.model == WFX_BILINEAR, WFX_ERR_STATE, "the adjoint supports the bilinear material");
+  REQUIRE(!(c->prm.cfl > 0.0), WFX_ERR_STATE, "the adjoint needs a fixed time step (wfx_set_cfl(ctx, 0))");
   REQUIRE(s.alpha_free == 0.0 && s.log_factor == 0.0 && s.J_min == 0.0 && s.hg_coeff_free == 0.0 &&
               s.pspg_scale == 0.0 && s.p_pspg_bulkfac == 0.0 && s.av_coeff_div == 0.0 && s.av_coeff_bulk == 0.0 &&
               s.artifvisc0 == 0.0 && s.artifvisc1 == 0.0,
@@ -447,7 +448,7 @@ static hipError_t fused_step(wfx_ctx *c, int predict, double *rec, hipStream_t s
                        el.pl_strain, rec);
   }
   for (int ph = 1; ph <= 4; ++ph) {
-    hipError_t e = launch_fused_phase<T>(c, ph, predict, st);
+    hipError_t e = launch_fused_phase<T>(c, ph, predict, st, -1);
     if (e != hipSuccess) return e;
   }
   if (rec) {
@@ -476,7 +477,7 @@ static int adjoint_forward(wfx_ctx *c, int nsteps, bool want_hist, hipStream_t s
     }
   }
   LAUNCH(launch_fused_unpack<T>(c, st));
-  LAUNCH(launch_fused_phase<T>(c, 5, 0, st));
+  LAUNCH(launch_fused_phase<T>(c, 5, 0, st, -1));
   LAUNCH(launch_copy(c->nd.prev_a, c->nd.a, 3L * c->N, st));
   for (size_t k = 0; k < A->final_bufs.size(); ++k)
     WFX_CHECK_HIP(hipMemcpyAsync(A->final_saved[k], A->final_bufs[k].first, A->final_bufs[k].second,

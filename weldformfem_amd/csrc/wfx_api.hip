@@ -60,8 +60,13 @@ static void free_all(wfx_ctx *c) {
   c->fused_stale = false;
   c->scalars = nullptr;
   c->status = nullptr;
+  c->cfl_part = nullptr;
+  c->cfl_cap_min = c->cfl_nvmax = 0;
   c->hist = nullptr;
   c->hist_cap = 0;
+  c->dt_hist = nullptr;
+  c->dt_cap = 0;
+  c->dt_hist_h.clear();
 }
 
 template <typename T>
@@ -92,6 +97,8 @@ static int setup_device(wfx_ctx *c) {
   A(c->ms.elnod, n * E) A(c->ms.nodel, n * E) A(c->ms.nodel_loc, n * E) A(c->ms.nodel_offset, N)
   A(c->ms.nodel_count, N) A(c->ms.nodel_pk, n * E) A(c->ms.nodel_pad, 8 * N) A(c->bc_slot, N)
   A(c->scalars, 64) A(c->status, 16)
+  c->cfl_cap_min = (int)((E + 63) / 64 + 1);  // blocks of the element passes at block size >= 64
+  A(c->cfl_part, c->cfl_cap_min + (N + 63) / 64 + 1)
   if (e == hipSuccess) e = (c->precision == WFX_FP64) ? alloc_elem(c, c->e64) : alloc_elem(c, c->e32);
   if (c->precision == WFX_FP32_MIXED) {
     A(c->v32, 3 * N) A(c->pa32, 3 * N) A(c->ud32, 3 * N) A(c->fi32, 3 * N) A(c->x_start, 3 * N)
@@ -125,6 +132,7 @@ static int setup_device(wfx_ctx *c) {
     for (int j = 0; j < std::min(8, c->nodel_count_h[i]); ++j) pad[j * N + i] = pk[c->nodel_offset_h[i] + j];
   WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_pad, pad.data(), 8 * N * sizeof(int), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemset(c->bc_slot, 0xff, N * sizeof(int)));  // -1: no boundary condition
+  WFX_CHECK_HIP(hipMemcpy(c->scalars + kCflDt, &c->prm.dt, sizeof(double), hipMemcpyHostToDevice));
   c->has_mesh = true;
   c->initialized = false;
   return WFX_OK;
@@ -554,6 +562,30 @@ int wfx_set_dt(wfx_ctx *c, double dt) {
   REQUIRE_CTX(c);
   REQUIRE(dt > 0.0 && std::isfinite(dt), WFX_ERR_ARG, "dt must be positive");
   c->prm.dt = dt;
+  // the device copy the first prediction of a per-step-CFL run reads
+  if (c->scalars) WFX_CHECK_HIP(hipMemcpy(c->scalars + kCflDt, &dt, sizeof(double), hipMemcpyHostToDevice));
+  return WFX_OK;
+}
+
+int wfx_set_cfl(wfx_ctx *c, double cfl_factor) {
+  REQUIRE_CTX(c);
+  REQUIRE(cfl_factor >= 0.0 && std::isfinite(cfl_factor), WFX_ERR_ARG, "CFL factor must be >= 0");
+  c->prm.cfl = cfl_factor;
+  return WFX_OK;
+}
+
+int wfx_get_dt(wfx_ctx *c, double *dt) {
+  REQUIRE_CTX(c);
+  REQUIRE(dt, WFX_ERR_ARG, "null output");
+  *dt = c->prm.dt;
+  return WFX_OK;
+}
+
+int wfx_dt_history(wfx_ctx *c, double *out, int count) {
+  REQUIRE_CTX(c);
+  REQUIRE(out && count >= 0 && (size_t)count <= c->dt_hist_h.size(), WFX_ERR_ARG,
+          "count exceeds the steps of the last run with a per-step CFL time step");
+  std::copy(c->dt_hist_h.begin(), c->dt_hist_h.begin() + count, out);
   return WFX_OK;
 }
 
@@ -748,10 +780,37 @@ int wfx_impose_bca(wfx_ctx *c, int dim, wfx_stream s) {
 
 // One SolveChungHulbert loop body (Solver_explicit.C:524-978, CPU order).
 // When hist is given, the reaction force after assembly goes to *hist.
+// per-step CFL time step (Solver_explicit.C:576-600) from the current
+// positions and velocities; the per-step kernels take dt by value, so it
+// comes back to the host
+static int cfl_step(wfx_ctx *c, hipStream_t st) {
+  LAUNCH(BY_PRECISION(c, launch_cfl_step, c, (int)c->dt_hist_h.size(), st));
+  double dt = 0.0;
+  WFX_CHECK_HIP(hipMemcpyAsync(&dt, c->scalars + kCflDt, sizeof(double), hipMemcpyDeviceToHost, st));
+  WFX_CHECK_HIP(hipStreamSynchronize(st));
+  c->prm.dt = dt;
+  c->dt_hist_h.push_back(dt);
+  return WFX_OK;
+}
+
+int wfx_calc_cfl_dt(wfx_ctx *c, double *dt, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->initialized, WFX_ERR_STATE, "call wfx_init first");
+  REQUIRE(c->prm.cfl > 0.0, WFX_ERR_STATE, "no CFL factor: call wfx_set_cfl");
+  int rc = cfl_step(c, as_stream(s));
+  if (rc) return rc;
+  if (dt) *dt = c->prm.dt;
+  return WFX_OK;
+}
+
 static int step_api(wfx_ctx *c, hipStream_t st, double *hist) {
   LAUNCH(launch_update_prediction(c, st));
   for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
   LAUNCH(BY_PRECISION(c, launch_elem_jacobians, c, st));
+  if (c->prm.cfl > 0.0) {
+    int rc = cfl_step(c, st);
+    if (rc) return rc;
+  }
   LAUNCH(BY_PRECISION(c, launch_elem_vol, c, st));
   LAUNCH(BY_PRECISION(c, launch_nodal_vol, c, st));
   LAUNCH(BY_PRECISION(c, launch_nodal_mass, c, st));
@@ -769,6 +828,26 @@ static int step_api(wfx_ctx *c, hipStream_t st, double *hist) {
   for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
   LAUNCH(launch_correction_pos(c, st));
   c->fused_stale = false;  // every per-step intermediate is current again
+  return WFX_OK;
+}
+
+// device per-step dt record of a fused run with the per-step CFL time step
+static int ensure_dt_hist(wfx_ctx *c, int nsteps) {
+  if (c->prm.cfl > 0.0 && c->dt_cap < nsteps) {
+    WFX_CHECK_HIP(dalloc(c, &c->dt_hist, nsteps));
+    c->dt_cap = nsteps;
+  }
+  return WFX_OK;
+}
+
+// after a fused run with the per-step CFL time step: the host's dt and the
+// run's dt record follow the device (synchronises)
+static int sync_cfl_dt(wfx_ctx *c, int nsteps, hipStream_t st) {
+  if (!(c->prm.cfl > 0.0)) return WFX_OK;
+  c->dt_hist_h.assign(nsteps, 0.0);
+  WFX_CHECK_HIP(hipMemcpyAsync(c->dt_hist_h.data(), c->dt_hist, nsteps * sizeof(double), hipMemcpyDeviceToHost, st));
+  WFX_CHECK_HIP(hipMemcpyAsync(&c->prm.dt, c->scalars + kCflDt, sizeof(double), hipMemcpyDeviceToHost, st));
+  WFX_CHECK_HIP(hipStreamSynchronize(st));
   return WFX_OK;
 }
 
@@ -800,6 +879,7 @@ int wfx_run(wfx_ctx *c, int nsteps, double *reaction_hist, wfx_stream s) {
     if (rc) return rc;
   }
   const int ng = c->ngroups > 0 ? c->ngroups : 1;
+  c->dt_hist_h.clear();
   for (int k = 0; k < nsteps; ++k) {
     int rc = step_api(c, st, dh ? dh + (long)k * ng : nullptr);
     if (rc) return rc;
@@ -824,11 +904,15 @@ int wfx_step_fused(wfx_ctx *c, int nsteps, double *reaction_hist, wfx_stream s) 
     int rc = ensure_hist(c, nsteps, &dh);
     if (rc) return rc;
   }
+  int rc = ensure_dt_hist(c, nsteps);
+  if (rc) return rc;
   // V0_n (vol0 may have been replaced through wfx_set_field), then the first
   // prediction; each fused step predicts the next one in its node pass.
-  int rc = enqueue_fused(c, nsteps, dh, st);
+  rc = enqueue_fused(c, nsteps, dh, st);
   if (rc) return rc;
   c->fused_stale = true;
+  rc = sync_cfl_dt(c, nsteps, st);
+  if (rc) return rc;
   if (dh) {
     const int ng = c->ngroups > 0 ? c->ngroups : 1;
     WFX_CHECK_HIP(hipMemcpyAsync(reaction_hist, dh, (size_t)nsteps * ng * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -903,9 +987,9 @@ int wfx_autotune(wfx_ctx *c, int reps, int *chosen, wfx_stream s) {
     for (int round = 0; round < 3 && err == hipSuccess; ++round)
       for (int k = 0; k < 4 && err == hipSuccess; ++k) {
         c->fused_bs[ph] = sizes[k];
-        err = BY_PRECISION(c, launch_fused_phase, c, ph, 1, st);  // warm-up
+        err = BY_PRECISION(c, launch_fused_phase, c, ph, 1, st, -1);  // warm-up
         if (err == hipSuccess) err = hipEventRecord(e0, st);
-        for (int r = 0; r < reps && err == hipSuccess; ++r) err = BY_PRECISION(c, launch_fused_phase, c, ph, 1, st);
+        for (int r = 0; r < reps && err == hipSuccess; ++r) err = BY_PRECISION(c, launch_fused_phase, c, ph, 1, st, -1);
         if (err == hipSuccess) err = hipEventRecord(e1, st);
         if (err == hipSuccess) err = hipEventSynchronize(e1);
         float ms = 0.f;
@@ -951,11 +1035,14 @@ static uint64_t graph_signature(wfx_ctx *c) {
     const unsigned char *b = (const unsigned char *)p;
     for (size_t k = 0; k < n; ++k) h = (h ^ b[k]) * 1099511628211ULL;
   };
-  mix(&c->prm.dt, sizeof(double) * 4);
+  // with the per-step CFL time step the graph reads dt from the device
+  const double dts[5] = {c->prm.cfl > 0.0 ? 0.0 : c->prm.dt, c->prm.alpha, c->prm.beta, c->prm.gamma,
+                         c->prm.cfl};
+  mix(dts, sizeof(dts));
   mix(&c->prm.mat, sizeof(c->prm.mat));
   mix(&c->prm.stab, sizeof(c->prm.stab));
   const void *ptrs[] = {c->bc_nod[0], c->bc_nod[1], c->bc_nod[2], c->bc_slot, c->bc_mask, c->bc_tab,
-                        c->react, c->hist, c->e64.He, c->e32.He, c->temp_e};
+                        c->react, c->hist, c->e64.He, c->e32.He, c->temp_e, c->dt_hist};
   mix(ptrs, sizeof(ptrs));
   int ints[] = {c->N, c->E, c->n, c->has_fe, c->bc_count[0], c->bc_count[1], c->bc_count[2],
                 (int)c->react_h.size(), c->fused_bs[1], c->fused_bs[2], c->fused_bs[3], c->fused_bs[4]};
@@ -986,6 +1073,10 @@ int wfx_graph_capture(wfx_ctx *c, int nsteps, int with_reaction, wfx_stream s) {
     int rc = ensure_hist(c, nsteps, &dh);
     if (rc) return rc;
   }
+  {
+    int rc = ensure_dt_hist(c, nsteps);
+    if (rc) return rc;
+  }
   WFX_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   int rc = enqueue_fused(c, nsteps, dh, st);
   hipGraph_t g = nullptr;
@@ -1008,24 +1099,24 @@ int wfx_graph_launch(wfx_ctx *c, double *reaction_hist, wfx_stream s) {
   REQUIRE(c->graph_exec, WFX_ERR_STATE, "no captured graph: call wfx_graph_capture");
   REQUIRE(c->graph_sig == graph_signature(c), WFX_ERR_STATE,
           "parameters, boundary conditions or block sizes changed since wfx_graph_capture: capture again");
+  REQUIRE(!reaction_hist || c->graph_hist, WFX_ERR_STATE, "graph was captured without the reaction history");
   hipStream_t st = as_stream(s);
   WFX_CHECK_HIP(hipGraphLaunch(c->graph_exec, st));
   c->fused_stale = true;
   if (reaction_hist) {
-    REQUIRE(c->graph_hist, WFX_ERR_STATE, "graph was captured without the reaction history");
     const int ng = c->ngroups > 0 ? c->ngroups : 1;
     WFX_CHECK_HIP(hipMemcpyAsync(reaction_hist, c->hist, (size_t)c->graph_steps * ng * sizeof(double),
                                  hipMemcpyDeviceToHost, st));
     WFX_CHECK_HIP(hipStreamSynchronize(st));
   }
-  return WFX_OK;
+  return sync_cfl_dt(c, c->graph_steps, st);
 }
 
 int wfx_fused_phase(wfx_ctx *c, int phase, int predict, wfx_stream s) {
   REQUIRE_MESH(c);
   REQUIRE(c->initialized, WFX_ERR_STATE, "call wfx_init first");
   REQUIRE(phase >= 1 && phase <= 5, WFX_ERR_ARG, "phase must be 1..5");
-  LAUNCH(BY_PRECISION(c, launch_fused_phase, c, phase, predict, as_stream(s)));
+  LAUNCH(BY_PRECISION(c, launch_fused_phase, c, phase, predict, as_stream(s), -1));
   c->fused_stale = true;
   return WFX_OK;
 }

@@ -86,13 +86,15 @@ __global__ __launch_bounds__(kBlock) void f0_vsum0(int N, const int *__restrict_
   vsum0[i] = s0;
 }
 
-// F1: vol = detJ * w (calcElemJAndDerivatives determinant + CalcElemVol)
+// F1: vol = detJ * w (calcElemJAndDerivatives determinant + CalcElemVol);
+// with minl_part also the block's minimum edge length (per-step CFL time step)
 template <typename T, int NN, int BS>
 __global__ __launch_bounds__(BS) void f1_volume(int E, int N, const int *__restrict__ elnod,
                                                     const double *__restrict__ x,
-                                                    T *__restrict__ vol) {
+                                                    T *__restrict__ vol, double *__restrict__ minl_part) {
   int e = gtid();
-  if (e >= E) return;
+  double medge = kMinEdgeInit;
+  if (e < E) {
   T X[NN][3];
   if constexpr (std::is_same<T, double>::value) {
 #pragma unroll
@@ -135,6 +137,12 @@ __global__ __launch_bounds__(BS) void f1_volume(int E, int N, const int *__restr
   T v = T(0);
   v += det * w * T(1);
   vol[e] = v;
+  if (minl_part) medge = min_edge4<T>(*reinterpret_cast<const T(*)[4][3]>(&X[0][0]));
+  }
+  if (minl_part) {
+    medge = block_extreme<BS, true>(medge);
+    if (threadIdx.x == 0) minl_part[blockIdx.x] = medge;
+  }
 }
 
 // F2: nodal volume sum in CSR order; ratio for the ANP pressure, nodal
@@ -208,6 +216,7 @@ struct FusedParams {
   int need_jlocal;    // alpha_free or hg_coeff_free nonzero: J_local needed
   double hg_coeff, cs0;
   double rho_u;       // > 0: every element density equals it (no density loads)
+  const double *dtp;  // non-null: the step's dt from the device (per-step CFL time step)
 };
 
 // dH(d,a) = adj(J)(d,:) . dN_a/dxi from the cofactor matrix C (adj = C^T),
@@ -240,7 +249,7 @@ __device__ __forceinline__ T shape_dH(const T (&C)[3][3], int d, int a) {
 
 // F3: the element chain of one step for element e.  The nodal forces of the
 // element (f - f_hg, what the assembly adds to each node) go to sink(a, fx, fy, fz).
-template <typename T, int NN, class Sink>
+template <typename T, int NN, int LAW, class Sink>
 __device__ __forceinline__ void element_chain(
     int e, int E, int N, const FusedParams &fp, const int *__restrict__ elnod, const double *__restrict__ x,
     const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
@@ -385,7 +394,7 @@ __device__ __forceinline__ void element_chain(
   p_out[e] = pe;
 
   // ---- Jaumann update and J2 return (CalcStressStrain) -------------------
-  T dt = (T)fp.dt;
+  T dt = (T)(fp.dtp ? *fp.dtp : fp.dt);
   const MatParams &mp = fp.mp;
   T3<T> tau = t3_from_sym(ld_stream(&tau_io[0 * Es + e]), ld_stream(&tau_io[1 * Es + e]),
                           ld_stream(&tau_io[2 * Es + e]), ld_stream(&tau_io[3 * Es + e]),
@@ -412,17 +421,17 @@ __device__ __forceinline__ void element_chain(
   T ep = ld_stream(&pl_strain[e]);
   // bilinear plastic modulus: per element when given (specimen batches)
   T sy, Hb = (T)mp.H;
-  const bool per_elem = mp.model == BILINEAR && He;
-  const double edot = (mp.model >= JOHNSON_COOK) ? eff_strain_rate(D) : 0.0;
+  const bool per_elem = LAW == LAW_PLAIN && mp.model == BILINEAR && He;
+  const double edot = (LAW == LAW_RATE) ? eff_strain_rate(D) : 0.0;
   if (per_elem) {
     Hb = ld_stream(&He[e]);
     sy = ld_stream(&sy0[e]) + Hb * ep;
   } else {
-    sy = yield_stress<T>(mp, e, ld_stream(&sy0[e]), ep, edot);
+    sy = yield_stress<T, LAW>(mp, e, ld_stream(&sy0[e]), ep, edot);
   }
   sigma_y[e] = sy;
   if (sy < sig_trial) {
-    T H = per_elem ? Hb : tangent_modulus<T>(mp, e, ep, cap_rate(mp, edot));
+    T H = per_elem ? Hb : tangent_modulus<T, LAW>(mp, e, ep, cap_rate(mp, edot));
     T dgamma = (sig_trial - sy) / (T(3) * G + H);
     T factor = T(1) - (T(3) * G * dgamma) / sig_trial;
     tau = t3_scale(s, factor);
@@ -473,7 +482,7 @@ __device__ __forceinline__ void element_chain(
   }
 }
 
-template <typename T, int NN, int BS>
+template <typename T, int NN, int BS, int LAW>
 __global__ __launch_bounds__(BS) void f3_element(
     int E, int N, FusedParams fp, const int *__restrict__ elnod, const double *__restrict__ x,
     const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
@@ -483,7 +492,7 @@ __global__ __launch_bounds__(BS) void f3_element(
   int e = gtid();
   if (e >= E) return;
   const long Es = E;
-  element_chain<T, NN>(e, E, N, fp, elnod, x, v, ratio, vol0, rho, sy0, He, tau_io, eps_io, p_out, pl_strain,
+  element_chain<T, NN, LAW>(e, E, N, fp, elnod, x, v, ratio, vol0, rho, sy0, He, tau_io, eps_io, p_out, pl_strain,
                        sigma_y, [&](int a, T f0, T f1, T f2) {
                          st_stream(&fout[fidx(a * 3 + 0, e, Es, 3 * NN)], f0);
                          st_stream(&fout[fidx(a * 3 + 1, e, Es, 3 * NN)], f1);
@@ -501,7 +510,19 @@ struct NodeParams {
   int predict;     // run the next step's prediction
   int has_fe;      // external forces present
   int nrow;        // rows of the element-force buffer (3n)
+  const double *dtp;  // per-step CFL time step: the step's dt on the device
+  double *vmax_part;  //   and the per-block largest predicted speed
 };
+
+// the step-length terms of NodeParams for step length dt (same expressions
+// as node_params on the host)
+__device__ __forceinline__ void set_step_length(NodeParams &np, double dt) {
+  np.dt = dt;
+  np.gdt = np.gamma * dt;
+  np.bdt2 = np.beta * dt * dt;
+  np.pred_u = (0.5 - np.beta) * dt;
+  np.pred_v = (1.0 - np.gamma) * dt;
+}
 
 // The nodal update of one step for node i given its assembled internal
 // force f: non-finite scrub (Solver_explicit.C:777-785), calcAccel, ImposeBCA,
@@ -511,8 +532,8 @@ struct NodeParams {
 // are stored in float (the update itself runs in double), positions stay
 // double, and u is not touched per step (recovered as x - x_start when the
 // run ends).
-template <typename T>
-__device__ __forceinline__ void node_update(
+template <typename T, bool CFL>
+__device__ __forceinline__ double node_update(
     int i, long Ns, const NodeParams &np, double (&f)[3], const double *__restrict__ fe,
     const double *__restrict__ mdiag, const int *__restrict__ bc_slot, const int *__restrict__ bc_mask,
     const double *__restrict__ bc_val, T *__restrict__ fi, T *__restrict__ v, T *__restrict__ prev_a,
@@ -525,6 +546,7 @@ __device__ __forceinline__ void node_update(
   const int slot = bc_slot[i];
   const int mask = slot >= 0 ? bc_mask[slot] : 0;
   const double m = mdiag[i];
+  double vs[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const long id = d * Ns + i;
@@ -547,7 +569,11 @@ __device__ __forceinline__ void node_update(
     u_dt[id] = (T)ud;
     prev_a[id] = (T)a;
     v[id] = (T)vv;
+    vs[d] = (double)(T)vv;
   }
+  // speed of the predicted velocity, for the next step's CFL time step
+  if constexpr (CFL) return sqrt(vs[0] * vs[0] + vs[1] * vs[1] + vs[2] * vs[2]);
+  return 0.0;
 }
 
 // Sum of the element forces of node i over its CSR entries, in CSR order
@@ -582,8 +608,9 @@ __device__ __forceinline__ void gather_force(int i, long Ns, long Es, int nrow, 
   }
 }
 
-// F4: assembly and the whole nodal update of one step
-template <typename T, int BS>
+// F4: assembly and the whole nodal update of one step; CFL: dt from the
+// device and each block's largest predicted speed for the next step's dt
+template <typename T, int BS, bool CFL>
 __global__ __launch_bounds__(BS) void f4_node(
     int N, int E, NodeParams np, const int *__restrict__ off, const int *__restrict__ cnt,
     const int *__restrict__ nodel_pk, const int *__restrict__ pad, const T *__restrict__ fel,
@@ -592,10 +619,20 @@ __global__ __launch_bounds__(BS) void f4_node(
     T *__restrict__ v, T *__restrict__ prev_a, T *__restrict__ u_dt,
     double *__restrict__ x, double *__restrict__ u, unsigned int *__restrict__ status) {
   int i = gtid();
-  if (i >= N) return;
-  double f[3];
-  gather_force<T>(i, N, E, np.nrow, off, cnt, nodel_pk, pad, fel, f);
-  node_update<T>(i, N, np, f, fe, mdiag, bc_slot, bc_mask, bc_val, fi, v, prev_a, u_dt, x, u, status);
+  NodeParams q = np;
+  if constexpr (CFL) set_step_length(q, *q.dtp);
+  double speed = 0.0;
+  if (i < N) {
+    double f[3];
+    gather_force<T>(i, N, E, q.nrow, off, cnt, nodel_pk, pad, fel, f);
+    speed = node_update<T, CFL>(i, N, q, f, fe, mdiag, bc_slot, bc_mask, bc_val, fi, v, prev_a, u_dt, x, u, status);
+  }
+  if constexpr (CFL) {
+    if (q.predict) {
+      speed = block_extreme<BS, false>(speed);
+      if (threadIdx.x == 0) q.vmax_part[blockIdx.x] = speed;
+    }
+  }
 }
 
 
@@ -678,13 +715,27 @@ inline NodeParams node_params(wfx_ctx *c, int predict) {
   np.predict = predict;
   np.has_fe = c->has_fe;
   np.nrow = 3 * c->n;
+  np.dtp = nullptr;
+  np.vmax_part = nullptr;
   return np;
+}
+
+// dt of step `step` of a run from the partial extremes (per-step CFL time step)
+template <typename T>
+hipError_t launch_cfl_reduce(wfx_ctx *c, int nmin, int step, hipStream_t s) {
+  hipLaunchKernelGGL((k_cfl_reduce<T>), dim3(1), dim3(kCflReduceBlock), 0, s, c->cfl_part, nmin,
+                     c->cfl_part + c->cfl_cap_min, c->cfl_nvmax, elem<T>(c).rho, c->prm.mat.K, c->prm.cfl,
+                     c->scalars, (step >= 0 && step < c->dt_cap) ? c->dt_hist : nullptr, step);
+  return hipGetLastError();
 }
 
 // Launch one phase (1..4) of a fused step on the current state with block
 // size BS; launch_fused_phase picks BS from the context (autotuned or 256).
 template <typename T, int BS>
-hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, hipStream_t s) {
+hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, hipStream_t s) {
+  // per-step CFL time step inside a run; otherwise (step < 0) the host's dt
+  const bool cfl = step >= 0 && c->prm.cfl > 0.0;
+  const double *dtp = cfl ? c->scalars + kCflDt : nullptr;
   auto &el = elem<T>(c);
   const auto &P = c->prm;
   const auto &st = P.stab;
@@ -698,10 +749,12 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, hipStream_t
     vN = c->v32; paN = c->pa32; udN = c->ud32; fiN = c->fi32;
   }
   if (phase == 1) {
+    double *minl = cfl ? c->cfl_part : nullptr;
     if (c->n == 8)
-      hipLaunchKernelGGL((f1_volume<T, 8, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol);
+      hipLaunchKernelGGL((f1_volume<T, 8, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol, minl);
     else
-      hipLaunchKernelGGL((f1_volume<T, 4, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol);
+      hipLaunchKernelGGL((f1_volume<T, 4, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol, minl);
+    if (cfl) return launch_cfl_reduce<T>(c, gE, step, s);
   } else if (phase == 2) {
     hipLaunchKernelGGL((f2_nodal<T, BS>), dim3(gN), dim3(BS), 0, s, N, c->n, c->ms.nodel_offset,
                        c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.vol, el.rho, c->nd.voln0_sum,
@@ -719,18 +772,20 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, hipStream_t
     fp.hg_coeff = (c->n == 8) ? st.hg_coeff : 0.0;
     fp.cs0 = P.mat.cs0;
     fp.rho_u = c->rho_uniform;
-    if (c->n == 8) {
-      hipLaunchKernelGGL((f3_element<T, 8, BS>), dim3(gE), dim3(BS), 0, s, E, N, fp, c->ms.elnod, c->nd.x,
-                         vN, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.He, el.tau, el.eps, el.p,
-                         el.pl_strain, el.sigma_y, el.f_elem);
-    } else {
-      hipLaunchKernelGGL((f3_element<T, 4, BS>), dim3(gE), dim3(BS), 0, s, E, N, fp, c->ms.elnod, c->nd.x,
-                         vN, c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.He, el.tau, el.eps, el.p,
-                         el.pl_strain, el.sigma_y, el.f_elem);
-    }
+    fp.dtp = dtp;
+    const bool rate = law_class(fp.mp.model) == LAW_RATE;
+    auto kern = (c->n == 8) ? (rate ? f3_element<T, 8, BS, LAW_RATE> : f3_element<T, 8, BS, LAW_PLAIN>)
+                            : (rate ? f3_element<T, 4, BS, LAW_RATE> : f3_element<T, 4, BS, LAW_PLAIN>);
+    hipLaunchKernelGGL(kern, dim3(gE), dim3(BS), 0, s, E, N, fp, c->ms.elnod, c->nd.x, vN, c->nd.anp_ratio,
+                       el.vol0, el.rho, el.sy0, el.He, el.tau, el.eps, el.p, el.pl_strain, el.sigma_y,
+                       el.f_elem);
   } else if (phase == 4) {
-    const NodeParams np = node_params(c, predict);
-    hipLaunchKernelGGL((f4_node<T, BS>), dim3(gN), dim3(BS), 0, s, N, E, np, c->ms.nodel_offset,
+    NodeParams np = node_params(c, predict);
+    np.dtp = dtp;
+    np.vmax_part = cfl ? c->cfl_part + c->cfl_cap_min : nullptr;
+    if (cfl && predict) c->cfl_nvmax = gN;
+    auto kern = cfl ? f4_node<T, BS, true> : f4_node<T, BS, false>;
+    hipLaunchKernelGGL(kern, dim3(gN), dim3(BS), 0, s, N, E, np, c->ms.nodel_offset,
                        c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.f_elem, c->nd.fe, c->nd.mdiag,
                        c->bc_slot, c->bc_mask, c->bc_tab, fiN, vN, paN, udN, c->nd.x, c->nd.u, c->status);
   } else if (phase == 5) {
@@ -741,15 +796,37 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, hipStream_t
   return hipGetLastError();
 }
 
+// step >= 0: step index within a run (per-step CFL slots); -1: host dt
 template <typename T>
-hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s) {
+hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s, int step) {
   const int bs = (phase >= 1 && phase <= 4) ? c->fused_bs[phase] : 256;
   switch (bs) {
-    case 64: return launch_fused_phase_bs<T, 64>(c, phase, predict, s);
-    case 128: return launch_fused_phase_bs<T, 128>(c, phase, predict, s);
-    case 512: return launch_fused_phase_bs<T, 512>(c, phase, predict, s);
-    default: return launch_fused_phase_bs<T, 256>(c, phase, predict, s);
+    case 64: return launch_fused_phase_bs<T, 64>(c, phase, predict, step, s);
+    case 128: return launch_fused_phase_bs<T, 128>(c, phase, predict, step, s);
+    case 512: return launch_fused_phase_bs<T, 512>(c, phase, predict, step, s);
+    default: return launch_fused_phase_bs<T, 256>(c, phase, predict, step, s);
   }
+}
+
+// Per-step CFL time step at the start of a run: the predicted velocities'
+// per-block largest speeds for the first step's dt.
+template <typename T>
+hipError_t launch_cfl_begin(wfx_ctx *c, hipStream_t s) {
+  hipLaunchKernelGGL(k_cfl_max_speed, dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N, c->nd.v,
+                     c->cfl_part + c->cfl_cap_min);
+  c->cfl_nvmax = nblocks(c->N);
+  return hipGetLastError();
+}
+
+// The time step of one per-step API step (after the prediction and the
+// Jacobians, Solver_explicit.C:576-600) into the dt slot.
+template <typename T>
+hipError_t launch_cfl_step(wfx_ctx *c, int step, hipStream_t s) {
+  hipError_t e = launch_cfl_begin<T>(c, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cfl_min_edge, dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, c->N, c->ms.elnod, c->nd.x,
+                     c->cfl_part);
+  return launch_cfl_reduce<T>(c, nblocks(c->E), -1, s);
 }
 
 // FP32-mixed runs keep the nodal velocity, previous acceleration and
@@ -780,9 +857,13 @@ hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t 
     hipError_t e = launch_fused_pack<T>(c, s);
     if (e != hipSuccess) return e;
   }
+  if (c->prm.cfl > 0.0) {
+    hipError_t e = launch_cfl_begin<T>(c, s);
+    if (e != hipSuccess) return e;
+  }
   for (int k = 0; k < nsteps; ++k) {
     for (int ph = 1; ph <= 4; ++ph) {
-      hipError_t e = launch_fused_phase<T>(c, ph, k + 1 < nsteps, s);
+      hipError_t e = launch_fused_phase<T>(c, ph, k + 1 < nsteps, s, k);
       if (e != hipSuccess) return e;
     }
     if (hist) {
@@ -795,7 +876,7 @@ hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t 
     hipError_t e = launch_fused_unpack<T>(c, s);
     if (e != hipSuccess) return e;
   }
-  return launch_fused_phase<T>(c, 5, 0, s);
+  return launch_fused_phase<T>(c, 5, 0, s, -1);
 }
 
 }  // namespace wfx

@@ -36,6 +36,11 @@ struct Adjoint;  // checkpointed adjoint run (wfx_adjoint.hip)
 
 enum Model { BILINEAR = 0, HOLLOMON = 1, JOHNSON_COOK = 2, GMT = 3 };
 
+// slots of the per-step CFL time step in the context's block of 64 device
+// scalars: minimum edge length, maximum speed, sound speed sqrt(K / rho[0])
+// and dt of the last step
+constexpr int kCflMinl = 16, kCflVmax = 17, kCflCs = 18, kCflDt = 20;
+
 struct Material {
   int model = BILINEAR;
   double E = 0, nu = 0, K = 0, G = 0, rho0 = 0, cs0 = 0, sy0 = 1.0e10;
@@ -84,6 +89,7 @@ struct DevMesh {
 struct StepParams {
   int N = 0, E = 0, n = 0;
   double dt = 0, alpha = 0, beta = 0, gamma = 0;
+  double cfl = 0;  // > 0: per-step CFL time step with this factor (dt is the initial one)
   Material mat;
   Stab stab;
 };
@@ -137,6 +143,12 @@ struct wfx_ctx {
   bool fused_stale = false;  // per-step intermediates not maintained by the fused path
   double *scalars = nullptr;      // device scratch for reductions (reaction force)
   unsigned int *status = nullptr; // [0] non-finite internal forces scrubbed
+  double *cfl_part = nullptr;     // per-block partial extremes of the CFL time step:
+  int cfl_cap_min = 0;            //   edge minima at [0, cfl_cap_min), speed maxima after
+  int cfl_nvmax = 0;              //   speed maxima left by the last node pass
+  double *dt_hist = nullptr;      // per-step dt of the last run (per-step CFL time step)
+  int dt_cap = 0;
+  std::vector<double> dt_hist_h;  // host copy of the last run's per-step dt
   double *hist = nullptr;         // reaction-force history of wfx_run
   int hist_cap = 0;
   std::vector<void *> allocations;

@@ -8,7 +8,8 @@ namespace wfx {
 
 WFX_DEFINE_LAUNCHERS(double)
 template hipError_t launch_vsum0<double>(wfx_ctx *, hipStream_t);
-template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipStream_t);
+template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipStream_t, int);
+template hipError_t launch_cfl_step<double>(wfx_ctx *, int, hipStream_t);
 template hipError_t launch_fused_steps<double>(wfx_ctx *, int, double *, hipStream_t);
 template hipError_t launch_fused_pack<double>(wfx_ctx *, hipStream_t);
 template hipError_t launch_fused_unpack<double>(wfx_ctx *, hipStream_t);
@@ -16,8 +17,10 @@ template hipError_t launch_fused_unpack<double>(wfx_ctx *, hipStream_t);
 hipError_t launch_update_prediction(wfx_ctx *c, hipStream_t s) {
   long n3 = 3L * c->N;
   const auto &p = c->prm;
+  // with the per-step CFL time step, dt is the device's value of the last step
+  const double *dtp = p.cfl > 0.0 ? c->scalars + kCflDt : nullptr;
   hipLaunchKernelGGL(k_update_prediction, dim3(nblocks(n3)), dim3(kBlock), 0, s, n3, c->nd.v,
-                     c->nd.prev_a, c->nd.u_dt, p.dt, p.beta, p.gamma);
+                     c->nd.prev_a, c->nd.u_dt, p.dt, p.beta, p.gamma, dtp);
   return hipGetLastError();
 }
 

@@ -33,6 +33,105 @@ __device__ __forceinline__ int xcd_block() {
 __device__ __forceinline__ int gtid() { return xcd_block() * blockDim.x + threadIdx.x; }
 
 // ---------------------------------------------------------------------------
+// Per-step CFL time step (Solver_explicit.C:576-600, the BUILD_REMESH CPU
+// solver): dt = cfl * min edge / (sqrt(K / rho[0]) + max |v|), from the
+// positions and the predicted velocities of the step.  The passes that
+// touch the elements and the nodes leave one partial minimum (edge length)
+// or maximum (speed) per block; one small kernel reduces them and writes dt
+// to the context's block of scalars ([kCflDt], with the extremes and the
+// sound speed beside it), where the later kernels of the step and the next
+// prediction read it.  NaNs are skipped, as the reference's `<` and `>`
+// comparisons skip them.
+constexpr double kMinEdgeInit = 1.0e6;  // calcMinEdgeLength's start value (Domain_d.C:2225)
+
+// the block's minimum (MIN) or maximum of v, valid in thread 0
+template <int BS, bool MIN>
+__device__ __forceinline__ double block_extreme(double v) {
+  __shared__ double sh[BS / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_xor(v, o);
+    v = MIN ? fmin(v, w) : fmax(v, w);
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = sh[0];
+#pragma unroll
+  for (int k = 1; k < BS / 64; ++k) r = MIN ? fmin(r, sh[k]) : fmax(r, sh[k]);
+  return r;
+}
+
+// smallest edge of the tet of an element's first four nodes
+// (calcMinEdgeLength, Domain_d.C:2224-2260)
+template <typename T>
+__device__ __forceinline__ double min_edge4(const T (&P)[4][3]) {
+  const int ep[6][2] = {{1, 0}, {2, 0}, {3, 0}, {2, 1}, {3, 1}, {3, 2}};
+  double m = kMinEdgeInit;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const T ex = P[ep[i][0]][0] - P[ep[i][1]][0], ey = P[ep[i][0]][1] - P[ep[i][1]][1],
+            ez = P[ep[i][0]][2] - P[ep[i][1]][2];
+    const double len = (double)sqrt(ex * ex + ey * ey + ez * ez);
+    if (len < m) m = len;
+  }
+  return m;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cfl_min_edge(int E, int N, const int *__restrict__ elnod,
+                                                         const double *__restrict__ x, double *__restrict__ part) {
+  const int e = gtid();
+  double m = kMinEdgeInit;
+  if (e < E) {
+    double P[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int g = elnod[(long)k * E + e];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) P[k][c] = x[(long)c * N + g];
+    }
+    m = min_edge4<double>(P);
+  }
+  m = block_extreme<kBlock, true>(m);
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// the largest nodal speed |v| (norm(getVelVec(i)))
+__global__ __launch_bounds__(kBlock) void k_cfl_max_speed(int N, const double *__restrict__ v,
+                                                          double *__restrict__ part) {
+  const int i = gtid();
+  double sp = 0.0;
+  if (i < N) {
+    const double vx = v[i], vy = v[(long)N + i], vz = v[2L * N + i];
+    sp = sqrt(vx * vx + vy * vy + vz * vz);
+  }
+  sp = block_extreme<kBlock, false>(sp);
+  if (threadIdx.x == 0) part[blockIdx.x] = sp;
+}
+
+// dt of a step from the partial extremes (one block)
+constexpr int kCflReduceBlock = 1024;
+template <typename T>
+__global__ __launch_bounds__(kCflReduceBlock) void k_cfl_reduce(
+    const double *__restrict__ minl_part, int nmin, const double *__restrict__ vmax_part, int nmax,
+    const T *__restrict__ rho, double K, double cfl, double *__restrict__ s, double *__restrict__ dt_hist,
+    int step) {
+  double m = kMinEdgeInit, v = 0.0;
+  for (int k = threadIdx.x; k < nmin; k += kCflReduceBlock) m = fmin(m, minl_part[k]);
+  for (int k = threadIdx.x; k < nmax; k += kCflReduceBlock) v = fmax(v, vmax_part[k]);
+  m = block_extreme<kCflReduceBlock, true>(m);
+  v = block_extreme<kCflReduceBlock, false>(v);
+  if (threadIdx.x == 0) {
+    const double cs = sqrt(K / (double)rho[0]);
+    const double dt = cfl * m / (cs + v);
+    s[kCflMinl] = m;
+    s[kCflVmax] = v;
+    s[kCflCs] = cs;
+    s[kCflDt] = dt;
+    if (dt_hist) dt_hist[step] = dt;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // tensor3 with the reference's operators (include/common/Tensor3.C)
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -115,13 +214,16 @@ __device__ __forceinline__ T3<T> t3_sub(const T3<T> &a, const T3<T> &b) {
 // nodal update kernels: flat over the [3][N] component-major arrays
 // ---------------------------------------------------------------------------
 
-// UpdatePrediction (Domain_d.C:961-974)
+// UpdatePrediction (Domain_d.C:961-974); with dtp the step length is read
+// from device memory (the per-step CFL time step of the previous step)
 __global__ __launch_bounds__(kBlock) void k_update_prediction(long n3, double *__restrict__ v,
                                                               const double *__restrict__ prev_a,
                                                               double *__restrict__ u_dt, double dt,
-                                                              double beta, double gamma) {
+                                                              double beta, double gamma,
+                                                              const double *__restrict__ dtp) {
   long i = gtid();
   if (i >= n3) return;
+  if (dtp) dt = *dtp;
   double pa = prev_a[i], vi = v[i];
   u_dt[i] = dt * (vi + (0.5 - beta) * dt * pa);
   v[i] = vi + (1.0 - gamma) * dt * pa;
@@ -474,17 +576,25 @@ __device__ __forceinline__ double clamp_range(double v, double lo, double hi) {
   return v;
 }
 
+// Kernels that update stresses are compiled twice: LAW_PLAIN handles the
+// bilinear and Hollomon laws, LAW_RATE Johnson-Cook and GMT, so that the
+// register budget of the common laws does not carry the rate laws' code.
+enum { LAW_PLAIN = 0, LAW_RATE = 1 };
+inline int law_class(int model) { return model >= JOHNSON_COOK ? LAW_RATE : LAW_PLAIN; }
+
 // Calc*YieldStress (Material.cuh:353-364, :378-388, :412-441) of element e
 // at plastic strain ep and effective strain rate edot; bilinear as
 // sigma_y = sy0_e + H*ep (the reference leaves it undefined).  The
 // rate-dependent laws evaluate in double in both precisions.
-template <typename T>
+template <typename T, int LAW>
 __device__ __forceinline__ T yield_stress(const MatParams &m, int e, T sy0e, T ep, double edot) {
-  if (m.model == HOLLOMON) {
-    double s = (double)ep + m.eps0;
-    return (s > m.eps1) ? (T)(m.Kh * pow(s, m.mh)) : (T)m.sy0;
-  }
-  if (m.model == JOHNSON_COOK) {
+  if constexpr (LAW == LAW_PLAIN) {
+    if (m.model == HOLLOMON) {
+      double s = (double)ep + m.eps0;
+      return (s > m.eps1) ? (T)(m.Kh * pow(s, m.mh)) : (T)m.sy0;
+    }
+    return sy0e + (T)m.H * ep;
+  } else if (m.model == JOHNSON_COOK) {
     const double temp = m.Te ? m.Te[e] : m.temp;
     const double T_h = (temp - m.T_t) / (m.T_m - m.T_t);
     const double sr = (edot == 0.0) ? 1.e-5 : edot;
@@ -498,18 +608,19 @@ __device__ __forceinline__ T yield_stress(const MatParams &m, int e, T sy0e, T e
     return (T)(m.C1 * exp(m.C2 * Tc) * pow(es, m.n1 * Tc + m.n2) * exp((m.I1 * Tc + m.I2) / es) *
                pow(er, m.m1 * Tc + m.m2));
   }
-  return sy0e + (T)m.H * ep;
 }
 
 // Calc*TangentModulus (Material.cuh:389-395, :397-411, :444-473); edot is
 // the capped rate.
-template <typename T>
+template <typename T, int LAW>
 __device__ __forceinline__ T tangent_modulus(const MatParams &m, int e, T ep, double edot) {
-  if (m.model == HOLLOMON) {
-    double s = (double)ep + m.eps0;
-    return (s > m.eps1) ? (T)(m.Kh * m.mh * pow(s, (m.mh - 1.0))) : T(0);
-  }
-  if (m.model == JOHNSON_COOK) {
+  if constexpr (LAW == LAW_PLAIN) {
+    if (m.model == HOLLOMON) {
+      double s = (double)ep + m.eps0;
+      return (s > m.eps1) ? (T)(m.Kh * m.mh * pow(s, (m.mh - 1.0))) : T(0);
+    }
+    return (T)m.H;
+  } else if (m.model == JOHNSON_COOK) {
     const double temp = m.Te ? m.Te[e] : m.temp;
     const double T_h = (temp - m.T_t) / (m.T_m - m.T_t);
     if (ep > T(0))
@@ -524,11 +635,10 @@ __device__ __forceinline__ T tangent_modulus(const MatParams &m, int e, T ep, do
     return (T)(m.C1 * exp(m.C2 * Tc) * pow(er, m.m1 * Tc + m.m2) * pow(es, Tc * m.n1 + m.n2 - 2.0) *
                (-m.I1 * Tc - m.I2 + es * (m.n1 * Tc + m.n2)) * exp((m.I1 * Tc + m.I2) / es));
   }
-  return (T)m.H;
 }
 
 // CalcStressStrain (Mechanical.C:1664-1839)
-template <typename T>
+template <typename T, int LAW>
 __global__ __launch_bounds__(kBlock) void k_stress_strain(
     int E, T dt, MatParams mp, const T *__restrict__ sr, const T *__restrict__ rr,
     const T *__restrict__ p, const T *__restrict__ sy0, const T *__restrict__ He, T *__restrict__ tau_io,
@@ -563,13 +673,13 @@ __global__ __launch_bounds__(kBlock) void k_stress_strain(
   T sig_trial = sqrt(T(3) * J2);
   T ep = pl_strain[e];
   // bilinear plastic modulus: per element when given (specimen batches)
-  const bool per_elem = mp.model == BILINEAR && He;
-  const double edot = (mp.model >= JOHNSON_COOK) ? eff_strain_rate(D) : 0.0;
-  T sy = per_elem ? sy0[e] + He[e] * ep : yield_stress<T>(mp, e, sy0[e], ep, edot);
+  const bool per_elem = LAW == LAW_PLAIN && mp.model == BILINEAR && He;
+  const double edot = (LAW == LAW_RATE) ? eff_strain_rate(D) : 0.0;
+  T sy = per_elem ? sy0[e] + He[e] * ep : yield_stress<T, LAW>(mp, e, sy0[e], ep, edot);
   sigma_y[e] = sy;
   T dep = T(0);
   if (sy < sig_trial) {
-    T H = per_elem ? He[e] : tangent_modulus<T>(mp, e, ep, cap_rate(mp, edot));
+    T H = per_elem ? He[e] : tangent_modulus<T, LAW>(mp, e, ep, cap_rate(mp, edot));
     T dgamma = (sig_trial - sy) / (T(3) * G + H);
     T factor = T(1) - (T(3) * G * dgamma) / sig_trial;
     tau = t3_scale(s, factor);
@@ -942,8 +1052,10 @@ template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, d
   template <> hipError_t launch_stress_strain<T>(wfx_ctx * c, double dt, hipStream_t s) {          \
     auto &el = elem<T>(c);                                                                         \
     const MatParams mp = mat_params(c);                                                            \
-    hipLaunchKernelGGL((k_stress_strain<T>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, (T)dt, \
-                       mp, el.str_rate, el.rot_rate, el.p, el.sy0, el.He, el.tau, el.eps, el.sigma, \
+    auto kern = law_class(mp.model) == LAW_RATE ? k_stress_strain<T, LAW_RATE>                     \
+                                                : k_stress_strain<T, LAW_PLAIN>;                   \
+    hipLaunchKernelGGL(kern, dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, (T)dt, mp, el.str_rate, \
+                       el.rot_rate, el.p, el.sy0, el.He, el.tau, el.eps, el.sigma,                 \
                        el.strain_pl_incr, el.pl_strain, el.sigma_y);                               \
     return hipGetLastError();                                                                      \
   }                                                                                                \

@@ -40,13 +40,16 @@ WFX_DECLARE_SPECIALISATIONS(float)
 
 template <typename T> hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t s);
 template <typename T> hipError_t launch_vsum0(wfx_ctx *c, hipStream_t s);
-template <typename T> hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s);
+template <typename T> hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s, int step);
+template <typename T> hipError_t launch_cfl_step(wfx_ctx *c, int step, hipStream_t s);
+extern template hipError_t launch_cfl_step<double>(wfx_ctx *, int, hipStream_t);
+extern template hipError_t launch_cfl_step<float>(wfx_ctx *, int, hipStream_t);
 extern template hipError_t launch_fused_steps<double>(wfx_ctx *, int, double *, hipStream_t);
 extern template hipError_t launch_fused_steps<float>(wfx_ctx *, int, double *, hipStream_t);
 extern template hipError_t launch_vsum0<double>(wfx_ctx *, hipStream_t);
 extern template hipError_t launch_vsum0<float>(wfx_ctx *, hipStream_t);
-extern template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipStream_t);
-extern template hipError_t launch_fused_phase<float>(wfx_ctx *, int, int, hipStream_t);
+extern template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipStream_t, int);
+extern template hipError_t launch_fused_phase<float>(wfx_ctx *, int, int, hipStream_t, int);
 template <typename T> hipError_t launch_fused_pack(wfx_ctx *c, hipStream_t s);
 template <typename T> hipError_t launch_fused_unpack(wfx_ctx *c, hipStream_t s);
 extern template hipError_t launch_fused_pack<double>(wfx_ctx *, hipStream_t);

@@ -127,6 +127,29 @@ class Domain:
     def SetDT(self, dt):
         check(self._L.wfx_set_dt(self._h, float(dt)))
 
+    def setCFL(self, cfl_factor):
+        """per-step CFL time step with this factor (Domain_d::setCFL with
+        m_fixed_dt false); 0 keeps dt fixed"""
+        check(self._L.wfx_set_cfl(self._h, float(cfl_factor)))
+
+    def calcCflDt(self):
+        """per-step CFL time step of the current state; the following
+        per-step calls use it"""
+        v = ctypes.c_double()
+        check(self._L.wfx_calc_cfl_dt(self._h, ctypes.byref(v), self.stream))
+        return v.value
+
+    def getDT(self):
+        v = ctypes.c_double()
+        check(self._L.wfx_get_dt(self._h, ctypes.byref(v)))
+        return v.value
+
+    def dtHistory(self, count):
+        """dt of each step of the last run with the per-step CFL time step"""
+        out = np.zeros(int(count), dtype=np.float64)
+        check(self._L.wfx_dt_history(self._h, out.ctypes.data, int(count)))
+        return out
+
     def setCH(self, alpha, beta, gamma):
         check(self._L.wfx_set_ch(self._h, alpha, beta, gamma))
 
