@@ -81,12 +81,27 @@ int wfx_set_material_jc(wfx_ctx *ctx, double E, double nu, double rho0, double A
 int wfx_set_material_gmt(wfx_ctx *ctx, double E, double nu, double rho0, double sy0,
                          const double c[8], const double range[6]);
 /* Domain_d::setTemp (Domain_d.h:678-683, IniTemp main.C:424-440, default 20):
- * uniform temperature read by the Johnson-Cook and GMT laws.  The thermal
- * solver is not part of this library, so the temperature stays as set. */
+ * uniform temperature read by the Johnson-Cook and GMT laws, and the value of
+ * every nodal temperature (the thermal coupling of wfx_set_thermal advances
+ * those). */
 int wfx_set_temperature(wfx_ctx *ctx, double T);
 /* per-element temperatures (the reference's T[e] of CalcStressStrain,
  * Mechanical.C:1725-1729); NULL returns to the uniform value */
 int wfx_set_elem_temperature(wfx_ctx *ctx, const double *T, int elem_count);
+/* Domain_d::setThermalOn + the material's thermalCond (k_T), thermalHeatCap
+ * (cp_T) and the domain's plHeatFrac (main.C:218, :590-592, :436-441):
+ * thermal coupling of the BUILD_REMESH / CPU solver.  Each step then stores
+ * the plastic heat source of CalcStressStrain (Mechanical.C:1784-1810) and
+ * runs ThermalCalcs after the position update (Thermal.C:27-115,
+ * Solver_explicit.C:1008-1012): element conduction on the step's Jacobians,
+ * heat split evenly over the element's nodes, T += dTdt * dt per node.  The
+ * Johnson-Cook and GMT laws then read the mean nodal temperature of each
+ * element.  Contact conduction and thermal expansion are not included.
+ * Nodal temperatures: field "T" (starting at wfx_set_temperature's value). */
+int wfx_set_thermal(wfx_ctx *ctx, int on, double k_T, double cp_T, double plheat_fraction);
+/* Domain_d::ThermalCalcs (Thermal.C:27-115) on the state of the last
+ * per-step API step */
+int wfx_thermal_calcs(wfx_ctx *ctx, wfx_stream stream);
 /* Domain_d::m_max_edot ("maxStrRate", main.C:228; default 1e6): cap of the
  * effective strain rate passed to the tangent modulus (Mechanical.C:1737) */
 int wfx_set_max_strain_rate(wfx_ctx *ctx, double edot);

@@ -74,6 +74,12 @@ class Domain_d {
     check(wfx_set_elem_temperature(c_, T.data(), (int)T.size()));
   }
   void setMaxStrainRate(const double &edot) { check(wfx_set_max_strain_rate(c_, edot)); }
+  // setThermalOn with the material's thermal constants
+  void setThermal(bool on, double k_T, double cp_T, double plheat_fraction = 0.9) {
+    check(wfx_set_thermal(c_, on ? 1 : 0, k_T, cp_T, plheat_fraction));
+    thermal_ = on;
+  }
+  void ThermalCalcs() { check(wfx_thermal_calcs(c_, s_)); }
   void AddBCVelNode(const int &node, const int &dim, const double &val) {
     check(wfx_add_bc_vel_node(c_, node, dim, val));
   }
@@ -159,6 +165,7 @@ class Domain_d {
     UpdateCorrectionAccVel();
     ImposeBCVAllDim();
     UpdateCorrectionPos();
+    if (thermal_) ThermalCalcs();
   }
 
   // SolveChungHulbert without output: init, then loop while Time < end_t.
@@ -194,6 +201,7 @@ class Domain_d {
   wfx_ctx *c_ = nullptr;
   wfx_stream s_ = nullptr;
   double dt_ = 0.0, end_t_ = 0.0, cfl_ = 0.0;
+  bool thermal_ = false;
 };
 
 }  // namespace wfx

@@ -113,7 +113,8 @@ static void free_fields(wfref_domain *d) {
                    &d->dHdz, &d->vol, &d->vol0, &d->rho, &d->rho0, &d->p,
                    &d->pl_strain, &d->sigma_y, &d->sy0, &d->str_rate,
                    &d->rot_rate, &d->sigma, &d->tau, &d->eps,
-                   &d->strain_pl_incr, &d->f_elem, &d->f_hg, &d->temp};
+                   &d->strain_pl_incr, &d->f_elem, &d->f_hg, &d->temp, &d->T,
+                   &d->q_plheat, &d->dTedt};
   int **ip[] = {&d->elnod, &d->nodel, &d->nodel_loc, &d->nodel_offset,
                 &d->nodel_count};
   for (size_t i = 0; i < sizeof(dp) / sizeof(dp[0]); ++i) { free(*dp[i]); *dp[i] = NULL; }
@@ -145,9 +146,11 @@ static void set_dimension(wfref_domain *d, int node_count, int elem_count) {
   RD(d->str_rate, 6 * E); RD(d->rot_rate, 6 * E); RD(d->sigma, 6 * E);
   RD(d->tau, 6 * E); RD(d->eps, 6 * E); RD(d->strain_pl_incr, 6 * E);
   RD(d->f_elem, 3 * n * E); RD(d->f_hg, 3 * n * E); RD(d->temp, E);
+  RD(d->T, N); RD(d->q_plheat, E); RD(d->dTedt, n * E);
   for (int e = 0; e < E; ++e) d->sy0[e] = d->mat.sy0;
   /* uniform initial temperature, IniTemp default (main.C:424-440) */
   for (int e = 0; e < E; ++e) d->temp[e] = 20.0;
+  for (int i = 0; i < N; ++i) d->T[i] = 20.0;
 }
 
 /* setNodElem (Domain_d.C:1508-1611): node -> element CSR.  Counts, an
@@ -627,9 +630,21 @@ static double clampr(double v, double lo, double hi) {
   return v;
 }
 
+/* The temperature the hardening law of element e reads.  Without thermal
+ * coupling: temp[e].  With it: the mean of the element's nodal
+ * temperatures (the reference reads the nodal array T at the element index,
+ * Mechanical.C:1725-1729, which is defined only for e < node count). */
+static double law_temperature(const wfref_domain *d, int e) {
+  if (!d->thermal) return d->temp[e];
+  int n = d->nodxelem;
+  double t = 0.0;
+  for (int a = 0; a < n; ++a) t += d->T[d->elnod[e * n + a]];
+  return t / n;
+}
+
 /* Calc*YieldStress (Material.cuh:353-364, :378-388, :412-441) of element e
  * at plastic strain ep and effective strain rate edot; the element
- * temperature is temp[e] (the reference reads T[e], :1725-1729 of
+ * temperature is law_temperature(d, e) (the reference reads T[e], :1725-1729 of
  * Mechanical.C).  BILINEAR is this build's definition (the reference leaves
  * it undefined). */
 double wfref_yield_stress(const wfref_domain *d, int e, double ep, double edot) {
@@ -639,7 +654,7 @@ double wfref_yield_stress(const wfref_domain *d, int e, double ep, double edot) 
     return m->sy0;
   }
   if (m->model == WFREF_JOHNSON_COOK) {
-    double T_h = (d->temp[e] - m->T_t) / (m->T_m - m->T_t);
+    double T_h = (law_temperature(d, e) - m->T_t) / (m->T_m - m->T_t);
     double sr = edot;
     if (edot == 0.0) sr = 1.e-5;
     return (d->sy0[e] + m->B * pow(ep, m->n)) * (1.0 + m->C * log(sr / m->eps_0)) *
@@ -648,7 +663,7 @@ double wfref_yield_stress(const wfref_domain *d, int e, double ep, double edot) 
   if (m->model == WFREF_GMT) {
     double es = clampr(ep, m->e_min, m->e_max);
     double er = clampr(edot, m->er_min, m->er_max);
-    double T = clampr(d->temp[e], m->T_min, m->T_max);
+    double T = clampr(law_temperature(d, e), m->T_min, m->T_max);
     return m->C1 * exp(m->C2 * T) * pow(es, m->n1 * T + m->n2) * exp((m->I1 * T + m->I2) / es) *
            pow(er, m->m1 * T + m->m2);
   }
@@ -664,7 +679,7 @@ double wfref_tangent_modulus(const wfref_domain *d, int e, double ep, double edo
     return 0.;
   }
   if (m->model == WFREF_JOHNSON_COOK) {
-    double T_h = (d->temp[e] - m->T_t) / (m->T_m - m->T_t);
+    double T_h = (law_temperature(d, e) - m->T_t) / (m->T_m - m->T_t);
     if (ep > 0.)
       return m->n * m->B * pow(ep, m->n - 1.) * (1.0 + m->C * log(edot / m->eps_0)) *
              (1.0 - pow(T_h, m->m));
@@ -673,7 +688,7 @@ double wfref_tangent_modulus(const wfref_domain *d, int e, double ep, double edo
   if (m->model == WFREF_GMT) {
     double es = clampr(ep, m->e_min, m->e_max);
     double er = clampr(edot, m->er_min, m->er_max);
-    double T = clampr(d->temp[e], m->T_min, m->T_max);
+    double T = clampr(law_temperature(d, e), m->T_min, m->T_max);
     return m->C1 * exp(m->C2 * T) * pow(er, m->m1 * T + m->m2) * pow(es, T * m->n1 + m->n2 - 2.0) *
            (-m->I1 * T - m->I2 + es * (m->n1 * T + m->n2)) * exp((m->I1 * T + m->I2) / es);
   }
@@ -726,6 +741,7 @@ void wfref_calc_stress_strain(wfref_domain *d, double dt) {
     }
     t3 sig = t3_add(&pI, &tau);
     double *pi = d->strain_pl_incr + 6 * e;
+    if (d->thermal) d->q_plheat[e] = 0.0;
     if (dep > 0.0) {
       double f = dep / d->sigma_y[e];
       pi[0] = f * (sig.xx - 0.5 * (sig.yy + sig.zz));
@@ -734,6 +750,13 @@ void wfref_calc_stress_strain(wfref_domain *d, double dt) {
       pi[3] = 1.5 * f * (sig.xy);
       pi[4] = 1.5 * f * (sig.yz);
       pi[5] = 1.5 * f * (sig.xz);
+      if (d->thermal) {
+        /* plastic work rate (Mechanical.C:1798-1810): depdt = (1/dt) incr */
+        double r = 1. / dt;
+        d->q_plheat[e] = d->plheat_frac * (sig.xx * (r * pi[0]) + 2.0 * sig.xy * (r * pi[3]) +
+                                           2.0 * sig.xz * (r * pi[5]) + sig.yy * (r * pi[1]) +
+                                           2.0 * sig.yz * (r * pi[4]) + sig.zz * (r * pi[2]));
+      }
     } else {
       for (int k = 0; k < 6; ++k) pi[k] = 0.0;
     }
@@ -915,6 +938,61 @@ double wfref_reaction_force(wfref_domain *d, int dim) {
 
 /* One pass of the SolveChungHulbert loop body, CPU branch, in its order
  * (Solver_explicit.C:524-978; remesh, contact, thermal and I/O left out). */
+/* setThermalOn and the thermal constants (the nodal temperatures stay as
+ * set: 20 at mesh creation, setTemp through the "T" field) */
+void wfref_set_thermal(wfref_domain *d, int on, double k_T, double cp_T, double plheat_frac) {
+  d->thermal = on;
+  d->k_T = k_T;
+  d->cp_T = cp_T;
+  d->plheat_frac = plheat_frac;
+}
+
+/* ThermalCalcs (Thermal.C:27-115), contact conduction absent: element
+ * conduction Kt(i,j) = (sum_d dH(d,i) dH(d,j)) k_T / detJ * w on this step's
+ * Jacobians, element-node rates -(Kt T)_i / m_i + 0.9 q_plheat vol / n /
+ * (m_i cp_T), summed per node in CSR order, T += dTdt * dt. */
+void wfref_thermal_calcs(wfref_domain *d) {
+  int n = d->nodxelem, E = d->elem_count, N = d->node_count;
+  double w = (n == 4) ? 1.0 / 6.0 : 8.0;
+#pragma omp parallel for
+  for (int e = 0; e < E; ++e) {
+    const double *hx = d->dHdx + n * e, *hy = d->dHdy + n * e, *hz = d->dHdz + n * e;
+    double Te[8], dTde[8];
+    for (int i = 0; i < n; ++i) Te[i] = d->T[d->elnod[n * e + i]];
+    for (int i = 0; i < n; ++i) {
+      double acc = 0.0;
+      for (int j = 0; j < n; ++j) {
+        double k = 0.0;
+        k += hx[i] * hx[j];
+        k += hy[i] * hy[j];
+        k += hz[i] * hz[j];
+        double kt = k * d->k_T / d->detJ[e] * w;
+        acc += kt * Te[j];
+      }
+      dTde[i] = acc;
+    }
+    double heat = 0.9 * d->q_plheat[e];
+    double elem_pow = heat * d->vol[e];
+    double pow_per_node = elem_pow / n;
+    for (int i = 0; i < n; ++i) {
+      int node = d->elnod[n * e + i];
+      double m_inv = 1.0 / d->mdiag[node];
+      double r = -m_inv * dTde[i];
+      r += pow_per_node / (d->mdiag[node] * d->cp_T);
+      d->dTedt[n * e + i] = r;
+    }
+  }
+#pragma omp parallel for
+  for (int i = 0; i < N; ++i) {
+    double dTdt = 0.0;
+    for (int k = 0; k < d->nodel_count[i]; ++k) {
+      int o = d->nodel_offset[i] + k;
+      dTdt += d->dTedt[d->nodel[o] * n + d->nodel_loc[o]];
+    }
+    d->T[i] += dTdt * d->dt;
+  }
+}
+
 /* Per-step time step of the BUILD_REMESH CPU solver (Solver_explicit.C:576-600):
  * after the prediction and the Jacobians, dt = cfl * min edge / (c + max |v|)
  * with c = sqrt(K / rho[0]) and |v| the predicted nodal speed; the step's
@@ -953,6 +1031,7 @@ void wfref_step(wfref_domain *d, double *reaction_z) {
   wfref_update_correction_accvel(d);
   for (int c = 0; c < 3; ++c) wfref_impose_bcv(d, c);
   wfref_update_correction_pos(d);
+  if (d->thermal) wfref_thermal_calcs(d);
 }
 
 void wfref_run(wfref_domain *d, int nsteps, double *hist) {
@@ -989,7 +1068,8 @@ void *wfref_field(wfref_domain *d, const char *name, long *count, int *is_int) {
       {"dHdx", d->dHdx, n * E, 0}, {"dHdy", d->dHdy, n * E, 0}, {"dHdz", d->dHdz, n * E, 0},
       {"vol", d->vol, E, 0}, {"vol0", d->vol0, E, 0}, {"rho", d->rho, E, 0},
       {"rho0", d->rho0, E, 0}, {"p", d->p, E, 0}, {"pl_strain", d->pl_strain, E, 0},
-      {"sigma_y", d->sigma_y, E, 0}, {"sy0", d->sy0, E, 0}, {"temp", d->temp, E, 0},
+      {"sigma_y", d->sigma_y, E, 0}, {"sy0", d->sy0, E, 0}, {"temp", d->temp, E, 0}, {"T", d->T, N, 0},
+      {"q_plheat", d->q_plheat, E, 0}, {"dTedt", d->dTedt, n * E, 0},
       {"str_rate", d->str_rate, 6 * E, 0}, {"rot_rate", d->rot_rate, 6 * E, 0},
       {"sigma", d->sigma, 6 * E, 0}, {"tau", d->tau, 6 * E, 0}, {"eps", d->eps, 6 * E, 0},
       {"strain_pl_incr", d->strain_pl_incr, 6 * E, 0}, {"f_elem", d->f_elem, 3 * n * E, 0},

@@ -62,6 +62,13 @@ typedef struct {
   double *str_rate, *rot_rate, *sigma, *tau, *eps, *strain_pl_incr; /* 6E */
   double *f_elem, *f_hg;                             /* 3nE, [e][a][d] */
   double *temp;       /* element temperature, E (IniTemp 20, main.C:424) */
+  /* thermal coupling (Thermal.C, setThermalOn): nodal temperature, plastic
+   * heat source, element-node temperature rates */
+  int thermal;
+  double k_T, cp_T, plheat_frac;
+  double *T;          /* N */
+  double *q_plheat;   /* E */
+  double *dTedt;      /* nE, [e][a] */
   double max_edot;    /* strain-rate cap of the tangent (Domain_d.h:824) */
   double cfl;         /* > 0: per-step dt = cfl*minl/(cs+max|v|); 0: fixed */
   /* material (one shared law; per-element initial yield in sy0[]) */
@@ -103,6 +110,10 @@ void wfref_set_material_jc(wfref_domain *d, double E, double nu, double rho0,
 void wfref_set_material_gmt(wfref_domain *d, double E, double nu, double rho0,
                             double sy0, const double c[8], const double range[6]);
 void wfref_set_elem_sy0(wfref_domain *d, const double *sy0);
+/* setThermalOn + thermalCond / thermalHeatCap / plHeatFrac (main.C:218,
+ * :590-592) */
+void wfref_set_thermal(wfref_domain *d, int on, double k_T, double cp_T, double plheat_frac);
+void wfref_thermal_calcs(wfref_domain *d);
 /* the hardening law of one element, exposed for the pin tests */
 double wfref_eff_strain_rate(const double D[6]);
 double wfref_yield_stress(const wfref_domain *d, int e, double ep, double edot);

@@ -41,6 +41,8 @@ def lib():
             "wfref_set_material": (None, [vp, i, d, d, d, d, d, d]),
             "wfref_set_elem_sy0": (None, [vp, dp]),
             "wfref_set_material_jc": (None, [vp, d, d, d, d, d, d, d, d, d, d, d]),
+            "wfref_set_thermal": (None, [vp, i, d, d, d]),
+            "wfref_thermal_calcs": (None, [vp]),
             "wfref_set_material_gmt": (None, [vp, d, d, d, d, dp, dp]),
             "wfref_eff_strain_rate": (d, [dp]),
             "wfref_yield_stress": (d, [vp, i, d, d]),
@@ -137,9 +139,13 @@ class RefDomain:
         self._L.wfref_set_material_gmt(self._d, E, nu, rho0, sy0, _ptr(ca), _ptr(ra))
 
     def setTemperature(self, T):
-        """uniform element temperature, or one value per element"""
-        t = np.broadcast_to(np.asarray(T, dtype=np.float64), (self.elem_count,))
+        """uniform temperature (elements and nodes, as setTemp), or one
+        value per element"""
+        a = np.asarray(T, dtype=np.float64)
+        t = np.broadcast_to(a, (self.elem_count,))
         self.setField("temp", np.ascontiguousarray(t))
+        if a.ndim == 0:
+            self.setField("T", np.full(self.node_count, float(a)))
 
     def setCFL(self, cfl_factor):
         """per-step CFL time step (0: fixed dt)"""
@@ -147,6 +153,13 @@ class RefDomain:
 
     def getDT(self):
         return self._L.wfref_get_double(self._d, b"dt")
+
+    def setThermal(self, on=True, k_T=50.0, cp_T=460.0, plheat_frac=0.9):
+        """thermal coupling (nodal temperatures in field "T")"""
+        self._L.wfref_set_thermal(self._d, int(on), float(k_T), float(cp_T), float(plheat_frac))
+
+    def ThermalCalcs(self):
+        self._L.wfref_thermal_calcs(self._d)
 
     def setMaxStrainRate(self, edot):
         self.set("max_edot", edot)

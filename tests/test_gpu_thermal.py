@@ -1,0 +1,142 @@
+"""Thermal coupling on the GPU against the oracle.
+
+ThermalCalcs alone on a random state (per-step API, FP64, relative 1e-13),
+then 100-step compressions of a Johnson-Cook block whose temperature starts
+as a gradient along z: per-step API and fused path, with and without the
+per-step CFL time step.  Temperature increments (T - T0), displacements and
+plastic strain within 1e-10 relative (FP64) and 1e-3 (FP32-mixed).
+A large conductivity makes conduction visible within 100 steps.
+"""
+import numpy as np
+import pytest
+
+from oracle.wfref import RefDomain
+from weldformfem_amd import Domain, WFX_FP32_MIXED, WFX_FP64, workloads
+
+pytestmark = pytest.mark.gpu
+
+JC = (300e6, 500e6, 0.3, 0.014, 1.0, 1.0, 1520.0, 20.0)
+K_T, CP = 5.0e4, 460.0
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def rel(a, b):
+    s = np.max(np.abs(b))
+    return np.max(np.abs(a - b)) / (s if s > 0 else 1.0)
+
+
+def jittered_box(nx, ny, nz, h, tet, seed, amp=0.15):
+    ref = RefDomain()
+    ref.AddBoxLength([0, 0, 0], [nx * h, ny * h, nz * h], h / 2, True, tet)
+    x = ref.field("x").reshape(-1, 3)
+    x = x + (2.0 * workloads.uniform(seed, x.size).reshape(-1, 3) - 1.0) * amp * h
+    return x, ref.field("elnod"), 4 if tet else 8
+
+
+@pytest.mark.parametrize("tet", [False, True])
+def test_thermal_calcs_random_state(tet):
+    say(f"case: ThermalCalcs FP64 on a random state, tet={tet}")
+    x, el, n = jittered_box(5, 4, 3, 0.02, tet, seed=17)
+    g, r = Domain(WFX_FP64), RefDomain()
+    for d in (g, r):
+        d.CreateFromMesh(x, el, n)
+        d.setDensity(7850.0)
+        d.setMaterial(0, 200e9, 0.3, 7850.0, 300e6, 1e9, 0.0)
+        d.SetDT(1e-4)
+        d.init()
+        d.setThermal(True, K_T, CP, 0.9)
+    rng = np.random.default_rng(5)
+    T0 = 20.0 + 800.0 * rng.random(g.node_count)
+    q = 1e9 * rng.random(g.elem_count)
+    for d in (g, r):
+        d.setField("T", T0)
+        d.setField("q_plheat", q)
+        d.ThermalCalcs()
+    e = rel(g.field("T") - T0, r.field("T") - T0)
+    assert np.max(np.abs(r.field("T") - T0)) > 1.0
+    assert e <= 1e-13, f"T increment: {e:.3e}"
+
+
+def _hexa16():
+    r = RefDomain()
+    r.AddBoxLength([0, 0, 0], [0.8, 0.8, 0.8], 0.025)
+    bottom, top = workloads.box_layers(16, 16, 16)
+    return r.field("x"), r.field("elnod"), 8, bottom, top
+
+
+def _cyl12():
+    x, el, bottom, top = workloads.cylinder_tets(cells=12)
+    return x, el, 4, bottom, top
+
+
+def _pair(mesh, precision=WFX_FP64, cfl=0.0):
+    x, el, n, bottom, top = mesh()
+    g = Domain(precision)
+    g.CreateFromMesh(x, el, n)
+    dt = workloads.setup_compression(g, bottom, top, -20.0)
+    r = RefDomain()
+    r.CreateFromMesh(x, el, n)
+    workloads.setup_compression(r, bottom, top, -20.0, dt=dt)
+    T0 = 20.0 + 500.0 * np.asarray(x).reshape(-1, 3)[:, 2]
+    for d in (g, r):
+        d.setMaterialJC(200e9, 0.3, 7850.0, *JC)
+        d.setTemperature(20.0)
+        d.setThermal(True, K_T, CP, 0.9)
+        d.setField("T", T0)
+        if cfl:
+            d.setCFL(cfl)
+        d.init()
+    return g, r, T0
+
+
+def check(g, r, T0, tol):
+    dT_g, dT_r = g.field("T") - T0, r.field("T") - T0
+    assert np.max(np.abs(dT_r)) > 0.5  # conduction and plastic heat acted
+    for name, a, b in [("T increment", dT_g, dT_r), ("u", g.field("u"), r.field("u")),
+                       ("pl_strain", g.field("pl_strain"), r.field("pl_strain"))]:
+        e = rel(a, b)
+        assert e <= tol, f"{name}: {e:.3e}"
+    assert np.max(r.field("pl_strain")) > 0
+
+
+@pytest.mark.parametrize("mesh", [_hexa16, _cyl12])
+def test_per_step_path_fp64(mesh):
+    say(f"case: thermal JC compression, per-step API, FP64, {mesh.__name__}")
+    g, r, T0 = _pair(mesh)
+    g.run(100)
+    r.run(100)
+    check(g, r, T0, 1e-10)
+
+
+@pytest.mark.parametrize("mesh", [_hexa16, _cyl12])
+@pytest.mark.parametrize("cfl", [0.0, 0.25])
+def test_fused_path_fp64(mesh, cfl):
+    say(f"case: thermal JC compression, fused, FP64, cfl {cfl}, {mesh.__name__}")
+    g, r, T0 = _pair(mesh, cfl=cfl)
+    g.runFused(40)
+    g.runFused(60)
+    r.run(100)
+    check(g, r, T0, 1e-10)
+    assert g.nonfiniteCount() == 0
+
+
+def test_fused_fp32_mixed():
+    say("case: thermal JC compression, fused FP32-mixed vs FP64 oracle")
+    g, r, T0 = _pair(_hexa16, precision=WFX_FP32_MIXED)
+    g.runFused(100)
+    r.run(100)
+    dT = rel(g.field("T") - T0, r.field("T") - T0)
+    say(f"  FP32-mixed rel err: T increment {dT:.3e} u {rel(g.field('u'), r.field('u')):.3e}")
+    check(g, r, T0, 1e-3)
+
+
+def test_adjoint_rejects_thermal():
+    from weldformfem_amd._lib import WfxError
+
+    g, _, _ = _pair(_hexa16)
+    g.setMaterial(0, 200e9, 0.3, 7850.0, 300e6, 1e9, 0.0)
+    with pytest.raises(WfxError, match="thermal"):
+        g.adjointForward(10, checkpoint_every=5)

@@ -38,6 +38,8 @@ SIGNATURES = {
     "wfx_set_elem_temperature": (_i, [_vp, _vp, _i]),
     "wfx_set_max_strain_rate": (_i, [_vp, _d]),
     "wfx_set_cfl": (_i, [_vp, _d]),
+    "wfx_set_thermal": (_i, [_vp, _i, _d, _d, _d]),
+    "wfx_thermal_calcs": (_i, [_vp, _vp]),
     "wfx_get_dt": (_i, [_vp, _vp]),
     "wfx_calc_cfl_dt": (_i, [_vp, _vp, _vp]),
     "wfx_dt_history": (_i, [_vp, _vp, _i]),

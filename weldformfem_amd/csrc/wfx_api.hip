@@ -93,7 +93,7 @@ static int setup_device(wfx_ctx *c) {
   if (e == hipSuccess) e = dalloc(c, &ptr, (cnt));
   A(c->nd.x, 3 * N) A(c->nd.v, 3 * N) A(c->nd.a, 3 * N) A(c->nd.u, 3 * N) A(c->nd.u_dt, 3 * N)
   A(c->nd.prev_a, 3 * N) A(c->nd.fi, 3 * N) A(c->nd.fe, 3 * N) A(c->nd.mdiag, N) A(c->nd.voln, N)
-  A(c->nd.voln_sum, N) A(c->nd.voln0_sum, N) A(c->nd.anp_ratio, N)
+  A(c->nd.voln_sum, N) A(c->nd.voln0_sum, N) A(c->nd.anp_ratio, N) A(c->nd.T, N)
   A(c->ms.elnod, n * E) A(c->ms.nodel, n * E) A(c->ms.nodel_loc, n * E) A(c->ms.nodel_offset, N)
   A(c->ms.nodel_count, N) A(c->ms.nodel_pk, n * E) A(c->ms.nodel_pad, 8 * N) A(c->bc_slot, N)
   A(c->scalars, 64) A(c->status, 16)
@@ -133,6 +133,10 @@ static int setup_device(wfx_ctx *c) {
   WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_pad, pad.data(), 8 * N * sizeof(int), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemset(c->bc_slot, 0xff, N * sizeof(int)));  // -1: no boundary condition
   WFX_CHECK_HIP(hipMemcpy(c->scalars + kCflDt, &c->prm.dt, sizeof(double), hipMemcpyHostToDevice));
+  {
+    std::vector<double> t0(N, c->prm.mat.temp);  // setTemp (Domain_d.h:678-683)
+    WFX_CHECK_HIP(hipMemcpy(c->nd.T, t0.data(), N * sizeof(double), hipMemcpyHostToDevice));
+  }
   c->has_mesh = true;
   c->initialized = false;
   return WFX_OK;
@@ -239,6 +243,8 @@ static bool find_field(wfx_ctx *c, const char *name, FieldDesc &f) {
   else if (s == "strain_pl_incr") el(E64(c->e64.strain_pl_incr, c->e32.strain_pl_incr), 6);
   else if (s == "f_elem") el(E64(c->e64.f_elem, c->e32.f_elem), (int)(3 * n));
   else if (s == "f_hg") el(E64(c->e64.f_hg, c->e32.f_hg), (int)(3 * n));
+  else if (s == "T") node1(c->nd.T);
+  else if (s == "q_plheat" && E64(c->e64.q_plheat, c->e32.q_plheat)) el(E64(c->e64.q_plheat, c->e32.q_plheat), 1);
   else return false;
   return true;
 }
@@ -459,6 +465,42 @@ int wfx_set_temperature(wfx_ctx *c, double T) {
   REQUIRE_CTX(c);
   REQUIRE(std::isfinite(T), WFX_ERR_ARG, "temperature must be finite");
   c->prm.mat.temp = T;
+  if (c->has_mesh) {  // setTemp: every node as well
+    std::vector<double> t0(c->N, T);
+    WFX_CHECK_HIP(hipMemcpy(c->nd.T, t0.data(), t0.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return WFX_OK;
+}
+
+int wfx_set_thermal(wfx_ctx *c, int on, double k_T, double cp_T, double plheat_fraction) {
+  REQUIRE_MESH(c);
+  if (on) {
+    REQUIRE(k_T >= 0.0 && cp_T > 0.0 && plheat_fraction >= 0.0 && std::isfinite(k_T) && std::isfinite(cp_T),
+            WFX_ERR_ARG, "thermal coupling needs k_T >= 0, cp_T > 0 and a non-negative heat fraction");
+    // element-node rates in the element-force layout: whole 64-element tiles
+    const size_t E = c->E, n = c->n, Ep = (E + 63) / 64 * 64;
+    if (c->precision == WFX_FP64) {
+      if (!c->e64.q_plheat) WFX_CHECK_HIP(dalloc(c, &c->e64.q_plheat, E));
+      if (!c->e64.dTedt) WFX_CHECK_HIP(dalloc(c, &c->e64.dTedt, n * Ep));
+    } else {
+      if (!c->e32.q_plheat) WFX_CHECK_HIP(dalloc(c, &c->e32.q_plheat, E));
+      if (!c->e32.dTedt) WFX_CHECK_HIP(dalloc(c, &c->e32.dTedt, n * Ep));
+    }
+    void *q = c->precision == WFX_FP64 ? (void *)c->e64.q_plheat : (void *)c->e32.q_plheat;
+    WFX_CHECK_HIP(hipMemset(q, 0, E * (c->precision == WFX_FP64 ? sizeof(double) : sizeof(float))));
+  }
+  c->thermal.on = on ? 1 : 0;
+  c->thermal.k_T = k_T;
+  c->thermal.cp_T = cp_T;
+  c->thermal.plheat_frac = plheat_fraction;
+  return WFX_OK;
+}
+
+int wfx_thermal_calcs(wfx_ctx *c, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->thermal.on, WFX_ERR_STATE, "thermal coupling is off: call wfx_set_thermal");
+  REQUIRE(!c->fused_stale, WFX_ERR_STATE, "ThermalCalcs needs the Jacobians of a per-step API step");
+  LAUNCH(BY_PRECISION(c, launch_thermal, c, false, false, as_stream(s)));
   return WFX_OK;
 }
 
@@ -827,6 +869,7 @@ static int step_api(wfx_ctx *c, hipStream_t st, double *hist) {
   LAUNCH(launch_correction_accvel(c, st));
   for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
   LAUNCH(launch_correction_pos(c, st));
+  if (c->thermal.on) LAUNCH(BY_PRECISION(c, launch_thermal, c, false, false, st));  // ThermalCalcs
   c->fused_stale = false;  // every per-step intermediate is current again
   return WFX_OK;
 }
@@ -950,14 +993,19 @@ std::vector<std::pair<void *, size_t>> wfx::fused_state_buffers(wfx_ctx *c) {
   const size_t es = c->precision == WFX_FP64 ? sizeof(double) : sizeof(float), E = c->E;
   std::vector<std::pair<void *, size_t>> v = {
       {c->nd.x, N3}, {c->nd.v, N3}, {c->nd.u, N3}, {c->nd.u_dt, N3}, {c->nd.prev_a, N3}, {c->nd.fi, N3},
-      {c->nd.mdiag, N1}, {c->nd.voln, N1}, {c->nd.anp_ratio, N1}, {c->status, 16 * sizeof(unsigned)}};
-  auto add = [&](void *p, size_t rows) { v.push_back({p, rows * E * es}); };
+      {c->nd.mdiag, N1}, {c->nd.voln, N1}, {c->nd.anp_ratio, N1}, {c->status, 16 * sizeof(unsigned)},
+      {c->nd.T, N1}};
+  auto add = [&](void *p, size_t rows) {
+    if (p) v.push_back({p, rows * E * es});
+  };
   if (c->precision == WFX_FP64) {
     auto &el = c->e64;
     add(el.vol, 1); add(el.tau, 6); add(el.eps, 6); add(el.p, 1); add(el.pl_strain, 1); add(el.sigma_y, 1);
+    add(el.q_plheat, 1);
   } else {
     auto &el = c->e32;
     add(el.vol, 1); add(el.tau, 6); add(el.eps, 6); add(el.p, 1); add(el.pl_strain, 1); add(el.sigma_y, 1);
+    add(el.q_plheat, 1);
   }
   return v;
 }
@@ -1041,6 +1089,7 @@ static uint64_t graph_signature(wfx_ctx *c) {
   mix(dts, sizeof(dts));
   mix(&c->prm.mat, sizeof(c->prm.mat));
   mix(&c->prm.stab, sizeof(c->prm.stab));
+  mix(&c->thermal, sizeof(c->thermal));
   const void *ptrs[] = {c->bc_nod[0], c->bc_nod[1], c->bc_nod[2], c->bc_slot, c->bc_mask, c->bc_tab,
                         c->react, c->hist, c->e64.He, c->e32.He, c->temp_e, c->dt_hist};
   mix(ptrs, sizeof(ptrs));

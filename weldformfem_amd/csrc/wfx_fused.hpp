@@ -217,6 +217,7 @@ struct FusedParams {
   double hg_coeff, cs0;
   double rho_u;       // > 0: every element density equals it (no density loads)
   const double *dtp;  // non-null: the step's dt from the device (per-step CFL time step)
+  double plheat_frac; // thermal coupling: fraction of plastic work into heat
 };
 
 // dH(d,a) = adj(J)(d,:) . dN_a/dxi from the cofactor matrix C (adj = C^T),
@@ -247,21 +248,12 @@ __device__ __forceinline__ T shape_dH(const T (&C)[3][3], int d, int a) {
   }
 }
 
-// F3: the element chain of one step for element e.  The nodal forces of the
-// element (f - f_hg, what the assembly adds to each node) go to sink(a, fx, fy, fz).
-template <typename T, int NN, int LAW, class Sink>
-__device__ __forceinline__ void element_chain(
-    int e, int E, int N, const FusedParams &fp, const int *__restrict__ elnod, const double *__restrict__ x,
-    const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
-    const T *__restrict__ rho, const T *__restrict__ sy0, const T *__restrict__ He,
-    T *__restrict__ tau_io, T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
-    T *__restrict__ sigma_y, Sink &&sink) {
-  const long Es = E;
-  int g[NN];
-#pragma unroll
-  for (int a = 0; a < NN; ++a) g[a] = ld_stream(&elnod[(long)a * Es + e]);
-
-  // ---- Jacobian, determinant, dH (calcElemJAndDerivatives) --------------
+// Jacobian cofactors C and determinant of element nodes g from the positions
+// (calcElemJAndDerivatives, Domain_d.C:1701-2044); FP32 differences the FP64
+// positions against node 0 before narrowing.
+template <typename T, int NN>
+__device__ __forceinline__ void elem_jacobian(const int (&g)[NN], const double *__restrict__ x, int N,
+                                              T (&C)[3][3], T &detJ) {
   T X[NN][3];
   if constexpr (std::is_same<T, double>::value) {
 #pragma unroll
@@ -293,7 +285,6 @@ __device__ __forceinline__ void element_chain(
       J[2][c] = X[3][c] - X[0][c];
     }
   }
-  T C[3][3];
   C[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
   C[0][1] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]);
   C[0][2] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
@@ -303,8 +294,28 @@ __device__ __forceinline__ void element_chain(
   C[2][0] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
   C[2][1] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]);
   C[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
-  T detJ = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[1][2] * J[2][1] - J[0][1] * J[1][0] * J[2][2] +
-           J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] - J[0][2] * J[1][1] * J[2][0];
+  detJ = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[1][2] * J[2][1] - J[0][1] * J[1][0] * J[2][2] +
+         J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] - J[0][2] * J[1][1] * J[2][0];
+}
+
+// F3: the element chain of one step for element e.  The nodal forces of the
+// element (f - f_hg, what the assembly adds to each node) go to sink(a, fx, fy, fz).
+template <typename T, int NN, int LAW, class Sink>
+__device__ __forceinline__ void element_chain(
+    int e, int E, int N, const FusedParams &fp, const int *__restrict__ elnod, const double *__restrict__ x,
+    const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
+    const T *__restrict__ rho, const T *__restrict__ sy0, const T *__restrict__ He,
+    T *__restrict__ tau_io, T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
+    T *__restrict__ sigma_y, T *__restrict__ q_plheat, Sink &&sink) {
+  const long Es = E;
+  int g[NN];
+#pragma unroll
+  for (int a = 0; a < NN; ++a) g[a] = ld_stream(&elnod[(long)a * Es + e]);
+
+  // ---- Jacobian, determinant, dH (calcElemJAndDerivatives) --------------
+  T C[3][3];
+  T detJ;
+  elem_jacobian<T, NN>(g, x, N, C, detJ);
   // dH(d,a) is re-evaluated from the 3x3 cofactor matrix where it is used
   // (same expression each time), so 9 values stay live instead of 3n.
   const T w = (NN == 4) ? T(1.0 / 6.0) : T(8.0);
@@ -423,21 +434,43 @@ __device__ __forceinline__ void element_chain(
   T sy, Hb = (T)mp.H;
   const bool per_elem = LAW == LAW_PLAIN && mp.model == BILINEAR && He;
   const double edot = (LAW == LAW_RATE) ? eff_strain_rate(D) : 0.0;
+  double temp = 0.0;  // the rate laws' temperature (law_temperature on the loaded nodes)
+  if constexpr (LAW == LAW_RATE) {
+    if (mp.Tn) {
+#pragma unroll
+      for (int a = 0; a < NN; ++a) temp += mp.Tn[g[a]];
+      temp = temp / NN;
+    } else {
+      temp = mp.Te ? mp.Te[e] : mp.temp;
+    }
+  }
   if (per_elem) {
     Hb = ld_stream(&He[e]);
     sy = ld_stream(&sy0[e]) + Hb * ep;
   } else {
-    sy = yield_stress<T, LAW>(mp, e, ld_stream(&sy0[e]), ep, edot);
+    sy = yield_stress<T, LAW>(mp, temp, ld_stream(&sy0[e]), ep, edot);
   }
   sigma_y[e] = sy;
+  T dep = T(0);
   if (sy < sig_trial) {
-    T H = per_elem ? Hb : tangent_modulus<T, LAW>(mp, e, ep, cap_rate(mp, edot));
+    T H = per_elem ? Hb : tangent_modulus<T, LAW>(mp, temp, ep, cap_rate(mp, edot));
     T dgamma = (sig_trial - sy) / (T(3) * G + H);
     T factor = T(1) - (T(3) * G * dgamma) / sig_trial;
     tau = t3_scale(s, factor);
     pl_strain[e] = ep + dgamma;
+    dep = dgamma;
   }
   T3<T> sig = t3_add(pI, tau);
+  if (q_plheat) {  // thermal coupling: plastic heat source of the step
+    T q = T(0);
+    if (dep > T(0)) {
+      const T f = dep / sy;
+      q = plastic_heat<T>(sig, f * (sig.xx - T(0.5) * (sig.yy + sig.zz)), f * (sig.yy - T(0.5) * (sig.xx + sig.zz)),
+                          f * (sig.zz - T(0.5) * (sig.xx + sig.yy)), T(1.5) * f * (sig.xy), T(1.5) * f * (sig.yz),
+                          T(1.5) * f * (sig.xz), dt, fp.plheat_frac);
+    }
+    q_plheat[e] = q;
+  }
   st_stream(&eps_io[0 * Es + e], ld_stream(&eps_io[0 * Es + e]) + D.xx * dt);
   st_stream(&eps_io[1 * Es + e], ld_stream(&eps_io[1 * Es + e]) + D.yy * dt);
   st_stream(&eps_io[2 * Es + e], ld_stream(&eps_io[2 * Es + e]) + D.zz * dt);
@@ -488,16 +521,125 @@ __global__ __launch_bounds__(BS) void f3_element(
     const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
     const T *__restrict__ rho, const T *__restrict__ sy0, const T *__restrict__ He,
     T *__restrict__ tau_io, T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
-    T *__restrict__ sigma_y, T *__restrict__ fout) {
+    T *__restrict__ sigma_y, T *__restrict__ fout, T *__restrict__ q_plheat) {
   int e = gtid();
   if (e >= E) return;
   const long Es = E;
   element_chain<T, NN, LAW>(e, E, N, fp, elnod, x, v, ratio, vol0, rho, sy0, He, tau_io, eps_io, p_out, pl_strain,
-                       sigma_y, [&](int a, T f0, T f1, T f2) {
+                       sigma_y, q_plheat, [&](int a, T f0, T f1, T f2) {
                          st_stream(&fout[fidx(a * 3 + 0, e, Es, 3 * NN)], f0);
                          st_stream(&fout[fidx(a * 3 + 1, e, Es, 3 * NN)], f1);
                          st_stream(&fout[fidx(a * 3 + 2, e, Es, 3 * NN)], f2);
                        });
+}
+
+// ---- thermal coupling ------------------------------------------------------
+// ThermalCalcs (Thermal.C:27-115) without contact conduction.  Element e:
+// Kt(i,j) = (sum_d dH(d,i) dH(d,j)) k_T / detJ * w, element-node rate
+// -(Kt T)_i / m_i + 0.9 q_plheat vol / n / (m_i cp_T) into dTedt[i][E]; the
+// node pass adds a node's rates in CSR order and advances T by dt.
+template <typename T, int NN>
+__device__ __forceinline__ void thermal_rates(int e, long Es, const int (&g)[NN], const double (&h)[3][NN],
+                                              double detJ, double vol, double q, const double *__restrict__ Tn,
+                                              const double *__restrict__ mdiag, double k_T, double cp_T,
+                                              T *__restrict__ dTedt) {
+  const double w = (NN == 4) ? 1.0 / 6.0 : 8.0;
+  double Te[NN];
+#pragma unroll
+  for (int a = 0; a < NN; ++a) Te[a] = Tn[g[a]];
+  const double heat = 0.9 * q;
+  const double elem_pow = heat * vol;
+  const double pow_per_node = elem_pow / NN;
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < NN; ++j) {
+      double k = 0.0;
+      k += h[0][i] * h[0][j];
+      k += h[1][i] * h[1][j];
+      k += h[2][i] * h[2][j];
+      const double kt = k * k_T / detJ * w;
+      acc += kt * Te[j];
+    }
+    const double m = mdiag[g[i]];
+    const double m_inv = 1.0 / m;
+    double r = -m_inv * acc;
+    r += pow_per_node / (m * cp_T);
+    dTedt[fidx(i, e, Es, NN)] = (T)r;
+  }
+}
+
+// per-step API: shape derivatives and determinant of this step's Jacobians
+template <typename T, int NN>
+__global__ __launch_bounds__(kBlock) void k_thermal_elem(
+    int E, const int *__restrict__ elnod, const T *__restrict__ dH, const T *__restrict__ detJ,
+    const T *__restrict__ vol, const T *__restrict__ q, const double *__restrict__ Tn,
+    const double *__restrict__ mdiag, double k_T, double cp_T, T *__restrict__ dTedt) {
+  const int e = gtid();
+  if (e >= E) return;
+  int g[NN];
+  double h[3][NN];
+#pragma unroll
+  for (int a = 0; a < NN; ++a) {
+    g[a] = elnod[(long)a * E + e];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) h[d][a] = (double)dH[(long)(d * NN + a) * E + e];
+  }
+  thermal_rates<T, NN>(e, E, g, h, (double)detJ[e], (double)vol[e], (double)q[e], Tn, mdiag, k_T, cp_T, dTedt);
+}
+
+// fused path: the Jacobian recomputed from the positions (still those of the
+// step's start: it runs before the node pass)
+template <typename T, int NN>
+__global__ __launch_bounds__(kBlock) void k_thermal_elem_x(
+    int E, int N, const int *__restrict__ elnod, const double *__restrict__ x, const T *__restrict__ vol,
+    const T *__restrict__ q, const double *__restrict__ Tn, const double *__restrict__ mdiag, double k_T,
+    double cp_T, T *__restrict__ dTedt) {
+  const int e = gtid();
+  if (e >= E) return;
+  int g[NN];
+#pragma unroll
+  for (int a = 0; a < NN; ++a) g[a] = elnod[(long)a * E + e];
+  T C[3][3];
+  T detJ;
+  elem_jacobian<T, NN>(g, x, N, C, detJ);
+  double h[3][NN];
+#pragma unroll
+  for (int a = 0; a < NN; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) h[d][a] = (double)shape_dH<T, NN>(C, d, a);
+  thermal_rates<T, NN>(e, E, g, h, (double)detJ, (double)vol[e], (double)q[e], Tn, mdiag, k_T, cp_T, dTedt);
+}
+
+// T += (sum of the node's element-node rates in CSR order) * dt; the rates
+// sit in the element-force layout (fidx) and are gathered like the forces
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_thermal_node(int N, long Es, int nn, const int *__restrict__ off,
+                                                         const int *__restrict__ cnt,
+                                                         const int *__restrict__ nodel_pk,
+                                                         const int *__restrict__ pad,
+                                                         const T *__restrict__ dTedt, double dt,
+                                                         const double *__restrict__ dtp, double *__restrict__ Tn) {
+  const int i = gtid();
+  if (i >= N) return;
+  if (dtp) dt = *dtp;
+  const int c = cnt[i], o = c > kChunk ? off[i] : 0;
+  double dTdt = 0.0;
+  for (int k = 0; k < c; k += kChunk) {
+    int pk[kChunk];
+    double q[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j)
+      pk[j] = (k == 0) ? pad[(long)j * N + i] : ((k + j < c) ? nodel_pk[o + k + j] : -1);
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j)
+      q[j] = (pk[j] >= 0) ? (double)dTedt[fidx(pk[j] & 7, pk[j] >> 3, Es, nn)] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j)
+      if (k + j < c) dTdt += q[j];
+  }
+  Tn[i] += dTdt * dt;
 }
 
 struct NodeParams {
@@ -729,6 +871,9 @@ hipError_t launch_cfl_reduce(wfx_ctx *c, int nmin, int step, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_thermal(wfx_ctx *c, bool fused, bool device_dt, hipStream_t s);
+
 // Launch one phase (1..4) of a fused step on the current state with block
 // size BS; launch_fused_phase picks BS from the context (autotuned or 256).
 template <typename T, int BS>
@@ -773,12 +918,16 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
     fp.cs0 = P.mat.cs0;
     fp.rho_u = c->rho_uniform;
     fp.dtp = dtp;
+    fp.plheat_frac = c->thermal.plheat_frac;
     const bool rate = law_class(fp.mp.model) == LAW_RATE;
     auto kern = (c->n == 8) ? (rate ? f3_element<T, 8, BS, LAW_RATE> : f3_element<T, 8, BS, LAW_PLAIN>)
                             : (rate ? f3_element<T, 4, BS, LAW_RATE> : f3_element<T, 4, BS, LAW_PLAIN>);
     hipLaunchKernelGGL(kern, dim3(gE), dim3(BS), 0, s, E, N, fp, c->ms.elnod, c->nd.x, vN, c->nd.anp_ratio,
                        el.vol0, el.rho, el.sy0, el.He, el.tau, el.eps, el.p, el.pl_strain, el.sigma_y,
-                       el.f_elem);
+                       el.f_elem, c->thermal.on ? el.q_plheat : (T *)nullptr);
+    // thermal coupling: ThermalCalcs on this step's geometry, before the
+    // node pass moves the positions
+    if (c->thermal.on) return launch_thermal<T>(c, true, cfl, s);
   } else if (phase == 4) {
     NodeParams np = node_params(c, predict);
     np.dtp = dtp;
@@ -827,6 +976,30 @@ hipError_t launch_cfl_step(wfx_ctx *c, int step, hipStream_t s) {
   hipLaunchKernelGGL(k_cfl_min_edge, dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, c->N, c->ms.elnod, c->nd.x,
                      c->cfl_part);
   return launch_cfl_reduce<T>(c, nblocks(c->E), -1, s);
+}
+
+// ThermalCalcs of one step: element rates (per-step API: stored Jacobians;
+// fused: recomputed from the positions) and the nodal update with the
+// host's dt or, with the per-step CFL time step inside a fused run, the
+// device's.
+template <typename T>
+hipError_t launch_thermal(wfx_ctx *c, bool fused, bool device_dt, hipStream_t s) {
+  auto &el = elem<T>(c);
+  const auto &th = c->thermal;
+  const int gE = nblocks(c->E);
+  if (fused) {
+    auto kern = c->n == 8 ? k_thermal_elem_x<T, 8> : k_thermal_elem_x<T, 4>;
+    hipLaunchKernelGGL(kern, dim3(gE), dim3(kBlock), 0, s, c->E, c->N, c->ms.elnod, c->nd.x, el.vol, el.q_plheat,
+                       c->nd.T, c->nd.mdiag, th.k_T, th.cp_T, el.dTedt);
+  } else {
+    auto kern = c->n == 8 ? k_thermal_elem<T, 8> : k_thermal_elem<T, 4>;
+    hipLaunchKernelGGL(kern, dim3(gE), dim3(kBlock), 0, s, c->E, c->ms.elnod, el.dH, el.detJ, el.vol, el.q_plheat,
+                       c->nd.T, c->nd.mdiag, th.k_T, th.cp_T, el.dTedt);
+  }
+  hipLaunchKernelGGL((k_thermal_node<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N, (long)c->E, c->n,
+                     c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.dTedt, c->prm.dt,
+                     device_dt ? c->scalars + kCflDt : nullptr, c->nd.T);
+  return hipGetLastError();
 }
 
 // FP32-mixed runs keep the nodal velocity, previous acceleration and

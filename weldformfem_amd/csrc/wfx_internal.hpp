@@ -69,11 +69,20 @@ struct DevElem {
     *strain_pl_incr = nullptr;
   T *f_elem = nullptr, *f_hg = nullptr;
   T *He = nullptr;  // optional per-element bilinear plastic modulus (specimen batches)
+  T *q_plheat = nullptr, *dTedt = nullptr;  // thermal coupling: plastic heat, element-node rates [n][E]
+};
+
+// Thermal coupling (setThermalOn, Thermal.C): conductivity, heat capacity,
+// fraction of plastic work turned into heat (plHeatFrac)
+struct Thermal {
+  int on = 0;
+  double k_T = 0, cp_T = 0, plheat_frac = 0.9;
 };
 
 struct DevNode {
   double *x = nullptr, *v = nullptr, *a = nullptr, *u = nullptr, *u_dt = nullptr,
          *prev_a = nullptr, *fi = nullptr, *fe = nullptr;
+  double *T = nullptr;  // nodal temperature (thermal coupling)
   double *mdiag = nullptr, *voln = nullptr, *voln_sum = nullptr, *voln0_sum = nullptr;
   double *anp_ratio = nullptr;  // fused path: V_n / V0_n
 };
@@ -131,6 +140,7 @@ struct wfx_ctx {
   double *x_start = nullptr;
   int has_fe = 0;            // external nodal forces set (non-zero)
   double rho_uniform = 0.0;  // > 0: every element density equals this value
+  wfx::Thermal thermal;
   double *temp_e = nullptr;  // per-element temperature (else prm.mat.temp for all)
   double *temp_buf = nullptr;  // device buffer behind temp_e, kept once allocated
   int fused_bs[5] = {256, 256, 256, 256, 256};  // block size of fused phases 1..4

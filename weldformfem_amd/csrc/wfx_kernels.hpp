@@ -546,6 +546,9 @@ struct MatParams {
   double e_min, e_max, er_min, er_max, T_min, T_max;
   double temp, max_edot;                   // uniform temperature, tangent strain-rate cap
   const double *Te;                        // per-element temperature, or null
+  const double *Tn;                        // thermal coupling: nodal temperature, or null
+  const int *elnod;                        //   with the connectivity [nn][Es]
+  int nn, Es;
 };
 
 inline MatParams mat_params(const wfx_ctx *c) {
@@ -553,7 +556,21 @@ inline MatParams mat_params(const wfx_ctx *c) {
   return MatParams{m.model, m.G,     m.H,     m.sy0,   m.Kh,     m.mh,     m.eps0, m.eps1, m.E,
                    m.B,     m.n,     m.C,     m.eps_0, m.m,      m.T_m,    m.T_t,  m.C1,   m.C2,
                    m.n1,    m.n2,    m.m1,    m.m2,    m.I1,     m.I2,     m.e_min, m.e_max,
-                   m.er_min, m.er_max, m.T_min, m.T_max, m.temp, m.max_edot, c->temp_e};
+                   m.er_min, m.er_max, m.T_min, m.T_max, m.temp, m.max_edot, c->temp_e,
+                   c->thermal.on ? c->nd.T : nullptr, c->ms.elnod, c->n, c->E};
+}
+
+// The temperature the hardening law of element e reads: with thermal
+// coupling the mean of its nodal temperatures (the reference reads the nodal
+// array at the element index, Mechanical.C:1725-1729, defined only for
+// e < node count), else the element's or the uniform temperature.
+__device__ __forceinline__ double law_temperature(const MatParams &m, int e) {
+  if (m.Tn) {
+    double t = 0.0;
+    for (int a = 0; a < m.nn; ++a) t += m.Tn[m.elnod[(long)a * m.Es + e]];
+    return t / m.nn;
+  }
+  return m.Te ? m.Te[e] : m.temp;
 }
 
 // effective strain rate of CalcStressStrain (Mechanical.C:1700-1705)
@@ -582,12 +599,12 @@ __device__ __forceinline__ double clamp_range(double v, double lo, double hi) {
 enum { LAW_PLAIN = 0, LAW_RATE = 1 };
 inline int law_class(int model) { return model >= JOHNSON_COOK ? LAW_RATE : LAW_PLAIN; }
 
-// Calc*YieldStress (Material.cuh:353-364, :378-388, :412-441) of element e
-// at plastic strain ep and effective strain rate edot; bilinear as
+// Calc*YieldStress (Material.cuh:353-364, :378-388, :412-441) at plastic
+// strain ep, effective strain rate edot and temperature temp (the rate laws); bilinear as
 // sigma_y = sy0_e + H*ep (the reference leaves it undefined).  The
 // rate-dependent laws evaluate in double in both precisions.
 template <typename T, int LAW>
-__device__ __forceinline__ T yield_stress(const MatParams &m, int e, T sy0e, T ep, double edot) {
+__device__ __forceinline__ T yield_stress(const MatParams &m, double temp, T sy0e, T ep, double edot) {
   if constexpr (LAW == LAW_PLAIN) {
     if (m.model == HOLLOMON) {
       double s = (double)ep + m.eps0;
@@ -595,7 +612,6 @@ __device__ __forceinline__ T yield_stress(const MatParams &m, int e, T sy0e, T e
     }
     return sy0e + (T)m.H * ep;
   } else if (m.model == JOHNSON_COOK) {
-    const double temp = m.Te ? m.Te[e] : m.temp;
     const double T_h = (temp - m.T_t) / (m.T_m - m.T_t);
     const double sr = (edot == 0.0) ? 1.e-5 : edot;
     return (T)(((double)sy0e + m.B * pow((double)ep, m.n)) * (1.0 + m.C * log(sr / m.eps_0)) *
@@ -604,7 +620,7 @@ __device__ __forceinline__ T yield_stress(const MatParams &m, int e, T sy0e, T e
   if (m.model == GMT) {
     const double es = clamp_range((double)ep, m.e_min, m.e_max);
     const double er = clamp_range(edot, m.er_min, m.er_max);
-    const double Tc = clamp_range(m.Te ? m.Te[e] : m.temp, m.T_min, m.T_max);
+    const double Tc = clamp_range(temp, m.T_min, m.T_max);
     return (T)(m.C1 * exp(m.C2 * Tc) * pow(es, m.n1 * Tc + m.n2) * exp((m.I1 * Tc + m.I2) / es) *
                pow(er, m.m1 * Tc + m.m2));
   }
@@ -613,7 +629,7 @@ __device__ __forceinline__ T yield_stress(const MatParams &m, int e, T sy0e, T e
 // Calc*TangentModulus (Material.cuh:389-395, :397-411, :444-473); edot is
 // the capped rate.
 template <typename T, int LAW>
-__device__ __forceinline__ T tangent_modulus(const MatParams &m, int e, T ep, double edot) {
+__device__ __forceinline__ T tangent_modulus(const MatParams &m, double temp, T ep, double edot) {
   if constexpr (LAW == LAW_PLAIN) {
     if (m.model == HOLLOMON) {
       double s = (double)ep + m.eps0;
@@ -621,7 +637,6 @@ __device__ __forceinline__ T tangent_modulus(const MatParams &m, int e, T ep, do
     }
     return (T)m.H;
   } else if (m.model == JOHNSON_COOK) {
-    const double temp = m.Te ? m.Te[e] : m.temp;
     const double T_h = (temp - m.T_t) / (m.T_m - m.T_t);
     if (ep > T(0))
       return (T)(m.n * m.B * pow((double)ep, m.n - 1.) * (1.0 + m.C * log(edot / m.eps_0)) *
@@ -631,10 +646,20 @@ __device__ __forceinline__ T tangent_modulus(const MatParams &m, int e, T ep, do
   if (m.model == GMT) {
     const double es = clamp_range((double)ep, m.e_min, m.e_max);
     const double er = clamp_range(edot, m.er_min, m.er_max);
-    const double Tc = clamp_range(m.Te ? m.Te[e] : m.temp, m.T_min, m.T_max);
+    const double Tc = clamp_range(temp, m.T_min, m.T_max);
     return (T)(m.C1 * exp(m.C2 * Tc) * pow(er, m.m1 * Tc + m.m2) * pow(es, Tc * m.n1 + m.n2 - 2.0) *
                (-m.I1 * Tc - m.I2 + es * (m.n1 * Tc + m.n2)) * exp((m.I1 * Tc + m.I2) / es));
   }
+}
+
+// plastic heat source (Mechanical.C:1798-1810): frac * sigma : depdt with
+// depdt = (1/dt) * strain_pl_incr
+template <typename T>
+__device__ __forceinline__ T plastic_heat(const T3<T> &sig, T i0, T i1, T i2, T i3, T i4, T i5, T dt,
+                                          double frac) {
+  const T r = T(1) / dt;
+  return (T)frac * (sig.xx * (r * i0) + T(2) * sig.xy * (r * i3) + T(2) * sig.xz * (r * i5) +
+                    sig.yy * (r * i1) + T(2) * sig.yz * (r * i4) + sig.zz * (r * i2));
 }
 
 // CalcStressStrain (Mechanical.C:1664-1839)
@@ -643,7 +668,7 @@ __global__ __launch_bounds__(kBlock) void k_stress_strain(
     int E, T dt, MatParams mp, const T *__restrict__ sr, const T *__restrict__ rr,
     const T *__restrict__ p, const T *__restrict__ sy0, const T *__restrict__ He, T *__restrict__ tau_io,
     T *__restrict__ eps_io, T *__restrict__ sigma, T *__restrict__ plincr,
-    T *__restrict__ pl_strain, T *__restrict__ sigma_y) {
+    T *__restrict__ pl_strain, T *__restrict__ sigma_y, T *__restrict__ q_plheat, double plheat_frac) {
   int e = gtid();
   if (e >= E) return;
   const long Es = E;
@@ -675,11 +700,12 @@ __global__ __launch_bounds__(kBlock) void k_stress_strain(
   // bilinear plastic modulus: per element when given (specimen batches)
   const bool per_elem = LAW == LAW_PLAIN && mp.model == BILINEAR && He;
   const double edot = (LAW == LAW_RATE) ? eff_strain_rate(D) : 0.0;
-  T sy = per_elem ? sy0[e] + He[e] * ep : yield_stress<T, LAW>(mp, e, sy0[e], ep, edot);
+  const double temp = (LAW == LAW_RATE) ? law_temperature(mp, e) : 0.0;
+  T sy = per_elem ? sy0[e] + He[e] * ep : yield_stress<T, LAW>(mp, temp, sy0[e], ep, edot);
   sigma_y[e] = sy;
   T dep = T(0);
   if (sy < sig_trial) {
-    T H = per_elem ? He[e] : tangent_modulus<T, LAW>(mp, e, ep, cap_rate(mp, edot));
+    T H = per_elem ? He[e] : tangent_modulus<T, LAW>(mp, temp, ep, cap_rate(mp, edot));
     T dgamma = (sig_trial - sy) / (T(3) * G + H);
     T factor = T(1) - (T(3) * G * dgamma) / sig_trial;
     tau = t3_scale(s, factor);
@@ -687,7 +713,7 @@ __global__ __launch_bounds__(kBlock) void k_stress_strain(
     dep = dgamma;
   }
   T3<T> sig = t3_add(pI, tau);
-  if (plincr) {
+  if (plincr || q_plheat) {
     T i0 = 0, i1 = 0, i2 = 0, i3 = 0, i4 = 0, i5 = 0;
     if (dep > T(0)) {
       T f = dep / sy;
@@ -698,8 +724,11 @@ __global__ __launch_bounds__(kBlock) void k_stress_strain(
       i4 = T(1.5) * f * (sig.yz);
       i5 = T(1.5) * f * (sig.xz);
     }
-    plincr[0 * Es + e] = i0; plincr[1 * Es + e] = i1; plincr[2 * Es + e] = i2;
-    plincr[3 * Es + e] = i3; plincr[4 * Es + e] = i4; plincr[5 * Es + e] = i5;
+    if (plincr) {
+      plincr[0 * Es + e] = i0; plincr[1 * Es + e] = i1; plincr[2 * Es + e] = i2;
+      plincr[3 * Es + e] = i3; plincr[4 * Es + e] = i4; plincr[5 * Es + e] = i5;
+    }
+    if (q_plheat) q_plheat[e] = (dep > T(0)) ? plastic_heat<T>(sig, i0, i1, i2, i3, i4, i5, dt, plheat_frac) : T(0);
   }
   eps_io[0 * Es + e] = eps_io[0 * Es + e] + D.xx * dt;
   eps_io[1 * Es + e] = eps_io[1 * Es + e] + D.yy * dt;
@@ -1056,7 +1085,8 @@ template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, d
                                                 : k_stress_strain<T, LAW_PLAIN>;                   \
     hipLaunchKernelGGL(kern, dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, (T)dt, mp, el.str_rate, \
                        el.rot_rate, el.p, el.sy0, el.He, el.tau, el.eps, el.sigma,                 \
-                       el.strain_pl_incr, el.pl_strain, el.sigma_y);                               \
+                       el.strain_pl_incr, el.pl_strain, el.sigma_y,                                \
+                       c->thermal.on ? el.q_plheat : (T *)nullptr, c->thermal.plheat_frac);        \
     return hipGetLastError();                                                                      \
   }                                                                                                \
   template <> hipError_t launch_artificial_viscosity<T>(wfx_ctx * c, hipStream_t s) {              \

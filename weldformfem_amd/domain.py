@@ -110,6 +110,15 @@ class Domain:
             a = np.ascontiguousarray(a.reshape(-1))
             check(self._L.wfx_set_elem_temperature(self._h, a.ctypes.data, a.size))
 
+    def setThermal(self, on=True, k_T=50.0, cp_T=460.0, plheat_frac=0.9):
+        """thermal coupling (setThermalOn, thermalCond, thermalHeatCap,
+        plHeatFrac); nodal temperatures in the field "T"
+        """
+        check(self._L.wfx_set_thermal(self._h, int(on), float(k_T), float(cp_T), float(plheat_frac)))
+
+    def ThermalCalcs(self):
+        self._call(self._L.wfx_thermal_calcs)
+
     def setMaxStrainRate(self, edot):
         """Domain_d::m_max_edot: cap of the rate the tangent modulus sees"""
         check(self._L.wfx_set_max_strain_rate(self._h, float(edot)))
