@@ -102,6 +102,41 @@ int wfx_set_thermal(wfx_ctx *ctx, int on, double k_T, double cp_T, double plheat
 /* Domain_d::ThermalCalcs (Thermal.C:27-115) on the state of the last
  * per-step API step */
 int wfx_thermal_calcs(wfx_ctx *ctx, wfx_stream stream);
+/* Rigid-surface penalty contact (Domain_d::CalcContactForces,
+ * Contact.C:31-335; TriMesh_d, Mesh.h; setTriMesh + setContactOn,
+ * main.C:636-848).  The rigid surfaces: node_count nodes [n][3] with their
+ * velocities [n][3] (NULL: at rest), tri_count triangles [t][3] (normal
+ * (x1-x0) x (x2-x0), pointing out of the die) and a surface id per triangle
+ * (NULL: all 0).  The body must be tetrahedral (the reference's
+ * external-face search works on tetrahedron faces, Domain_d.h:243-249); its
+ * external faces are found here (SearchExtNodes, Domain_d.C:110-191).  Each
+ * step then refreshes the external nodal areas every 10 steps, computes the
+ * contact forces after the element forces (first triangle a node is behind
+ * and inside of; penalty min(E A/l, 0.2 m/dt^2) with factor
+ * wfx_set_contact_params' penalty_factor, damping ratio 0.2, stick/slip
+ * Coulomb friction), adds them in calcAccel and moves the surfaces after the
+ * position update.  Nodes that leave contact keep their last contact force,
+ * as in the reference (its reset is commented out, Contact.C:54). */
+int wfx_set_rigid_mesh(wfx_ctx *ctx, int node_count, const double *nodes, const double *node_vel,
+                       int tri_count, const int *tris, const int *tri_mesh_id);
+/* the contact steps for a caller-driven loop (wfx_run and the fused path
+ * run them themselves): Domain_d::CalcExtFaceAreas (Domain_d.C:207-296),
+ * CalcContactForces (Contact.C:31-335, after calcElemHourglassForces),
+ * TriMesh_d::Move + CalcNormals + UpdatePlaneCoeff (Mesh.h:236-330, after
+ * UpdateCorrectionPos) */
+int wfx_calc_ext_face_areas(wfx_ctx *ctx, wfx_stream stream);
+int wfx_calc_contact_forces(wfx_ctx *ctx, wfx_stream stream);
+int wfx_rigid_update(wfx_ctx *ctx, wfx_stream stream);
+/* m_contPF ("penaltyFactor", default 0.1), fricCoeffStatic, fricCoeffDynamic
+ * (main.C:716-727) */
+int wfx_set_contact_params(wfx_ctx *ctx, double penalty_factor, double mu_static, double mu_dynamic);
+/* TriMesh_d::node_v: new rigid-node velocities [n][3] (the reference's start-up
+ * ramp of the die velocity, Solver_explicit.C:981-992, is a driver's choice) */
+int wfx_set_rigid_velocity(wfx_ctx *ctx, const double *node_vel);
+/* trimesh->react_force[mesh_id].z of Contact.C:318-321: the sum, in node
+ * order, of |normal + damping force| over the nodes in contact with that
+ * surface at the last step (synchronises) */
+int wfx_contact_force(wfx_ctx *ctx, int mesh_id, double *out);
 /* Domain_d::m_max_edot ("maxStrRate", main.C:228; default 1e6): cap of the
  * effective strain rate passed to the tangent modulus (Mechanical.C:1737) */
 int wfx_set_max_strain_rate(wfx_ctx *ctx, double edot);

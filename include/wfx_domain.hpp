@@ -74,6 +74,19 @@ class Domain_d {
     check(wfx_set_elem_temperature(c_, T.data(), (int)T.size()));
   }
   void setMaxStrainRate(const double &edot) { check(wfx_set_max_strain_rate(c_, edot)); }
+  // setTriMesh + setContactOn: rigid triangle surfaces for penalty contact
+  void setTriMesh(const std::vector<double> &nodes, const std::vector<int> &tris,
+                  const std::vector<double> &node_vel = {}, const std::vector<int> &mesh_id = {}) {
+    check(wfx_set_rigid_mesh(c_, (int)(nodes.size() / 3), nodes.data(), node_vel.empty() ? nullptr : node_vel.data(),
+                             (int)(tris.size() / 3), tris.data(), mesh_id.empty() ? nullptr : mesh_id.data()));
+    contact_ = true;
+  }
+  void setContactPF(double pf, double mu_static = 0.0, double mu_dynamic = 0.0) {
+    check(wfx_set_contact_params(c_, pf, mu_static, mu_dynamic));
+  }
+  double contactForce(int mesh_id = 0) { double f; check(wfx_contact_force(c_, mesh_id, &f)); return f; }
+  void CalcExtFaceAreas() { check(wfx_calc_ext_face_areas(c_, s_)); }
+  void CalcContactForces() { check(wfx_calc_contact_forces(c_, s_)); }
   // setThermalOn with the material's thermal constants
   void setThermal(bool on, double k_T, double cp_T, double plheat_fraction = 0.9) {
     check(wfx_set_thermal(c_, on ? 1 : 0, k_T, cp_T, plheat_fraction));
@@ -97,7 +110,7 @@ class Domain_d {
   double getDT() const { return dt_; }
 
   // ---- init block and per-step API (Solver_explicit.C order) ----
-  void InitSolve() { check(wfx_init(c_, s_)); }
+  void InitSolve() { check(wfx_init(c_, s_)); step_ = 0; }
   void UpdatePrediction() { check(wfx_update_prediction(c_, s_)); }
   void ImposeBCV(const int dim) { check(wfx_impose_bcv(c_, dim, s_)); }
   void ImposeBCA(const int dim) { check(wfx_impose_bca(c_, dim, s_)); }
@@ -146,6 +159,7 @@ class Domain_d {
   // The CPU-branch loop body of SolveChungHulbert written with the per-step
   // calls (remeshing, contact, thermal and output left out).
   void StepChungHulbert() {
+    if (contact_ && step_ % 10 == 0) CalcExtFaceAreas();
     UpdatePrediction();
     ImposeBCVAllDim();
     calcElemJAndDerivatives();
@@ -159,13 +173,16 @@ class Domain_d {
     calcArtificialViscosity();
     calcElemForces();
     calcElemHourglassForces();
+    if (contact_) CalcContactForces();
     assemblyForces();
     calcAccel();
     ImposeBCAAllDim();
     UpdateCorrectionAccVel();
     ImposeBCVAllDim();
     UpdateCorrectionPos();
+    if (contact_) check(wfx_rigid_update(c_, s_));
     if (thermal_) ThermalCalcs();
+    ++step_;
   }
 
   // SolveChungHulbert without output: init, then loop while Time < end_t.
@@ -201,7 +218,8 @@ class Domain_d {
   wfx_ctx *c_ = nullptr;
   wfx_stream s_ = nullptr;
   double dt_ = 0.0, end_t_ = 0.0, cfl_ = 0.0;
-  bool thermal_ = false;
+  bool thermal_ = false, contact_ = false;
+  long step_ = 0;
 };
 
 }  // namespace wfx

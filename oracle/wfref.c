@@ -103,6 +103,7 @@ wfref_domain *wfref_create(void) {
   wfref_domain *d = (wfref_domain *)calloc(1, sizeof(wfref_domain));
   wfref_set_ch_default(d);
   d->max_edot = 1.0e6; /* Domain_d.h:824 */
+  d->contPF = 0.1;     /* Domain_d.h:256 */
   /* m_artifvisc is left uninitialised by the C++ API drivers; zero here */
   return d;
 }
@@ -114,16 +115,26 @@ static void free_fields(wfref_domain *d) {
                    &d->pl_strain, &d->sigma_y, &d->sy0, &d->str_rate,
                    &d->rot_rate, &d->sigma, &d->tau, &d->eps,
                    &d->strain_pl_incr, &d->f_elem, &d->f_hg, &d->temp, &d->T,
-                   &d->q_plheat, &d->dTedt};
+                   &d->q_plheat, &d->dTedt, &d->node_area, &d->elem_length,
+                   &d->contforce, &d->ut_prev, &d->cfn};
   int **ip[] = {&d->elnod, &d->nodel, &d->nodel_loc, &d->nodel_offset,
-                &d->nodel_count};
+                &d->nodel_count, &d->ext_face, &d->ext_node, &d->mesh_in_contact};
   for (size_t i = 0; i < sizeof(dp) / sizeof(dp[0]); ++i) { free(*dp[i]); *dp[i] = NULL; }
   for (size_t i = 0; i < sizeof(ip) / sizeof(ip[0]); ++i) { free(*ip[i]); *ip[i] = NULL; }
+}
+
+static void free_rigid(wfref_domain *d) {
+  free(d->rnode); free(d->rvel); free(d->rtri); free(d->rmesh);
+  free(d->rnormal); free(d->rpplane);
+  d->rnode = d->rvel = d->rnormal = d->rpplane = NULL;
+  d->rtri = d->rmesh = NULL;
+  d->rn = d->rt = 0;
 }
 
 void wfref_destroy(wfref_domain *d) {
   if (!d) return;
   free_fields(d);
+  free_rigid(d);
   for (int k = 0; k < 3; ++k) { free(d->bc_nod[k]); free(d->bc_val[k]); }
   free(d->react_nod);
   free(d);
@@ -147,6 +158,9 @@ static void set_dimension(wfref_domain *d, int node_count, int elem_count) {
   RD(d->tau, 6 * E); RD(d->eps, 6 * E); RD(d->strain_pl_incr, 6 * E);
   RD(d->f_elem, 3 * n * E); RD(d->f_hg, 3 * n * E); RD(d->temp, E);
   RD(d->T, N); RD(d->q_plheat, E); RD(d->dTedt, n * E);
+  RD(d->node_area, N); RD(d->elem_length, E); RD(d->contforce, 3 * N); RD(d->ut_prev, 3 * N);
+  RD(d->cfn, N); RI(d->ext_node, N); RI(d->mesh_in_contact, N);
+  free(d->ext_face); d->ext_face = NULL; d->nf = 0; d->contact = 0;
   for (int e = 0; e < E; ++e) d->sy0[e] = d->mat.sy0;
   /* uniform initial temperature, IniTemp default (main.C:424-440) */
   for (int e = 0; e < E; ++e) d->temp[e] = 20.0;
@@ -878,6 +892,10 @@ void wfref_calc_accel(wfref_domain *d) {
   for (int i = 0; i < d->node_count; ++i)
     for (int c = 0; c < 3; ++c)
       d->a[3 * i + c] = (d->fe[3 * i + c] - d->fi[3 * i + c]) / d->mdiag[i];
+  if (d->contact) /* Mechanical.C:330-335 */
+#pragma omp parallel for
+    for (int i = 0; i < d->node_count; ++i)
+      for (int c = 0; c < 3; ++c) d->a[3 * i + c] += d->contforce[3 * i + c] / d->mdiag[i];
 }
 
 /* UpdateCorrectionAccVel (Domain_d.C:981-997) */
@@ -928,6 +946,12 @@ void wfref_init(wfref_domain *d) {
   wfref_calc_elem_density(d);
   wfref_calc_nodal_vol(d);
   wfref_calc_nodal_mass_from_vol(d);
+  if (d->contact) {
+    /* calcMinEdgeLength before the solve (main.C:862) sets m_elem_length */
+    wfref_calc_elem_length(d);
+    for (int i = 0; i < 3 * d->node_count; ++i) d->contforce[i] = d->ut_prev[i] = 0.0;
+    d->step_count = 0;
+  }
 }
 
 double wfref_reaction_force(wfref_domain *d, int dim) {
@@ -936,8 +960,6 @@ double wfref_reaction_force(wfref_domain *d, int dim) {
   return r;
 }
 
-/* One pass of the SolveChungHulbert loop body, CPU branch, in its order
- * (Solver_explicit.C:524-978; remesh, contact, thermal and I/O left out). */
 /* setThermalOn and the thermal constants (the nodal temperatures stay as
  * set: 20 at mesh creation, setTemp through the "T" field) */
 void wfref_set_thermal(wfref_domain *d, int on, double k_T, double cp_T, double plheat_frac) {
@@ -1009,11 +1031,244 @@ static void cfl_time_step(wfref_domain *d) {
   d->dt = d->cfl * minl / (mat_cs + max_vel);
 }
 
+
+/* ------------------------------------------------------------------------ */
+/* rigid-surface penalty contact                                            */
+/* ------------------------------------------------------------------------ */
+typedef struct { double x, y, z; } v3;
+static v3 v3s(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static v3 v3at(const double *p, int i) { return v3s(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+static v3 v3sub(v3 a, v3 b) { return v3s(a.x - b.x, a.y - b.y, a.z - b.z); }
+static v3 v3add(v3 a, v3 b) { return v3s(a.x + b.x, a.y + b.y, a.z + b.z); }
+static v3 v3mul(v3 a, double f) { return v3s(a.x * f, a.y * f, a.z * f); }
+static v3 v3smul(double f, v3 a) { return v3s(f * a.x, f * a.y, f * a.z); }
+static v3 v3div(v3 a, double f) { return v3s(a.x / f, a.y / f, a.z / f); }
+static double v3dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static v3 v3cross(v3 a, v3 b) {
+  return v3s(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static double v3len(v3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+/* CalcNormals (Mesh.h:236-248) and UpdatePlaneCoeff with nfar = 2
+ * (CalcSpheres leaves the last node, Mesh.h:312-326; :275-283) */
+static void rigid_planes(wfref_domain *d) {
+  for (int e = 0; e < d->rt; ++e) {
+    const int *t = d->rtri + 3 * e;
+    v3 u = v3sub(v3at(d->rnode, t[1]), v3at(d->rnode, t[0]));
+    v3 v = v3sub(v3at(d->rnode, t[2]), v3at(d->rnode, t[0]));
+    v3 w = v3cross(u, v);
+    v3 nrm = v3div(w, v3len(w));
+    d->rnormal[3 * e] = nrm.x; d->rnormal[3 * e + 1] = nrm.y; d->rnormal[3 * e + 2] = nrm.z;
+    d->rpplane[e] = v3dot(v3at(d->rnode, t[2]), nrm);
+  }
+}
+
+/* tetra_faces (Domain_d.h:184-186) */
+static const int TET_FACES[4][3] = {{0, 1, 2}, {0, 1, 3}, {1, 2, 3}, {0, 2, 3}};
+
+static int cmp_key(const void *a, const void *b) {
+  const long long *x = (const long long *)a, *y = (const long long *)b;
+  if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+  return 0;
+}
+
+/* SearchExtNodes (Domain_d.C:110-191): every tet face in element and
+ * tetra_faces order, deduplicated by node set; faces met once are external,
+ * kept in that order with the nodes of their element's table entry. */
+static void search_ext_nodes(wfref_domain *d) {
+  int E = d->elem_count, N = d->node_count, F = 4 * E;
+  long long (*key)[2] = malloc(sizeof(*key) * (size_t)(F > 0 ? F : 1));
+  for (int e = 0; e < E; ++e)
+    for (int f = 0; f < 4; ++f) {
+      int a = d->elnod[4 * e + TET_FACES[f][0]], b = d->elnod[4 * e + TET_FACES[f][1]],
+          c = d->elnod[4 * e + TET_FACES[f][2]], t;
+      if (a > b) { t = a; a = b; b = t; }
+      if (b > c) { t = b; b = c; c = t; }
+      if (a > b) { t = a; a = b; b = t; }
+      key[4 * e + f][0] = ((long long)a * N + b) * (long long)N + c;
+      key[4 * e + f][1] = 4 * e + f;
+    }
+  qsort(key, (size_t)F, sizeof(*key), cmp_key);
+  int *cnt = calloc((size_t)(F > 0 ? F : 1), sizeof(int));
+  for (int k = 0; k < F;) {
+    int j = k;
+    while (j < F && key[j][0] == key[k][0]) ++j;
+    for (int q = k; q < j; ++q) cnt[key[q][1]] = j - k;
+    k = j;
+  }
+  free(d->ext_face);
+  d->ext_face = malloc(sizeof(int) * 3 * (size_t)(F > 0 ? F : 1));
+  d->nf = 0;
+  for (int i = 0; i < N; ++i) d->ext_node[i] = 0;
+  for (int g = 0; g < F; ++g) {
+    if (cnt[g] != 1) continue;
+    int e = g / 4, f = g % 4;
+    for (int k = 0; k < 3; ++k) {
+      int nd = d->elnod[4 * e + TET_FACES[f][k]];
+      d->ext_face[3 * d->nf + k] = nd;
+      d->ext_node[nd] = 1;
+    }
+    d->nf++;
+  }
+  free(cnt);
+  free(key);
+}
+
+/* CalcExtFaceAreas (Domain_d.C:207-296), 3D: a third of each external
+ * triangle's area to each of its nodes */
+void wfref_calc_ext_face_areas(wfref_domain *d) {
+  for (int i = 0; i < d->node_count; ++i) d->node_area[i] = 0.0;
+  for (int f = 0; f < d->nf; ++f) {
+    const int *t = d->ext_face + 3 * f;
+    v3 p0 = v3at(d->x, t[0]), p1 = v3at(d->x, t[1]), p2 = v3at(d->x, t[2]);
+    v3 c = v3cross(v3sub(p1, p0), v3sub(p2, p0));
+    double area = 0.5 * v3len(c);
+    double share = area / 3.0;
+    d->node_area[t[0]] += share;
+    d->node_area[t[1]] += share;
+    d->node_area[t[2]] += share;
+  }
+}
+
+/* m_elem_length of calcMinEdgeLength (Domain_d.C:2242-2276), tetrahedra:
+ * the smallest height over the four faces */
+void wfref_calc_elem_length(wfref_domain *d) {
+  for (int e = 0; e < d->elem_count; ++e) {
+    const int *en = d->elnod + 4 * e;
+    v3 P[4];
+    for (int k = 0; k < 4; ++k) P[k] = v3at(d->x, en[k]);
+    const int fc[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+    double hmin = 1.0e6;
+    for (int i = 0; i < 4; ++i) {
+      v3 x1 = v3sub(P[fc[i][1]], P[fc[i][0]]), x2 = v3sub(P[fc[i][2]], P[fc[i][0]]);
+      v3 nrm = v3cross(x1, x2);
+      double area = v3len(nrm);
+      if (area < 1e-12) continue;
+      nrm = v3div(nrm, area);
+      double h = fabs(v3dot(v3sub(P[i], P[fc[i][0]]), nrm));
+      if (h < hmin) hmin = h;
+    }
+    d->elem_length[e] = hmin;
+  }
+}
+
+int wfref_set_rigid_mesh(wfref_domain *d, int nn, const double *nodes, const double *vel, int nt,
+                         const int *tris, const int *mesh_id) {
+  if (d->nodxelem != 4 || nn <= 0 || nt <= 0) return -1;
+  free_rigid(d);
+  d->rn = nn; d->rt = nt;
+  RD(d->rnode, 3 * nn); RD(d->rvel, 3 * nn); RI(d->rtri, 3 * nt); RI(d->rmesh, nt);
+  RD(d->rnormal, 3 * nt); RD(d->rpplane, nt);
+  memcpy(d->rnode, nodes, sizeof(double) * 3 * (size_t)nn);
+  if (vel) memcpy(d->rvel, vel, sizeof(double) * 3 * (size_t)nn);
+  memcpy(d->rtri, tris, sizeof(int) * 3 * (size_t)nt);
+  if (mesh_id) memcpy(d->rmesh, mesh_id, sizeof(int) * (size_t)nt);
+  rigid_planes(d);
+  search_ext_nodes(d);
+  wfref_calc_ext_face_areas(d);
+  d->contact = 1;
+  return 0;
+}
+
+/* TriMesh_d::Move + CalcNormals + UpdatePlaneCoeff after the position
+ * update (Solver_explicit.C:996-1004) */
+void wfref_rigid_update(wfref_domain *d) {
+  for (int n = 0; n < d->rn; ++n)
+    for (int c = 0; c < 3; ++c) d->rnode[3 * n + c] = d->rnode[3 * n + c] + d->rvel[3 * n + c] * d->dt;
+  rigid_planes(d);
+}
+
+/* CalcContactForces (Contact.C:31-335): for every external node, the first
+ * rigid triangle (in order) it lies behind and inside of gives a penalty
+ * force along the triangle normal with critical damping ratio 0.2, plus a
+ * stick/slip Coulomb friction force from the accumulated tangential
+ * displacement.  Nodes out of contact keep their previous contact force
+ * (the reference's reset is commented out, Contact.C:54). */
+void wfref_calc_contact_forces(wfref_domain *d) {
+  int N = d->node_count;
+  double dt = d->dt;
+#pragma omp parallel for
+  for (int i = 0; i < N; ++i) {
+    d->mesh_in_contact[i] = -1;
+    if (!d->ext_node[i]) continue;
+    v3 xi = v3at(d->x, i), vi = v3at(d->v, i);
+    for (int j = 0; j < d->rt; ++j) {
+      v3 nj = v3at(d->rnormal, j);
+      double dd = v3dot(nj, xi) - d->rpplane[j];
+      if (!(dd < 0)) continue;
+      v3 Qj = v3sub(xi, v3smul(dd, nj));
+      int inside = 1;
+      for (int l = 0; l < 3 && inside; ++l) {
+        int m = l + 1;
+        if (m > 2) m = 0;
+        v3 Pl = v3at(d->rnode, d->rtri[3 * j + l]), Pm = v3at(d->rnode, d->rtri[3 * j + m]);
+        double crit = v3dot(v3cross(v3sub(Pm, Pl), v3sub(Qj, Pl)), nj);
+        if (crit < 0.0) inside = 0;
+      }
+      if (!inside) continue;
+      double nodlen = 0.0;
+      for (int k = 0; k < d->nodel_count[i]; ++k)
+        nodlen += d->elem_length[d->nodel[d->nodel_offset[i] + k]];
+      nodlen /= d->nodel_count[i];
+      double m = d->mdiag[i];
+      double v_reln = v3dot(vi, nj);
+      double kcont_geo = d->mat.E * d->node_area[i] / nodlen;
+      double kcont_mass = 0.2 * m / (dt * dt);
+      double kcont = (kcont_mass < kcont_geo) ? kcont_mass : kcont_geo; /* std::min */
+      double omega = sqrt(kcont / m);
+      double ccrit = 2.0 * m * omega;
+      double F_damp = 0.2 * ccrit * v_reln;
+      double F_normal = d->contPF * kcont * dd;
+      v3 cf = v3smul(-(F_normal + F_damp), nj);
+      double *cfo = d->contforce + 3 * i;
+      cfo[0] = cf.x; cfo[1] = cf.y; cfo[2] = cf.z;
+      d->cfn[i] = v3len(cf);
+      d->mesh_in_contact[i] = d->rmesh[j];
+      /* friction (Contact.C:248-306) */
+      v3 v_tan = v3sub(vi, v3smul(v3dot(vi, nj), nj));
+      v3 du_t = v3mul(v_tan, dt);
+      double *ut = d->ut_prev + 3 * i;
+      v3 ut_acc = v3add(v3s(ut[0], ut[1], ut[2]), du_t);
+      ut[0] += du_t.x; ut[1] += du_t.y; ut[2] += du_t.z;
+      v3 Ft_trial = v3smul(-kcont, ut_acc);
+      double Ft_mag = v3len(Ft_trial);
+      v3 Fn = v3smul(v3dot(cf, nj), nj);
+      double Ft_max_static = d->mu_sta * v3len(Fn);
+      v3 Ft;
+      if (Ft_mag <= Ft_max_static) {
+        Ft = Ft_trial;
+      } else {
+        double Ft_max_dynamic = d->mu_dyn * v3len(Fn);
+        Ft = v3div(v3smul(-Ft_max_dynamic, v_tan), v3len(v_tan));
+        ut[0] = ut[1] = ut[2] = 0.0;
+      }
+      cfo[0] += Ft.x; cfo[1] += Ft.y; cfo[2] += Ft.z;
+      break; /* one master element per slave node */
+    }
+  }
+}
+
+/* trimesh->react_force[m].z of Contact.C:318-321: sum of |normal + damping
+ * force| over the nodes in contact with rigid mesh m, in node order */
+double wfref_contact_force(wfref_domain *d, int mesh_id) {
+  double r = 0.0;
+  for (int i = 0; i < d->node_count; ++i)
+    if (d->mesh_in_contact[i] == mesh_id) r += d->cfn[i];
+  return r;
+}
+
+/* One pass of the SolveChungHulbert loop body, CPU branch, in its order
+ * (Solver_explicit.C:524-1012; remesh and I/O left out). */
 void wfref_step(wfref_domain *d, double *reaction_z) {
+  /* external face areas every 10 steps (Solver_explicit.C:446-451) */
+  if (d->contact && d->step_count % 10 == 0) wfref_calc_ext_face_areas(d);
   wfref_update_prediction(d);
   for (int c = 0; c < 3; ++c) wfref_impose_bcv(d, c);
   wfref_calc_elem_jacobians(d);
-  if (d->cfl > 0.0) cfl_time_step(d);
+  if (d->cfl > 0.0) {
+    cfl_time_step(d);
+    if (d->contact) wfref_calc_elem_length(d); /* calcMinEdgeLength */
+  }
   wfref_calc_elem_vol(d);
   wfref_calc_nodal_vol(d);
   wfref_calc_nodal_mass_from_vol(d);
@@ -1023,6 +1278,7 @@ void wfref_step(wfref_domain *d, double *reaction_z) {
   wfref_calc_artificial_viscosity(d);
   wfref_calc_elem_forces(d);
   wfref_calc_elem_hourglass_forces(d);
+  if (d->contact) wfref_calc_contact_forces(d);
   wfref_assembly_forces(d);
   wfref_scrub_nonfinite_fi(d);
   if (reaction_z) *reaction_z = wfref_reaction_force(d, 2);
@@ -1031,7 +1287,9 @@ void wfref_step(wfref_domain *d, double *reaction_z) {
   wfref_update_correction_accvel(d);
   for (int c = 0; c < 3; ++c) wfref_impose_bcv(d, c);
   wfref_update_correction_pos(d);
+  if (d->contact) wfref_rigid_update(d);
   if (d->thermal) wfref_thermal_calcs(d);
+  d->step_count++;
 }
 
 void wfref_run(wfref_domain *d, int nsteps, double *hist) {
@@ -1070,6 +1328,12 @@ void *wfref_field(wfref_domain *d, const char *name, long *count, int *is_int) {
       {"rho0", d->rho0, E, 0}, {"p", d->p, E, 0}, {"pl_strain", d->pl_strain, E, 0},
       {"sigma_y", d->sigma_y, E, 0}, {"sy0", d->sy0, E, 0}, {"temp", d->temp, E, 0}, {"T", d->T, N, 0},
       {"q_plheat", d->q_plheat, E, 0}, {"dTedt", d->dTedt, n * E, 0},
+      {"node_area", d->node_area, N, 0}, {"elem_length", d->elem_length, E, 0},
+      {"contforce", d->contforce, 3 * N, 0}, {"ut_prev", d->ut_prev, 3 * N, 0},
+      {"cfn", d->cfn, N, 0}, {"mesh_in_contact", d->mesh_in_contact, N, 1},
+      {"ext_node", d->ext_node, N, 1}, {"rigid_x", d->rnode, 3 * (long)d->rn, 0},
+      {"rigid_v", d->rvel, 3 * (long)d->rn, 0}, {"rigid_normal", d->rnormal, 3 * (long)d->rt, 0},
+      {"rigid_pplane", d->rpplane, d->rt, 0},
       {"str_rate", d->str_rate, 6 * E, 0}, {"rot_rate", d->rot_rate, 6 * E, 0},
       {"sigma", d->sigma, 6 * E, 0}, {"tau", d->tau, 6 * E, 0}, {"eps", d->eps, 6 * E, 0},
       {"strain_pl_incr", d->strain_pl_incr, 6 * E, 0}, {"f_elem", d->f_elem, 3 * n * E, 0},
@@ -1113,7 +1377,8 @@ int wfref_set_double(wfref_domain *d, const char *name, double v) {
       {"pspg_scale", &d->pspg_scale}, {"p_pspg_bulkfac", &d->p_pspg_bulkfac},
       {"av_coeff_div", &d->av_coeff_div}, {"av_coeff_bulk", &d->av_coeff_bulk},
       {"artifvisc0", &d->artifvisc[0]}, {"artifvisc1", &d->artifvisc[1]},
-      {"hg_coeff", &d->hg_coeff}, {"max_edot", &d->max_edot}, {"cfl", &d->cfl}};
+      {"hg_coeff", &d->hg_coeff}, {"max_edot", &d->max_edot}, {"cfl", &d->cfl},
+      {"contPF", &d->contPF}, {"mu_sta", &d->mu_sta}, {"mu_dyn", &d->mu_dyn}};
   for (size_t k = 0; k < sizeof(t) / sizeof(t[0]); ++k)
     if (strcmp(t[k].nm, name) == 0) { *t[k].p = v; return 0; }
   return -1;

@@ -69,6 +69,21 @@ typedef struct {
   double *T;          /* N */
   double *q_plheat;   /* E */
   double *dTedt;      /* nE, [e][a] */
+  /* rigid-surface penalty contact (Contact.C:31-335, TriMesh_d Mesh.h) */
+  int contact;
+  double contPF, mu_sta, mu_dyn;   /* m_contPF (Domain_d.h:256), friction */
+  int rn, rt;                      /* rigid nodes, rigid triangles */
+  double *rnode, *rvel;            /* 3rn */
+  int *rtri, *rmesh;               /* 3rt, rt */
+  double *rnormal, *rpplane;       /* 3rt, rt */
+  int nf;                          /* external faces (count 1), face-list order */
+  int *ext_face;                   /* 3nf */
+  int *ext_node;                   /* N: 1 on an external face */
+  double *node_area, *elem_length; /* N, E */
+  double *contforce, *ut_prev;     /* 3N, persistent */
+  int *mesh_in_contact;            /* N */
+  double *cfn;                     /* N */
+  long step_count;
   double max_edot;    /* strain-rate cap of the tangent (Domain_d.h:824) */
   double cfl;         /* > 0: per-step dt = cfl*minl/(cs+max|v|); 0: fixed */
   /* material (one shared law; per-element initial yield in sy0[]) */
@@ -114,6 +129,16 @@ void wfref_set_elem_sy0(wfref_domain *d, const double *sy0);
  * :590-592) */
 void wfref_set_thermal(wfref_domain *d, int on, double k_T, double cp_T, double plheat_frac);
 void wfref_thermal_calcs(wfref_domain *d);
+/* rigid triangle surfaces (TriMesh_d: nodes, node velocities, triangles,
+ * mesh id per triangle) for penalty contact on a tetrahedral body; turns
+ * contact on, finds the external faces (SearchExtNodes) */
+int wfref_set_rigid_mesh(wfref_domain *d, int nn, const double *nodes, const double *vel, int nt,
+                         const int *tris, const int *mesh_id);
+void wfref_calc_ext_face_areas(wfref_domain *d);
+void wfref_calc_elem_length(wfref_domain *d);
+void wfref_calc_contact_forces(wfref_domain *d);
+void wfref_rigid_update(wfref_domain *d);
+double wfref_contact_force(wfref_domain *d, int mesh_id);
 /* the hardening law of one element, exposed for the pin tests */
 double wfref_eff_strain_rate(const double D[6]);
 double wfref_yield_stress(const wfref_domain *d, int e, double ep, double edot);

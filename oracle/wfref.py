@@ -42,6 +42,12 @@ def lib():
             "wfref_set_elem_sy0": (None, [vp, dp]),
             "wfref_set_material_jc": (None, [vp, d, d, d, d, d, d, d, d, d, d, d]),
             "wfref_set_thermal": (None, [vp, i, d, d, d]),
+            "wfref_set_rigid_mesh": (i, [vp, i, dp, dp, i, ip, ip]),
+            "wfref_calc_ext_face_areas": (None, [vp]),
+            "wfref_calc_elem_length": (None, [vp]),
+            "wfref_calc_contact_forces": (None, [vp]),
+            "wfref_rigid_update": (None, [vp]),
+            "wfref_contact_force": (d, [vp, i]),
             "wfref_thermal_calcs": (None, [vp]),
             "wfref_set_material_gmt": (None, [vp, d, d, d, d, dp, dp]),
             "wfref_eff_strain_rate": (d, [dp]),
@@ -160,6 +166,37 @@ class RefDomain:
 
     def ThermalCalcs(self):
         self._L.wfref_thermal_calcs(self._d)
+
+    def setRigidMesh(self, nodes, tris, vel=None, mesh_id=None):
+        """rigid triangle surface for penalty contact (tetrahedral body)"""
+        x = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1)
+        t = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1)
+        v = np.zeros_like(x) if vel is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(vel, dtype=np.float64), (x.size // 3, 3))).reshape(-1)
+        m = np.zeros(t.size // 3, np.int32) if mesh_id is None else np.ascontiguousarray(mesh_id, dtype=np.int32)
+        rc = self._L.wfref_set_rigid_mesh(self._d, x.size // 3, _ptr(x), _ptr(v), t.size // 3, _ptr(t), _ptr(m))
+        if rc:
+            raise ValueError("rigid mesh: needs a tetrahedral body and a non-empty surface")
+
+    def setContactParams(self, penalty_factor=0.1, mu_static=0.0, mu_dynamic=0.0):
+        self.set("contPF", penalty_factor)
+        self.set("mu_sta", mu_static)
+        self.set("mu_dyn", mu_dynamic)
+
+    def CalcExtFaceAreas(self):
+        self._L.wfref_calc_ext_face_areas(self._d)
+
+    def calcElemLength(self):
+        self._L.wfref_calc_elem_length(self._d)
+
+    def CalcContactForces(self):
+        self._L.wfref_calc_contact_forces(self._d)
+
+    def rigidUpdate(self):
+        self._L.wfref_rigid_update(self._d)
+
+    def contactForce(self, mesh_id=0):
+        return self._L.wfref_contact_force(self._d, int(mesh_id))
 
     def setMaxStrainRate(self, edot):
         self.set("max_edot", edot)

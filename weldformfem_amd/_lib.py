@@ -39,6 +39,13 @@ SIGNATURES = {
     "wfx_set_max_strain_rate": (_i, [_vp, _d]),
     "wfx_set_cfl": (_i, [_vp, _d]),
     "wfx_set_thermal": (_i, [_vp, _i, _d, _d, _d]),
+    "wfx_set_rigid_mesh": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp]),
+    "wfx_set_contact_params": (_i, [_vp, _d, _d, _d]),
+    "wfx_set_rigid_velocity": (_i, [_vp, _vp]),
+    "wfx_contact_force": (_i, [_vp, _i, _vp]),
+    "wfx_calc_ext_face_areas": (_i, [_vp, _vp]),
+    "wfx_calc_contact_forces": (_i, [_vp, _vp]),
+    "wfx_rigid_update": (_i, [_vp, _vp]),
     "wfx_thermal_calcs": (_i, [_vp, _vp]),
     "wfx_get_dt": (_i, [_vp, _vp]),
     "wfx_calc_cfl_dt": (_i, [_vp, _vp, _vp]),

@@ -342,6 +342,7 @@ static int check_supported(wfx_ctx *c) {
   REQUIRE(c->prm.mat.model == WFX_BILINEAR, WFX_ERR_STATE, "the adjoint supports the bilinear material");
   REQUIRE(!(c->prm.cfl > 0.0), WFX_ERR_STATE, "the adjoint needs a fixed time step (wfx_set_cfl(ctx, 0))");
   REQUIRE(!c->thermal.on, WFX_ERR_STATE, "the adjoint does not cover thermal coupling");
+  REQUIRE(!c->contact.on, WFX_ERR_STATE, "the adjoint does not cover contact");
   REQUIRE(s.alpha_free == 0.0 && s.log_factor == 0.0 && s.J_min == 0.0 && s.hg_coeff_free == 0.0 &&
               s.pspg_scale == 0.0 && s.p_pspg_bulkfac == 0.0 && s.av_coeff_div == 0.0 && s.av_coeff_bulk == 0.0 &&
               s.artifvisc0 == 0.0 && s.artifvisc1 == 0.0,

@@ -1,6 +1,7 @@
 // C-ABI of the wfx library: context lifetime, mesh and set-up, the per-step
 // Domain_d API and state transfer.  Kernels live in wfx_k64.hip / wfx_k32.hip.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -56,6 +57,7 @@ static void free_all(wfx_ctx *c) {
   c->v32 = c->pa32 = c->ud32 = c->fi32 = nullptr;
   c->x_start = nullptr;
   c->temp_e = c->temp_buf = nullptr;
+  c->contact = wfx::Contact();
   c->has_fe = 0;
   c->fused_stale = false;
   c->scalars = nullptr;
@@ -244,6 +246,15 @@ static bool find_field(wfx_ctx *c, const char *name, FieldDesc &f) {
   else if (s == "f_elem") el(E64(c->e64.f_elem, c->e32.f_elem), (int)(3 * n));
   else if (s == "f_hg") el(E64(c->e64.f_hg, c->e32.f_hg), (int)(3 * n));
   else if (s == "T") node1(c->nd.T);
+  else if (c->contact.on && (s == "contforce" || s == "ut_prev"))
+    node3(s == "contforce" ? c->contact.contforce : c->contact.ut_prev);
+  else if (c->contact.on && s == "node_area") node1(c->contact.node_area);
+  else if (c->contact.on && s == "cfn") node1(c->contact.cfn);
+  else if (c->contact.on && s == "mesh_in_contact") { f.dev = c->contact.mesh_in_contact; f.cnt = N; f.is_int = 1; }
+  else if (c->contact.on && s == "elem_length") { f.dev = c->contact.elem_length; f.cnt = E; }
+  else if (c->contact.on && s == "rigid_x") { f.dev = c->contact.rnode; f.cnt = 3L * c->contact.rn; }
+  else if (c->contact.on && s == "rigid_normal") { f.dev = c->contact.rnormal; f.cnt = 3L * c->contact.rt; }
+  else if (c->contact.on && s == "rigid_pplane") { f.dev = c->contact.rpplane; f.cnt = c->contact.rt; }
   else if (s == "q_plheat" && E64(c->e64.q_plheat, c->e32.q_plheat)) el(E64(c->e64.q_plheat, c->e32.q_plheat), 1);
   else return false;
   return true;
@@ -469,6 +480,170 @@ int wfx_set_temperature(wfx_ctx *c, double T) {
     std::vector<double> t0(c->N, T);
     WFX_CHECK_HIP(hipMemcpy(c->nd.T, t0.data(), t0.size() * sizeof(double), hipMemcpyHostToDevice));
   }
+  return WFX_OK;
+}
+
+// ---- rigid-surface contact --------------------------------------------------
+// SearchExtNodes (Domain_d.C:110-191) for tetrahedra: every face in element
+// and tetra_faces order (Domain_d.h:184-186), deduplicated by node set; the
+// faces met once are external and keep that order and their element's node
+// order.  External nodes list their external faces in face order.
+static void external_faces(const wfx_ctx *c, std::vector<int> &faces, std::vector<int> &ext_nodes,
+                           std::vector<int> &afc_off, std::vector<int> &afc_face) {
+  static const int tf[4][3] = {{0, 1, 2}, {0, 1, 3}, {1, 2, 3}, {0, 2, 3}};
+  const int E = c->E, N = c->N;
+  std::vector<std::pair<std::array<int, 3>, int>> key((size_t)4 * E);
+  for (int e = 0; e < E; ++e)
+    for (int f = 0; f < 4; ++f) {
+      std::array<int, 3> k = {c->elnod_h[4 * e + tf[f][0]], c->elnod_h[4 * e + tf[f][1]],
+                              c->elnod_h[4 * e + tf[f][2]]};
+      std::sort(k.begin(), k.end());
+      key[4 * e + f] = {k, 4 * e + f};
+    }
+  std::sort(key.begin(), key.end());
+  std::vector<int> cnt((size_t)4 * E, 0);
+  for (size_t a = 0; a < key.size();) {
+    size_t b = a;
+    while (b < key.size() && key[b].first == key[a].first) ++b;
+    for (size_t q = a; q < b; ++q) cnt[key[q].second] = (int)(b - a);
+    a = b;
+  }
+  faces.clear();
+  std::vector<std::vector<int>> nf((size_t)N);
+  for (int g = 0; g < 4 * E; ++g) {
+    if (cnt[g] != 1) continue;
+    const int e = g / 4, f = g % 4, id = (int)(faces.size() / 3);
+    for (int k = 0; k < 3; ++k) {
+      const int nd = c->elnod_h[4 * e + tf[f][k]];
+      faces.push_back(nd);
+      nf[nd].push_back(id);
+    }
+  }
+  ext_nodes.clear();
+  afc_off.assign(1, 0);
+  afc_face.clear();
+  for (int i = 0; i < N; ++i) {
+    if (nf[i].empty()) continue;
+    ext_nodes.push_back(i);
+    afc_face.insert(afc_face.end(), nf[i].begin(), nf[i].end());
+    afc_off.push_back((int)afc_face.size());
+  }
+}
+
+int wfx_set_rigid_mesh(wfx_ctx *c, int node_count, const double *nodes, const double *node_vel,
+                       int tri_count, const int *tris, const int *tri_mesh_id) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->n == 4, WFX_ERR_ARG,
+          "contact needs a tetrahedral body (the external-face search uses tetrahedron faces)");
+  REQUIRE(node_count > 0 && tri_count > 0 && nodes && tris, WFX_ERR_ARG, "empty rigid surface");
+  for (int k = 0; k < 3 * tri_count; ++k)
+    REQUIRE(tris[k] >= 0 && tris[k] < node_count, WFX_ERR_ARG, "rigid triangle node out of range");
+  std::vector<int> faces, ext, off, afc;
+  external_faces(c, faces, ext, off, afc);
+  REQUIRE(!ext.empty(), WFX_ERR_ARG, "the body has no external face");
+  auto &ct = c->contact;
+  const size_t N = c->N;
+  ct.rn = node_count;
+  ct.rt = tri_count;
+  ct.next = (int)ext.size();
+  WFX_CHECK_HIP(dalloc(c, &ct.rnode, 3 * (size_t)node_count));
+  WFX_CHECK_HIP(dalloc(c, &ct.rvel, 3 * (size_t)node_count));
+  WFX_CHECK_HIP(dalloc(c, &ct.rnormal, 3 * (size_t)tri_count));
+  WFX_CHECK_HIP(dalloc(c, &ct.rpplane, (size_t)tri_count));
+  WFX_CHECK_HIP(dalloc(c, &ct.rtri, 3 * (size_t)tri_count));
+  WFX_CHECK_HIP(dalloc(c, &ct.rmesh, (size_t)tri_count));
+  WFX_CHECK_HIP(dalloc(c, &ct.ext_nodes, ext.size()));
+  WFX_CHECK_HIP(dalloc(c, &ct.afc_off, off.size()));
+  WFX_CHECK_HIP(dalloc(c, &ct.afc_face, afc.size()));
+  WFX_CHECK_HIP(dalloc(c, &ct.ext_face, faces.size()));
+  WFX_CHECK_HIP(dalloc(c, &ct.node_area, N));
+  WFX_CHECK_HIP(dalloc(c, &ct.elem_length, (size_t)c->E));
+  WFX_CHECK_HIP(dalloc(c, &ct.contforce, 3 * N));
+  WFX_CHECK_HIP(dalloc(c, &ct.ut_prev, 3 * N));
+  WFX_CHECK_HIP(dalloc(c, &ct.mesh_in_contact, N));
+  WFX_CHECK_HIP(dalloc(c, &ct.cfn, N));
+  std::vector<double> vel(3 * (size_t)node_count, 0.0);
+  if (node_vel) std::copy(node_vel, node_vel + vel.size(), vel.begin());
+  std::vector<int> mid((size_t)tri_count, 0);
+  if (tri_mesh_id) std::copy(tri_mesh_id, tri_mesh_id + tri_count, mid.begin());
+  auto up = [](void *dst, const void *src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); };
+  WFX_CHECK_HIP(up(ct.rnode, nodes, 3 * sizeof(double) * node_count));
+  WFX_CHECK_HIP(up(ct.rvel, vel.data(), vel.size() * sizeof(double)));
+  WFX_CHECK_HIP(up(ct.rtri, tris, 3 * sizeof(int) * tri_count));
+  WFX_CHECK_HIP(up(ct.rmesh, mid.data(), mid.size() * sizeof(int)));
+  WFX_CHECK_HIP(up(ct.ext_nodes, ext.data(), ext.size() * sizeof(int)));
+  WFX_CHECK_HIP(up(ct.afc_off, off.data(), off.size() * sizeof(int)));
+  WFX_CHECK_HIP(up(ct.afc_face, afc.data(), afc.size() * sizeof(int)));
+  WFX_CHECK_HIP(up(ct.ext_face, faces.data(), faces.size() * sizeof(int)));
+  WFX_CHECK_HIP(hipMemset(ct.node_area, 0, N * sizeof(double)));
+  WFX_CHECK_HIP(hipMemset(ct.elem_length, 0, (size_t)c->E * sizeof(double)));
+  WFX_CHECK_HIP(hipMemset(ct.contforce, 0, 3 * N * sizeof(double)));
+  WFX_CHECK_HIP(hipMemset(ct.ut_prev, 0, 3 * N * sizeof(double)));
+  WFX_CHECK_HIP(hipMemset(ct.cfn, 0, N * sizeof(double)));
+  WFX_CHECK_HIP(hipMemset(ct.mesh_in_contact, 0xff, N * sizeof(int)));
+  ct.on = 1;
+  drop_graph(c);
+  // normals and plane offsets (a zero-length move), areas of the external faces
+  LAUNCH(contact_rigid_update(c, 0.0, nullptr));
+  LAUNCH(contact_face_areas(c, nullptr));
+  LAUNCH(contact_elem_length(c, nullptr));
+  WFX_CHECK_HIP(hipDeviceSynchronize());
+  return WFX_OK;
+}
+
+int wfx_calc_ext_face_areas(wfx_ctx *c, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->contact.on, WFX_ERR_STATE, "no rigid surface: call wfx_set_rigid_mesh");
+  LAUNCH(contact_face_areas(c, as_stream(s)));
+  return WFX_OK;
+}
+
+int wfx_calc_contact_forces(wfx_ctx *c, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->contact.on, WFX_ERR_STATE, "no rigid surface: call wfx_set_rigid_mesh");
+  REQUIRE(c->initialized, WFX_ERR_STATE, "call wfx_init first");
+  LAUNCH(contact_forces(c, as_stream(s)));
+  return WFX_OK;
+}
+
+int wfx_rigid_update(wfx_ctx *c, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->contact.on, WFX_ERR_STATE, "no rigid surface: call wfx_set_rigid_mesh");
+  LAUNCH(contact_rigid_update(c, c->prm.dt, as_stream(s)));
+  return WFX_OK;
+}
+
+int wfx_set_contact_params(wfx_ctx *c, double penalty_factor, double mu_static, double mu_dynamic) {
+  REQUIRE_CTX(c);
+  REQUIRE(penalty_factor >= 0.0 && mu_static >= 0.0 && mu_dynamic >= 0.0, WFX_ERR_ARG,
+          "penalty factor and friction coefficients must be non-negative");
+  c->contact.pf = penalty_factor;
+  c->contact.mu_sta = mu_static;
+  c->contact.mu_dyn = mu_dynamic;
+  return WFX_OK;
+}
+
+int wfx_set_rigid_velocity(wfx_ctx *c, const double *node_vel) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->contact.on, WFX_ERR_STATE, "no rigid surface: call wfx_set_rigid_mesh");
+  REQUIRE(node_vel, WFX_ERR_ARG, "null velocities");
+  WFX_CHECK_HIP(hipMemcpy(c->contact.rvel, node_vel, 3 * sizeof(double) * c->contact.rn, hipMemcpyHostToDevice));
+  return WFX_OK;
+}
+
+int wfx_contact_force(wfx_ctx *c, int mesh_id, double *out) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->contact.on, WFX_ERR_STATE, "no rigid surface: call wfx_set_rigid_mesh");
+  REQUIRE(out, WFX_ERR_ARG, "null output");
+  std::vector<int> mic(c->N);
+  std::vector<double> cfn(c->N);
+  WFX_CHECK_HIP(hipDeviceSynchronize());
+  WFX_CHECK_HIP(hipMemcpy(mic.data(), c->contact.mesh_in_contact, mic.size() * sizeof(int), hipMemcpyDeviceToHost));
+  WFX_CHECK_HIP(hipMemcpy(cfn.data(), c->contact.cfn, cfn.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double r = 0.0;  // node order, as the reference's serial sum (Contact.C:318-321)
+  for (int i = 0; i < c->N; ++i)
+    if (mic[i] == mesh_id) r += cfn[i];
+  *out = r;
   return WFX_OK;
 }
 
@@ -764,6 +939,18 @@ int wfx_init(wfx_ctx *c, wfx_stream s) {
   LAUNCH(BY_PRECISION(c, launch_nodal_vol, c, st));
   LAUNCH(BY_PRECISION(c, launch_nodal_mass, c, st));
   WFX_CHECK_HIP(hipMemsetAsync(c->status, 0, 16 * sizeof(unsigned int), st));
+  if (c->contact.on) {
+    // contact state starts clear; calcMinEdgeLength before the solve
+    // (main.C:862) gives the element heights
+    auto &ct = c->contact;
+    WFX_CHECK_HIP(hipMemsetAsync(ct.contforce, 0, 3 * N * sizeof(double), st));
+    WFX_CHECK_HIP(hipMemsetAsync(ct.ut_prev, 0, 3 * N * sizeof(double), st));
+    WFX_CHECK_HIP(hipMemsetAsync(ct.cfn, 0, N * sizeof(double), st));
+    WFX_CHECK_HIP(hipMemsetAsync(ct.mesh_in_contact, 0xff, N * sizeof(int), st));
+    LAUNCH(contact_elem_length(c, st));
+    LAUNCH(contact_face_areas(c, st));
+    ct.step_count = 0;
+  }
   WFX_CHECK_HIP(hipStreamSynchronize(st));
   c->initialized = true;
   return detect_uniform_rho(c);
@@ -827,6 +1014,7 @@ int wfx_impose_bca(wfx_ctx *c, int dim, wfx_stream s) {
 // comes back to the host
 static int cfl_step(wfx_ctx *c, hipStream_t st) {
   LAUNCH(BY_PRECISION(c, launch_cfl_step, c, (int)c->dt_hist_h.size(), st));
+  if (c->contact.on) LAUNCH(contact_elem_length(c, st));  // calcMinEdgeLength's element heights
   double dt = 0.0;
   WFX_CHECK_HIP(hipMemcpyAsync(&dt, c->scalars + kCflDt, sizeof(double), hipMemcpyDeviceToHost, st));
   WFX_CHECK_HIP(hipStreamSynchronize(st));
@@ -846,6 +1034,8 @@ int wfx_calc_cfl_dt(wfx_ctx *c, double *dt, wfx_stream s) {
 }
 
 static int step_api(wfx_ctx *c, hipStream_t st, double *hist) {
+  // external face areas every 10 steps, at the step's start (Solver_explicit.C:446-451)
+  if (c->contact.on && c->contact.step_count % 10 == 0) LAUNCH(contact_face_areas(c, st));
   LAUNCH(launch_update_prediction(c, st));
   for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
   LAUNCH(BY_PRECISION(c, launch_elem_jacobians, c, st));
@@ -862,6 +1052,7 @@ static int step_api(wfx_ctx *c, hipStream_t st, double *hist) {
   LAUNCH(BY_PRECISION(c, launch_artificial_viscosity, c, st));
   LAUNCH(BY_PRECISION(c, launch_elem_forces, c, st));
   LAUNCH(BY_PRECISION(c, launch_hourglass, c, st));
+  if (c->contact.on) LAUNCH(contact_forces(c, st));  // CalcContactForces (Solver_explicit.C:769-770)
   LAUNCH(BY_PRECISION(c, launch_assembly, c, st));
   if (hist && c->ngroups) LAUNCH(launch_reaction(c, 2, hist, st));
   LAUNCH(launch_calc_accel(c, st));
@@ -869,7 +1060,9 @@ static int step_api(wfx_ctx *c, hipStream_t st, double *hist) {
   LAUNCH(launch_correction_accvel(c, st));
   for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
   LAUNCH(launch_correction_pos(c, st));
+  if (c->contact.on) LAUNCH(contact_rigid_update(c, c->prm.dt, st));  // TriMesh_d::Move (:996-1004)
   if (c->thermal.on) LAUNCH(BY_PRECISION(c, launch_thermal, c, false, false, st));  // ThermalCalcs
+  c->contact.step_count++;
   c->fused_stale = false;  // every per-step intermediate is current again
   return WFX_OK;
 }
@@ -953,6 +1146,7 @@ int wfx_step_fused(wfx_ctx *c, int nsteps, double *reaction_hist, wfx_stream s) 
   // prediction; each fused step predicts the next one in its node pass.
   rc = enqueue_fused(c, nsteps, dh, st);
   if (rc) return rc;
+  c->contact.step_count += nsteps;
   c->fused_stale = true;
   rc = sync_cfl_dt(c, nsteps, st);
   if (rc) return rc;
@@ -995,6 +1189,13 @@ std::vector<std::pair<void *, size_t>> wfx::fused_state_buffers(wfx_ctx *c) {
       {c->nd.x, N3}, {c->nd.v, N3}, {c->nd.u, N3}, {c->nd.u_dt, N3}, {c->nd.prev_a, N3}, {c->nd.fi, N3},
       {c->nd.mdiag, N1}, {c->nd.voln, N1}, {c->nd.anp_ratio, N1}, {c->status, 16 * sizeof(unsigned)},
       {c->nd.T, N1}};
+  if (c->contact.on) {  // the contact kernel runs in the element phase
+    const auto &ct = c->contact;
+    v.push_back({ct.contforce, N3});
+    v.push_back({ct.ut_prev, N3});
+    v.push_back({ct.cfn, N1});
+    v.push_back({ct.mesh_in_contact, (size_t)c->N * sizeof(int)});
+  }
   auto add = [&](void *p, size_t rows) {
     if (p) v.push_back({p, rows * E * es});
   };
@@ -1090,6 +1291,15 @@ static uint64_t graph_signature(wfx_ctx *c) {
   mix(&c->prm.mat, sizeof(c->prm.mat));
   mix(&c->prm.stab, sizeof(c->prm.stab));
   mix(&c->thermal, sizeof(c->thermal));
+  {
+    const auto &ct = c->contact;
+    const double cp[3] = {ct.pf, ct.mu_sta, ct.mu_dyn};
+    const int ci[4] = {ct.on, ct.rn, ct.rt, ct.next};
+    const void *cptr[3] = {ct.rnode, ct.contforce, ct.ext_nodes};
+    mix(cp, sizeof(cp));
+    mix(ci, sizeof(ci));
+    mix(cptr, sizeof(cptr));
+  }
   const void *ptrs[] = {c->bc_nod[0], c->bc_nod[1], c->bc_nod[2], c->bc_slot, c->bc_mask, c->bc_tab,
                         c->react, c->hist, c->e64.He, c->e32.He, c->temp_e, c->dt_hist};
   mix(ptrs, sizeof(ptrs));
@@ -1139,6 +1349,7 @@ int wfx_graph_capture(wfx_ctx *c, int nsteps, int with_reaction, wfx_stream s) {
   WFX_CHECK_HIP(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
   c->graph_steps = nsteps;
   c->graph_hist = dh != nullptr;
+  c->graph_phase = (int)(c->contact.step_count % 10);
   c->graph_sig = graph_signature(c);
   return WFX_OK;
 }
@@ -1149,8 +1360,12 @@ int wfx_graph_launch(wfx_ctx *c, double *reaction_hist, wfx_stream s) {
   REQUIRE(c->graph_sig == graph_signature(c), WFX_ERR_STATE,
           "parameters, boundary conditions or block sizes changed since wfx_graph_capture: capture again");
   REQUIRE(!reaction_hist || c->graph_hist, WFX_ERR_STATE, "graph was captured without the reaction history");
+  REQUIRE(!c->contact.on || c->contact.step_count % 10 == c->graph_phase, WFX_ERR_STATE,
+          "with contact the graph's face-area refresh follows the step counter mod 10: capture a "
+          "multiple of 10 steps, or capture again");
   hipStream_t st = as_stream(s);
   WFX_CHECK_HIP(hipGraphLaunch(c->graph_exec, st));
+  c->contact.step_count += c->graph_steps;
   c->fused_stale = true;
   if (reaction_hist) {
     const int ng = c->ngroups > 0 ? c->ngroups : 1;

@@ -32,6 +32,7 @@
 //   anp_ratio[N]   V_n / V0_n,   vsum0[N] = V0_n (constant after init)
 #pragma once
 
+#include "wfx_contact.hpp"
 #include "wfx_kernels.hpp"
 
 namespace wfx {
@@ -674,9 +675,10 @@ __device__ __forceinline__ void set_step_length(NodeParams &np, double dt) {
 // are stored in float (the update itself runs in double), positions stay
 // double, and u is not touched per step (recovered as x - x_start when the
 // run ends).
-template <typename T, bool CFL>
+template <typename T, bool CFL, bool CONTACT>
 __device__ __forceinline__ double node_update(
-    int i, long Ns, const NodeParams &np, double (&f)[3], const double *__restrict__ fe,
+    int i, long Ns, const NodeParams &np, double (&f)[3], const double *__restrict__ cf,
+    const double *__restrict__ fe,
     const double *__restrict__ mdiag, const int *__restrict__ bc_slot, const int *__restrict__ bc_mask,
     const double *__restrict__ bc_val, T *__restrict__ fi, T *__restrict__ v, T *__restrict__ prev_a,
     T *__restrict__ u_dt, double *__restrict__ x, double *__restrict__ u, unsigned int *__restrict__ status) {
@@ -695,6 +697,7 @@ __device__ __forceinline__ double node_update(
     fi[id] = (T)f[d];
     double fed = np.has_fe ? fe[id] : 0.0;
     double a = (fed - f[d]) / m;                       // calcAccel
+    if constexpr (CONTACT) a += cf[id] / m;             //   + contact force
     if (mask & (1 << d)) a = 0.0;                       // ImposeBCA
     double pa = (double)prev_a[id];
     a = np.f * (a - np.alpha * pa);                     // UpdateCorrectionAccVel
@@ -752,9 +755,10 @@ __device__ __forceinline__ void gather_force(int i, long Ns, long Es, int nrow, 
 
 // F4: assembly and the whole nodal update of one step; CFL: dt from the
 // device and each block's largest predicted speed for the next step's dt
-template <typename T, int BS, bool CFL>
+template <typename T, int BS, bool CFL, bool CONTACT>
 __global__ __launch_bounds__(BS) void f4_node(
-    int N, int E, NodeParams np, const int *__restrict__ off, const int *__restrict__ cnt,
+    int N, int E, NodeParams np, const double *__restrict__ cf, const int *__restrict__ off,
+    const int *__restrict__ cnt,
     const int *__restrict__ nodel_pk, const int *__restrict__ pad, const T *__restrict__ fel,
     const double *__restrict__ fe, const double *__restrict__ mdiag, const int *__restrict__ bc_slot,
     const int *__restrict__ bc_mask, const double *__restrict__ bc_val, T *__restrict__ fi,
@@ -767,7 +771,7 @@ __global__ __launch_bounds__(BS) void f4_node(
   if (i < N) {
     double f[3];
     gather_force<T>(i, N, E, q.nrow, off, cnt, nodel_pk, pad, fel, f);
-    speed = node_update<T, CFL>(i, N, q, f, fe, mdiag, bc_slot, bc_mask, bc_val, fi, v, prev_a, u_dt, x, u, status);
+    speed = node_update<T, CFL, CONTACT>(i, N, q, f, cf, fe, mdiag, bc_slot, bc_mask, bc_val, fi, v, prev_a, u_dt, x, u, status);
   }
   if constexpr (CFL) {
     if (q.predict) {
@@ -828,6 +832,39 @@ __global__ __launch_bounds__(kBlock) void f_sigma(int E, const T *__restrict__ p
   T3<T> s = t3_add(pI, t);
   sigma[0 * Es + e] = s.xx; sigma[1 * Es + e] = s.yy; sigma[2 * Es + e] = s.zz;
   sigma[3 * Es + e] = s.xy; sigma[4 * Es + e] = s.yz; sigma[5 * Es + e] = s.xz;
+}
+
+// Contact launchers (internal to each translation unit; the per-step API
+// reaches them through the wrappers of wfx_k64.hip).
+hipError_t launch_ext_face_areas(wfx_ctx *c, hipStream_t s) {
+  const auto &ct = c->contact;
+  hipLaunchKernelGGL(k_ext_face_areas, dim3(nblocks(ct.next)), dim3(kBlock), 0, s, ct.next, ct.ext_nodes,
+                     ct.afc_off, ct.afc_face, ct.ext_face, c->N, c->nd.x, ct.node_area);
+  return hipGetLastError();
+}
+hipError_t launch_elem_length(wfx_ctx *c, hipStream_t s) {
+  hipLaunchKernelGGL(k_elem_length, dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, c->N, c->ms.elnod, c->nd.x,
+                     c->contact.elem_length);
+  return hipGetLastError();
+}
+// CalcContactForces with the predicted velocities v (T: the fused FP32 path's
+// narrowed copy) and the host's dt or the device's (per-step CFL)
+template <typename T>
+hipError_t launch_contact(wfx_ctx *c, const T *v, bool device_dt, hipStream_t s) {
+  const auto &ct = c->contact;
+  ContactParams cp{ct.rt,        ct.rnode,  ct.rnormal, ct.rpplane, ct.rtri, ct.rmesh,
+                   c->prm.mat.E, ct.pf,     ct.mu_sta,  ct.mu_dyn,  c->prm.dt,
+                   device_dt ? c->scalars + kCflDt : nullptr};
+  hipLaunchKernelGGL((k_contact<T>), dim3(nblocks(ct.next)), dim3(kBlock), 0, s, ct.next, ct.ext_nodes, c->N, cp,
+                     c->nd.x, v, c->nd.mdiag, c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel, ct.elem_length,
+                     ct.node_area, ct.contforce, ct.ut_prev, ct.mesh_in_contact, ct.cfn);
+  return hipGetLastError();
+}
+hipError_t launch_rigid_update(wfx_ctx *c, bool device_dt, hipStream_t s) {
+  auto &ct = c->contact;
+  hipLaunchKernelGGL(k_rigid_update, dim3(1), dim3(kBlock), 0, s, ct.rn, ct.rt, ct.rnode, ct.rvel, ct.rtri,
+                     ct.rnormal, ct.rpplane, c->prm.dt, device_dt ? c->scalars + kCflDt : nullptr);
+  return hipGetLastError();
 }
 
 }  // namespace
@@ -899,7 +936,12 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
       hipLaunchKernelGGL((f1_volume<T, 8, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol, minl);
     else
       hipLaunchKernelGGL((f1_volume<T, 4, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol, minl);
-    if (cfl) return launch_cfl_reduce<T>(c, gE, step, s);
+    if (cfl) {
+      hipError_t e = launch_cfl_reduce<T>(c, gE, step, s);
+      // calcMinEdgeLength also refreshes the element heights contact uses
+      if (e == hipSuccess && c->contact.on) e = launch_elem_length(c, s);
+      return e;
+    }
   } else if (phase == 2) {
     hipLaunchKernelGGL((f2_nodal<T, BS>), dim3(gN), dim3(BS), 0, s, N, c->n, c->ms.nodel_offset,
                        c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.vol, el.rho, c->nd.voln0_sum,
@@ -927,16 +969,25 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
                        el.f_elem, c->thermal.on ? el.q_plheat : (T *)nullptr);
     // thermal coupling: ThermalCalcs on this step's geometry, before the
     // node pass moves the positions
-    if (c->thermal.on) return launch_thermal<T>(c, true, cfl, s);
+    if (c->thermal.on) {
+      hipError_t e = launch_thermal<T>(c, true, cfl, s);
+      if (e != hipSuccess) return e;
+    }
+    // contact forces after the element forces, before the assembly
+    if (c->contact.on) return launch_contact<T>(c, vN, cfl, s);
   } else if (phase == 4) {
     NodeParams np = node_params(c, predict);
     np.dtp = dtp;
     np.vmax_part = cfl ? c->cfl_part + c->cfl_cap_min : nullptr;
     if (cfl && predict) c->cfl_nvmax = gN;
-    auto kern = cfl ? f4_node<T, BS, true> : f4_node<T, BS, false>;
-    hipLaunchKernelGGL(kern, dim3(gN), dim3(BS), 0, s, N, E, np, c->ms.nodel_offset,
+    const bool ct = c->contact.on;
+    auto kern = cfl ? (ct ? f4_node<T, BS, true, true> : f4_node<T, BS, true, false>)
+                    : (ct ? f4_node<T, BS, false, true> : f4_node<T, BS, false, false>);
+    hipLaunchKernelGGL(kern, dim3(gN), dim3(BS), 0, s, N, E, np, c->contact.contforce, c->ms.nodel_offset,
                        c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.f_elem, c->nd.fe, c->nd.mdiag,
                        c->bc_slot, c->bc_mask, c->bc_tab, fiN, vN, paN, udN, c->nd.x, c->nd.u, c->status);
+    // the rigid surfaces move after the position update
+    if (c->contact.on && step >= 0) return launch_rigid_update(c, cfl, s);
   } else if (phase == 5) {
     hipLaunchKernelGGL((f_sigma<T>), dim3(nblocks(E)), dim3(kBlock), 0, s, E, el.p, el.tau, el.sigma);
   } else {
@@ -1035,6 +1086,10 @@ hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t 
     if (e != hipSuccess) return e;
   }
   for (int k = 0; k < nsteps; ++k) {
+    if (c->contact.on && (c->contact.step_count + k) % 10 == 0) {
+      hipError_t e = launch_ext_face_areas(c, s);  // every 10 steps, at the step's start
+      if (e != hipSuccess) return e;
+    }
     for (int ph = 1; ph <= 4; ++ph) {
       hipError_t e = launch_fused_phase<T>(c, ph, k + 1 < nsteps, s, k);
       if (e != hipSuccess) return e;

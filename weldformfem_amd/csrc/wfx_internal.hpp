@@ -72,6 +72,26 @@ struct DevElem {
   T *q_plheat = nullptr, *dTedt = nullptr;  // thermal coupling: plastic heat, element-node rates [n][E]
 };
 
+// Rigid-surface penalty contact (Contact.C, TriMesh_d): penalty factor
+// (m_contPF), friction coefficients, sizes, the solver's step counter (the
+// external face areas are refreshed every 10 steps)
+struct Contact {
+  int on = 0;
+  double pf = 0.1, mu_sta = 0.0, mu_dyn = 0.0;
+  int rn = 0, rt = 0, next = 0;
+  long step_count = 0;
+  // device
+  double *rnode = nullptr, *rvel = nullptr, *rnormal = nullptr, *rpplane = nullptr;  // [n][3] / [t][3] / [t]
+  int *rtri = nullptr, *rmesh = nullptr;
+  int *ext_nodes = nullptr;                      // external node ids
+  int *afc_off = nullptr, *afc_face = nullptr;   // external node k: its external faces, in face order
+  int *ext_face = nullptr;                       // [nf][3], node order of the element's face table
+  double *node_area = nullptr, *elem_length = nullptr;
+  double *contforce = nullptr, *ut_prev = nullptr;  // [3][N], kept between steps
+  int *mesh_in_contact = nullptr;
+  double *cfn = nullptr;
+};
+
 // Thermal coupling (setThermalOn, Thermal.C): conductivity, heat capacity,
 // fraction of plastic work turned into heat (plHeatFrac)
 struct Thermal {
@@ -141,6 +161,8 @@ struct wfx_ctx {
   int has_fe = 0;            // external nodal forces set (non-zero)
   double rho_uniform = 0.0;  // > 0: every element density equals this value
   wfx::Thermal thermal;
+  wfx::Contact contact;
+  int graph_phase = 0;       // step counter mod 10 when the graph was captured
   double *temp_e = nullptr;  // per-element temperature (else prm.mat.temp for all)
   double *temp_buf = nullptr;  // device buffer behind temp_e, kept once allocated
   int fused_bs[5] = {256, 256, 256, 256, 256};  // block size of fused phases 1..4

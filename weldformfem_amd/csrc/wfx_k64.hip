@@ -50,7 +50,7 @@ hipError_t launch_impose(wfx_ctx *c, int dim, bool accel, hipStream_t s) {
 
 hipError_t launch_calc_accel(wfx_ctx *c, hipStream_t s) {
   hipLaunchKernelGGL(k_calc_accel, dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N, c->nd.fe, c->nd.fi,
-                     c->nd.mdiag, c->nd.a);
+                     c->nd.mdiag, c->nd.a, c->contact.on ? c->contact.contforce : nullptr);
   return hipGetLastError();
 }
 
@@ -93,6 +93,18 @@ hipError_t launch_min_edge(wfx_ctx *c, double *partial, double *dev_out, hipStre
                      partial);
   hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kBlock), 0, s, nb, partial, dev_out);
   return hipGetLastError();
+}
+
+// contact kernels for the per-step API (the fused path launches them itself)
+hipError_t contact_face_areas(wfx_ctx *c, hipStream_t s) { return launch_ext_face_areas(c, s); }
+hipError_t contact_elem_length(wfx_ctx *c, hipStream_t s) { return launch_elem_length(c, s); }
+hipError_t contact_forces(wfx_ctx *c, hipStream_t s) { return launch_contact<double>(c, c->nd.v, false, s); }
+hipError_t contact_rigid_update(wfx_ctx *c, double dt, hipStream_t s) {
+  const double keep = c->prm.dt;
+  c->prm.dt = dt;
+  hipError_t e = launch_rigid_update(c, false, s);
+  c->prm.dt = keep;
+  return e;
 }
 
 }  // namespace wfx

@@ -244,12 +244,17 @@ __global__ __launch_bounds__(kBlock) void k_impose_value(int count, const int *_
 __global__ __launch_bounds__(kBlock) void k_calc_accel(int N, const double *__restrict__ fe,
                                                        const double *__restrict__ fi,
                                                        const double *__restrict__ m,
-                                                       double *__restrict__ a) {
+                                                       double *__restrict__ a,
+                                                       const double *__restrict__ cf) {
   int i = gtid();
   if (i >= N) return;
   double mi = m[i];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) a[(long)d * N + i] = (fe[(long)d * N + i] - fi[(long)d * N + i]) / mi;
+  for (int d = 0; d < 3; ++d) {
+    double ad = (fe[(long)d * N + i] - fi[(long)d * N + i]) / mi;
+    if (cf) ad += cf[(long)d * N + i] / mi;  // contact force (Mechanical.C:330-335)
+    a[(long)d * N + i] = ad;
+  }
 }
 
 // UpdateCorrectionAccVel (Domain_d.C:981-997)

@@ -71,6 +71,12 @@ hipError_t launch_min_edge(wfx_ctx *c, double *partial, double *dev_out, hipStre
 int min_edge_partial_count(wfx_ctx *c);
 hipError_t launch_reaction_f32(wfx_ctx *c, const float *fi32, double *dev_out, hipStream_t s);
 
+// rigid-surface contact, per-step API (wfx_k64.hip)
+hipError_t contact_face_areas(wfx_ctx *c, hipStream_t s);
+hipError_t contact_elem_length(wfx_ctx *c, hipStream_t s);
+hipError_t contact_forces(wfx_ctx *c, hipStream_t s);
+hipError_t contact_rigid_update(wfx_ctx *c, double dt, hipStream_t s);
+
 // device buffers a fused run writes (saved and restored around autotuning
 // and around the adjoint's recomputation)
 std::vector<std::pair<void *, size_t>> fused_state_buffers(wfx_ctx *c);

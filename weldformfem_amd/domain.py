@@ -110,6 +110,34 @@ class Domain:
             a = np.ascontiguousarray(a.reshape(-1))
             check(self._L.wfx_set_elem_temperature(self._h, a.ctypes.data, a.size))
 
+    def setRigidMesh(self, nodes, tris, vel=None, mesh_id=None):
+        """rigid triangle surfaces for penalty contact (TriMesh_d + setContactOn);
+        the body must be tetrahedral"""
+        x = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1)
+        t = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1)
+        v = None
+        if vel is not None:
+            v = np.ascontiguousarray(np.broadcast_to(np.asarray(vel, dtype=np.float64), (x.size // 3, 3))).reshape(-1)
+        m = None if mesh_id is None else np.ascontiguousarray(mesh_id, dtype=np.int32)
+        check(self._L.wfx_set_rigid_mesh(self._h, x.size // 3, x.ctypes.data, None if v is None else v.ctypes.data,
+                                         t.size // 3, t.ctypes.data, None if m is None else m.ctypes.data))
+
+    def setContactParams(self, penalty_factor=0.1, mu_static=0.0, mu_dynamic=0.0):
+        check(self._L.wfx_set_contact_params(self._h, float(penalty_factor), float(mu_static), float(mu_dynamic)))
+
+    def setRigidVelocity(self, vel, node_count):
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(vel, dtype=np.float64), (int(node_count), 3))).reshape(-1)
+        check(self._L.wfx_set_rigid_velocity(self._h, v.ctypes.data))
+
+    def contactForce(self, mesh_id=0):
+        v = ctypes.c_double()
+        check(self._L.wfx_contact_force(self._h, int(mesh_id), ctypes.byref(v)))
+        return v.value
+
+    def CalcExtFaceAreas(self): self._call(self._L.wfx_calc_ext_face_areas)
+    def CalcContactForces(self): self._call(self._L.wfx_calc_contact_forces)
+    def rigidUpdate(self): self._call(self._L.wfx_rigid_update)
+
     def setThermal(self, on=True, k_T=50.0, cp_T=460.0, plheat_frac=0.9):
         """thermal coupling (setThermalOn, thermalCond, thermalHeatCap,
         plHeatFrac); nodal temperatures in the field "T"
