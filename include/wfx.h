@@ -287,6 +287,14 @@ int wfx_read_lsdyna_k(wfx_ctx *ctx, const char *path);
  * DISP, Velocity, Acceleration, nod_mass, pressure, pl_strain, Vol; plus
  * sigma_y and the cell stress tensor).  Synchronous. */
 int wfx_write_vtk(wfx_ctx *ctx, const char *path, wfx_stream s);
+/* the same frame, asynchronously: the state is snapshotted on the stream
+ * (device-to-device copies ordered with the step), copied to pinned host
+ * memory on a library stream and written by a host thread while the caller
+ * keeps stepping; two frames in flight at most (a third waits for the
+ * oldest).  No reference counterpart (VTKWriter.C writes synchronously). */
+int wfx_write_vtk_async(wfx_ctx *ctx, const char *path, wfx_stream stream);
+/* waits for every pending asynchronous frame; the first write error */
+int wfx_vtk_wait(wfx_ctx *ctx);
 
 /* ---- state transfer (reference layout on the host side, synchronous) ----- */
 /* names: x v a u u_dt prev_a fi fe mdiag voln elnod nodel nodel_loc

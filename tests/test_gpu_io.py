@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 from oracle.wfref import RefDomain
-from weldformfem_amd import Domain, WFX_FP64, workloads
+from weldformfem_amd import Domain, WFX_FP32_MIXED, WFX_FP64, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -73,3 +73,40 @@ def test_keyword_mesh_run_and_vtk_frame(tmp_path):
     np.testing.assert_allclose(sig[:, 0, 1], s6[:, 3], rtol=1e-8, atol=1e-3)
     np.testing.assert_allclose(sig[:, 1, 0], s6[:, 3], rtol=1e-8, atol=1e-3)
     assert v["DISP"].size == 3 * N
+
+
+def _block(precision):
+    ref = RefDomain()
+    ref.AddBoxLength([0, 0, 0], [1.0, 0.75, 0.5], 0.125)
+    g = Domain(precision)
+    g.CreateFromMesh(ref.field("x"), ref.field("elnod"), 8)
+    bottom, top = workloads.box_layers(4, 3, 2)
+    workloads.setup_compression(g, bottom, top, -5.0, hg_coeff=0.06)
+    g.init()
+    return g
+
+
+@pytest.mark.parametrize("precision", [WFX_FP64, WFX_FP32_MIXED])
+def test_async_frames_match_sync_frames(tmp_path, precision):
+    print(f"case: three asynchronous VTK frames between fused runs, precision {precision}", flush=True)
+    g, h = _block(precision), _block(precision)
+    for k in range(3):  # the third frame reuses the first slot
+        g.runFused(20)
+        g.writeVTKAsync(tmp_path / f"a{k}.vtk")
+    g.waitVTK()
+    for k in range(3):
+        h.runFused(20)
+        h.writeVTK(tmp_path / f"s{k}.vtk")
+    for k in range(3):
+        assert (tmp_path / f"a{k}.vtk").read_bytes() == (tmp_path / f"s{k}.vtk").read_bytes(), f"frame {k}"
+
+
+def test_async_frame_write_error_is_reported(tmp_path):
+    from weldformfem_amd._lib import WfxError
+
+    print("case: asynchronous frame into a missing directory", flush=True)
+    g = _block(WFX_FP64)
+    g.writeVTKAsync(tmp_path / "missing" / "f.vtk")
+    with pytest.raises(WfxError, match="cannot write"):
+        g.waitVTK()
+    g.waitVTK()  # reported once

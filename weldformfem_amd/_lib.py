@@ -94,6 +94,8 @@ SIGNATURES = {
     "wfx_parse_lsdyna_k": (_i, [_cp, _pi, _pi, _pi, _vp, _vp]),
     "wfx_read_lsdyna_k": (_i, [_vp, _cp]),
     "wfx_write_vtk": (_i, [_vp, _cp, _vp]),
+    "wfx_write_vtk_async": (_i, [_vp, _cp, _vp]),
+    "wfx_vtk_wait": (_i, [_vp]),
     "wfx_reaction_force": (_i, [_vp, _i, _pd, _vp]),
     "wfx_energies": (_i, [_vp, _pd, _pd, _vp]),
     "wfx_nonfinite_count": (_i, [_vp, ctypes.POINTER(ctypes.c_long)]),

@@ -23,6 +23,7 @@ UNITS = [
     ("wfx_k32.hip", []),
     ("wfx_api.hip", ["-ffp-contract=off"]),
     ("wfx_adjoint.hip", ["-ffp-contract=off"]),
+    ("wfx_frames.hip", []),
 ]
 
 

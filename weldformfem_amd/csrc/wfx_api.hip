@@ -44,6 +44,7 @@ static void drop_graph(wfx_ctx *c) {
 static void free_all(wfx_ctx *c) {
   drop_graph(c);
   adjoint_free(c);
+  frames_free(c);  // pending frames are written first
   for (void *p : c->allocations) (void)hipFree(p);
   c->allocations.clear();
   c->nd = DevNode();

@@ -33,6 +33,7 @@ namespace wfx {
 
 void set_error(const std::string &msg);
 struct Adjoint;  // checkpointed adjoint run (wfx_adjoint.hip)
+struct Frames;   // asynchronous VTK frames (wfx_frames.hip)
 
 enum Model { BILINEAR = 0, HOLLOMON = 1, JOHNSON_COOK = 2, GMT = 3 };
 
@@ -185,4 +186,5 @@ struct wfx_ctx {
   int hist_cap = 0;
   std::vector<void *> allocations;
   wfx::Adjoint *adj = nullptr;  // owned, freed with the context or a new mesh
+  wfx::Frames *frames = nullptr;  // owned, joined and freed with the context or a new mesh
 };

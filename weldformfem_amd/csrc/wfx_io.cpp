@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/wfx.h"
+#include "wfx_vtk.hpp"
 
 namespace wfx {
 void set_error(const std::string &msg);
@@ -188,6 +189,63 @@ int to_arrays(const KMesh &m, int &nodxelem, std::vector<int> &elnod, std::strin
 
 }  // namespace
 
+namespace wfx {
+
+// VTKWriter::writeFile's legacy unstructured grid (VTKWriter.C:365-625),
+// restricted to the fields this library keeps
+int write_vtk_frame(const char *path, const VtkFrame &fr, std::string &err) {
+  const int N = fr.N, E = fr.E, n = fr.n;
+  FILE *f = std::fopen(path, "w");
+  if (!f) {
+    err = std::string("cannot write ") + path;
+    return 1;
+  }
+  std::fprintf(f, "# vtk DataFile Version 3.0\nwfx explicit step\nASCII\nDATASET UNSTRUCTURED_GRID\n");
+  std::fprintf(f, "POINTS %d double\n", N);
+  for (int i = 0; i < N; ++i) std::fprintf(f, "%.9e %.9e %.9e\n", fr.x[3 * i], fr.x[3 * i + 1], fr.x[3 * i + 2]);
+  std::fprintf(f, "CELLS %d %ld\n", E, (long)E * (n + 1));
+  for (int e = 0; e < E; ++e) {
+    std::fprintf(f, "%d", n);
+    for (int k = 0; k < n; ++k) std::fprintf(f, " %d", fr.elnod[(long)e * n + k]);
+    std::fprintf(f, "\n");
+  }
+  std::fprintf(f, "CELL_TYPES %d\n", E);
+  for (int e = 0; e < E; ++e) std::fprintf(f, "%d\n", n == 8 ? 12 : 10);
+  auto vec = [&](const char *name, const std::vector<double> &q) {
+    std::fprintf(f, "VECTORS %s double\n", name);
+    for (int i = 0; i < N; ++i) std::fprintf(f, "%.9e %.9e %.9e\n", q[3 * i], q[3 * i + 1], q[3 * i + 2]);
+  };
+  auto scal = [&](const char *name, const std::vector<double> &q, int count) {
+    std::fprintf(f, "SCALARS %s double 1\nLOOKUP_TABLE default\n", name);
+    for (int i = 0; i < count; ++i) std::fprintf(f, "%.9e\n", q[i]);
+  };
+  std::fprintf(f, "POINT_DATA %d\n", N);
+  vec("DISP", fr.u);
+  vec("Velocity", fr.v);
+  vec("Acceleration", fr.a);
+  scal("nod_mass", fr.mdiag, N);
+  std::fprintf(f, "CELL_DATA %d\n", E);
+  scal("pressure", fr.p, E);
+  scal("pl_strain", fr.pl_strain, E);
+  scal("sigma_y", fr.sigma_y, E);
+  scal("Vol", fr.vol, E);
+  std::fprintf(f, "TENSORS SIGMA double\n");
+  for (int e = 0; e < E; ++e) {
+    const double *q = &fr.sigma[6L * e];  // xx yy zz xy yz xz
+    std::fprintf(f, "%.9e %.9e %.9e\n%.9e %.9e %.9e\n%.9e %.9e %.9e\n", q[0], q[3], q[5], q[3], q[1], q[4], q[5],
+                 q[4], q[2]);
+  }
+  const bool ok = std::ferror(f) == 0;
+  std::fclose(f);
+  if (!ok) {
+    err = std::string("write error on ") + path;
+    return 1;
+  }
+  return 0;
+}
+
+}  // namespace wfx
+
 extern "C" {
 
 int wfx_parse_lsdyna_k(const char *path, int *node_count, int *elem_count, int *nodxelem, double *x, int *elnod) {
@@ -229,8 +287,8 @@ int wfx_read_lsdyna_k(wfx_ctx *ctx, const char *path) {
 // pressure, pl_strain, sigma_y, Vol and the stress tensor.
 int wfx_write_vtk(wfx_ctx *ctx, const char *path, wfx_stream s) {
   (void)s;
-  int N = 0, E = 0, n = 0;
-  int rc = wfx_get_sizes(ctx, &N, &E, &n);
+  wfx::VtkFrame fr;
+  int rc = wfx_get_sizes(ctx, &fr.N, &fr.E, &fr.n);
   if (rc != WFX_OK) return rc;
   if (!path) {
     wfx::set_error("null path");
@@ -244,59 +302,18 @@ int wfx_write_vtk(wfx_ctx *ctx, const char *path, wfx_stream s) {
     out.resize(cnt);
     return wfx_get_field(ctx, name, out.data(), cnt);
   };
-  std::vector<double> x, u, v, a, m, p, pl, sy, vol, sig;
-  std::vector<int> el((long)E * n);
-  for (auto pr : {std::make_pair("x", &x), {"u", &u}, {"v", &v}, {"a", &a}, {"mdiag", &m}, {"p", &p},
-                  {"pl_strain", &pl}, {"sigma_y", &sy}, {"vol", &vol}, {"sigma", &sig}}) {
+  for (auto pr : {std::make_pair("x", &fr.x), {"u", &fr.u}, {"v", &fr.v}, {"a", &fr.a}, {"mdiag", &fr.mdiag},
+                  {"p", &fr.p}, {"pl_strain", &fr.pl_strain}, {"sigma_y", &fr.sigma_y}, {"vol", &fr.vol},
+                  {"sigma", &fr.sigma}}) {
     rc = get(pr.first, *pr.second);
     if (rc != WFX_OK) return rc;
   }
-  rc = wfx_get_field(ctx, "elnod", el.data(), (long)el.size());
+  fr.elnod.resize((size_t)fr.E * fr.n);
+  rc = wfx_get_field(ctx, "elnod", fr.elnod.data(), (long)fr.elnod.size());
   if (rc != WFX_OK) return rc;
-  FILE *f = std::fopen(path, "w");
-  if (!f) {
-    wfx::set_error(std::string("cannot write ") + path);
-    return WFX_ERR_ARG;
-  }
-  std::fprintf(f, "# vtk DataFile Version 3.0\nwfx explicit step\nASCII\nDATASET UNSTRUCTURED_GRID\n");
-  std::fprintf(f, "POINTS %d double\n", N);
-  for (int i = 0; i < N; ++i) std::fprintf(f, "%.9e %.9e %.9e\n", x[3 * i], x[3 * i + 1], x[3 * i + 2]);
-  std::fprintf(f, "CELLS %d %ld\n", E, (long)E * (n + 1));
-  for (int e = 0; e < E; ++e) {
-    std::fprintf(f, "%d", n);
-    for (int k = 0; k < n; ++k) std::fprintf(f, " %d", el[(long)e * n + k]);
-    std::fprintf(f, "\n");
-  }
-  std::fprintf(f, "CELL_TYPES %d\n", E);
-  for (int e = 0; e < E; ++e) std::fprintf(f, "%d\n", n == 8 ? 12 : 10);
-  auto vec = [&](const char *name, const std::vector<double> &q) {
-    std::fprintf(f, "VECTORS %s double\n", name);
-    for (int i = 0; i < N; ++i) std::fprintf(f, "%.9e %.9e %.9e\n", q[3 * i], q[3 * i + 1], q[3 * i + 2]);
-  };
-  auto scal = [&](const char *name, const std::vector<double> &q, int count) {
-    std::fprintf(f, "SCALARS %s double 1\nLOOKUP_TABLE default\n", name);
-    for (int i = 0; i < count; ++i) std::fprintf(f, "%.9e\n", q[i]);
-  };
-  std::fprintf(f, "POINT_DATA %d\n", N);
-  vec("DISP", u);
-  vec("Velocity", v);
-  vec("Acceleration", a);
-  scal("nod_mass", m, N);
-  std::fprintf(f, "CELL_DATA %d\n", E);
-  scal("pressure", p, E);
-  scal("pl_strain", pl, E);
-  scal("sigma_y", sy, E);
-  scal("Vol", vol, E);
-  std::fprintf(f, "TENSORS SIGMA double\n");
-  for (int e = 0; e < E; ++e) {
-    const double *q = &sig[6L * e];  // xx yy zz xy yz xz
-    std::fprintf(f, "%.9e %.9e %.9e\n%.9e %.9e %.9e\n%.9e %.9e %.9e\n", q[0], q[3], q[5], q[3], q[1], q[4], q[5],
-                 q[4], q[2]);
-  }
-  const bool ok = std::ferror(f) == 0;
-  std::fclose(f);
-  if (!ok) {
-    wfx::set_error(std::string("write error on ") + path);
+  std::string err;
+  if (wfx::write_vtk_frame(path, fr, err)) {
+    wfx::set_error(err);
     return WFX_ERR_ARG;
   }
   return WFX_OK;

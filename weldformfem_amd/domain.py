@@ -62,6 +62,14 @@ class Domain:
         """legacy VTK frame of the current state"""
         check(self._L.wfx_write_vtk(self._h, str(path).encode(), self.stream))
 
+    def writeVTKAsync(self, path):
+        """the same frame written by a host thread while stepping continues"""
+        check(self._L.wfx_write_vtk_async(self._h, str(path).encode(), self.stream))
+
+    def waitVTK(self):
+        """waits for the pending asynchronous frames"""
+        check(self._L.wfx_vtk_wait(self._h))
+
     def sizes(self):
         N, E, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(self._L.wfx_get_sizes(self._h, ctypes.byref(N), ctypes.byref(E), ctypes.byref(n)))
