@@ -276,6 +276,70 @@ def config_legs(torch, log):
     return out
 
 
+# Johnson-Cook (A B n C eps_0 m T_m T_t) and GMT (n1 n2 C1 C2 m1 m2 I1 I2; ranges)
+# constants of the extension timings
+JC_LAW = (300e6, 500e6, 0.3, 0.014, 1.0, 1.0, 1520.0, 20.0)
+GMT_LAW = ((-2e-4, 0.15, 600e6, -1.5e-3, 1e-4, 0.01, -1e-5, 1e-4), (0.002, 3.0, 1e-3, 1e5, 20.0, 1200.0))
+
+
+def extensions_leg(torch, doms, steps, log):
+    """Step cost of the features around the path, fused, on the 4M-element
+    block (the benchmark's own domains, after their measurements: hardening
+    laws, thermal coupling, per-step CFL) and on the 64^3-cell tet cylinder
+    compressed by a rigid plane with friction (contact, then everything)."""
+    from weldformfem_amd import WFX_FP64, Domain, workloads
+
+    out = {}
+    for tag, d in doms:
+        if d is None:
+            continue
+        row = {}
+        variants = [("jc", lambda: d.setMaterialJC(200e9, 0.3, 7850.0, *JC_LAW)),
+                    ("gmt", lambda: d.setMaterialGMT(200e9, 0.3, 7850.0, 250e6, *GMT_LAW)),
+                    ("jc_thermal", lambda: (d.setMaterialJC(200e9, 0.3, 7850.0, *JC_LAW),
+                                            d.setThermal(True, 50.0, 460.0, 0.9))),
+                    ("jc_thermal_cfl", lambda: d.setCFL(0.3))]
+        d.runFused(5, reaction=False)
+        row["bilinear"] = timed(lambda: d.runFused(steps, reaction=False), torch) / steps * 1e3
+        for name, setup in variants:
+            setup()
+            d.runFused(5, reaction=False)
+            row[name] = timed(lambda: d.runFused(steps, reaction=False), torch) / steps * 1e3
+        log(f"extensions {tag} (ms/step): " + ", ".join(f"{k} {v:.3f}" for k, v in row.items()))
+        out[f"block_{tag}_ms_per_step"] = row
+    x, el, bottom, top = workloads.cylinder_tets(cells=64)
+    zt = float(np.asarray(x).reshape(-1, 3)[:, 2].max())
+    die = np.array([[-1, -1, zt], [1, -1, zt], [-1, 1, zt], [1, 1, zt]], float)
+    s = torch.cuda.Stream()
+    d = Domain(WFX_FP64, stream=s.cuda_stream)
+    d.CreateFromMesh(x, el, 4)
+    d.setDensity(7850.0)
+    d.setMaterial(0, 200e9, 0.3, 7850.0, 300e6, 1e9, 0.0)
+    for nd in bottom:
+        d.AddBCVelNode(int(nd), 2, 0.0)
+    d.AllocateBCs()
+    d.SetDT(workloads.cfl_dt(d.calcMinEdgeLength(), 200e9, 0.3, 7850.0, 4))
+    d.init()
+    row = {}
+    d.runFused(5, reaction=False)
+    row["plain"] = timed(lambda: d.runFused(steps, reaction=False), torch) / steps * 1e3
+    d.setRigidMesh(die, np.array([[0, 2, 1], [1, 2, 3]]), vel=[0.0, 0.0, -2.0])
+    d.setContactParams(0.1, 0.2, 0.1)
+    d.runFused(5, reaction=False)
+    row["contact"] = timed(lambda: d.runFused(steps, reaction=False), torch) / steps * 1e3
+    d.setMaterialJC(200e9, 0.3, 7850.0, *JC_LAW)
+    d.setThermal(True, 50.0, 460.0, 0.9)
+    d.setCFL(0.25)
+    d.runFused(5, reaction=False)
+    row["contact_jc_thermal_cfl"] = timed(lambda: d.runFused(steps, reaction=False), torch) / steps * 1e3
+    N, E, _ = d.sizes()
+    log("extensions tet cylinder + die (ms/step): " + ", ".join(f"{k} {v:.3f}" for k, v in row.items()))
+    out["cylinder_tets_die_fp64_ms_per_step"] = dict(row, elements=E, nodes=N,
+                                                      contact_force_N=d.contactForce(0))
+    del d
+    return out
+
+
 def adjoint_leg(torch, log, steps=2000, cells=30, with_oracle=True):
     """Parameter identification workload: 30^3 hexa compression, reaction
     history loss against a target solved at hidden parameters; adjoint
@@ -403,6 +467,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--no-extensions", action="store_true")
     ap.add_argument("--bs64", default="", help="fused block sizes for FP64 (f1,f2,f3,f4) instead of autotuning")
     ap.add_argument("--bs32", default="", help="the same for FP32-mixed")
     ap.add_argument("--batch-steps", type=int, default=1000)
@@ -508,6 +573,11 @@ def main():
         api["per_kernel"] = per_kernel_table(event_times(api_calls(d64), args.kernel_reps, torch),
                                              api_bytes(n, 8), E, N)
 
+    extensions = None
+    if not args.no_extensions:
+        extensions = extensions_leg(torch, [("fp64", d64), ("fp32_mixed", None if args.no_mixed else d32)],
+                                    100, log)
+
     batch = None
     if not args.no_batch:
         batch = batch_leg(torch, args.batch_steps, log)
@@ -565,6 +635,7 @@ def main():
         "fp32_mixed": mixed,
         "api_path_fp64": api,
         "other_configs": others,
+        "extensions": extensions,
         "batched_specimens": batch,
         "adjoint": adjoint,
         "cpu_baseline": cpu,

@@ -164,3 +164,23 @@ def test_hexa_body_rejected():
     n, t = tilted_plane(0.1)
     with pytest.raises(WfxError, match="tetrahedral"):
         g.setRigidMesh(n, t)
+
+
+def test_forging_combination_fused_fp64():
+    # the features together, as in a forging deck: Johnson-Cook hardening,
+    # thermal coupling, per-step CFL time step, a descending die with friction
+    say("case: JC + thermal + per-step CFL + contact, fused FP64 vs oracle, 120 steps")
+    g, r = _pair(cfl=0.25)
+    for d in (g, r):
+        d.setMaterialJC(E_, NU, RHO, 300e6, 500e6, 0.3, 0.014, 1.0, 1.0, 1520.0, 20.0)
+        d.setTemperature(20.0)
+        d.setThermal(True, 5.0e4, 460.0, 0.9)
+        d.setCFL(0.25)
+        d.init()
+    hg = np.concatenate([g.runFused(33), g.runFused(87)])
+    hr = r.run(120)
+    check(g, r, hg, hr, 1e-10)
+    dT = r.field("T") - 20.0
+    assert np.max(np.abs(dT)) > 0.01
+    assert rel(g.field("T") - 20.0, dT) <= 1e-10
+    assert g.getDT() == pytest.approx(r.getDT(), rel=1e-14)
