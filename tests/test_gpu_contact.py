@@ -184,3 +184,20 @@ def test_forging_combination_fused_fp64():
     assert np.max(np.abs(dT)) > 0.01
     assert rel(g.field("T") - 20.0, dT) <= 1e-10
     assert g.getDT() == pytest.approx(r.getDT(), rel=1e-14)
+
+
+def test_second_surface_replaces_the_first():
+    say("case: wfx_set_rigid_mesh twice, the second surface replaces the first")
+    g, r = _pair()
+    x = np.asarray(workloads.cylinder_tets(cells=12)[0]).reshape(-1, 3)
+    zt = x[:, 2].max()
+    nodes, tris = tilted_plane(zt, slope=0.02)
+    for d in (g, r):
+        d.setRigidMesh(nodes, tris, vel=[0.0, 0.0, -15.0])
+        d.init()
+    hg, hr = g.runFused(60), r.run(60)
+    assert np.sum(r.field("mesh_in_contact") == 0) > 10
+    for f in ["u", "contforce"]:
+        e = rel(g.field(f), r.field(f))
+        assert e <= 1e-10, f"{f}: {e:.3e}"
+    np.testing.assert_allclose(g.field("rigid_x"), r.field("rigid_x"), rtol=1e-14)

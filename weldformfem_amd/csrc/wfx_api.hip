@@ -21,6 +21,19 @@ int box_length_mesh(const double V[3], const double L[3], double r, bool tritetr
 int build_nodel(int N, int E, int n, const int *elnod, std::vector<int> &nodel,
                 std::vector<int> &nodel_loc, std::vector<int> &offset, std::vector<int> &count);
 
+// frees one of the context's allocations (before it grows or is replaced)
+template <typename T>
+static void dfree(wfx_ctx *c, T *&p) {
+  if (!p) return;
+  auto it = std::find(c->allocations.begin(), c->allocations.end(), (void *)p);
+  if (it != c->allocations.end()) {
+    (void)hipDeviceSynchronize();  // no queued work may still use it
+    (void)hipFree(p);
+    c->allocations.erase(it);
+  }
+  p = nullptr;
+}
+
 template <typename T>
 static hipError_t dalloc(wfx_ctx *c, T **p, size_t count) {
   void *q = nullptr;
@@ -544,6 +557,12 @@ int wfx_set_rigid_mesh(wfx_ctx *c, int node_count, const double *nodes, const do
   REQUIRE(!ext.empty(), WFX_ERR_ARG, "the body has no external face");
   auto &ct = c->contact;
   const size_t N = c->N;
+  drop_graph(c);
+  // a new surface replaces the previous one
+  dfree(c, ct.rnode); dfree(c, ct.rvel); dfree(c, ct.rnormal); dfree(c, ct.rpplane); dfree(c, ct.rtri);
+  dfree(c, ct.rmesh); dfree(c, ct.ext_nodes); dfree(c, ct.afc_off); dfree(c, ct.afc_face);
+  dfree(c, ct.ext_face); dfree(c, ct.node_area); dfree(c, ct.elem_length); dfree(c, ct.contforce);
+  dfree(c, ct.ut_prev); dfree(c, ct.mesh_in_contact); dfree(c, ct.cfn);
   ct.rn = node_count;
   ct.rt = tri_count;
   ct.next = (int)ext.size();
@@ -583,7 +602,6 @@ int wfx_set_rigid_mesh(wfx_ctx *c, int node_count, const double *nodes, const do
   WFX_CHECK_HIP(hipMemset(ct.cfn, 0, N * sizeof(double)));
   WFX_CHECK_HIP(hipMemset(ct.mesh_in_contact, 0xff, N * sizeof(int)));
   ct.on = 1;
-  drop_graph(c);
   // normals and plane offsets (a zero-length move), areas of the external faces
   LAUNCH(contact_rigid_update(c, 0.0, nullptr));
   LAUNCH(contact_face_areas(c, nullptr));
@@ -1071,6 +1089,7 @@ static int step_api(wfx_ctx *c, hipStream_t st, double *hist) {
 // device per-step dt record of a fused run with the per-step CFL time step
 static int ensure_dt_hist(wfx_ctx *c, int nsteps) {
   if (c->prm.cfl > 0.0 && c->dt_cap < nsteps) {
+    dfree(c, c->dt_hist);
     WFX_CHECK_HIP(dalloc(c, &c->dt_hist, nsteps));
     c->dt_cap = nsteps;
   }
@@ -1092,6 +1111,7 @@ static int sync_cfl_dt(wfx_ctx *c, int nsteps, hipStream_t st) {
 static int ensure_hist(wfx_ctx *c, int nsteps, double **out) {
   const long need = (long)nsteps * (c->ngroups > 0 ? c->ngroups : 1);
   if (c->hist_cap < need) {
+    dfree(c, c->hist);
     WFX_CHECK_HIP(dalloc(c, &c->hist, need));
     c->hist_cap = (int)need;
   }
