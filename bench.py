@@ -282,17 +282,18 @@ JC_LAW = (300e6, 500e6, 0.3, 0.014, 1.0, 1.0, 1520.0, 20.0)
 GMT_LAW = ((-2e-4, 0.15, 600e6, -1.5e-3, 1e-4, 0.01, -1e-5, 1e-4), (0.002, 3.0, 1e-3, 1e5, 20.0, 1200.0))
 
 
-def extensions_leg(torch, doms, steps, log):
-    """Step cost of the features around the path, fused, on the 4M-element
-    block (the benchmark's own domains, after their measurements: hardening
-    laws, thermal coupling, per-step CFL) and on the 64^3-cell tet cylinder
-    compressed by a rigid plane with friction (contact, then everything)."""
+def extensions_leg(torch, blocks, steps, log):
+    """Step cost of the features around the path, fused, on a fresh copy of
+    the benchmark's block (hardening laws, thermal coupling, per-step CFL;
+    the benchmark's own domains have been disturbed by the single-kernel
+    timings) and on the 64^3-cell tet cylinder compressed by a rigid plane
+    with friction (contact, then everything)."""
     from weldformfem_amd import WFX_FP64, Domain, workloads
 
     out = {}
-    for tag, d in doms:
-        if d is None:
-            continue
+    for tag, prec, dims, bs in blocks:
+        d, _ = build_domain(prec, *dims, -2.0, torch.cuda.current_stream().cuda_stream)
+        d.setBlockSizes(bs)
         row = {}
         variants = [("jc", lambda: d.setMaterialJC(200e9, 0.3, 7850.0, *JC_LAW)),
                     ("gmt", lambda: d.setMaterialGMT(200e9, 0.3, 7850.0, 250e6, *GMT_LAW)),
@@ -307,6 +308,7 @@ def extensions_leg(torch, doms, steps, log):
             row[name] = timed(lambda: d.runFused(steps, reaction=False), torch) / steps * 1e3
         log(f"extensions {tag} (ms/step): " + ", ".join(f"{k} {v:.3f}" for k, v in row.items()))
         out[f"block_{tag}_ms_per_step"] = row
+        del d
     x, el, bottom, top = workloads.cylinder_tets(cells=64)
     zt = float(np.asarray(x).reshape(-1, 3)[:, 2].max())
     die = np.array([[-1, -1, zt], [1, -1, zt], [-1, 1, zt], [1, 1, zt]], float)
@@ -575,8 +577,13 @@ def main():
 
     extensions = None
     if not args.no_extensions:
-        extensions = extensions_leg(torch, [("fp64", d64), ("fp32_mixed", None if args.no_mixed else d32)],
-                                    100, log)
+        blocks = [("fp64", WFX_FP64, (nx, ny, nz, L), bs64)]
+        if not args.no_mixed:
+            blocks.append(("fp32_mixed", WFX_FP32_MIXED, (nx, ny, nz, L), bs32))
+        del d64
+        if not args.no_mixed:
+            del d32
+        extensions = extensions_leg(torch, blocks, 100, log)
 
     batch = None
     if not args.no_batch:
