@@ -301,7 +301,22 @@ __device__ __forceinline__ void elem_jacobian(const int (&g)[NN], const double *
 
 // F3: the element chain of one step for element e.  The nodal forces of the
 // element (f - f_hg, what the assembly adds to each node) go to sink(a, fx, fy, fz).
-template <typename T, int NN, int LAW, class Sink>
+// Hexahedral element passes of the rate laws in FP64 keep the hourglass
+// amplitudes and the Jacobian cofactors in LDS across the stress update: the
+// laws' transcendental functions otherwise push the pass past 256 VGPRs
+// (occupancy 1).  Tetrahedra stay at occupancy 2 without it.
+template <typename T, int NN, int LAW>
+constexpr bool kStashLDS = NN == 8 && LAW == LAW_RATE && std::is_same<T, double>::value;
+constexpr int kStashVals = 22;  // hm[3][4], c_h, C[3][3]
+
+// one LDS array per block size, shared by the store and the reload
+template <int BS>
+__device__ __forceinline__ double *stash_lds() {
+  __shared__ double stash[kStashVals * BS];
+  return stash;
+}
+
+template <typename T, int NN, int LAW, int BS, class Sink>
 __device__ __forceinline__ void element_chain(
     int e, int E, int N, const FusedParams &fp, const int *__restrict__ elnod, const double *__restrict__ x,
     const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
@@ -367,6 +382,18 @@ __device__ __forceinline__ void element_chain(
       const double rho_e = fp.rho_u > 0.0 ? fp.rho_u : (double)rho[e];
       c_h = (T)(fp.hg_coeff * pow((double)volv, 0.6666666) * rho_e * 0.25 * fp.cs0);
     }
+  }
+  if constexpr (kStashLDS<T, NN, LAW>) {
+    double *st = stash_lds<BS>() + threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) st[(4 * c + j) * BS] = fp.hg_coeff != 0.0 ? hm[c][j] : 0.0;
+    st[12 * BS] = c_h;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) st[(13 + 3 * r + q) * BS] = C[r][q];
   }
   // ---- ANP pressure (calcElemPressure element part) ----------------------
   const PressParams &pp = fp.pp;
@@ -435,26 +462,31 @@ __device__ __forceinline__ void element_chain(
   T sy, Hb = (T)mp.H;
   const bool per_elem = LAW == LAW_PLAIN && mp.model == BILINEAR && He;
   const double edot = (LAW == LAW_RATE) ? eff_strain_rate(D) : 0.0;
-  double temp = 0.0;  // the rate laws' temperature (law_temperature on the loaded nodes)
+  RateTemp rt = mp.ut;  // the rate laws' temperature factors (law_temperature on the loaded nodes)
   if constexpr (LAW == LAW_RATE) {
-    if (mp.Tn) {
+    if (!mp.t_uniform) {
+      double temp = 0.0;
+      if (mp.Tn) {
 #pragma unroll
-      for (int a = 0; a < NN; ++a) temp += mp.Tn[g[a]];
-      temp = temp / NN;
-    } else {
-      temp = mp.Te ? mp.Te[e] : mp.temp;
+        for (int a = 0; a < NN; ++a) temp += mp.Tn[g[a]];
+        temp = temp / NN;
+      } else {
+        temp = mp.Te[e];
+      }
+      rt = rate_temp_at(mp, temp);
     }
   }
+  RateCache rc;
   if (per_elem) {
     Hb = ld_stream(&He[e]);
     sy = ld_stream(&sy0[e]) + Hb * ep;
   } else {
-    sy = yield_stress<T, LAW>(mp, temp, ld_stream(&sy0[e]), ep, edot);
+    sy = yield_stress<T, LAW>(mp, rt, ld_stream(&sy0[e]), ep, edot, rc);
   }
   sigma_y[e] = sy;
   T dep = T(0);
   if (sy < sig_trial) {
-    T H = per_elem ? Hb : tangent_modulus<T, LAW>(mp, temp, ep, cap_rate(mp, edot));
+    T H = per_elem ? Hb : tangent_modulus<T, LAW>(mp, rt, ep, cap_rate(mp, edot), rc);
     T dgamma = (sig_trial - sy) / (T(3) * G + H);
     T factor = T(1) - (T(3) * G * dgamma) / sig_trial;
     tau = t3_scale(s, factor);
@@ -482,6 +514,18 @@ __device__ __forceinline__ void element_chain(
   st_stream(&tau_io[2 * Es + e], tau.zz); st_stream(&tau_io[3 * Es + e], tau.xy);
   st_stream(&tau_io[4 * Es + e], tau.yz); st_stream(&tau_io[5 * Es + e], tau.xz);
 
+  if constexpr (kStashLDS<T, NN, LAW>) {  // back from LDS (volatile: real loads)
+    volatile double *sv = stash_lds<BS>() + threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) hm[c][j] = sv[(4 * c + j) * BS];
+    c_h = sv[12 * BS];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) C[r][q] = sv[(13 + 3 * r + q) * BS];
+  }
   // ---- internal forces (calcElemForces) and hourglass ---------------------
   T sg0 = sig.xx, sg1 = sig.yy, sg2 = sig.zz, sg3 = sig.xy, sg4 = sig.yz, sg5 = sig.xz;
 #pragma unroll
@@ -526,7 +570,7 @@ __global__ __launch_bounds__(BS) void f3_element(
   int e = gtid();
   if (e >= E) return;
   const long Es = E;
-  element_chain<T, NN, LAW>(e, E, N, fp, elnod, x, v, ratio, vol0, rho, sy0, He, tau_io, eps_io, p_out, pl_strain,
+  element_chain<T, NN, LAW, BS>(e, E, N, fp, elnod, x, v, ratio, vol0, rho, sy0, He, tau_io, eps_io, p_out, pl_strain,
                        sigma_y, q_plheat, [&](int a, T f0, T f1, T f2) {
                          st_stream(&fout[fidx(a * 3 + 0, e, Es, 3 * NN)], f0);
                          st_stream(&fout[fidx(a * 3 + 1, e, Es, 3 * NN)], f1);

@@ -542,6 +542,35 @@ __global__ __launch_bounds__(kBlock) void k_pressure_elem(int E, int N, const in
   p[e] = (T)(p_physical + p_pspg + p_hg + p_q);
 }
 
+// Temperature factors of the rate laws, evaluated once per element, or once
+// per launch on the host when every element has the same temperature
+struct RateTemp {
+  double jc_soft;              // Johnson-Cook 1 - T_h^m
+  double g_a, g_n, g_m, g_i;   // GMT C1 exp(C2 Tc), n1 Tc + n2, m1 Tc + m2, I1 Tc + I2
+  double g_ni;                 // GMT -I1 Tc - I2
+};
+
+// the same expressions as the device's rate_temp_at (and the reference's)
+__host__ __device__ inline double clamp_range(double v, double lo, double hi) {
+  if (v < lo) return lo;
+  if (v > hi) return hi;
+  return v;
+}
+__host__ __device__ inline RateTemp rate_temp(double T_t, double T_m, double m_, double C1, double C2,
+                                              double n1, double n2, double m1, double m2, double I1,
+                                              double I2, double T_min, double T_max, double temp) {
+  RateTemp r;
+  const double T_h = (temp - T_t) / (T_m - T_t);
+  r.jc_soft = 1.0 - pow(T_h, m_);
+  const double Tc = clamp_range(temp, T_min, T_max);
+  r.g_a = C1 * exp(C2 * Tc);
+  r.g_n = n1 * Tc + n2;
+  r.g_m = m1 * Tc + m2;
+  r.g_i = I1 * Tc + I2;
+  r.g_ni = -I1 * Tc - I2;
+  return r;
+}
+
 struct MatParams {
   int model;
   double G, H, sy0, Kh, mh, eps0, eps1;
@@ -554,15 +583,25 @@ struct MatParams {
   const double *Tn;                        // thermal coupling: nodal temperature, or null
   const int *elnod;                        //   with the connectivity [nn][Es]
   int nn, Es;
+  int t_uniform;                           // every element at temp: ut holds the factors
+  RateTemp ut;
 };
+
+// the rate laws' temperature factors at temperature temp
+__device__ __forceinline__ RateTemp rate_temp_at(const MatParams &m, double temp) {
+  return rate_temp(m.T_t, m.T_m, m.m, m.C1, m.C2, m.n1, m.n2, m.m1, m.m2, m.I1, m.I2, m.T_min, m.T_max, temp);
+}
 
 inline MatParams mat_params(const wfx_ctx *c) {
   const Material &m = c->prm.mat;
+  const bool uniform = !c->temp_e && !c->thermal.on;
   return MatParams{m.model, m.G,     m.H,     m.sy0,   m.Kh,     m.mh,     m.eps0, m.eps1, m.E,
                    m.B,     m.n,     m.C,     m.eps_0, m.m,      m.T_m,    m.T_t,  m.C1,   m.C2,
                    m.n1,    m.n2,    m.m1,    m.m2,    m.I1,     m.I2,     m.e_min, m.e_max,
                    m.er_min, m.er_max, m.T_min, m.T_max, m.temp, m.max_edot, c->temp_e,
-                   c->thermal.on ? c->nd.T : nullptr, c->ms.elnod, c->n, c->E};
+                   c->thermal.on ? c->nd.T : nullptr, c->ms.elnod, c->n, c->E, uniform ? 1 : 0,
+                   rate_temp(m.T_t, m.T_m, m.m, m.C1, m.C2, m.n1, m.n2, m.m1, m.m2, m.I1, m.I2, m.T_min,
+                             m.T_max, m.temp)};
 }
 
 // The temperature the hardening law of element e reads: with thermal
@@ -591,12 +630,6 @@ __device__ __forceinline__ double cap_rate(const MatParams &m, double edot) {
   return (m.max_edot < edot) ? m.max_edot : edot;
 }
 
-// the clamps of the GMT law (Material.cuh:427-435, :451-459)
-__device__ __forceinline__ double clamp_range(double v, double lo, double hi) {
-  if (v < lo) return lo;
-  if (v > hi) return hi;
-  return v;
-}
 
 // Kernels that update stresses are compiled twice: LAW_PLAIN handles the
 // bilinear and Hollomon laws, LAW_RATE Johnson-Cook and GMT, so that the
@@ -604,12 +637,22 @@ __device__ __forceinline__ double clamp_range(double v, double lo, double hi) {
 enum { LAW_PLAIN = 0, LAW_RATE = 1 };
 inline int law_class(int model) { return model >= JOHNSON_COOK ? LAW_RATE : LAW_PLAIN; }
 
+// Values the tangent of a rate law shares with the yield stress of the same
+// element and step (equal inputs give equal values, so they are reused)
+struct RateCache {
+  double sr, lterm;  // Johnson-Cook: the rate and 1 + C ln(rate/eps_0)
+  double er, pw_er;  // GMT: the clamped rate and er^(m1 Tc + m2)
+  double ex_i;       // GMT: exp((I1 Tc + I2)/e)
+};
+
 // Calc*YieldStress (Material.cuh:353-364, :378-388, :412-441) at plastic
-// strain ep, effective strain rate edot and temperature temp (the rate laws); bilinear as
-// sigma_y = sy0_e + H*ep (the reference leaves it undefined).  The
-// rate-dependent laws evaluate in double in both precisions.
+// strain ep and effective strain rate edot, the rate laws with the
+// temperature factors rt; bilinear as sigma_y = sy0_e + H*ep (the reference
+// leaves it undefined).  The rate laws evaluate in double in both precisions
+// (the clamps of GMT: Material.cuh:427-435, :451-459).
 template <typename T, int LAW>
-__device__ __forceinline__ T yield_stress(const MatParams &m, double temp, T sy0e, T ep, double edot) {
+__device__ __forceinline__ T yield_stress(const MatParams &m, const RateTemp &rt, T sy0e, T ep, double edot,
+                                          RateCache &rc) {
   if constexpr (LAW == LAW_PLAIN) {
     if (m.model == HOLLOMON) {
       double s = (double)ep + m.eps0;
@@ -617,24 +660,23 @@ __device__ __forceinline__ T yield_stress(const MatParams &m, double temp, T sy0
     }
     return sy0e + (T)m.H * ep;
   } else if (m.model == JOHNSON_COOK) {
-    const double T_h = (temp - m.T_t) / (m.T_m - m.T_t);
-    const double sr = (edot == 0.0) ? 1.e-5 : edot;
-    return (T)(((double)sy0e + m.B * pow((double)ep, m.n)) * (1.0 + m.C * log(sr / m.eps_0)) *
-               (1.0 - pow(T_h, m.m)));
-  }
-  if (m.model == GMT) {
+    rc.sr = (edot == 0.0) ? 1.e-5 : edot;
+    rc.lterm = 1.0 + m.C * log(rc.sr / m.eps_0);
+    return (T)(((double)sy0e + m.B * pow((double)ep, m.n)) * rc.lterm * rt.jc_soft);
+  } else {
     const double es = clamp_range((double)ep, m.e_min, m.e_max);
-    const double er = clamp_range(edot, m.er_min, m.er_max);
-    const double Tc = clamp_range(temp, m.T_min, m.T_max);
-    return (T)(m.C1 * exp(m.C2 * Tc) * pow(es, m.n1 * Tc + m.n2) * exp((m.I1 * Tc + m.I2) / es) *
-               pow(er, m.m1 * Tc + m.m2));
+    rc.er = clamp_range(edot, m.er_min, m.er_max);
+    rc.pw_er = pow(rc.er, rt.g_m);
+    rc.ex_i = exp(rt.g_i / es);
+    return (T)(rt.g_a * pow(es, rt.g_n) * rc.ex_i * rc.pw_er);
   }
 }
 
 // Calc*TangentModulus (Material.cuh:389-395, :397-411, :444-473); edot is
 // the capped rate.
 template <typename T, int LAW>
-__device__ __forceinline__ T tangent_modulus(const MatParams &m, double temp, T ep, double edot) {
+__device__ __forceinline__ T tangent_modulus(const MatParams &m, const RateTemp &rt, T ep, double edot,
+                                             const RateCache &rc) {
   if constexpr (LAW == LAW_PLAIN) {
     if (m.model == HOLLOMON) {
       double s = (double)ep + m.eps0;
@@ -642,18 +684,16 @@ __device__ __forceinline__ T tangent_modulus(const MatParams &m, double temp, T 
     }
     return (T)m.H;
   } else if (m.model == JOHNSON_COOK) {
-    const double T_h = (temp - m.T_t) / (m.T_m - m.T_t);
-    if (ep > T(0))
-      return (T)(m.n * m.B * pow((double)ep, m.n - 1.) * (1.0 + m.C * log(edot / m.eps_0)) *
-                 (1.0 - pow(T_h, m.m)));
+    if (ep > T(0)) {
+      const double lterm = (edot == rc.sr) ? rc.lterm : 1.0 + m.C * log(edot / m.eps_0);
+      return (T)(m.n * m.B * pow((double)ep, m.n - 1.) * lterm * rt.jc_soft);
+    }
     return (T)(m.E * 0.1);
-  }
-  if (m.model == GMT) {
+  } else {
     const double es = clamp_range((double)ep, m.e_min, m.e_max);
     const double er = clamp_range(edot, m.er_min, m.er_max);
-    const double Tc = clamp_range(temp, m.T_min, m.T_max);
-    return (T)(m.C1 * exp(m.C2 * Tc) * pow(er, m.m1 * Tc + m.m2) * pow(es, Tc * m.n1 + m.n2 - 2.0) *
-               (-m.I1 * Tc - m.I2 + es * (m.n1 * Tc + m.n2)) * exp((m.I1 * Tc + m.I2) / es));
+    const double pw_er = (er == rc.er) ? rc.pw_er : pow(er, rt.g_m);
+    return (T)(rt.g_a * pw_er * pow(es, rt.g_n - 2.0) * (rt.g_ni + es * rt.g_n) * rc.ex_i);
   }
 }
 
@@ -705,12 +745,14 @@ __global__ __launch_bounds__(kBlock) void k_stress_strain(
   // bilinear plastic modulus: per element when given (specimen batches)
   const bool per_elem = LAW == LAW_PLAIN && mp.model == BILINEAR && He;
   const double edot = (LAW == LAW_RATE) ? eff_strain_rate(D) : 0.0;
-  const double temp = (LAW == LAW_RATE) ? law_temperature(mp, e) : 0.0;
-  T sy = per_elem ? sy0[e] + He[e] * ep : yield_stress<T, LAW>(mp, temp, sy0[e], ep, edot);
+  RateTemp rt = mp.ut;
+  if (LAW == LAW_RATE && !mp.t_uniform) rt = rate_temp_at(mp, law_temperature(mp, e));
+  RateCache rc;
+  T sy = per_elem ? sy0[e] + He[e] * ep : yield_stress<T, LAW>(mp, rt, sy0[e], ep, edot, rc);
   sigma_y[e] = sy;
   T dep = T(0);
   if (sy < sig_trial) {
-    T H = per_elem ? He[e] : tangent_modulus<T, LAW>(mp, temp, ep, cap_rate(mp, edot));
+    T H = per_elem ? He[e] : tangent_modulus<T, LAW>(mp, rt, ep, cap_rate(mp, edot), rc);
     T dgamma = (sig_trial - sy) / (T(3) * G + H);
     T factor = T(1) - (T(3) * G * dgamma) / sig_trial;
     tau = t3_scale(s, factor);
