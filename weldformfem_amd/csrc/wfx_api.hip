@@ -694,7 +694,7 @@ int wfx_thermal_calcs(wfx_ctx *c, wfx_stream s) {
   REQUIRE_MESH(c);
   REQUIRE(c->thermal.on, WFX_ERR_STATE, "thermal coupling is off: call wfx_set_thermal");
   REQUIRE(!c->fused_stale, WFX_ERR_STATE, "ThermalCalcs needs the Jacobians of a per-step API step");
-  LAUNCH(BY_PRECISION(c, launch_thermal, c, false, false, as_stream(s)));
+  LAUNCH(BY_PRECISION(c, launch_thermal, c, as_stream(s)));
   return WFX_OK;
 }
 
@@ -1080,7 +1080,7 @@ static int step_api(wfx_ctx *c, hipStream_t st, double *hist) {
   for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
   LAUNCH(launch_correction_pos(c, st));
   if (c->contact.on) LAUNCH(contact_rigid_update(c, c->prm.dt, st));  // TriMesh_d::Move (:996-1004)
-  if (c->thermal.on) LAUNCH(BY_PRECISION(c, launch_thermal, c, false, false, st));  // ThermalCalcs
+  if (c->thermal.on) LAUNCH(BY_PRECISION(c, launch_thermal, c, st));  // ThermalCalcs
   c->contact.step_count++;
   c->fused_stale = false;  // every per-step intermediate is current again
   return WFX_OK;

@@ -219,6 +219,7 @@ struct FusedParams {
   double rho_u;       // > 0: every element density equals it (no density loads)
   const double *dtp;  // non-null: the step's dt from the device (per-step CFL time step)
   double plheat_frac; // thermal coupling: fraction of plastic work into heat
+  double k_T, cp_T;   //   conductivity and specific heat (rates when mp.Tn is set)
 };
 
 // dH(d,a) = adj(J)(d,:) . dN_a/dxi from the cofactor matrix C (adj = C^T),
@@ -307,22 +308,27 @@ __device__ __forceinline__ void elem_jacobian(const int (&g)[NN], const double *
 // (occupancy 1).  Tetrahedra stay at occupancy 2 without it.
 template <typename T, int NN, int LAW>
 constexpr bool kStashLDS = NN == 8 && LAW == LAW_RATE && std::is_same<T, double>::value;
-constexpr int kStashVals = 22;  // hm[3][4], c_h, C[3][3]
+// hm[3][4], c_h, C[3][3] and with thermal coupling the gradient term G[3]
+template <bool TH>
+constexpr int kStashVals = TH ? 25 : 22;
 
 // one LDS array per block size, shared by the store and the reload
-template <int BS>
+template <int BS, bool TH>
 __device__ __forceinline__ double *stash_lds() {
-  __shared__ double stash[kStashVals * BS];
+  __shared__ double stash[kStashVals<TH> * BS];
   return stash;
 }
 
-template <typename T, int NN, int LAW, int BS, class Sink>
+// TH: thermal coupling (nodal temperatures, plastic heat and the element's
+// thermal terms); a compile-time switch so that runs without it keep their
+// register budget.
+template <typename T, int NN, int LAW, int BS, bool TH, class Sink>
 __device__ __forceinline__ void element_chain(
     int e, int E, int N, const FusedParams &fp, const int *__restrict__ elnod, const double *__restrict__ x,
     const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
     const T *__restrict__ rho, const T *__restrict__ sy0, const T *__restrict__ He,
     T *__restrict__ tau_io, T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
-    T *__restrict__ sigma_y, T *__restrict__ q_plheat, Sink &&sink) {
+    T *__restrict__ sigma_y, T *__restrict__ q_plheat, T *__restrict__ dTe, Sink &&sink) {
   const long Es = E;
   int g[NN];
 #pragma unroll
@@ -383,8 +389,32 @@ __device__ __forceinline__ void element_chain(
       c_h = (T)(fp.hg_coeff * pow((double)volv, 0.6666666) * rho_e * 0.25 * fp.cs0);
     }
   }
+  // ---- thermal coupling (ThermalCalcs, Thermal.C:27-115) -----------------
+  // (Kt T)_i = sum_d dH(d,i) G_d with G_d = k_T w / detJ sum_j dH(d,j) T_j:
+  // the conduction matrix applied through the element's temperature gradient.
+  // The nodal temperatures also give the rate laws' temperature (their mean).
+  double gT0 = 0.0, gT1 = 0.0, gT2 = 0.0, tsum = 0.0;
+  if constexpr (TH) {
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      const double ta = fp.mp.Tn[g[a]];
+      tsum += ta;
+      gT0 += (double)shape_dH<T, NN>(C, 0, a) * ta;
+      gT1 += (double)shape_dH<T, NN>(C, 1, a) * ta;
+      gT2 += (double)shape_dH<T, NN>(C, 2, a) * ta;
+    }
+    const double kf = fp.k_T / (double)detJ * (NN == 4 ? 1.0 / 6.0 : 8.0);
+    gT0 *= kf;
+    gT1 *= kf;
+    gT2 *= kf;
+  }
   if constexpr (kStashLDS<T, NN, LAW>) {
-    double *st = stash_lds<BS>() + threadIdx.x;
+    double *st = stash_lds<BS, TH>() + threadIdx.x;
+    if constexpr (TH) {
+      st[22 * BS] = gT0;
+      st[23 * BS] = gT1;
+      st[24 * BS] = gT2;
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -465,14 +495,7 @@ __device__ __forceinline__ void element_chain(
   RateTemp rt = mp.ut;  // the rate laws' temperature factors (law_temperature on the loaded nodes)
   if constexpr (LAW == LAW_RATE) {
     if (!mp.t_uniform) {
-      double temp = 0.0;
-      if (mp.Tn) {
-#pragma unroll
-        for (int a = 0; a < NN; ++a) temp += mp.Tn[g[a]];
-        temp = temp / NN;
-      } else {
-        temp = mp.Te[e];
-      }
+      const double temp = TH ? tsum / NN : mp.Te[e];
       rt = rate_temp_at(mp, temp);
     }
   }
@@ -494,7 +517,8 @@ __device__ __forceinline__ void element_chain(
     dep = dgamma;
   }
   T3<T> sig = t3_add(pI, tau);
-  if (q_plheat) {  // thermal coupling: plastic heat source of the step
+  double heat = 0.0;  // thermal coupling: the element's heat per node and cp
+  if constexpr (TH) {  // plastic heat source of the step
     T q = T(0);
     if (dep > T(0)) {
       const T f = dep / sy;
@@ -503,6 +527,7 @@ __device__ __forceinline__ void element_chain(
                           T(1.5) * f * (sig.xz), dt, fp.plheat_frac);
     }
     q_plheat[e] = q;
+    heat = 0.9 * (double)q * (double)volv / NN / fp.cp_T;
   }
   st_stream(&eps_io[0 * Es + e], ld_stream(&eps_io[0 * Es + e]) + D.xx * dt);
   st_stream(&eps_io[1 * Es + e], ld_stream(&eps_io[1 * Es + e]) + D.yy * dt);
@@ -515,7 +540,7 @@ __device__ __forceinline__ void element_chain(
   st_stream(&tau_io[4 * Es + e], tau.yz); st_stream(&tau_io[5 * Es + e], tau.xz);
 
   if constexpr (kStashLDS<T, NN, LAW>) {  // back from LDS (volatile: real loads)
-    volatile double *sv = stash_lds<BS>() + threadIdx.x;
+    volatile double *sv = stash_lds<BS, TH>() + threadIdx.x;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -525,6 +550,11 @@ __device__ __forceinline__ void element_chain(
     for (int r = 0; r < 3; ++r)
 #pragma unroll
       for (int q = 0; q < 3; ++q) C[r][q] = sv[(13 + 3 * r + q) * BS];
+    if constexpr (TH) {
+      gT0 = sv[22 * BS];
+      gT1 = sv[23 * BS];
+      gT2 = sv[24 * BS];
+    }
   }
   // ---- internal forces (calcElemForces) and hourglass ---------------------
   T sg0 = sig.xx, sg1 = sig.yy, sg2 = sig.zz, sg3 = sig.xy, sg4 = sig.yz, sg5 = sig.xz;
@@ -557,21 +587,23 @@ __device__ __forceinline__ void element_chain(
       }
     }
     sink(a, f0, f1, f2);
+    // the node's thermal term times its mass: -(Kt T)_a + heat / cp
+    if constexpr (TH) dTe[fidx(a, e, Es, NN)] = (T)(heat - ((double)h0 * gT0 + (double)h1 * gT1 + (double)h2 * gT2));
   }
 }
 
-template <typename T, int NN, int BS, int LAW>
+template <typename T, int NN, int BS, int LAW, bool TH>
 __global__ __launch_bounds__(BS) void f3_element(
     int E, int N, FusedParams fp, const int *__restrict__ elnod, const double *__restrict__ x,
     const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
     const T *__restrict__ rho, const T *__restrict__ sy0, const T *__restrict__ He,
     T *__restrict__ tau_io, T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
-    T *__restrict__ sigma_y, T *__restrict__ fout, T *__restrict__ q_plheat) {
+    T *__restrict__ sigma_y, T *__restrict__ fout, T *__restrict__ q_plheat, T *__restrict__ dTe) {
   int e = gtid();
   if (e >= E) return;
   const long Es = E;
-  element_chain<T, NN, LAW, BS>(e, E, N, fp, elnod, x, v, ratio, vol0, rho, sy0, He, tau_io, eps_io, p_out, pl_strain,
-                       sigma_y, q_plheat, [&](int a, T f0, T f1, T f2) {
+  element_chain<T, NN, LAW, BS, TH>(e, E, N, fp, elnod, x, v, ratio, vol0, rho, sy0, He, tau_io, eps_io, p_out, pl_strain,
+                       sigma_y, q_plheat, dTe, [&](int a, T f0, T f1, T f2) {
                          st_stream(&fout[fidx(a * 3 + 0, e, Es, 3 * NN)], f0);
                          st_stream(&fout[fidx(a * 3 + 1, e, Es, 3 * NN)], f1);
                          st_stream(&fout[fidx(a * 3 + 2, e, Es, 3 * NN)], f2);
@@ -634,29 +666,6 @@ __global__ __launch_bounds__(kBlock) void k_thermal_elem(
   thermal_rates<T, NN>(e, E, g, h, (double)detJ[e], (double)vol[e], (double)q[e], Tn, mdiag, k_T, cp_T, dTedt);
 }
 
-// fused path: the Jacobian recomputed from the positions (still those of the
-// step's start: it runs before the node pass)
-template <typename T, int NN>
-__global__ __launch_bounds__(kBlock) void k_thermal_elem_x(
-    int E, int N, const int *__restrict__ elnod, const double *__restrict__ x, const T *__restrict__ vol,
-    const T *__restrict__ q, const double *__restrict__ Tn, const double *__restrict__ mdiag, double k_T,
-    double cp_T, T *__restrict__ dTedt) {
-  const int e = gtid();
-  if (e >= E) return;
-  int g[NN];
-#pragma unroll
-  for (int a = 0; a < NN; ++a) g[a] = elnod[(long)a * E + e];
-  T C[3][3];
-  T detJ;
-  elem_jacobian<T, NN>(g, x, N, C, detJ);
-  double h[3][NN];
-#pragma unroll
-  for (int a = 0; a < NN; ++a)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) h[d][a] = (double)shape_dH<T, NN>(C, d, a);
-  thermal_rates<T, NN>(e, E, g, h, (double)detJ, (double)vol[e], (double)q[e], Tn, mdiag, k_T, cp_T, dTedt);
-}
-
 // T += (sum of the node's element-node rates in CSR order) * dt; the rates
 // sit in the element-force layout (fidx) and are gathered like the forces
 template <typename T>
@@ -665,10 +674,9 @@ __global__ __launch_bounds__(kBlock) void k_thermal_node(int N, long Es, int nn,
                                                          const int *__restrict__ nodel_pk,
                                                          const int *__restrict__ pad,
                                                          const T *__restrict__ dTedt, double dt,
-                                                         const double *__restrict__ dtp, double *__restrict__ Tn) {
+                                                         double *__restrict__ Tn) {
   const int i = gtid();
   if (i >= N) return;
-  if (dtp) dt = *dtp;
   const int c = cnt[i], o = c > kChunk ? off[i] : 0;
   double dTdt = 0.0;
   for (int k = 0; k < c; k += kChunk) {
@@ -768,16 +776,18 @@ __device__ __forceinline__ double node_update(
 // Sum of the element forces of node i over its CSR entries, in CSR order
 // (assemblyForces, Matrices.C:42-87); the first eight entries come from the
 // padded [8][N] table.
-template <typename T>
+// TH: also the element-node thermal terms dTe ([n][E] in the same tiling) into f[3].
+template <typename T, bool TH = false>
 __device__ __forceinline__ void gather_force(int i, long Ns, long Es, int nrow, const int *__restrict__ off,
                                              const int *__restrict__ cnt, const int *__restrict__ nodel_pk,
                                              const int *__restrict__ pad, const T *__restrict__ fel,
-                                             double (&f)[3]) {
+                                             double (&f)[TH ? 4 : 3], const T *__restrict__ dTe = nullptr) {
   const int c = cnt[i], o = c > kChunk ? off[i] : 0;
-  f[0] = f[1] = f[2] = 0.0;
+#pragma unroll
+  for (int d = 0; d < (TH ? 4 : 3); ++d) f[d] = 0.0;
   for (int k = 0; k < c; k += kChunk) {
     int pk[kChunk];
-    double q[kChunk][3];
+    T q[kChunk][TH ? 4 : 3];  // widened when summed
 #pragma unroll
     for (int j = 0; j < kChunk; ++j)
       pk[j] = (k == 0) ? pad[(long)j * Ns + i] : ((k + j < c) ? nodel_pk[o + k + j] : -1);
@@ -785,21 +795,24 @@ __device__ __forceinline__ void gather_force(int i, long Ns, long Es, int nrow, 
     for (int j = 0; j < kChunk; ++j) {
       long e = pk[j] >> 3, b = (long)(pk[j] & 7) * 3;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) q[j][d] = (pk[j] >= 0) ? (double)fel[fidx((int)(b + d), e, Es, nrow)] : 0.0;
+      for (int d = 0; d < 3; ++d) q[j][d] = (pk[j] >= 0) ? fel[fidx((int)(b + d), e, Es, nrow)] : T(0);
+      if constexpr (TH) q[j][3] = (pk[j] >= 0) ? dTe[fidx(pk[j] & 7, e, Es, nrow / 3)] : T(0);
     }
 #pragma unroll
     for (int j = 0; j < kChunk; ++j)
       if (k + j < c) {
-        f[0] += q[j][0];
-        f[1] += q[j][1];
-        f[2] += q[j][2];
+        f[0] += (double)q[j][0];
+        f[1] += (double)q[j][1];
+        f[2] += (double)q[j][2];
+        if constexpr (TH) f[3] += (double)q[j][3];
       }
   }
 }
 
 // F4: assembly and the whole nodal update of one step; CFL: dt from the
-// device and each block's largest predicted speed for the next step's dt
-template <typename T, int BS, bool CFL, bool CONTACT>
+// device and each block's largest predicted speed for the next step's dt;
+// TH: thermal coupling, T += (sum of the node's thermal terms) / m * dt
+template <typename T, int BS, bool CFL, bool CONTACT, bool TH>
 __global__ __launch_bounds__(BS) void f4_node(
     int N, int E, NodeParams np, const double *__restrict__ cf, const int *__restrict__ off,
     const int *__restrict__ cnt,
@@ -807,15 +820,18 @@ __global__ __launch_bounds__(BS) void f4_node(
     const double *__restrict__ fe, const double *__restrict__ mdiag, const int *__restrict__ bc_slot,
     const int *__restrict__ bc_mask, const double *__restrict__ bc_val, T *__restrict__ fi,
     T *__restrict__ v, T *__restrict__ prev_a, T *__restrict__ u_dt,
-    double *__restrict__ x, double *__restrict__ u, unsigned int *__restrict__ status) {
+    double *__restrict__ x, double *__restrict__ u, unsigned int *__restrict__ status,
+    const T *__restrict__ dTe, double *__restrict__ Tn) {
   int i = gtid();
   NodeParams q = np;
   if constexpr (CFL) set_step_length(q, *q.dtp);
   double speed = 0.0;
   if (i < N) {
-    double f[3];
-    gather_force<T>(i, N, E, q.nrow, off, cnt, nodel_pk, pad, fel, f);
-    speed = node_update<T, CFL, CONTACT>(i, N, q, f, cf, fe, mdiag, bc_slot, bc_mask, bc_val, fi, v, prev_a, u_dt, x, u, status);
+    double f[TH ? 4 : 3];
+    gather_force<T, TH>(i, N, E, q.nrow, off, cnt, nodel_pk, pad, fel, f, dTe);
+    speed = node_update<T, CFL, CONTACT>(i, N, q, *reinterpret_cast<double(*)[3]>(f), cf, fe, mdiag, bc_slot, bc_mask,
+                                         bc_val, fi, v, prev_a, u_dt, x, u, status);
+    if constexpr (TH) Tn[i] += f[3] / mdiag[i] * q.dt;
   }
   if constexpr (CFL) {
     if (q.predict) {
@@ -953,10 +969,20 @@ hipError_t launch_cfl_reduce(wfx_ctx *c, int nmin, int step, hipStream_t s) {
 }
 
 template <typename T>
-hipError_t launch_thermal(wfx_ctx *c, bool fused, bool device_dt, hipStream_t s);
+hipError_t launch_thermal(wfx_ctx *c, hipStream_t s);
 
 // Launch one phase (1..4) of a fused step on the current state with block
 // size BS; launch_fused_phase picks BS from the context (autotuned or 256).
+template <typename T, int NN, int BS, int LAW>
+auto f3_pick(bool th) {
+  return th ? f3_element<T, NN, BS, LAW, true> : f3_element<T, NN, BS, LAW, false>;
+}
+
+template <typename T, int BS, bool CFL, bool CT>
+auto f4_pick(bool th) {
+  return th ? f4_node<T, BS, CFL, CT, true> : f4_node<T, BS, CFL, CT, false>;
+}
+
 template <typename T, int BS>
 hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, hipStream_t s) {
   // per-step CFL time step inside a run; otherwise (step < 0) the host's dt
@@ -1005,18 +1031,15 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
     fp.rho_u = c->rho_uniform;
     fp.dtp = dtp;
     fp.plheat_frac = c->thermal.plheat_frac;
-    const bool rate = law_class(fp.mp.model) == LAW_RATE;
-    auto kern = (c->n == 8) ? (rate ? f3_element<T, 8, BS, LAW_RATE> : f3_element<T, 8, BS, LAW_PLAIN>)
-                            : (rate ? f3_element<T, 4, BS, LAW_RATE> : f3_element<T, 4, BS, LAW_PLAIN>);
+    fp.k_T = c->thermal.k_T;
+    fp.cp_T = c->thermal.cp_T;
+    const bool rate = law_class(fp.mp.model) == LAW_RATE, th = c->thermal.on;
+    auto kern = (c->n == 8) ? (rate ? f3_pick<T, 8, BS, LAW_RATE>(th) : f3_pick<T, 8, BS, LAW_PLAIN>(th))
+                            : (rate ? f3_pick<T, 4, BS, LAW_RATE>(th) : f3_pick<T, 4, BS, LAW_PLAIN>(th));
     hipLaunchKernelGGL(kern, dim3(gE), dim3(BS), 0, s, E, N, fp, c->ms.elnod, c->nd.x, vN, c->nd.anp_ratio,
                        el.vol0, el.rho, el.sy0, el.He, el.tau, el.eps, el.p, el.pl_strain, el.sigma_y,
-                       el.f_elem, c->thermal.on ? el.q_plheat : (T *)nullptr);
-    // thermal coupling: ThermalCalcs on this step's geometry, before the
-    // node pass moves the positions
-    if (c->thermal.on) {
-      hipError_t e = launch_thermal<T>(c, true, cfl, s);
-      if (e != hipSuccess) return e;
-    }
+                       el.f_elem, c->thermal.on ? el.q_plheat : (T *)nullptr,
+                       c->thermal.on ? el.dTedt : (T *)nullptr);
     // contact forces after the element forces, before the assembly
     if (c->contact.on) return launch_contact<T>(c, vN, cfl, s);
   } else if (phase == 4) {
@@ -1024,12 +1047,13 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
     np.dtp = dtp;
     np.vmax_part = cfl ? c->cfl_part + c->cfl_cap_min : nullptr;
     if (cfl && predict) c->cfl_nvmax = gN;
-    const bool ct = c->contact.on;
-    auto kern = cfl ? (ct ? f4_node<T, BS, true, true> : f4_node<T, BS, true, false>)
-                    : (ct ? f4_node<T, BS, false, true> : f4_node<T, BS, false, false>);
+    const bool ct = c->contact.on, th = c->thermal.on;
+    auto kern = cfl ? (ct ? f4_pick<T, BS, true, true>(th) : f4_pick<T, BS, true, false>(th))
+                    : (ct ? f4_pick<T, BS, false, true>(th) : f4_pick<T, BS, false, false>(th));
     hipLaunchKernelGGL(kern, dim3(gN), dim3(BS), 0, s, N, E, np, c->contact.contforce, c->ms.nodel_offset,
                        c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.f_elem, c->nd.fe, c->nd.mdiag,
-                       c->bc_slot, c->bc_mask, c->bc_tab, fiN, vN, paN, udN, c->nd.x, c->nd.u, c->status);
+                       c->bc_slot, c->bc_mask, c->bc_tab, fiN, vN, paN, udN, c->nd.x, c->nd.u, c->status,
+                       el.dTedt, c->nd.T);
     // the rigid surfaces move after the position update
     if (c->contact.on && step >= 0) return launch_rigid_update(c, cfl, s);
   } else if (phase == 5) {
@@ -1073,27 +1097,19 @@ hipError_t launch_cfl_step(wfx_ctx *c, int step, hipStream_t s) {
   return launch_cfl_reduce<T>(c, nblocks(c->E), -1, s);
 }
 
-// ThermalCalcs of one step: element rates (per-step API: stored Jacobians;
-// fused: recomputed from the positions) and the nodal update with the
-// host's dt or, with the per-step CFL time step inside a fused run, the
-// device's.
+// ThermalCalcs of one per-step API step: element rates from the stored
+// Jacobians, then the nodal update with the host's dt.  (The fused path
+// folds both into its element and node passes.)
 template <typename T>
-hipError_t launch_thermal(wfx_ctx *c, bool fused, bool device_dt, hipStream_t s) {
+hipError_t launch_thermal(wfx_ctx *c, hipStream_t s) {
   auto &el = elem<T>(c);
   const auto &th = c->thermal;
-  const int gE = nblocks(c->E);
-  if (fused) {
-    auto kern = c->n == 8 ? k_thermal_elem_x<T, 8> : k_thermal_elem_x<T, 4>;
-    hipLaunchKernelGGL(kern, dim3(gE), dim3(kBlock), 0, s, c->E, c->N, c->ms.elnod, c->nd.x, el.vol, el.q_plheat,
-                       c->nd.T, c->nd.mdiag, th.k_T, th.cp_T, el.dTedt);
-  } else {
-    auto kern = c->n == 8 ? k_thermal_elem<T, 8> : k_thermal_elem<T, 4>;
-    hipLaunchKernelGGL(kern, dim3(gE), dim3(kBlock), 0, s, c->E, c->ms.elnod, el.dH, el.detJ, el.vol, el.q_plheat,
-                       c->nd.T, c->nd.mdiag, th.k_T, th.cp_T, el.dTedt);
-  }
+  auto kern = c->n == 8 ? k_thermal_elem<T, 8> : k_thermal_elem<T, 4>;
+  hipLaunchKernelGGL(kern, dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, c->ms.elnod, el.dH, el.detJ, el.vol,
+                     el.q_plheat, c->nd.T, c->nd.mdiag, th.k_T, th.cp_T, el.dTedt);
   hipLaunchKernelGGL((k_thermal_node<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N, (long)c->E, c->n,
                      c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.dTedt, c->prm.dt,
-                     device_dt ? c->scalars + kCflDt : nullptr, c->nd.T);
+                     c->nd.T);
   return hipGetLastError();
 }
 

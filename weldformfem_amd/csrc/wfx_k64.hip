@@ -10,7 +10,7 @@ WFX_DEFINE_LAUNCHERS(double)
 template hipError_t launch_vsum0<double>(wfx_ctx *, hipStream_t);
 template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipStream_t, int);
 template hipError_t launch_cfl_step<double>(wfx_ctx *, int, hipStream_t);
-template hipError_t launch_thermal<double>(wfx_ctx *, bool, bool, hipStream_t);
+template hipError_t launch_thermal<double>(wfx_ctx *, hipStream_t);
 template hipError_t launch_fused_steps<double>(wfx_ctx *, int, double *, hipStream_t);
 template hipError_t launch_fused_pack<double>(wfx_ctx *, hipStream_t);
 template hipError_t launch_fused_unpack<double>(wfx_ctx *, hipStream_t);
