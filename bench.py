@@ -420,7 +420,7 @@ def adjoint_leg(torch, log, steps=2000, cells=30, with_oracle=True):
                          "loss_rel_err": abs(L32 - L) / L}
     # identification from theta0
     t0 = time.perf_counter()
-    theta, loss, nit, losses = identify(pb, target, theta0, scale=np.array([300e6, 1e9, 0.06]), max_iter=30)
+    theta, loss, nit, losses = identify(pb, target, theta0, scale=np.array([300e6, 1e9, 0.06]), max_iter=100)
     out["identification"] = {"theta0": theta0.tolist(), "hidden": hidden.tolist(), "found": theta.tolist(),
                              "rel_err": (np.abs(theta - hidden) / hidden).tolist(), "iterations": nit,
                              "loss_ratio": loss / losses[0], "seconds": time.perf_counter() - t0}

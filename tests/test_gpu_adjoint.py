@@ -204,7 +204,8 @@ def test_torch_autograd_function(fp64_case):
 def test_parameter_identification(fp64_case):
     pb, target, L, g = fp64_case
     say("case: identification of (sigma_y0, Et, hg_coeff) from the reaction history")
-    theta, loss, nit, losses = identify(pb, target, THETA0, scale=np.array([300e6, 1e9, 0.06]), max_iter=40)
+    theta, loss, nit, losses = identify(pb, target, THETA0, scale=np.array([300e6, 1e9, 0.06]), max_iter=100)
     say(f"  theta {theta} hidden {HIDDEN} loss {loss:.3e} after {nit} iterations")
-    assert loss < 1e-4 * losses[0]
-    assert abs(theta[0] / HIDDEN[0] - 1.0) < 1e-2
+    assert loss < 1e-12 * losses[0]
+    # all three parameters, hg_coeff included (its signal is weak but noise-free)
+    np.testing.assert_allclose(theta, HIDDEN, rtol=1e-6)
