@@ -394,6 +394,27 @@ __global__ __launch_bounds__(kBlock) void k_elem_density(int E, const T *__restr
   rho[e] = rho0[e] * vol0[e] / vol[e];
 }
 
+// Walk of node i's CSR entries [o, o + c) in batches of kCsrBatch: the batch's
+// entry indices, then its data loads, are issued independently of each other,
+// and acc(value) runs in CSR order (sums as in a plain loop, bit for bit).
+constexpr int kCsrBatch = 8;
+template <class Load, class Acc>
+__device__ __forceinline__ void csr_walk(int o, int c, const int *__restrict__ nodel, Load load, Acc acc) {
+  using V = decltype(load(0, 0));
+  for (int k = 0; k < c; k += kCsrBatch) {
+    int ent[kCsrBatch];
+#pragma unroll
+    for (int j = 0; j < kCsrBatch; ++j) ent[j] = (k + j < c) ? nodel[o + k + j] : -1;
+    V val[kCsrBatch];
+#pragma unroll
+    for (int j = 0; j < kCsrBatch; ++j)
+      if (ent[j] >= 0) val[j] = load(ent[j], o + k + j);
+#pragma unroll
+    for (int j = 0; j < kCsrBatch; ++j)
+      if (k + j < c) acc(val[j]);
+  }
+}
+
 // CalcNodalVol (Mechanical.C:1555-1572): deterministic CSR gather
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_nodal_vol(int N, const int *__restrict__ off,
@@ -405,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_nodal_vol(int N, const int *__restri
   if (i >= N) return;
   int o = off[i], c = cnt[i];
   double s = 0.0;
-  for (int k = 0; k < c; ++k) s += (double)vol[nodel[o + k]];
+  csr_walk(o, c, nodel, [&](int e, int) { return (double)vol[e]; }, [&](double x) { s += x; });
   s /= nodxelem;
   voln[i] = s;
 }
@@ -422,7 +443,8 @@ __global__ __launch_bounds__(kBlock) void k_nodal_mass(int N, const int *__restr
   if (i >= N) return;
   int o = off[i], c = cnt[i];
   double vn = voln[i], dc = (double)c, mass = 0.0;
-  for (int k = 0; k < c; ++k) mass += 1.0 * (double)rho[nodel[o + k]] * vn / dc;
+  csr_walk(o, c, nodel, [&](int e, int) { return (double)rho[e]; },
+           [&](double r) { mass += 1.0 * r * vn / dc; });
   mdiag[i] = mass;
 }
 
@@ -472,11 +494,11 @@ __global__ __launch_bounds__(kBlock) void k_pressure_nodes(int N, const int *__r
   if (i >= N) return;
   int o = off[i], c = cnt[i];
   double s0 = 0.0, s = 0.0;
-  for (int k = 0; k < c; ++k) {
-    int e = nodel[o + k];
-    s0 += (double)vol0[e];
-    s += (double)vol[e];
-  }
+  csr_walk(o, c, nodel, [&](int e, int) { return double2{(double)vol0[e], (double)vol[e]}; },
+           [&](double2 q) {
+             s0 += q.x;
+             s += q.y;
+           });
   v0s[i] = s0;
   vs[i] = s;
 }
@@ -900,20 +922,23 @@ __global__ __launch_bounds__(kBlock) void k_assembly(int N, int E, const int *__
   const long Es = E;
   int o = off[i], c = cnt[i];
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  for (int k = 0; k < c; ++k) {
-    long e = nodel[o + k], b = (long)nloc[o + k] * 3;
-    f0 += (double)fel[(b + 0) * Es + e];
-    f1 += (double)fel[(b + 1) * Es + e];
-    f2 += (double)fel[(b + 2) * Es + e];
-  }
-  if (fhg) {
-    for (int k = 0; k < c; ++k) {
-      long e = nodel[o + k], b = (long)nloc[o + k] * 3;
-      f0 -= (double)fhg[(b + 0) * Es + e];
-      f1 -= (double)fhg[(b + 1) * Es + e];
-      f2 -= (double)fhg[(b + 2) * Es + e];
-    }
-  }
+  auto row3 = [&](const T *__restrict__ f) {
+    return [=](int e, int k) {
+      const long b = (long)nloc[k] * 3;
+      return double3{(double)f[(b + 0) * Es + e], (double)f[(b + 1) * Es + e], (double)f[(b + 2) * Es + e]};
+    };
+  };
+  csr_walk(o, c, nodel, row3(fel), [&](double3 q) {
+    f0 += q.x;
+    f1 += q.y;
+    f2 += q.z;
+  });
+  if (fhg)
+    csr_walk(o, c, nodel, row3(fhg), [&](double3 q) {
+      f0 -= q.x;
+      f1 -= q.y;
+      f2 -= q.z;
+    });
   unsigned bad = 0;
   if (!isfinite(f0)) { f0 = 0.0; ++bad; }
   if (!isfinite(f1)) { f1 = 0.0; ++bad; }
