@@ -1,0 +1,145 @@
+"""Edge cases of the boundary on the GPU.
+
+* The smallest meshes: one hexahedron, one tetrahedron, one cube split into
+  six tetrahedra: 200-step compressions on the per-step API and the fused
+  path against the oracle (FP64, relative 1e-12).
+* Non-finite state: a NaN velocity planted at one node.  The reference zeroes
+  non-finite internal-force components before the acceleration
+  (Solver_explicit.C:777-785); both paths and the oracle must agree on how
+  many were zeroed, on which entries became NaN and on every finite value.
+* Invalid arguments: connectivity outside the node range, an unsupported
+  element, field sizes and names, material constants and time steps are
+  refused with WfxError before anything reaches the device.
+"""
+import numpy as np
+import pytest
+
+from oracle.wfref import RefDomain
+from weldformfem_amd import Domain, WFX_FP64, workloads
+from weldformfem_amd._lib import WfxError
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def rel(a, b):
+    s = np.max(np.abs(b))
+    return np.max(np.abs(a - b)) / (s if s > 0 else 1.0)
+
+
+def one_tet(h=0.1):
+    x = np.array([[0, 0, 0], [h, 0, 0], [0, h, 0], [0, 0, h]], float)
+    return x.reshape(-1), np.array([0, 1, 2, 3], np.int32), 4, np.array([0, 1, 2]), np.array([3])
+
+
+def one_cube(tet, h=0.1):
+    r = RefDomain()
+    r.AddBoxLength([0, 0, 0], [h, h, h], h / 2, True, tet)
+    bottom, top = workloads.box_layers(1, 1, 1)
+    return r.field("x"), r.field("elnod"), 4 if tet else 8, bottom, top
+
+
+MESHES = {
+    "one_hexa": lambda: one_cube(False),
+    "one_tet": one_tet,
+    "cube_six_tets": lambda: one_cube(True),
+}
+
+
+def _pair(mesh, v_top=-1.0):
+    x, el, n, bottom, top = MESHES[mesh]()
+    g, r = Domain(WFX_FP64), RefDomain()
+    for d in (g, r):
+        d.CreateFromMesh(x, el, n)
+    dt = workloads.setup_compression(r, bottom, top, v_top)
+    workloads.setup_compression(g, bottom, top, v_top, dt=dt)
+    for d in (g, r):
+        d.init()
+    return g, r
+
+
+@pytest.mark.parametrize("mesh", sorted(MESHES))
+@pytest.mark.parametrize("path", ["api", "fused"])
+def test_smallest_meshes(mesh, path):
+    say(f"case: {mesh}, {path} path, 200 steps FP64 vs oracle")
+    g, r = _pair(mesh)
+    hg = g.run(200) if path == "api" else np.concatenate([g.runFused(1), g.runFused(199)])
+    hr = r.run(200)
+    assert np.max(r.field("pl_strain")) > 0  # the specimen yields
+    for f in ["u", "v", "pl_strain", "p", "sigma"]:
+        e = rel(g.field(f), r.field(f))
+        assert e <= 1e-12, f"{f}: {e:.3e}"
+    assert rel(hg, hr) <= 1e-12
+
+
+@pytest.mark.parametrize("path", ["api", "fused"])
+def test_nonfinite_velocity_is_scrubbed_like_the_reference(path):
+    say(f"case: NaN velocity at one node, {path} path, 2 steps FP64 vs oracle")
+    r0 = RefDomain()
+    r0.AddBoxLength([0, 0, 0], [0.4, 0.4, 0.4], 0.05)
+    bottom, top = workloads.box_layers(4, 4, 4)
+    x, el = r0.field("x"), r0.field("elnod")
+    g, r = Domain(WFX_FP64), RefDomain()
+    for d in (g, r):
+        d.CreateFromMesh(x, el, 8)
+    dt = workloads.setup_compression(r, bottom, top, -1.0)
+    workloads.setup_compression(g, bottom, top, -1.0, dt=dt)
+    k = 2 * 25 + 2 * 5 + 2  # the centre node of the 5x5x5 node grid
+    for d in (g, r):
+        d.init()
+        v = d.field("v")
+        v[3 * k] = np.nan
+        d.setField("v", v)
+    if path == "api":
+        g.run(2)
+    else:
+        g.runFused(2)
+    r.run(2)
+    # the NaN reaches the pressure of every element through the nodal volume
+    # ratios within two steps; plastic strain stays finite (a NaN trial
+    # stress never exceeds the yield stress) and the scrubbed forces are finite
+    assert r.nonfinite_fi > 0
+    assert g.nonfiniteCount() == r.nonfinite_fi
+    for f in ["u", "v", "fi", "p", "pl_strain", "sigma"]:
+        a, b = g.field(f), r.field(f)
+        np.testing.assert_array_equal(np.isnan(a), np.isnan(b), err_msg=f)
+        ok = np.isfinite(b)
+        if f in ("u", "v"):
+            assert 0 < np.sum(~ok) < ok.size, f
+        if ok.any():
+            assert rel(a[ok], b[ok]) <= 1e-12, f
+
+
+def test_invalid_arguments_are_refused():
+    say("case: invalid arguments at the boundary")
+    g = Domain(WFX_FP64)
+    x = np.zeros(3 * 8)
+    with pytest.raises(WfxError, match="outside"):
+        g.CreateFromMesh(x, np.array([0, 1, 2, 3, 4, 5, 6, 8], np.int32), 8)
+    with pytest.raises(WfxError, match="nodxelem"):
+        g.CreateFromMesh(x, np.arange(6, dtype=np.int32), 6)
+    g.AddBoxLength([0, 0, 0], [0.1, 0.1, 0.1], 0.025)
+    with pytest.raises(WfxError, match="unknown field"):
+        g.field("no_such_field")
+    with pytest.raises(WfxError, match="size mismatch"):
+        g.setField("v", np.zeros(5))
+    with pytest.raises(WfxError, match="elastic constants"):
+        g.setMaterial(0, 200e9, 0.5, 7850.0, 300e6, 1e9, 0.0)
+    with pytest.raises(WfxError, match="tangent modulus"):
+        g.setMaterial(0, 200e9, 0.3, 7850.0, 300e6, 300e9, 0.0)
+    with pytest.raises(WfxError, match="dt must be positive"):
+        g.SetDT(-1.0)
+    with pytest.raises(WfxError, match="node out of range"):
+        g.AddBCVelNode(g.node_count, 2, 0.0)
+    with pytest.raises(WfxError, match="unknown parameter"):
+        g.set("no_such_parameter", 1.0)
+    # the context is still usable after the refusals
+    g.setDensity(7850.0)
+    g.setMaterial(0, 200e9, 0.3, 7850.0, 300e6, 1e9, 0.0)
+    g.SetDT(1e-7)
+    g.init()
+    g.run(3)
+    assert g.nonfiniteCount() == 0
