@@ -1250,29 +1250,36 @@ int wfx_autotune(wfx_ctx *c, int reps, int *chosen, wfx_stream s) {
   WFX_CHECK_HIP(hipEventCreate(&e1));
   static const int sizes[4] = {64, 128, 256, 512};
   hipError_t err = hipSuccess;
-  for (int ph = 1; ph <= 4 && err == hipSuccess; ++ph) {
-    // three rounds over the candidates, the best round of each counts:
-    // interleaving the candidates keeps clock and cache drift out of the choice
-    float best_t[4] = {1e30f, 1e30f, 1e30f, 1e30f};
-    for (int round = 0; round < 3 && err == hipSuccess; ++round)
-      for (int k = 0; k < 4 && err == hipSuccess; ++k) {
-        c->fused_bs[ph] = sizes[k];
-        err = BY_PRECISION(c, launch_fused_phase, c, ph, 1, st, -1);  // warm-up
-        if (err == hipSuccess) err = hipEventRecord(e0, st);
-        for (int r = 0; r < reps && err == hipSuccess; ++r) err = BY_PRECISION(c, launch_fused_phase, c, ph, 1, st, -1);
-        if (err == hipSuccess) err = hipEventRecord(e1, st);
-        if (err == hipSuccess) err = hipEventSynchronize(e1);
-        float ms = 0.f;
-        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-        if (err == hipSuccess && ms < best_t[k]) best_t[k] = ms;
-      }
-    int best_bs = 256;
-    float best = 1e30f;
-    for (int k = 0; k < 4; ++k)
-      if (best_t[k] < best) { best = best_t[k]; best_bs = sizes[k]; }
-    c->fused_bs[ph] = best_bs;
-    if (chosen) chosen[ph - 1] = best_bs;
-  }
+  // whole fused steps (the four passes in order), so that each candidate sees
+  // the caches its predecessor in the step leaves behind
+  auto step = [&]() {
+    for (int ph = 1; ph <= 4 && err == hipSuccess; ++ph) err = BY_PRECISION(c, launch_fused_phase, c, ph, 1, st, -1);
+  };
+  // coordinate descent over the four passes, two sweeps; within a pass three
+  // rounds over the candidates, the best round of each counts: interleaving
+  // the candidates keeps clock and cache drift out of the choice
+  for (int sweep = 0; sweep < 2 && err == hipSuccess; ++sweep)
+    for (int ph = 1; ph <= 4 && err == hipSuccess; ++ph) {
+      float best_t[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+      for (int round = 0; round < 3 && err == hipSuccess; ++round)
+        for (int k = 0; k < 4 && err == hipSuccess; ++k) {
+          c->fused_bs[ph] = sizes[k];
+          step();  // warm-up
+          if (err == hipSuccess) err = hipEventRecord(e0, st);
+          for (int r = 0; r < reps && err == hipSuccess; ++r) step();
+          if (err == hipSuccess) err = hipEventRecord(e1, st);
+          if (err == hipSuccess) err = hipEventSynchronize(e1);
+          float ms = 0.f;
+          if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+          if (err == hipSuccess && ms < best_t[k]) best_t[k] = ms;
+        }
+      int best_bs = 256;
+      float best = 1e30f;
+      for (int k = 0; k < 4; ++k)
+        if (best_t[k] < best) { best = best_t[k]; best_bs = sizes[k]; }
+      c->fused_bs[ph] = best_bs;
+      if (chosen) chosen[ph - 1] = best_bs;
+    }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   for (size_t k = 0; k < state.size(); ++k) {
