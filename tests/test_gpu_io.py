@@ -110,3 +110,21 @@ def test_async_frame_write_error_is_reported(tmp_path):
     with pytest.raises(WfxError, match="cannot write"):
         g.waitVTK()
     g.waitVTK()  # reported once
+
+
+def test_mesh_change_waits_for_frames_in_flight(tmp_path):
+    # the frame writer threads read the host connectivity: a new mesh must
+    # wait for them, and the frames must come out whole
+    print("case: new mesh right after two asynchronous frames", flush=True)
+    g, h = _block(WFX_FP64), _block(WFX_FP64)
+    g.runFused(10)
+    g.writeVTKAsync(tmp_path / "a0.vtk")
+    g.runFused(10)
+    g.writeVTKAsync(tmp_path / "a1.vtk")
+    g.AddBoxLength([0, 0, 0], [0.1, 0.1, 0.1], 0.025)  # replaces the mesh
+    assert g.elem_count == 8
+    g.waitVTK()
+    for k in range(2):
+        h.runFused(10)
+        h.writeVTK(tmp_path / f"s{k}.vtk")
+        assert (tmp_path / f"a{k}.vtk").read_bytes() == (tmp_path / f"s{k}.vtk").read_bytes(), f"frame {k}"

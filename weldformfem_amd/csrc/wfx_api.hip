@@ -352,6 +352,7 @@ int wfx_add_box_length(wfx_ctx *c, const double V[3], const double L[3], double 
   REQUIRE(V && L && r > 0.0, WFX_ERR_ARG, "bad box arguments");
   REQUIRE(L[2] > 0.0, WFX_ERR_ARG, "2D boxes are not supported (L.z must be > 0)");
   REQUIRE(red_int, WFX_ERR_ARG, "only reduced integration is supported");
+  frames_join(c);  // pending frame writers read the host connectivity
   int nodxelem = 0;
   if (box_length_mesh(V, L, r, tritetra != 0, c->x_h, c->elnod_h, nodxelem)) {
     set_error("box mesh: no elements or sizes beyond int32");
@@ -369,6 +370,7 @@ int wfx_set_mesh(wfx_ctx *c, int node_count, int elem_count, int nodxelem, const
   REQUIRE(node_count > 0 && elem_count > 0 && x && elnod, WFX_ERR_ARG, "bad mesh arguments");
   REQUIRE(nodxelem == 4 || nodxelem == 8, WFX_ERR_ARG, "nodxelem must be 4 (tet) or 8 (hexa)");
   REQUIRE((long)elem_count * nodxelem < 2147483647L, WFX_ERR_ARG, "mesh too large for int32 CSR");
+  frames_join(c);  // pending frame writers read the host connectivity
   c->n = nodxelem;
   c->N = node_count;
   c->E = elem_count;
@@ -1240,16 +1242,19 @@ int wfx_autotune(wfx_ctx *c, int reps, int *chosen, wfx_stream s) {
   REQUIRE(reps > 0, WFX_ERR_ARG, "reps must be positive");
   hipStream_t st = as_stream(s);
   auto state = fused_state_buffers(c);
-  std::vector<void *> saved(state.size(), nullptr);
-  for (size_t k = 0; k < state.size(); ++k) {
-    WFX_CHECK_HIP(hipMalloc(&saved[k], state[k].second));
-    WFX_CHECK_HIP(hipMemcpyAsync(saved[k], state[k].first, state[k].second, hipMemcpyDeviceToDevice, st));
-  }
-  hipEvent_t e0, e1;
-  WFX_CHECK_HIP(hipEventCreate(&e0));
-  WFX_CHECK_HIP(hipEventCreate(&e1));
-  static const int sizes[4] = {64, 128, 256, 512};
+  // every failure below still reaches the restore and the frees at the end
   hipError_t err = hipSuccess;
+  std::vector<void *> saved(state.size(), nullptr);
+  for (size_t k = 0; k < state.size() && err == hipSuccess; ++k) {
+    err = hipMalloc(&saved[k], state[k].second);
+    if (err == hipSuccess)
+      err = hipMemcpyAsync(saved[k], state[k].first, state[k].second, hipMemcpyDeviceToDevice, st);
+  }
+  const bool have_copy = err == hipSuccess;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (err == hipSuccess) err = hipEventCreate(&e0);
+  if (err == hipSuccess) err = hipEventCreate(&e1);
+  static const int sizes[4] = {64, 128, 256, 512};
   // whole fused steps (the four passes in order), so that each candidate sees
   // the caches its predecessor in the step leaves behind
   auto step = [&]() {
@@ -1280,14 +1285,21 @@ int wfx_autotune(wfx_ctx *c, int reps, int *chosen, wfx_stream s) {
       c->fused_bs[ph] = best_bs;
       if (chosen) chosen[ph - 1] = best_bs;
     }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  for (size_t k = 0; k < state.size(); ++k) {
-    if (err == hipSuccess)
-      err = hipMemcpyAsync(state[k].first, saved[k], state[k].second, hipMemcpyDeviceToDevice, st);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  // the state is put back whenever the copy was taken, also after a failed
+  // measurement (the launches may have advanced it)
+  if (have_copy) {
+    hipError_t r = hipSuccess;
+    for (size_t k = 0; k < state.size() && r == hipSuccess; ++k)
+      r = hipMemcpyAsync(state[k].first, saved[k], state[k].second, hipMemcpyDeviceToDevice, st);
+    if (r == hipSuccess) r = hipStreamSynchronize(st);
+    if (err == hipSuccess) err = r;
+  } else {
+    (void)hipStreamSynchronize(st);
   }
-  if (err == hipSuccess) err = hipStreamSynchronize(st);
-  for (void *p : saved) (void)hipFree(p);
+  for (void *p : saved)
+    if (p) (void)hipFree(p);
   drop_graph(c);
   WFX_CHECK_HIP(err);
   return WFX_OK;

@@ -58,6 +58,14 @@ static void join_slot(Frames *fr, FrameSlot &s) {
   s.status = 0;
 }
 
+// Waits for the frames in flight (their writer threads read the host
+// connectivity); a failure stays recorded for wfx_vtk_wait.
+void frames_join(wfx_ctx *c) {
+  Frames *fr = c->frames;
+  if (!fr) return;
+  for (auto &s : fr->slot) join_slot(fr, s);
+}
+
 void frames_free(wfx_ctx *c) {
   Frames *fr = c->frames;
   if (!fr) return;

@@ -82,5 +82,6 @@ hipError_t contact_rigid_update(wfx_ctx *c, double dt, hipStream_t s);
 std::vector<std::pair<void *, size_t>> fused_state_buffers(wfx_ctx *c);
 void adjoint_free(wfx_ctx *c);
 void frames_free(wfx_ctx *c);
+void frames_join(wfx_ctx *c);
 
 }  // namespace wfx
