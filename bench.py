@@ -342,6 +342,91 @@ def extensions_leg(torch, blocks, steps, log):
     return out
 
 
+def write_k_file(path, x, elnod):
+    """LS-DYNA keyword file (*NODE fixed 8+16+16+16, *ELEMENT_SOLID 8+8 then
+    8-wide node ids), ids from 1"""
+    N, E = x.shape[0], elnod.shape[0]
+    with open(path, "w") as f:
+        f.write("*KEYWORD\n*NODE\n")
+        np.savetxt(f, np.column_stack([np.arange(1, N + 1), x]), fmt="%8d%16.9e%16.9e%16.9e")
+        f.write("*ELEMENT_SOLID\n")
+        np.savetxt(f, np.column_stack([np.arange(1, E + 1), np.ones(E, np.int64), elnod + 1]),
+                   fmt="%8d" * (2 + elnod.shape[1]))
+        f.write("*END\n")
+
+
+def io_leg(torch, log, nx=100, ny=100, nz=50, steps=200):
+    """Mesh in and frames out on a 100x100x50 hexa compression block (500,000
+    elements): the keyword-file parser and the whole mesh set-up from a file;
+    a synchronous VTK frame; an asynchronous one: how long the caller's stream
+    is held (the device snapshot), the step rate while a host thread writes
+    the file against the rate without a frame, and the wait for the file."""
+    import shutil
+    import tempfile
+
+    from weldformfem_amd import WFX_FP64, Domain, workloads
+    from weldformfem_amd.domain import parse_lsdyna_k
+
+    tmp = tempfile.mkdtemp(prefix="wfx_io_")
+    out = {"workload": f"{nx}x{ny}x{nz} hexa block ({nx * ny * nz} elements), FP64, files in a temporary "
+                       "directory of the host"}
+    try:
+        h = 1.0 / nx
+        x, el = workloads.hexa_box(nx, ny, nz, h)
+        kpath = os.path.join(tmp, "block.k")
+        write_k_file(kpath, x, el)
+        mb = os.path.getsize(kpath) / 1e6
+        t0 = time.perf_counter()
+        xs, els, n = parse_lsdyna_k(kpath)
+        t_parse = time.perf_counter() - t0
+        assert n == 8 and els.shape[0] == el.shape[0] and np.array_equal(els, el)
+        d = Domain(WFX_FP64, stream=torch.cuda.current_stream().cuda_stream)
+        t0 = time.perf_counter()
+        d.readLSDyna(kpath)
+        torch.cuda.synchronize()
+        t_read = time.perf_counter() - t0
+        out["k_file_MB"] = mb
+        out["k_parse_s"] = t_parse
+        out["k_parse_MB_per_s"] = mb / t_parse
+        out["k_parse_elements_per_s"] = el.shape[0] / t_parse
+        out["k_read_and_setup_s"] = t_read
+        log(f"io: .k {mb:.1f} MB parsed in {t_parse:.3f} s, read + device set-up {t_read:.3f} s")
+        bottom, top = workloads.box_layers(nx, ny, nz)
+        workloads.setup_compression(d, bottom, top, -2.0)
+        d.init()
+        d.runFused(20, reaction=False)
+        t_ref = timed(lambda: d.runFused(steps, reaction=False), torch) / steps
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d.writeVTK(os.path.join(tmp, "sync.vtk"))
+        t_sync = time.perf_counter() - t0
+        vtk_mb = os.path.getsize(os.path.join(tmp, "sync.vtk")) / 1e6
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d.writeVTKAsync(os.path.join(tmp, "async.vtk"))
+        t_call = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        t_snap = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        d.runFused(steps, reaction=False)
+        torch.cuda.synchronize()
+        t_during = (time.perf_counter() - t0) / steps
+        t0 = time.perf_counter()
+        d.waitVTK()
+        t_wait = time.perf_counter() - t0
+        out.update({"vtk_file_MB": vtk_mb, "vtk_sync_s": t_sync, "vtk_async_call_s": t_call,
+                    "vtk_async_snapshot_s": t_snap, "ms_per_step_no_frame": t_ref * 1e3,
+                    "ms_per_step_while_writing": t_during * 1e3, "steps_while_writing": steps,
+                    "vtk_async_wait_after_steps_s": t_wait})
+        log(f"io: VTK {vtk_mb:.1f} MB sync {t_sync:.3f} s; async call {t_call * 1e3:.2f} ms, snapshot "
+            f"{t_snap * 1e3:.2f} ms, steps {t_ref * 1e3:.3f} -> {t_during * 1e3:.3f} ms while writing, "
+            f"wait {t_wait:.3f} s")
+        del d
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return out
+
+
 def adjoint_leg(torch, log, steps=2000, cells=30, with_oracle=True):
     """Parameter identification workload: 30^3 hexa compression, reaction
     history loss against a target solved at hidden parameters; adjoint
@@ -470,6 +555,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--no-extensions", action="store_true")
+    ap.add_argument("--no-io", action="store_true")
     ap.add_argument("--bs64", default="", help="fused block sizes for FP64 (f1,f2,f3,f4) instead of autotuning")
     ap.add_argument("--bs32", default="", help="the same for FP32-mixed")
     ap.add_argument("--batch-steps", type=int, default=1000)
@@ -585,6 +671,10 @@ def main():
             del d32
         extensions = extensions_leg(torch, blocks, 100, log)
 
+    io = None
+    if not args.no_io:
+        io = io_leg(torch, log)
+
     batch = None
     if not args.no_batch:
         batch = batch_leg(torch, args.batch_steps, log)
@@ -643,6 +733,7 @@ def main():
         "api_path_fp64": api,
         "other_configs": others,
         "extensions": extensions,
+        "io": io,
         "batched_specimens": batch,
         "adjoint": adjoint,
         "cpu_baseline": cpu,

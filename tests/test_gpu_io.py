@@ -128,3 +128,46 @@ def test_mesh_change_waits_for_frames_in_flight(tmp_path):
         h.runFused(10)
         h.writeVTK(tmp_path / f"s{k}.vtk")
         assert (tmp_path / f"a{k}.vtk").read_bytes() == (tmp_path / f"s{k}.vtk").read_bytes(), f"frame {k}"
+
+
+def expected_vtk(g):
+    """the legacy VTK text of g's state, formatted in Python (one thread)"""
+    N, E, n = g.node_count, g.elem_count, g.nodxelem
+    e9 = lambda v: "%.9e" % v
+    vec3 = lambda q: "".join(f"{e9(a)} {e9(b)} {e9(c)}\n" for a, b, c in q.reshape(-1, 3))
+    out = ["# vtk DataFile Version 3.0\nwfx explicit step\nASCII\nDATASET UNSTRUCTURED_GRID\n",
+           f"POINTS {N} double\n", vec3(g.field("x")), f"CELLS {E} {E * (n + 1)}\n"]
+    out += [f"{n} " + " ".join(str(q) for q in row) + "\n" for row in g.field("elnod").reshape(-1, n)]
+    out += [f"CELL_TYPES {E}\n", ("12\n" if n == 8 else "10\n") * E, f"POINT_DATA {N}\n"]
+    for name, f in [("DISP", "u"), ("Velocity", "v"), ("Acceleration", "a")]:
+        out += [f"VECTORS {name} double\n", vec3(g.field(f))]
+    out += ["SCALARS nod_mass double 1\nLOOKUP_TABLE default\n", "".join(e9(v) + "\n" for v in g.field("mdiag"))]
+    out += [f"CELL_DATA {E}\n"]
+    for name, f in [("pressure", "p"), ("pl_strain", "pl_strain"), ("sigma_y", "sigma_y"), ("Vol", "vol")]:
+        out += [f"SCALARS {name} double 1\nLOOKUP_TABLE default\n", "".join(e9(v) + "\n" for v in g.field(f))]
+    out += ["TENSORS SIGMA double\n"]
+    for s in g.field("sigma").reshape(-1, 6):  # xx yy zz xy yz xz
+        out += [vec3(np.array([s[0], s[3], s[5], s[3], s[1], s[4], s[5], s[4], s[2]]))]
+    return "".join(out).encode()
+
+
+@pytest.mark.parametrize("sync", [True, False])
+def test_frame_text_of_a_larger_block(tmp_path, sync):
+    # above the writer's one-thread size: the sections are formatted by several
+    # threads and must come out as one thread writes them
+    print(f"case: 20^3 block frame (sync={sync}) against a Python rendering of the fields", flush=True)
+    ref = RefDomain()
+    ref.AddBoxLength([0, 0, 0], [1.0, 1.0, 1.0], 0.025)
+    g = Domain(WFX_FP64)
+    g.CreateFromMesh(ref.field("x"), ref.field("elnod"), 8)
+    bottom, top = workloads.box_layers(20, 20, 20)
+    workloads.setup_compression(g, bottom, top, -5.0, hg_coeff=0.06)
+    g.init()
+    g.runFused(15)
+    p = tmp_path / "f.vtk"
+    if sync:
+        g.writeVTK(p)
+    else:
+        g.writeVTKAsync(p)
+        g.waitVTK()
+    assert p.read_bytes() == expected_vtk(g)
