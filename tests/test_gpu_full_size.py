@@ -1,5 +1,7 @@
-"""The 200x200x100 hexa compression block (4,000,000 elements, 4,080,501
-nodes, sigma_y0 ~ LogNormal(ln 300 MPa, 0.1)) at its own size.
+"""The benchmark workloads at their own sizes.
+
+The 200x200x100 hexa compression block (4,000,000 elements, 4,080,501
+nodes, sigma_y0 ~ LogNormal(ln 300 MPa, 0.1)):
 
 * Mesh, connectivity and CSR bit for bit against the oracle, and the first
   10 FP64 fused steps within 1e-12 of it (the oracle steps the full block in
@@ -18,6 +20,12 @@ differences in strain the two orders of summation leave become ~4e-15 in
 dgamma per step, ~4e-12 after a thousand plastic steps: ~4e-10 of the
 largest plastic strain (0.9%).  The pressure K (1 - J) amplifies the same
 way.
+
+The 256-specimen batch (2,911,675 elements): 1000 FP64 fused steps, four
+specimens (the smallest, the largest and two others) equal to the same
+specimen run alone, bit for bit; the smallest and largest within 1e-10 of
+the oracle (plastic strain 1e-9); FP32-mixed, one captured graph of 1000
+steps equal to eager launches, bit for bit.
 """
 import numpy as np
 import pytest
@@ -122,3 +130,66 @@ def test_fp32_mixed_against_fp64(fused_run):
     eu, er = rel(m.field("u"), ref["u"]), rel(hm, hf)
     say(f"  FP32-mixed rel err: u {eu:.2e} reaction {er:.2e}")
     assert eu <= 1e-4 and er <= 1e-4
+
+
+# ---- 256 hexa compression specimens (edges U{6..40} per axis, seed 3,
+# 2,911,675 elements) packed into one domain, 1000 steps ---------------------
+BATCH_STEPS = 1000
+
+
+def specimen_single(b, s, precision):
+    one = Domain(precision)
+    b.setup_single(one, s)
+    one.init()
+    return one
+
+
+def test_specimen_batch_fp64_against_single_runs_and_oracle():
+    say(f"case: 256-specimen batch FP64, {BATCH_STEPS} fused steps, specimens vs single runs and oracle")
+    b = workloads.SpecimenBatch()
+    g = Domain(WFX_FP64)
+    b.setup(g)
+    g.init()
+    assert g.elem_count == 2911675 and g.reactionGroupCount() == 256
+    hist = g.runFused(BATCH_STEPS)
+    u, pl = g.field("u").reshape(-1, 3), g.field("pl_strain")
+    sizes = np.diff(b.elem_off)
+    picks = [int(np.argmin(sizes)), int(np.argmax(sizes)), 100, 200]
+    for s in picks:
+        n0, n1, e0, e1 = b.node_off[s], b.node_off[s + 1], b.elem_off[s], b.elem_off[s + 1]
+        one = specimen_single(b, s, WFX_FP64)
+        h1 = one.runFused(BATCH_STEPS)
+        np.testing.assert_array_equal(hist[:, s], h1)  # the specimens never mix
+        np.testing.assert_array_equal(u[n0:n1].reshape(-1), one.field("u"))
+        np.testing.assert_array_equal(pl[e0:e1], one.field("pl_strain"))
+        if s in picks[:2]:  # smallest and largest against the oracle
+            r = RefDomain()
+            b.setup_single(r, s)
+            r.init()
+            rr = r.run(BATCH_STEPS)
+            eu, er = rel(u[n0:n1].reshape(-1), r.field("u")), rel(hist[:, s], rr)
+            ep = rel(pl[e0:e1], r.field("pl_strain"))
+            say(f"  specimen {s} edges {tuple(b.edges[s])}: u {eu:.2e} reaction {er:.2e} pl {ep:.2e}")
+            assert np.max(r.field("pl_strain")) > 0
+            assert eu <= 1e-10 and er <= 1e-10 and ep <= 1e-9
+    assert g.nonfiniteCount() == 0
+
+
+def test_specimen_batch_graph_equals_eager_fp32_mixed():
+    import torch
+
+    say(f"case: 256-specimen batch FP32-mixed, one graph of {BATCH_STEPS} steps vs eager")
+    b = workloads.SpecimenBatch()
+    stream = torch.cuda.Stream()
+    g = Domain(WFX_FP32_MIXED, stream=stream.cuda_stream)
+    e = Domain(WFX_FP32_MIXED)
+    for d in (g, e):
+        b.setup(d)
+        d.init()
+    g.graphCapture(BATCH_STEPS)
+    hg = g.graphLaunch()
+    torch.cuda.synchronize()
+    he = e.runFused(BATCH_STEPS)
+    np.testing.assert_array_equal(hg, he)
+    np.testing.assert_array_equal(g.field("u"), e.field("u"))
+    assert np.all(np.isfinite(hg))
