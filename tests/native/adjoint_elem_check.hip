@@ -3,7 +3,8 @@
 // by tests/test_adjoint_math.py; runs on the CPU.
 //
 //   adjoint_elem_check forward   < elements.txt   element forward, one line per element
-//   adjoint_elem_check vjp NN SEED COUNT PLASTIC  reverse vs central differences
+//   adjoint_elem_check vjp NN SEED COUNT PLASTIC [LAW]  reverse vs central
+//                                    differences (LAW 0 bilinear, 1 Hollomon)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -29,6 +30,9 @@ static int forward_mode() {
   ElemConst k;
   int M;
   if (scanf("%lf %lf %lf %lf %lf %d %d", &k.dt, &k.G, &k.K, &k.cs0, &k.hg, &k.hourglass, &M) != 7) return 2;
+  // the law: 0 (bilinear) or 1 Kh mh eps0 eps1 sy0m (Hollomon)
+  if (scanf("%d", &k.law) != 1) return 2;
+  if (k.law == 1 && scanf("%lf %lf %lf %lf %lf", &k.Kh, &k.mh, &k.eps0, &k.eps1, &k.sy0m) != 5) return 2;
   for (int e = 0; e < M; ++e) {
     ElemIn<NN> in;
     for (int a = 0; a < NN; ++a)
@@ -63,11 +67,18 @@ static double phi(const ElemConst &k, const ElemIn<NN> &in, const double fb[NN][
 }
 
 template <int NN>
-static int vjp_mode(int count, int want_plastic) {
+static int vjp_mode(int count, int want_plastic, int law) {
   double worst = 0.0;
   int used = 0;
   for (int t = 0; t < count * 20 && used < count; ++t) {
     ElemConst k{1.0e-6, 76.9e9, 166.7e9, 4600.0, 0.06, NN == 8};
+    if (law == 1) {  // power branch throughout (eps1 below every s)
+      k.law = 1;
+      k.mh = 0.1 + 0.3 * urand();
+      k.eps0 = 1.5e-3;
+      k.eps1 = 1.0e-4;
+      k.sy0m = 2.0e8;
+    }
     ElemIn<NN> in;
     const double h = 0.02;
     for (int a = 0; a < NN; ++a) {
@@ -92,6 +103,8 @@ static int vjp_mode(int count, int want_plastic) {
       elem_forward<NN>(k, probe, F);
       in.sy0 = (want_plastic ? 0.6 : 1.6) * F.st - in.H * in.ep;
       if (in.sy0 <= 0.0) continue;
+      // Hollomon: Kh with Kh s^mh the same fraction of the trial stress
+      if (law == 1) k.Kh = (want_plastic ? 0.6 : 1.6) * F.st / pow(in.ep + k.eps0, k.mh);
     }
     double fb[NN][3], tb[6], epb = 1.0e3 * sym();
     for (int a = 0; a < NN; ++a)
@@ -107,7 +120,7 @@ static int vjp_mode(int count, int want_plastic) {
     // directions: all inputs at once, then every parameter alone
     for (int dir = 0; dir < 6; ++dir) {
       ElemIn<NN> d{};
-      double dhg = 0.0;
+      double dhg = 0.0, dK = 0.0, dm = 0.0;
       if (dir < 3) {
         for (int a = 0; a < NN; ++a) {
           for (int c = 0; c < 3; ++c) {
@@ -119,21 +132,25 @@ static int vjp_mode(int count, int want_plastic) {
         for (int q = 0; q < 6; ++q) d.tau[q] = (dir == 2) ? 1.0e8 * sym() : 0.0;
         d.ep = (dir == 2) ? 0.01 * sym() : 0.0;
       } else if (dir == 3) {
-        d.sy0 = 1.0e8;
+        if (law == 1) dK = 1.0e-3 * k.Kh;
+        else d.sy0 = 1.0e8;
       } else if (dir == 4) {
-        d.H = 1.0e9;
+        if (law == 1) dm = 0.01;
+        else d.H = 1.0e9;
       } else {
         dhg = 0.05;
       }
-      double ad = B.ep * d.ep + B.sy0 * d.sy0 + B.H * d.H + B.hg * dhg;
+      double ad = B.ep * d.ep + B.sy0 * d.sy0 + B.H * d.H + B.Kh * dK + B.mh * dm + B.hg * dhg;
       for (int a = 0; a < NN; ++a) {
         for (int c = 0; c < 3; ++c) ad += (B.X[a][c] + Xv[a][c]) * d.X[a][c] + B.V[a][c] * d.V[a][c];
         ad += B.r[a] * d.r[a];
       }
       for (int q = 0; q < 6; ++q) ad += B.tau[q] * d.tau[q];
       // central differences: truncation ~eps^2 and rounding ~1/eps balance
-      // near 1e-4 for these input scales (error 2e-8 there)
-      const double eps = 1.0e-4;
+      // near 1e-4 for these input scales (error 2e-8 there); the Hollomon
+      // law's s^(mh-2) curvature at small plastic strain (s = ep + 1.5e-3)
+      // needs the smaller step 3e-5 for the same balance
+      const double eps = law == 1 ? 3.0e-5 : 1.0e-4;
       auto shifted = [&](double s, int *pl) {
         ElemIn<NN> p = in;
         ElemConst kk = k;
@@ -149,6 +166,8 @@ static int vjp_mode(int count, int want_plastic) {
         p.sy0 += s * d.sy0;
         p.H += s * d.H;
         kk.hg += s * dhg;
+        kk.Kh += s * dK;
+        kk.mh += s * dm;
         return phi<NN>(kk, p, fb, tb, epb, pl);
       };
       int p1 = 0, p2 = 0;
@@ -172,9 +191,9 @@ int main(int argc, char **argv) {
   if (argc >= 6 && !strcmp(argv[1], "vjp")) {
     int nn = atoi(argv[2]);
     g_state = strtoull(argv[3], nullptr, 10);
-    int count = atoi(argv[4]), pl = atoi(argv[5]);
-    return nn == 8 ? vjp_mode<8>(count, pl) : vjp_mode<4>(count, pl);
+    int count = atoi(argv[4]), pl = atoi(argv[5]), law = argc >= 7 ? atoi(argv[6]) : 0;
+    return nn == 8 ? vjp_mode<8>(count, pl, law) : vjp_mode<4>(count, pl, law);
   }
-  fprintf(stderr, "usage: forward NN < input | vjp NN SEED COUNT PLASTIC\n");
+  fprintf(stderr, "usage: forward NN < input | vjp NN SEED COUNT PLASTIC [LAW]\n");
   return 1;
 }

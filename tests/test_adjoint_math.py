@@ -33,10 +33,12 @@ def checker(tmp_path_factory):
     return out
 
 
+@pytest.mark.parametrize("law", [0, 1])  # bilinear, Hollomon
 @pytest.mark.parametrize("nn", [8, 4])
 @pytest.mark.parametrize("plastic", [0, 1])
-def test_element_reverse_matches_central_differences(checker, nn, plastic):
-    r = subprocess.run([checker, "vjp", str(nn), "11", "25", str(plastic)], capture_output=True, text=True)
+def test_element_reverse_matches_central_differences(checker, nn, plastic, law):
+    r = subprocess.run([checker, "vjp", str(nn), "11", "25", str(plastic), str(law)], capture_output=True,
+                       text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     used, worst = r.stdout.split()
     assert int(used) == 25
@@ -55,8 +57,9 @@ def _mesh(tet, seed):
     return x, el, n
 
 
+@pytest.mark.parametrize("law", ["bilinear", "hollomon"])
 @pytest.mark.parametrize("tet", [False, True])
-def test_element_forward_matches_oracle(checker, tet):
+def test_element_forward_matches_oracle(checker, tet, law):
     x, el, n = _mesh(tet, seed=5)
     N, E = x.shape[0], el.shape[0]
     r = RefDomain()
@@ -64,7 +67,10 @@ def test_element_forward_matches_oracle(checker, tet):
     K_E, nu, rho = 200e9, 0.3, 7850.0
     H_t = 1.0e9
     r.setDensity(rho)
-    r.setMaterial(0, K_E, nu, rho, 300e6, H_t, 0.0)
+    if law == "bilinear":
+        r.setMaterial(0, K_E, nu, rho, 300e6, H_t, 0.0)
+    else:  # plateau below eps1 = (300e6/1e9)^(1/0.2) = 2.4e-3: some elements on each side
+        r.setMaterial(1, K_E, nu, rho, 300e6, 1.0e9, 0.2)
     r.set("hg_coeff", 0.06)
     dt = 2.0e-6
     r.SetDT(dt)
@@ -98,6 +104,11 @@ def test_element_forward_matches_oracle(checker, tet):
     cs0 = np.sqrt(K / rho)
     H = K_E * H_t / (K_E - H_t)
     lines = [" ".join(repr(float(q)) for q in (dt, G, K, cs0, 0.06)) + f" {int(n == 8)} {E}"]
+    if law == "bilinear":
+        lines.append("0")
+    else:
+        lines.append("1 " + " ".join(repr(float(q)) for q in (1.0e9, 0.2, 300e6 / K_E, (300e6 / 1.0e9) ** 5.0,
+                                                               300e6)))
     v3 = v.reshape(-1, 3)
     for e in range(E):
         vals = list(x[el[e]].reshape(-1)) + list(v3[el[e]].reshape(-1)) + list(ratio[el[e]])

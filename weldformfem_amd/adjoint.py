@@ -4,15 +4,17 @@ The gradient comes from the library's hand-written adjoint
 (wfx_adjoint_forward / wfx_adjoint_backward: checkpoint every K steps,
 recompute each segment, reverse step kernels).  ``ReactionHistory`` exposes
 the forward solve as a torch.autograd.Function of the parameter vector
-theta = (sigma_y0, Et, hg_coeff), so a loss written in torch differentiates
-through the whole explicit solve.
+theta = (sigma_y0, Et, hg_coeff) for the bilinear law or (K, m, hg_coeff)
+for the Hollomon law (its plateau stress sigma_y0 held fixed), so a loss
+written in torch differentiates through the whole explicit solve.
 """
 import numpy as np
 
 from . import workloads
-from .domain import WFX_FP64, Domain
+from .domain import WFX_BILINEAR, WFX_FP64, WFX_HOLLOMON, Domain
 
 PARAMS = ("sy0", "Et", "hg_coeff")
+LAW_PARAMS = {"bilinear": PARAMS, "hollomon": ("K", "m", "hg_coeff")}
 
 
 class CompressionProblem:
@@ -20,10 +22,14 @@ class CompressionProblem:
     down: by default a (cells)^3 reduced-integration hexa cube of edge
     `size` (or any mesh given as (x, elnod, nodxelem, bottom, top)), nsteps
     steps at dt = 0.8 x the critical step.  The reaction force of the top
-    face is the observed history."""
+    face is the observed history.  law "bilinear": theta = (sigma_y0, Et,
+    hg_coeff); "hollomon": theta = (K, m, hg_coeff) with sigma_y0 from mat."""
 
     def __init__(self, cells=30, size=1.0, v_top=-1.0, nsteps=2000, every=50, precision=WFX_FP64,
-                 mat=workloads.STEEL, stream=0, mesh=None):
+                 mat=workloads.STEEL, stream=0, mesh=None, law="bilinear"):
+        if law not in LAW_PARAMS:
+            raise ValueError(f"law must be one of {sorted(LAW_PARAMS)}")
+        self.law, self.params = law, LAW_PARAMS[law]
         d = Domain(precision, stream=stream)
         if mesh is None:
             d.AddBoxLength([0.0, 0.0, 0.0], [size, size, size], 0.5 * size / cells)
@@ -37,13 +43,16 @@ class CompressionProblem:
         self.theta = None
 
     def set_params(self, theta):
-        sy0, Et, hg = (float(t) for t in theta)
+        a, b, hg = (float(t) for t in theta)
         m = self.mat
         d = self.domain
-        d.setMaterial(0, m["E"], m["nu"], m["rho"], sy0, Et, 0.0)
+        if self.law == "hollomon":
+            d.setMaterial(WFX_HOLLOMON, m["E"], m["nu"], m["rho"], m["sy0"], a, b)
+        else:
+            d.setMaterial(WFX_BILINEAR, m["E"], m["nu"], m["rho"], a, b, 0.0)
         d.set("hg_coeff", hg)
         d.SetDT(self.dt)
-        self.theta = (sy0, Et, hg)
+        self.theta = (a, b, hg)
 
     def forward(self, theta):
         """reaction history for parameters theta, recorded for backward()"""
@@ -56,7 +65,7 @@ class CompressionProblem:
     def backward(self, hist_bar):
         """dL/dtheta for dL/dR = hist_bar of the last forward()"""
         g = self.domain.adjointBackward(hist_bar)
-        return np.array([g[k] for k in PARAMS])
+        return np.array([g[k] for k in self.params])
 
     def loss_and_grad(self, theta, target):
         """L = sum_k (R_k - R*_k)^2 and its gradient"""

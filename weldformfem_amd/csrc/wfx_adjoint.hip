@@ -221,8 +221,9 @@ __global__ __launch_bounds__(kB) void a_elem1(int E, int N, adj::ElemConst K, co
     contrib[(a * 7 + 6) * Es + e] = B.r[a];
   }
   volb[e] = B.volb;
-  grad[e] += B.sy0;
-  grad[Es + e] += B.H;
+  // rows: the law's two parameters (bilinear sy0, H; Hollomon Kh, mh), hg
+  grad[e] += K.law == 1 ? B.Kh : B.sy0;
+  grad[Es + e] += K.law == 1 ? B.mh : B.H;
   grad[2 * Es + e] += B.hg;
 }
 
@@ -277,18 +278,21 @@ __global__ void a_node3(int N, int E, const int *off, const int *cnt, const int 
   }
 }
 
-// parameter gradients: one work-group per parameter row, fixed order;
-// the tangent-modulus row is converted from H to Et on the way
-// (H = E Et/(E - Et), dH/dEt = (E + H)^2 / E^2)
-__global__ void a_reduce(int E, double Emod, const double *grad, const double *Hd, double *gEt, double *out) {
+// parameter gradients: one work-group per parameter row, fixed order; for
+// the bilinear law the tangent-modulus row is converted from H to Et on the
+// way (H = E Et/(E - Et), dH/dEt = (E + H)^2 / E^2)
+__global__ void a_reduce(int E, double Emod, int bilinear, const double *grad, const double *Hd, double *gEt,
+                         double *out) {
   __shared__ double sh[1024];
   const int row = blockIdx.x;
   double s = 0.0;
   for (int e = threadIdx.x; e < E; e += blockDim.x) {
     double g = grad[(long)row * E + e];
     if (row == 1) {
-      const double f = (Emod + Hd[e]) / Emod;
-      g *= f * f;
+      if (bilinear) {
+        const double f = (Emod + Hd[e]) / Emod;
+        g *= f * f;
+      }
       gEt[e] = g;
     }
     s += g;
@@ -339,7 +343,8 @@ template <> DevElem<float> &el_of<float>(wfx_ctx *c) { return c->e32; }
 
 static int check_supported(wfx_ctx *c) {
   const Stab &s = c->prm.stab;
-  REQUIRE(c->prm.mat.model == WFX_BILINEAR, WFX_ERR_STATE, "the adjoint supports the bilinear material");
+  REQUIRE(c->prm.mat.model == WFX_BILINEAR || c->prm.mat.model == WFX_HOLLOMON, WFX_ERR_STATE,
+          "the adjoint supports the bilinear and Hollomon materials");
   REQUIRE(!(c->prm.cfl > 0.0), WFX_ERR_STATE, "the adjoint needs a fixed time step (wfx_set_cfl(ctx, 0))");
   REQUIRE(!c->thermal.on, WFX_ERR_STATE, "the adjoint does not cover thermal coupling");
   REQUIRE(!c->contact.on, WFX_ERR_STATE, "the adjoint does not cover contact");
@@ -504,6 +509,14 @@ static int adjoint_backward(wfx_ctx *c, hipStream_t st) {
                      c->ms.nodel_pk, A->rhod, A->rho_avg);
   LAUNCH(hipGetLastError());
   adj::ElemConst K{P.dt, P.mat.G, P.mat.K, P.mat.cs0, P.stab.hg_coeff, n == 8};
+  if (P.mat.model == WFX_HOLLOMON) {
+    K.law = 1;
+    K.Kh = P.mat.Kh;
+    K.mh = P.mat.mh;
+    K.eps0 = P.mat.eps0;
+    K.eps1 = P.mat.eps1;
+    K.sy0m = P.mat.sy0;
+  }
   NodeAdj NP{P.dt, P.alpha, P.beta, P.gamma, c->has_fe, A->ng};
   for (int seg = A->nck - 1; seg >= 0; --seg) {
     const int k0 = seg * A->every, k1 = std::min(A->nsteps, k0 + A->every);
@@ -538,7 +551,8 @@ static int adjoint_backward(wfx_ctx *c, hipStream_t st) {
     WFX_CHECK_HIP(hipMemcpyAsync(A->final_bufs[k].first, A->final_saved[k], A->final_bufs[k].second,
                                  hipMemcpyDeviceToDevice, st));
   LAUNCH(launch_copy(c->nd.prev_a, c->nd.a, 3L * c->N, st));
-  hipLaunchKernelGGL(a_reduce, dim3(3), dim3(1024), 0, st, E, P.mat.E, A->grad, A->Hd, A->sums + 8, A->sums);
+  hipLaunchKernelGGL(a_reduce, dim3(3), dim3(1024), 0, st, E, P.mat.E, P.mat.model == WFX_BILINEAR ? 1 : 0,
+                     A->grad, A->Hd, A->sums + 8, A->sums);
   LAUNCH(hipGetLastError());
   return WFX_OK;
 }

@@ -23,7 +23,8 @@
 //             R_yz = txy Wxz + tyy Wyz - Wxy txz + Wyz tzz,
 //             R_xz = txx Wxz + txy Wyz + Wxy tyz + Wxz tzz
 //   s       = dev(tau1 - p I), st = sqrt(3 J2(s))      (Mechanical.C:1710-1721)
-//   sy      = sy0 + H ep (bilinear)
+//   sy, H   = sy0 + H ep, H (bilinear); Kh s^mh, Kh mh s^(mh-1) with s = ep + eps0
+//             if s > eps1, else sy0 and 0 (Hollomon, Material.cuh:357-360, :391-392)
 //   plastic (sy < st): dg = (st - sy)/(3G + H), tau2 = (1 - 3G dg/st) s, ep2 = ep + dg
 //   elastic:           tau2 = tau1, ep2 = ep           (Mechanical.C:1723-1762)
 //   sigma   = tau2 - p I                               (:1766)
@@ -46,6 +47,11 @@ namespace adj {
 struct ElemConst {
   double dt, G, K, cs0, hg;
   int hourglass;  // hexa viscous hourglass on
+  // hardening law: 0 bilinear (per-element sy0 and H of ElemIn), 1 Hollomon
+  // sigma_y = Kh (ep + eps0)^mh above eps1, the plateau sy0m below
+  // (Material.cuh:357-360, :391-392)
+  int law = 0;
+  double Kh = 0.0, mh = 0.0, eps0 = 0.0, eps1 = 0.0, sy0m = 0.0;
 };
 
 template <int NN>
@@ -64,7 +70,9 @@ struct ElemFwd {
   double f, D[6], W[3];
   // stress
   double p, t1[6], s[6], st, sy, dg, fac, t2[6], ep2, sig[6];
-  int plastic;
+  double Ht;  // tangent modulus of the return (bilinear H; Hollomon Kh mh s^(mh-1) or 0)
+  double hs;  // Hollomon: s = ep + eps0
+  int plastic, hpow;  // hpow: Hollomon power branch (s > eps1)
   // forces
   double hm[3][4], ch0, ch, fnet[NN][3];
 };
@@ -160,10 +168,20 @@ WFX_HD void elem_forward(const ElemConst &k, const ElemIn<NN> &in, ElemFwd<NN> &
   const double J2 = 0.5 * (F.s[0] * F.s[0] + F.s[1] * F.s[1] + F.s[2] * F.s[2]) +
                     (F.s[3] * F.s[3] + F.s[4] * F.s[4] + F.s[5] * F.s[5]);
   F.st = sqrt(3.0 * J2);
-  F.sy = in.sy0 + in.H * in.ep;
+  if (k.law == 1) {
+    F.hs = in.ep + k.eps0;
+    F.hpow = F.hs > k.eps1;
+    F.sy = F.hpow ? k.Kh * pow(F.hs, k.mh) : k.sy0m;
+    F.Ht = F.hpow ? k.Kh * k.mh * pow(F.hs, k.mh - 1.0) : 0.0;
+  } else {
+    F.hs = 0.0;
+    F.hpow = 0;
+    F.sy = in.sy0 + in.H * in.ep;
+    F.Ht = in.H;
+  }
   F.plastic = F.sy < F.st;
   if (F.plastic) {
-    F.dg = (F.st - F.sy) / (3.0 * k.G + in.H);
+    F.dg = (F.st - F.sy) / (3.0 * k.G + F.Ht);
     F.fac = 1.0 - (3.0 * k.G * F.dg) / F.st;
     for (int q = 0; q < 6; ++q) F.t2[q] = F.fac * F.s[q];
     F.ep2 = in.ep + F.dg;
@@ -208,10 +226,10 @@ WFX_HD void elem_forward(const ElemConst &k, const ElemIn<NN> &in, ElemFwd<NN> &
 // strain (epb).  Outputs: cotangents of X, V, the nodal ratios r, tau, ep,
 // the element volume through the hourglass coefficient (volb, the volume
 // paths through the nodal volumes are added by the caller) and of the
-// parameters sy0, H and hg.
+// parameters: sy0 and H (bilinear) or Kh and mh (Hollomon), and hg.
 template <int NN>
 struct ElemBar {
-  double X[NN][3], V[NN][3], r[NN], tau[6], ep, volb, sy0, H, hg;
+  double X[NN][3], V[NN][3], r[NN], tau[6], ep, volb, sy0, H, Kh, mh, hg;
 };
 
 template <int NN>
@@ -274,8 +292,10 @@ WFX_HD void elem_reverse(const ElemConst &k, const ElemIn<NN> &in, const ElemFwd
   B.ep = epb;
   B.sy0 = 0.0;
   B.H = 0.0;
+  B.Kh = 0.0;
+  B.mh = 0.0;
   if (F.plastic) {
-    const double G3 = 3.0 * k.G, den = G3 + in.H;
+    const double G3 = 3.0 * k.G, den = G3 + F.Ht;
     double sb[6], facb = 0.0;
     for (int q = 0; q < 6; ++q) {
       sb[q] = F.fac * t2b[q];
@@ -284,9 +304,19 @@ WFX_HD void elem_reverse(const ElemConst &k, const ElemIn<NN> &in, const ElemFwd
     const double dgb = epb - G3 / F.st * facb;
     double stb = G3 * F.dg / (F.st * F.st) * facb + dgb / den;
     const double syb = -dgb / den;
-    B.H = -dgb * F.dg / den + syb * in.ep;
-    B.sy0 = syb;
-    B.ep += syb * in.H;
+    const double Htb = -dgb * F.dg / den;
+    if (k.law == 1) {
+      if (F.hpow) {  // sy = Kh s^mh, Ht = Kh mh s^(mh-1), s = ep + eps0
+        const double pm1 = pow(F.hs, k.mh - 1.0), lns = log(F.hs);
+        B.ep += syb * F.Ht + Htb * k.Kh * k.mh * (k.mh - 1.0) * pm1 / F.hs;
+        B.Kh = syb * pm1 * F.hs + Htb * k.mh * pm1;
+        B.mh = syb * F.sy * lns + Htb * k.Kh * pm1 * (1.0 + k.mh * lns);
+      }  // plateau: sy and Ht do not depend on ep, Kh or mh
+    } else {
+      B.H = Htb + syb * in.ep;
+      B.sy0 = syb;
+      B.ep += syb * in.H;
+    }
     // st = sqrt(3 J2)
     const double J2b = 1.5 * stb / F.st;
     for (int q = 0; q < 3; ++q) sb[q] += J2b * F.s[q];

@@ -30,6 +30,7 @@ class Domain:
         self._h = h
         self.precision = precision
         self.stream = ctypes.c_void_p(stream or None)
+        self.law = WFX_BILINEAR  # the hardening law last set
 
     def close(self):
         if getattr(self, "_h", None):
@@ -92,12 +93,15 @@ class Domain:
         check(self._L.wfx_set_density(self._h, float(rho)))
 
     def setMaterial(self, model, E, nu, rho0, sy0, p1=0.0, p2=0.0):
+        """bilinear (p1 = Et) or Hollomon (p1 = K, p2 = m)"""
         check(self._L.wfx_set_material(self._h, int(model), E, nu, rho0, sy0, p1, p2))
+        self.law = int(model)
 
     def setMaterialJC(self, E, nu, rho0, A, B, n, C, eps_0, m, T_m, T_t):
         """Johnson-Cook law (Material.cuh:191-203, :378-411); A is the
         per-element initial yield"""
         check(self._L.wfx_set_material_jc(self._h, E, nu, rho0, A, B, n, C, eps_0, m, T_m, T_t))
+        self.law = WFX_JOHNSON_COOK
 
     def setMaterialGMT(self, E, nu, rho0, sy0, c, ranges):
         """GMT law (Material.cuh:271-291, :412-473); c = (n1, n2, C1, C2, m1,
@@ -107,6 +111,7 @@ class Domain:
         if ca.size != 8 or ra.size != 6:
             raise ValueError("GMT needs 8 constants and 6 range bounds")
         check(self._L.wfx_set_material_gmt(self._h, E, nu, rho0, sy0, ca.ctypes.data, ra.ctypes.data))
+        self.law = WFX_GMT
 
     def setTemperature(self, T):
         """uniform temperature (Domain_d::setTemp), or one value per element"""
@@ -322,8 +327,9 @@ class Domain:
         return self._hist(nsteps, hist)
 
     def adjointBackward(self, hist_bar, per_element=False):
-        """dL/d(sigma_y0, Et, hg_coeff) for the cotangent hist_bar = dL/dR of
-        the recorded history (shape as adjointForward returned)"""
+        """dL/d(sigma_y0, Et, hg_coeff) (bilinear law) or dL/d(K, m,
+        hg_coeff) (Hollomon) for the cotangent hist_bar = dL/dR of the
+        recorded history (shape as adjointForward returned)"""
         hb = np.ascontiguousarray(hist_bar, dtype=np.float64).reshape(-1)
         ng = max(self.reactionGroupCount(), 1)
         if hb.size != self._adj_steps * ng:
@@ -334,9 +340,10 @@ class Domain:
         check(self._L.wfx_adjoint_backward(self._h, hb.ctypes.data, g.ctypes.data,
                                            *[(a.ctypes.data if a is not None else None) for a in el],
                                            self.stream))
-        out = {"sy0": g[0], "Et": g[1], "hg_coeff": g[2]}
+        names = ("K", "m", "hg_coeff") if self.law == WFX_HOLLOMON else ("sy0", "Et", "hg_coeff")
+        out = dict(zip(names, g))
         if per_element:
-            out.update({"sy0_elem": el[0], "Et_elem": el[1], "hg_coeff_elem": el[2]})
+            out.update({f"{k}_elem": a for k, a in zip(names, el)})
         return out
 
     def adjointRelease(self):
