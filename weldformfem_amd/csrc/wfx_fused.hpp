@@ -491,7 +491,7 @@ __device__ __forceinline__ void element_chain(
   // bilinear plastic modulus: per element when given (specimen batches)
   T sy, Hb = (T)mp.H;
   const bool per_elem = LAW == LAW_PLAIN && mp.model == BILINEAR && He;
-  const double edot = (LAW == LAW_RATE) ? eff_strain_rate(D) : 0.0;
+  const T edot = (LAW == LAW_RATE) ? eff_strain_rate(D) : T(0);
   RateTemp rt = mp.ut;  // the rate laws' temperature factors (law_temperature on the loaded nodes)
   if constexpr (LAW == LAW_RATE) {
     if (!mp.t_uniform) {
@@ -499,7 +499,7 @@ __device__ __forceinline__ void element_chain(
       rt = rate_temp_at(mp, temp);
     }
   }
-  RateCache rc;
+  RateCache<T> rc;
   if (per_elem) {
     Hb = ld_stream(&He[e]);
     sy = ld_stream(&sy0[e]) + Hb * ep;

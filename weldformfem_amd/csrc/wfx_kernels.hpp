@@ -639,17 +639,33 @@ __device__ __forceinline__ double law_temperature(const MatParams &m, int e) {
   return m.Te ? m.Te[e] : m.temp;
 }
 
+// The hardening laws evaluate in the element precision T: FP64 as the
+// reference does, FP32-mixed in float (its element arithmetic is float).
+__device__ __forceinline__ double rpow(double a, double b) { return pow(a, b); }
+__device__ __forceinline__ float rpow(float a, float b) { return powf(a, b); }
+__device__ __forceinline__ double rlog(double a) { return log(a); }
+__device__ __forceinline__ float rlog(float a) { return logf(a); }
+__device__ __forceinline__ double rexp(double a) { return exp(a); }
+__device__ __forceinline__ float rexp(float a) { return expf(a); }
+template <typename T>
+__device__ __forceinline__ T rclamp(T v, double lo, double hi) {
+  if (v < (T)lo) return (T)lo;
+  if (v > (T)hi) return (T)hi;
+  return v;
+}
+
 // effective strain rate of CalcStressStrain (Mechanical.C:1700-1705)
 template <typename T>
-__device__ __forceinline__ double eff_strain_rate(const T3<T> &D) {
-  const double xx = D.xx, yy = D.yy, zz = D.zz, xy = D.xy, yz = D.yz, zx = D.xz;
-  return sqrt(0.5 * ((xx - yy) * (xx - yy) + (yy - zz) * (yy - zz) + (zz - xx) * (zz - xx)) +
-              3.0 * (xy * xy + yz * yz + zx * zx));
+__device__ __forceinline__ T eff_strain_rate(const T3<T> &D) {
+  const T xx = D.xx, yy = D.yy, zz = D.zz, xy = D.xy, yz = D.yz, zx = D.xz;
+  return sqrt(T(0.5) * ((xx - yy) * (xx - yy) + (yy - zz) * (yy - zz) + (zz - xx) * (zz - xx)) +
+              T(3.0) * (xy * xy + yz * yz + zx * zx));
 }
 
 // std::min(edot, cap) (Mechanical.C:1737)
-__device__ __forceinline__ double cap_rate(const MatParams &m, double edot) {
-  return (m.max_edot < edot) ? m.max_edot : edot;
+template <typename T>
+__device__ __forceinline__ T cap_rate(const MatParams &m, T edot) {
+  return ((T)m.max_edot < edot) ? (T)m.max_edot : edot;
 }
 
 
@@ -661,61 +677,61 @@ inline int law_class(int model) { return model >= JOHNSON_COOK ? LAW_RATE : LAW_
 
 // Values the tangent of a rate law shares with the yield stress of the same
 // element and step (equal inputs give equal values, so they are reused)
+template <typename T>
 struct RateCache {
-  double sr, lterm;  // Johnson-Cook: the rate and 1 + C ln(rate/eps_0)
-  double er, pw_er;  // GMT: the clamped rate and er^(m1 Tc + m2)
-  double ex_i;       // GMT: exp((I1 Tc + I2)/e)
+  T sr, lterm;  // Johnson-Cook: the rate and 1 + C ln(rate/eps_0)
+  T er, pw_er;  // GMT: the clamped rate and er^(m1 Tc + m2)
+  T ex_i;       // GMT: exp((I1 Tc + I2)/e)
 };
 
 // Calc*YieldStress (Material.cuh:353-364, :378-388, :412-441) at plastic
 // strain ep and effective strain rate edot, the rate laws with the
 // temperature factors rt; bilinear as sigma_y = sy0_e + H*ep (the reference
-// leaves it undefined).  The rate laws evaluate in double in both precisions
-// (the clamps of GMT: Material.cuh:427-435, :451-459).
+// leaves it undefined).  The clamps of GMT: Material.cuh:427-435, :451-459.
 template <typename T, int LAW>
-__device__ __forceinline__ T yield_stress(const MatParams &m, const RateTemp &rt, T sy0e, T ep, double edot,
-                                          RateCache &rc) {
+__device__ __forceinline__ T yield_stress(const MatParams &m, const RateTemp &rt, T sy0e, T ep, T edot,
+                                          RateCache<T> &rc) {
   if constexpr (LAW == LAW_PLAIN) {
     if (m.model == HOLLOMON) {
-      double s = (double)ep + m.eps0;
-      return (s > m.eps1) ? (T)(m.Kh * pow(s, m.mh)) : (T)m.sy0;
+      const T s = ep + (T)m.eps0;
+      return (s > (T)m.eps1) ? (T)m.Kh * rpow(s, (T)m.mh) : (T)m.sy0;
     }
     return sy0e + (T)m.H * ep;
   } else if (m.model == JOHNSON_COOK) {
-    rc.sr = (edot == 0.0) ? 1.e-5 : edot;
-    rc.lterm = 1.0 + m.C * log(rc.sr / m.eps_0);
-    return (T)(((double)sy0e + m.B * pow((double)ep, m.n)) * rc.lterm * rt.jc_soft);
+    rc.sr = (edot == T(0)) ? (T)1.e-5 : edot;
+    rc.lterm = T(1) + (T)m.C * rlog(rc.sr / (T)m.eps_0);
+    return (sy0e + (T)m.B * rpow(ep, (T)m.n)) * rc.lterm * (T)rt.jc_soft;
   } else {
-    const double es = clamp_range((double)ep, m.e_min, m.e_max);
-    rc.er = clamp_range(edot, m.er_min, m.er_max);
-    rc.pw_er = pow(rc.er, rt.g_m);
-    rc.ex_i = exp(rt.g_i / es);
-    return (T)(rt.g_a * pow(es, rt.g_n) * rc.ex_i * rc.pw_er);
+    const T es = rclamp(ep, m.e_min, m.e_max);
+    rc.er = rclamp(edot, m.er_min, m.er_max);
+    rc.pw_er = rpow(rc.er, (T)rt.g_m);
+    rc.ex_i = rexp((T)rt.g_i / es);
+    return (T)rt.g_a * rpow(es, (T)rt.g_n) * rc.ex_i * rc.pw_er;
   }
 }
 
 // Calc*TangentModulus (Material.cuh:389-395, :397-411, :444-473); edot is
 // the capped rate.
 template <typename T, int LAW>
-__device__ __forceinline__ T tangent_modulus(const MatParams &m, const RateTemp &rt, T ep, double edot,
-                                             const RateCache &rc) {
+__device__ __forceinline__ T tangent_modulus(const MatParams &m, const RateTemp &rt, T ep, T edot,
+                                             const RateCache<T> &rc) {
   if constexpr (LAW == LAW_PLAIN) {
     if (m.model == HOLLOMON) {
-      double s = (double)ep + m.eps0;
-      return (s > m.eps1) ? (T)(m.Kh * m.mh * pow(s, (m.mh - 1.0))) : T(0);
+      const T s = ep + (T)m.eps0;
+      return (s > (T)m.eps1) ? (T)(m.Kh * m.mh) * rpow(s, (T)(m.mh - 1.0)) : T(0);
     }
     return (T)m.H;
   } else if (m.model == JOHNSON_COOK) {
     if (ep > T(0)) {
-      const double lterm = (edot == rc.sr) ? rc.lterm : 1.0 + m.C * log(edot / m.eps_0);
-      return (T)(m.n * m.B * pow((double)ep, m.n - 1.) * lterm * rt.jc_soft);
+      const T lterm = (edot == rc.sr) ? rc.lterm : T(1) + (T)m.C * rlog(edot / (T)m.eps_0);
+      return (T)(m.n * m.B) * rpow(ep, (T)(m.n - 1.)) * lterm * (T)rt.jc_soft;
     }
     return (T)(m.E * 0.1);
   } else {
-    const double es = clamp_range((double)ep, m.e_min, m.e_max);
-    const double er = clamp_range(edot, m.er_min, m.er_max);
-    const double pw_er = (er == rc.er) ? rc.pw_er : pow(er, rt.g_m);
-    return (T)(rt.g_a * pw_er * pow(es, rt.g_n - 2.0) * (rt.g_ni + es * rt.g_n) * rc.ex_i);
+    const T es = rclamp(ep, m.e_min, m.e_max);
+    const T er = rclamp(edot, m.er_min, m.er_max);
+    const T pw_er = (er == rc.er) ? rc.pw_er : rpow(er, (T)rt.g_m);
+    return (T)rt.g_a * pw_er * rpow(es, (T)(rt.g_n - 2.0)) * ((T)rt.g_ni + es * (T)rt.g_n) * rc.ex_i;
   }
 }
 
@@ -766,10 +782,10 @@ __global__ __launch_bounds__(kBlock) void k_stress_strain(
   T ep = pl_strain[e];
   // bilinear plastic modulus: per element when given (specimen batches)
   const bool per_elem = LAW == LAW_PLAIN && mp.model == BILINEAR && He;
-  const double edot = (LAW == LAW_RATE) ? eff_strain_rate(D) : 0.0;
+  const T edot = (LAW == LAW_RATE) ? eff_strain_rate(D) : T(0);
   RateTemp rt = mp.ut;
   if (LAW == LAW_RATE && !mp.t_uniform) rt = rate_temp_at(mp, law_temperature(mp, e));
-  RateCache rc;
+  RateCache<T> rc;
   T sy = per_elem ? sy0[e] + He[e] * ep : yield_stress<T, LAW>(mp, rt, sy0[e], ep, edot, rc);
   sigma_y[e] = sy;
   T dep = T(0);
