@@ -242,9 +242,10 @@ int wfx_reaction_force(wfx_ctx *ctx, int dim, double *out, wfx_stream s);
 
 /* ---- launch tuning and graphs (no reference counterpart) ----------------- */
 /* Chooses the block size (64/128/256/512) of each fused phase on the current
- * mesh: coordinate descent on the time of whole fused steps (reps steps per
- * candidate, two sweeps over the phases), fastest kept (chosen[4], may be
- * NULL); the state is saved and restored around the measurement. */
+ * mesh: coordinate descent on the time of whole fused steps (max(reps, 10)
+ * steps per candidate, best of five rounds, two sweeps over the phases),
+ * fastest kept (chosen[4], may be NULL); the state is saved and restored
+ * around the measurement. */
 int wfx_autotune(wfx_ctx *ctx, int reps, int *chosen, wfx_stream s);
 int wfx_set_block_sizes(wfx_ctx *ctx, const int *block_sizes4);
 /* Capture nsteps fused steps (what wfx_step_fused(nsteps) enqueues) into a

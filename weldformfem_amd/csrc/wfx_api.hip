@@ -1260,18 +1260,21 @@ int wfx_autotune(wfx_ctx *c, int reps, int *chosen, wfx_stream s) {
   auto step = [&]() {
     for (int ph = 1; ph <= 4 && err == hipSuccess; ++ph) err = BY_PRECISION(c, launch_fused_phase, c, ph, 1, st, -1);
   };
-  // coordinate descent over the four passes, two sweeps; within a pass three
-  // rounds over the candidates, the best round of each counts: interleaving
-  // the candidates keeps clock and cache drift out of the choice
+  // coordinate descent over the four passes, two sweeps; within a pass five
+  // rounds over the candidates of at least 10 steps each, the best round of
+  // each counts: interleaving the candidates keeps clock and cache drift out
+  // of the choice (with three rounds of 5 steps a pass of 0.66 ms was once
+  // tuned to a choice 4% slower)
+  const int nrep = reps > 10 ? reps : 10;
   for (int sweep = 0; sweep < 2 && err == hipSuccess; ++sweep)
     for (int ph = 1; ph <= 4 && err == hipSuccess; ++ph) {
       float best_t[4] = {1e30f, 1e30f, 1e30f, 1e30f};
-      for (int round = 0; round < 3 && err == hipSuccess; ++round)
+      for (int round = 0; round < 5 && err == hipSuccess; ++round)
         for (int k = 0; k < 4 && err == hipSuccess; ++k) {
           c->fused_bs[ph] = sizes[k];
           step();  // warm-up
           if (err == hipSuccess) err = hipEventRecord(e0, st);
-          for (int r = 0; r < reps && err == hipSuccess; ++r) step();
+          for (int r = 0; r < nrep && err == hipSuccess; ++r) step();
           if (err == hipSuccess) err = hipEventRecord(e1, st);
           if (err == hipSuccess) err = hipEventSynchronize(e1);
           float ms = 0.f;
