@@ -94,3 +94,54 @@ def test_reference_example_meshes(name, n, E, N):
     assert np.unique(el).size == N
     rxy = np.hypot(x[:, 0], x[:, 1])
     assert rxy.max() < 0.0128
+
+
+def test_sparse_ids_crlf_and_lowercase_keywords(tmp_path):
+    # ids spread over a wide range take the hash-map path; Windows line ends
+    # and lowercase keywords are accepted
+    ids = [7, 1_000_003, 42, 99_999_937]  # within the 8-column id field
+    lines = ["*keyword", "*node"]
+    for k, i in enumerate(ids):
+        lines.append(f"{i:8d}{float(k):16.9e}{-0.5 * k:16.9e}{1e-3 * k:16.9e}")
+    lines += ["*element_solid", "       1       1" + "".join(f"{i:8d}" for i in ids) + "".join(f"{ids[3]:8d}" * 4),
+              "*end"]
+    p = tmp_path / "sparse.k"
+    p.write_bytes(("\r\n".join(lines) + "\r\n").encode())
+    x, el, n = parse_lsdyna_k(p)
+    assert n == 4
+    np.testing.assert_array_equal(el[0], [0, 1, 2, 3])
+    np.testing.assert_array_equal(x[:, 1], [0.0, -0.5, -1.0, -1.5])
+
+
+def test_duplicate_ids_and_mixed_elements_are_refused(tmp_path):
+    from weldformfem_amd._lib import WfxError
+
+    nodes = "*NODE\n" + "".join(f"{i:8d}{0.0:16.9e}{0.0:16.9e}{float(i):16.9e}\n" for i in range(1, 9))
+    p = tmp_path / "dup.k"
+    p.write_text(nodes + f"{3:8d}{1.0:16.9e}{0.0:16.9e}{0.0:16.9e}\n*ELEMENT_SOLID\n1 1 1 2 3 4 5 6 7 8\n")
+    with pytest.raises(WfxError, match="duplicate node id 3"):
+        parse_lsdyna_k(p)
+    p.write_text(nodes + "*ELEMENT_SOLID\n1 1 1 2 3 4 5 6 7 8\n2 1 1 2 3 4 4 4 4 4\n")
+    with pytest.raises(WfxError, match="mixed"):
+        parse_lsdyna_k(p)
+
+
+def test_generated_block_round_trip(tmp_path):
+    # a 12x10x8 hexa block written with 9-digit coordinates reads back to the
+    # correctly rounded values of the text and the same connectivity
+    from weldformfem_amd import workloads
+
+    x, el = workloads.hexa_box(12, 10, 8, 0.0123, origin=(-0.05, 0.1, 0.0))
+    p = tmp_path / "block.k"
+    with open(p, "w") as f:
+        f.write("*KEYWORD\n*NODE\n")
+        np.savetxt(f, np.column_stack([np.arange(1, len(x) + 1), x]), fmt="%8d%16.9e%16.9e%16.9e")
+        f.write("*ELEMENT_SOLID\n")
+        np.savetxt(f, np.column_stack([np.arange(1, len(el) + 1), np.ones(len(el), np.int64), el + 1]),
+                   fmt="%8d" * 10)
+        f.write("*END\n")
+    xs, els, n = parse_lsdyna_k(p)
+    assert n == 8
+    np.testing.assert_array_equal(els, el)
+    want = np.array([float("%.9e" % v) for v in x.reshape(-1)]).reshape(-1, 3)
+    np.testing.assert_array_equal(xs, want)
