@@ -342,6 +342,36 @@ def extensions_leg(torch, blocks, steps, log):
     return out
 
 
+def large_leg(torch, log, nx=400, ny=400, nz=200, steps=50):
+    """The same compression block eight times larger (400x400x200, 32M
+    elements): device memory per element and the fused step rate, FP64 and
+    FP32-mixed, autotuned"""
+    from weldformfem_amd import WFX_FP32_MIXED, WFX_FP64
+
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {"workload": f"hexa compression block {nx}x{ny}x{nz} ({nx * ny * nz} elements), fused step"}
+    for prec, tag in ((WFX_FP64, "fp64"), (WFX_FP32_MIXED, "fp32_mixed")):
+        torch.cuda.synchronize()
+        free0, _ = torch.cuda.mem_get_info()
+        t0 = time.perf_counter()
+        d, _ = build_domain(prec, nx, ny, nz, [nx / 100.0, ny / 100.0, nz / 100.0], -2.0, stream)
+        t_setup = time.perf_counter() - t0
+        d.runFused(3, reaction=False)
+        torch.cuda.synchronize()
+        free1, _ = torch.cuda.mem_get_info()
+        bs = d.autotune(reps=5)
+        d.runFused(5, reaction=False)
+        t = timed(lambda: d.runFused(steps, reaction=False), torch) / steps
+        E = nx * ny * nz
+        out[tag] = {"value": E / t, "ms_per_step": t * 1e3, "device_bytes_per_element": (free0 - free1) / E,
+                    "setup_s": t_setup, "block_sizes": bs}
+        log(f"large {tag}: {t * 1e3:.3f} ms/step, {E / t:.3e} element-steps/s, "
+            f"{(free0 - free1) / E:.0f} B/element on the device, set-up {t_setup:.1f} s")
+        del d
+        torch.cuda.synchronize()
+    return out
+
+
 def write_k_file(path, x, elnod):
     """LS-DYNA keyword file (*NODE fixed 8+16+16+16, *ELEMENT_SOLID 8+8 then
     8-wide node ids), ids from 1"""
@@ -556,6 +586,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--no-extensions", action="store_true")
     ap.add_argument("--no-io", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--bs64", default="", help="fused block sizes for FP64 (f1,f2,f3,f4) instead of autotuning")
     ap.add_argument("--bs32", default="", help="the same for FP32-mixed")
     ap.add_argument("--batch-steps", type=int, default=1000)
@@ -675,6 +706,10 @@ def main():
     if not args.no_io:
         io = io_leg(torch, log)
 
+    large = None
+    if not args.no_large and not args.quick:
+        large = large_leg(torch, log)
+
     batch = None
     if not args.no_batch:
         batch = batch_leg(torch, args.batch_steps, log)
@@ -734,6 +769,7 @@ def main():
         "other_configs": others,
         "extensions": extensions,
         "io": io,
+        "large_block": large,
         "batched_specimens": batch,
         "adjoint": adjoint,
         "cpu_baseline": cpu,
