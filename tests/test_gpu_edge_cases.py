@@ -143,3 +143,21 @@ def test_invalid_arguments_are_refused():
     g.init()
     g.run(3)
     assert g.nonfiniteCount() == 0
+
+
+def test_sizes_beyond_int32_are_refused():
+    # refused from the sizes alone, before any array is read or allocated
+    say("case: meshes beyond the int32 connectivity and CSR")
+    import ctypes
+
+    g = Domain(WFX_FP64)
+    with pytest.raises(WfxError, match="no elements or sizes beyond int32"):
+        g.AddBoxLength([0, 0, 0], [10.0, 10.0, 10.0], 0.005)  # 1e9 hexahedra
+    with pytest.raises(WfxError, match="no elements or sizes beyond int32"):
+        g.AddBoxLength([0, 0, 0], [0.01, 0.01, 0.01], 0.025)  # box smaller than one element
+    x = np.zeros(24)
+    el = np.zeros(8, np.int32)
+    rc = g._L.wfx_set_mesh(g._h, 8, 300_000_000, 8, x.ctypes.data, el.ctypes.data)
+    assert rc != 0 and b"too large" in g._L.wfx_last_error()
+    g.AddBoxLength([0, 0, 0], [0.1, 0.1, 0.1], 0.025)  # still usable
+    assert g.elem_count == 8

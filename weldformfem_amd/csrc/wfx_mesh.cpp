@@ -19,8 +19,11 @@ namespace wfx {
 int box_length_mesh(const double V[3], const double L[3], double r, bool tritetra,
                     std::vector<double> &x, std::vector<int> &elnod, int &nodxelem) {
   const double step = 2.0 * r;
-  const int nx = (int)(L[0] / step), ny = (int)(L[1] / step), nz = (int)(L[2] / step);
-  if (nx <= 0 || ny <= 0 || nz <= 0) return -1;
+  const double fx = L[0] / step, fy = L[1] / step, fz = L[2] / step;
+  // counts in int range before the conversion (also refuses NaN)
+  if (!(fx >= 1.0 && fy >= 1.0 && fz >= 1.0 && fx < 2147483647.0 && fy < 2147483647.0 && fz < 2147483647.0))
+    return -1;
+  const int nx = (int)fx, ny = (int)fy, nz = (int)fz;
   const int px = nx + 1, py = ny + 1, pz = nz + 1;
   const int64_t nn = (int64_t)px * py * pz;
   const int64_t ncube = (int64_t)nx * ny * nz;
