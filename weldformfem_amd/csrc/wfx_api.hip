@@ -109,7 +109,7 @@ static int setup_device(wfx_ctx *c) {
   if (e == hipSuccess) e = dalloc(c, &ptr, (cnt));
   A(c->nd.x, 3 * N) A(c->nd.v, 3 * N) A(c->nd.a, 3 * N) A(c->nd.u, 3 * N) A(c->nd.u_dt, 3 * N)
   A(c->nd.prev_a, 3 * N) A(c->nd.fi, 3 * N) A(c->nd.fe, 3 * N) A(c->nd.mdiag, N) A(c->nd.voln, N)
-  A(c->nd.voln_sum, N) A(c->nd.voln0_sum, N) A(c->nd.anp_ratio, N) A(c->nd.T, N)
+  A(c->nd.voln0_sum, N) A(c->nd.anp_ratio, N) A(c->nd.T, N)
   A(c->ms.elnod, n * E) A(c->ms.nodel, n * E) A(c->ms.nodel_loc, n * E) A(c->ms.nodel_offset, N)
   A(c->ms.nodel_count, N) A(c->ms.nodel_pk, n * E) A(c->ms.nodel_pad, 8 * N) A(c->bc_slot, N)
   A(c->scalars, 64) A(c->status, 16)

@@ -104,7 +104,7 @@ struct DevNode {
   double *x = nullptr, *v = nullptr, *a = nullptr, *u = nullptr, *u_dt = nullptr,
          *prev_a = nullptr, *fi = nullptr, *fe = nullptr;
   double *T = nullptr;  // nodal temperature (thermal coupling)
-  double *mdiag = nullptr, *voln = nullptr, *voln_sum = nullptr, *voln0_sum = nullptr;
+  double *mdiag = nullptr, *voln = nullptr, *voln0_sum = nullptr;
   double *anp_ratio = nullptr;  // fused path: V_n / V0_n
 };
 

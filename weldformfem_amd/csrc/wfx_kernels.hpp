@@ -482,14 +482,14 @@ __global__ __launch_bounds__(kBlock) void k_strain_rates(int E, int N, const int
 }
 
 // calcElemPressure node part (Mechanical.C:698-705): CSR sums of vol, vol0
+// and their ratio, the value the element part averages (:720-730)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pressure_nodes(int N, const int *__restrict__ off,
                                                            const int *__restrict__ cnt,
                                                            const int *__restrict__ nodel,
                                                            const T *__restrict__ vol,
                                                            const T *__restrict__ vol0,
-                                                           double *__restrict__ vs,
-                                                           double *__restrict__ v0s) {
+                                                           double *__restrict__ ratio) {
   int i = gtid();
   if (i >= N) return;
   int o = off[i], c = cnt[i];
@@ -499,8 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_nodes(int N, const int *__r
              s0 += q.x;
              s += q.y;
            });
-  v0s[i] = s0;
-  vs[i] = s;
+  ratio[i] = s / s0;
 }
 
 struct PressParams {
@@ -514,8 +513,7 @@ struct PressParams {
 // div v are exactly +-0 and are skipped (full == 0).
 template <typename T, int NN>
 __global__ __launch_bounds__(kBlock) void k_pressure_elem(int E, int N, const int *__restrict__ elnod,
-                                                          const double *__restrict__ vs,
-                                                          const double *__restrict__ v0s,
+                                                          const double *__restrict__ ratio,
                                                           const T *__restrict__ vol,
                                                           const T *__restrict__ vol0,
                                                           const T *__restrict__ rho,
@@ -529,7 +527,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_elem(int E, int N, const in
 #pragma unroll
   for (int a = 0; a < NN; ++a) {
     int nid = elnod[(long)a * E + e];
-    J_avg += vs[nid] / v0s[nid];
+    J_avg += ratio[nid];
   }
   J_avg /= NN;
   double al = pp.alpha_free;
@@ -1149,7 +1147,7 @@ template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, d
     auto &el = elem<T>(c);                                                                         \
     hipLaunchKernelGGL((k_pressure_nodes<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N,       \
                        c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel, el.vol, el.vol0,        \
-                       c->nd.voln_sum, c->nd.voln0_sum);                                           \
+                       c->nd.anp_ratio);                                                           \
     const auto &st = c->prm.stab;                                                                  \
     PressParams pp{c->prm.mat.K,       st.alpha_free, st.log_factor,     st.J_min,                 \
                    st.hg_coeff_free,   st.pspg_scale, st.p_pspg_bulkfac, st.av_coeff_div,          \
@@ -1158,11 +1156,11 @@ template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, d
                st.av_coeff_bulk != 0.0);                                                           \
     if (c->n == 8)                                                                                 \
       hipLaunchKernelGGL((k_pressure_elem<T, 8>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,   \
-                         c->N, c->ms.elnod, c->nd.voln_sum, c->nd.voln0_sum, el.vol, el.vol0,      \
+                         c->N, c->ms.elnod, c->nd.anp_ratio, el.vol, el.vol0,                      \
                          el.rho, el.dH, c->nd.v, el.p, pp);                                        \
     else                                                                                           \
       hipLaunchKernelGGL((k_pressure_elem<T, 4>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,   \
-                         c->N, c->ms.elnod, c->nd.voln_sum, c->nd.voln0_sum, el.vol, el.vol0,      \
+                         c->N, c->ms.elnod, c->nd.anp_ratio, el.vol, el.vol0,                      \
                          el.rho, el.dH, c->nd.v, el.p, pp);                                        \
     return hipGetLastError();                                                                      \
   }                                                                                                \
