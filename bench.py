@@ -110,22 +110,36 @@ def event_times(calls, reps, torch):
 
 
 def copy_bandwidth(torch, gib=2, reps=10):
-    """device-to-device copy bandwidth (read + write bytes / time), HIP events"""
+    """device-to-device copy bandwidth (read + write bytes / time, HIP events):
+    torch's copy and the library's streaming copy (16-byte non-temporal
+    accesses); returns (best, torch, stream) in bytes/s"""
+    from weldformfem_amd import _lib
+
+    L = _lib.load()
     n = (gib << 30) // 8
     src = torch.ones(n, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
-    dst.copy_(src)
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(reps):
-        dst.copy_(src)
-    e1.record(stream)
-    e1.synchronize()
-    t = e0.elapsed_time(e1) * 1e-3 / reps
+
+    def rate(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return 2 * n * 8 / (e0.elapsed_time(e1) * 1e-3 / reps)
+
+    def stream_copy():
+        _lib.check(L.wfx_stream_copy(src.data_ptr(), dst.data_ptr(), n * 8, stream.cuda_stream))
+
+    r_torch = rate(lambda: dst.copy_(src))
+    r_stream = rate(stream_copy)
+    assert torch.equal(src, dst)
     del src, dst
-    return 2 * n * 8 / t
+    return max(r_torch, r_stream), r_torch, r_stream
 
 
 def fused_calls(d):
@@ -608,7 +622,7 @@ def main():
     torch.cuda.init()
     stream = torch.cuda.current_stream().cuda_stream
     nx, ny, nz, L = (200, 200, 100, [2.0, 2.0, 1.0]) if not args.quick else (100, 100, 50, [2.0, 2.0, 1.0])
-    copy_bw = copy_bandwidth(torch)
+    copy_bw, copy_torch, copy_stream = copy_bandwidth(torch)
 
     def log(msg):
         print("bench: " + msg, file=sys.stderr, flush=True)
@@ -753,7 +767,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic,
                      "algorithmic_bytes": dom_bytes, "avg_launch_us": kt64[dom] * 1e6,
-                     "measured_copy_GBs": copy_bw / 1e9, "frac_of_measured_copy": achieved * 1e9 / copy_bw},
+                     "measured_copy_GBs": copy_bw / 1e9, "frac_of_measured_copy": achieved * 1e9 / copy_bw,
+                     "copy_GBs_torch": copy_torch / 1e9, "copy_GBs_stream": copy_stream / 1e9},
         "step_algorithmic_bytes_per_element": step_bytes / E,
         "step_hbm_frac": step_bytes / s64 / HBM_PEAK,
         # the per-step API's compulsory bytes (one pass per Domain_d call) at

@@ -279,6 +279,11 @@ int wfx_adjoint_release(wfx_ctx *ctx);
 int wfx_energies(wfx_ctx *ctx, double *ekin, double *deint, wfx_stream s);
 /* number of non-finite internal-force entries scrubbed so far */
 int wfx_nonfinite_count(wfx_ctx *ctx, long *out);
+/* Bandwidth probe, no reference counterpart: copies bytes (a positive
+ * multiple of 16, both pointers 16-byte aligned, device memory) from src to
+ * dst on stream s with 16-byte non-temporal accesses; the benchmark times it
+ * as the streaming ceiling of the HBM-bound passes. */
+int wfx_stream_copy(const void *src, void *dst, long bytes, wfx_stream s);
 
 /* ---- file formats (host side) -------------------------------------------- */
 /* LS-DYNA keyword mesh: *NODE and *ELEMENT_SOLID (CreateFromLSDyna,

@@ -161,3 +161,20 @@ def test_sizes_beyond_int32_are_refused():
     assert rc != 0 and b"too large" in g._L.wfx_last_error()
     g.AddBoxLength([0, 0, 0], [0.1, 0.1, 0.1], 0.025)  # still usable
     assert g.elem_count == 8
+
+
+def test_stream_copy_probe():
+    import torch
+
+    from weldformfem_amd import _lib
+
+    say("case: wfx_stream_copy copies, refuses misaligned pointers and sizes")
+    L = _lib.load()
+    src = torch.arange(1 << 20, dtype=torch.float64, device="cuda")
+    dst = torch.zeros_like(src)
+    s = torch.cuda.current_stream().cuda_stream
+    assert L.wfx_stream_copy(src.data_ptr(), dst.data_ptr(), src.numel() * 8, s) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(src, dst)
+    assert L.wfx_stream_copy(src.data_ptr() + 8, dst.data_ptr(), 1024, s) != 0
+    assert L.wfx_stream_copy(src.data_ptr(), dst.data_ptr(), 1000, s) != 0

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1467,6 +1468,28 @@ int wfx_nonfinite_count(wfx_ctx *c, long *out) {
   unsigned int h = 0;
   WFX_CHECK_HIP(hipMemcpy(&h, c->status, sizeof(unsigned int), hipMemcpyDeviceToHost));
   *out = h;
+  return WFX_OK;
+}
+
+// ---- bandwidth probe ----------------------------------------------------------
+// A streaming copy in the access pattern that reaches the highest rate measured
+// on MI355X: one 16-byte pair per thread, non-temporal loads and stores,
+// 256-thread blocks (6.5 TB/s read + write on 2 GiB, where an 8-byte copy in
+// 64-thread blocks reaches 4.8).  The benchmark's ceiling for HBM-bound passes.
+typedef double wfx_d2v __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256) void k_stream_copy(const wfx_d2v *__restrict__ a, wfx_d2v *__restrict__ b, long n) {
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(&a[i]), &b[i]);
+}
+
+int wfx_stream_copy(const void *src, void *dst, long bytes, wfx_stream s) {
+  REQUIRE(src && dst && bytes > 0 && bytes % 16 == 0 && ((uintptr_t)src | (uintptr_t)dst) % 16 == 0, WFX_ERR_ARG,
+          "stream copy needs 16-byte aligned pointers and a positive multiple of 16 bytes");
+  const long n = bytes / 16;
+  REQUIRE((n + 255) / 256 < 2147483647L, WFX_ERR_ARG, "stream copy too large for one launch");
+  hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(s),
+                     (const wfx_d2v *)src, (wfx_d2v *)dst, n);
+  WFX_CHECK_HIP(hipGetLastError());
   return WFX_OK;
 }
 
