@@ -3,8 +3,9 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is the oracle that the HIP library in
  * weldformfem_amd/ is checked against.  Only tests/, __graft_entry__.smoke()
- * and bench.py's cpu_baseline leg may load it, and only as the checker or as
- * the timed CPU baseline; the product path never calls it.
+ * and bench.py's cpu_baseline leg may load it, and only as the reference
+ * results are compared against or as the timed CPU baseline; the product path
+ * never calls it.
  *
  * Every function restates the reference's algorithm in plain C with the
  * reference's own data layout (flat arrays, AoS inside a field) and the same

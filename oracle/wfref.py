@@ -1,8 +1,8 @@
 """ctypes front end of the oracle (oracle/libwfref.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker and by bench.py's cpu_baseline leg.  The product path never uses
-it.  Arrays come back in the reference's own layout (Domain_d: AoS inside a
+the results to compare against and by bench.py's cpu_baseline leg.  The
+product path never uses it.  Arrays come back in the reference's own layout (Domain_d: AoS inside a
 field, element tensors as [xx,yy,zz,xy,yz,xz], element forces as [e][a][d]).
 """
 import ctypes

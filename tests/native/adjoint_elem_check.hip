@@ -1,4 +1,4 @@
-// Host-only checker of the element forward/reverse functions of the adjoint
+// Host-only test program for the element forward/reverse functions of the adjoint
 // (weldformfem_amd/csrc/wfx_adjoint.hpp).  Built with hipcc --offload-host-only
 // by tests/test_adjoint_math.py; runs on the CPU.
 //

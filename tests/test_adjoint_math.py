@@ -23,7 +23,7 @@ SRC = os.path.join(HERE, "native", "adjoint_elem_check.hip")
 
 
 @pytest.fixture(scope="module")
-def checker(tmp_path_factory):
+def elem_check(tmp_path_factory):
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     if not os.path.exists(hipcc):
         pytest.skip("hipcc not available")
@@ -36,8 +36,8 @@ def checker(tmp_path_factory):
 @pytest.mark.parametrize("law", [0, 1])  # bilinear, Hollomon
 @pytest.mark.parametrize("nn", [8, 4])
 @pytest.mark.parametrize("plastic", [0, 1])
-def test_element_reverse_matches_central_differences(checker, nn, plastic, law):
-    r = subprocess.run([checker, "vjp", str(nn), "11", "25", str(plastic), str(law)], capture_output=True,
+def test_element_reverse_matches_central_differences(elem_check, nn, plastic, law):
+    r = subprocess.run([elem_check, "vjp", str(nn), "11", "25", str(plastic), str(law)], capture_output=True,
                        text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     used, worst = r.stdout.split()
@@ -59,7 +59,7 @@ def _mesh(tet, seed):
 
 @pytest.mark.parametrize("law", ["bilinear", "hollomon"])
 @pytest.mark.parametrize("tet", [False, True])
-def test_element_forward_matches_oracle(checker, tet, law):
+def test_element_forward_matches_oracle(elem_check, tet, law):
     x, el, n = _mesh(tet, seed=5)
     N, E = x.shape[0], el.shape[0]
     r = RefDomain()
@@ -114,7 +114,7 @@ def test_element_forward_matches_oracle(checker, tet, law):
         vals = list(x[el[e]].reshape(-1)) + list(v3[el[e]].reshape(-1)) + list(ratio[el[e]])
         vals += list(tau[6 * e:6 * e + 6]) + [ep[e], rho, sy0[e], H]
         lines.append(" ".join(repr(float(q)) for q in vals))
-    out = subprocess.run([checker, "forward", str(n)], input="\n".join(lines) + "\n", capture_output=True,
+    out = subprocess.run([elem_check, "forward", str(n)], input="\n".join(lines) + "\n", capture_output=True,
                          text=True, check=True).stdout.split("\n")
     res = np.array([[float(q) for q in line.split()] for line in out if line.strip()])
     fnet = res[:, :3 * n].reshape(E, n, 3)
