@@ -94,8 +94,10 @@ static hipError_t alloc_elem(wfx_ctx *c, DevElem<T> &el) {
   if (e == hipSuccess) e = dalloc(c, &el.ptr, (cnt));
   A(detJ, E) A(dH, 3 * n * E) A(vol, E) A(vol0, E) A(rho, E) A(rho0, E) A(p, E) A(pl_strain, E)
   A(sigma_y, E) A(sy0, E) A(str_rate, 6 * E) A(rot_rate, 3 * E) A(sigma, 6 * E) A(tau, 6 * E)
-  // f_elem is padded to whole 64-element tiles (the fused path may tile it)
-  A(eps, 6 * E) A(strain_pl_incr, 6 * E) A(f_elem, 3 * n * ((E + 63) / 64 * 64)) A(f_hg, 3 * n * E)
+  // f_elem is padded to whole 64-element tiles (the fused path may tile it);
+  // FP32-mixed fused runs keep 4 floats per element-node there (float4)
+  A(eps, 6 * E) A(strain_pl_incr, 6 * E) A(f_elem, (std::is_same<T, float>::value ? 4 : 3) * n * ((E + 63) / 64 * 64))
+  A(f_hg, 3 * n * E)
 #undef A
   return e;
 }
