@@ -62,6 +62,13 @@ __device__ __forceinline__ long fidx(int row, long e, long Es, int nrow) {
 // bytes cost the element pass more than the wider gather saves (measured:
 // step 0.656 -> 0.665 ms), so the plain runs keep one row per component.
 typedef float f4v __attribute__((ext_vector_type(4)));
+// FP32-mixed without thermal coupling: an element-node's (fx, fy) is one
+// 8-byte pair in the first 2n rows' worth of the buffer, tiled [E/64][n][64],
+// and fz a float in the last n rows' worth at the same index: two accesses
+// per element-node instead of three, the same bytes (node pass 260 -> 234
+// µs at 4M elements).  FP64 keeps one row per component: the same split with
+// 16-byte pairs made its node pass slower (405 -> 412 µs).
+typedef float f2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ long fidx4(int a, long e, int nn) { return (e >> 6) * ((long)nn * 64) + (long)a * 64 + (e & 63); }
 template <typename T>
 __device__ __forceinline__ void st_stream(T *p, T v) {
@@ -617,6 +624,11 @@ __global__ __launch_bounds__(BS) void f3_element(
                          if constexpr (std::is_same<T, float>::value && TH) {
                            f4v q = {f0, f1, f2, th};
                            st_stream(reinterpret_cast<f4v *>(fout) + fidx4(a, e, NN), q);
+                         } else if constexpr (std::is_same<T, float>::value) {
+                           const long Ep = (Es + 63) / 64 * 64, k = fidx4(a, e, NN);
+                           f2v xy = {f0, f1};
+                           st_stream(reinterpret_cast<f2v *>(fout) + k, xy);
+                           st_stream(fout + 2L * NN * Ep + k, f2);
                          } else {
                            st_stream(&fout[fidx(a * 3 + 0, e, Es, 3 * NN)], f0);
                            st_stream(&fout[fidx(a * 3 + 1, e, Es, 3 * NN)], f1);
@@ -809,7 +821,7 @@ __device__ __forceinline__ void gather_force(int i, long Ns, long Es, int nrow, 
       pk[j] = (k == 0) ? pad[(long)j * Ns + i] : ((k + j < c) ? nodel_pk[o + k + j] : -1);
 #pragma unroll
     for (int j = 0; j < kChunk; ++j) {
-      long e = pk[j] >> 3, b = (long)(pk[j] & 7) * 3;
+      const long e = pk[j] >> 3;
       if constexpr (std::is_same<T, float>::value && TH) {  // one float4 per element-node
         const f4v z = {0.f, 0.f, 0.f, 0.f};
         const f4v v = (pk[j] >= 0) ? reinterpret_cast<const f4v *>(fel)[fidx4(pk[j] & 7, e, nrow / 3)] : z;
@@ -817,7 +829,16 @@ __device__ __forceinline__ void gather_force(int i, long Ns, long Es, int nrow, 
         q[j][1] = v.y;
         q[j][2] = v.z;
         if constexpr (TH) q[j][3] = v.w;
+      } else if constexpr (std::is_same<T, float>::value) {  // an (fx, fy) pair and fz
+        const int nn = nrow / 3;
+        const long k = fidx4(pk[j] & 7, e, nn), Ep = (Es + 63) / 64 * 64;
+        const f2v z2 = {0.f, 0.f};
+        const f2v xy = (pk[j] >= 0) ? reinterpret_cast<const f2v *>(fel)[k] : z2;
+        q[j][0] = xy.x;
+        q[j][1] = xy.y;
+        q[j][2] = (pk[j] >= 0) ? fel[2L * nn * Ep + k] : T(0);
       } else {
+        const long b = (long)(pk[j] & 7) * 3;
 #pragma unroll
         for (int d = 0; d < 3; ++d) q[j][d] = (pk[j] >= 0) ? fel[fidx((int)(b + d), e, Es, nrow)] : T(0);
         if constexpr (TH) q[j][3] = (pk[j] >= 0) ? dTe[fidx(pk[j] & 7, e, Es, nrow / 3)] : T(0);
