@@ -611,7 +611,7 @@ def main():
     if args.pmc_probe:
         return pmc_probe(args)
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
               f"(sharding across GPUs is a design only)", file=sys.stderr)
         sys.exit(2)
 

@@ -288,7 +288,7 @@ int wfx_stream_copy(const void *src, void *dst, long bytes, wfx_stream s);
 /* ---- file formats (host side) -------------------------------------------- */
 /* LS-DYNA keyword mesh: *NODE and *ELEMENT_SOLID (CreateFromLSDyna,
  * Domain_d.C:1647-1699; its reader is an un-vendored submodule, so this
- * parser is this build's own).  Node ids map to indices in their order under
+ * parser is this library's own).  Node ids map to indices in their order under
  * *NODE; solids whose nodes 4..8 repeat node 4 are tetrahedra.  The parse
  * call fills sizes first (x, elnod NULL), then x[3N] and elnod[E*n]. */
 int wfx_parse_lsdyna_k(const char *path, int *node_count, int *elem_count, int *nodxelem, double *x, int *elnod);

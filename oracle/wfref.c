@@ -659,7 +659,7 @@ static double law_temperature(const wfref_domain *d, int e) {
 /* Calc*YieldStress (Material.cuh:353-364, :378-388, :412-441) of element e
  * at plastic strain ep and effective strain rate edot; the element
  * temperature is law_temperature(d, e) (the reference reads T[e], :1725-1729 of
- * Mechanical.C).  BILINEAR is this build's definition (the reference leaves
+ * Mechanical.C).  BILINEAR is this library's definition (the reference leaves
  * it undefined). */
 double wfref_yield_stress(const wfref_domain *d, int e, double ep, double edot) {
   const wfref_material *m = &d->mat;

@@ -403,7 +403,7 @@ def test_free_flight_translation_is_exact():
 def test_one_element_trajectory_matches_validation_log(case, hg):
     # 126 steps of the 0.1 m cube (src/common/main_1_elem_3d.C recipe).  The
     # logs come from an older code revision that evaluated the geometry after
-    # the position update; this build (and the current reference) lag it by one
+    # the position update; this library (and the current reference) lag it by one
     # step, worth about 1% here.  Tolerance: 3% of the largest lateral
     # displacement; the hourglass-free and hourglass cases differ by 100%.
     g = GOLDEN[case]

@@ -1,7 +1,7 @@
 // The on-disk formats either side of the step: LS-DYNA keyword meshes in
 // (*NODE, *ELEMENT_SOLID; what CreateFromLSDyna consumes, Domain_d.C:1647-1699)
 // and legacy VTK unstructured-grid frames out (the fields VTKWriter.C:365-625
-// writes that this build keeps).  Host code only.
+// writes that this library keeps).  Host code only.
 #include <cctype>
 #include <cstdio>
 #include <cstdlib>
