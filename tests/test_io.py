@@ -103,8 +103,8 @@ def test_sparse_ids_crlf_and_lowercase_keywords(tmp_path):
     lines = ["*keyword", "*node"]
     for k, i in enumerate(ids):
         lines.append(f"{i:8d}{float(k):16.9e}{-0.5 * k:16.9e}{1e-3 * k:16.9e}")
-    lines += ["*element_solid", "       1       1" + "".join(f"{i:8d}" for i in ids) + "".join(f"{ids[3]:8d}" * 4),
-              "*end"]
+    lines += ["   ", "*element_solid", "       1       1" + "".join(f"{i:8d}" for i in ids) +
+              "".join(f"{ids[3]:8d}" * 4), "\t", "*end"]  # blank lines are skipped
     p = tmp_path / "sparse.k"
     p.write_bytes(("\r\n".join(lines) + "\r\n").encode())
     x, el, n = parse_lsdyna_k(p)

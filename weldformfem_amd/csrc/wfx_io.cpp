@@ -131,7 +131,9 @@ int parse_k(const char *path, KMesh &m, std::string &err) {
     p = nl ? nl + 1 : end;
     ++lineno;
     if (e > b && e[-1] == '\r') --e;
-    if (e == b || *b == '$') continue;
+    const char *t = b;  // blank lines (spaces only) are skipped like empty ones
+    while (t < e && std::isspace((unsigned char)*t)) ++t;
+    if (t == e || *b == '$') continue;
     if (*b == '*') {
       const char *kb = b, *ke = e;
       trim(kb, ke);
