@@ -111,6 +111,25 @@ class Domain_d {
 
   // ---- init block and per-step API (Solver_explicit.C order) ----
   void InitSolve() { check(wfx_init(c_, s_)); step_ = 0; }
+  // The pieces of SolveChungHulbert's init block (Solver_explicit.C:101-340):
+  // InitValues (Domain_d.C:393-416) runs the whole block (wfx_init: zero
+  // state, BC velocities, Jacobians, vol0 = vol, rho = rho0, nodal volume and
+  // mass, sigma_y = sy0); the other three are part of it and do nothing here.
+  void InitValues() { InitSolve(); }
+  void AssignMatAddress() {}      // Domain_d.C:945-949
+  void CalcElemInitialVol() {}    // Mechanical.C:343-357
+  void calcElemDensity() {}       // Mechanical.C:295-319
+  // computeEnergies (Mechanical.C:2145-2185): kinetic energy and the internal
+  // energy of the step, dEint = dt sum sigma:D vol; the reference leaves Evisc
+  // unassigned, here it is 0
+  void computeEnergies(double dt, double &Ekin, double &dEint, double &Evisc) {
+    check(wfx_energies(c_, &Ekin, &dEint, s_));
+    if (dt != dt_ && dt_ > 0.0) dEint = dEint / dt_ * dt;  // wfx_energies uses the domain's dt
+    Evisc = 0.0;
+  }
+  // SearchExtNodes (Domain_d.C:110-296): the external faces are found when the
+  // rigid surface is set; this refreshes the nodal areas
+  void SearchExtNodes() { CalcExtFaceAreas(); }
   void UpdatePrediction() { check(wfx_update_prediction(c_, s_)); }
   void ImposeBCV(const int dim) { check(wfx_impose_bcv(c_, dim, s_)); }
   void ImposeBCA(const int dim) { check(wfx_impose_bca(c_, dim, s_)); }

@@ -19,7 +19,9 @@ def run_cpp(n, steps, fused):
     assert out.returncode == 0, out.stderr
     lines = out.stdout.strip().split("\n")
     hist = [float(l.split()[3]) for l in lines if l.startswith("step")]
+    energies = [[float(q) for q in l.split()[1:]] for l in lines if l.startswith("energies")]
     last = lines[-1].split()
+    run_cpp.energies = energies[0] if energies else None
     return hist, float(last[1]), float(last[3]), float(last[5])
 
 
@@ -39,3 +41,6 @@ def test_cpp_driver_matches_python(fused):
     u = d.field("u").reshape(-1, 3)
     assert u[:, 2].min() == pytest.approx(umin, rel=1e-12)
     assert d.field("pl_strain").max() == pytest.approx(epmax, rel=1e-12, abs=1e-300)
+    if not fused:  # computeEnergies through the C++ mirror
+        ek, de = d.energies()
+        assert run_cpp.energies == pytest.approx([ek, de], rel=1e-12)
