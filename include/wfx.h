@@ -37,6 +37,8 @@ enum {
 enum { WFX_FP64 = 0, WFX_FP32_MIXED = 1 };
 enum { WFX_BILINEAR = 0, WFX_HOLLOMON = 1, WFX_JOHNSON_COOK = 2, WFX_GMT = 3 };
 
+/* Error message and library version: no reference counterpart (the
+ * reference's methods return nothing and print their diagnostics). */
 const char *wfx_last_error(void);
 int wfx_version(void);
 
@@ -307,7 +309,9 @@ int wfx_write_vtk_async(wfx_ctx *ctx, const char *path, wfx_stream stream);
 int wfx_vtk_wait(wfx_ctx *ctx);
 
 /* ---- state transfer (reference layout on the host side, synchronous) ----- */
-/* names: x v a u u_dt prev_a fi fe mdiag voln elnod nodel nodel_loc
+/* Replace direct access to the Domain_d member arrays (Domain_d.h:841-1043,
+ * allocated in SetDimension, Domain_d.C:457-621) and their host mirrors.
+ * names: x v a u u_dt prev_a fi fe mdiag voln elnod nodel nodel_loc
  * nodel_offset nodel_count detJ dHdx dHdy dHdz vol vol0 rho rho0 p pl_strain
  * sigma_y sy0 str_rate rot_rate sigma tau eps strain_pl_incr f_elem f_hg.
  * Integer fields take int32 buffers, the rest double buffers. */
