@@ -313,7 +313,9 @@ int wfx_vtk_wait(wfx_ctx *ctx);
  * allocated in SetDimension, Domain_d.C:457-621) and their host mirrors.
  * names: x v a u u_dt prev_a fi fe mdiag voln elnod nodel nodel_loc
  * nodel_offset nodel_count detJ dHdx dHdy dHdz vol vol0 rho rho0 p pl_strain
- * sigma_y sy0 str_rate rot_rate sigma tau eps strain_pl_incr f_elem f_hg.
+ * sigma_y sy0 str_rate rot_rate sigma tau eps strain_pl_incr f_elem f_hg;
+ * thermal coupling: T q_plheat; contact: contforce ut_prev node_area cfn
+ * mesh_in_contact elem_length rigid_x rigid_normal rigid_pplane.
  * Integer fields take int32 buffers, the rest double buffers. */
 int wfx_field_count(wfx_ctx *ctx, const char *name, long *count, int *is_int);
 int wfx_get_field(wfx_ctx *ctx, const char *name, void *host, long count);
