@@ -38,7 +38,7 @@ def fused_bytes(n, s, hg=True, rho_uniform=True):
     u_rw = 2 * 3 * 8 if s == 8 else 0  # FP32-mixed recovers u from x at the end of a run
     return {
         "f1_volume": (n * I4 + s, 3 * 8),
-        "f2_nodal": (n * I4 + 2 * s, 2 * I4 + 8 + 3 * 8),
+        "f2_nodal": (n * I4 + 2 * s, 2 * I4 + 8 + 2 * 8),  # writes ratio and mass; V_n once per run
         "f3_element": (n * I4 + f3_reals * s, 3 * 8 + 3 * sn + 8),
         "f4_node": (n * I4 + 3 * n * s, 2 * I4 + 8 + I4 + 3 * sn + 2 * 3 * 8 + 3 * 2 * 3 * sn + u_rw),
     }

@@ -162,8 +162,8 @@ __global__ __launch_bounds__(BS) void f1_volume(int E, int N, const int *__restr
   }
 }
 
-// F2: nodal volume sum in CSR order; ratio for the ANP pressure, nodal
-// volume and nodal mass.
+// F2: nodal volume sum in CSR order; ratio for the ANP pressure and nodal
+// mass (the nodal volume field itself is written once per run, in phase 5).
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void f2_nodal(int N, int nodxelem, const int *__restrict__ off,
                                                    const int *__restrict__ cnt,
@@ -173,7 +173,6 @@ __global__ __launch_bounds__(BS) void f2_nodal(int N, int nodxelem, const int *_
                                                    const T *__restrict__ rho,
                                                    const double *__restrict__ vsum0,
                                                    double *__restrict__ ratio,
-                                                   double *__restrict__ voln,
                                                    double *__restrict__ mdiag, double rho_u) {
   int i = gtid();
   if (i >= N) return;
@@ -205,7 +204,6 @@ __global__ __launch_bounds__(BS) void f2_nodal(int N, int nodxelem, const int *_
   ratio[i] = s / vsum0[i];
   double vn = s;
   vn /= nodxelem;
-  voln[i] = vn;
   double dc = (double)c, mass = 0.0;
   for (int k = 0; k < c; k += kChunk) {
     double rr[kChunk];
@@ -1061,8 +1059,7 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
   } else if (phase == 2) {
     hipLaunchKernelGGL((f2_nodal<T, BS>), dim3(gN), dim3(BS), 0, s, N, c->n, c->ms.nodel_offset,
                        c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.vol, el.rho, c->nd.voln0_sum,
-                       c->nd.anp_ratio,
-                       c->nd.voln, c->nd.mdiag, c->rho_uniform);
+                       c->nd.anp_ratio, c->nd.mdiag, c->rho_uniform);
   } else if (phase == 3) {
     FusedParams fp;
     fp.dt = P.dt;
@@ -1104,6 +1101,10 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
     if (c->contact.on && step >= 0) return launch_rigid_update(c, cfl, s);
   } else if (phase == 5) {
     hipLaunchKernelGGL((f_sigma<T>), dim3(nblocks(E)), dim3(kBlock), 0, s, E, el.p, el.tau, el.sigma);
+    // the nodal volumes no step reads, once per run: the same CSR-order sum
+    // and division as CalcNodalVol, so the field equals the per-step API's
+    hipLaunchKernelGGL((k_nodal_vol<T>), dim3(nblocks(N)), dim3(kBlock), 0, s, N, c->ms.nodel_offset,
+                       c->ms.nodel_count, c->ms.nodel, el.vol, c->nd.voln, c->n);
   } else {
     return hipErrorInvalidValue;
   }
