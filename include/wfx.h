@@ -171,10 +171,20 @@ int wfx_get_dt(wfx_ctx *ctx, double *dt);
 int wfx_dt_history(wfx_ctx *ctx, double *out, int count);
 /* m_alpha, m_beta, m_gamma (Solver_explicit.C:193-197); defaults rho_b=0.8182 */
 int wfx_set_ch(wfx_ctx *ctx, double alpha, double beta, double gamma);
-/* Domain_d::m_stab and m_artifvisc (Domain_d.h:283-294, main.C:84-120) and the
- * 3D hexa hourglass coefficient ("hg_coeff").  Names: alpha_free, log_factor,
- * J_min, hg_coeff_free, pspg_scale, p_pspg_bulkfac, av_coeff_div,
- * av_coeff_bulk, artifvisc0, artifvisc1, hg_coeff. */
+/* Domain_d::m_stab and m_artifvisc (Domain_d.h:139-147, :283-294, main.C:84-120,
+ * :293-298) and the 3D hexa hourglass coefficient ("hg_coeff").  Names:
+ * alpha_free, alpha_contact, log_factor, J_min, hg_coeff_free,
+ * hg_coeff_contact, pspg_scale, p_pspg_bulkfac, av_coeff_div, av_coeff_bulk,
+ * artifvisc0, artifvisc1, hg_coeff (all default 0).  With contact on, an
+ * element with a node carrying a contact force uses alpha_contact,
+ * hg_coeff_contact and the mean of the two viscous pressure terms
+ * (calcElemPressure, Mechanical.C:729-799).  artifvisc0/1 add Wilkins'
+ * artificial viscosity to the diagonal stresses after the stress update
+ * (calcArtificialViscosity, Mechanical.C:1948-1977), on every path.
+ * "check_status" (1 by default, no reference counterpart): wfx_run,
+ * wfx_step_fused and wfx_graph_launch read the status counters at their end
+ * (synchronising) and return WFX_ERR_NONFINITE when they grew; 0 keeps those
+ * calls asynchronous and leaves the counters to the query calls. */
 int wfx_set_param(wfx_ctx *ctx, const char *name, double value);
 int wfx_get_param(wfx_ctx *ctx, const char *name, double *value);
 /* nodes whose internal force is summed by wfx_reaction_force (no counterpart
@@ -224,7 +234,11 @@ int wfx_update_correction_pos(wfx_ctx *ctx, wfx_stream s);      /* Domain_d.C:10
  * in the reference order (Solver_explicit.C:524-978). */
 int wfx_step_api(wfx_ctx *ctx, wfx_stream s);
 /* nsteps loop bodies; reaction-force history (length nsteps, host memory,
- * may be NULL) is copied back at the end. */
+ * may be NULL) is copied back at the end.  Returns WFX_ERR_NONFINITE (after
+ * completing the steps, as the reference's loop goes on) when an element
+ * Jacobian had detJ <= 0 or an internal force was not finite during the run
+ * (see "check_status" under wfx_set_param); the same for wfx_step_fused and
+ * wfx_graph_launch. */
 int wfx_run(wfx_ctx *ctx, int nsteps, double *reaction_hist, wfx_stream s);
 /* nsteps loop bodies of the same sequence regrouped into four fused kernels
  * per step (two element passes, two node passes).  Same state on return as
@@ -279,8 +293,13 @@ int wfx_adjoint_backward(wfx_ctx *ctx, const double *reaction_hist_bar, double *
 int wfx_adjoint_release(wfx_ctx *ctx);
 /* Domain_d::computeEnergies (Mechanical.C:2145-2185), synchronous. */
 int wfx_energies(wfx_ctx *ctx, double *ekin, double *deint, wfx_stream s);
-/* number of non-finite internal-force entries scrubbed so far */
+/* number of non-finite internal-force entries scrubbed so far (the scrub of
+ * Solver_explicit.C:777-785) */
 int wfx_nonfinite_count(wfx_ctx *ctx, long *out);
+/* number of element Jacobians found with detJ <= 0 or not finite so far, one
+ * count per element and step (the inverted-element check of
+ * Solver_explicit.C:639-652, which prints them) */
+int wfx_inverted_count(wfx_ctx *ctx, long *out);
 /* Bandwidth probe, no reference counterpart: copies bytes (a positive
  * multiple of 16, both pointers 16-byte aligned, device memory) from src to
  * dst on stream s with 16-byte non-temporal accesses; the benchmark times it

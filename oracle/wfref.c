@@ -486,6 +486,11 @@ void wfref_calc_elem_jacobians(wfref_domain *d) {
       }
     }
     d->detJ[e] = det33(J);
+    /* inverted or degenerate (the check of Solver_explicit.C:639-652) */
+    if (!(d->detJ[e] > 0.0)) {
+#pragma omp atomic
+      d->inverted++;
+    }
     for (int a = 0; a < n; ++a) {
       d->dHdx[e * n + a] = dH[0][a];
       d->dHdy[e * n + a] = dH[1][a];
@@ -572,9 +577,13 @@ void wfref_calc_elem_strain_rates(wfref_domain *d) {
 static double dmin(double a, double b) { return (b < a) ? b : a; } /* std::min */
 static double dmax(double a, double b) { return (a < b) ? b : a; } /* std::max */
 
-/* calcElemPressure (Mechanical.C:691-819), press_algorithm 0, no contact.
+/* calcElemPressure (Mechanical.C:691-819), press_algorithm 0.
  * Node part :698-705, element part :720-815.  With the default (all zero)
- * stabilisation this is p = -K * (mean over element nodes of V_n/V_n0 - 1). */
+ * stabilisation this is p = -K * (mean over element nodes of V_n/V_n0 - 1).
+ * With contact on, an element with a node whose contact force (the previous
+ * step's, contforce is never reset) has a positive squared length uses
+ * alpha_contact and hg_coeff_contact and p_q = (q1 + q2) / 2 (:729-745,
+ * :757, :779, :792-799). */
 void wfref_calc_elem_pressure(wfref_domain *d) {
   int N = d->node_count, n = d->nodxelem;
   double *voln0 = (double *)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
@@ -595,13 +604,23 @@ void wfref_calc_elem_pressure(wfref_domain *d) {
     double vol0 = d->vol0[e], vol1 = d->vol[e];
     double J_local = vol1 / vol0;
     double h = pow(vol1, 1.0 / 3.0);
+    int is_contact = 0;
+    if (d->contact) {
+      for (int a = 0; a < n; ++a) {
+        const double *cf = d->contforce + 3 * d->elnod[e * n + a];
+        if (cf[0] * cf[0] + cf[1] * cf[1] + cf[2] * cf[2] > 0) {
+          is_contact = 1;
+          break;
+        }
+      }
+    }
     double J_avg = 0.0;
     for (int a = 0; a < n; ++a) {
       int nid = d->elnod[e * n + a];
       J_avg += voln[nid] / voln0[nid];
     }
     J_avg /= n;
-    double alpha = d->alpha_free;
+    double alpha = is_contact ? d->alpha_contact : d->alpha_free;
     double J_bar = alpha * J_local + (1 - alpha) * J_avg;
     if (J_bar < d->J_min) J_bar = 0.2;
     double p_physical = -K * (d->log_factor * log(J_bar) + (1.0 - d->log_factor) * (J_bar - 1.0));
@@ -614,14 +633,14 @@ void wfref_calc_elem_pressure(wfref_domain *d) {
                d->dHdz[e * n + a] * d->v[3 * nid + 2];
     }
     double p_pspg = 0.0;
-    double p_hg = d->hg_coeff_free * K * fabs(J_local - J_avg);
+    double p_hg = (is_contact ? d->hg_coeff_contact : d->hg_coeff_free) * K * fabs(J_local - J_avg);
     double p_q = 0.0;
     if (div_v < 0.0) {
       p_pspg = dmin(d->pspg_scale * tau * div_v * K, d->p_pspg_bulkfac * K);
       double q1 = d->av_coeff_div * rho_e * h * c * (-div_v);
       double delta_J = 1.0 - J_local;
       double q2 = d->av_coeff_bulk * K * delta_J;
-      p_q = dmax(q1, q2);
+      p_q = is_contact ? 0.5 * (q1 + q2) : dmax(q1, q2);
     }
     d->p[e] = p_physical + p_pspg + p_hg + p_q;
   }
@@ -940,6 +959,7 @@ void wfref_init(wfref_domain *d) {
     wfref_impose_bcv(d, c);
   }
   wfref_calc_elem_jacobians(d);
+  d->inverted = 0; /* counted from the first step on */
   wfref_calc_elem_vol(d);
   memcpy(d->vol0, d->vol, sizeof(double) * (size_t)d->elem_count);
   wfref_calc_elem_vol(d);
@@ -1354,6 +1374,7 @@ int wfref_get_int(wfref_domain *d, const char *name) {
   if (!strcmp(name, "elem_count")) return d->elem_count;
   if (!strcmp(name, "nodxelem")) return d->nodxelem;
   if (!strcmp(name, "nonfinite_fi")) return d->nonfinite_fi;
+  if (!strcmp(name, "inverted")) return d->inverted;
   return -1;
 }
 
@@ -1374,6 +1395,7 @@ int wfref_set_double(wfref_domain *d, const char *name, double v) {
   struct { const char *nm; double *p; } t[] = {
       {"dt", &d->dt}, {"alpha_free", &d->alpha_free}, {"log_factor", &d->log_factor},
       {"J_min", &d->J_min}, {"hg_coeff_free", &d->hg_coeff_free},
+      {"alpha_contact", &d->alpha_contact}, {"hg_coeff_contact", &d->hg_coeff_contact},
       {"pspg_scale", &d->pspg_scale}, {"p_pspg_bulkfac", &d->p_pspg_bulkfac},
       {"av_coeff_div", &d->av_coeff_div}, {"av_coeff_bulk", &d->av_coeff_bulk},
       {"artifvisc0", &d->artifvisc[0]}, {"artifvisc1", &d->artifvisc[1]},

@@ -101,9 +101,11 @@ typedef struct {
   /* stabilisation (Domain_d.h:283-294 defaults are all zero) */
   double alpha_free, log_factor, J_min, hg_coeff_free, pspg_scale,
          p_pspg_bulkfac, av_coeff_div, av_coeff_bulk;
+  double alpha_contact, hg_coeff_contact; /* elements with a node in contact */
   double artifvisc[2];
   double hg_coeff;    /* 3D hexa viscous hourglass coefficient (0 = none) */
   int nonfinite_fi;   /* count of scrubbed non-finite internal forces      */
+  int inverted;       /* count of Jacobians with detJ <= 0 or not finite   */
 } wfref_domain;
 
 wfref_domain *wfref_create(void);

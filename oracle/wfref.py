@@ -257,6 +257,10 @@ class RefDomain:
     def nonfinite_fi(self):
         return self._L.wfref_get_int(self._d, b"nonfinite_fi")
 
+    @property
+    def inverted(self):
+        return self._L.wfref_get_int(self._d, b"inverted")
+
     def field(self, name, copy=True):
         cnt = ctypes.c_long()
         isint = ctypes.c_int()

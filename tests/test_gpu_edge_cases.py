@@ -93,16 +93,20 @@ def test_nonfinite_velocity_is_scrubbed_like_the_reference(path):
         v = d.field("v")
         v[3 * k] = np.nan
         d.setField("v", v)
-    if path == "api":
-        g.run(2)
-    else:
-        g.runFused(2)
+    # the run completes and reports the scrubbed entries and the Jacobians
+    # the NaN position made non-finite
+    with pytest.raises(WfxError, match="non-finite internal-force entries"):
+        if path == "api":
+            g.run(2)
+        else:
+            g.runFused(2)
     r.run(2)
     # the NaN reaches the pressure of every element through the nodal volume
     # ratios within two steps; plastic strain stays finite (a NaN trial
     # stress never exceeds the yield stress) and the scrubbed forces are finite
     assert r.nonfinite_fi > 0
     assert g.nonfiniteCount() == r.nonfinite_fi
+    assert r.inverted > 0 and g.invertedCount() == r.inverted
     for f in ["u", "v", "fi", "p", "pl_strain", "sigma"]:
         a, b = g.field(f), r.field(f)
         np.testing.assert_array_equal(np.isnan(a), np.isnan(b), err_msg=f)

@@ -440,3 +440,189 @@ def test_cylinder_counts():
     x, el, bottom, top = workloads.cylinder_tets()
     assert el.shape == (1239552, 4) and x.shape == (218205, 3)
     assert bottom.size == top.size
+
+
+# ---------------------------------------------------------------- artificial viscosity
+def _av_element(al, be, D6, rho=7850.0):
+    d = one_element()
+    _, _, vol = geometry(d)
+    d.setField("rho", [rho])
+    d.set("artifvisc0", al)
+    d.set("artifvisc1", be)
+    d.setField("str_rate", D6)
+    s0 = np.array([0.0, 0.0, 0.0, 4.0e6, 5.0e6, 6.0e6])  # diagonal 0: the difference is q exactly
+    d.setField("sigma", s0)
+    d.calcArtificialViscosity()
+    return d.field("sigma") - s0, vol[0]
+
+
+@pytest.mark.parametrize("trace_sign", [-1.0, 1.0])
+def test_artificial_viscosity_closed_form(trace_sign):
+    # Wilkins' viscosity on one element with uniform volumetric rate tr D:
+    # q = a c |tr D| l + b (tr D l)^2, c = sqrt(K/rho), l = V^(1/3), added to
+    # the three diagonal stresses with the sign opposite to tr D, shear
+    # stresses untouched.  Catches a dropped length scale, the mean instead of
+    # the trace, a flipped sign and a term applied to the shear stresses.
+    K = 200e9 / (3 * (1 - 0.6))
+    rho = 7850.0
+    c = math.sqrt(K / rho)
+    D6 = trace_sign * np.array([30.0, 20.0, 10.0, 7.0, -3.0, 5.0])  # tr D = 60 (+-), shear rates ignored
+    tr = D6[:3].sum()
+    for al, be in [(0.1, 0.0), (0.0, 0.05), (0.1, 0.05)]:
+        dsig, vol = _av_element(al, be, D6)
+        l = vol ** (1.0 / 3.0)
+        assert l == pytest.approx(0.1, rel=1e-14)  # a 0.1 m cube
+        q = al * c * abs(tr) * l + be * (tr * l) ** 2
+        want = -math.copysign(q, tr)
+        np.testing.assert_allclose(dsig[:3], want, rtol=1e-13)
+        np.testing.assert_array_equal(dsig[3:], 0.0)
+        # the reference's sign makes the added stress power negative:
+        # delta(sigma : D) = -q |tr D|
+        assert dsig[:3].sum() / 3 * tr == pytest.approx(-q * abs(tr), rel=1e-13)
+
+
+def test_artificial_viscosity_cap_and_threshold():
+    # q is capped at 1e9 Pa; |tr D| <= 1e-12 adds nothing
+    dsig, _ = _av_element(1.0e3, 0.0, np.array([-3.0e3, 0, 0, 0, 0, 0]))
+    np.testing.assert_allclose(dsig[:3], 1.0e9, rtol=0)
+    dsig, _ = _av_element(0.5, 0.5, np.array([4e-13, 4e-13, 2e-13, 1.0, 1.0, 1.0]))
+    np.testing.assert_array_equal(dsig, 0.0)
+    dsig, _ = _av_element(0.5, 0.0, np.array([4e-13, 4e-13, 3e-13, 0, 0, 0]))
+    assert np.all(dsig[:3] < 0)  # 1.1e-12 > 1e-12: applied (tension)
+
+
+# ---------------------------------------------------------------- energies, density, assembly
+def test_kinetic_energy_of_a_translation():
+    # Ekin = 1/2 M |v|^2 with M = rho * total volume (lumped mass sums to it)
+    d, _ = jittered(3, 2, 2, 0.1, False, seed=21)
+    material(d)
+    d.init()
+    vt = np.array([0.3, -1.2, 2.5])
+    d.setField("v", np.tile(vt, d.node_count))
+    ek, _ = d.energies()
+    M = 7850.0 * d.field("vol").sum()
+    assert ek == pytest.approx(0.5 * M * vt.dot(vt), rel=1e-12)
+
+
+def _energy_state(Lg, s6, seed):
+    d, x = jittered(3, 3, 2, 0.1, True, seed=seed)
+    material(d)
+    d.init()
+    d.SetDT(1.0e-6)
+    d.setField("v", (x @ Lg.T).reshape(-1))
+    geometry(d)
+    d.calcElemStrainRates()
+    E = d.elem_count
+    d.setField("sigma", np.tile(s6, E))
+    d.calcElemForces()
+    d.setField("f_hg", np.zeros(3 * d.nodxelem * E))
+    d.assemblyForces()
+    _, de = d.energies()
+    power = float(np.dot(d.field("fi"), d.field("v")))
+    return de / 1.0e-6, power
+
+
+def test_internal_energy_rate_equals_force_power():
+    # dEint/dt = sum_e sigma : D vol_e = sum_nodes f_int . v for a linear
+    # field with a diagonal gradient (no shear): ties computeEnergies to the
+    # element forces and the assembly
+    s6 = np.array([-2.0e8, 1.0e8, 5.0e7, 3.0e7, -2.0e7, 1.0e7])
+    rate, power = _energy_state(np.diag([1.5, -0.7, 0.4]), s6, seed=22)
+    assert rate == pytest.approx(power, rel=1e-11)
+
+
+def test_internal_energy_rate_counts_shear_once():
+    # computeEnergies sums s3*e3 + s4*e4 + s5*e5 once (Mechanical.C:2166-2170),
+    # where sigma : D counts each shear pair twice: for a pure xy shear field
+    # and a pure xy stress its rate is half the force power
+    s6 = np.array([0.0, 0.0, 0.0, 8.0e7, 0.0, 0.0])
+    Lg = np.zeros((3, 3))
+    Lg[0, 1] = Lg[1, 0] = 2.0
+    rate, power = _energy_state(Lg, s6, seed=23)
+    assert power != 0.0
+    assert rate == pytest.approx(0.5 * power, rel=1e-11)
+
+
+def test_density_conserves_element_mass():
+    # rho vol = rho0 vol0 after the mesh deforms (calcElemDensity)
+    d, x = jittered(3, 2, 2, 0.1, False, seed=24)
+    material(d)
+    d.init()
+    vol0 = d.field("vol")
+    rng = np.random.default_rng(7)
+    d.setField("x", (x + rng.uniform(-0.01, 0.01, x.shape)).reshape(-1))
+    d.calcElemJAndDerivatives()
+    d.CalcElemVol()
+    d.calcElemDensity()
+    vol = d.field("vol")
+    assert np.max(np.abs(vol / vol0 - 1)) > 1e-3  # the volumes did change
+    np.testing.assert_allclose(d.field("rho") * vol, 7850.0 * vol0, rtol=1e-14)
+
+
+@pytest.mark.parametrize("tet", [False, True])
+def test_assembly_matches_element_scatter(tet):
+    # fi(node) = sum of f_elem - sum of f_hg over the node's element-nodes, by
+    # an element scatter (np.add.at) instead of the node->element CSR: a
+    # wrong local index or element in the CSR walk fails it
+    d, _ = jittered(3, 3, 2, 0.1, tet, seed=25)
+    n, E = d.nodxelem, d.elem_count
+    rng = np.random.default_rng(8)
+    fe = rng.uniform(-1, 1, (E, n, 3))
+    fh = rng.uniform(-1, 1, (E, n, 3)) * 0.1
+    d.setField("f_elem", fe.reshape(-1))
+    d.setField("f_hg", fh.reshape(-1))
+    d.assemblyForces()
+    el = d.field("elnod").reshape(E, n)
+    want = np.zeros((d.node_count, 3))
+    np.add.at(want, el.reshape(-1), (fe - fh).reshape(-1, 3))
+    np.testing.assert_allclose(d.field("fi").reshape(-1, 3), want, rtol=1e-13, atol=1e-13)
+
+
+# ---------------------------------------------------------------- pressure, contact branch
+def test_anp_pressure_contact_blend():
+    # with contact on, the elements around a node carrying a contact force use
+    # alpha_contact (J_bar = J_local at 1) and hg_coeff_contact; the others
+    # the free values (J_bar = J_avg at alpha_free = 0)
+    d = box(2, 2, 2, 0.1, tet=True)
+    material(d)
+    x = d.field("x").reshape(-1, 3)
+    tris = np.array([[0, 1, 2]], dtype=np.int32)
+    d.setRigidMesh(np.array([[-1.0, -1.0, 5.0], [1.0, -1.0, 5.0], [0.0, 1.0, 5.0]]), tris)
+    _, _, vol = geometry(d)
+    rng = np.random.default_rng(11)
+    J = rng.uniform(0.98, 1.02, vol.size)
+    d.setField("vol0", vol / J)
+    k = int(np.argmax(x[:, 2]))  # a top node
+    cf = np.zeros(3 * d.node_count)
+    cf[3 * k + 2] = -5.0
+    d.setField("contforce", cf)
+    for name, v in dict(alpha_free=0.0, alpha_contact=1.0, hg_coeff_free=0.0, hg_coeff_contact=0.5).items():
+        d.set(name, v)
+    d.calcElemPressure()
+    p = d.field("p")
+    K = 200e9 / (3 * (1 - 0.6))
+    el = d.field("elnod").reshape(-1, 4)
+    S = np.zeros(d.node_count)
+    S0 = np.zeros(d.node_count)
+    np.add.at(S, el.reshape(-1), np.repeat(vol, 4))
+    np.add.at(S0, el.reshape(-1), np.repeat(vol / J, 4))
+    J_avg = (S[el] / S0[el]).mean(axis=1)
+    touch = np.any(el == k, axis=1)
+    assert 0 < touch.sum() < el.shape[0]
+    want = np.where(touch, -K * (J - 1.0) + 0.5 * K * np.abs(J - J_avg), -K * (J_avg - 1.0))
+    np.testing.assert_allclose(p, want, rtol=1e-9, atol=1e-3)
+
+
+def test_inverted_elements_are_counted():
+    # detJ <= 0 after a node is pushed through its element's opposite face
+    d = box(2, 2, 2, 0.1)
+    material(d)
+    d.init()
+    assert d.inverted == 0
+    x = d.field("x").reshape(-1, 3)
+    k = int(np.argmax(x[:, 2] + x[:, 1] + x[:, 0]))  # the top corner, one element
+    x[k] -= 0.25
+    d.setField("x", x.reshape(-1))
+    d.calcElemJAndDerivatives()
+    det = d.field("detJ")
+    assert d.inverted == int(np.sum(det <= 0)) >= 1

@@ -99,6 +99,7 @@ SIGNATURES = {
     "wfx_reaction_force": (_i, [_vp, _i, _pd, _vp]),
     "wfx_energies": (_i, [_vp, _pd, _pd, _vp]),
     "wfx_nonfinite_count": (_i, [_vp, ctypes.POINTER(ctypes.c_long)]),
+    "wfx_inverted_count": (_i, [_vp, ctypes.POINTER(ctypes.c_long)]),
     "wfx_stream_copy": (_i, [_vp, _vp, ctypes.c_long, _vp]),
     "wfx_field_count": (_i, [_vp, _cp, ctypes.POINTER(ctypes.c_long), _pi]),
     "wfx_get_field": (_i, [_vp, _cp, _vp, ctypes.c_long]),

@@ -97,7 +97,7 @@ static hipError_t alloc_elem(wfx_ctx *c, DevElem<T> &el) {
   // f_elem is padded to whole 64-element tiles (the fused path may tile it);
   // FP32-mixed fused runs keep 4 floats per element-node there (float4)
   A(eps, 6 * E) A(strain_pl_incr, 6 * E) A(f_elem, (std::is_same<T, float>::value ? 4 : 3) * n * ((E + 63) / 64 * 64))
-  A(f_hg, 3 * n * E)
+  A(f_hg, 3 * n * E) A(avq, E)
 #undef A
   return e;
 }
@@ -843,7 +843,8 @@ static double *param_slot(wfx_ctx *c, const char *name) {
   Stab &s = c->prm.stab;
   struct { const char *nm; double *p; } t[] = {
       {"alpha_free", &s.alpha_free}, {"log_factor", &s.log_factor}, {"J_min", &s.J_min},
-      {"hg_coeff_free", &s.hg_coeff_free}, {"pspg_scale", &s.pspg_scale},
+      {"hg_coeff_free", &s.hg_coeff_free}, {"alpha_contact", &s.alpha_contact},
+      {"hg_coeff_contact", &s.hg_coeff_contact}, {"pspg_scale", &s.pspg_scale},
       {"p_pspg_bulkfac", &s.p_pspg_bulkfac}, {"av_coeff_div", &s.av_coeff_div},
       {"av_coeff_bulk", &s.av_coeff_bulk}, {"artifvisc0", &s.artifvisc0},
       {"artifvisc1", &s.artifvisc1}, {"hg_coeff", &s.hg_coeff}, {"dt", &c->prm.dt},
@@ -856,6 +857,11 @@ static double *param_slot(wfx_ctx *c, const char *name) {
 int wfx_set_param(wfx_ctx *c, const char *name, double value) {
   REQUIRE_CTX(c);
   REQUIRE(name, WFX_ERR_ARG, "null name");
+  if (!std::strcmp(name, "check_status")) {
+    REQUIRE(value == 0.0 || value == 1.0, WFX_ERR_ARG, "check_status is 0 or 1");
+    c->check_status = (int)value;
+    return WFX_OK;
+  }
   double *p = param_slot(c, name);
   REQUIRE(p, WFX_ERR_ARG, std::string("unknown parameter ") + name);
   *p = value;
@@ -871,6 +877,7 @@ int wfx_get_param(wfx_ctx *c, const char *name, double *value) {
   if (s == "G") { *value = m.G; return WFX_OK; }
   if (s == "cs0") { *value = m.cs0; return WFX_OK; }
   if (s == "H") { *value = m.H; return WFX_OK; }
+  if (s == "check_status") { *value = c->check_status; return WFX_OK; }
   double *p = param_slot(c, name);
   REQUIRE(p, WFX_ERR_ARG, std::string("unknown parameter ") + name);
   *value = *p;
@@ -963,6 +970,7 @@ int wfx_init(wfx_ctx *c, wfx_stream s) {
   LAUNCH(BY_PRECISION(c, launch_nodal_vol, c, st));
   LAUNCH(BY_PRECISION(c, launch_nodal_mass, c, st));
   WFX_CHECK_HIP(hipMemsetAsync(c->status, 0, 16 * sizeof(unsigned int), st));
+  c->status_seen[0] = c->status_seen[1] = 0;
   if (c->contact.on) {
     // contact state starts clear; calcMinEdgeLength before the solve
     // (main.C:862) gives the element heights
@@ -1055,6 +1063,25 @@ int wfx_calc_cfl_dt(wfx_ctx *c, double *dt, wfx_stream s) {
   if (rc) return rc;
   if (dt) *dt = c->prm.dt;
   return WFX_OK;
+}
+
+// End of a run: the device counters of non-finite internal forces (scrubbed
+// to zero, Solver_explicit.C:777-785) and of elements with detJ <= 0 or not
+// finite (Solver_explicit.C:639-652).  The run itself completes, as the
+// reference's loop prints and goes on; WFX_ERR_NONFINITE reports that either
+// counter grew during it.  Synchronises the stream (off with check_status 0).
+static int check_run_status(wfx_ctx *c, hipStream_t st) {
+  if (!c->check_status) return WFX_OK;
+  unsigned int h[2] = {0, 0};
+  WFX_CHECK_HIP(hipMemcpyAsync(h, c->status, sizeof(h), hipMemcpyDeviceToHost, st));
+  WFX_CHECK_HIP(hipStreamSynchronize(st));
+  const unsigned int nf = h[kStatNonfinite] - c->status_seen[0], inv = h[kStatInverted] - c->status_seen[1];
+  c->status_seen[0] = h[kStatNonfinite];
+  c->status_seen[1] = h[kStatInverted];
+  if (nf == 0 && inv == 0) return WFX_OK;
+  set_error("run completed with " + std::to_string(inv) + " element Jacobians with detJ <= 0 or not finite and " +
+            std::to_string(nf) + " non-finite internal-force entries set to zero");
+  return WFX_ERR_NONFINITE;
 }
 
 static int step_api(wfx_ctx *c, hipStream_t st, double *hist) {
@@ -1150,7 +1177,7 @@ int wfx_run(wfx_ctx *c, int nsteps, double *reaction_hist, wfx_stream s) {
     WFX_CHECK_HIP(hipMemcpyAsync(reaction_hist, dh, (size_t)nsteps * ng * sizeof(double), hipMemcpyDeviceToHost, st));
     WFX_CHECK_HIP(hipStreamSynchronize(st));
   }
-  return WFX_OK;
+  return check_run_status(c, st);
 }
 
 static int enqueue_fused(wfx_ctx *c, int nsteps, double *dev_hist, hipStream_t st);
@@ -1181,7 +1208,7 @@ int wfx_step_fused(wfx_ctx *c, int nsteps, double *reaction_hist, wfx_stream s) 
     WFX_CHECK_HIP(hipMemcpyAsync(reaction_hist, dh, (size_t)nsteps * ng * sizeof(double), hipMemcpyDeviceToHost, st));
     WFX_CHECK_HIP(hipStreamSynchronize(st));
   }
-  return WFX_OK;
+  return check_run_status(c, st);
 }
 
 // ---- specimen batches, autotuning, graph capture --------------------------
@@ -1419,7 +1446,8 @@ int wfx_graph_launch(wfx_ctx *c, double *reaction_hist, wfx_stream s) {
                                  hipMemcpyDeviceToHost, st));
     WFX_CHECK_HIP(hipStreamSynchronize(st));
   }
-  return sync_cfl_dt(c, c->graph_steps, st);
+  int rc = sync_cfl_dt(c, c->graph_steps, st);
+  return rc ? rc : check_run_status(c, st);
 }
 
 int wfx_fused_phase(wfx_ctx *c, int phase, int predict, wfx_stream s) {
@@ -1468,7 +1496,16 @@ int wfx_nonfinite_count(wfx_ctx *c, long *out) {
   REQUIRE_MESH(c);
   REQUIRE(out, WFX_ERR_ARG, "null output");
   unsigned int h = 0;
-  WFX_CHECK_HIP(hipMemcpy(&h, c->status, sizeof(unsigned int), hipMemcpyDeviceToHost));
+  WFX_CHECK_HIP(hipMemcpy(&h, c->status + kStatNonfinite, sizeof(unsigned int), hipMemcpyDeviceToHost));
+  *out = h;
+  return WFX_OK;
+}
+
+int wfx_inverted_count(wfx_ctx *c, long *out) {
+  REQUIRE_MESH(c);
+  REQUIRE(out, WFX_ERR_ARG, "null output");
+  unsigned int h = 0;
+  WFX_CHECK_HIP(hipMemcpy(&h, c->status + kStatInverted, sizeof(unsigned int), hipMemcpyDeviceToHost));
   *out = h;
   return WFX_OK;
 }

@@ -108,7 +108,8 @@ __global__ __launch_bounds__(kBlock) void f0_vsum0(int N, const int *__restrict_
 template <typename T, int NN, int BS>
 __global__ __launch_bounds__(BS) void f1_volume(int E, int N, const int *__restrict__ elnod,
                                                     const double *__restrict__ x,
-                                                    T *__restrict__ vol, double *__restrict__ minl_part) {
+                                                    T *__restrict__ vol, double *__restrict__ minl_part,
+                                                    unsigned int *__restrict__ status) {
   int e = gtid();
   double medge = kMinEdgeInit;
   if (e < E) {
@@ -154,6 +155,7 @@ __global__ __launch_bounds__(BS) void f1_volume(int E, int N, const int *__restr
   T v = T(0);
   v += det * w * T(1);
   vol[e] = v;
+  count_inverted(det, status);
   if (minl_part) medge = min_edge4<T>(*reinterpret_cast<const T(*)[4][3]>(&X[0][0]));
   }
   if (minl_part) {
@@ -228,7 +230,9 @@ struct FusedParams {
   double dt;
   MatParams mp;
   PressParams pp;     // pp.full handled as in k_pressure_elem
-  int need_jlocal;    // alpha_free or hg_coeff_free nonzero: J_local needed
+  int need_jlocal;    // J_local = vol/vol0 enters the pressure
+  double av_al, av_be;  // artificial viscosity (calcArtificialViscosity); both zero: none
+  void *avq;          //   its diagonal stress term per element (T[E]), for the sigma field
   double hg_coeff, cs0;
   double rho_u;       // > 0: every element density equals it (no density loads)
   const double *dtp;  // non-null: the step's dt from the device (per-step CFL time step)
@@ -441,39 +445,23 @@ __device__ __forceinline__ void element_chain(
   }
   // ---- ANP pressure (calcElemPressure element part) ----------------------
   const PressParams &pp = fp.pp;
+  const bool is_contact = elem_in_contact<NN>(pp, [&](int a) { return g[a]; });
   double J_avg = 0.0;
 #pragma unroll
   for (int a = 0; a < NN; ++a) J_avg += ratio[g[a]];
   J_avg /= NN;
   double J_local = 0.0;
-  if (fp.need_jlocal || pp.full) J_local = (double)volv / (double)vol0[e];
-  double al = pp.alpha_free;
-  double J_bar = al * J_local + (1 - al) * J_avg;
-  if (J_bar < pp.J_min) J_bar = 0.2;
-  double lf = pp.log_factor;
-  double lg = (lf != 0.0) ? log(J_bar) : 0.0;
-  double p_physical = -pp.K * (lf * lg + (1.0 - lf) * (J_bar - 1.0));
-  double p_pspg = 0.0, p_q = 0.0;
-  double p_hg = pp.hg_coeff_free * pp.K * fabs(J_local - J_avg);
-  if (pp.full) {
-    double rho_e = (double)rho[e];
-    double h = pow((double)volv, 1.0 / 3.0);
-    double c = sqrt(pp.K / rho_e);
-    double tau = h / (2.0 * c);
-    double div_v = 0.0;
+  if (fp.need_jlocal) J_local = (double)volv / (double)vol0[e];
+  const T pe = (T)anp_elem_pressure(
+      pp, is_contact, J_local, J_avg, (double)volv, [&] { return (double)rho[e]; },
+      [&] {
+        double div_v = 0.0;
 #pragma unroll
-    for (int a = 0; a < NN; ++a)
-      div_v += (double)shape_dH<T, NN>(C, 0, a) * v[g[a]] + (double)shape_dH<T, NN>(C, 1, a) * v[(long)N + g[a]] +
-               (double)shape_dH<T, NN>(C, 2, a) * v[2L * N + g[a]];
-    if (div_v < 0.0) {
-      double x1 = pp.pspg_scale * tau * div_v * pp.K, x2 = pp.p_pspg_bulkfac * pp.K;
-      p_pspg = (x2 < x1) ? x2 : x1;
-      double q1 = pp.av_coeff_div * rho_e * h * c * (-div_v);
-      double q2 = pp.av_coeff_bulk * pp.K * (1.0 - J_local);
-      p_q = (q1 < q2) ? q2 : q1;
-    }
-  }
-  T pe = (T)(p_physical + p_pspg + p_hg + p_q);
+        for (int a = 0; a < NN; ++a)
+          div_v += (double)shape_dH<T, NN>(C, 0, a) * v[g[a]] + (double)shape_dH<T, NN>(C, 1, a) * v[(long)N + g[a]] +
+                   (double)shape_dH<T, NN>(C, 2, a) * v[2L * N + g[a]];
+        return div_v;
+      });
   p_out[e] = pe;
 
   // ---- Jaumann update and J2 return (CalcStressStrain) -------------------
@@ -542,6 +530,20 @@ __device__ __forceinline__ void element_chain(
     }
     q_plheat[e] = q;
     heat = 0.9 * (double)q * (double)volv / NN / fp.cp_T;
+  }
+  // ---- calcArtificialViscosity (Mechanical.C:1948-1977), after the stress
+  // update as in the reference's step (Solver_explicit.C:755-757): added to
+  // the diagonal stresses the forces use; kept per element for the sigma field
+  if (fp.av_al != 0.0 || fp.av_be != 0.0) {
+    const double rho_e = fp.rho_u > 0.0 ? fp.rho_u : (double)rho[e];
+    double qs = 0.0;
+    if (artificial_viscosity_q(pp.K, fp.av_al, fp.av_be, rho_e, (double)volv,
+                               (double)D.xx + (double)D.yy + (double)D.zz, qs)) {
+      sig.xx = (T)((double)sig.xx + qs);
+      sig.yy = (T)((double)sig.yy + qs);
+      sig.zz = (T)((double)sig.zz + qs);
+    }
+    static_cast<T *>(fp.avq)[e] = (T)qs;
   }
   st_stream(&eps_io[0 * Es + e], ld_stream(&eps_io[0 * Es + e]) + D.xx * dt);
   st_stream(&eps_io[1 * Es + e], ld_stream(&eps_io[1 * Es + e]) + D.yy * dt);
@@ -922,9 +924,11 @@ __global__ __launch_bounds__(kBlock) void f_unpack(long n3, const float *__restr
 }
 
 // sigma = -p I + tau (CalcStressStrain :1766), for the field after a fused run
+// (+ the step's artificial-viscosity term, calcArtificialViscosity, when avq)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void f_sigma(int E, const T *__restrict__ p,
-                                                  const T *__restrict__ tau, T *__restrict__ sigma) {
+                                                  const T *__restrict__ tau, const T *__restrict__ avq,
+                                                  T *__restrict__ sigma) {
   int e = gtid();
   if (e >= E) return;
   const long Es = E;
@@ -934,6 +938,12 @@ __global__ __launch_bounds__(kBlock) void f_sigma(int E, const T *__restrict__ p
   T3<T> t = t3_from_sym(tau[0 * Es + e], tau[1 * Es + e], tau[2 * Es + e], tau[3 * Es + e], tau[4 * Es + e],
                         tau[5 * Es + e]);
   T3<T> s = t3_add(pI, t);
+  if (avq && avq[e] != T(0)) {
+    const double qs = (double)avq[e];
+    s.xx = (T)((double)s.xx + qs);
+    s.yy = (T)((double)s.yy + qs);
+    s.zz = (T)((double)s.zz + qs);
+  }
   sigma[0 * Es + e] = s.xx; sigma[1 * Es + e] = s.yy; sigma[2 * Es + e] = s.zz;
   sigma[3 * Es + e] = s.xy; sigma[4 * Es + e] = s.yz; sigma[5 * Es + e] = s.xz;
 }
@@ -1047,9 +1057,11 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
   if (phase == 1) {
     double *minl = cfl ? c->cfl_part : nullptr;
     if (c->n == 8)
-      hipLaunchKernelGGL((f1_volume<T, 8, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol, minl);
+      hipLaunchKernelGGL((f1_volume<T, 8, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol, minl,
+                         c->status);
     else
-      hipLaunchKernelGGL((f1_volume<T, 4, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol, minl);
+      hipLaunchKernelGGL((f1_volume<T, 4, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol, minl,
+                         c->status);
     if (cfl) {
       hipError_t e = launch_cfl_reduce<T>(c, gE, step, s);
       // calcMinEdgeLength also refreshes the element heights contact uses
@@ -1064,11 +1076,11 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
     FusedParams fp;
     fp.dt = P.dt;
     fp.mp = mat_params(c);
-    fp.pp = PressParams{P.mat.K,        st.alpha_free, st.log_factor,     st.J_min,        st.hg_coeff_free,
-                        st.pspg_scale,  st.p_pspg_bulkfac, st.av_coeff_div, st.av_coeff_bulk, 0};
-    fp.pp.full = (st.pspg_scale != 0.0 || st.p_pspg_bulkfac != 0.0 || st.av_coeff_div != 0.0 ||
-                  st.av_coeff_bulk != 0.0);
-    fp.need_jlocal = (st.alpha_free != 0.0 || st.hg_coeff_free != 0.0);
+    fp.pp = press_params(c);
+    fp.need_jlocal = press_needs_jlocal(fp.pp);
+    fp.av_al = st.artifvisc0;
+    fp.av_be = st.artifvisc1;
+    fp.avq = el.avq;
     fp.hg_coeff = (c->n == 8) ? st.hg_coeff : 0.0;
     fp.cs0 = P.mat.cs0;
     fp.rho_u = c->rho_uniform;
@@ -1100,7 +1112,9 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
     // the rigid surfaces move after the position update
     if (c->contact.on && step >= 0) return launch_rigid_update(c, cfl, s);
   } else if (phase == 5) {
-    hipLaunchKernelGGL((f_sigma<T>), dim3(nblocks(E)), dim3(kBlock), 0, s, E, el.p, el.tau, el.sigma);
+    const bool av = st.artifvisc0 != 0.0 || st.artifvisc1 != 0.0;
+    hipLaunchKernelGGL((f_sigma<T>), dim3(nblocks(E)), dim3(kBlock), 0, s, E, el.p, el.tau,
+                       av ? (const T *)el.avq : (const T *)nullptr, el.sigma);
     // the nodal volumes no step reads, once per run: the same CSR-order sum
     // and division as CalcNodalVol, so the field equals the per-step API's
     hipLaunchKernelGGL((k_nodal_vol<T>), dim3(nblocks(N)), dim3(kBlock), 0, s, N, c->ms.nodel_offset,

@@ -57,9 +57,14 @@ struct Material {
 struct Stab {
   double alpha_free = 0, log_factor = 0, J_min = 0, hg_coeff_free = 0;
   double pspg_scale = 0, p_pspg_bulkfac = 0, av_coeff_div = 0, av_coeff_bulk = 0;
+  double alpha_contact = 0, hg_coeff_contact = 0;  // elements with a node in contact
   double artifvisc0 = 0, artifvisc1 = 0;
   double hg_coeff = 0;  // 3D hexa viscous hourglass coefficient
 };
+
+// slots of the context's device status counters (unsigned int[16])
+constexpr int kStatNonfinite = 0;  // non-finite internal-force entries scrubbed
+constexpr int kStatInverted = 1;   // element Jacobians with detJ <= 0 or not finite
 
 // Device buffers of one domain.  T is the element-field real type.
 template <typename T>
@@ -71,6 +76,7 @@ struct DevElem {
   T *f_elem = nullptr, *f_hg = nullptr;
   T *He = nullptr;  // optional per-element bilinear plastic modulus (specimen batches)
   T *q_plheat = nullptr, *dTedt = nullptr;  // thermal coupling: plastic heat, element-node rates [n][E]
+  T *avq = nullptr;  // fused path: the artificial-viscosity term added to the diagonal stresses
 };
 
 // Rigid-surface penalty contact (Contact.C, TriMesh_d): penalty factor
@@ -175,7 +181,9 @@ struct wfx_ctx {
   uint64_t graph_sig = 0;
   bool fused_stale = false;  // per-step intermediates not maintained by the fused path
   double *scalars = nullptr;      // device scratch for reductions (reaction force)
-  unsigned int *status = nullptr; // [0] non-finite internal forces scrubbed
+  unsigned int *status = nullptr; // counters, slots kStat*
+  unsigned int status_seen[2] = {0, 0};  // counters at the end of the last checked run
+  int check_status = 1;           // runs read the counters at their end (synchronising)
   double *cfl_part = nullptr;     // per-block partial extremes of the CFL time step:
   int cfl_cap_min = 0;            //   edge minima at [0, cfl_cap_min), speed maxima after
   int cfl_nvmax = 0;              //   speed maxima left by the last node pass

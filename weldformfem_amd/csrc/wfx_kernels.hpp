@@ -289,12 +289,21 @@ __global__ __launch_bounds__(kBlock) void k_correction_pos(long n3, const double
 // element kernels
 // ---------------------------------------------------------------------------
 
+// Inverted or degenerate element: detJ <= 0 or not finite (the check of
+// Solver_explicit.C:639-652, which prints and counts them).  Counted in the
+// context's status block; the step goes on as the reference's does.
+template <typename T>
+__device__ __forceinline__ void count_inverted(T det, unsigned int *status) {
+  if (status && !(det > T(0))) atomicAdd(status + kStatInverted, 1u);
+}
+
 // calcElemJAndDerivatives (Domain_d.C:1701-2044): hexa :1822-1850, tet
 // :1855-1887, determinant Matrix.h:593-618, adjugate Matrix.h:694-727.
 template <typename T, int NN>
 __global__ __launch_bounds__(kBlock) void k_elem_jacobians(int E, int N, const int *__restrict__ elnod,
                                                            const double *__restrict__ x,
-                                                           T *__restrict__ detJ, T *__restrict__ dH) {
+                                                           T *__restrict__ detJ, T *__restrict__ dH,
+                                                           unsigned int *__restrict__ status) {
   int e = gtid();
   if (e >= E) return;
   T X[NN][3];
@@ -346,8 +355,10 @@ __global__ __launch_bounds__(kBlock) void k_elem_jacobians(int E, int N, const i
   C[2][0] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
   C[2][1] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]);
   C[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
-  detJ[e] = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[1][2] * J[2][1] - J[0][1] * J[1][0] * J[2][2] +
-            J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] - J[0][2] * J[1][1] * J[2][0];
+  const T det = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[1][2] * J[2][1] - J[0][1] * J[1][0] * J[2][2] +
+                J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] - J[0][2] * J[1][1] * J[2][0];
+  detJ[e] = det;
+  count_inverted(det, status);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     // adj(d,i) = C(i,d)
@@ -506,9 +517,94 @@ struct PressParams {
   double K, alpha_free, log_factor, J_min, hg_coeff_free, pspg_scale, p_pspg_bulkfac,
       av_coeff_div, av_coeff_bulk;
   int full;  // 0: all stabilisation terms that need h, c, div v are zero
+  double alpha_contact, hg_coeff_contact;
+  // contact forces [3][N] when an element's contact state can change its
+  // pressure (contact on and contact coefficients that differ from the free
+  // ones, or the terms of `full`); null otherwise
+  const double *cf;
+  int N;
 };
 
-// calcElemPressure element part (Mechanical.C:720-815), no contact.  With
+// Parameters of calcElemPressure for the context (per-step and fused paths).
+inline PressParams press_params(const wfx_ctx *c) {
+  const auto &st = c->prm.stab;
+  PressParams pp;
+  pp.K = c->prm.mat.K;
+  pp.alpha_free = st.alpha_free;
+  pp.log_factor = st.log_factor;
+  pp.J_min = st.J_min;
+  pp.hg_coeff_free = st.hg_coeff_free;
+  pp.pspg_scale = st.pspg_scale;
+  pp.p_pspg_bulkfac = st.p_pspg_bulkfac;
+  pp.av_coeff_div = st.av_coeff_div;
+  pp.av_coeff_bulk = st.av_coeff_bulk;
+  pp.full = (st.pspg_scale != 0.0 || st.p_pspg_bulkfac != 0.0 || st.av_coeff_div != 0.0 ||
+             st.av_coeff_bulk != 0.0);
+  pp.alpha_contact = st.alpha_contact;
+  pp.hg_coeff_contact = st.hg_coeff_contact;
+  const bool blend = pp.full || st.alpha_contact != st.alpha_free || st.hg_coeff_contact != st.hg_coeff_free;
+  pp.cf = (c->contact.on && blend) ? c->contact.contforce : nullptr;
+  pp.N = c->N;
+  return pp;
+}
+
+// J_local = vol/vol0 enters the pressure only through these coefficients or
+// the terms of `full`
+__host__ __device__ inline bool press_needs_jlocal(const PressParams &pp) {
+  return pp.full || pp.alpha_free != 0.0 || pp.hg_coeff_free != 0.0 ||
+         (pp.cf && (pp.alpha_contact != 0.0 || pp.hg_coeff_contact != 0.0));
+}
+
+// Contact detection of calcElemPressure (Mechanical.C:729-745): an element is
+// in contact when one of its nodes carries a contact force of positive
+// squared length (the last step's: contforce is never reset).
+template <int NN, class NodeId>
+__device__ __forceinline__ bool elem_in_contact(const PressParams &pp, NodeId node) {
+  bool in = false;
+  if (pp.cf) {
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      const long nid = node(a);
+      const double cx = pp.cf[nid], cy = pp.cf[(long)pp.N + nid], cz = pp.cf[2L * pp.N + nid];
+      in = in || (cx * cx + cy * cy + cz * cz > 0);
+    }
+  }
+  return in;
+}
+
+// The element pressure of calcElemPressure (Mechanical.C:746-815) from the
+// element's averaged nodal ratio J_avg.  rho() and div_v() (the velocity
+// divergence times detJ, sum_a dH_a . v_a) are evaluated only when the
+// stabilisation terms need them.
+template <class Rho, class DivV>
+__device__ __forceinline__ double anp_elem_pressure(const PressParams &pp, bool is_contact, double J_local,
+                                                    double J_avg, double vol, Rho rho, DivV div_v_fn) {
+  double al = is_contact ? pp.alpha_contact : pp.alpha_free;
+  double J_bar = al * J_local + (1 - al) * J_avg;
+  if (J_bar < pp.J_min) J_bar = 0.2;
+  double lf = pp.log_factor;
+  double lg = (lf != 0.0) ? log(J_bar) : 0.0;
+  double p_physical = -pp.K * (lf * lg + (1.0 - lf) * (J_bar - 1.0));
+  double p_pspg = 0.0, p_q = 0.0;
+  double p_hg = (is_contact ? pp.hg_coeff_contact : pp.hg_coeff_free) * pp.K * fabs(J_local - J_avg);
+  if (pp.full) {
+    double rho_e = rho();
+    double h = pow(vol, 1.0 / 3.0);
+    double c = sqrt(pp.K / rho_e);
+    double tau = h / (2.0 * c);
+    double div_v = div_v_fn();
+    if (div_v < 0.0) {
+      double x1 = pp.pspg_scale * tau * div_v * pp.K, x2 = pp.p_pspg_bulkfac * pp.K;
+      p_pspg = (x2 < x1) ? x2 : x1;
+      double q1 = pp.av_coeff_div * rho_e * h * c * (-div_v);
+      double q2 = pp.av_coeff_bulk * pp.K * (1.0 - J_local);
+      p_q = is_contact ? 0.5 * (q1 + q2) : ((q1 < q2) ? q2 : q1);
+    }
+  }
+  return p_physical + p_pspg + p_hg + p_q;
+}
+
+// calcElemPressure element part (Mechanical.C:720-815).  With
 // pspg/artificial-viscosity coefficients at zero the terms that need h, c and
 // div v are exactly +-0 and are skipped (full == 0).
 template <typename T, int NN>
@@ -522,44 +618,27 @@ __global__ __launch_bounds__(kBlock) void k_pressure_elem(int E, int N, const in
                                                           T *__restrict__ p, PressParams pp) {
   int e = gtid();
   if (e >= E) return;
+  auto node = [&](int a) { return elnod[(long)a * E + e]; };
   double J_local = (double)vol[e] / (double)vol0[e];
+  const bool is_contact = elem_in_contact<NN>(pp, node);
   double J_avg = 0.0;
 #pragma unroll
-  for (int a = 0; a < NN; ++a) {
-    int nid = elnod[(long)a * E + e];
-    J_avg += ratio[nid];
-  }
+  for (int a = 0; a < NN; ++a) J_avg += ratio[node(a)];
   J_avg /= NN;
-  double al = pp.alpha_free;
-  double J_bar = al * J_local + (1 - al) * J_avg;
-  if (J_bar < pp.J_min) J_bar = 0.2;
-  double lf = pp.log_factor;
-  double lg = (lf != 0.0) ? log(J_bar) : 0.0;
-  double p_physical = -pp.K * (lf * lg + (1.0 - lf) * (J_bar - 1.0));
-  double p_pspg = 0.0, p_q = 0.0;
-  double p_hg = pp.hg_coeff_free * pp.K * fabs(J_local - J_avg);
-  if (pp.full) {
-    double rho_e = (double)rho[e];
-    double h = pow((double)vol[e], 1.0 / 3.0);
-    double c = sqrt(pp.K / rho_e);
-    double tau = h / (2.0 * c);
-    double div_v = 0.0;
+  const double pe = anp_elem_pressure(
+      pp, is_contact, J_local, J_avg, (double)vol[e], [&] { return (double)rho[e]; },
+      [&] {
+        double div_v = 0.0;
 #pragma unroll
-    for (int a = 0; a < NN; ++a) {
-      int nid = elnod[(long)a * E + e];
-      div_v += (double)dH[(long)(0 * NN + a) * E + e] * v[nid] +
-               (double)dH[(long)(1 * NN + a) * E + e] * v[(long)N + nid] +
-               (double)dH[(long)(2 * NN + a) * E + e] * v[2L * N + nid];
-    }
-    if (div_v < 0.0) {
-      double x1 = pp.pspg_scale * tau * div_v * pp.K, x2 = pp.p_pspg_bulkfac * pp.K;
-      p_pspg = (x2 < x1) ? x2 : x1;
-      double q1 = pp.av_coeff_div * rho_e * h * c * (-div_v);
-      double q2 = pp.av_coeff_bulk * pp.K * (1.0 - J_local);
-      p_q = (q1 < q2) ? q2 : q1;
-    }
-  }
-  p[e] = (T)(p_physical + p_pspg + p_hg + p_q);
+        for (int a = 0; a < NN; ++a) {
+          const int nid = node(a);
+          div_v += (double)dH[(long)(0 * NN + a) * E + e] * v[nid] +
+                   (double)dH[(long)(1 * NN + a) * E + e] * v[(long)N + nid] +
+                   (double)dH[(long)(2 * NN + a) * E + e] * v[2L * N + nid];
+        }
+        return div_v;
+      });
+  p[e] = (T)pe;
 }
 
 // Temperature factors of the rate laws, evaluated once per element, or once
@@ -825,7 +904,21 @@ __global__ __launch_bounds__(kBlock) void k_stress_strain(
   tau_io[3 * Es + e] = tau.xy; tau_io[4 * Es + e] = tau.yz; tau_io[5 * Es + e] = tau.xz;
 }
 
-// calcArtificialViscosity (Mechanical.C:1948-1977)
+// calcArtificialViscosity (Mechanical.C:1948-1977): the term added to each
+// diagonal stress of an element with volumetric strain rate eps_v; false
+// (nothing added) for abs(eps_v) <= 1e-12.  pow(y, 2) is y * y.
+__device__ __forceinline__ bool artificial_viscosity_q(double K, double al, double be, double rho, double vol,
+                                                       double eps_v, double &qs) {
+  if (!(fabs(eps_v) > 1e-12)) return false;
+  const double c = sqrt(K / rho);
+  const double l = pow(vol, 1.0 / 3.0);
+  const double y = eps_v * l;
+  double q = al * c * fabs(eps_v) * l + be * (y * y);
+  q = (1e9 < q) ? 1e9 : q;
+  qs = (eps_v > 0) ? -q : q;
+  return true;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_artificial_viscosity(int E, double K, double al,
                                                                  double be,
@@ -836,15 +929,10 @@ __global__ __launch_bounds__(kBlock) void k_artificial_viscosity(int E, double K
   int e = gtid();
   if (e >= E) return;
   const long Es = E;
-  double c = sqrt(K / (double)rho[e]);
-  double eps_v = (double)sr[e] + (double)sr[Es + e] + (double)sr[2 * Es + e];
-  if (fabs(eps_v) > 1e-12) {
-    double l = pow((double)vol[e], 1.0 / 3.0);
-    double q = al * c * fabs(eps_v) * l + be * pow(eps_v * l, 2);
-    q = (1e9 < q) ? 1e9 : q;
-    double qs = (eps_v > 0) ? -q : q;
+  const double eps_v = (double)sr[e] + (double)sr[Es + e] + (double)sr[2 * Es + e];
+  double qs;
+  if (artificial_viscosity_q(K, al, be, (double)rho[e], (double)vol[e], eps_v, qs))
     for (int k = 0; k < 3; ++k) sigma[k * Es + e] = (T)((double)sigma[k * Es + e] + qs);
-  }
 }
 
 // calcElemForces (Mechanical.C:375-481), 3D branch :448-457
@@ -1100,10 +1188,10 @@ template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, d
     auto &el = elem<T>(c);                                                                         \
     if (c->n == 8)                                                                                 \
       hipLaunchKernelGGL((k_elem_jacobians<T, 8>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,  \
-                         c->N, c->ms.elnod, c->nd.x, el.detJ, el.dH);                              \
+                         c->N, c->ms.elnod, c->nd.x, el.detJ, el.dH, c->status);                   \
     else                                                                                           \
       hipLaunchKernelGGL((k_elem_jacobians<T, 4>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,  \
-                         c->N, c->ms.elnod, c->nd.x, el.detJ, el.dH);                              \
+                         c->N, c->ms.elnod, c->nd.x, el.detJ, el.dH, c->status);                   \
     return hipGetLastError();                                                                      \
   }                                                                                                \
   template <> hipError_t launch_elem_vol<T>(wfx_ctx * c, hipStream_t s) {                          \
@@ -1148,12 +1236,7 @@ template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, d
     hipLaunchKernelGGL((k_pressure_nodes<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N,       \
                        c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel, el.vol, el.vol0,        \
                        c->nd.anp_ratio);                                                           \
-    const auto &st = c->prm.stab;                                                                  \
-    PressParams pp{c->prm.mat.K,       st.alpha_free, st.log_factor,     st.J_min,                 \
-                   st.hg_coeff_free,   st.pspg_scale, st.p_pspg_bulkfac, st.av_coeff_div,          \
-                   st.av_coeff_bulk,   0};                                                         \
-    pp.full = (st.pspg_scale != 0.0 || st.p_pspg_bulkfac != 0.0 || st.av_coeff_div != 0.0 ||     \
-               st.av_coeff_bulk != 0.0);                                                           \
+    const PressParams pp = press_params(c);                                                        \
     if (c->n == 8)                                                                                 \
       hipLaunchKernelGGL((k_pressure_elem<T, 8>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,   \
                          c->N, c->ms.elnod, c->nd.anp_ratio, el.vol, el.vol0,                      \

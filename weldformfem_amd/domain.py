@@ -365,6 +365,13 @@ class Domain:
         check(self._L.wfx_nonfinite_count(self._h, ctypes.byref(v)))
         return v.value
 
+    def invertedCount(self):
+        """element Jacobians seen with detJ <= 0 or not finite (one count per
+        element and step)"""
+        v = ctypes.c_long()
+        check(self._L.wfx_inverted_count(self._h, ctypes.byref(v)))
+        return v.value
+
     # ---- fields (reference layout) ------------------------------------------
     def field(self, name):
         cnt, isint = ctypes.c_long(), ctypes.c_int()
