@@ -2,6 +2,7 @@
 // Domain_d API and state transfer.  Kernels live in wfx_k64.hip / wfx_k32.hip.
 #include <algorithm>
 #include <array>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -281,15 +282,29 @@ static hipStream_t as_stream(wfx_stream s) { return (hipStream_t)s; }
 
 // The fused nodal pass skips the density loads when every element has the
 // same density (the stored value, so the mass sums are unchanged).
-static int detect_uniform_rho(wfx_ctx *c) {
-  std::vector<double> r(c->E);
-  void *src = c->precision == WFX_FP64 ? (void *)c->e64.rho : (void *)c->e32.rho;
-  int rc = download_real(c, r.data(), src, r.size(), true);
+// Host scan of the element densities and initial volumes after they are set:
+// a uniform density lets the fused pass skip its loads; an initial volume
+// that is not finite or below 1e-200 (J_local = vol/vol0 not finite) or a
+// density that is not positive and finite sets the guard under which the
+// fused pass evaluates every pressure and viscosity term in full.
+static int scan_elements(wfx_ctx *c) {
+  std::vector<double> r(c->E), v0(c->E);
+  const bool f64 = c->precision == WFX_FP64;
+  int rc = download_real(c, r.data(), f64 ? (void *)c->e64.rho : (void *)c->e32.rho, r.size(), true);
+  if (!rc) rc = download_real(c, v0.data(), f64 ? (void *)c->e64.vol0 : (void *)c->e32.vol0, v0.size(), true);
   if (rc) return rc;
   c->rho_uniform = 0.0;
   if (!r.empty() && r[0] > 0.0 &&
       std::all_of(r.begin(), r.end(), [&](double v) { return v == r[0]; }))
     c->rho_uniform = r[0];
+  bool bad = false;
+  double rmin = DBL_MAX;
+  for (long e = 0; e < c->E; ++e) {
+    bad = bad || !(r[e] > 0.0 && r[e] <= DBL_MAX) || !(std::fabs(v0[e]) >= 1e-200 && std::fabs(v0[e]) <= DBL_MAX);
+    rmin = std::min(rmin, r[e]);
+  }
+  c->elem_guard = bad ? 1 : 0;
+  c->rho_min = (c->E > 0 && !bad) ? rmin : 0.0;
   return WFX_OK;
 }
 
@@ -985,7 +1000,7 @@ int wfx_init(wfx_ctx *c, wfx_stream s) {
   }
   WFX_CHECK_HIP(hipStreamSynchronize(st));
   c->initialized = true;
-  return detect_uniform_rho(c);
+  return scan_elements(c);
 }
 
 // ---- per-step API -----------------------------------------------------------
@@ -1004,7 +1019,10 @@ STEP_FN(wfx_calc_elem_vol, BY_PRECISION(c, launch_elem_vol, c, st))
 int wfx_calc_elem_density(wfx_ctx *c, wfx_stream s) {
   REQUIRE_MESH(c);
   LAUNCH(BY_PRECISION(c, launch_elem_density, c, as_stream(s)));
-  c->rho_uniform = 0.0;  // unknown until the next check: the fused pass loads densities
+  // unknown until the next scan: the fused pass loads densities and
+  // evaluates the pressure terms in full
+  c->rho_uniform = 0.0;
+  c->elem_guard = 1;
   return WFX_OK;
 }
 STEP_FN(wfx_calc_nodal_vol, BY_PRECISION(c, launch_nodal_vol, c, st))
@@ -1380,6 +1398,8 @@ static uint64_t graph_signature(wfx_ctx *c) {
                 (int)c->react_h.size(), c->fused_bs[1], c->fused_bs[2], c->fused_bs[3], c->fused_bs[4]};
   mix(ints, sizeof(ints));
   mix(&c->rho_uniform, sizeof(double));
+  mix(&c->elem_guard, sizeof(int));
+  mix(&c->rho_min, sizeof(double));
   return h;
 }
 
@@ -1610,7 +1630,7 @@ int wfx_set_field(wfx_ctx *c, const char *name, const void *host, long count) {
       if (v != 0.0) { c->has_fe = 1; break; }
   }
   int rc = upload_real(c, f.dev, tmp.data(), tmp.size(), f.elem);
-  if (rc == WFX_OK && !std::strcmp(name, "rho")) rc = detect_uniform_rho(c);
+  if (rc == WFX_OK && (!std::strcmp(name, "rho") || !std::strcmp(name, "vol0"))) rc = scan_elements(c);
   return rc;
 }
 

@@ -450,10 +450,11 @@ __device__ __forceinline__ void element_chain(
 #pragma unroll
   for (int a = 0; a < NN; ++a) J_avg += ratio[g[a]];
   J_avg /= NN;
+  const bool general = press_general(pp, (double)volv, J_avg);
   double J_local = 0.0;
-  if (fp.need_jlocal) J_local = (double)volv / (double)vol0[e];
+  if (fp.need_jlocal || general) J_local = (double)volv / (double)vol0[e];
   const T pe = (T)anp_elem_pressure(
-      pp, is_contact, J_local, J_avg, (double)volv, [&] { return (double)rho[e]; },
+      pp, is_contact, general, J_local, J_avg, (double)volv, [&] { return (double)rho[e]; },
       [&] {
         double div_v = 0.0;
 #pragma unroll
@@ -533,8 +534,12 @@ __device__ __forceinline__ void element_chain(
   }
   // ---- calcArtificialViscosity (Mechanical.C:1948-1977), after the stress
   // update as in the reference's step (Solver_explicit.C:755-757): added to
-  // the diagonal stresses the forces use; kept per element for the sigma field
-  if (fp.av_al != 0.0 || fp.av_be != 0.0) {
+  // the diagonal stresses the forces use; kept per element for the sigma field.
+  // With zero coefficients it is zero except where the volume is negative or
+  // not finite, or the scan flagged the densities (0 * NaN, as in the
+  // reference); an infinite diagonal strain rate already makes the diagonal
+  // stresses NaN.
+  if (fp.av_al != 0.0 || fp.av_be != 0.0 || pp.guard || !vol_regular((double)volv)) {
     const double rho_e = fp.rho_u > 0.0 ? fp.rho_u : (double)rho[e];
     double qs = 0.0;
     if (artificial_viscosity_q(pp.K, fp.av_al, fp.av_be, rho_e, (double)volv,
@@ -925,9 +930,13 @@ __global__ __launch_bounds__(kBlock) void f_unpack(long n3, const float *__restr
 
 // sigma = -p I + tau (CalcStressStrain :1766), for the field after a fused run
 // (+ the step's artificial-viscosity term, calcArtificialViscosity, when avq)
+// With zero viscosity coefficients (av_all false) the term is read only for
+// the elements the element pass wrote it for: negative or non-finite volume
+// in the last step (vol holds that step's volumes), or all under the guard.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void f_sigma(int E, const T *__restrict__ p,
                                                   const T *__restrict__ tau, const T *__restrict__ avq,
+                                                  const T *__restrict__ vol, int av_all,
                                                   T *__restrict__ sigma) {
   int e = gtid();
   if (e >= E) return;
@@ -938,7 +947,7 @@ __global__ __launch_bounds__(kBlock) void f_sigma(int E, const T *__restrict__ p
   T3<T> t = t3_from_sym(tau[0 * Es + e], tau[1 * Es + e], tau[2 * Es + e], tau[3 * Es + e], tau[4 * Es + e],
                         tau[5 * Es + e]);
   T3<T> s = t3_add(pI, t);
-  if (avq && avq[e] != T(0)) {
+  if ((av_all || !vol_regular((double)vol[e])) && avq[e] != T(0)) {
     const double qs = (double)avq[e];
     s.xx = (T)((double)s.xx + qs);
     s.yy = (T)((double)s.yy + qs);
@@ -1112,9 +1121,9 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
     // the rigid surfaces move after the position update
     if (c->contact.on && step >= 0) return launch_rigid_update(c, cfl, s);
   } else if (phase == 5) {
-    const bool av = st.artifvisc0 != 0.0 || st.artifvisc1 != 0.0;
-    hipLaunchKernelGGL((f_sigma<T>), dim3(nblocks(E)), dim3(kBlock), 0, s, E, el.p, el.tau,
-                       av ? (const T *)el.avq : (const T *)nullptr, el.sigma);
+    const bool av = st.artifvisc0 != 0.0 || st.artifvisc1 != 0.0 || press_params(c).guard;
+    hipLaunchKernelGGL((f_sigma<T>), dim3(nblocks(E)), dim3(kBlock), 0, s, E, el.p, el.tau, el.avq, el.vol,
+                       (int)av, el.sigma);
     // the nodal volumes no step reads, once per run: the same CSR-order sum
     // and division as CalcNodalVol, so the field equals the per-step API's
     hipLaunchKernelGGL((k_nodal_vol<T>), dim3(nblocks(N)), dim3(kBlock), 0, s, N, c->ms.nodel_offset,

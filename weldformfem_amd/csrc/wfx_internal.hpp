@@ -167,6 +167,12 @@ struct wfx_ctx {
   double *x_start = nullptr;
   int has_fe = 0;            // external nodal forces set (non-zero)
   double rho_uniform = 0.0;  // > 0: every element density equals this value
+  // host scan of vol0 and rho (wfx_init, wfx_set_field): elem_guard 1 when an
+  // initial volume is not finite or (nearly) zero or a density is not positive
+  // and finite, or when they changed on the device since the last scan;
+  // rho_min is the smallest density found
+  int elem_guard = 1;
+  double rho_min = 0.0;
   wfx::Thermal thermal;
   wfx::Contact contact;
   int graph_phase = 0;       // step counter mod 10 when the graph was captured

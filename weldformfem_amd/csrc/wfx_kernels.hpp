@@ -8,6 +8,7 @@
 //   wfx_k32.hip  instantiates T=float with contraction allowed.
 #pragma once
 
+#include <cfloat>
 #include <type_traits>
 
 #include "wfx_internal.hpp"
@@ -523,6 +524,10 @@ struct PressParams {
   // ones, or the terms of `full`); null otherwise
   const double *cf;
   int N;
+  // fused path: 1 when an element's initial volume or density may make a
+  // term non-finite (found by the host scan of vol0 and rho), so that every
+  // element evaluates the pressure and viscosity terms in full
+  int guard;
 };
 
 // Parameters of calcElemPressure for the context (per-step and fused paths).
@@ -545,6 +550,7 @@ inline PressParams press_params(const wfx_ctx *c) {
   const bool blend = pp.full || st.alpha_contact != st.alpha_free || st.hg_coeff_contact != st.hg_coeff_free;
   pp.cf = (c->contact.on && blend) ? c->contact.contforce : nullptr;
   pp.N = c->N;
+  pp.guard = c->elem_guard || !(pp.K / c->rho_min > 0.0 && pp.K / c->rho_min <= DBL_MAX);
   return pp;
 }
 
@@ -572,22 +578,39 @@ __device__ __forceinline__ bool elem_in_contact(const PressParams &pp, NodeId no
   return in;
 }
 
+// An element whose volume is negative or not finite (inverted or broken):
+// the reference still evaluates every stabilisation and viscosity term there,
+// and h = vol^(1/3) is NaN, so zero coefficients give 0 * NaN = NaN.
+__host__ __device__ inline bool vol_regular(double vol) { return vol >= 0.0 && vol <= DBL_MAX; }
+
+// Elements of the pressure's general evaluation: every term computed as the
+// reference writes it, whatever the coefficients.  Elements outside it have
+// finite J_local, J_avg, h, c and tau, where a zero coefficient makes its
+// term exactly zero.
+__device__ __forceinline__ bool press_general(const PressParams &pp, double vol, double J_avg) {
+  return pp.guard || !vol_regular(vol) || !(fabs(J_avg) <= DBL_MAX);
+}
+
 // The element pressure of calcElemPressure (Mechanical.C:746-815) from the
 // element's averaged nodal ratio J_avg.  rho() and div_v() (the velocity
 // divergence times detJ, sum_a dH_a . v_a) are evaluated only when the
-// stabilisation terms need them.
+// stabilisation terms need them or the element is in the general evaluation
+// (`general`: the reference's 0 * NaN then makes the pressure NaN, as it
+// does for an inverted element in compression).
 template <class Rho, class DivV>
-__device__ __forceinline__ double anp_elem_pressure(const PressParams &pp, bool is_contact, double J_local,
-                                                    double J_avg, double vol, Rho rho, DivV div_v_fn) {
+__device__ __forceinline__ double anp_elem_pressure(const PressParams &pp, bool is_contact, bool general,
+                                                    double J_local, double J_avg, double vol, Rho rho,
+                                                    DivV div_v_fn) {
   double al = is_contact ? pp.alpha_contact : pp.alpha_free;
   double J_bar = al * J_local + (1 - al) * J_avg;
   if (J_bar < pp.J_min) J_bar = 0.2;
   double lf = pp.log_factor;
-  double lg = (lf != 0.0) ? log(J_bar) : 0.0;
+  // log_factor 0: 0 * log(J_bar) is exactly 0 for a positive finite J_bar
+  double lg = (lf != 0.0 || !(J_bar > 0.0 && J_bar <= DBL_MAX)) ? log(J_bar) : 0.0;
   double p_physical = -pp.K * (lf * lg + (1.0 - lf) * (J_bar - 1.0));
   double p_pspg = 0.0, p_q = 0.0;
   double p_hg = (is_contact ? pp.hg_coeff_contact : pp.hg_coeff_free) * pp.K * fabs(J_local - J_avg);
-  if (pp.full) {
+  if (pp.full || general) {
     double rho_e = rho();
     double h = pow(vol, 1.0 / 3.0);
     double c = sqrt(pp.K / rho_e);
@@ -619,14 +642,21 @@ __global__ __launch_bounds__(kBlock) void k_pressure_elem(int E, int N, const in
   int e = gtid();
   if (e >= E) return;
   auto node = [&](int a) { return elnod[(long)a * E + e]; };
-  double J_local = (double)vol[e] / (double)vol0[e];
+  const double ve = (double)vol[e], rho_e = (double)rho[e];
+  double J_local = ve / (double)vol0[e];
   const bool is_contact = elem_in_contact<NN>(pp, node);
   double J_avg = 0.0;
 #pragma unroll
   for (int a = 0; a < NN; ++a) J_avg += ratio[node(a)];
   J_avg /= NN;
+  // this kernel reads J_local and rho anyway: the general evaluation is
+  // decided per element (finite J_local, positive finite K/rho) instead of
+  // by the scan the fused pass relies on
+  const double kr = pp.K / rho_e;
+  const bool general = !vol_regular(ve) || !(fabs(J_avg) <= DBL_MAX) || !(fabs(J_local) <= DBL_MAX) ||
+                       !(kr > 0.0 && kr <= DBL_MAX);
   const double pe = anp_elem_pressure(
-      pp, is_contact, J_local, J_avg, (double)vol[e], [&] { return (double)rho[e]; },
+      pp, is_contact, general, J_local, J_avg, ve, [&] { return rho_e; },
       [&] {
         double div_v = 0.0;
 #pragma unroll
@@ -919,6 +949,9 @@ __device__ __forceinline__ bool artificial_viscosity_q(double K, double al, doub
   return true;
 }
 
+// With both coefficients zero only the elements whose term can still be
+// non-finite are visited (negative or non-finite volume, K/rho not positive
+// and finite: the reference's 0 * NaN); the others add exactly zero.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_artificial_viscosity(int E, double K, double al,
                                                                  double be,
@@ -929,9 +962,14 @@ __global__ __launch_bounds__(kBlock) void k_artificial_viscosity(int E, double K
   int e = gtid();
   if (e >= E) return;
   const long Es = E;
+  const double ve = (double)vol[e], rho_e = (double)rho[e];
+  if (al == 0.0 && be == 0.0) {
+    const double kr = K / rho_e;
+    if (vol_regular(ve) && kr > 0.0 && kr <= DBL_MAX) return;
+  }
   const double eps_v = (double)sr[e] + (double)sr[Es + e] + (double)sr[2 * Es + e];
   double qs;
-  if (artificial_viscosity_q(K, al, be, (double)rho[e], (double)vol[e], eps_v, qs))
+  if (artificial_viscosity_q(K, al, be, rho_e, ve, eps_v, qs))
     for (int k = 0; k < 3; ++k) sigma[k * Es + e] = (T)((double)sigma[k * Es + e] + qs);
 }
 
@@ -1260,7 +1298,6 @@ template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, d
   }                                                                                                \
   template <> hipError_t launch_artificial_viscosity<T>(wfx_ctx * c, hipStream_t s) {              \
     const auto &st = c->prm.stab;                                                                  \
-    if (st.artifvisc0 == 0.0 && st.artifvisc1 == 0.0) return hipSuccess;                           \
     auto &el = elem<T>(c);                                                                         \
     hipLaunchKernelGGL((k_artificial_viscosity<T>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, \
                        c->prm.mat.K, st.artifvisc0, st.artifvisc1, el.rho, el.vol, el.str_rate,    \
