@@ -293,6 +293,14 @@ int wfx_adjoint_backward(wfx_ctx *ctx, const double *reaction_hist_bar, double *
 int wfx_adjoint_release(wfx_ctx *ctx);
 /* Domain_d::computeEnergies (Mechanical.C:2145-2185), synchronous. */
 int wfx_energies(wfx_ctx *ctx, double *ekin, double *deint, wfx_stream s);
+/* wfx_energies without any device allocation, for callers that cannot
+ * allocate (HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The reduction partials live
+ * in a static 1 MiB device array, one per GPU, held until the process ends:
+ * max(node count, element count) must be at most 16,776,960 (WFX_ERR_ARG
+ * beyond).  Calls that overlap on one GPU share that array and race.  Same
+ * kernels and results as wfx_energies; synchronous. */
+int wfx_energies_noalloc(wfx_ctx *ctx, double *ekin, double *deint, wfx_stream s);
 /* number of non-finite internal-force entries scrubbed so far (the scrub of
  * Solver_explicit.C:777-785) */
 int wfx_nonfinite_count(wfx_ctx *ctx, long *out);

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -103,10 +104,53 @@ static hipError_t alloc_elem(wfx_ctx *c, DevElem<T> &el) {
   return e;
 }
 
+// ---- static scratch of the allocation-free entry point ----------------------
+// wfx_energies_noalloc keeps its reduction partials (two doubles per
+// 256-thread block over max(N, E)) in this file-scope array instead of a
+// buffer it allocates.  Each GPU holds its own copy until the process ends.
+// On the host the array's name is only a stand-in, never a device address:
+// host code uses what hipGetSymbolAddress returns on each device, kept in
+// g_scratch_dev and copied into every context set up on that device.
+constexpr long kScratchDoubles = 1L << 17;  // 1 MiB: max(N, E) up to 16,776,960
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kScratchMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kScratchMaxDevices] = {};
+
+// the current device's address of `scratch`, looked up once per device
+static int scratch_address(double **out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error(std::string("hipGetDevice: ") + hipGetErrorString(e));
+    return WFX_ERR_HIP;
+  }
+  if (dev < 0 || dev >= kScratchMaxDevices) {
+    set_error("device index " + std::to_string(dev) + " beyond the static scratch table");
+    return WFX_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return WFX_ERR_HIP;
+    }
+    g_scratch_dev[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return WFX_OK;
+}
+
 // Allocates every device buffer for the mesh held on the host and uploads
 // coordinates, connectivity and CSR.
 static int setup_device(wfx_ctx *c) {
   free_all(c);
+  {
+    int rc = scratch_address(&c->scratch);  // the device this context's buffers live on
+    if (rc) return rc;
+  }
   const size_t N = c->N, E = c->E, n = c->n;
   hipError_t e = hipSuccess;
 #define A(ptr, cnt) \
@@ -1493,6 +1537,20 @@ int wfx_reaction_force(wfx_ctx *c, int dim, double *out, wfx_stream s) {
   return WFX_OK;
 }
 
+// computeEnergies with the reduction partials in `partial` (2 doubles per
+// 256-thread block over max(N, E), every one written before it is read)
+static hipError_t energies_into(wfx_ctx *c, double *partial, double *ekin, double *deint, hipStream_t st) {
+  hipError_t e = BY_PRECISION(c, launch_energies, c, c->scalars + 2, partial, st);
+  double h[2] = {0, 0};
+  if (e == hipSuccess) e = hipMemcpyAsync(h, c->scalars + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) {
+    *ekin = h[0];
+    *deint = h[1] * c->prm.dt;
+  }
+  return e;
+}
+
 int wfx_energies(wfx_ctx *c, double *ekin, double *deint, wfx_stream s) {
   REQUIRE_MESH(c);
   REQUIRE(ekin && deint, WFX_ERR_ARG, "null output");
@@ -1501,14 +1559,25 @@ int wfx_energies(wfx_ctx *c, double *ekin, double *deint, wfx_stream s) {
   int nb = (std::max(c->N, c->E) + 255) / 256;
   double *partial = nullptr;
   WFX_CHECK_HIP(hipMalloc(&partial, sizeof(double) * 2 * (nb + 1)));
-  hipError_t e = BY_PRECISION(c, launch_energies, c, c->scalars + 2, partial, st);
-  double h[2] = {0, 0};
-  if (e == hipSuccess) e = hipMemcpyAsync(h, c->scalars + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipError_t e = energies_into(c, partial, ekin, deint, st);
   (void)hipFree(partial);
   WFX_CHECK_HIP(e);
-  *ekin = h[0];
-  *deint = h[1] * c->prm.dt;
+  return WFX_OK;
+}
+
+// wfx_energies without allocating: the partials go to this device's copy of
+// the static scratch array.  Calls that overlap on one GPU share it and race.
+int wfx_energies_noalloc(wfx_ctx *c, double *ekin, double *deint, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(ekin && deint, WFX_ERR_ARG, "null output");
+  REQUIRE(!c->fused_stale, WFX_ERR_STATE, "energies need the strain rates of a per-step API step");
+  REQUIRE(c->scratch, WFX_ERR_STATE, "no static scratch address for this context's device");
+  const long nb = (std::max<long>(c->N, c->E) + 255) / 256;
+  REQUIRE(2 * nb <= kScratchDoubles, WFX_ERR_ARG,
+          "wfx_energies_noalloc: max(node count, element count) = " + std::to_string(std::max(c->N, c->E)) +
+              " exceeds the static scratch (at most " + std::to_string(kScratchDoubles / 2 * 256) +
+              "); use wfx_energies");
+  WFX_CHECK_HIP(energies_into(c, c->scratch, ekin, deint, as_stream(s)));
   return WFX_OK;
 }
 

@@ -187,6 +187,9 @@ struct wfx_ctx {
   uint64_t graph_sig = 0;
   bool fused_stale = false;  // per-step intermediates not maintained by the fused path
   double *scalars = nullptr;      // device scratch for reductions (reaction force)
+  // this context's device's address of the library's static scratch array
+  // (the allocation-free entry point's reduction partials)
+  double *scratch = nullptr;
   unsigned int *status = nullptr; // counters, slots kStat*
   unsigned int status_seen[2] = {0, 0};  // counters at the end of the last checked run
   int check_status = 1;           // runs read the counters at their end (synchronising)
