@@ -13,7 +13,10 @@ case for hg_coeff: steps 1e-4 and 1e-3 agree to 1e-7, 1e-5 and 1e-6 scatter
 by 1e-5).  The recorded forward must equal the plain fused run bit for bit,
 and the gradient must not depend on the checkpoint interval.  The Hollomon
 law's gradient with respect to (K, m, hg_coeff) meets the same bar, and both
-laws' parameters are identified from a reaction history.
+laws' parameters are identified from a reaction history.  The identification
+workload at its own size (30^3 hexa, 2000 steps, checkpoint every 50) meets
+the 1e-5 bar against GPU and oracle differences, with the hg_coeff step
+chosen from a sweep (see the comment above its tests).
 """
 import numpy as np
 import pytest
@@ -273,3 +276,86 @@ def test_hollomon_identification():
     say(f"  theta {theta} hidden {HOLL_HIDDEN} loss {loss:.3e} after {nit} iterations")
     assert loss < 1e-12 * losses[0]
     np.testing.assert_allclose(theta, HOLL_HIDDEN, rtol=1e-6)
+
+
+# ---- the identification workload at its own size ---------------------------
+# 30^3 hexa (27,000 elements), 2000 steps, checkpoint every 50, theta0 at the
+# nominal values, target at the seeded hidden parameters.  hg_coeff moves the
+# loss little (|theta dL/dtheta| / L ~ 2.5e-4) and the loss is piecewise
+# smooth (elements cross the yield surface at slightly different steps as
+# theta moves), so its central differences carry two errors of opposite
+# trend: the kinks crossed within +-h, falling with h, and the loss's own
+# rounding (~3e-14 of L) divided by the step, growing as h falls.  Measured
+# with the oracle: steps 3e-3, 1e-3, 3e-4, 1e-4, 3e-5 give 3.0e-5, 3.0e-5,
+# 2.4e-5, 1.8e-5, 2.7e-6 from the adjoint (falling towards it), 1e-5 and
+# below scatter by the rounding (1.2e-5, 8.4e-5, 5.1e-4).  The check takes the
+# sweep down to 3e-5, where both errors are below 1e-5.
+CFG4_CELLS, CFG4_STEPS, CFG4_EVERY = 30, 2000, 50
+CFG4_HG_STEPS = (3e-4, 1e-4, 3e-5)
+
+
+@pytest.fixture(scope="module")
+def cfg4_case():
+    hidden = workloads.hidden_params(seed=4)
+    theta0 = np.array(workloads.NOMINAL_PARAMS)
+    pb = CompressionProblem(cells=CFG4_CELLS, nsteps=CFG4_STEPS, every=CFG4_EVERY)
+    target = pb.forward(hidden)
+    L, g = pb.loss_and_grad(theta0, target)
+    return pb, hidden, theta0, target, L, g
+
+
+def test_cfg4_gradient_against_gpu_differences(cfg4_case):
+    pb, hidden, theta0, target, L, g = cfg4_case
+    say(f"case: 30^3 hexa, {CFG4_STEPS} steps, checkpoint {CFG4_EVERY}: adjoint {g} vs GPU central differences")
+    assert pb.domain.field("pl_strain").max() > 0
+    loss = lambda t: gpu_loss(pb, t, target)  # noqa: E731
+    fd = central_fd(loss, theta0, [1e-6, 1e-6, 1e-6], which=(0, 1))
+    err = np.abs(g[:2] - fd[:2]) / np.abs(fd[:2])
+    say(f"  sigma_y0, Et at step 1e-6: fd {fd[:2]} rel err {err}")
+    assert np.all(err <= 1e-5)
+    errs = []
+    for h in CFG4_HG_STEPS:
+        fdh = central_fd(loss, theta0, [0, 0, h], which=(2,))[2]
+        errs.append(abs(g[2] - fdh) / abs(fdh))
+        say(f"  hg_coeff at step {h:.0e}: fd {fdh:.9e} rel err {errs[-1]:.2e}")
+    assert errs[-1] <= 1e-5
+    assert errs[0] >= errs[-1]  # the differences approach the adjoint as the step falls
+
+
+def test_cfg4_gradient_against_oracle_differences(cfg4_case):
+    pb, hidden, theta0, target, L, g = cfg4_case
+    say("case: 30^3 hexa: adjoint vs central differences of the oracle (7 oracle solves)")
+
+    def ohist(t):
+        r = RefDomain()
+        r.AddBoxLength([0, 0, 0], [1.0, 1.0, 1.0], 0.5 / CFG4_CELLS)
+        bottom, top = workloads.box_layers(CFG4_CELLS, CFG4_CELLS, CFG4_CELLS)
+        m = dict(workloads.STEEL, sy0=float(t[0]), Et=float(t[1]))
+        workloads.setup_compression(r, bottom, top, -1.0, m, hg_coeff=float(t[2]))
+        r.init()
+        return r.run(CFG4_STEPS)
+
+    tgt = ohist(hidden)
+    assert np.max(np.abs(tgt - target)) <= 1e-9 * np.max(np.abs(tgt))
+
+    def loss(t):
+        d = ohist(t) - tgt
+        return float(np.dot(d, d))
+
+    fd = central_fd(loss, theta0, [1e-6, 1e-6, CFG4_HG_STEPS[-1]])
+    err = np.abs(g - fd) / np.abs(fd)
+    say(f"  oracle fd {fd} rel err {err}")
+    assert np.all(err <= 1e-5)
+
+
+def test_cfg4_mixed_precision_gradient(cfg4_case):
+    pb, hidden, theta0, target, L, g = cfg4_case
+    say("case: 30^3 hexa: FP32-mixed forward with the FP64 adjoint vs the FP64 gradient")
+    q = CompressionProblem(cells=CFG4_CELLS, nsteps=CFG4_STEPS, every=CFG4_EVERY, precision=WFX_FP32_MIXED)
+    L2, g2 = q.loss_and_grad(theta0, target)
+    err = np.abs(g2 - g) / np.abs(g)
+    say(f"  mixed grad {g2} rel err {err}, loss rel err {abs(L2 - L) / L:.2e}")
+    # sigma_y0 and Et within 1e-4; hg_coeff, whose effect on the loss is
+    # 2.5e-4 of it, within the FP32 rounding of the forces it scales
+    assert np.all(err[:2] <= 1e-4) and err[2] <= 1e-2
+    assert abs(L2 - L) <= 1e-3 * L

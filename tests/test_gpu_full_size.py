@@ -4,8 +4,11 @@ The 200x200x100 hexa compression block (4,000,000 elements, 4,080,501
 nodes, sigma_y0 ~ LogNormal(ln 300 MPa, 0.1)):
 
 * Mesh, connectivity and CSR bit for bit against the oracle, and the first
-  10 FP64 fused steps within 1e-12 of it (the oracle steps the full block in
-  about a second per step on the host, so only a few steps are affordable).
+  100 FP64 fused steps within 1e-10 of it in displacement, equivalent
+  plastic strain and reaction history (1e-9 in velocity and pressure, which
+  are differences of nearly equal values), the FP32-mixed fused path
+  within 1e-4 in displacement (the oracle steps the full block in about
+  0.15 s per step on 16 host cores).
 * Past yield, where the oracle cannot follow at this size, properties that
   do not depend on it, over the workload's 2000 steps: the fused path and the
   per-step API (two separate kernel sets) agree, and with a uniform yield
@@ -57,8 +60,11 @@ def block(dom, uniform_sy0=False):
     return dt
 
 
+ORACLE_STEPS = 100
+
+
 def test_mesh_and_first_steps_against_the_oracle():
-    say("case: 200x200x100 block, mesh/CSR bit for bit and 10 FP64 fused steps vs oracle")
+    say(f"case: 200x200x100 block, mesh/CSR bit for bit and {ORACLE_STEPS} FP64 fused steps vs oracle")
     g, r = Domain(WFX_FP64), RefDomain()
     block(g)
     block(r)
@@ -66,12 +72,25 @@ def test_mesh_and_first_steps_against_the_oracle():
     for f in ("elnod", "nodel", "nodel_loc", "nodel_offset", "nodel_count"):
         np.testing.assert_array_equal(g.field(f), r.field(f), err_msg=f)
     np.testing.assert_array_equal(g.field("x"), r.field("x"))
-    hg = g.runFused(10)
-    hr = r.run(10)
-    for f in ("u", "v", "p"):
-        e = rel(g.field(f), r.field(f))
-        assert e <= 1e-12, f"{f}: {e:.3e}"
-    assert rel(hg, hr) <= 1e-12
+    hg = g.runFused(ORACLE_STEPS)
+    say("  GPU done, oracle stepping")
+    hr = np.concatenate([r.run(ORACLE_STEPS // 2), r.run(ORACLE_STEPS - ORACLE_STEPS // 2)])
+    errs = {f: rel(g.field(f), r.field(f)) for f in ("u", "v", "p", "pl_strain")}
+    errs["reaction"] = rel(hg, hr)
+    say("  " + " ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    # displacement, reaction and plastic strain at 1e-10; velocity and
+    # pressure are differences of nearly equal values (p = -K (J_avg - 1)
+    # with J_avg - 1 ~ 1e-4), where the hourglass force's other summation
+    # order on the fused path shows at ~1e-10: bar 1e-9
+    for k, e in errs.items():
+        assert e <= (1e-9 if k in ("v", "p") else 1e-10), f"{k}: {e:.3e}"
+    del g
+    m = Domain(WFX_FP32_MIXED)
+    block(m)
+    hm = m.runFused(ORACLE_STEPS)
+    eu, er = rel(m.field("u"), r.field("u")), rel(hm, hr)
+    say(f"  FP32-mixed: u {eu:.2e} reaction {er:.2e}")
+    assert eu <= 1e-4 and er <= 1e-3
 
 
 STEPS = 2000
