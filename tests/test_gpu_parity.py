@@ -290,11 +290,19 @@ def test_fused_trajectory_100_steps_fp64(mesh):
     assert g.nonfiniteCount() == 0
 
 
-def test_fused_without_hourglass_is_bitexact():
-    say("case: fused FP64 without hourglass, bit-exact vs oracle")
+def test_fused_without_hourglass_differs_only_in_summation_order():
+    # the fused node pass adds a node's element forces per element cluster
+    # (ascending element within a cluster, clusters in order): the only
+    # difference to the reference's single ascending sum is the rounding of
+    # those additions, ~1e-16 per step (measured after 60 plastic steps:
+    # u 1.5e-13, plastic strain 4.8e-13, tau 1.2e-12, p 4.4e-12, the pressure
+    # K (1 - J) amplifying its J's rounding)
+    say("case: fused FP64 without hourglass vs oracle, 60 plastic steps")
     g, r, rg, rr = _trajectory_fused(_hexa20, -20.0, 60, hg=0.0)
-    for f in ["u", "v", "x", "pl_strain", "tau", "p"]:
-        np.testing.assert_array_equal(g.field(f), r.field(f), err_msg=f)
+    assert np.max(r.field("pl_strain")) > 0
+    errs = {f: rel(g.field(f), r.field(f)) for f in ["u", "v", "x", "pl_strain", "tau", "p"]}
+    say("  " + " ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    assert all(e <= 1e-11 for e in errs.values()), errs
 
 
 @pytest.mark.parametrize("mesh", [_hexa20, _cyl16])

@@ -49,7 +49,7 @@ def build(verbose=False, jobs=None):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         cmds.append([hipcc] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    for src in ("wfx_mesh.cpp", "wfx_io.cpp"):
+    for src in ("wfx_mesh.cpp", "wfx_io.cpp", "wfx_cluster.cpp"):
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -598,11 +598,21 @@ int wfx_adjoint_backward(wfx_ctx *c, const double *reaction_hist_bar, double *gr
   rc = c->precision == WFX_FP64 ? adjoint_backward<double>(c, st) : adjoint_backward<float>(c, st);
   if (rc) return rc;
   WFX_CHECK_HIP(hipMemcpyAsync(grad, A->sums, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
-  const size_t eb = (size_t)c->E * sizeof(double);
-  if (grad_sy0_elem) WFX_CHECK_HIP(hipMemcpyAsync(grad_sy0_elem, A->grad, eb, hipMemcpyDeviceToHost, st));
-  if (grad_Et_elem) WFX_CHECK_HIP(hipMemcpyAsync(grad_Et_elem, A->sums + 8, eb, hipMemcpyDeviceToHost, st));
-  if (grad_hg_elem) WFX_CHECK_HIP(hipMemcpyAsync(grad_hg_elem, A->grad + 2 * (size_t)c->E, eb, hipMemcpyDeviceToHost, st));
+  // per-element gradients: device rows in the stored (cluster) order, handed
+  // back in the original element order
+  const size_t E = (size_t)c->E, eb = E * sizeof(double);
+  double *outs[3] = {grad_sy0_elem, grad_Et_elem, grad_hg_elem};
+  const double *srcs[3] = {A->grad, A->sums + 8, A->grad + 2 * E};
+  std::vector<double> rows[3];
+  for (int k = 0; k < 3; ++k)
+    if (outs[k]) {
+      rows[k].resize(E);
+      WFX_CHECK_HIP(hipMemcpyAsync(rows[k].data(), srcs[k], eb, hipMemcpyDeviceToHost, st));
+    }
   WFX_CHECK_HIP(hipStreamSynchronize(st));
+  for (int k = 0; k < 3; ++k)
+    if (outs[k])
+      for (size_t e = 0; e < E; ++e) outs[k][e] = rows[k][c->el_o2s[e]];
   return WFX_OK;
 }
 

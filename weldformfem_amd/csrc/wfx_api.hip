@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/wfx.h"
+#include "wfx_cluster.hpp"
 #include "wfx_internal.hpp"
 #include "wfx_launch.hpp"
 
@@ -99,7 +100,7 @@ static hipError_t alloc_elem(wfx_ctx *c, DevElem<T> &el) {
   // f_elem is padded to whole 64-element tiles (the fused path may tile it);
   // FP32-mixed fused runs keep 4 floats per element-node there (float4)
   A(eps, 6 * E) A(strain_pl_incr, 6 * E) A(f_elem, (std::is_same<T, float>::value ? 4 : 3) * n * ((E + 63) / 64 * 64))
-  A(f_hg, 3 * n * E) A(avq, E)
+  A(f_hg, 3 * n * E) A(avq, E) A(part, 4 * (c->ms.nslot > 0 ? c->ms.nslot : 1))
 #undef A
   return e;
 }
@@ -145,7 +146,7 @@ static int scratch_address(double **out) {
 
 // Allocates every device buffer for the mesh held on the host and uploads
 // coordinates, connectivity and CSR.
-static int setup_device(wfx_ctx *c) {
+static int setup_device(wfx_ctx *c, const ClusterPlan &plan) {
   free_all(c);
   {
     int rc = scratch_address(&c->scratch);  // the device this context's buffers live on
@@ -161,6 +162,10 @@ static int setup_device(wfx_ctx *c) {
   A(c->ms.elnod, n * E) A(c->ms.nodel, n * E) A(c->ms.nodel_loc, n * E) A(c->ms.nodel_offset, N)
   A(c->ms.nodel_count, N) A(c->ms.nodel_pk, n * E) A(c->ms.nodel_pad, 8 * N) A(c->bc_slot, N)
   A(c->scalars, 64) A(c->status, 16)
+  c->ms.ncl = plan.clusters();
+  c->ms.nslot = plan.slots();
+  A(c->ms.cl_eoff, c->ms.ncl + 1) A(c->ms.cl_sbase, c->ms.ncl + 1) A(c->ms.cl_loff, c->ms.nslot + c->ms.ncl)
+  A(c->ms.cl_ent, n * E) A(c->ms.nsl_off, N + 1) A(c->ms.nsl, c->ms.nslot)
   c->cfl_cap_min = (int)((E + 63) / 64 + 1);  // blocks of the element passes at block size >= 64
   A(c->cfl_part, c->cfl_cap_min + (N + 63) / 64 + 1)
   if (e == hipSuccess) e = (c->precision == WFX_FP64) ? alloc_elem(c, c->e64) : alloc_elem(c, c->e32);
@@ -173,21 +178,31 @@ static int setup_device(wfx_ctx *c) {
     free_all(c);
     return WFX_ERR_HIP;
   }
-  // coordinates [N][3] -> [3][N]; connectivity [E][n] -> [n][E]
+  // coordinates [N][3] -> [3][N]; connectivity [E][n] -> [n][E] in the
+  // stored (cluster) order; the CSR keeps the reference's order of entries
+  // (ascending original element) and names the stored elements
   std::vector<double> xs(3 * N);
   for (size_t i = 0; i < N; ++i)
     for (int d = 0; d < 3; ++d) xs[d * N + i] = c->x_h[3 * i + d];
   std::vector<int> es(n * E);
   for (size_t k = 0; k < E; ++k)
-    for (size_t a = 0; a < n; ++a) es[a * E + k] = c->elnod_h[k * n + a];
+    for (size_t a = 0; a < n; ++a) es[a * E + k] = c->elnod_h[(size_t)c->el_s2o[k] * n + a];
+  std::vector<int> nodel_s(n * E);
+  for (size_t k = 0; k < n * E; ++k) nodel_s[k] = c->el_o2s[c->nodel_h[k]];
   WFX_CHECK_HIP(hipMemcpy(c->nd.x, xs.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemcpy(c->ms.elnod, es.data(), n * E * sizeof(int), hipMemcpyHostToDevice));
-  WFX_CHECK_HIP(hipMemcpy(c->ms.nodel, c->nodel_h.data(), n * E * sizeof(int), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(c->ms.nodel, nodel_s.data(), n * E * sizeof(int), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_loc, c->nodel_loc_h.data(), n * E * sizeof(int), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_offset, c->nodel_offset_h.data(), N * sizeof(int), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_count, c->nodel_count_h.data(), N * sizeof(int), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(c->ms.cl_eoff, plan.eoff.data(), plan.eoff.size() * sizeof(int), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(c->ms.cl_sbase, plan.sbase.data(), plan.sbase.size() * sizeof(int), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(c->ms.cl_loff, plan.loff.data(), plan.loff.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(c->ms.cl_ent, plan.ent.data(), plan.ent.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(c->ms.nsl_off, plan.nsl_off.data(), plan.nsl_off.size() * sizeof(int), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(c->ms.nsl, plan.nsl.data(), plan.nsl.size() * sizeof(int), hipMemcpyHostToDevice));
   std::vector<int> pk(n * E);
-  for (size_t k = 0; k < n * E; ++k) pk[k] = (c->nodel_h[k] << 3) | c->nodel_loc_h[k];
+  for (size_t k = 0; k < n * E; ++k) pk[k] = (nodel_s[k] << 3) | c->nodel_loc_h[k];
   WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_pk, pk.data(), n * E * sizeof(int), hipMemcpyHostToDevice));
   // entry j of node i at j*N + i: the node passes load their first eight
   // entries with unit stride across a wavefront and need no offset for them
@@ -214,7 +229,11 @@ static int finish_mesh(wfx_ctx *c) {
     return WFX_ERR_ARG;
   }
   c->sy0_h.assign(c->E, c->prm.mat.sy0);
-  int rc = setup_device(c);
+  ClusterPlan plan;
+  plan_clusters(c->N, c->E, c->n, c->elnod_h.data(), c->x_h.data(), kCluster, plan);
+  c->el_s2o.swap(plan.s2o);
+  c->el_o2s.swap(plan.o2s);
+  int rc = setup_device(c, plan);
   if (rc) return rc;
   c->prm.N = c->N;
   c->prm.E = c->E;
@@ -250,6 +269,23 @@ static int download_real(wfx_ctx *c, double *dst, const void *src, size_t count,
   return WFX_OK;
 }
 
+// Element rows [E][rows] between the original order (host) and the stored
+// order (device rows [rows][E]): transposed and permuted in one pass.
+static void elem_rows_to_device(const wfx_ctx *c, const double *h, int rows, double *dev_rows) {
+  const long E = c->E;
+  for (long o = 0; o < E; ++o) {
+    const long st = c->el_o2s[o];
+    for (int r = 0; r < rows; ++r) dev_rows[(long)r * E + st] = h[o * rows + r];
+  }
+}
+
+// one value per element, original order -> stored order, uploaded
+static int upload_elem_values(wfx_ctx *c, void *dst, const double *h, long count) {
+  std::vector<double> t((size_t)count);
+  elem_rows_to_device(c, h, 1, t.data());
+  return upload_real(c, dst, t.data(), t.size(), true);
+}
+
 // Description of one transferable field.  Host layout is [cnt][rows] (the
 // reference's), device layout is [rows][cnt] starting at dev.
 struct FieldDesc {
@@ -259,6 +295,8 @@ struct FieldDesc {
   int is_int = 0;  // int32 field
   int elem = 0;    // element real type (T) rather than double
   int kind = 0;    // 0 plain transpose, 1 spin (3 device rows -> 6 host columns)
+  int per_elem = 0;  // one item per element: stored order on the device
+  int elem_ids = 0;  // values are element indices (stored on the device)
 };
 
 static bool find_field(wfx_ctx *c, const char *name, FieldDesc &f) {
@@ -269,7 +307,7 @@ static bool find_field(wfx_ctx *c, const char *name, FieldDesc &f) {
   f = FieldDesc();
   auto node3 = [&](double *p) { f.dev = p; f.cnt = N; f.rows = 3; };
   auto node1 = [&](double *p) { f.dev = p; f.cnt = N; f.rows = 1; };
-  auto el = [&](void *p, int rows) { f.dev = p; f.cnt = E; f.rows = rows; f.elem = 1; };
+  auto el = [&](void *p, int rows) { f.dev = p; f.cnt = E; f.rows = rows; f.elem = 1; f.per_elem = 1; };
   if (s == "x") node3(c->nd.x);
   else if (s == "v") node3(c->nd.v);
   else if (s == "a") node3(c->nd.a);
@@ -280,8 +318,8 @@ static bool find_field(wfx_ctx *c, const char *name, FieldDesc &f) {
   else if (s == "fe") node3(c->nd.fe);
   else if (s == "mdiag") node1(c->nd.mdiag);
   else if (s == "voln") node1(c->nd.voln);
-  else if (s == "elnod") { f.dev = c->ms.elnod; f.cnt = E; f.rows = (int)n; f.is_int = 1; }
-  else if (s == "nodel") { f.dev = c->ms.nodel; f.cnt = E * n; f.is_int = 1; }
+  else if (s == "elnod") { f.dev = c->ms.elnod; f.cnt = E; f.rows = (int)n; f.is_int = 1; f.per_elem = 1; }
+  else if (s == "nodel") { f.dev = c->ms.nodel; f.cnt = E * n; f.is_int = 1; f.elem_ids = 1; }
   else if (s == "nodel_loc") { f.dev = c->ms.nodel_loc; f.cnt = E * n; f.is_int = 1; }
   else if (s == "nodel_offset") { f.dev = c->ms.nodel_offset; f.cnt = N; f.is_int = 1; }
   else if (s == "nodel_count") { f.dev = c->ms.nodel_count; f.cnt = N; f.is_int = 1; }
@@ -313,7 +351,7 @@ static bool find_field(wfx_ctx *c, const char *name, FieldDesc &f) {
   else if (c->contact.on && s == "node_area") node1(c->contact.node_area);
   else if (c->contact.on && s == "cfn") node1(c->contact.cfn);
   else if (c->contact.on && s == "mesh_in_contact") { f.dev = c->contact.mesh_in_contact; f.cnt = N; f.is_int = 1; }
-  else if (c->contact.on && s == "elem_length") { f.dev = c->contact.elem_length; f.cnt = E; }
+  else if (c->contact.on && s == "elem_length") { f.dev = c->contact.elem_length; f.cnt = E; f.per_elem = 1; }
   else if (c->contact.on && s == "rigid_x") { f.dev = c->contact.rnode; f.cnt = 3L * c->contact.rn; }
   else if (c->contact.on && s == "rigid_normal") { f.dev = c->contact.rnormal; f.cnt = 3L * c->contact.rt; }
   else if (c->contact.on && s == "rigid_pplane") { f.dev = c->contact.rpplane; f.cnt = c->contact.rt; }
@@ -481,7 +519,7 @@ static int assign_material(wfx_ctx *c, const Material &m) {
   if (c->has_mesh) {
     c->sy0_h.assign(c->E, m.sy0);
     void *dst = c->precision == WFX_FP64 ? (void *)c->e64.sy0 : (void *)c->e32.sy0;
-    return upload_real(c, dst, c->sy0_h.data(), c->sy0_h.size(), true);
+    return upload_elem_values(c, dst, c->sy0_h.data(), (long)c->sy0_h.size());
   }
   return WFX_OK;
 }
@@ -770,7 +808,11 @@ int wfx_set_elem_temperature(wfx_ctx *c, const double *T, int count) {
   }
   REQUIRE(count == c->E, WFX_ERR_ARG, "temperature needs one value per element");
   if (!c->temp_buf) WFX_CHECK_HIP(dalloc(c, &c->temp_buf, c->E));
-  WFX_CHECK_HIP(hipMemcpy(c->temp_buf, T, sizeof(double) * (size_t)count, hipMemcpyHostToDevice));
+  {
+    std::vector<double> t((size_t)count);
+    elem_rows_to_device(c, T, 1, t.data());
+    WFX_CHECK_HIP(hipMemcpy(c->temp_buf, t.data(), sizeof(double) * (size_t)count, hipMemcpyHostToDevice));
+  }
   c->temp_e = c->temp_buf;
   return WFX_OK;
 }
@@ -787,7 +829,7 @@ int wfx_set_elem_sy0(wfx_ctx *c, const double *sy0, int count) {
   REQUIRE(sy0 && count == c->E, WFX_ERR_ARG, "sy0 needs one value per element");
   c->sy0_h.assign(sy0, sy0 + count);
   void *dst = c->precision == WFX_FP64 ? (void *)c->e64.sy0 : (void *)c->e32.sy0;
-  return upload_real(c, dst, sy0, count, true);
+  return upload_elem_values(c, dst, sy0, count);
 }
 
 int wfx_add_bc_vel_node(wfx_ctx *c, int node, int dim, double val) {
@@ -1291,7 +1333,7 @@ int wfx_set_elem_tangent(wfx_ctx *c, const double *Et, int count) {
     else WFX_CHECK_HIP(dalloc(c, &c->e32.He, c->E));
   }
   drop_graph(c);  // kernel arguments changed
-  return upload_real(c, *slot, H.data(), H.size(), true);
+  return upload_elem_values(c, *slot, H.data(), (long)H.size());
 }
 
 }  // extern "C"
@@ -1358,8 +1400,11 @@ int wfx_autotune(wfx_ctx *c, int reps, int *chosen, wfx_stream s) {
   // of the choice (with three rounds of 5 steps a pass of 0.66 ms was once
   // tuned to a choice 4% slower)
   const int nrep = reps > 10 ? reps : 10;
+  // the element pass (phase 3) runs one work-group per element cluster
+  if (chosen) chosen[2] = kCluster;
   for (int sweep = 0; sweep < 2 && err == hipSuccess; ++sweep)
     for (int ph = 1; ph <= 4 && err == hipSuccess; ++ph) {
+      if (ph == 3) continue;
       float best_t[4] = {1e30f, 1e30f, 1e30f, 1e30f};
       for (int round = 0; round < 5 && err == hipSuccess; ++round)
         for (int k = 0; k < 4 && err == hipSuccess; ++k) {
@@ -1650,12 +1695,17 @@ int wfx_get_field(wfx_ctx *c, const char *name, void *host, long count) {
           "call the per-step entry point that produces it first");
   WFX_CHECK_HIP(hipDeviceSynchronize());
   const long cnt = f.cnt;
+  // device column of host item i (element fields are stored in cluster order)
+  auto col = [&](long i) { return f.per_elem ? (long)c->el_o2s[i] : i; };
   if (f.is_int) {
     std::vector<int> tmp(cnt * f.rows);
     WFX_CHECK_HIP(hipMemcpy(tmp.data(), f.dev, tmp.size() * sizeof(int), hipMemcpyDeviceToHost));
     int *h = (int *)host;
     for (long i = 0; i < cnt; ++i)
-      for (int r = 0; r < f.rows; ++r) h[i * f.rows + r] = tmp[(long)r * cnt + i];
+      for (int r = 0; r < f.rows; ++r) {
+        const int v = tmp[(long)r * cnt + col(i)];
+        h[i * f.rows + r] = f.elem_ids ? c->el_s2o[v] : v;
+      }
     return WFX_OK;
   }
   int drows = f.kind == 1 ? 3 : f.rows;
@@ -1666,11 +1716,11 @@ int wfx_get_field(wfx_ctx *c, const char *name, void *host, long count) {
   if (f.kind == 1) {
     for (long i = 0; i < cnt; ++i) {
       h[i * 6 + 0] = h[i * 6 + 1] = h[i * 6 + 2] = 0.0;
-      for (int r = 0; r < 3; ++r) h[i * 6 + 3 + r] = tmp[(long)r * cnt + i];
+      for (int r = 0; r < 3; ++r) h[i * 6 + 3 + r] = tmp[(long)r * cnt + col(i)];
     }
   } else {
     for (long i = 0; i < cnt; ++i)
-      for (int r = 0; r < f.rows; ++r) h[i * f.rows + r] = tmp[(long)r * cnt + i];
+      for (int r = 0; r < f.rows; ++r) h[i * f.rows + r] = tmp[(long)r * cnt + col(i)];
   }
   return WFX_OK;
 }
@@ -1686,12 +1736,13 @@ int wfx_set_field(wfx_ctx *c, const char *name, const void *host, long count) {
   const double *h = (const double *)host;
   int drows = f.kind == 1 ? 3 : f.rows;
   std::vector<double> tmp((size_t)drows * cnt);
+  auto col = [&](long i) { return f.per_elem ? (long)c->el_o2s[i] : i; };
   if (f.kind == 1) {
     for (long i = 0; i < cnt; ++i)
-      for (int r = 0; r < 3; ++r) tmp[(long)r * cnt + i] = h[i * 6 + 3 + r];
+      for (int r = 0; r < 3; ++r) tmp[(long)r * cnt + col(i)] = h[i * 6 + 3 + r];
   } else {
     for (long i = 0; i < cnt; ++i)
-      for (int r = 0; r < f.rows; ++r) tmp[(long)r * cnt + i] = h[i * f.rows + r];
+      for (int r = 0; r < f.rows; ++r) tmp[(long)r * cnt + col(i)] = h[i * f.rows + r];
   }
   if (!std::strcmp(name, "fe")) {
     c->has_fe = 0;

@@ -141,9 +141,10 @@ int wfx_write_vtk_async(wfx_ctx *c, const char *path, wfx_stream s) {
   WFX_CHECK_HIP(hipEventRecord(sl.done, fr->copy));
   const bool f64 = c->precision == WFX_FP64;
   const std::vector<int> *elnod = &c->elnod_h;  // stable until the mesh changes (which joins first)
+  const std::vector<int> *o2s = &c->el_o2s;     // element fields are stored in cluster order
   std::string p(path);
   FrameSlot *slp = &sl;
-  sl.writer = std::thread([slp, N, E, n, nb, es, f64, elnod, p]() {
+  sl.writer = std::thread([slp, N, E, n, nb, es, f64, elnod, o2s, p]() {
     if (hipEventSynchronize(slp->done) != hipSuccess) {
       slp->status = WFX_ERR_HIP;
       slp->err = "frame copy failed";
@@ -169,11 +170,11 @@ int wfx_write_vtk_async(wfx_ctx *c, const char *path, wfx_stream s) {
     std::vector<double> *ef[4] = {&fr.p, &fr.pl_strain, &fr.sigma_y, &fr.vol};
     for (int k = 0; k < 4; ++k) {
       ef[k]->resize(E);
-      for (size_t e = 0; e < E; ++e) (*ef[k])[e] = elv(k * E + e);
+      for (size_t e = 0; e < E; ++e) (*ef[k])[e] = elv(k * E + (*o2s)[e]);
     }
     fr.sigma.resize(6 * E);
     for (size_t e = 0; e < E; ++e)
-      for (int r = 0; r < 6; ++r) fr.sigma[6 * e + r] = elv(4 * E + r * E + e);
+      for (int r = 0; r < 6; ++r) fr.sigma[6 * e + r] = elv(4 * E + r * E + (*o2s)[e]);
     fr.elnod = *elnod;
     std::string err;
     if (write_vtk_frame(p.c_str(), fr, err)) {

@@ -8,14 +8,16 @@
 //                                                   calcElemPressure nodes)
 //   F3 element pass   J and dH recomputed from x, strain rate and spin from
 //                     the predicted v, ANP pressure, Jaumann + J2 return,
-//                     internal and hourglass forces -> f_elem - f_hg
+//                     internal and hourglass forces (f_elem - f_hg), added per
+//                     node of the work-group's element cluster in LDS -> one
+//                     partial sum per (cluster, node) slot
 //                                                  (calcElemJAndDerivatives,
 //                                                   calcElemStrainRates,
 //                                                   calcElemPressure,
 //                                                   CalcStressStrain,
 //                                                   calcElemForces,
 //                                                   calcElemHourglassForces)
-//   F4 node pass      CSR gather of element forces, acceleration, BCs,
+//   F4 node pass      sum of the node's slots, acceleration, BCs,
 //                     Chung-Hulbert correction, position update and the
 //                     next step's prediction   (assemblyForces, calcAccel,
 //                                                   ImposeBCA/BCV,
@@ -23,8 +25,11 @@
 //                                                   UpdatePrediction)
 //
 // Every expression is the one of the per-step kernels, so the FP64 fused path
-// differs from the per-step path only where the hourglass force is
-// subtracted per element instead of after the element-force sum.
+// differs from the per-step path only in the order of the nodal force sums:
+// the hourglass force is subtracted per element instead of after the
+// element-force sum, and a node's element forces are added per cluster
+// (ascending element within a cluster, clusters in order) instead of in one
+// ascending run.
 //
 // Device inputs beyond the per-step state:
 //   nodel_pk[nnz]  packed CSR entry (element << 3) | local node
@@ -40,52 +45,24 @@ namespace {
 
 // CSR entries fetched per batch of independent loads in the node passes
 constexpr int kChunk = 8;  // also the width of the padded first-entries table
-// Layout of the fused path's element-force buffer (written by F3, gathered by
-// F4, private to the fused path): row-major [3n][E], or tiled in blocks of 64
-// elements [E/64][3n][64] so that a wavefront's force rows are contiguous.
-#ifndef WFX_F_TILED
-#define WFX_F_TILED 1
-#endif
-// Non-temporal stores for the element streams written once per step.
-#ifndef WFX_NT_STORES
-#define WFX_NT_STORES 1
-#endif
+// Element-node rows of the per-step thermal rates (dTedt), tiled in blocks of
+// 64 elements [E/64][n][64] so that a wavefront's rows are contiguous.
 __device__ __forceinline__ long fidx(int row, long e, long Es, int nrow) {
-  if constexpr (WFX_F_TILED) return (e >> 6) * ((long)nrow * 64) + (long)row * 64 + (e & 63);
-  else return (long)row * Es + e;
+  return (e >> 6) * ((long)nrow * 64) + (long)row * 64 + (e & 63);
 }
-// FP32-mixed with thermal coupling keeps an element-node's three force
-// components and its thermal term together as one 16-byte float4, tiled
-// [E/64][n][64]: the element pass stores and the node pass gathers 16 bytes
-// per element-node (4-byte accesses stream at about 60% of 16-byte ones on
-// MI355X).  Without the thermal term the fourth lane would be padding, whose
-// bytes cost the element pass more than the wider gather saves (measured:
-// step 0.656 -> 0.665 ms), so the plain runs keep one row per component.
-typedef float f4v __attribute__((ext_vector_type(4)));
-// FP32-mixed without thermal coupling: an element-node's (fx, fy) is one
-// 8-byte pair in the first 2n rows' worth of the buffer, tiled [E/64][n][64],
-// and fz a float in the last n rows' worth at the same index: two accesses
-// per element-node instead of three, the same bytes (node pass 260 -> 234
-// µs at 4M elements).  FP64 keeps one row per component: the same split with
-// 16-byte pairs made its node pass slower (405 -> 412 µs).
-typedef float f2v __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ long fidx4(int a, long e, int nn) { return (e >> 6) * ((long)nn * 64) + (long)a * 64 + (e & 63); }
+// Element streams written once per step: non-temporal stores.
 template <typename T>
 __device__ __forceinline__ void st_stream(T *p, T v) {
-  if constexpr (WFX_NT_STORES) __builtin_nontemporal_store(v, p);
-  else *p = v;
+  __builtin_nontemporal_store(v, p);
 }
 // Element data the element pass reads once per step (connectivity, stress,
-// strain, yield stress): with WFX_NT_LOADS these loads do not displace the
-// nodal arrays its gathers reuse from L2 (measured: f3 -2%; the same loads in
-// the volume and node passes made those slower, so they stay plain there).
-#ifndef WFX_NT_LOADS
-#define WFX_NT_LOADS 1
-#endif
+// strain, yield stress): non-temporal loads, so that they do not displace the
+// nodal arrays its gathers reuse from L2 (measured: element pass -2%; the
+// same loads in the volume and node passes made those slower, so they stay
+// plain there).
 template <typename T>
 __device__ __forceinline__ T ld_stream(const T *p) {
-  if constexpr (WFX_NT_LOADS) return __builtin_nontemporal_load(p);
-  else return *p;
+  return __builtin_nontemporal_load(p);
 }
 
 // V0_n: CSR-ordered sum of the initial volumes (calcElemPressure :698-705)
@@ -330,23 +307,19 @@ constexpr bool kStashLDS = NN == 8 && LAW == LAW_RATE && std::is_same<T, double>
 template <bool TH>
 constexpr int kStashVals = TH ? 25 : 22;
 
-// one LDS array per block size, shared by the store and the reload
-template <int BS, bool TH>
-__device__ __forceinline__ double *stash_lds() {
-  __shared__ double stash[kStashVals<TH> * BS];
-  return stash;
-}
 
 // TH: thermal coupling (nodal temperatures, plastic heat and the element's
 // thermal terms); a compile-time switch so that runs without it keep their
 // register budget.
+// stash: this thread's column of the work-group's LDS (stride BS), used when
+// kStashLDS
 template <typename T, int NN, int LAW, int BS, bool TH, class Sink>
 __device__ __forceinline__ void element_chain(
     int e, int E, int N, const FusedParams &fp, const int *__restrict__ elnod, const double *__restrict__ x,
     const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
     const T *__restrict__ rho, const T *__restrict__ sy0, const T *__restrict__ He,
     T *__restrict__ tau_io, T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
-    T *__restrict__ sigma_y, T *__restrict__ q_plheat, T *__restrict__ dTe, Sink &&sink) {
+    T *__restrict__ sigma_y, T *__restrict__ q_plheat, double *stash, Sink &&sink) {
   const long Es = E;
   int g[NN];
 #pragma unroll
@@ -427,7 +400,7 @@ __device__ __forceinline__ void element_chain(
     gT2 *= kf;
   }
   if constexpr (kStashLDS<T, NN, LAW>) {
-    double *st = stash_lds<BS, TH>() + threadIdx.x;
+    double *st = stash;
     if constexpr (TH) {
       st[22 * BS] = gT0;
       st[23 * BS] = gT1;
@@ -561,7 +534,7 @@ __device__ __forceinline__ void element_chain(
   st_stream(&tau_io[4 * Es + e], tau.yz); st_stream(&tau_io[5 * Es + e], tau.xz);
 
   if constexpr (kStashLDS<T, NN, LAW>) {  // back from LDS (volatile: real loads)
-    volatile double *sv = stash_lds<BS, TH>() + threadIdx.x;
+    volatile double *sv = stash;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -614,33 +587,105 @@ __device__ __forceinline__ void element_chain(
   }
 }
 
-template <typename T, int NN, int BS, int LAW, bool TH>
-__global__ __launch_bounds__(BS) void f3_element(
-    int E, int N, FusedParams fp, const int *__restrict__ elnod, const double *__restrict__ x,
+// Cluster tables of the fused element pass on the device (wfx_cluster.hpp)
+struct ClusterArgs {
+  const int *eoff;       // [C+1] first stored element of each cluster
+  const int *sbase;      // [C+1] first slot of each cluster
+  const uint16_t *loff;  // cluster c: its M+1 entry offsets at sbase[c] + c
+  const uint16_t *ent;   // element-node entries t * 8 + a grouped by slot, cluster c at eoff[c] * n
+  long S;                // slots: the row length of the partial sums
+};
+
+// Values per element-node the element pass adds up: the force and, with
+// thermal coupling, the node's thermal term
+template <bool TH>
+constexpr int kNodeVals = TH ? 4 : 3;
+
+// LDS of one work-group of the element pass: the element-node values
+// [kNodeVals * n][kCluster], and in the same space (each thread only ever
+// touches its own column, and reloads its stash before it writes its
+// values) the per-thread stash of the FP64 hexahedral rate-law passes
+template <typename T, int NN, int LAW, bool TH>
+constexpr int kF3Lds = (kStashLDS<T, NN, LAW> && kStashVals<TH> * 8 > kNodeVals<TH> * NN * (int)sizeof(T))
+                           ? kStashVals<TH> * 8 * kCluster
+                           : kNodeVals<TH> * NN * (int)sizeof(T) * kCluster;
+
+// F3: one work-group per element cluster.  Each thread runs the element
+// chain of one element and leaves its element-node values in LDS; then the
+// work-group adds them per cluster node (the slot's entries, ascending
+// element, in double) and writes one partial sum per slot (assemblyForces,
+// Matrices.C:42-87, for the cluster's share of each node).
+template <typename T, int NN, int LAW, bool TH>
+__global__ __launch_bounds__(kCluster) void f3_cluster(
+    int E, int N, FusedParams fp, ClusterArgs ca, const int *__restrict__ elnod, const double *__restrict__ x,
     const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
     const T *__restrict__ rho, const T *__restrict__ sy0, const T *__restrict__ He,
     T *__restrict__ tau_io, T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
-    T *__restrict__ sigma_y, T *__restrict__ fout, T *__restrict__ q_plheat, T *__restrict__ dTe) {
-  int e = gtid();
-  if (e >= E) return;
-  const long Es = E;
-  element_chain<T, NN, LAW, BS, TH>(e, E, N, fp, elnod, x, v, ratio, vol0, rho, sy0, He, tau_io, eps_io, p_out, pl_strain,
-                       sigma_y, q_plheat, dTe, [&](int a, T f0, T f1, T f2, T th) {
-                         if constexpr (std::is_same<T, float>::value && TH) {
-                           f4v q = {f0, f1, f2, th};
-                           st_stream(reinterpret_cast<f4v *>(fout) + fidx4(a, e, NN), q);
-                         } else if constexpr (std::is_same<T, float>::value) {
-                           const long Ep = (Es + 63) / 64 * 64, k = fidx4(a, e, NN);
-                           f2v xy = {f0, f1};
-                           st_stream(reinterpret_cast<f2v *>(fout) + k, xy);
-                           st_stream(fout + 2L * NN * Ep + k, f2);
-                         } else {
-                           st_stream(&fout[fidx(a * 3 + 0, e, Es, 3 * NN)], f0);
-                           st_stream(&fout[fidx(a * 3 + 1, e, Es, 3 * NN)], f1);
-                           st_stream(&fout[fidx(a * 3 + 2, e, Es, 3 * NN)], f2);
-                           if constexpr (TH) dTe[fidx(a, e, Es, NN)] = th;
-                         }
-                       });
+    T *__restrict__ sigma_y, T *__restrict__ part, T *__restrict__ q_plheat) {
+  constexpr int R = kNodeVals<TH>;
+  __shared__ __align__(16) unsigned char lds_raw[kF3Lds<T, NN, LAW, TH>];
+  __shared__ uint32_t l_ent[kCluster * NN / 2];  // the cluster's entries, two per word
+  __shared__ uint16_t l_off[kCluster * NN + 1];  // its slots' entry offsets
+  T *lf = reinterpret_cast<T *>(lds_raw);
+  const int c = xcd_block(), t = threadIdx.x;
+  const int e0 = ca.eoff[c], ne = ca.eoff[c + 1] - e0;
+  const int s0 = ca.sbase[c], M = ca.sbase[c + 1] - s0;
+  const uint16_t *lo = ca.loff + s0 + c;
+  // The cluster's slot tables are loaded into registers before the element
+  // chain, so that the loads complete under its arithmetic, and stored to LDS
+  // after it (entries: ne * NN uint16 from a 4-byte aligned start, since
+  // e0 * NN is even).
+  const uint32_t *en32 = reinterpret_cast<const uint32_t *>(ca.ent + (long)e0 * NN);
+  const int nw = ne * NN / 2;
+  uint32_t ew[NN / 2];
+#pragma unroll
+  for (int k = 0; k < NN / 2; ++k) {
+    const int w = t + k * kCluster;
+    ew[k] = w < nw ? en32[w] : 0u;
+  }
+  uint16_t ow[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int q = t + k * kCluster;
+    ow[k] = q <= M ? lo[q] : (uint16_t)0;
+  }
+  if (t < ne) {
+    element_chain<T, NN, LAW, kCluster, TH>(
+        e0 + t, E, N, fp, elnod, x, v, ratio, vol0, rho, sy0, He, tau_io, eps_io, p_out, pl_strain, sigma_y,
+        q_plheat, reinterpret_cast<double *>(lds_raw) + t, [&](int a, T f0, T f1, T f2, T th) {
+          lf[(a * R + 0) * kCluster + t] = f0;
+          lf[(a * R + 1) * kCluster + t] = f1;
+          lf[(a * R + 2) * kCluster + t] = f2;
+          if constexpr (TH) lf[(a * R + 3) * kCluster + t] = th;
+        });
+  }
+#pragma unroll
+  for (int k = 0; k < NN / 2; ++k) {
+    const int w = t + k * kCluster;
+    if (w < nw) l_ent[w] = ew[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int q = t + k * kCluster;
+    if (q <= M) l_off[q] = ow[k];
+  }
+  for (int q = t + 2 * kCluster; q <= M; q += kCluster) l_off[q] = lo[q];  // clusters with > 2 slots per thread
+  __syncthreads();
+  const uint16_t *en = reinterpret_cast<const uint16_t *>(l_ent);
+  for (int j = t; j < M; j += kCluster) {
+    const int k0 = l_off[j], k1 = l_off[j + 1];
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    for (int k = k0; k < k1; ++k) {
+      const int q = en[k];
+      const int tt = q >> 3, a = q & 7;
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] += (double)lf[(a * R + r) * kCluster + tt];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) st_stream(&part[r * ca.S + s0 + j], (T)acc[r]);
+  }
 }
 
 // ---- thermal coupling ------------------------------------------------------
@@ -737,7 +782,6 @@ struct NodeParams {
   double pred_v;   // (1-gamma)*dt
   int predict;     // run the next step's prediction
   int has_fe;      // external forces present
-  int nrow;        // rows of the element-force buffer (3n)
   const double *dtp;  // per-step CFL time step: the step's dt on the device
   double *vmax_part;  //   and the per-block largest predicted speed
 };
@@ -806,57 +850,31 @@ __device__ __forceinline__ double node_update(
   return 0.0;
 }
 
-// Sum of the element forces of node i over its CSR entries, in CSR order
-// (assemblyForces, Matrices.C:42-87); the first eight entries come from the
-// padded [8][N] table.
-// TH: also the element-node thermal terms dTe ([n][E] in the same tiling) into f[3].
+// Internal force of node i (and with TH its thermal term in f[3]): the sum of
+// its slots' partial sums in ascending cluster order (assemblyForces,
+// Matrices.C:42-87, completed across clusters).  Slot ids are loaded four
+// at a time, then their partial sums.
 template <typename T, bool TH = false>
-__device__ __forceinline__ void gather_force(int i, long Ns, long Es, int nrow, const int *__restrict__ off,
-                                             const int *__restrict__ cnt, const int *__restrict__ nodel_pk,
-                                             const int *__restrict__ pad, const T *__restrict__ fel,
-                                             double (&f)[TH ? 4 : 3], const T *__restrict__ dTe = nullptr) {
-  const int c = cnt[i], o = c > kChunk ? off[i] : 0;
+__device__ __forceinline__ void gather_slots(int i, const int *__restrict__ nsl_off, const int *__restrict__ nsl,
+                                             const T *__restrict__ part, long S, double (&f)[TH ? 4 : 3]) {
+  constexpr int R = TH ? 4 : 3;
+  const int o = nsl_off[i], c = nsl_off[i + 1] - o;
 #pragma unroll
-  for (int d = 0; d < (TH ? 4 : 3); ++d) f[d] = 0.0;
-  for (int k = 0; k < c; k += kChunk) {
-    int pk[kChunk];
-    T q[kChunk][TH ? 4 : 3];  // widened when summed
+  for (int d = 0; d < R; ++d) f[d] = 0.0;
+  for (int k = 0; k < c; k += 4) {
+    int sl[4];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j)
-      pk[j] = (k == 0) ? pad[(long)j * Ns + i] : ((k + j < c) ? nodel_pk[o + k + j] : -1);
+    for (int j = 0; j < 4; ++j) sl[j] = (k + j < c) ? nsl[o + k + j] : -1;
+    T q[4][R];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) {
-      const long e = pk[j] >> 3;
-      if constexpr (std::is_same<T, float>::value && TH) {  // one float4 per element-node
-        const f4v z = {0.f, 0.f, 0.f, 0.f};
-        const f4v v = (pk[j] >= 0) ? reinterpret_cast<const f4v *>(fel)[fidx4(pk[j] & 7, e, nrow / 3)] : z;
-        q[j][0] = v.x;
-        q[j][1] = v.y;
-        q[j][2] = v.z;
-        if constexpr (TH) q[j][3] = v.w;
-      } else if constexpr (std::is_same<T, float>::value) {  // an (fx, fy) pair and fz
-        const int nn = nrow / 3;
-        const long k = fidx4(pk[j] & 7, e, nn), Ep = (Es + 63) / 64 * 64;
-        const f2v z2 = {0.f, 0.f};
-        const f2v xy = (pk[j] >= 0) ? reinterpret_cast<const f2v *>(fel)[k] : z2;
-        q[j][0] = xy.x;
-        q[j][1] = xy.y;
-        q[j][2] = (pk[j] >= 0) ? fel[2L * nn * Ep + k] : T(0);
-      } else {
-        const long b = (long)(pk[j] & 7) * 3;
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) q[j][d] = (pk[j] >= 0) ? fel[fidx((int)(b + d), e, Es, nrow)] : T(0);
-        if constexpr (TH) q[j][3] = (pk[j] >= 0) ? dTe[fidx(pk[j] & 7, e, Es, nrow / 3)] : T(0);
-      }
-    }
+      for (int d = 0; d < R; ++d) q[j][d] = (sl[j] >= 0) ? part[d * S + sl[j]] : T(0);
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j)
-      if (k + j < c) {
-        f[0] += (double)q[j][0];
-        f[1] += (double)q[j][1];
-        f[2] += (double)q[j][2];
-        if constexpr (TH) f[3] += (double)q[j][3];
-      }
+    for (int j = 0; j < 4; ++j)
+      if (k + j < c)
+#pragma unroll
+        for (int d = 0; d < R; ++d) f[d] += (double)q[j][d];
   }
 }
 
@@ -865,21 +883,20 @@ __device__ __forceinline__ void gather_force(int i, long Ns, long Es, int nrow, 
 // TH: thermal coupling, T += (sum of the node's thermal terms) / m * dt
 template <typename T, int BS, bool CFL, bool CONTACT, bool TH>
 __global__ __launch_bounds__(BS) void f4_node(
-    int N, int E, NodeParams np, const double *__restrict__ cf, const int *__restrict__ off,
-    const int *__restrict__ cnt,
-    const int *__restrict__ nodel_pk, const int *__restrict__ pad, const T *__restrict__ fel,
+    int N, NodeParams np, const double *__restrict__ cf, const int *__restrict__ nsl_off,
+    const int *__restrict__ nsl, const T *__restrict__ part, long S,
     const double *__restrict__ fe, const double *__restrict__ mdiag, const int *__restrict__ bc_slot,
     const int *__restrict__ bc_mask, const double *__restrict__ bc_val, T *__restrict__ fi,
     T *__restrict__ v, T *__restrict__ prev_a, T *__restrict__ u_dt,
     double *__restrict__ x, double *__restrict__ u, unsigned int *__restrict__ status,
-    const T *__restrict__ dTe, double *__restrict__ Tn) {
+    double *__restrict__ Tn) {
   int i = gtid();
   NodeParams q = np;
   if constexpr (CFL) set_step_length(q, *q.dtp);
   double speed = 0.0;
   if (i < N) {
     double f[TH ? 4 : 3];
-    gather_force<T, TH>(i, N, E, q.nrow, off, cnt, nodel_pk, pad, fel, f, dTe);
+    gather_slots<T, TH>(i, nsl_off, nsl, part, S, f);
     speed = node_update<T, CFL, CONTACT>(i, N, q, *reinterpret_cast<double(*)[3]>(f), cf, fe, mdiag, bc_slot, bc_mask,
                                          bc_val, fi, v, prev_a, u_dt, x, u, status);
     if constexpr (TH) Tn[i] += f[3] / mdiag[i] * q.dt;
@@ -1016,7 +1033,6 @@ inline NodeParams node_params(wfx_ctx *c, int predict) {
   np.pred_v = (1.0 - P.gamma) * P.dt;
   np.predict = predict;
   np.has_fe = c->has_fe;
-  np.nrow = 3 * c->n;
   np.dtp = nullptr;
   np.vmax_part = nullptr;
   return np;
@@ -1036,9 +1052,9 @@ hipError_t launch_thermal(wfx_ctx *c, hipStream_t s);
 
 // Launch one phase (1..4) of a fused step on the current state with block
 // size BS; launch_fused_phase picks BS from the context (autotuned or 256).
-template <typename T, int NN, int BS, int LAW>
+template <typename T, int NN, int LAW>
 auto f3_pick(bool th) {
-  return th ? f3_element<T, NN, BS, LAW, true> : f3_element<T, NN, BS, LAW, false>;
+  return th ? f3_cluster<T, NN, LAW, true> : f3_cluster<T, NN, LAW, false>;
 }
 
 template <typename T, int BS, bool CFL, bool CT>
@@ -1098,12 +1114,12 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
     fp.k_T = c->thermal.k_T;
     fp.cp_T = c->thermal.cp_T;
     const bool rate = law_class(fp.mp.model) == LAW_RATE, th = c->thermal.on;
-    auto kern = (c->n == 8) ? (rate ? f3_pick<T, 8, BS, LAW_RATE>(th) : f3_pick<T, 8, BS, LAW_PLAIN>(th))
-                            : (rate ? f3_pick<T, 4, BS, LAW_RATE>(th) : f3_pick<T, 4, BS, LAW_PLAIN>(th));
-    hipLaunchKernelGGL(kern, dim3(gE), dim3(BS), 0, s, E, N, fp, c->ms.elnod, c->nd.x, vN, c->nd.anp_ratio,
-                       el.vol0, el.rho, el.sy0, el.He, el.tau, el.eps, el.p, el.pl_strain, el.sigma_y,
-                       el.f_elem, c->thermal.on ? el.q_plheat : (T *)nullptr,
-                       c->thermal.on ? el.dTedt : (T *)nullptr);
+    auto kern = (c->n == 8) ? (rate ? f3_pick<T, 8, LAW_RATE>(th) : f3_pick<T, 8, LAW_PLAIN>(th))
+                            : (rate ? f3_pick<T, 4, LAW_RATE>(th) : f3_pick<T, 4, LAW_PLAIN>(th));
+    const ClusterArgs ca{c->ms.cl_eoff, c->ms.cl_sbase, c->ms.cl_loff, c->ms.cl_ent, c->ms.nslot};
+    hipLaunchKernelGGL(kern, dim3(c->ms.ncl), dim3(kCluster), 0, s, E, N, fp, ca, c->ms.elnod, c->nd.x, vN,
+                       c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.He, el.tau, el.eps, el.p, el.pl_strain,
+                       el.sigma_y, el.part, c->thermal.on ? el.q_plheat : (T *)nullptr);
     // contact forces after the element forces, before the assembly
     if (c->contact.on) return launch_contact<T>(c, vN, cfl, s);
   } else if (phase == 4) {
@@ -1114,10 +1130,9 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
     const bool ct = c->contact.on, th = c->thermal.on;
     auto kern = cfl ? (ct ? f4_pick<T, BS, true, true>(th) : f4_pick<T, BS, true, false>(th))
                     : (ct ? f4_pick<T, BS, false, true>(th) : f4_pick<T, BS, false, false>(th));
-    hipLaunchKernelGGL(kern, dim3(gN), dim3(BS), 0, s, N, E, np, c->contact.contforce, c->ms.nodel_offset,
-                       c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.f_elem, c->nd.fe, c->nd.mdiag,
-                       c->bc_slot, c->bc_mask, c->bc_tab, fiN, vN, paN, udN, c->nd.x, c->nd.u, c->status,
-                       el.dTedt, c->nd.T);
+    hipLaunchKernelGGL(kern, dim3(gN), dim3(BS), 0, s, N, np, c->contact.contforce, c->ms.nsl_off, c->ms.nsl,
+                       el.part, c->ms.nslot, c->nd.fe, c->nd.mdiag, c->bc_slot, c->bc_mask, c->bc_tab, fiN, vN,
+                       paN, udN, c->nd.x, c->nd.u, c->status, c->nd.T);
     // the rigid surfaces move after the position update
     if (c->contact.on && step >= 0) return launch_rigid_update(c, cfl, s);
   } else if (phase == 5) {

@@ -62,6 +62,9 @@ struct Stab {
   double hg_coeff = 0;  // 3D hexa viscous hourglass coefficient
 };
 
+// elements per cluster of the fused element pass (its work-group size)
+constexpr int kCluster = 256;
+
 // slots of the context's device status counters (unsigned int[16])
 constexpr int kStatNonfinite = 0;  // non-finite internal-force entries scrubbed
 constexpr int kStatInverted = 1;   // element Jacobians with detJ <= 0 or not finite
@@ -77,6 +80,7 @@ struct DevElem {
   T *He = nullptr;  // optional per-element bilinear plastic modulus (specimen batches)
   T *q_plheat = nullptr, *dTedt = nullptr;  // thermal coupling: plastic heat, element-node rates [n][E]
   T *avq = nullptr;  // fused path: the artificial-viscosity term added to the diagonal stresses
+  T *part = nullptr;  // fused path: per-slot partial nodal forces [4][S] (fx, fy, fz, thermal term)
 };
 
 // Rigid-surface penalty contact (Contact.C, TriMesh_d): penalty factor
@@ -119,6 +123,14 @@ struct DevMesh {
   int *nodel = nullptr, *nodel_loc = nullptr, *nodel_offset = nullptr, *nodel_count = nullptr;
   int *nodel_pk = nullptr;  // (element << 3) | local node, CSR order
   int *nodel_pad = nullptr; // the first 8 packed entries of every node, [8][N], -1 past its count
+  // element clusters of the fused element pass (wfx_cluster.hpp): cluster
+  // element and slot offsets [C+1], per-cluster slot entry offsets [S+C],
+  // element-node entries [E*n], node -> slots CSR [N+1] / [S]
+  int ncl = 0;
+  long nslot = 0;
+  int *cl_eoff = nullptr, *cl_sbase = nullptr;
+  uint16_t *cl_loff = nullptr, *cl_ent = nullptr;
+  int *nsl_off = nullptr, *nsl = nullptr;
 };
 
 // Kernel launch parameters shared by the precision variants.
@@ -141,6 +153,10 @@ struct wfx_ctx {
   // host copies of the mesh and boundary conditions
   std::vector<double> x_h;       // reference layout [N][3]
   std::vector<int> elnod_h;      // [E][n]
+  // element storage order on the device (cluster order): stored index s holds
+  // original element el_s2o[s]; every element field crosses the boundary in
+  // the original order
+  std::vector<int> el_s2o, el_o2s;
   std::vector<int> nodel_h, nodel_loc_h, nodel_offset_h, nodel_count_h;
   std::vector<double> sy0_h;     // per element
   std::vector<int> bc_nod_h[3];
