@@ -27,21 +27,44 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 I4 = 4
 
 
-def fused_bytes(n, s, hg=True, rho_uniform=True):
+def fused_bytes(n, s, slots_per_elem, clusters_per_elem, hg=True, rho_uniform=True, over8_per_node=0.0):
     """Algorithmic bytes per element and per node of the fused kernels: each
     array a kernel touches is moved once, gathered nodal data once per node.
-    s = element real size (8 FP64, 4 FP32-mixed); nodal arrays are double.
-    With a uniform density (the benchmark's) no density is read."""
+    s = element real size (8 FP64, 4 FP32-mixed); nodal positions are
+    double, v, prev_a, u_dt and fi are s bytes.  With a uniform density (the
+    benchmark's) no density is read.  slots_per_elem = (cluster, node) slots
+    per element (the element pass writes one partial force per slot, the
+    node pass reads them with their ids), clusters_per_elem = clusters per
+    element; over8_per_node = share of nodes with more than 8 elements (their
+    CSR offset is read)."""
     rho = 1 if (hg and not rho_uniform) else 0
-    f3_reals = 12 + 12 + 1 + 2 + 1 + 1 + rho + 3 * n  # tau, eps rw; p; ep rw; sy0; sigma_y; [rho]; f
-    sn = s  # v, prev_a, u_dt, fi between steps: double (FP64) or float (FP32-mixed)
+    sn = s
     u_rw = 2 * 3 * 8 if s == 8 else 0  # FP32-mixed recovers u from x at the end of a run
+    # tau, eps read and written; p; plastic strain read and written; sy0;
+    # sigma_y; [rho]
+    f3_reals = 12 + 12 + 1 + 2 + 1 + 1 + rho
+    # elnod, element reals, slot entries (uint16 per element-node), slot
+    # offsets (uint16 per slot and per cluster), partial forces per slot
+    f3_elem = n * I4 + f3_reals * s + 2 * n + 2 * (slots_per_elem + clusters_per_elem) + 3 * s * slots_per_elem
     return {
         "f1_volume": (n * I4 + s, 3 * 8),
-        "f2_nodal": (n * I4 + 2 * s, 2 * I4 + 8 + 2 * 8),  # writes ratio and mass; V_n once per run
-        "f3_element": (n * I4 + f3_reals * s, 3 * 8 + 3 * sn + 8),
-        "f4_node": (n * I4 + 3 * n * s, 2 * I4 + 8 + I4 + 3 * sn + 2 * 3 * 8 + 3 * 2 * 3 * sn + u_rw),
+        # vol gathered (V_n once per run), [rho]; padded first-8 table, count,
+        # [offset], V0_n read, ratio and mass written
+        "f2_nodal": (s + rho * s, 8 * I4 + I4 + 4 * over8_per_node + 8 + 2 * 8),
+        # gathers x, v, ratio
+        "f3_element": (f3_elem, 3 * 8 + 3 * sn + 8),
+        # slot ids and partial forces; offsets, mass, BC slot; v, prev_a, u_dt
+        # read and written, x read and written, [u read and written], fi written
+        "f4_node": (slots_per_elem * (I4 + 3 * s), I4 + 8 + I4 + 3 * 2 * 3 * sn + 2 * 3 * 8 + u_rw + 3 * sn),
     }
+
+
+def domain_fused_bytes(d, s, hg=True):
+    """fused_bytes for domain d (its slots, clusters and node valences)"""
+    N, E, n = d.sizes()
+    cnt = d.field("nodel_count")
+    return fused_bytes(n, s, d.get("slot_count") / E, d.get("cluster_count") / E, hg=hg,
+                       over8_per_node=float(np.mean(cnt > 8)))
 
 
 def api_bytes(n, s, hg=True):
@@ -281,10 +304,10 @@ def config_legs(torch, log):
         N, E, _ = d.sizes()
         d.runFused(20, reaction=False)
         t = timed(lambda: d.runFused(500, reaction=False), torch) / 500
+        fb = domain_fused_bytes(d, 8 if prec == WFX_FP64 else 4, hg=False)
         out[f"cylinder_tets_{tag}"] = {"elements": E, "nodes": N, "ms_per_step": t * 1e3, "value": E / t,
-                                       "step_algorithmic_bytes_per_element": total(fused_bytes(4, 8 if prec ==
-                                                                                               WFX_FP64 else 4,
-                                                                                               hg=False), E, N) / E}
+                                       "step_algorithmic_bytes_per_element": total(fb, E, N) / E,
+                                       "step_hbm_frac": total(fb, E, N) / t / HBM_PEAK}
         log(f"tet cylinder {tag}: {t * 1e3:.3f} ms/step, {E / t:.3e} element-steps/s")
         del d
     return out
@@ -560,7 +583,10 @@ def adjoint_leg(torch, log, steps=2000, cells=30, with_oracle=True):
 def pmc_probe(args):
     """Kernels for hardware-counter runs (rocprofv3 --pmc): a plain 1 GiB
     device-to-device copy whose FETCH_SIZE/WRITE_SIZE calibrate the counters,
-    then a few steps of each path on the benchmark mesh."""
+    then, on the benchmark mesh at the given block sizes, the headline's
+    window: --pre-steps fused steps (past first yield), then three fused
+    steps of each path (the last three dispatches of each kernel are the
+    ones summarised) and one per-step API step."""
     import torch
 
     from weldformfem_amd import WFX_FP32_MIXED, WFX_FP64
@@ -574,12 +600,17 @@ def pmc_probe(args):
     torch.cuda.synchronize()
     del src, dst
     nx, ny, nz, L = (200, 200, 100, [2.0, 2.0, 1.0]) if not args.quick else (100, 100, 50, [2.0, 2.0, 1.0])
-    for prec in (WFX_FP64, WFX_FP32_MIXED):
+    for prec, bs in ((WFX_FP64, args.bs64), (WFX_FP32_MIXED, args.bs32)):
         d, _ = build_domain(prec, nx, ny, nz, L, -2.0, stream)
+        if bs:
+            d.setBlockSizes([int(q) for q in bs.split(",")])
+        d.runFused(args.pre_steps, reaction=False)
         d.runFused(3, reaction=False)
         if prec == WFX_FP64:
             d.run(1, reaction=False)
         torch.cuda.synchronize()
+        print(f"pmc probe: precision {prec} done, largest plastic strain {d.field('pl_strain').max():.3e}",
+              flush=True)
         del d
     print("pmc probe done", flush=True)
 
@@ -604,6 +635,8 @@ def main():
     ap.add_argument("--bs64", default="", help="fused block sizes for FP64 (f1,f2,f3,f4) instead of autotuning")
     ap.add_argument("--bs32", default="", help="the same for FP32-mixed")
     ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--pre-steps", type=int, default=1000,
+                    help="fused steps before the headline's warmup and timed steps (the block yields near step 730)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short run for counter collection: a 1 GiB device copy (calibration), "
                          "then 3 fused steps FP64 and FP32-mixed and one per-step API step")
@@ -636,9 +669,13 @@ def main():
     else:
         bs64 = d64.autotune(reps=5)
     log(f"FP64 block sizes {'(given)' if args.bs64 else '(autotuned)'} {bs64}")
-    steps64 = 0
+    # the timed window lies past first yield (near step 730 at 2 m/s), where
+    # the return map and the plastic-strain stores run
+    d64.runFused(args.pre_steps, reaction=False)
+    steps64 = args.pre_steps
     d64.runFused(args.warmup, reaction=False)
     steps64 += args.warmup
+    yielded64 = float(np.mean(d64.field("pl_strain") > 0))
     s64 = timed(lambda: d64.runFused(args.steps, reaction=False), torch) / args.steps
     steps64 += args.steps
     log(f"FP64 fused {s64 * 1e3:.3f} ms/step")
@@ -662,9 +699,9 @@ def main():
         else:
             bs32 = d32.autotune(reps=5)
         log(f"FP32-mixed block sizes {'(given)' if args.bs32 else '(autotuned)'} {bs32}")
-        d32.runFused(args.warmup, reaction=False)
+        d32.runFused(args.pre_steps + args.warmup, reaction=False)
         s32 = timed(lambda: d32.runFused(args.steps, reaction=False), torch) / args.steps
-        steps32 = args.warmup + args.steps
+        steps32 = args.pre_steps + args.warmup + args.steps
         log(f"FP32-mixed fused {s32 * 1e3:.3f} ms/step")
         # accuracy: both paths to err_steps, then one more step with the reaction
         target = max(args.err_steps, steps64, steps32)
@@ -688,15 +725,15 @@ def main():
                                 "max_eq_plastic_strain": float(d64.field("pl_strain").max()),
                                 "after_steps": target + 1,
                                 "reference": "FP64 GPU path (parity-tested against the oracle)"},
-            "step_algorithmic_bytes_per_element": total(fused_bytes(n, 4), E, N) / E,
-            "step_hbm_frac": total(fused_bytes(n, 4), E, N) / s32 / HBM_PEAK,
+            "step_algorithmic_bytes_per_element": total(domain_fused_bytes(d32, 4), E, N) / E,
+            "step_hbm_frac": total(domain_fused_bytes(d32, 4), E, N) / s32 / HBM_PEAK,
         }
         kt32 = event_times(fused_calls(d32), args.kernel_reps, torch)
-        mixed["per_kernel"] = per_kernel_table(kt32, fused_bytes(n, 4), E, N)
+        mixed["per_kernel"] = per_kernel_table(kt32, domain_fused_bytes(d32, 4), E, N)
         log(f"mixed error: {mixed['rel_err_vs_fp64']}")
 
     kt64 = event_times(fused_calls(d64), args.kernel_reps, torch)
-    fb64 = fused_bytes(n, 8)
+    fb64 = domain_fused_bytes(d64, 8)
     dom = max(kt64, key=kt64.get)
     be, bn = fb64[dom]
     dom_bytes = be * E + bn * N
@@ -763,7 +800,9 @@ def main():
         "config": {"workload": f"hexa compression block {nx}x{ny}x{nz} ({E} elements, {N} nodes), "
                                "fused four-kernel FP64 step",
                    "elements": E, "nodes": N, "dt": dt, "parallelism": "single GPU",
-                   "block_sizes": bs64},
+                   "block_sizes": bs64,
+                   "timed_steps": [args.pre_steps + args.warmup + 1, args.pre_steps + args.warmup + args.steps],
+                   "yielded_element_fraction_at_timing": yielded64},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic,
                      "algorithmic_bytes": dom_bytes, "avg_launch_us": kt64[dom] * 1e6,
@@ -771,13 +810,6 @@ def main():
                      "copy_GBs_torch": copy_torch / 1e9, "copy_GBs_stream": copy_stream / 1e9},
         "step_algorithmic_bytes_per_element": step_bytes / E,
         "step_hbm_frac": step_bytes / s64 / HBM_PEAK,
-        # the per-step API's compulsory bytes (one pass per Domain_d call) at
-        # the fused path's rate: the fraction a kernel-per-call port would need
-        "b_api_equivalent": {"bytes_per_element_step_fp64": total(api_bytes(n, 8), E, N) / E,
-                             "bytes_per_element_step_mixed": total(api_bytes(n, 4), E, N) / E,
-                             "frac_fp64": E / s64 * total(api_bytes(n, 8), E, N) / E / HBM_PEAK,
-                             "frac_mixed": (E / mixed["ms_per_step"] * 1e3 * total(api_bytes(n, 4), E, N) / E
-                                            / HBM_PEAK) if mixed else None},
         "per_kernel": per_kernel_table(kt64, fb64, E, N),
         "fp32_mixed": mixed,
         "api_path_fp64": api,

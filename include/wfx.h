@@ -104,6 +104,16 @@ int wfx_set_thermal(wfx_ctx *ctx, int on, double k_T, double cp_T, double plheat
 /* Domain_d::ThermalCalcs (Thermal.C:27-115) on the state of the last
  * per-step API step */
 int wfx_thermal_calcs(wfx_ctx *ctx, wfx_stream stream);
+/* Material_::exp_T, the material's "thermalExp" (main.C:577): with thermal
+ * coupling on, every step then runs calcThermalExpansion after the strain
+ * rates (Solver_explicit.C:719-720).  0 (the default) leaves the strain rate
+ * unchanged. */
+int wfx_set_thermal_expansion(wfx_ctx *ctx, double exp_T);
+/* Domain_d::calcThermalExpansion (Thermal.C:164-178) on the strain rates of
+ * the last per-step API step: D_ii -= exp_T * (mean over the element's nodes
+ * of the element-node temperature rates at the flat index [e*n + a] equal to
+ * the node id, which is how the reference indexes m_dTedt) */
+int wfx_calc_thermal_expansion(wfx_ctx *ctx, wfx_stream stream);
 /* Rigid-surface penalty contact (Domain_d::CalcContactForces,
  * Contact.C:31-335; TriMesh_d, Mesh.h; setTriMesh + setContactOn,
  * main.C:636-848).  The rigid surfaces: node_count nodes [n][3] with their
@@ -186,6 +196,9 @@ int wfx_set_ch(wfx_ctx *ctx, double alpha, double beta, double gamma);
  * (synchronising) and return WFX_ERR_NONFINITE when they grew; 0 keeps those
  * calls asynchronous and leaves the counters to the query calls. */
 int wfx_set_param(wfx_ctx *ctx, const char *name, double value);
+/* wfx_get_param also reads K, G, cs0, H (the material's derived moduli) and,
+ * read-only, "cluster_count" and "slot_count" (the element clusters of the
+ * fused element pass and their (cluster, node) slots; no counterpart). */
 int wfx_get_param(wfx_ctx *ctx, const char *name, double *value);
 /* nodes whose internal force is summed by wfx_reaction_force (no counterpart
  * in the reference; it has no reaction output): one group */

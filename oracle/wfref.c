@@ -1035,6 +1035,35 @@ void wfref_thermal_calcs(wfref_domain *d) {
   }
 }
 
+void wfref_set_thermal_expansion(wfref_domain *d, double exp_T) { d->exp_T = exp_T; }
+
+/* calcThermalExpansion (Thermal.C:164-178): the element's mean of the
+ * element-node temperature rates, taken as the reference writes it, at the
+ * entries m_dTedt[node] of the element-node array (flat index = node id, the
+ * rates the last ThermalCalcs left), is removed from the diagonal of the
+ * strain rate: D = D - (exp_T dTdt) I. */
+void wfref_calc_thermal_expansion(wfref_domain *d) {
+  int n = d->nodxelem;
+#pragma omp parallel for
+  for (int e = 0; e < d->elem_count; ++e) {
+    double dTdt_gp = 0.0;
+    for (int i = 0; i < n; ++i) {
+      int node = d->elnod[e * n + i];
+      dTdt_gp += 1.0 / n * d->dTedt[node];
+    }
+    double s = d->exp_T * dTdt_gp;
+    double *sr = d->str_rate + 6 * e;
+    /* StrRate - s * Identity(): the diagonal loses s, the off-diagonal
+     * entries s * 0 */
+    sr[0] = sr[0] - s * 1.0;
+    sr[1] = sr[1] - s * 1.0;
+    sr[2] = sr[2] - s * 1.0;
+    sr[3] = sr[3] - s * 0.0;
+    sr[4] = sr[4] - s * 0.0;
+    sr[5] = sr[5] - s * 0.0;
+  }
+}
+
 /* Per-step time step of the BUILD_REMESH CPU solver (Solver_explicit.C:576-600):
  * after the prediction and the Jacobians, dt = cfl * min edge / (c + max |v|)
  * with c = sqrt(K / rho[0]) and |v| the predicted nodal speed; the step's
@@ -1293,6 +1322,7 @@ void wfref_step(wfref_domain *d, double *reaction_z) {
   wfref_calc_nodal_vol(d);
   wfref_calc_nodal_mass_from_vol(d);
   wfref_calc_elem_strain_rates(d);
+  if (d->thermal) wfref_calc_thermal_expansion(d); /* Solver_explicit.C:719-720 */
   wfref_calc_elem_pressure(d);
   wfref_calc_stress_strain(d, d->dt);
   wfref_calc_artificial_viscosity(d);

@@ -67,6 +67,7 @@ typedef struct {
    * heat source, element-node temperature rates */
   int thermal;
   double k_T, cp_T, plheat_frac;
+  double exp_T;       /* thermal expansion coefficient (Material_::exp_T, "thermalExp") */
   double *T;          /* N */
   double *q_plheat;   /* E */
   double *dTedt;      /* nE, [e][a] */
@@ -132,6 +133,9 @@ void wfref_set_elem_sy0(wfref_domain *d, const double *sy0);
  * :590-592) */
 void wfref_set_thermal(wfref_domain *d, int on, double k_T, double cp_T, double plheat_frac);
 void wfref_thermal_calcs(wfref_domain *d);
+/* material thermalExp (main.C:577) and calcThermalExpansion (Thermal.C:164-178) */
+void wfref_set_thermal_expansion(wfref_domain *d, double exp_T);
+void wfref_calc_thermal_expansion(wfref_domain *d);
 /* rigid triangle surfaces (TriMesh_d: nodes, node velocities, triangles,
  * mesh id per triangle) for penalty contact on a tetrahedral body; turns
  * contact on, finds the external faces (SearchExtNodes) */

@@ -49,6 +49,8 @@ def lib():
             "wfref_rigid_update": (None, [vp]),
             "wfref_contact_force": (d, [vp, i]),
             "wfref_thermal_calcs": (None, [vp]),
+            "wfref_set_thermal_expansion": (None, [vp, d]),
+            "wfref_calc_thermal_expansion": (None, [vp]),
             "wfref_set_material_gmt": (None, [vp, d, d, d, d, dp, dp]),
             "wfref_eff_strain_rate": (d, [dp]),
             "wfref_yield_stress": (d, [vp, i, d, d]),
@@ -166,6 +168,13 @@ class RefDomain:
 
     def ThermalCalcs(self):
         self._L.wfref_thermal_calcs(self._d)
+
+    def setThermalExpansion(self, exp_T):
+        """material thermal expansion coefficient (thermalExp)"""
+        self._L.wfref_set_thermal_expansion(self._d, float(exp_T))
+
+    def calcThermalExpansion(self):
+        self._L.wfref_calc_thermal_expansion(self._d)
 
     def setRigidMesh(self, nodes, tris, vel=None, mesh_id=None):
         """rigid triangle surface for penalty contact (tetrahedral body)"""

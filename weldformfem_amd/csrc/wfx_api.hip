@@ -356,6 +356,15 @@ static bool find_field(wfx_ctx *c, const char *name, FieldDesc &f) {
   else if (c->contact.on && s == "rigid_normal") { f.dev = c->contact.rnormal; f.cnt = 3L * c->contact.rt; }
   else if (c->contact.on && s == "rigid_pplane") { f.dev = c->contact.rpplane; f.cnt = c->contact.rt; }
   else if (s == "q_plheat" && E64(c->e64.q_plheat, c->e32.q_plheat)) el(E64(c->e64.q_plheat, c->e32.q_plheat), 1);
+  // thermal expansion: the element-node temperature rates at the flat
+  // indices below N (the reference's m_dTedt[0..N)), and the fused path's
+  // per-element means of the last step
+  else if (s == "dTedt_flat" && E64(c->e64.dte_flat, c->e32.dte_flat)) {
+    f.dev = E64(c->e64.dte_flat, c->e32.dte_flat);
+    f.cnt = N;
+    f.elem = 1;
+  }
+  else if (s == "texp" && E64(c->e64.texp, c->e32.texp)) el(E64(c->e64.texp, c->e32.texp), 1);
   else return false;
   return true;
 }
@@ -792,6 +801,44 @@ int wfx_set_thermal(wfx_ctx *c, int on, double k_T, double cp_T, double plheat_f
   return WFX_OK;
 }
 
+// Material_::exp_T ("thermalExp", main.C:577): with thermal coupling on,
+// every step removes exp_T times the element's temperature rate from the
+// diagonal strain rate (calcThermalExpansion, Thermal.C:164-178).  The rates
+// it reads are the last step's element-node rates at the flat indices
+// [e*n + a] equal to the element's node ids, as the reference reads them:
+// the library keeps those below N (flat_base maps each stored element to
+// e*n of its original index) and starts them at zero.
+int wfx_set_thermal_expansion(wfx_ctx *c, double exp_T) {
+  REQUIRE_MESH(c);
+  REQUIRE(std::isfinite(exp_T), WFX_ERR_ARG, "thermal expansion coefficient must be finite");
+  if (exp_T != 0.0) {
+    const size_t E = c->E, N = c->N;
+    if (!c->ms.flat_base) {
+      WFX_CHECK_HIP(dalloc(c, &c->ms.flat_base, E));
+      std::vector<int> fb(E);
+      for (size_t st = 0; st < E; ++st) fb[st] = c->el_s2o[st] * c->n;
+      WFX_CHECK_HIP(hipMemcpy(c->ms.flat_base, fb.data(), E * sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (c->precision == WFX_FP64) {
+      if (!c->e64.dte_flat) WFX_CHECK_HIP(dalloc(c, &c->e64.dte_flat, N));
+      if (!c->e64.texp) WFX_CHECK_HIP(dalloc(c, &c->e64.texp, E));
+    } else {
+      if (!c->e32.dte_flat) WFX_CHECK_HIP(dalloc(c, &c->e32.dte_flat, N));
+      if (!c->e32.texp) WFX_CHECK_HIP(dalloc(c, &c->e32.texp, E));
+    }
+  }
+  c->thermal.exp_T = exp_T;
+  drop_graph(c);
+  return WFX_OK;
+}
+
+int wfx_calc_thermal_expansion(wfx_ctx *c, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(!c->fused_stale, WFX_ERR_STATE, "calcThermalExpansion needs the strain rates of a per-step API step");
+  LAUNCH(BY_PRECISION(c, launch_thermal_expansion, c, as_stream(s)));
+  return WFX_OK;
+}
+
 int wfx_thermal_calcs(wfx_ctx *c, wfx_stream s) {
   REQUIRE_MESH(c);
   REQUIRE(c->thermal.on, WFX_ERR_STATE, "thermal coupling is off: call wfx_set_thermal");
@@ -979,6 +1026,9 @@ int wfx_get_param(wfx_ctx *c, const char *name, double *value) {
   if (s == "cs0") { *value = m.cs0; return WFX_OK; }
   if (s == "H") { *value = m.H; return WFX_OK; }
   if (s == "check_status") { *value = c->check_status; return WFX_OK; }
+  // read-only: the element clusters of the fused element pass
+  if (s == "cluster_count") { *value = c->ms.ncl; return WFX_OK; }
+  if (s == "slot_count") { *value = (double)c->ms.nslot; return WFX_OK; }
   double *p = param_slot(c, name);
   REQUIRE(p, WFX_ERR_ARG, std::string("unknown parameter ") + name);
   *value = *p;
@@ -1072,6 +1122,10 @@ int wfx_init(wfx_ctx *c, wfx_stream s) {
   LAUNCH(BY_PRECISION(c, launch_nodal_mass, c, st));
   WFX_CHECK_HIP(hipMemsetAsync(c->status, 0, 16 * sizeof(unsigned int), st));
   c->status_seen[0] = c->status_seen[1] = 0;
+  {  // the temperature rates thermal expansion reads start at zero
+    void *dte = c->precision == WFX_FP64 ? (void *)c->e64.dte_flat : (void *)c->e32.dte_flat;
+    if (dte) WFX_CHECK_HIP(hipMemsetAsync(dte, 0, N * (c->precision == WFX_FP64 ? sizeof(double) : sizeof(float)), st));
+  }
   if (c->contact.on) {
     // contact state starts clear; calcMinEdgeLength before the solve
     // (main.C:862) gives the element heights
@@ -1202,6 +1256,7 @@ static int step_api(wfx_ctx *c, hipStream_t st, double *hist) {
   LAUNCH(BY_PRECISION(c, launch_nodal_vol, c, st));
   LAUNCH(BY_PRECISION(c, launch_nodal_mass, c, st));
   LAUNCH(BY_PRECISION(c, launch_strain_rates, c, st));
+  if (c->thermal.on) LAUNCH(BY_PRECISION(c, launch_thermal_expansion, c, st));  // Solver_explicit.C:719-720
   LAUNCH(BY_PRECISION(c, launch_pressure, c, st));
   LAUNCH(BY_PRECISION(c, launch_stress_strain, c, c->prm.dt, st));
   LAUNCH(BY_PRECISION(c, launch_artificial_viscosity, c, st));

@@ -82,14 +82,24 @@ __global__ __launch_bounds__(kBlock) void f0_vsum0(int N, const int *__restrict_
 
 // F1: vol = detJ * w (calcElemJAndDerivatives determinant + CalcElemVol);
 // with minl_part also the block's minimum edge length (per-step CFL time step)
+// With texp (thermal expansion): also each element's mean of the last
+// step's element-node temperature rates at the flat indices of its node ids
+// (calcThermalExpansion's dTdt), for the element pass.
 template <typename T, int NN, int BS>
 __global__ __launch_bounds__(BS) void f1_volume(int E, int N, const int *__restrict__ elnod,
                                                     const double *__restrict__ x,
                                                     T *__restrict__ vol, double *__restrict__ minl_part,
-                                                    unsigned int *__restrict__ status) {
+                                                    unsigned int *__restrict__ status,
+                                                    const T *__restrict__ dte_flat, T *__restrict__ texp) {
   int e = gtid();
   double medge = kMinEdgeInit;
   if (e < E) {
+  if (texp) {
+    double m = 0.0;
+#pragma unroll
+    for (int a = 0; a < NN; ++a) m += 1.0 / NN * (double)dte_flat[elnod[(long)a * E + e]];
+    texp[e] = (T)m;
+  }
   T X[NN][3];
   if constexpr (std::is_same<T, double>::value) {
 #pragma unroll
@@ -215,6 +225,14 @@ struct FusedParams {
   const double *dtp;  // non-null: the step's dt from the device (per-step CFL time step)
   double plheat_frac; // thermal coupling: fraction of plastic work into heat
   double k_T, cp_T;   //   conductivity and specific heat (rates when mp.Tn is set)
+  // thermal expansion (with thermal coupling; texp null: none): exp_T, the
+  // element means of the last rates (from F1), and where this step's rates
+  // go: flat index flat_base[e] + a below N, divided by the node's mass
+  double exp_T;
+  const void *texp;
+  void *dte_flat;
+  const int *flat_base;
+  const double *mdiag;
 };
 
 // dH(d,a) = adj(J)(d,:) . dN_a/dxi from the cofactor matrix C (adj = C^T),
@@ -359,6 +377,17 @@ __device__ __forceinline__ void element_chain(
   }
   s01 = s01 * T(0.5); s12 = s12 * T(0.5); s02 = s02 * T(0.5);
   w01 = w01 * T(0.5); w12 = w12 * T(0.5); w02 = w02 * T(0.5);
+  if constexpr (TH) {  // calcThermalExpansion (Thermal.C:164-178)
+    if (fp.texp) {
+      const double sh = fp.exp_T * (double)static_cast<const T *>(fp.texp)[e];
+      s00 = (T)((double)s00 - sh * 1.0);
+      s11 = (T)((double)s11 - sh * 1.0);
+      s22 = (T)((double)s22 - sh * 1.0);
+      s01 = (T)((double)s01 - sh * 0.0);
+      s12 = (T)((double)s12 - sh * 0.0);
+      s02 = (T)((double)s02 - sh * 0.0);
+    }
+  }
 
   // hourglass mode amplitudes now, so the nodal velocities die here and do
   // not stay live through the stress update (register pressure)
@@ -583,7 +612,7 @@ __device__ __forceinline__ void element_chain(
     // the node's thermal term times its mass: -(Kt T)_a + heat / cp
     T th = T(0);
     if constexpr (TH) th = (T)(heat - ((double)h0 * gT0 + (double)h1 * gT1 + (double)h2 * gT2));
-    sink(a, f0, f1, f2, th);
+    sink(a, g[a], f0, f1, f2, th);
   }
 }
 
@@ -625,16 +654,16 @@ __global__ __launch_bounds__(kCluster) void f3_cluster(
   constexpr int R = kNodeVals<TH>;
   __shared__ __align__(16) unsigned char lds_raw[kF3Lds<T, NN, LAW, TH>];
   __shared__ uint32_t l_ent[kCluster * NN / 2];  // the cluster's entries, two per word
-  __shared__ uint16_t l_off[kCluster * NN + 1];  // its slots' entry offsets
   T *lf = reinterpret_cast<T *>(lds_raw);
   const int c = xcd_block(), t = threadIdx.x;
   const int e0 = ca.eoff[c], ne = ca.eoff[c + 1] - e0;
   const int s0 = ca.sbase[c], M = ca.sbase[c + 1] - s0;
   const uint16_t *lo = ca.loff + s0 + c;
   // The cluster's slot tables are loaded into registers before the element
-  // chain, so that the loads complete under its arithmetic, and stored to LDS
-  // after it (entries: ne * NN uint16 from a 4-byte aligned start, since
-  // e0 * NN is even).
+  // chain, so that the loads complete under its arithmetic: the entries
+  // (ne * NN uint16 from a 4-byte aligned start, since e0 * NN is even) go to
+  // LDS after it; the entry offsets of the thread's own slots t and
+  // t + kCluster stay in registers.
   const uint32_t *en32 = reinterpret_cast<const uint32_t *>(ca.ent + (long)e0 * NN);
   const int nw = ne * NN / 2;
   uint32_t ew[NN / 2];
@@ -643,20 +672,26 @@ __global__ __launch_bounds__(kCluster) void f3_cluster(
     const int w = t + k * kCluster;
     ew[k] = w < nw ? en32[w] : 0u;
   }
-  uint16_t ow[2];
+  uint32_t ow[2];  // (end << 16) | start of slots t and t + kCluster
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int q = t + k * kCluster;
-    ow[k] = q <= M ? lo[q] : (uint16_t)0;
+    ow[k] = q < M ? ((uint32_t)lo[q + 1] << 16) | lo[q] : 0u;
   }
   if (t < ne) {
     element_chain<T, NN, LAW, kCluster, TH>(
         e0 + t, E, N, fp, elnod, x, v, ratio, vol0, rho, sy0, He, tau_io, eps_io, p_out, pl_strain, sigma_y,
-        q_plheat, reinterpret_cast<double *>(lds_raw) + t, [&](int a, T f0, T f1, T f2, T th) {
+        q_plheat, reinterpret_cast<double *>(lds_raw) + t, [&](int a, int node, T f0, T f1, T f2, T th) {
           lf[(a * R + 0) * kCluster + t] = f0;
           lf[(a * R + 1) * kCluster + t] = f1;
           lf[(a * R + 2) * kCluster + t] = f2;
-          if constexpr (TH) lf[(a * R + 3) * kCluster + t] = th;
+          if constexpr (TH) {
+            lf[(a * R + 3) * kCluster + t] = th;
+            if (fp.dte_flat) {  // this step's rate at the flat index calcThermalExpansion reads
+              const long q = (long)fp.flat_base[e0 + t] + a;
+              if (q < N) static_cast<T *>(fp.dte_flat)[q] = (T)((double)th / fp.mdiag[node]);
+            }
+          }
         });
   }
 #pragma unroll
@@ -664,16 +699,11 @@ __global__ __launch_bounds__(kCluster) void f3_cluster(
     const int w = t + k * kCluster;
     if (w < nw) l_ent[w] = ew[k];
   }
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int q = t + k * kCluster;
-    if (q <= M) l_off[q] = ow[k];
-  }
-  for (int q = t + 2 * kCluster; q <= M; q += kCluster) l_off[q] = lo[q];  // clusters with > 2 slots per thread
   __syncthreads();
   const uint16_t *en = reinterpret_cast<const uint16_t *>(l_ent);
-  for (int j = t; j < M; j += kCluster) {
-    const int k0 = l_off[j], k1 = l_off[j + 1];
+  for (int j = t, r = 0; j < M; j += kCluster, ++r) {
+    const uint32_t o = r < 2 ? ow[r] : ((uint32_t)lo[j + 1] << 16) | lo[j];  // > 2 slots per thread: reloaded
+    const int k0 = o & 0xffffu, k1 = o >> 16;
     double acc[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = 0.0;
@@ -697,7 +727,7 @@ template <typename T, int NN>
 __device__ __forceinline__ void thermal_rates(int e, long Es, const int (&g)[NN], const double (&h)[3][NN],
                                               double detJ, double vol, double q, const double *__restrict__ Tn,
                                               const double *__restrict__ mdiag, double k_T, double cp_T,
-                                              T *__restrict__ dTedt) {
+                                              T *__restrict__ dTedt, long q0, int N, T *__restrict__ dte_flat) {
   const double w = (NN == 4) ? 1.0 / 6.0 : 8.0;
   double Te[NN];
 #pragma unroll
@@ -722,6 +752,7 @@ __device__ __forceinline__ void thermal_rates(int e, long Es, const int (&g)[NN]
     double r = -m_inv * acc;
     r += pow_per_node / (m * cp_T);
     dTedt[fidx(i, e, Es, NN)] = (T)r;
+    if (dte_flat && q0 + i < N) dte_flat[q0 + i] = (T)r;  // the entries thermal expansion reads
   }
 }
 
@@ -730,7 +761,8 @@ template <typename T, int NN>
 __global__ __launch_bounds__(kBlock) void k_thermal_elem(
     int E, const int *__restrict__ elnod, const T *__restrict__ dH, const T *__restrict__ detJ,
     const T *__restrict__ vol, const T *__restrict__ q, const double *__restrict__ Tn,
-    const double *__restrict__ mdiag, double k_T, double cp_T, T *__restrict__ dTedt) {
+    const double *__restrict__ mdiag, double k_T, double cp_T, T *__restrict__ dTedt,
+    const int *__restrict__ flat_base, int N, T *__restrict__ dte_flat) {
   const int e = gtid();
   if (e >= E) return;
   int g[NN];
@@ -741,7 +773,27 @@ __global__ __launch_bounds__(kBlock) void k_thermal_elem(
 #pragma unroll
     for (int d = 0; d < 3; ++d) h[d][a] = (double)dH[(long)(d * NN + a) * E + e];
   }
-  thermal_rates<T, NN>(e, E, g, h, (double)detJ[e], (double)vol[e], (double)q[e], Tn, mdiag, k_T, cp_T, dTedt);
+  thermal_rates<T, NN>(e, E, g, h, (double)detJ[e], (double)vol[e], (double)q[e], Tn, mdiag, k_T, cp_T, dTedt,
+                       dte_flat ? (long)flat_base[e] : 0L, N, dte_flat);
+}
+
+// calcThermalExpansion (Thermal.C:164-178): the strain rate loses (exp_T times
+// the mean of the last step's element-node temperature rates at the flat
+// indices given by the element's node ids) on its diagonal; the shear rates
+// lose that times 0.
+template <typename T, int NN>
+__global__ __launch_bounds__(kBlock) void k_thermal_expansion(int E, const int *__restrict__ elnod,
+                                                              const T *__restrict__ dte_flat, double exp_T,
+                                                              T *__restrict__ sr) {
+  const int e = gtid();
+  if (e >= E) return;
+  const long Es = E;
+  double m = 0.0;
+#pragma unroll
+  for (int i = 0; i < NN; ++i) m += 1.0 / NN * (double)dte_flat[elnod[(long)i * Es + e]];
+  const double sh = exp_T * m;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sr[k * Es + e] = (T)((double)sr[k * Es + e] - sh * (k < 3 ? 1.0 : 0.0));
 }
 
 // T += (sum of the node's element-node rates in CSR order) * dt; the rates
@@ -1081,12 +1133,13 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
   }
   if (phase == 1) {
     double *minl = cfl ? c->cfl_part : nullptr;
+    const bool texp = c->thermal.on && c->thermal.exp_T != 0.0;
     if (c->n == 8)
       hipLaunchKernelGGL((f1_volume<T, 8, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol, minl,
-                         c->status);
+                         c->status, el.dte_flat, texp ? el.texp : (T *)nullptr);
     else
       hipLaunchKernelGGL((f1_volume<T, 4, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol, minl,
-                         c->status);
+                         c->status, el.dte_flat, texp ? el.texp : (T *)nullptr);
     if (cfl) {
       hipError_t e = launch_cfl_reduce<T>(c, gE, step, s);
       // calcMinEdgeLength also refreshes the element heights contact uses
@@ -1113,6 +1166,12 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
     fp.plheat_frac = c->thermal.plheat_frac;
     fp.k_T = c->thermal.k_T;
     fp.cp_T = c->thermal.cp_T;
+    const bool texp = c->thermal.on && c->thermal.exp_T != 0.0;
+    fp.exp_T = c->thermal.exp_T;
+    fp.texp = texp ? el.texp : nullptr;
+    fp.dte_flat = texp ? el.dte_flat : nullptr;
+    fp.flat_base = c->ms.flat_base;
+    fp.mdiag = c->nd.mdiag;
     const bool rate = law_class(fp.mp.model) == LAW_RATE, th = c->thermal.on;
     auto kern = (c->n == 8) ? (rate ? f3_pick<T, 8, LAW_RATE>(th) : f3_pick<T, 8, LAW_PLAIN>(th))
                             : (rate ? f3_pick<T, 4, LAW_RATE>(th) : f3_pick<T, 4, LAW_PLAIN>(th));
@@ -1190,11 +1249,24 @@ hipError_t launch_thermal(wfx_ctx *c, hipStream_t s) {
   auto &el = elem<T>(c);
   const auto &th = c->thermal;
   auto kern = c->n == 8 ? k_thermal_elem<T, 8> : k_thermal_elem<T, 4>;
+  const bool texp = th.on && th.exp_T != 0.0;
   hipLaunchKernelGGL(kern, dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, c->ms.elnod, el.dH, el.detJ, el.vol,
-                     el.q_plheat, c->nd.T, c->nd.mdiag, th.k_T, th.cp_T, el.dTedt);
+                     el.q_plheat, c->nd.T, c->nd.mdiag, th.k_T, th.cp_T, el.dTedt, c->ms.flat_base, c->N,
+                     texp ? el.dte_flat : (T *)nullptr);
   hipLaunchKernelGGL((k_thermal_node<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N, (long)c->E, c->n,
                      c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.dTedt, c->prm.dt,
                      c->nd.T);
+  return hipGetLastError();
+}
+
+// calcThermalExpansion of one per-step API step (no-op at exp_T = 0)
+template <typename T>
+hipError_t launch_thermal_expansion(wfx_ctx *c, hipStream_t s) {
+  auto &el = elem<T>(c);
+  if (c->thermal.exp_T == 0.0 || !el.dte_flat) return hipSuccess;
+  auto kern = c->n == 8 ? k_thermal_expansion<T, 8> : k_thermal_expansion<T, 4>;
+  hipLaunchKernelGGL(kern, dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, c->ms.elnod, el.dte_flat, c->thermal.exp_T,
+                     el.str_rate);
   return hipGetLastError();
 }
 

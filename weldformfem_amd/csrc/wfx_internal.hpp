@@ -81,6 +81,11 @@ struct DevElem {
   T *q_plheat = nullptr, *dTedt = nullptr;  // thermal coupling: plastic heat, element-node rates [n][E]
   T *avq = nullptr;  // fused path: the artificial-viscosity term added to the diagonal stresses
   T *part = nullptr;  // fused path: per-slot partial nodal forces [4][S] (fx, fy, fz, thermal term)
+  // thermal expansion: the element-node temperature rates of the last step at
+  // the reference's flat indices [e*n + a] below N (the entries
+  // calcThermalExpansion reads, at node ids), and the fused path's
+  // per-element mean of them [E]
+  T *dte_flat = nullptr, *texp = nullptr;
 };
 
 // Rigid-surface penalty contact (Contact.C, TriMesh_d): penalty factor
@@ -108,6 +113,7 @@ struct Contact {
 struct Thermal {
   int on = 0;
   double k_T = 0, cp_T = 0, plheat_frac = 0.9;
+  double exp_T = 0;  // thermal expansion coefficient (Material_::exp_T); 0: none
 };
 
 struct DevNode {
@@ -131,6 +137,7 @@ struct DevMesh {
   int *cl_eoff = nullptr, *cl_sbase = nullptr;
   uint16_t *cl_loff = nullptr, *cl_ent = nullptr;
   int *nsl_off = nullptr, *nsl = nullptr;
+  int *flat_base = nullptr;  // thermal expansion: original element * n of each stored element
 };
 
 // Kernel launch parameters shared by the precision variants.

@@ -1,5 +1,6 @@
 // FP64 instantiation of the per-step kernels (built with -ffp-contract=off)
-// and the precision-independent nodal launchers.
+// and the precision-independent nodal launchers; the fused FP64 step is
+// instantiated in wfx_f64.hip.
 #include "wfx_kernels.hpp"
 #include "wfx_fused.hpp"
 #include "wfx_launch.hpp"
@@ -7,13 +8,9 @@
 namespace wfx {
 
 WFX_DEFINE_LAUNCHERS(double)
-template hipError_t launch_vsum0<double>(wfx_ctx *, hipStream_t);
-template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipStream_t, int);
 template hipError_t launch_cfl_step<double>(wfx_ctx *, int, hipStream_t);
 template hipError_t launch_thermal<double>(wfx_ctx *, hipStream_t);
-template hipError_t launch_fused_steps<double>(wfx_ctx *, int, double *, hipStream_t);
-template hipError_t launch_fused_pack<double>(wfx_ctx *, hipStream_t);
-template hipError_t launch_fused_unpack<double>(wfx_ctx *, hipStream_t);
+template hipError_t launch_thermal_expansion<double>(wfx_ctx *, hipStream_t);
 
 hipError_t launch_update_prediction(wfx_ctx *c, hipStream_t s) {
   long n3 = 3L * c->N;

@@ -43,6 +43,9 @@ template <typename T> hipError_t launch_vsum0(wfx_ctx *c, hipStream_t s);
 template <typename T> hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s, int step);
 template <typename T> hipError_t launch_cfl_step(wfx_ctx *c, int step, hipStream_t s);
 template <typename T> hipError_t launch_thermal(wfx_ctx *c, hipStream_t s);
+template <typename T> hipError_t launch_thermal_expansion(wfx_ctx *c, hipStream_t s);
+extern template hipError_t launch_thermal_expansion<double>(wfx_ctx *, hipStream_t);
+extern template hipError_t launch_thermal_expansion<float>(wfx_ctx *, hipStream_t);
 extern template hipError_t launch_thermal<double>(wfx_ctx *, hipStream_t);
 extern template hipError_t launch_thermal<float>(wfx_ctx *, hipStream_t);
 extern template hipError_t launch_cfl_step<double>(wfx_ctx *, int, hipStream_t);

@@ -160,6 +160,14 @@ class Domain:
     def ThermalCalcs(self):
         self._call(self._L.wfx_thermal_calcs)
 
+    def setThermalExpansion(self, exp_T):
+        """material thermalExp (Material_::exp_T): with thermal coupling,
+        every step runs calcThermalExpansion after the strain rates"""
+        check(self._L.wfx_set_thermal_expansion(self._h, float(exp_T)))
+
+    def calcThermalExpansion(self):
+        self._call(self._L.wfx_calc_thermal_expansion)
+
     def setMaxStrainRate(self, edot):
         """Domain_d::m_max_edot: cap of the rate the tangent modulus sees"""
         check(self._L.wfx_set_max_strain_rate(self._h, float(edot)))
