@@ -105,13 +105,19 @@ __global__ void k_rec_pre(long N, long E, const double *x, const T *v, const T *
   else if (i < 2 * n3 + 6 * E) dst[o + i - 2 * n3] = (double)tau[i - 2 * n3];
   else if (i < 2 * n3 + 7 * E) dst[o + i - 2 * n3] = (double)ep[i - 2 * n3 - 6 * E];
 }
+// (the ANP ratio as the fused node pass leaves it: double, or for FP32-mixed
+// 1 + a float deviation in the same buffer)
 template <typename T>
 __global__ void k_rec_post(long N, const T *fi, const double *m, const double *r, double *dst) {
   long i = tid();
   const long n3 = 3 * N;
   if (i < n3) dst[6 * N + i] = (double)fi[i];
   else if (i < n3 + N) dst[9 * N + i - n3] = m[i - n3];
-  else if (i < n3 + 2 * N) dst[10 * N + i - n3 - N] = r[i - n3 - N];
+  else if (i < n3 + 2 * N) {
+    const long k = i - n3 - N;
+    if constexpr (std::is_same<T, double>::value) dst[10 * N + k] = r[k];
+    else dst[10 * N + k] = 1.0 + (double)reinterpret_cast<const float *>(r)[k];
+  }
 }
 
 __global__ void k_zero(long n, double *p) {

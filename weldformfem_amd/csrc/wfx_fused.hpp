@@ -65,6 +65,14 @@ __device__ __forceinline__ T ld_stream(const T *p) {
   return __builtin_nontemporal_load(p);
 }
 
+// The nodal ANP ratio F2 leaves for the element pass: double, or (FP32-mixed)
+// 1 + a float deviation
+template <typename T>
+__device__ __forceinline__ double anp_ratio_at(const double *ratio, int i) {
+  if constexpr (std::is_same<T, double>::value) return ratio[i];
+  else return 1.0 + (double)reinterpret_cast<const float *>(ratio)[i];
+}
+
 // V0_n: CSR-ordered sum of the initial volumes (calcElemPressure :698-705)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void f0_vsum0(int N, const int *__restrict__ off,
@@ -190,7 +198,11 @@ __global__ __launch_bounds__(BS) void f2_nodal(int N, int nodxelem, const int *_
     for (int j = 0; j < kChunk; ++j)
       if (k + j < c) s += vv[j];
   }
-  ratio[i] = s / vsum0[i];
+  // the ANP ratio V_n / V0_n: FP64 stores it; FP32-mixed stores its
+  // deviation from 1 as a float in the same buffer (the element pass adds the
+  // 1 back; the deviation keeps the float's relative precision)
+  if constexpr (std::is_same<T, double>::value) ratio[i] = s / vsum0[i];
+  else reinterpret_cast<float *>(ratio)[i] = (float)(s / vsum0[i] - 1.0);
   double vn = s;
   vn /= nodxelem;
   double dc = (double)c, mass = 0.0;
@@ -450,7 +462,7 @@ __device__ __forceinline__ void element_chain(
   const bool is_contact = elem_in_contact<NN>(pp, [&](int a) { return g[a]; });
   double J_avg = 0.0;
 #pragma unroll
-  for (int a = 0; a < NN; ++a) J_avg += ratio[g[a]];
+  for (int a = 0; a < NN; ++a) J_avg += anp_ratio_at<T>(ratio, g[a]);
   J_avg /= NN;
   const bool general = press_general(pp, (double)volv, J_avg);
   double J_local = 0.0;
@@ -833,6 +845,7 @@ struct NodeParams {
   double pred_u;   // (0.5-beta)*dt   (next prediction)
   double pred_v;   // (1-gamma)*dt
   int predict;     // run the next step's prediction
+  int write_fi;    // store the internal force (last step of a run, or its reaction is recorded)
   int has_fe;      // external forces present
   const double *dtp;  // per-step CFL time step: the step's dt on the device
   double *vmax_part;  //   and the per-block largest predicted speed
@@ -875,7 +888,7 @@ __device__ __forceinline__ double node_update(
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const long id = d * Ns + i;
-    fi[id] = (T)f[d];
+    if (np.write_fi) fi[id] = (T)f[d];
     double fed = np.has_fe ? fe[id] : 0.0;
     double a = (fed - f[d]) / m;                       // calcAccel
     if constexpr (CONTACT) a += cf[id] / m;             //   + contact force
@@ -1084,6 +1097,7 @@ inline NodeParams node_params(wfx_ctx *c, int predict) {
   np.pred_u = (0.5 - P.beta) * P.dt;
   np.pred_v = (1.0 - P.gamma) * P.dt;
   np.predict = predict;
+  np.write_fi = c->fused_write_fi;
   np.has_fe = c->has_fe;
   np.dtp = nullptr;
   np.vmax_part = nullptr;
@@ -1307,10 +1321,16 @@ hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t 
       hipError_t e = launch_ext_face_areas(c, s);  // every 10 steps, at the step's start
       if (e != hipSuccess) return e;
     }
+    // the internal force is observed after the run and by the reaction history
+    c->fused_write_fi = (k + 1 == nsteps || hist) ? 1 : 0;
     for (int ph = 1; ph <= 4; ++ph) {
       hipError_t e = launch_fused_phase<T>(c, ph, k + 1 < nsteps, s, k);
-      if (e != hipSuccess) return e;
+      if (e != hipSuccess) {
+        c->fused_write_fi = 1;
+        return e;
+      }
     }
+    c->fused_write_fi = 1;
     if (hist) {
       hipError_t e = mixed ? launch_reaction_f32(c, c->fi32, hist + (long)k * c->ngroups, s)
                            : launch_reaction(c, 2, hist + (long)k * c->ngroups, s);
