@@ -245,6 +245,9 @@ struct FusedParams {
   void *dte_flat;
   const int *flat_base;
   const double *mdiag;
+  // store the element outputs no later pass of the step reads (pressure,
+  // yield stress, plastic heat, viscosity term): on a run's last step
+  int write_out;
 };
 
 // dH(d,a) = adj(J)(d,:) . dN_a/dxi from the cofactor matrix C (adj = C^T),
@@ -477,7 +480,7 @@ __device__ __forceinline__ void element_chain(
                    (double)shape_dH<T, NN>(C, 2, a) * v[2L * N + g[a]];
         return div_v;
       });
-  p_out[e] = pe;
+  if (fp.write_out) p_out[e] = pe;
 
   // ---- Jaumann update and J2 return (CalcStressStrain) -------------------
   T dt = (T)(fp.dtp ? *fp.dtp : fp.dt);
@@ -523,7 +526,7 @@ __device__ __forceinline__ void element_chain(
   } else {
     sy = yield_stress<T, LAW>(mp, rt, ld_stream(&sy0[e]), ep, edot, rc);
   }
-  sigma_y[e] = sy;
+  if (fp.write_out) sigma_y[e] = sy;
   T dep = T(0);
   if (sy < sig_trial) {
     T H = per_elem ? Hb : tangent_modulus<T, LAW>(mp, rt, ep, cap_rate(mp, edot), rc);
@@ -543,7 +546,7 @@ __device__ __forceinline__ void element_chain(
                           f * (sig.zz - T(0.5) * (sig.xx + sig.yy)), T(1.5) * f * (sig.xy), T(1.5) * f * (sig.yz),
                           T(1.5) * f * (sig.xz), dt, fp.plheat_frac);
     }
-    q_plheat[e] = q;
+    if (fp.write_out) q_plheat[e] = q;
     heat = 0.9 * (double)q * (double)volv / NN / fp.cp_T;
   }
   // ---- calcArtificialViscosity (Mechanical.C:1948-1977), after the stress
@@ -562,7 +565,7 @@ __device__ __forceinline__ void element_chain(
       sig.yy = (T)((double)sig.yy + qs);
       sig.zz = (T)((double)sig.zz + qs);
     }
-    static_cast<T *>(fp.avq)[e] = (T)qs;
+    if (fp.write_out) static_cast<T *>(fp.avq)[e] = (T)qs;
   }
   st_stream(&eps_io[0 * Es + e], ld_stream(&eps_io[0 * Es + e]) + D.xx * dt);
   st_stream(&eps_io[1 * Es + e], ld_stream(&eps_io[1 * Es + e]) + D.yy * dt);
@@ -1186,6 +1189,7 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
     fp.dte_flat = texp ? el.dte_flat : nullptr;
     fp.flat_base = c->ms.flat_base;
     fp.mdiag = c->nd.mdiag;
+    fp.write_out = c->fused_write_out;
     const bool rate = law_class(fp.mp.model) == LAW_RATE, th = c->thermal.on;
     auto kern = (c->n == 8) ? (rate ? f3_pick<T, 8, LAW_RATE>(th) : f3_pick<T, 8, LAW_PLAIN>(th))
                             : (rate ? f3_pick<T, 4, LAW_RATE>(th) : f3_pick<T, 4, LAW_PLAIN>(th));
@@ -1321,16 +1325,18 @@ hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t 
       hipError_t e = launch_ext_face_areas(c, s);  // every 10 steps, at the step's start
       if (e != hipSuccess) return e;
     }
-    // the internal force is observed after the run and by the reaction history
+    // the internal force is observed after the run and by the reaction
+    // history, the element outputs after the run
     c->fused_write_fi = (k + 1 == nsteps || hist) ? 1 : 0;
+    c->fused_write_out = (k + 1 == nsteps) ? 1 : 0;
     for (int ph = 1; ph <= 4; ++ph) {
       hipError_t e = launch_fused_phase<T>(c, ph, k + 1 < nsteps, s, k);
       if (e != hipSuccess) {
-        c->fused_write_fi = 1;
+        c->fused_write_fi = c->fused_write_out = 1;
         return e;
       }
     }
-    c->fused_write_fi = 1;
+    c->fused_write_fi = c->fused_write_out = 1;
     if (hist) {
       hipError_t e = mixed ? launch_reaction_f32(c, c->fi32, hist + (long)k * c->ngroups, s)
                            : launch_reaction(c, 2, hist + (long)k * c->ngroups, s);

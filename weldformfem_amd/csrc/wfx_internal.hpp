@@ -210,6 +210,7 @@ struct wfx_ctx {
   uint64_t graph_sig = 0;
   bool fused_stale = false;  // per-step intermediates not maintained by the fused path
   int fused_write_fi = 1;    // the fused node pass stores the internal force (cleared inside runs)
+  int fused_write_out = 1;   // the fused element pass stores p, sigma_y, plastic heat (cleared inside runs)
   double *scalars = nullptr;      // device scratch for reductions (reaction force)
   // this context's device's address of the library's static scratch array
   // (the allocation-free entry point's reduction partials)
