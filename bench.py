@@ -31,7 +31,7 @@ def fused_bytes(n, s, slots_per_elem, clusters_per_elem, hg=True, rho_uniform=Tr
     """Algorithmic bytes per element and per node of the fused kernels: each
     array a kernel touches is moved once, gathered nodal data once per node.
     s = element real size (8 FP64, 4 FP32-mixed); nodal positions are
-    double, v, prev_a, u_dt and fi are s bytes.  With a uniform density (the
+    double, v, prev_a, u_dt, fi and the ANP ratio are s bytes.  With a uniform density (the
     benchmark's) no density is read.  slots_per_elem = (cluster, node) slots
     per element (the element pass writes one partial force per slot, the
     node pass reads them with their ids), clusters_per_elem = clusters per
@@ -40,22 +40,24 @@ def fused_bytes(n, s, slots_per_elem, clusters_per_elem, hg=True, rho_uniform=Tr
     rho = 1 if (hg and not rho_uniform) else 0
     sn = s
     u_rw = 2 * 3 * 8 if s == 8 else 0  # FP32-mixed recovers u from x at the end of a run
-    # tau, eps read and written; p; plastic strain read and written; sy0;
-    # sigma_y; [rho]
-    f3_reals = 12 + 12 + 1 + 2 + 1 + 1 + rho
+    # tau, eps read and written; plastic strain read and written; sy0;
+    # [rho] (pressure and yield stress are stored on a run's last step only)
+    f3_reals = 12 + 12 + 2 + 1 + rho
     # elnod, element reals, slot entries (uint16 per element-node), slot
     # offsets (uint16 per slot and per cluster), partial forces per slot
     f3_elem = n * I4 + f3_reals * s + 2 * n + 2 * (slots_per_elem + clusters_per_elem) + 3 * s * slots_per_elem
     return {
         "f1_volume": (n * I4 + s, 3 * 8),
         # vol gathered (V_n once per run), [rho]; padded first-8 table, count,
-        # [offset], V0_n read, ratio and mass written
-        "f2_nodal": (s + rho * s, 8 * I4 + I4 + 4 * over8_per_node + 8 + 2 * 8),
+        # [offset], V0_n read, mass and ANP ratio written (FP32-mixed stores the
+        # ratio's deviation from 1 as a float)
+        "f2_nodal": (s + rho * s, 8 * I4 + I4 + 4 * over8_per_node + 8 + 8 + s),
         # gathers x, v, ratio
-        "f3_element": (f3_elem, 3 * 8 + 3 * sn + 8),
+        "f3_element": (f3_elem, 3 * 8 + 3 * sn + s),
         # slot ids and partial forces; offsets, mass, BC slot; v, prev_a, u_dt
-        # read and written, x read and written, [u read and written], fi written
-        "f4_node": (slots_per_elem * (I4 + 3 * s), I4 + 8 + I4 + 3 * 2 * 3 * sn + 2 * 3 * 8 + u_rw + 3 * sn),
+        # read and written, x read and written, [u read and written] (fi is
+        # stored on a run's last step only)
+        "f4_node": (slots_per_elem * (I4 + 3 * s), I4 + 8 + I4 + 3 * 2 * 3 * sn + 2 * 3 * 8 + u_rw),
     }
 
 
@@ -165,13 +167,7 @@ def copy_bandwidth(torch, gib=2, reps=10):
     return max(r_torch, r_stream), r_torch, r_stream
 
 
-def fused_calls(d):
-    return {
-        "f1_volume": lambda: d.fusedPhase(1),
-        "f2_nodal": lambda: d.fusedPhase(2),
-        "f3_element": lambda: d.fusedPhase(3),
-        "f4_node": lambda: d.fusedPhase(4, True),
-    }
+PHASES = ("f1_volume", "f2_nodal", "f3_element", "f4_node")
 
 
 def api_calls(d):
@@ -202,15 +198,27 @@ def per_kernel_table(times, model, E, N):
     return out
 
 
-def cpu_baseline(seconds_target=12.0):
-    """oracle (reference-faithful C restatement, OpenMP) on a slab of the same
-    workload: 200 x 200 x 10 hexa at the same element size"""
+def cpu_model():
+    """the host CPU's model name (lscpu's "Model name", from /proc/cpuinfo)"""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(seconds_target=15.0, nx=200, ny=200, nz=100, L=(2.0, 2.0, 1.0)):
+    """oracle (reference-faithful C restatement, OpenMP on the host cores) on
+    the benchmark block itself: a bounded number of steps right after init,
+    in-loop timing"""
     from oracle import wfref
     from weldformfem_amd import workloads
 
-    nx, ny, nz = 200, 200, 10
     r = wfref.RefDomain()
-    r.AddBoxLength([0.0, 0.0, 0.0], [2.0, 2.0, 0.1], 0.005)
+    r.AddBoxLength([0.0, 0.0, 0.0], list(L), 0.5 * L[0] / nx)
     E = r.elem_count
     bottom, top = workloads.box_layers(nx, ny, nz)
     workloads.setup_compression(r, bottom, top, -2.0, sy0=workloads.lognormal_sy0(E, seed=2), hg_coeff=0.06)
@@ -226,9 +234,10 @@ def cpu_baseline(seconds_target=12.0):
         "value": E / per,
         "unit": "element-steps/s",
         "cores": wfref.num_threads(),
+        "cpu_model": cpu_model(),
         "kind": "port",
-        "sample": f"oracle (C restatement, OpenMP) on 200x200x10 hexa ({E} elements) of the same workload, "
-                  f"{n} steps after init, in-loop timing",
+        "sample": f"oracle (C restatement, OpenMP) on the {nx}x{ny}x{nz} hexa block ({E} elements) of the "
+                  f"workload, steps 3..{n + 2} after init (elastic), in-loop timing",
     }
 
 
@@ -609,10 +618,96 @@ def pmc_probe(args):
         if prec == WFX_FP64:
             d.run(1, reaction=False)
         torch.cuda.synchronize()
+        N, E, n = d.sizes()
+        model = domain_fused_bytes(d, 8 if prec == WFX_FP64 else 4)
+        print("PMC_MODEL " + json.dumps({"precision": "fp64" if prec == WFX_FP64 else "fp32_mixed",
+                                         "bytes": {k: be * E + bn * N for k, (be, bn) in model.items()}}),
+              flush=True)
         print(f"pmc probe: precision {prec} done, largest plastic strain {d.field('pl_strain').max():.3e}",
               flush=True)
         del d
     print("pmc probe done", flush=True)
+
+
+def pmc_summary(fetch_csv, write_csv, probe_log):
+    """Markdown table of HBM traffic per dispatch of the fused kernels from two
+    rocprofv3 --pmc runs of --pmc-probe (FETCH_SIZE, WRITE_SIZE; KiB per
+    dispatch): each counter scaled by what it reads for the probe's 1 GiB
+    device copy, the median over the last three dispatches of each kernel,
+    against the byte model the probe printed."""
+    import csv
+    import statistics
+
+    def load(path, counter):
+        by = {}
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row["Counter_Name"] == counter:
+                    by.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+        return by
+
+    fetch, write = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    gib_kib = float(1 << 20)
+    cf = gib_kib / max(v for v in fetch["__amd_rocclr_copyBuffer"])  # KiB read for a 1 GiB copy
+    cw = gib_kib / max(v for v in write["__amd_rocclr_copyBuffer"])
+    models = {}
+    with open(probe_log) as f:
+        for line in f:
+            if line.startswith("PMC_MODEL "):
+                m = json.loads(line[len("PMC_MODEL "):])
+                models[m["precision"]] = m["bytes"]
+    lines = [f"Calibration: the probe's 1 GiB device copy reads {gib_kib / cf:.0f} KiB of FETCH_SIZE and "
+             f"{gib_kib / cw:.0f} KiB of WRITE_SIZE: reads = {cf:.3f} x FETCH_SIZE, writes = {cw:.3f} x WRITE_SIZE.",
+             "Per-dispatch values: median of each kernel's last three dispatches (past first yield).", "",
+             "| kernel | reads MB | writes MB | HBM MB | model MB | HBM / model |", "|---|---:|---:|---:|---:|---:|"]
+    out = {}
+    kernels = {"f1_volume": "f1_volume<", "f2_nodal": "f2_nodal<", "f3_element": "f3_cluster<",
+               "f4_node": "f4_node<"}
+    for name in fetch:
+        short = next((k for k, tag in kernels.items() if tag in name), None)
+        if short is None or name not in write:
+            continue
+        prec = "fp64" if "<double" in name else "fp32_mixed"
+        rd = statistics.median(fetch[name][-3:]) * cf * 1024 / 1e6
+        wr = statistics.median(write[name][-3:]) * cw * 1024 / 1e6
+        mb = models.get(prec, {}).get(short)
+        ratio = (rd + wr) / (mb / 1e6) if mb else float("nan")
+        out.setdefault(prec, {})[short] = (rd + wr) * 1e6
+        label = name.replace("void wfx::(anonymous namespace)::", "").split(">(")[0] + ">"
+        lines.append(f"| `{label}` | {rd:.1f} | "
+                     f"{wr:.1f} | {rd + wr:.1f} | {mb / 1e6 if mb else float('nan'):.1f} | {ratio:.2f} |")
+    print("\n".join(lines))
+    print("\nPMC_TRAFFIC " + json.dumps(out))
+
+
+def stats_summary(stats_csv, trace_csv, rows=24, window=25):
+    """Markdown table of a rocprofv3 --kernel-trace --stats run: the `rows`
+    kernels with the most total time, and, per kernel, the mean duration of
+    its last `window` dispatches from the trace (for the fused passes: the
+    plastic window the benchmark times)."""
+    import csv
+
+    def label(name):
+        return name.replace("void wfx::(anonymous namespace)::", "").replace("wfx::(anonymous namespace)::", "") \
+            .split(">(")[0].split("(")[0] + (">" if ">(" in name else "")
+
+    durations = {}
+    with open(trace_csv) as f:
+        for row in csv.DictReader(f):
+            durations.setdefault(row["Kernel_Name"], []).append(
+                (int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) / 1e3)
+    lines = [f"| kernel | calls | avg (us) | min (us) | max (us) | total (ms) | % | last {window} avg (us) |",
+             "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    with open(stats_csv) as f:
+        for k, row in enumerate(csv.DictReader(f)):
+            if k >= rows:
+                break
+            d = durations.get(row["Name"], [])
+            tail = sum(d[-window:]) / len(d[-window:]) if d else float("nan")
+            lines.append(f"| `{label(row['Name'])}` | {row['Calls']} | {float(row['AverageNs']) / 1e3:.2f} | "
+                         f"{float(row['MinNs']) / 1e3:.2f} | {float(row['MaxNs']) / 1e3:.2f} | "
+                         f"{float(row['TotalDurationNs']) / 1e6:.2f} | {float(row['Percentage']):.2f} | {tail:.2f} |")
+    print("\n".join(lines))
 
 
 def main():
@@ -637,10 +732,18 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--pre-steps", type=int, default=1000,
                     help="fused steps before the headline's warmup and timed steps (the block yields near step 730)")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "PROBE_LOG"),
+                    help="print the traffic table of two --pmc-probe counter runs")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short run for counter collection: a 1 GiB device copy (calibration), "
                          "then 3 fused steps FP64 and FP32-mixed and one per-step API step")
+    ap.add_argument("--stats-summary", nargs=2, metavar=("KERNEL_STATS_CSV", "KERNEL_TRACE_CSV"),
+                    help="print the kernel table of a rocprofv3 --kernel-trace --stats run")
     args = ap.parse_args()
+    if args.pmc_summary:
+        return pmc_summary(*args.pmc_summary)
+    if args.stats_summary:
+        return stats_summary(*args.stats_summary)
     if args.pmc_probe:
         return pmc_probe(args)
     if args.gpus != 1:
@@ -679,6 +782,10 @@ def main():
     s64 = timed(lambda: d64.runFused(args.steps, reaction=False), torch) / args.steps
     steps64 += args.steps
     log(f"FP64 fused {s64 * 1e3:.3f} ms/step")
+    # per-pass durations in the run's own order (HIP events between the
+    # passes on the stream they run on), in the same plastic window
+    ph64 = dict(zip(PHASES, (u * 1e-6 for u in d64.timeFusedPhases(args.kernel_reps))))
+    steps64 += args.kernel_reps
 
     api = None
     if not args.no_api:
@@ -701,8 +808,9 @@ def main():
         log(f"FP32-mixed block sizes {'(given)' if args.bs32 else '(autotuned)'} {bs32}")
         d32.runFused(args.pre_steps + args.warmup, reaction=False)
         s32 = timed(lambda: d32.runFused(args.steps, reaction=False), torch) / args.steps
-        steps32 = args.pre_steps + args.warmup + args.steps
         log(f"FP32-mixed fused {s32 * 1e3:.3f} ms/step")
+        ph32 = dict(zip(PHASES, (u * 1e-6 for u in d32.timeFusedPhases(args.kernel_reps))))
+        steps32 = args.pre_steps + args.warmup + args.steps + args.kernel_reps
         # accuracy: both paths to err_steps, then one more step with the reaction
         target = max(args.err_steps, steps64, steps32)
         d64.runFused(target - steps64, reaction=False)
@@ -728,11 +836,11 @@ def main():
             "step_algorithmic_bytes_per_element": total(domain_fused_bytes(d32, 4), E, N) / E,
             "step_hbm_frac": total(domain_fused_bytes(d32, 4), E, N) / s32 / HBM_PEAK,
         }
-        kt32 = event_times(fused_calls(d32), args.kernel_reps, torch)
-        mixed["per_kernel"] = per_kernel_table(kt32, domain_fused_bytes(d32, 4), E, N)
+        mixed["per_kernel"] = per_kernel_table(ph32, domain_fused_bytes(d32, 4), E, N)
+        mixed["step_minus_passes_us"] = (s32 - sum(ph32.values())) * 1e6
         log(f"mixed error: {mixed['rel_err_vs_fp64']}")
 
-    kt64 = event_times(fused_calls(d64), args.kernel_reps, torch)
+    kt64 = ph64
     fb64 = domain_fused_bytes(d64, 8)
     dom = max(kt64, key=kt64.get)
     be, bn = fb64[dom]
@@ -782,7 +890,7 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get(f"{nx}x{ny}x{nz}", {}).get(dom)
+            traffic = json.load(f).get(f"{nx}x{ny}x{nz}", {}).get("fp64", {}).get(dom)
 
     out = {
         "metric": "element-steps/s at 1 GPU (FP64 vs FP32-mixed, stated err); % HBM peak",
@@ -811,6 +919,11 @@ def main():
         "step_algorithmic_bytes_per_element": step_bytes / E,
         "step_hbm_frac": step_bytes / s64 / HBM_PEAK,
         "per_kernel": per_kernel_table(kt64, fb64, E, N),
+        "per_kernel_timing": "HIP events between the four passes of fused runs (wfx_time_fused_phases), "
+                             "in the plastic window right after the timed steps",
+        # the step time minus the sum of the four event-timed passes (the
+        # events between passes add a little, so this can be slightly negative)
+        "step_minus_passes_us": (s64 - sum(kt64.values())) * 1e6,
         "fp32_mixed": mixed,
         "api_path_fp64": api,
         "other_configs": others,

@@ -277,6 +277,11 @@ int wfx_reaction_force(wfx_ctx *ctx, int dim, double *out, wfx_stream s);
  * around the measurement. */
 int wfx_autotune(wfx_ctx *ctx, int reps, int *chosen, wfx_stream s);
 int wfx_set_block_sizes(wfx_ctx *ctx, const int *block_sizes4);
+/* nsteps fused steps as wfx_step_fused runs them, with HIP events recorded on
+ * stream s between the four passes: us4[k] = average duration of pass k+1 in
+ * microseconds, in the order the passes run.  A measurement call with no
+ * reference counterpart; synchronous. */
+int wfx_time_fused_phases(wfx_ctx *ctx, int nsteps, double *us4, wfx_stream s);
 /* Capture nsteps fused steps (what wfx_step_fused(nsteps) enqueues) into a
  * HIP graph on stream s (a created stream, not NULL); wfx_graph_launch
  * replays it.  A launch fails with WFX_ERR_STATE when parameters, boundary

@@ -100,6 +100,7 @@ SIGNATURES = {
     "wfx_vtk_wait": (_i, [_vp]),
     "wfx_reaction_force": (_i, [_vp, _i, _pd, _vp]),
     "wfx_energies": (_i, [_vp, _pd, _pd, _vp]),
+    "wfx_time_fused_phases": (_i, [_vp, _i, _pd, _vp]),
     "wfx_energies_noalloc": (_i, [_vp, _pd, _pd, _vp]),
     "wfx_nonfinite_count": (_i, [_vp, ctypes.POINTER(ctypes.c_long)]),
     "wfx_inverted_count": (_i, [_vp, ctypes.POINTER(ctypes.c_long)]),

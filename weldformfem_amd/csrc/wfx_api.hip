@@ -1552,9 +1552,47 @@ static int enqueue_fused(wfx_ctx *c, int nsteps, double *dev_hist, hipStream_t s
   LAUNCH(BY_PRECISION(c, launch_vsum0, c, st));
   LAUNCH(launch_update_prediction(c, st));
   for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
-  LAUNCH(BY_PRECISION(c, launch_fused_steps, c, nsteps, dev_hist, st));
+  LAUNCH(BY_PRECISION(c, launch_fused_steps, c, nsteps, dev_hist, st, (hipEvent_t *)nullptr));
   LAUNCH(launch_copy(c->nd.prev_a, c->nd.a, 3L * c->N, st));  // a == prev_a after the step
   return WFX_OK;
+}
+
+// The fused step of wfx_step_fused with HIP events recorded on the stream
+// before each of its four passes and after the last step: the average
+// duration of each pass in microseconds over nsteps steps, measured in the
+// order the passes run (each sees the caches its predecessor leaves).
+int wfx_time_fused_phases(wfx_ctx *c, int nsteps, double *us4, wfx_stream s) {
+  REQUIRE_MESH(c);
+  REQUIRE(c->initialized, WFX_ERR_STATE, "call wfx_init first");
+  REQUIRE(nsteps > 0 && us4, WFX_ERR_ARG, "nsteps must be positive and us4 non-null");
+  int rc = ensure_dt_hist(c, nsteps);
+  if (rc) return rc;
+  hipStream_t st = as_stream(s);
+  std::vector<hipEvent_t> ev(4 * (size_t)nsteps + 1, nullptr);
+  hipError_t e = hipSuccess;
+  for (auto &x : ev)
+    if (e == hipSuccess) e = hipEventCreate(&x);
+  if (e == hipSuccess) e = BY_PRECISION(c, launch_vsum0, c, st);
+  if (e == hipSuccess) e = launch_update_prediction(c, st);
+  for (int d = 0; d < 3 && e == hipSuccess; ++d) e = launch_impose(c, d, false, st);
+  if (e == hipSuccess) e = BY_PRECISION(c, launch_fused_steps, c, nsteps, nullptr, st, ev.data());
+  if (e == hipSuccess) e = launch_copy(c->nd.prev_a, c->nd.a, 3L * c->N, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  double acc[4] = {0, 0, 0, 0};
+  for (int k = 0; k < nsteps && e == hipSuccess; ++k)
+    for (int ph = 0; ph < 4 && e == hipSuccess; ++ph) {
+      float ms = 0.f;
+      e = hipEventElapsedTime(&ms, ev[4 * k + ph], ev[4 * k + ph + 1]);
+      acc[ph] += ms;
+    }
+  for (auto &x : ev)
+    if (x) (void)hipEventDestroy(x);
+  WFX_CHECK_HIP(e);
+  for (int ph = 0; ph < 4; ++ph) us4[ph] = acc[ph] * 1e3 / nsteps;
+  c->contact.step_count += nsteps;
+  c->fused_stale = true;
+  rc = sync_cfl_dt(c, nsteps, st);
+  return rc ? rc : check_run_status(c, st);
 }
 
 int wfx_graph_capture(wfx_ctx *c, int nsteps, int with_reaction, wfx_stream s) {

@@ -10,7 +10,7 @@
 namespace wfx {
 template hipError_t launch_vsum0<double>(wfx_ctx *, hipStream_t);
 template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipStream_t, int);
-template hipError_t launch_fused_steps<double>(wfx_ctx *, int, double *, hipStream_t);
+template hipError_t launch_fused_steps<double>(wfx_ctx *, int, double *, hipStream_t, hipEvent_t *);
 template hipError_t launch_fused_pack<double>(wfx_ctx *, hipStream_t);
 template hipError_t launch_fused_unpack<double>(wfx_ctx *, hipStream_t);
 }  // namespace wfx

@@ -1310,7 +1310,7 @@ hipError_t launch_fused_unpack(wfx_ctx *c, hipStream_t s) {
 }
 
 template <typename T>
-hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t s) {
+hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t s, hipEvent_t *ev) {
   constexpr bool mixed = !std::is_same<T, double>::value;
   {
     hipError_t e = launch_fused_pack<T>(c, s);
@@ -1330,6 +1330,10 @@ hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t 
     c->fused_write_fi = (k + 1 == nsteps || hist) ? 1 : 0;
     c->fused_write_out = (k + 1 == nsteps) ? 1 : 0;
     for (int ph = 1; ph <= 4; ++ph) {
+      if (ev) {
+        hipError_t e = hipEventRecord(ev[4 * k + ph - 1], s);
+        if (e != hipSuccess) return e;
+      }
       hipError_t e = launch_fused_phase<T>(c, ph, k + 1 < nsteps, s, k);
       if (e != hipSuccess) {
         c->fused_write_fi = c->fused_write_out = 1;
@@ -1337,6 +1341,10 @@ hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t 
       }
     }
     c->fused_write_fi = c->fused_write_out = 1;
+    if (ev && k + 1 == nsteps) {
+      hipError_t e = hipEventRecord(ev[4 * nsteps], s);
+      if (e != hipSuccess) return e;
+    }
     if (hist) {
       hipError_t e = mixed ? launch_reaction_f32(c, c->fi32, hist + (long)k * c->ngroups, s)
                            : launch_reaction(c, 2, hist + (long)k * c->ngroups, s);

@@ -10,7 +10,7 @@ template hipError_t launch_fused_phase<float>(wfx_ctx *, int, int, hipStream_t, 
 template hipError_t launch_cfl_step<float>(wfx_ctx *, int, hipStream_t);
 template hipError_t launch_thermal<float>(wfx_ctx *, hipStream_t);
 template hipError_t launch_thermal_expansion<float>(wfx_ctx *, hipStream_t);
-template hipError_t launch_fused_steps<float>(wfx_ctx *, int, double *, hipStream_t);
+template hipError_t launch_fused_steps<float>(wfx_ctx *, int, double *, hipStream_t, hipEvent_t *);
 template hipError_t launch_fused_pack<float>(wfx_ctx *, hipStream_t);
 template hipError_t launch_fused_unpack<float>(wfx_ctx *, hipStream_t);
 }  // namespace wfx

@@ -38,7 +38,10 @@ template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, d
 WFX_DECLARE_SPECIALISATIONS(double)
 WFX_DECLARE_SPECIALISATIONS(float)
 
-template <typename T> hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t s);
+// ev (optional, 4 * nsteps + 1 events): recorded before each phase of each
+// step and after the last, for per-phase durations of a run
+template <typename T>
+hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t s, hipEvent_t *ev);
 template <typename T> hipError_t launch_vsum0(wfx_ctx *c, hipStream_t s);
 template <typename T> hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s, int step);
 template <typename T> hipError_t launch_cfl_step(wfx_ctx *c, int step, hipStream_t s);
@@ -50,8 +53,8 @@ extern template hipError_t launch_thermal<double>(wfx_ctx *, hipStream_t);
 extern template hipError_t launch_thermal<float>(wfx_ctx *, hipStream_t);
 extern template hipError_t launch_cfl_step<double>(wfx_ctx *, int, hipStream_t);
 extern template hipError_t launch_cfl_step<float>(wfx_ctx *, int, hipStream_t);
-extern template hipError_t launch_fused_steps<double>(wfx_ctx *, int, double *, hipStream_t);
-extern template hipError_t launch_fused_steps<float>(wfx_ctx *, int, double *, hipStream_t);
+extern template hipError_t launch_fused_steps<double>(wfx_ctx *, int, double *, hipStream_t, hipEvent_t *);
+extern template hipError_t launch_fused_steps<float>(wfx_ctx *, int, double *, hipStream_t, hipEvent_t *);
 extern template hipError_t launch_vsum0<double>(wfx_ctx *, hipStream_t);
 extern template hipError_t launch_vsum0<float>(wfx_ctx *, hipStream_t);
 extern template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipStream_t, int);

@@ -300,6 +300,13 @@ class Domain:
         """launch one kernel of the fused step (timing and profiling)"""
         check(self._L.wfx_fused_phase(self._h, int(phase), int(predict), self.stream))
 
+    def timeFusedPhases(self, nsteps):
+        """nsteps fused steps with HIP events between the four passes: the
+        average microseconds of each pass, in the order they run"""
+        us = (ctypes.c_double * 4)()
+        check(self._L.wfx_time_fused_phases(self._h, int(nsteps), us, self.stream))
+        return list(us)
+
     def autotune(self, reps=5):
         """pick the block size of each fused kernel by timing; returns the four sizes"""
         out = (ctypes.c_int * 4)()
