@@ -528,7 +528,12 @@ def adjoint_leg(torch, log, steps=2000, cells=30, with_oracle=True):
         d = pb.forward(t) - target
         return float(np.dot(d, d))
 
-    steps_rel = [1e-6 if abs(t * gi) >= 0.1 * L else 1e-4 for t, gi in zip(theta0, g)]
+    # relative difference steps: 1e-6 where the loss is sensitive; hg_coeff
+    # moves the loss by ~2.5e-4 of itself and its loss is piecewise smooth
+    # (elements cross the yield surface at slightly different steps), so its
+    # difference converges to the adjoint only as the step falls to ~3e-5,
+    # below which the loss's rounding takes over (hg_fd_sweep shows the trend)
+    steps_rel = [1e-6 if abs(t * gi) >= 0.1 * L else 3e-5 for t, gi in zip(theta0, g)]
 
     def cfd(loss):
         out = np.zeros(3)
@@ -542,6 +547,14 @@ def adjoint_leg(torch, log, steps=2000, cells=30, with_oracle=True):
 
     fd = cfd(gpu_loss)
     log(f"adjoint: grad {g}, GPU fd {fd}")
+    sweep = []
+    for hr in (3e-4, 1e-4, 3e-5):
+        h = hr * theta0[2]
+        tp, tm = theta0.copy(), theta0.copy()
+        tp[2] += h
+        tm[2] -= h
+        fdh = (gpu_loss(tp) - gpu_loss(tm)) / (2 * h)
+        sweep.append({"rel_step": hr, "fd": fdh, "rel_err": abs(g[2] - fdh) / abs(fdh)})
     out = {
         "workload": f"{cells}^3 hexa compression ({E} elements), {steps} steps, loss = sum (R - R*)^2 over the "
                     "top-face reaction history, theta = (sigma_y0, Et, hg_coeff), checkpoint every 50 steps",
@@ -549,6 +562,7 @@ def adjoint_leg(torch, log, steps=2000, cells=30, with_oracle=True):
         "backward_over_forward": t_bwd / t_fwd,
         "grad": g.tolist(), "fd_rel_steps": steps_rel,
         "fd_gpu": fd.tolist(), "rel_err_vs_fd_gpu": (np.abs(g - fd) / np.abs(fd)).tolist(),
+        "hg_fd_sweep_gpu": sweep,
     }
     if with_oracle:
         from oracle import wfref
