@@ -7,6 +7,9 @@
   exactly the node's CSR entries (setNodElem, Domain_d.C:1508-1611): each
   (element, local node) once, ascending original element within a slot,
   slots in ascending cluster order;
+* each slot names its node and each element's nodes name their slots;
+  no cluster has more slots than the element pass stages (a thin column,
+  whose 256-element runs touch 1028 nodes, is cut into smaller clusters);
 * a structured hexa block is cut into 16 x 4 x 4 bricks;
 * a specimen packed with another gets the same clusters and slots (shifted)
   as on its own.
@@ -46,8 +49,10 @@ def plan(planner, tmp_path, x, el, n, cs=256):
     r = subprocess.run([planner, str(len(x)), str(len(el)), str(n), str(cs), str(tmp_path / "el.i32"),
                         str(tmp_path / "x.f64"), pre], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
-    return {k: np.fromfile(pre + k, dtype=np.int32) for k in
-            ("s2o", "eoff", "sbase", "loff", "ent", "nsl_off", "nsl")}
+    out = {k: np.fromfile(pre + k, dtype=np.int32) for k in
+           ("s2o", "eoff", "sbase", "loff", "ent", "nsl_off", "nsl", "snode", "lslot")}
+    _, _, out["max_slots"], out["slot_cap"] = (int(q) for q in r.stdout.split())
+    return out
 
 
 def check_plan(p, el, N, n, cs=256):
@@ -77,6 +82,14 @@ def check_plan(p, el, N, n, cs=256):
             assert np.all(np.diff(t) >= 0)
             slot_node[sbase[c] + j] = nodes[0]
             entries[sbase[c] + j] = list(zip(els[t].tolist(), a.tolist()))
+    # slot -> node, element-node -> slot of its cluster, and the slot cap
+    np.testing.assert_array_equal(p["snode"], slot_node)
+    M = np.diff(sbase)
+    assert p["max_slots"] == M.max() <= p["slot_cap"]
+    cl_of = np.repeat(np.arange(C), sizes)
+    ls = p["lslot"].reshape(E, n)
+    assert np.all(ls < M[cl_of][:, None])
+    np.testing.assert_array_equal(p["snode"][sbase[cl_of][:, None] + ls], el[s2o])
     # the node's slots in cluster order reproduce its CSR list exactly
     ref = RefDomain()
     ref.CreateFromMesh(np.zeros((N, 3)), el.reshape(-1), n)
@@ -138,3 +151,13 @@ def test_packed_specimen_same_plan(planner, tmp_path):
     np.testing.assert_array_equal(both["sbase"][c1:] - s1, alone["sbase"])
     np.testing.assert_array_equal(both["loff"][s1 + c1:], alone["loff"])
     np.testing.assert_array_equal(both["ent"][len(e1) * 8:], alone["ent"])
+
+
+def test_slot_cap_splits_clusters(planner, tmp_path):
+    r = RefDomain()
+    r.AddBoxLength([0, 0, 0], [0.05, 0.05, 30.0], 0.025)  # a 1 x 1 x 600 column
+    x, el = r.field("x"), r.field("elnod").reshape(-1, 8)
+    p = plan(planner, tmp_path, x, el, 8)
+    check_plan(p, el, len(x) // 3, 8)
+    assert p["slot_cap"] == 512 and p["max_slots"] <= 512
+    assert np.diff(p["eoff"]).max() < 256  # 256 elements would touch 1028 nodes

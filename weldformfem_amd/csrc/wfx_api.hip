@@ -164,8 +164,14 @@ static int setup_device(wfx_ctx *c, const ClusterPlan &plan) {
   A(c->scalars, 64) A(c->status, 16)
   c->ms.ncl = plan.clusters();
   c->ms.nslot = plan.slots();
+  if (plan.max_slots > slot_cap((int)n)) {  // the element pass stages at most slot_cap slots
+    set_error("element cluster with " + std::to_string(plan.max_slots) + " nodes");
+    free_all(c);
+    return WFX_ERR_STATE;
+  }
   A(c->ms.cl_eoff, c->ms.ncl + 1) A(c->ms.cl_sbase, c->ms.ncl + 1) A(c->ms.cl_loff, c->ms.nslot + c->ms.ncl)
   A(c->ms.cl_ent, n * E) A(c->ms.nsl_off, N + 1) A(c->ms.nsl, c->ms.nslot)
+  A(c->ms.cl_snode, c->ms.nslot) A(c->ms.cl_lslot, n * E)
   c->cfl_cap_min = (int)((E + 63) / 64 + 1);  // blocks of the element passes at block size >= 64
   A(c->cfl_part, c->cfl_cap_min + (N + 63) / 64 + 1)
   if (e == hipSuccess) e = (c->precision == WFX_FP64) ? alloc_elem(c, c->e64) : alloc_elem(c, c->e32);
@@ -201,6 +207,9 @@ static int setup_device(wfx_ctx *c, const ClusterPlan &plan) {
   WFX_CHECK_HIP(hipMemcpy(c->ms.cl_ent, plan.ent.data(), plan.ent.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemcpy(c->ms.nsl_off, plan.nsl_off.data(), plan.nsl_off.size() * sizeof(int), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemcpy(c->ms.nsl, plan.nsl.data(), plan.nsl.size() * sizeof(int), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(c->ms.cl_snode, plan.snode.data(), plan.snode.size() * sizeof(int), hipMemcpyHostToDevice));
+  WFX_CHECK_HIP(hipMemcpy(c->ms.cl_lslot, plan.lslot.data(), plan.lslot.size() * sizeof(uint16_t),
+                          hipMemcpyHostToDevice));
   std::vector<int> pk(n * E);
   for (size_t k = 0; k < n * E; ++k) pk[k] = (nodel_s[k] << 3) | c->nodel_loc_h[k];
   WFX_CHECK_HIP(hipMemcpy(c->ms.nodel_pk, pk.data(), n * E * sizeof(int), hipMemcpyHostToDevice));

@@ -71,10 +71,16 @@ void slot_tables(int n, const int *elnod, ClusterPlan &p, int c0, int c1, bool w
     }
     uint16_t *lo = p.loff.data() + p.sbase[c] + c;
     uint16_t *en = p.ent.data() + (size_t)e0 * n;
+    int *sn = p.snode.data() + p.sbase[c];
     int j = 0;
     for (size_t k = 0; k < pairs.size(); ++k) {
-      if (k == 0 || (pairs[k] >> 16) != (pairs[k - 1] >> 16)) lo[j++] = (uint16_t)k;
-      en[k] = (uint16_t)(pairs[k] & 0xffffULL);
+      if (k == 0 || (pairs[k] >> 16) != (pairs[k - 1] >> 16)) {
+        sn[j] = (int)(pairs[k] >> 16);
+        lo[j++] = (uint16_t)k;
+      }
+      const int q = (int)(pairs[k] & 0xffffULL);
+      en[k] = (uint16_t)q;
+      p.lslot[(size_t)(e0 + (q >> 3)) * n + (q & 7)] = (uint16_t)(j - 1);
     }
     lo[j] = (uint16_t)pairs.size();
   }
@@ -191,14 +197,34 @@ int plan_clusters(int N, int E, int n, const int *elnod, const double *x, int cs
   }
   p.o2s.resize(E);
   for (int s = 0; s < E; ++s) p.o2s[p.s2o[s]] = s;
+  // slot counts; a cluster touching more than slot_cap(n) nodes is halved
+  // until none does (a single element has n <= slot_cap(n) nodes)
+  std::vector<int> m;
+  for (;;) {
+    const int C = p.clusters();
+    m.assign(C, 0);
+    parallel_ranges(C, [&](int c0, int c1) { slot_tables(n, elnod, p, c0, c1, false, m); });
+    std::vector<int> eoff(1, 0);
+    bool split = false;
+    for (int c = 0; c < C; ++c) {
+      if (m[c] > slot_cap(n)) {
+        eoff.push_back(p.eoff[c] + (p.eoff[c + 1] - p.eoff[c]) / 2);
+        split = true;
+      }
+      eoff.push_back(p.eoff[c + 1]);
+    }
+    if (!split) break;
+    p.eoff.swap(eoff);
+  }
   const int C = p.clusters();
-  std::vector<int> m(C, 0);
-  parallel_ranges(C, [&](int c0, int c1) { slot_tables(n, elnod, p, c0, c1, false, m); });
+  p.max_slots = C ? *std::max_element(m.begin(), m.end()) : 0;
   p.sbase.assign(C + 1, 0);
   for (int c = 0; c < C; ++c) p.sbase[c + 1] = p.sbase[c] + m[c];
   const long S = p.slots();
   p.loff.assign((size_t)S + C, 0);
   p.ent.assign((size_t)E * n, 0);
+  p.snode.assign((size_t)S, 0);
+  p.lslot.assign((size_t)E * n, 0);
   parallel_ranges(C, [&](int c0, int c1) { slot_tables(n, elnod, p, c0, c1, true, m); });
   // node -> slots, in cluster order
   p.nsl_off.assign((size_t)N + 1, 0);

@@ -27,9 +27,17 @@ struct ClusterPlan {
                                // grouped by slot; cluster c's entries at eoff[c] * n
   std::vector<int> nsl_off;    // [N+1] node -> its slots, ascending cluster
   std::vector<int> nsl;        // [S]
+  std::vector<int> snode;      // [S] slot -> its node (ascending node within a cluster)
+  std::vector<uint16_t> lslot; // [E][n] stored element's nodes as slot indices within its cluster
+  int max_slots = 0;           // largest slot count of a cluster (<= slot_cap(n))
   int clusters() const { return (int)eoff.size() - 1; }
   long slots() const { return sbase.empty() ? 0 : sbase.back(); }
 };
+
+// Most slots a cluster may have: the element pass stages each slot's nodal
+// data in the LDS its element-node forces use later, which holds this many
+// slots for every precision (clusters with more are split).
+constexpr int slot_cap(int n) { return n == 8 ? 512 : 256; }
 
 // Plan for elnod [E][n] (original order, nodes in [0, N)) with node
 // coordinates x [N][3]; cs <= 256 elements per cluster.  Returns 0.

@@ -37,6 +37,7 @@
 //   anp_ratio[N]   V_n / V0_n,   vsum0[N] = V0_n (constant after init)
 #pragma once
 
+#include "wfx_cluster.hpp"
 #include "wfx_contact.hpp"
 #include "wfx_kernels.hpp"
 
@@ -278,43 +279,62 @@ __device__ __forceinline__ T shape_dH(const T (&C)[3][3], int d, int a) {
   }
 }
 
-// Jacobian cofactors C and determinant of element nodes g from the positions
-// (calcElemJAndDerivatives, Domain_d.C:1701-2044); FP32 differences the FP64
-// positions against node 0 before narrowing.
-template <typename T, int NN>
-__device__ __forceinline__ void elem_jacobian(const int (&g)[NN], const double *__restrict__ x, int N,
-                                              T (&C)[3][3], T &detJ) {
-  T X[NN][3];
-  if constexpr (std::is_same<T, double>::value) {
-#pragma unroll
-    for (int a = 0; a < NN; ++a)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) X[a][c] = x[(long)c * N + g[a]];
-  } else {
-    double x0[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) { x0[c] = x[(long)c * N + g[0]]; X[0][c] = T(0); }
-#pragma unroll
-    for (int a = 1; a < NN; ++a)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) X[a][c] = T(x[(long)c * N + g[a]] - x0[c]);
-  }
-  T J[3][3];
-  if constexpr (NN == 8) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      J[0][c] = T(0.125) * (-X[0][c] + X[1][c] + X[2][c] - X[3][c] - X[4][c] + X[5][c] + X[6][c] - X[7][c]);
-      J[1][c] = T(0.125) * (-X[0][c] - X[1][c] + X[2][c] + X[3][c] - X[4][c] - X[5][c] + X[6][c] + X[7][c]);
-      J[2][c] = T(0.125) * (-X[0][c] - X[1][c] - X[2][c] - X[3][c] + X[4][c] + X[5][c] + X[6][c] + X[7][c]);
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      J[0][c] = X[1][c] - X[0][c];
-      J[1][c] = X[2][c] - X[0][c];
-      J[2][c] = X[3][c] - X[0][c];
-    }
-  }
+// Walsh modes of a hexahedron's nodal values y[a] (local order of
+// Domain_d.C:1313-1346): m[k] = sum_a y[a] prod_{i in k} s_i(a), the bits of
+// k naming the natural directions (1 xi, 2 eta, 4 zeta) and s_i(a) = +-1 the
+// sign of node a's natural coordinate (the table of Domain_d.C:1824-1826).
+// m[1], m[2], m[4] are the sums calcElemJAndDerivatives weighs with 0.125
+// (the rows of J for positions, of the velocity gradient in natural
+// coordinates for velocities); m[6], m[5], m[3], m[7] are the hourglass
+// amplitudes of Gamma rows 0..3 (Mechanical.f90:259-263: eta zeta, xi zeta,
+// xi eta, xi eta zeta).  Three butterfly stages over the nodes in binary
+// position order (local 0, 1, 3, 2, 4, 5, 7, 6); m[0] is not formed.
+template <typename T>
+__device__ __forceinline__ void hexa_modes(const T (&y)[8], T (&m)[8]) {
+  const T a0 = y[1] + y[0], b0 = y[1] - y[0], a1 = y[2] + y[3], b1 = y[2] - y[3];
+  const T a2 = y[5] + y[4], b2 = y[5] - y[4], a3 = y[6] + y[7], b3 = y[6] - y[7];
+  const T c0 = a1 + a0, d0 = a1 - a0, c1 = b1 + b0, d1 = b1 - b0;
+  const T c2 = a3 + a2, d2 = a3 - a2, c3 = b3 + b2, d3 = b3 - b2;
+  m[0] = T(0);
+  m[4] = c2 - c0;
+  m[1] = c3 + c1;
+  m[5] = c3 - c1;
+  m[2] = d2 + d0;
+  m[6] = d2 - d0;
+  m[3] = d3 + d1;
+  m[7] = d3 - d1;
+}
+// Only the linear modes m[1], m[2], m[4] (the Jacobian rows times 8)
+template <typename T>
+__device__ __forceinline__ void hexa_linear_modes(const T (&y)[8], T &m1, T &m2, T &m4) {
+  const T a0 = y[1] + y[0], b0 = y[1] - y[0], a1 = y[2] + y[3], b1 = y[2] - y[3];
+  const T a2 = y[5] + y[4], b2 = y[5] - y[4], a3 = y[6] + y[7], b3 = y[6] - y[7];
+  m1 = (b3 + b2) + (b1 + b0);
+  m2 = (a3 - a2) + (a1 - a0);
+  m4 = (a3 + a2) - (a1 + a0);
+}
+// The inverse: nodal values z[a] = sum_k q[k] prod_{i in k} s_i(a) for
+// mode coefficients q[1..7] (q[0] = 0), the same three stages in reverse.
+template <typename T>
+__device__ __forceinline__ void hexa_synthesis(const T (&q)[8], T (&z)[8]) {
+  const T u0 = -q[4], u4 = q[4], u1 = q[1] - q[5], u5 = q[1] + q[5];
+  const T u2 = q[2] - q[6], u6 = q[2] + q[6], u3 = q[3] - q[7], u7 = q[3] + q[7];
+  const T v0 = u0 - u2, v2 = u0 + u2, v1 = u1 - u3, v3 = u1 + u3;
+  const T v4 = u4 - u6, v6 = u4 + u6, v5 = u5 - u7, v7 = u5 + u7;
+  z[0] = v0 - v1;
+  z[1] = v0 + v1;
+  z[3] = v2 - v3;
+  z[2] = v2 + v3;
+  z[4] = v4 - v5;
+  z[5] = v4 + v5;
+  z[7] = v6 - v7;
+  z[6] = v6 + v7;
+}
+
+// Cofactors C (adj(J) = C^T, Matrix.h:694-727) and the Sarrus determinant
+// (Matrix.h:610-615) of J.
+template <typename T>
+__device__ __forceinline__ void cofactors(const T (&J)[3][3], T (&C)[3][3], T &detJ) {
   C[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
   C[0][1] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]);
   C[0][2] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
@@ -328,70 +348,178 @@ __device__ __forceinline__ void elem_jacobian(const int (&g)[NN], const double *
          J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] - J[0][2] * J[1][1] * J[2][0];
 }
 
-// F3: the element chain of one step for element e.  The nodal forces of the
-// element (f - f_hg, what the assembly adds to each node) go to sink(a, fx, fy, fz).
-// Hexahedral element passes of the rate laws in FP64 keep the hourglass
-// amplitudes and the Jacobian cofactors in LDS across the stress update: the
-// laws' transcendental functions otherwise push the pass past 256 VGPRs
-// (occupancy 1).  Tetrahedra stay at occupancy 2 without it.
+// pow(vol, 0.6666666) of the hourglass coefficient (Mechanical.f90:299): the
+// FP64 path in double, FP32-mixed with the hardware base-2 logarithm and
+// exponential (the same NaN, zero and infinity cases)
+template <typename T>
+__device__ __forceinline__ T vol_pow_hg(T vol) {
+  if constexpr (std::is_same<T, double>::value) return pow(vol, 0.6666666);
+  else return exp2f(0.6666666f * log2f(vol));
+}
+
+// The cluster's slots staged in LDS by the element pass: positions (double),
+// predicted velocities (T), ANP ratio as F2 stored it (double, or the float
+// deviation from 1) and node ids, [field][slot_cap].
+template <typename T, int NN>
+struct SlotLds {
+  static constexpr int cap = slot_cap(NN);
+  static constexpr int bytes = cap * (3 * 8 + 3 * (int)sizeof(T) + (int)sizeof(T) + 4);
+  double *x;
+  T *v, *r;
+  int *node;
+  __device__ explicit SlotLds(unsigned char *base)
+      : x(reinterpret_cast<double *>(base)),
+        v(reinterpret_cast<T *>(base + 3 * 8 * cap)),
+        r(reinterpret_cast<T *>(base + (3 * 8 + 3 * (int)sizeof(T)) * cap)),
+        node(reinterpret_cast<int *>(base + (3 * 8 + 4 * (int)sizeof(T)) * cap)) {}
+};
+
+// What the first part of the element chain leaves for the second: geometry,
+// strain rate and spin, hourglass amplitudes, the nodal ANP average and,
+// with thermal coupling, the element's temperature terms.
+template <typename T, int NN, bool TH>
+struct ElemGeo {
+  T C[3][3];
+  T detJ, volv;
+  T s00, s11, s22, s01, s12, s02, w01, w12, w02;
+  T hm[3][4];
+  double J_avg, div_v;
+  bool is_contact;
+  int g[TH ? NN : 1];
+  double tsum, gT0, gT1, gT2;
+};
+
+// F3, first part: everything that reads the staged nodal data of element
+// `ls` (its nodes as slot indices): Jacobian, cofactors and determinant
+// (calcElemJAndDerivatives), the velocity gradient and from it the strain
+// rate and spin (calcElemStrainRates), the hourglass amplitudes, the ANP
+// average of the nodal ratios (calcElemPressure) and the contact test.
+// sum_a dH(d,a) y_a = sum_i C[i][d] (0.125 m_i(y)) for a hexahedron
+// (dH(d,a) = 0.125 sum_i s_i(a) C[i][d], Domain_d.C:1843-1850), and
+// sum_i C[i][d] (y_{i+1} - y_0) for a tetrahedron (:1866-1887).
+template <typename T, int NN, bool TH>
+__device__ __forceinline__ void element_geometry(const FusedParams &fp, const SlotLds<T, NN> &L,
+                                                 const int (&ls)[NN], ElemGeo<T, NN, TH> &G) {
+  T J[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double *xc = L.x + c * L.cap;
+    if constexpr (NN == 8) {
+      double y[8], m1, m2, m4;
+#pragma unroll
+      for (int a = 0; a < 8; ++a) y[a] = xc[ls[a]];
+      hexa_linear_modes(y, m1, m2, m4);
+      J[0][c] = T(0.125 * m1);
+      J[1][c] = T(0.125 * m2);
+      J[2][c] = T(0.125 * m4);
+    } else {
+      const double x0 = xc[ls[0]];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) J[i][c] = T(xc[ls[i + 1]] - x0);
+    }
+  }
+  cofactors(J, G.C, G.detJ);
+  const T w = (NN == 4) ? T(1.0 / 6.0) : T(8.0);
+  G.volv = G.detJ * w;
+  // velocity modes: Mv[i][c] = 0.125 m_i(v_c) / the tetrahedron's differences
+  T Mv[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T *vc = L.v + c * L.cap;
+    if constexpr (NN == 8) {
+      T y[8], m[8];
+#pragma unroll
+      for (int a = 0; a < 8; ++a) y[a] = vc[ls[a]];
+      hexa_modes(y, m);
+      Mv[0][c] = m[1];
+      Mv[1][c] = m[2];
+      Mv[2][c] = m[4];
+      G.hm[c][0] = m[6];
+      G.hm[c][1] = m[5];
+      G.hm[c][2] = m[3];
+      G.hm[c][3] = m[7];
+    } else {
+      const T v0 = vc[ls[0]];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) Mv[i][c] = vc[ls[i + 1]] - v0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) G.hm[c][j] = T(0);
+    }
+  }
+  // Gv(d,c) = sum_a dH(d,a) v_a,c up to the hexahedron's 0.125; the
+  // reference's D and W (Mechanical.C:58-116): f = 1/detJ, off-diagonals
+  // halved
+  T Gv[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Gv[d][c] = G.C[0][d] * Mv[0][c] + G.C[1][d] * Mv[1][c] + G.C[2][d] * Mv[2][c];
+  const T f = T(1) / G.detJ;
+  const T q = (NN == 8) ? T(0.125) * f : f, qh = T(0.5) * q;
+  G.s00 = q * Gv[0][0];
+  G.s11 = q * Gv[1][1];
+  G.s22 = q * Gv[2][2];
+  G.s01 = qh * (Gv[1][0] + Gv[0][1]);
+  G.w01 = qh * (Gv[1][0] - Gv[0][1]);
+  G.s12 = qh * (Gv[2][1] + Gv[1][2]);
+  G.w12 = qh * (Gv[2][1] - Gv[1][2]);
+  G.s02 = qh * (Gv[2][0] + Gv[0][2]);
+  G.w02 = qh * (Gv[2][0] - Gv[0][2]);
+  // div v times detJ for the pressure's stabilisation terms
+  G.div_v = (NN == 8 ? 0.125 : 1.0) * ((double)Gv[0][0] + (double)Gv[1][1] + (double)Gv[2][2]);
+  double J_avg = 0.0;
+#pragma unroll
+  for (int a = 0; a < NN; ++a) {
+    if constexpr (std::is_same<T, double>::value) J_avg += L.r[ls[a]];
+    else J_avg += 1.0 + (double)L.r[ls[a]];
+  }
+  G.J_avg = J_avg / NN;
+  G.is_contact = elem_in_contact<NN>(fp.pp, [&](int a) { return L.node[ls[a]]; });
+  // thermal coupling (ThermalCalcs, Thermal.C:27-115): the conduction matrix
+  // applied through the element's temperature gradient,
+  // (Kt T)_i = sum_d dH(d,i) G_d with G_d = k_T w / detJ sum_j dH(d,j) T_j;
+  // the nodal temperatures also give the rate laws' temperature (their mean)
+  G.tsum = G.gT0 = G.gT1 = G.gT2 = 0.0;
+  if constexpr (TH) {
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      G.g[a] = L.node[ls[a]];
+      const double ta = fp.mp.Tn[G.g[a]];
+      G.tsum += ta;
+      G.gT0 += (double)shape_dH<T, NN>(G.C, 0, a) * ta;
+      G.gT1 += (double)shape_dH<T, NN>(G.C, 1, a) * ta;
+      G.gT2 += (double)shape_dH<T, NN>(G.C, 2, a) * ta;
+    }
+    const double kf = fp.k_T / (double)G.detJ * (NN == 4 ? 1.0 / 6.0 : 8.0);
+    G.gT0 *= kf;
+    G.gT1 *= kf;
+    G.gT2 *= kf;
+  }
+}
+
+// F3, second part: thermal expansion, hourglass coefficient, ANP pressure,
+// Jaumann update and J2 return (CalcStressStrain), artificial viscosity,
+// the element's internal and hourglass forces (calcElemForces,
+// calcElemHourglassForces: f - f_hg, what the assembly adds to each node),
+// handed to sink(a, node, fx, fy, fz, thermal term).  Hexahedral element
+// passes of the rate laws in FP64 keep the hourglass amplitudes and the
+// cofactors in LDS across the stress update (stash: this thread's column,
+// stride BS): the laws' transcendental functions otherwise push the pass
+// past 256 VGPRs.
 template <typename T, int NN, int LAW>
 constexpr bool kStashLDS = NN == 8 && LAW == LAW_RATE && std::is_same<T, double>::value;
 // hm[3][4], c_h, C[3][3] and with thermal coupling the gradient term G[3]
 template <bool TH>
 constexpr int kStashVals = TH ? 25 : 22;
 
-
-// TH: thermal coupling (nodal temperatures, plastic heat and the element's
-// thermal terms); a compile-time switch so that runs without it keep their
-// register budget.
-// stash: this thread's column of the work-group's LDS (stride BS), used when
-// kStashLDS
 template <typename T, int NN, int LAW, int BS, bool TH, class Sink>
-__device__ __forceinline__ void element_chain(
-    int e, int E, int N, const FusedParams &fp, const int *__restrict__ elnod, const double *__restrict__ x,
-    const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
-    const T *__restrict__ rho, const T *__restrict__ sy0, const T *__restrict__ He,
-    T *__restrict__ tau_io, T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
-    T *__restrict__ sigma_y, T *__restrict__ q_plheat, double *stash, Sink &&sink) {
+__device__ __forceinline__ void element_material(
+    int e, int E, const FusedParams &fp, ElemGeo<T, NN, TH> &G, const T *__restrict__ vol0,
+    const T *__restrict__ rho, const T *__restrict__ sy0, const T *__restrict__ He, T *__restrict__ tau_io,
+    T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain, T *__restrict__ sigma_y,
+    T *__restrict__ q_plheat, double *stash, Sink &&sink) {
   const long Es = E;
-  int g[NN];
-#pragma unroll
-  for (int a = 0; a < NN; ++a) g[a] = ld_stream(&elnod[(long)a * Es + e]);
-
-  // ---- Jacobian, determinant, dH (calcElemJAndDerivatives) --------------
-  T C[3][3];
-  T detJ;
-  elem_jacobian<T, NN>(g, x, N, C, detJ);
-  // dH(d,a) is re-evaluated from the 3x3 cofactor matrix where it is used
-  // (same expression each time), so 9 values stay live instead of 3n.
-  const T w = (NN == 4) ? T(1.0 / 6.0) : T(8.0);
-  T volv = T(0);
-  volv += detJ * w * T(1);
-
-  // ---- strain rate and spin (calcElemStrainRates) -----------------------
-  T V[NN][3];
-#pragma unroll
-  for (int a = 0; a < NN; ++a)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) V[a][c] = (T)v[(long)c * N + g[a]];
-  T s00 = 0, s11 = 0, s22 = 0, s01 = 0, s12 = 0, s02 = 0, w01 = 0, w12 = 0, w02 = 0;
-  T f = T(1) / detJ;
-#pragma unroll
-  for (int a = 0; a < NN; ++a) {
-    T vx = V[a][0], vy = V[a][1], vz = V[a][2];
-    T h0 = shape_dH<T, NN>(C, 0, a), h1 = shape_dH<T, NN>(C, 1, a), h2 = shape_dH<T, NN>(C, 2, a);
-    s00 = s00 + h0 * f * vx;
-    s11 = s11 + h1 * f * vy;
-    s22 = s22 + h2 * f * vz;
-    s01 = s01 + f * (h1 * vx + h0 * vy);
-    w01 = w01 + f * (h1 * vx - h0 * vy);
-    s12 = s12 + f * (h2 * vy + h1 * vz);
-    s02 = s02 + f * (h2 * vx + h0 * vz);
-    w12 = w12 + f * (h2 * vy - h1 * vz);
-    w02 = w02 + f * (h2 * vx - h0 * vz);
-  }
-  s01 = s01 * T(0.5); s12 = s12 * T(0.5); s02 = s02 * T(0.5);
-  w01 = w01 * T(0.5); w12 = w12 * T(0.5); w02 = w02 * T(0.5);
+  T s00 = G.s00, s11 = G.s11, s22 = G.s22, s01 = G.s01, s12 = G.s12, s02 = G.s02;
   if constexpr (TH) {  // calcThermalExpansion (Thermal.C:164-178)
     if (fp.texp) {
       const double sh = fp.exp_T * (double)static_cast<const T *>(fp.texp)[e];
@@ -403,83 +531,40 @@ __device__ __forceinline__ void element_chain(
       s02 = (T)((double)s02 - sh * 0.0);
     }
   }
-
-  // hourglass mode amplitudes now, so the nodal velocities die here and do
-  // not stay live through the stress update (register pressure)
-  T hm[3][4];
+  const T volv = G.volv;
   T c_h = T(0);
   if constexpr (NN == 8) {
     if (fp.hg_coeff != 0.0) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) hm[c][j] = T(0);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int a = 0; a < 8; ++a)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) hm[c][j] = hm[c][j] + V[a][c] * (T)kHgGamma[j][a];
       const double rho_e = fp.rho_u > 0.0 ? fp.rho_u : (double)rho[e];
-      c_h = (T)(fp.hg_coeff * pow((double)volv, 0.6666666) * rho_e * 0.25 * fp.cs0);
+      c_h = (T)(fp.hg_coeff * rho_e * 0.25 * fp.cs0) * vol_pow_hg<T>(volv);
     }
-  }
-  // ---- thermal coupling (ThermalCalcs, Thermal.C:27-115) -----------------
-  // (Kt T)_i = sum_d dH(d,i) G_d with G_d = k_T w / detJ sum_j dH(d,j) T_j:
-  // the conduction matrix applied through the element's temperature gradient.
-  // The nodal temperatures also give the rate laws' temperature (their mean).
-  double gT0 = 0.0, gT1 = 0.0, gT2 = 0.0, tsum = 0.0;
-  if constexpr (TH) {
-#pragma unroll
-    for (int a = 0; a < NN; ++a) {
-      const double ta = fp.mp.Tn[g[a]];
-      tsum += ta;
-      gT0 += (double)shape_dH<T, NN>(C, 0, a) * ta;
-      gT1 += (double)shape_dH<T, NN>(C, 1, a) * ta;
-      gT2 += (double)shape_dH<T, NN>(C, 2, a) * ta;
-    }
-    const double kf = fp.k_T / (double)detJ * (NN == 4 ? 1.0 / 6.0 : 8.0);
-    gT0 *= kf;
-    gT1 *= kf;
-    gT2 *= kf;
   }
   if constexpr (kStashLDS<T, NN, LAW>) {
     double *st = stash;
     if constexpr (TH) {
-      st[22 * BS] = gT0;
-      st[23 * BS] = gT1;
-      st[24 * BS] = gT2;
+      st[22 * BS] = G.gT0;
+      st[23 * BS] = G.gT1;
+      st[24 * BS] = G.gT2;
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) st[(4 * c + j) * BS] = fp.hg_coeff != 0.0 ? hm[c][j] : 0.0;
+      for (int j = 0; j < 4; ++j) st[(4 * c + j) * BS] = G.hm[c][j];
     st[12 * BS] = c_h;
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int q = 0; q < 3; ++q) st[(13 + 3 * r + q) * BS] = C[r][q];
+      for (int q = 0; q < 3; ++q) st[(13 + 3 * r + q) * BS] = G.C[r][q];
   }
   // ---- ANP pressure (calcElemPressure element part) ----------------------
   const PressParams &pp = fp.pp;
-  const bool is_contact = elem_in_contact<NN>(pp, [&](int a) { return g[a]; });
-  double J_avg = 0.0;
-#pragma unroll
-  for (int a = 0; a < NN; ++a) J_avg += anp_ratio_at<T>(ratio, g[a]);
-  J_avg /= NN;
-  const bool general = press_general(pp, (double)volv, J_avg);
+  const bool general = press_general(pp, (double)volv, G.J_avg);
   double J_local = 0.0;
   if (fp.need_jlocal || general) J_local = (double)volv / (double)vol0[e];
+  const double div_v = G.div_v;
   const T pe = (T)anp_elem_pressure(
-      pp, is_contact, general, J_local, J_avg, (double)volv, [&] { return (double)rho[e]; },
-      [&] {
-        double div_v = 0.0;
-#pragma unroll
-        for (int a = 0; a < NN; ++a)
-          div_v += (double)shape_dH<T, NN>(C, 0, a) * v[g[a]] + (double)shape_dH<T, NN>(C, 1, a) * v[(long)N + g[a]] +
-                   (double)shape_dH<T, NN>(C, 2, a) * v[2L * N + g[a]];
-        return div_v;
-      });
+      pp, G.is_contact, general, J_local, G.J_avg, (double)volv, [&] { return (double)rho[e]; },
+      [&] { return div_v; });
   if (fp.write_out) p_out[e] = pe;
 
   // ---- Jaumann update and J2 return (CalcStressStrain) -------------------
@@ -489,15 +574,15 @@ __device__ __forceinline__ void element_chain(
                           ld_stream(&tau_io[2 * Es + e]), ld_stream(&tau_io[3 * Es + e]),
                           ld_stream(&tau_io[4 * Es + e]), ld_stream(&tau_io[5 * Es + e]));
   T3<T> D = t3_from_sym(s00, s11, s22, s01, s12, s02);
-  T3<T> W = t3_from_antisym(w01, w12, w02);
+  T3<T> W = t3_from_antisym(G.w01, G.w12, G.w02);
   T3<T> I = t3_identity<T>();
-  T G = (T)mp.G;
+  T Gm = (T)mp.G;
   T3<T> Wt = t3_trans(W);
   T3<T> SRT = t3_mul(tau, Wt);
   T3<T> RS = t3_mul(W, tau);
   T3<T> cI = t3_scale(I, T(1.0 / 3.0) * t3_trace(D));
   T3<T> devD = t3_sub(D, cI);
-  T3<T> A = t3_scale(devD, T(2) * G);
+  T3<T> A = t3_scale(devD, T(2) * Gm);
   T3<T> B = t3_add(A, SRT);
   T3<T> Cc = t3_add(B, RS);
   tau = t3_add(tau, t3_scale(Cc, dt));
@@ -515,7 +600,7 @@ __device__ __forceinline__ void element_chain(
   RateTemp rt = mp.ut;  // the rate laws' temperature factors (law_temperature on the loaded nodes)
   if constexpr (LAW == LAW_RATE) {
     if (!mp.t_uniform) {
-      const double temp = TH ? tsum / NN : mp.Te[e];
+      const double temp = TH ? G.tsum / NN : mp.Te[e];
       rt = rate_temp_at(mp, temp);
     }
   }
@@ -530,8 +615,8 @@ __device__ __forceinline__ void element_chain(
   T dep = T(0);
   if (sy < sig_trial) {
     T H = per_elem ? Hb : tangent_modulus<T, LAW>(mp, rt, ep, cap_rate(mp, edot), rc);
-    T dgamma = (sig_trial - sy) / (T(3) * G + H);
-    T factor = T(1) - (T(3) * G * dgamma) / sig_trial;
+    T dgamma = (sig_trial - sy) / (T(3) * Gm + H);
+    T factor = T(1) - (T(3) * Gm * dgamma) / sig_trial;
     tau = t3_scale(s, factor);
     pl_strain[e] = ep + dgamma;
     dep = dgamma;
@@ -577,6 +662,8 @@ __device__ __forceinline__ void element_chain(
   st_stream(&tau_io[2 * Es + e], tau.zz); st_stream(&tau_io[3 * Es + e], tau.xy);
   st_stream(&tau_io[4 * Es + e], tau.yz); st_stream(&tau_io[5 * Es + e], tau.xz);
 
+  T C[3][3], hm[3][4];
+  double gT0 = G.gT0, gT1 = G.gT1, gT2 = G.gT2;
   if constexpr (kStashLDS<T, NN, LAW>) {  // back from LDS (volatile: real loads)
     volatile double *sv = stash;
 #pragma unroll
@@ -593,41 +680,69 @@ __device__ __forceinline__ void element_chain(
       gT1 = sv[23 * BS];
       gT2 = sv[24 * BS];
     }
+  } else {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) C[r][q] = G.C[r][q];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) hm[c][j] = G.hm[c][j];
   }
-  // ---- internal forces (calcElemForces) and hourglass ---------------------
-  T sg0 = sig.xx, sg1 = sig.yy, sg2 = sig.zz, sg3 = sig.xy, sg4 = sig.yz, sg5 = sig.xz;
+  // ---- internal forces (calcElemForces, Mechanical.C:448-457) and hourglass
+  // f_c(a) = w sum_d dH(d,a) sigma(d,c): the cofactors times the stress give
+  // the coefficients of the linear modes, Q(i,c) = 0.125 w sum_d C[i][d]
+  // sigma(d,c) (0.125 w = 1 for the hexahedron); the hourglass force
+  // (fi -= f_hg, Matrices.f90:641) adds c_h times the amplitudes to the
+  // hourglass modes; one synthesis per component gives the nodal forces.
+  const T sg0 = sig.xx, sg1 = sig.yy, sg2 = sig.zz, sg3 = sig.xy, sg4 = sig.yz, sg5 = sig.xz;
+  T Q[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Q[i][0] = C[i][0] * sg0 + C[i][1] * sg3 + C[i][2] * sg5;
+    Q[i][1] = C[i][0] * sg3 + C[i][1] * sg1 + C[i][2] * sg4;
+    Q[i][2] = C[i][0] * sg5 + C[i][1] * sg4 + C[i][2] * sg2;
+  }
+  T fn[3][NN];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if constexpr (NN == 8) {
+      T qm[8], z[8];
+      qm[0] = T(0);
+      qm[1] = Q[0][c];
+      qm[2] = Q[1][c];
+      qm[4] = Q[2][c];
+      if (fp.hg_coeff != 0.0) {
+        qm[6] = c_h * hm[c][0];
+        qm[5] = c_h * hm[c][1];
+        qm[3] = c_h * hm[c][2];
+        qm[7] = c_h * hm[c][3];
+      } else {
+        qm[3] = qm[5] = qm[6] = qm[7] = T(0);
+      }
+      hexa_synthesis(qm, z);
+#pragma unroll
+      for (int a = 0; a < 8; ++a) fn[c][a] = z[a];
+    } else {
+      const T w = T(1.0 / 6.0);
+      fn[c][1] = w * Q[0][c];
+      fn[c][2] = w * Q[1][c];
+      fn[c][3] = w * Q[2][c];
+      fn[c][0] = -(fn[c][1] + fn[c][2] + fn[c][3]);
+    }
+  }
 #pragma unroll
   for (int a = 0; a < NN; ++a) {
-    T h0 = shape_dH<T, NN>(C, 0, a), h1 = shape_dH<T, NN>(C, 1, a), h2 = shape_dH<T, NN>(C, 2, a);
-    T f0 = T(0), f1 = T(0), f2 = T(0);
-    f0 += h0 * sg0 * T(1);
-    f1 += h1 * sg1 * T(1);
-    f2 += h2 * sg2 * T(1);
-    f0 += h1 * sg3 + h2 * sg5;
-    f1 += h0 * sg3 + h2 * sg4;
-    f2 += h1 * sg4 + h0 * sg5;
-    f0 = f0 * w;
-    f1 = f1 * w;
-    f2 = f2 * w;
-    if constexpr (NN == 8) {
-      if (fp.hg_coeff != 0.0) {
-        T q[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          T hf = T(0);
-#pragma unroll
-          for (int j = 0; j < 4; ++j) hf = hf - hm[c][j] * (T)kHgGamma[j][a];
-          q[c] = hf * c_h;
-        }
-        f0 = f0 - q[0];
-        f1 = f1 - q[1];
-        f2 = f2 - q[2];
-      }
-    }
     // the node's thermal term times its mass: -(Kt T)_a + heat / cp
     T th = T(0);
-    if constexpr (TH) th = (T)(heat - ((double)h0 * gT0 + (double)h1 * gT1 + (double)h2 * gT2));
-    sink(a, g[a], f0, f1, f2, th);
+    int node = 0;
+    if constexpr (TH) {
+      const T h0 = shape_dH<T, NN>(C, 0, a), h1 = shape_dH<T, NN>(C, 1, a), h2 = shape_dH<T, NN>(C, 2, a);
+      th = (T)(heat - ((double)h0 * gT0 + (double)h1 * gT1 + (double)h2 * gT2));
+      node = G.g[a];
+    }
+    sink(a, node, fn[0][a], fn[1][a], fn[2][a], th);
   }
 }
 
@@ -637,6 +752,8 @@ struct ClusterArgs {
   const int *sbase;      // [C+1] first slot of each cluster
   const uint16_t *loff;  // cluster c: its M+1 entry offsets at sbase[c] + c
   const uint16_t *ent;   // element-node entries t * 8 + a grouped by slot, cluster c at eoff[c] * n
+  const int *snode;      // [S] slot -> node
+  const uint16_t *lslot; // [E][n] element-node -> slot within its cluster
   long S;                // slots: the row length of the partial sums
 };
 
@@ -645,34 +762,54 @@ struct ClusterArgs {
 template <bool TH>
 constexpr int kNodeVals = TH ? 4 : 3;
 
-// LDS of one work-group of the element pass: the element-node values
-// [kNodeVals * n][kCluster], and in the same space (each thread only ever
-// touches its own column, and reloads its stash before it writes its
-// values) the per-thread stash of the FP64 hexahedral rate-law passes
+// LDS of one work-group of the element pass, in turn: the cluster's staged
+// slots (SlotLds), then the element-node values [kNodeVals * n][kCluster]
+// and, in the same space (each thread only ever touches its own column, and
+// reloads its stash before it writes its values), the per-thread stash of
+// the FP64 hexahedral rate-law passes
 template <typename T, int NN, int LAW, bool TH>
-constexpr int kF3Lds = (kStashLDS<T, NN, LAW> && kStashVals<TH> * 8 > kNodeVals<TH> * NN * (int)sizeof(T))
-                           ? kStashVals<TH> * 8 * kCluster
-                           : kNodeVals<TH> * NN * (int)sizeof(T) * kCluster;
+constexpr int kF3Vals = (kStashLDS<T, NN, LAW> && kStashVals<TH> * 8 > kNodeVals<TH> * NN * (int)sizeof(T))
+                            ? kStashVals<TH> * 8 * kCluster
+                            : kNodeVals<TH> * NN * (int)sizeof(T) * kCluster;
+template <typename T, int NN, int LAW, bool TH>
+constexpr int kF3Lds = kF3Vals<T, NN, LAW, TH> > SlotLds<T, NN>::bytes ? kF3Vals<T, NN, LAW, TH>
+                                                                        : SlotLds<T, NN>::bytes;
 
-// F3: one work-group per element cluster.  Each thread runs the element
-// chain of one element and leaves its element-node values in LDS; then the
-// work-group adds them per cluster node (the slot's entries, ascending
-// element, in double) and writes one partial sum per slot (assemblyForces,
-// Matrices.C:42-87, for the cluster's share of each node).
+// One element's nodes as slot indices of its cluster (a 16- or 8-byte load)
+template <int NN>
+__device__ __forceinline__ void load_lslot(const uint16_t *p, int (&ls)[NN]) {
+  typedef unsigned int u32v __attribute__((ext_vector_type(NN / 2)));
+  const u32v w = __builtin_nontemporal_load(reinterpret_cast<const u32v *>(p));
+#pragma unroll
+  for (int k = 0; k < NN / 2; ++k) {
+    ls[2 * k] = (int)(w[k] & 0xffffu);
+    ls[2 * k + 1] = (int)(w[k] >> 16);
+  }
+}
+
+// F3: one work-group per element cluster.  The work-group first stages its
+// cluster's slots (the distinct nodes of its elements: positions,
+// velocities, ANP ratio, ids) in LDS with one load per slot and field; each
+// thread then runs the element chain of one element from LDS and leaves its
+// element-node values in the same LDS; then the work-group adds them per
+// cluster node (the slot's entries, ascending element, in double) and writes
+// one partial sum per slot (assemblyForces, Matrices.C:42-87, for the
+// cluster's share of each node).
 template <typename T, int NN, int LAW, bool TH>
 __global__ __launch_bounds__(kCluster) void f3_cluster(
-    int E, int N, FusedParams fp, ClusterArgs ca, const int *__restrict__ elnod, const double *__restrict__ x,
-    const T *__restrict__ v, const double *__restrict__ ratio, const T *__restrict__ vol0,
-    const T *__restrict__ rho, const T *__restrict__ sy0, const T *__restrict__ He,
-    T *__restrict__ tau_io, T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain,
-    T *__restrict__ sigma_y, T *__restrict__ part, T *__restrict__ q_plheat) {
+    int E, int N, FusedParams fp, ClusterArgs ca, const double *__restrict__ x, const T *__restrict__ v,
+    const double *__restrict__ ratio, const T *__restrict__ vol0, const T *__restrict__ rho,
+    const T *__restrict__ sy0, const T *__restrict__ He, T *__restrict__ tau_io, T *__restrict__ eps_io,
+    T *__restrict__ p_out, T *__restrict__ pl_strain, T *__restrict__ sigma_y, T *__restrict__ part,
+    T *__restrict__ q_plheat) {
   constexpr int R = kNodeVals<TH>;
   __shared__ __align__(16) unsigned char lds_raw[kF3Lds<T, NN, LAW, TH>];
   __shared__ uint32_t l_ent[kCluster * NN / 2];  // the cluster's entries, two per word
   T *lf = reinterpret_cast<T *>(lds_raw);
+  const SlotLds<T, NN> L(lds_raw);
   const int c = xcd_block(), t = threadIdx.x;
   const int e0 = ca.eoff[c], ne = ca.eoff[c + 1] - e0;
-  const int s0 = ca.sbase[c], M = ca.sbase[c + 1] - s0;
+  const int s0 = ca.sbase[c], M = ca.sbase[c + 1] - s0;  // M <= slot_cap(NN), checked at set-up
   const uint16_t *lo = ca.loff + s0 + c;
   // The cluster's slot tables are loaded into registers before the element
   // chain, so that the loads complete under its arithmetic: the entries
@@ -693,10 +830,28 @@ __global__ __launch_bounds__(kCluster) void f3_cluster(
     const int q = t + k * kCluster;
     ow[k] = q < M ? ((uint32_t)lo[q + 1] << 16) | lo[q] : 0u;
   }
+  int ls[NN];
+  if (t < ne) load_lslot<NN>(ca.lslot + (long)(e0 + t) * NN, ls);
+  // stage the slots
+  for (int j = t; j < M; j += kCluster) {
+    const int node = ca.snode[s0 + j];
+    L.node[j] = node;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      L.x[d * L.cap + j] = x[(long)d * N + node];
+      L.v[d * L.cap + j] = v[(long)d * N + node];
+    }
+    if constexpr (std::is_same<T, double>::value) L.r[j] = ratio[node];
+    else L.r[j] = reinterpret_cast<const float *>(ratio)[node];
+  }
+  __syncthreads();
+  ElemGeo<T, NN, TH> G;
+  if (t < ne) element_geometry<T, NN, TH>(fp, L, ls, G);
+  __syncthreads();  // the staged slots are read: their LDS takes the element-node values
   if (t < ne) {
-    element_chain<T, NN, LAW, kCluster, TH>(
-        e0 + t, E, N, fp, elnod, x, v, ratio, vol0, rho, sy0, He, tau_io, eps_io, p_out, pl_strain, sigma_y,
-        q_plheat, reinterpret_cast<double *>(lds_raw) + t, [&](int a, int node, T f0, T f1, T f2, T th) {
+    element_material<T, NN, LAW, kCluster, TH>(
+        e0 + t, E, fp, G, vol0, rho, sy0, He, tau_io, eps_io, p_out, pl_strain, sigma_y, q_plheat,
+        reinterpret_cast<double *>(lds_raw) + t, [&](int a, int node, T f0, T f1, T f2, T th) {
           lf[(a * R + 0) * kCluster + t] = f0;
           lf[(a * R + 1) * kCluster + t] = f1;
           lf[(a * R + 2) * kCluster + t] = f2;
@@ -1193,8 +1348,9 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
     const bool rate = law_class(fp.mp.model) == LAW_RATE, th = c->thermal.on;
     auto kern = (c->n == 8) ? (rate ? f3_pick<T, 8, LAW_RATE>(th) : f3_pick<T, 8, LAW_PLAIN>(th))
                             : (rate ? f3_pick<T, 4, LAW_RATE>(th) : f3_pick<T, 4, LAW_PLAIN>(th));
-    const ClusterArgs ca{c->ms.cl_eoff, c->ms.cl_sbase, c->ms.cl_loff, c->ms.cl_ent, c->ms.nslot};
-    hipLaunchKernelGGL(kern, dim3(c->ms.ncl), dim3(kCluster), 0, s, E, N, fp, ca, c->ms.elnod, c->nd.x, vN,
+    const ClusterArgs ca{c->ms.cl_eoff, c->ms.cl_sbase, c->ms.cl_loff,  c->ms.cl_ent,
+                         c->ms.cl_snode, c->ms.cl_lslot, c->ms.nslot};
+    hipLaunchKernelGGL(kern, dim3(c->ms.ncl), dim3(kCluster), 0, s, E, N, fp, ca, c->nd.x, vN,
                        c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.He, el.tau, el.eps, el.p, el.pl_strain,
                        el.sigma_y, el.part, c->thermal.on ? el.q_plheat : (T *)nullptr);
     // contact forces after the element forces, before the assembly

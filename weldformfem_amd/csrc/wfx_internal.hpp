@@ -137,6 +137,9 @@ struct DevMesh {
   int *cl_eoff = nullptr, *cl_sbase = nullptr;
   uint16_t *cl_loff = nullptr, *cl_ent = nullptr;
   int *nsl_off = nullptr, *nsl = nullptr;
+  // slot -> node [S]; each stored element's nodes as slot indices of its cluster [E][n]
+  int *cl_snode = nullptr;
+  uint16_t *cl_lslot = nullptr;
   int *flat_base = nullptr;  // thermal expansion: original element * n of each stored element
 };
 
