@@ -506,6 +506,30 @@ __device__ __forceinline__ void element_geometry(const FusedParams &fp, const Sl
 // cofactors in LDS across the stress update (stash: this thread's column,
 // stride BS): the laws' transcendental functions otherwise push the pass
 // past 256 VGPRs.
+// The element's state the second part reads (CalcStressStrain's stress,
+// strain, plastic strain and yield stress; the per-element plastic modulus
+// of specimen batches): loaded by the work-group before it stages its
+// slots, so that these loads are in flight together with the staging
+template <typename T>
+struct ElemState {
+  T tau[6], eps[6], ep, sy0, He;
+};
+template <typename T>
+__device__ __forceinline__ void load_elem_state(int e, int E, const T *__restrict__ tau_io,
+                                                const T *__restrict__ eps_io, const T *__restrict__ pl_strain,
+                                                const T *__restrict__ sy0, const T *__restrict__ He,
+                                                ElemState<T> &S) {
+  const long Es = E;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    S.tau[k] = ld_stream(&tau_io[k * Es + e]);
+    S.eps[k] = ld_stream(&eps_io[k * Es + e]);
+  }
+  S.ep = ld_stream(&pl_strain[e]);
+  S.sy0 = ld_stream(&sy0[e]);
+  S.He = He ? ld_stream(&He[e]) : T(0);
+}
+
 template <typename T, int NN, int LAW>
 constexpr bool kStashLDS = NN == 8 && LAW == LAW_RATE && std::is_same<T, double>::value;
 // hm[3][4], c_h, C[3][3] and with thermal coupling the gradient term G[3]
@@ -514,8 +538,8 @@ constexpr int kStashVals = TH ? 25 : 22;
 
 template <typename T, int NN, int LAW, int BS, bool TH, class Sink>
 __device__ __forceinline__ void element_material(
-    int e, int E, const FusedParams &fp, ElemGeo<T, NN, TH> &G, const T *__restrict__ vol0,
-    const T *__restrict__ rho, const T *__restrict__ sy0, const T *__restrict__ He, T *__restrict__ tau_io,
+    int e, int E, const FusedParams &fp, ElemGeo<T, NN, TH> &G, const ElemState<T> &S,
+    const T *__restrict__ vol0, const T *__restrict__ rho, const T *__restrict__ He, T *__restrict__ tau_io,
     T *__restrict__ eps_io, T *__restrict__ p_out, T *__restrict__ pl_strain, T *__restrict__ sigma_y,
     T *__restrict__ q_plheat, double *stash, Sink &&sink) {
   const long Es = E;
@@ -570,9 +594,7 @@ __device__ __forceinline__ void element_material(
   // ---- Jaumann update and J2 return (CalcStressStrain) -------------------
   T dt = (T)(fp.dtp ? *fp.dtp : fp.dt);
   const MatParams &mp = fp.mp;
-  T3<T> tau = t3_from_sym(ld_stream(&tau_io[0 * Es + e]), ld_stream(&tau_io[1 * Es + e]),
-                          ld_stream(&tau_io[2 * Es + e]), ld_stream(&tau_io[3 * Es + e]),
-                          ld_stream(&tau_io[4 * Es + e]), ld_stream(&tau_io[5 * Es + e]));
+  T3<T> tau = t3_from_sym(S.tau[0], S.tau[1], S.tau[2], S.tau[3], S.tau[4], S.tau[5]);
   T3<T> D = t3_from_sym(s00, s11, s22, s01, s12, s02);
   T3<T> W = t3_from_antisym(G.w01, G.w12, G.w02);
   T3<T> I = t3_identity<T>();
@@ -592,7 +614,7 @@ __device__ __forceinline__ void element_material(
   T J2 = T(0.5) * (s.xx * s.xx + T(2) * s.xy * s.xy + T(2) * s.xz * s.xz + s.yy * s.yy +
                    T(2) * s.yz * s.yz + s.zz * s.zz);
   T sig_trial = sqrt(T(3) * J2);
-  T ep = ld_stream(&pl_strain[e]);
+  T ep = S.ep;
   // bilinear plastic modulus: per element when given (specimen batches)
   T sy, Hb = (T)mp.H;
   const bool per_elem = LAW == LAW_PLAIN && mp.model == BILINEAR && He;
@@ -606,10 +628,10 @@ __device__ __forceinline__ void element_material(
   }
   RateCache<T> rc;
   if (per_elem) {
-    Hb = ld_stream(&He[e]);
-    sy = ld_stream(&sy0[e]) + Hb * ep;
+    Hb = S.He;
+    sy = S.sy0 + Hb * ep;
   } else {
-    sy = yield_stress<T, LAW>(mp, rt, ld_stream(&sy0[e]), ep, edot, rc);
+    sy = yield_stress<T, LAW>(mp, rt, S.sy0, ep, edot, rc);
   }
   if (fp.write_out) sigma_y[e] = sy;
   T dep = T(0);
@@ -652,12 +674,12 @@ __device__ __forceinline__ void element_material(
     }
     if (fp.write_out) static_cast<T *>(fp.avq)[e] = (T)qs;
   }
-  st_stream(&eps_io[0 * Es + e], ld_stream(&eps_io[0 * Es + e]) + D.xx * dt);
-  st_stream(&eps_io[1 * Es + e], ld_stream(&eps_io[1 * Es + e]) + D.yy * dt);
-  st_stream(&eps_io[2 * Es + e], ld_stream(&eps_io[2 * Es + e]) + D.zz * dt);
-  st_stream(&eps_io[3 * Es + e], ld_stream(&eps_io[3 * Es + e]) + D.xy * dt);
-  st_stream(&eps_io[4 * Es + e], ld_stream(&eps_io[4 * Es + e]) + D.yz * dt);
-  st_stream(&eps_io[5 * Es + e], ld_stream(&eps_io[5 * Es + e]) + D.xz * dt);
+  st_stream(&eps_io[0 * Es + e], S.eps[0] + D.xx * dt);
+  st_stream(&eps_io[1 * Es + e], S.eps[1] + D.yy * dt);
+  st_stream(&eps_io[2 * Es + e], S.eps[2] + D.zz * dt);
+  st_stream(&eps_io[3 * Es + e], S.eps[3] + D.xy * dt);
+  st_stream(&eps_io[4 * Es + e], S.eps[4] + D.yz * dt);
+  st_stream(&eps_io[5 * Es + e], S.eps[5] + D.xz * dt);
   st_stream(&tau_io[0 * Es + e], tau.xx); st_stream(&tau_io[1 * Es + e], tau.yy);
   st_stream(&tau_io[2 * Es + e], tau.zz); st_stream(&tau_io[3 * Es + e], tau.xy);
   st_stream(&tau_io[4 * Es + e], tau.yz); st_stream(&tau_io[5 * Es + e], tau.xz);
@@ -811,6 +833,15 @@ __global__ __launch_bounds__(kCluster) void f3_cluster(
   const int e0 = ca.eoff[c], ne = ca.eoff[c + 1] - e0;
   const int s0 = ca.sbase[c], M = ca.sbase[c + 1] - s0;  // M <= slot_cap(NN), checked at set-up
   const uint16_t *lo = ca.loff + s0 + c;
+  // the nodes of this thread's slots t and t + kCluster: first, as the
+  // staging loads depend on them
+  static_assert(slot_cap(NN) <= 2 * kCluster, "two slots per thread");
+  int sn[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int j = t + k * kCluster;
+    sn[k] = j < M ? ca.snode[s0 + j] : -1;
+  }
   // The cluster's slot tables are loaded into registers before the element
   // chain, so that the loads complete under its arithmetic: the entries
   // (ne * NN uint16 from a 4-byte aligned start, since e0 * NN is even) go to
@@ -831,18 +862,39 @@ __global__ __launch_bounds__(kCluster) void f3_cluster(
     ow[k] = q < M ? ((uint32_t)lo[q + 1] << 16) | lo[q] : 0u;
   }
   int ls[NN];
-  if (t < ne) load_lslot<NN>(ca.lslot + (long)(e0 + t) * NN, ls);
-  // stage the slots
-  for (int j = t; j < M; j += kCluster) {
-    const int node = ca.snode[s0 + j];
-    L.node[j] = node;
+  ElemState<T> S;
+  if (t < ne) {
+    load_lslot<NN>(ca.lslot + (long)(e0 + t) * NN, ls);
+    load_elem_state<T>(e0 + t, E, tau_io, eps_io, pl_strain, sy0, He, S);
+  }
+  // stage the slots: all loads first, then the LDS stores
+  double sx[2][3];
+  T sv[2][3], sr[2];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      L.x[d * L.cap + j] = x[(long)d * N + node];
-      L.v[d * L.cap + j] = v[(long)d * N + node];
+  for (int k = 0; k < 2; ++k) {
+    const int node = sn[k];
+    if (node >= 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        sx[k][d] = x[(long)d * N + node];
+        sv[k][d] = v[(long)d * N + node];
+      }
+      if constexpr (std::is_same<T, double>::value) sr[k] = ratio[node];
+      else sr[k] = reinterpret_cast<const float *>(ratio)[node];
     }
-    if constexpr (std::is_same<T, double>::value) L.r[j] = ratio[node];
-    else L.r[j] = reinterpret_cast<const float *>(ratio)[node];
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int j = t + k * kCluster;
+    if (sn[k] >= 0) {
+      L.node[j] = sn[k];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        L.x[d * L.cap + j] = sx[k][d];
+        L.v[d * L.cap + j] = sv[k][d];
+      }
+      L.r[j] = sr[k];
+    }
   }
   __syncthreads();
   ElemGeo<T, NN, TH> G;
@@ -850,7 +902,7 @@ __global__ __launch_bounds__(kCluster) void f3_cluster(
   __syncthreads();  // the staged slots are read: their LDS takes the element-node values
   if (t < ne) {
     element_material<T, NN, LAW, kCluster, TH>(
-        e0 + t, E, fp, G, vol0, rho, sy0, He, tau_io, eps_io, p_out, pl_strain, sigma_y, q_plheat,
+        e0 + t, E, fp, G, S, vol0, rho, He, tau_io, eps_io, p_out, pl_strain, sigma_y, q_plheat,
         reinterpret_cast<double *>(lds_raw) + t, [&](int a, int node, T f0, T f1, T f2, T th) {
           lf[(a * R + 0) * kCluster + t] = f0;
           lf[(a * R + 1) * kCluster + t] = f1;
