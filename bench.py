@@ -33,31 +33,41 @@ def fused_bytes(n, s, slots_per_elem, clusters_per_elem, hg=True, rho_uniform=Tr
     s = element real size (8 FP64, 4 FP32-mixed); nodal positions are
     double, v, prev_a, u_dt, fi and the ANP ratio are s bytes.  With a uniform density (the
     benchmark's) no density is read.  slots_per_elem = (cluster, node) slots
-    per element (the element pass writes one partial force per slot, the
-    node pass reads them with their ids), clusters_per_elem = clusters per
-    element; over8_per_node = share of nodes with more than 8 elements (their
-    CSR offset is read)."""
+    per element, clusters_per_elem = clusters per element; over8_per_node =
+    share of nodes with more than 8 elements (their CSR offset is read).
+    Both element passes stage their cluster's slots: they read each
+    element's nodes as slot indices (uint16 per element-node) and each
+    slot's node id (int32), and gather the nodal data once per node.  The
+    element pass writes one partial force per slot: FP64 in slot order (the
+    node pass reads the node's slot ids), FP32-mixed node-major (it reads
+    each slot's position instead, the node pass none)."""
     rho = 1 if (hg and not rho_uniform) else 0
     sn = s
+    spe, cpe = slots_per_elem, clusters_per_elem
     u_rw = 2 * 3 * 8 if s == 8 else 0  # FP32-mixed recovers u from x at the end of a run
+    node_major = s == 4
+    stage = 2 * n + I4 * spe  # slot indices of the element's nodes, node id per slot
     # tau, eps read and written; plastic strain read and written; sy0;
     # [rho] (pressure and yield stress are stored on a run's last step only)
     f3_reals = 12 + 12 + 2 + 1 + rho
-    # elnod, element reals, slot entries (uint16 per element-node), slot
-    # offsets (uint16 per slot and per cluster), partial forces per slot
-    f3_elem = n * I4 + f3_reals * s + 2 * n + 2 * (slots_per_elem + clusters_per_elem) + 3 * s * slots_per_elem
+    # staging, element reals, slot entries (uint16 per element-node), slot
+    # offsets (uint16 per slot and per cluster), [slot positions], partial
+    # forces per slot
+    f3_elem = (stage + f3_reals * s + 2 * n + 2 * (spe + cpe) + (I4 * spe if node_major else 0)
+               + 3 * s * spe)
     return {
-        "f1_volume": (n * I4 + s, 3 * 8),
+        "f1_volume": (stage + s, 3 * 8),
         # vol gathered (V_n once per run), [rho]; padded first-8 table, count,
         # [offset], V0_n read, mass and ANP ratio written (FP32-mixed stores the
         # ratio's deviation from 1 as a float)
         "f2_nodal": (s + rho * s, 8 * I4 + I4 + 4 * over8_per_node + 8 + 8 + s),
         # gathers x, v, ratio
         "f3_element": (f3_elem, 3 * 8 + 3 * sn + s),
-        # slot ids and partial forces; offsets, mass, BC slot; v, prev_a, u_dt
+        # [slot ids] and partial forces; offsets, mass, BC slot; v, prev_a, u_dt
         # read and written, x read and written, [u read and written] (fi is
         # stored on a run's last step only)
-        "f4_node": (slots_per_elem * (I4 + 3 * s), I4 + 8 + I4 + 3 * 2 * 3 * sn + 2 * 3 * 8 + u_rw),
+        "f4_node": (spe * ((0 if node_major else I4) + 3 * s),
+                    I4 + 8 + I4 + 3 * 2 * 3 * sn + 2 * 3 * 8 + u_rw),
     }
 
 

@@ -270,12 +270,15 @@ int wfx_fused_phase(wfx_ctx *ctx, int phase, int predict, wfx_stream s);
 int wfx_reaction_force(wfx_ctx *ctx, int dim, double *out, wfx_stream s);
 
 /* ---- launch tuning and graphs (no reference counterpart) ----------------- */
-/* Chooses the block size (64/128/256/512) of each fused phase on the current
- * mesh: coordinate descent on the time of whole fused steps (max(reps, 10)
- * steps per candidate, best of five rounds, two sweeps over the phases),
- * fastest kept (chosen[4], may be NULL); the state is saved and restored
- * around the measurement. */
+/* Chooses the block size (64/128/256/512) of the fused node passes (phases
+ * 2 and 4) on the current mesh: coordinate descent on the time of whole
+ * fused steps (max(reps, 10) steps per candidate, best of five rounds, two
+ * sweeps), fastest kept (chosen[4], may be NULL; phases 1 and 3 run one
+ * 256-thread work-group per element cluster and report 256); the state is
+ * saved and restored around the measurement. */
 int wfx_autotune(wfx_ctx *ctx, int reps, int *chosen, wfx_stream s);
+/* Block sizes of the four fused phases (64/128/256/512 each; the element
+ * passes' entries are accepted and not used). */
 int wfx_set_block_sizes(wfx_ctx *ctx, const int *block_sizes4);
 /* nsteps fused steps as wfx_step_fused runs them, with HIP events recorded on
  * stream s between the four passes: us4[k] = average duration of pass k+1 in

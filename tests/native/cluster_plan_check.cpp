@@ -45,7 +45,8 @@ int main(int argc, char **argv) {
   const std::string o = argv[7];
   bool ok = write_all(o + "s2o", p.s2o) && write_all(o + "eoff", p.eoff) && write_all(o + "sbase", p.sbase) &&
             write_all(o + "loff", p.loff) && write_all(o + "ent", p.ent) && write_all(o + "nsl_off", p.nsl_off) &&
-            write_all(o + "nsl", p.nsl) && write_all(o + "snode", p.snode) && write_all(o + "lslot", p.lslot);
+            write_all(o + "nsl", p.nsl) && write_all(o + "snode", p.snode) && write_all(o + "lslot", p.lslot) &&
+            write_all(o + "spos", p.spos);
   std::printf("%d %ld %d %d\n", p.clusters(), p.slots(), p.max_slots, wfx::slot_cap(n));
   return ok ? 0 : 1;
 }

@@ -50,7 +50,7 @@ def plan(planner, tmp_path, x, el, n, cs=256):
                         str(tmp_path / "x.f64"), pre], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     out = {k: np.fromfile(pre + k, dtype=np.int32) for k in
-           ("s2o", "eoff", "sbase", "loff", "ent", "nsl_off", "nsl", "snode", "lslot")}
+           ("s2o", "eoff", "sbase", "loff", "ent", "nsl_off", "nsl", "snode", "lslot", "spos")}
     _, _, out["max_slots"], out["slot_cap"] = (int(q) for q in r.stdout.split())
     return out
 
@@ -90,6 +90,8 @@ def check_plan(p, el, N, n, cs=256):
     ls = p["lslot"].reshape(E, n)
     assert np.all(ls < M[cl_of][:, None])
     np.testing.assert_array_equal(p["snode"][sbase[cl_of][:, None] + ls], el[s2o])
+    # each slot's partial sum is stored at its place in the node's slot list
+    np.testing.assert_array_equal(p["nsl"][p["spos"]], np.arange(sbase[-1]))
     # the node's slots in cluster order reproduce its CSR list exactly
     ref = RefDomain()
     ref.CreateFromMesh(np.zeros((N, 3)), el.reshape(-1), n)

@@ -170,9 +170,16 @@ static int setup_device(wfx_ctx *c, const ClusterPlan &plan) {
     return WFX_ERR_STATE;
   }
   A(c->ms.cl_eoff, c->ms.ncl + 1) A(c->ms.cl_sbase, c->ms.ncl + 1) A(c->ms.cl_loff, c->ms.nslot + c->ms.ncl)
-  A(c->ms.cl_ent, n * E) A(c->ms.nsl_off, N + 1) A(c->ms.nsl, c->ms.nslot)
+  A(c->ms.cl_ent, n * E) A(c->ms.nsl_off, N + 1)
   A(c->ms.cl_snode, c->ms.nslot) A(c->ms.cl_lslot, n * E)
-  c->cfl_cap_min = (int)((E + 63) / 64 + 1);  // blocks of the element passes at block size >= 64
+  if (c->precision == WFX_FP64) {
+    A(c->ms.nsl, c->ms.nslot)
+  } else {
+    A(c->ms.spos, c->ms.nslot)
+  }
+  // partial minima of the volume pass (one per element cluster) and of the
+  // per-step kernels at block size >= 64
+  c->cfl_cap_min = (int)std::max<size_t>((E + 63) / 64 + 1, (size_t)c->ms.ncl + 1);
   A(c->cfl_part, c->cfl_cap_min + (N + 63) / 64 + 1)
   if (e == hipSuccess) e = (c->precision == WFX_FP64) ? alloc_elem(c, c->e64) : alloc_elem(c, c->e32);
   if (c->precision == WFX_FP32_MIXED) {
@@ -206,7 +213,10 @@ static int setup_device(wfx_ctx *c, const ClusterPlan &plan) {
   WFX_CHECK_HIP(hipMemcpy(c->ms.cl_loff, plan.loff.data(), plan.loff.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemcpy(c->ms.cl_ent, plan.ent.data(), plan.ent.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemcpy(c->ms.nsl_off, plan.nsl_off.data(), plan.nsl_off.size() * sizeof(int), hipMemcpyHostToDevice));
-  WFX_CHECK_HIP(hipMemcpy(c->ms.nsl, plan.nsl.data(), plan.nsl.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (c->ms.nsl)
+    WFX_CHECK_HIP(hipMemcpy(c->ms.nsl, plan.nsl.data(), plan.nsl.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (c->ms.spos)
+    WFX_CHECK_HIP(hipMemcpy(c->ms.spos, plan.spos.data(), plan.spos.size() * sizeof(int), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemcpy(c->ms.cl_snode, plan.snode.data(), plan.snode.size() * sizeof(int), hipMemcpyHostToDevice));
   WFX_CHECK_HIP(hipMemcpy(c->ms.cl_lslot, plan.lslot.data(), plan.lslot.size() * sizeof(uint16_t),
                           hipMemcpyHostToDevice));
@@ -1464,10 +1474,10 @@ int wfx_autotune(wfx_ctx *c, int reps, int *chosen, wfx_stream s) {
   // of the choice (with three rounds of 5 steps a pass of 0.66 ms was once
   // tuned to a choice 4% slower)
   const int nrep = reps > 10 ? reps : 10;
-  // the element pass (phase 3) runs one work-group per element cluster
-  if (chosen) chosen[2] = kCluster;
+  // the element passes (phases 1 and 3) run one work-group per element cluster
+  if (chosen) chosen[0] = chosen[2] = kCluster;
   for (int sweep = 0; sweep < 2 && err == hipSuccess; ++sweep)
-    for (int ph = 1; ph <= 4 && err == hipSuccess; ++ph) {
+    for (int ph = 2; ph <= 4 && err == hipSuccess; ++ph) {
       if (ph == 3) continue;
       float best_t[4] = {1e30f, 1e30f, 1e30f, 1e30f};
       for (int round = 0; round < 5 && err == hipSuccess; ++round)

@@ -250,6 +250,8 @@ int plan_clusters(int N, int E, int n, const int *elnod, const double *x, int cs
       p.nsl[fill[node]++] = p.sbase[c] + j;
     }
   }
+  p.spos.assign((size_t)S, 0);
+  for (long k = 0; k < S; ++k) p.spos[p.nsl[k]] = (int)k;
   return 0;
 }
 

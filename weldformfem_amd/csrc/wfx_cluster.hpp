@@ -27,6 +27,7 @@ struct ClusterPlan {
                                // grouped by slot; cluster c's entries at eoff[c] * n
   std::vector<int> nsl_off;    // [N+1] node -> its slots, ascending cluster
   std::vector<int> nsl;        // [S]
+  std::vector<int> spos;       // [S] slot -> its position k in nsl (nsl[spos[s]] = s)
   std::vector<int> snode;      // [S] slot -> its node (ascending node within a cluster)
   std::vector<uint16_t> lslot; // [E][n] stored element's nodes as slot indices within its cluster
   int max_slots = 0;           // largest slot count of a cluster (<= slot_cap(n))

@@ -89,77 +89,6 @@ __global__ __launch_bounds__(kBlock) void f0_vsum0(int N, const int *__restrict_
   vsum0[i] = s0;
 }
 
-// F1: vol = detJ * w (calcElemJAndDerivatives determinant + CalcElemVol);
-// with minl_part also the block's minimum edge length (per-step CFL time step)
-// With texp (thermal expansion): also each element's mean of the last
-// step's element-node temperature rates at the flat indices of its node ids
-// (calcThermalExpansion's dTdt), for the element pass.
-template <typename T, int NN, int BS>
-__global__ __launch_bounds__(BS) void f1_volume(int E, int N, const int *__restrict__ elnod,
-                                                    const double *__restrict__ x,
-                                                    T *__restrict__ vol, double *__restrict__ minl_part,
-                                                    unsigned int *__restrict__ status,
-                                                    const T *__restrict__ dte_flat, T *__restrict__ texp) {
-  int e = gtid();
-  double medge = kMinEdgeInit;
-  if (e < E) {
-  if (texp) {
-    double m = 0.0;
-#pragma unroll
-    for (int a = 0; a < NN; ++a) m += 1.0 / NN * (double)dte_flat[elnod[(long)a * E + e]];
-    texp[e] = (T)m;
-  }
-  T X[NN][3];
-  if constexpr (std::is_same<T, double>::value) {
-#pragma unroll
-    for (int a = 0; a < NN; ++a) {
-      int g = elnod[(long)a * E + e];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) X[a][c] = x[(long)c * N + g];
-    }
-  } else {
-    int g0 = elnod[e];
-    double x0[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) { x0[c] = x[(long)c * N + g0]; X[0][c] = T(0); }
-#pragma unroll
-    for (int a = 1; a < NN; ++a) {
-      int g = elnod[(long)a * E + e];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) X[a][c] = T(x[(long)c * N + g] - x0[c]);
-    }
-  }
-  T J[3][3];
-  if constexpr (NN == 8) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      J[0][c] = T(0.125) * (-X[0][c] + X[1][c] + X[2][c] - X[3][c] - X[4][c] + X[5][c] + X[6][c] - X[7][c]);
-      J[1][c] = T(0.125) * (-X[0][c] - X[1][c] + X[2][c] + X[3][c] - X[4][c] - X[5][c] + X[6][c] + X[7][c]);
-      J[2][c] = T(0.125) * (-X[0][c] - X[1][c] - X[2][c] - X[3][c] + X[4][c] + X[5][c] + X[6][c] + X[7][c]);
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      J[0][c] = X[1][c] - X[0][c];
-      J[1][c] = X[2][c] - X[0][c];
-      J[2][c] = X[3][c] - X[0][c];
-    }
-  }
-  T det = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[1][2] * J[2][1] - J[0][1] * J[1][0] * J[2][2] +
-          J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] - J[0][2] * J[1][1] * J[2][0];
-  const T w = (NN == 4) ? T(1.0 / 6.0) : T(8.0);
-  T v = T(0);
-  v += det * w * T(1);
-  vol[e] = v;
-  count_inverted(det, status);
-  if (minl_part) medge = min_edge4<T>(*reinterpret_cast<const T(*)[4][3]>(&X[0][0]));
-  }
-  if (minl_part) {
-    medge = block_extreme<BS, true>(medge);
-    if (threadIdx.x == 0) minl_part[blockIdx.x] = medge;
-  }
-}
-
 // F2: nodal volume sum in CSR order; ratio for the ANP pressure and nodal
 // mass (the nodal volume field itself is written once per run, in phase 5).
 template <typename T, int BS>
@@ -389,21 +318,14 @@ struct ElemGeo {
   double tsum, gT0, gT1, gT2;
 };
 
-// F3, first part: everything that reads the staged nodal data of element
-// `ls` (its nodes as slot indices): Jacobian, cofactors and determinant
-// (calcElemJAndDerivatives), the velocity gradient and from it the strain
-// rate and spin (calcElemStrainRates), the hourglass amplitudes, the ANP
-// average of the nodal ratios (calcElemPressure) and the contact test.
-// sum_a dH(d,a) y_a = sum_i C[i][d] (0.125 m_i(y)) for a hexahedron
-// (dH(d,a) = 0.125 sum_i s_i(a) C[i][d], Domain_d.C:1843-1850), and
-// sum_i C[i][d] (y_{i+1} - y_0) for a tetrahedron (:1866-1887).
-template <typename T, int NN, bool TH>
-__device__ __forceinline__ void element_geometry(const FusedParams &fp, const SlotLds<T, NN> &L,
-                                                 const int (&ls)[NN], ElemGeo<T, NN, TH> &G) {
-  T J[3][3];
+// The Jacobian of an element from its staged positions (rows J(i,d) =
+// 0.125 m_i(x_d) for the hexahedron, x_{i+1} - x_0 for the tetrahedron),
+// formed in double from the positions and narrowed to T
+template <typename T, int NN>
+__device__ __forceinline__ void staged_jacobian(const double *lx, int cap, const int (&ls)[NN], T (&J)[3][3]) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const double *xc = L.x + c * L.cap;
+    const double *xc = lx + c * cap;
     if constexpr (NN == 8) {
       double y[8], m1, m2, m4;
 #pragma unroll
@@ -418,6 +340,21 @@ __device__ __forceinline__ void element_geometry(const FusedParams &fp, const Sl
       for (int i = 0; i < 3; ++i) J[i][c] = T(xc[ls[i + 1]] - x0);
     }
   }
+}
+
+// F3, first part: everything that reads the staged nodal data of element
+// `ls` (its nodes as slot indices): Jacobian, cofactors and determinant
+// (calcElemJAndDerivatives), the velocity gradient and from it the strain
+// rate and spin (calcElemStrainRates), the hourglass amplitudes, the ANP
+// average of the nodal ratios (calcElemPressure) and the contact test.
+// sum_a dH(d,a) y_a = sum_i C[i][d] (0.125 m_i(y)) for a hexahedron
+// (dH(d,a) = 0.125 sum_i s_i(a) C[i][d], Domain_d.C:1843-1850), and
+// sum_i C[i][d] (y_{i+1} - y_0) for a tetrahedron (:1866-1887).
+template <typename T, int NN, bool TH>
+__device__ __forceinline__ void element_geometry(const FusedParams &fp, const SlotLds<T, NN> &L,
+                                                 const int (&ls)[NN], ElemGeo<T, NN, TH> &G) {
+  T J[3][3];
+  staged_jacobian<T, NN>(L.x, L.cap, ls, J);
   cofactors(J, G.C, G.detJ);
   const T w = (NN == 4) ? T(1.0 / 6.0) : T(8.0);
   G.volv = G.detJ * w;
@@ -776,8 +713,19 @@ struct ClusterArgs {
   const uint16_t *ent;   // element-node entries t * 8 + a grouped by slot, cluster c at eoff[c] * n
   const int *snode;      // [S] slot -> node
   const uint16_t *lslot; // [E][n] element-node -> slot within its cluster
+  const int *spos;       // [S] FP32-mixed: slot -> where its partial sum is stored (node-major)
   long S;                // slots: the row length of the partial sums
 };
+
+// Where the element pass stores a slot's partial sum.  FP64: in slot order
+// (a cluster's sums are one contiguous run; the node pass reads a node's
+// slots through their ids, and moves its bytes near the copy rate anyway).
+// FP32-mixed: node-major, at the slot's place in its node's slot list (the
+// node pass reads a node's sums contiguously from its offset, one dependent
+// load less; measured on the 200x200x100 block: node pass 154 -> 118 us,
+// element pass +12 us, while FP64 loses 15 us in the element pass).
+template <typename T>
+constexpr bool kNodeMajor = std::is_same<T, float>::value;
 
 // Values per element-node the element pass adds up: the force and, with
 // thermal coupling, the node's thermal term
@@ -856,10 +804,12 @@ __global__ __launch_bounds__(kCluster) void f3_cluster(
     ew[k] = w < nw ? en32[w] : 0u;
   }
   uint32_t ow[2];  // (end << 16) | start of slots t and t + kCluster
+  int sp[2];       //   and where their sums go
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int q = t + k * kCluster;
     ow[k] = q < M ? ((uint32_t)lo[q + 1] << 16) | lo[q] : 0u;
+    sp[k] = (kNodeMajor<T> && q < M) ? ca.spos[s0 + q] : s0 + q;
   }
   int ls[NN];
   ElemState<T> S;
@@ -923,20 +873,101 @@ __global__ __launch_bounds__(kCluster) void f3_cluster(
   }
   __syncthreads();
   const uint16_t *en = reinterpret_cast<const uint16_t *>(l_ent);
-  for (int j = t, r = 0; j < M; j += kCluster, ++r) {
-    const uint32_t o = r < 2 ? ow[r] : ((uint32_t)lo[j + 1] << 16) | lo[j];  // > 2 slots per thread: reloaded
-    const int k0 = o & 0xffffu, k1 = o >> 16;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {  // M <= 2 * kCluster
+    const int j = t + r * kCluster;
+    if (j >= M) break;
+    const int k0 = ow[r] & 0xffffu, k1 = ow[r] >> 16;
     double acc[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    for (int q = 0; q < R; ++q) acc[q] = 0.0;
     for (int k = k0; k < k1; ++k) {
       const int q = en[k];
       const int tt = q >> 3, a = q & 7;
 #pragma unroll
-      for (int r = 0; r < R; ++r) acc[r] += (double)lf[(a * R + r) * kCluster + tt];
+      for (int u = 0; u < R; ++u) acc[u] += (double)lf[(a * R + u) * kCluster + tt];
     }
 #pragma unroll
-    for (int r = 0; r < R; ++r) st_stream(&part[r * ca.S + s0 + j], (T)acc[r]);
+    for (int q = 0; q < R; ++q) {
+      if constexpr (kNodeMajor<T>) part[q * ca.S + sp[r]] = (T)acc[q];
+      else st_stream(&part[q * ca.S + sp[r]], (T)acc[q]);
+    }
+  }
+}
+
+// F1 as a cluster pass: one work-group per element cluster stages its
+// slots' positions (and node ids when thermal expansion reads them) in LDS,
+// then each thread forms its element's Jacobian from LDS: vol = detJ * w
+// (calcElemJAndDerivatives determinant + CalcElemVol), the inverted-element
+// count, with minl_part the work-group's smallest edge (per-step CFL time
+// step), with texp each element's mean of the last step's element-node
+// temperature rates at the flat indices of its node ids (calcThermalExpansion's
+// dTdt).  The same Jacobian as the element pass's.
+template <typename T, int NN>
+__global__ __launch_bounds__(kCluster) void f1_cluster(int E, ClusterArgs ca, const double *__restrict__ x, int N,
+                                                       T *__restrict__ vol, double *__restrict__ minl_part,
+                                                       unsigned int *__restrict__ status,
+                                                       const T *__restrict__ dte_flat, T *__restrict__ texp) {
+  constexpr int cap = slot_cap(NN);
+  __shared__ double lx[3 * cap];
+  __shared__ int lnode[cap];
+  const int c = xcd_block(), t = threadIdx.x;
+  const int e0 = ca.eoff[c], ne = ca.eoff[c + 1] - e0;
+  const int s0 = ca.sbase[c], M = ca.sbase[c + 1] - s0;  // M <= slot_cap(NN), checked at set-up
+  int sn[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int j = t + k * kCluster;
+    sn[k] = j < M ? ca.snode[s0 + j] : -1;
+  }
+  int ls[NN];
+  if (t < ne) load_lslot<NN>(ca.lslot + (long)(e0 + t) * NN, ls);
+  double sx[2][3];
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (sn[k] >= 0)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) sx[k][d] = x[(long)d * N + sn[k]];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int j = t + k * kCluster;
+    if (sn[k] >= 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) lx[d * cap + j] = sx[k][d];
+      if (texp) lnode[j] = sn[k];
+    }
+  }
+  __syncthreads();
+  double medge = kMinEdgeInit;
+  if (t < ne) {
+    const int e = e0 + t;
+    if (texp) {
+      double m = 0.0;
+#pragma unroll
+      for (int a = 0; a < NN; ++a) m += 1.0 / NN * (double)dte_flat[lnode[ls[a]]];
+      texp[e] = (T)m;
+    }
+    T J[3][3], C[3][3], det;
+    staged_jacobian<T, NN>(lx, cap, ls, J);
+    cofactors(J, C, det);
+    const T w = (NN == 4) ? T(1.0 / 6.0) : T(8.0);
+    vol[e] = det * w;
+    count_inverted(det, status);
+    if (minl_part) {  // the first four nodes, as calcMinEdgeLength takes them
+      T P[4][3];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          if constexpr (std::is_same<T, double>::value) P[a][d] = lx[d * cap + ls[a]];
+          else P[a][d] = T(lx[d * cap + ls[a]] - lx[d * cap + ls[0]]);
+        }
+      medge = min_edge4<T>(P);
+    }
+  }
+  if (minl_part) {
+    medge = block_extreme<kCluster, true>(medge);
+    if (threadIdx.x == 0) minl_part[blockIdx.x] = medge;
   }
 }
 
@@ -1071,6 +1102,41 @@ __device__ __forceinline__ void set_step_length(NodeParams &np, double dt) {
   np.pred_v = (1.0 - np.gamma) * dt;
 }
 
+// The state of node i the nodal update reads, loaded before the node's
+// partial force sums so that these loads are in flight together with them.
+template <typename T, bool CONTACT>
+struct NodeIn {
+  T v[3], pa[3], ud[3];
+  double x[3], u[3], fe[3], cf[3], m;
+  int slot;
+};
+template <typename T, bool CONTACT>
+__device__ __forceinline__ void load_node(int i, long Ns, const NodeParams &np, const double *__restrict__ cf,
+                                          const double *__restrict__ fe, const double *__restrict__ mdiag,
+                                          const int *__restrict__ bc_slot, const T *__restrict__ v,
+                                          const T *__restrict__ prev_a, const T *__restrict__ u_dt,
+                                          const double *__restrict__ x, const double *__restrict__ u,
+                                          NodeIn<T, CONTACT> &s) {
+  // row bases are uniform; the unsigned node index lets every row share one
+  // 32-bit offset register
+  const unsigned ui = (unsigned)i;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const long r = d * Ns;
+    s.v[d] = (v + r)[ui];
+    s.pa[d] = (prev_a + r)[ui];
+    s.ud[d] = (u_dt + r)[ui];
+    s.x[d] = (x + r)[ui];
+    s.u[d] = 0.0;
+    if constexpr (std::is_same<T, double>::value) s.u[d] = (u + r)[ui];  // FP32-mixed: u from x when a run ends
+    s.fe[d] = np.has_fe ? (fe + r)[ui] : 0.0;
+    s.cf[d] = 0.0;
+    if constexpr (CONTACT) s.cf[d] = (cf + r)[ui];
+  }
+  s.m = mdiag[ui];
+  s.slot = bc_slot[ui];
+}
+
 // The nodal update of one step for node i given its assembled internal
 // force f: non-finite scrub (Solver_explicit.C:777-785), calcAccel, ImposeBCA,
 // UpdateCorrectionAccVel, ImposeBCV, UpdateCorrectionPos and the next step's
@@ -1081,43 +1147,44 @@ __device__ __forceinline__ void set_step_length(NodeParams &np, double dt) {
 // run ends).
 template <typename T, bool CFL, bool CONTACT>
 __device__ __forceinline__ double node_update(
-    int i, long Ns, const NodeParams &np, double (&f)[3], const double *__restrict__ cf,
-    const double *__restrict__ fe,
-    const double *__restrict__ mdiag, const int *__restrict__ bc_slot, const int *__restrict__ bc_mask,
-    const double *__restrict__ bc_val, T *__restrict__ fi, T *__restrict__ v, T *__restrict__ prev_a,
-    T *__restrict__ u_dt, double *__restrict__ x, double *__restrict__ u, unsigned int *__restrict__ status) {
+    int i, long Ns, const NodeParams &np, double (&f)[3], const NodeIn<T, CONTACT> &s,
+    const int *__restrict__ bc_mask, const double *__restrict__ bc_val, T *__restrict__ fi, T *__restrict__ v,
+    T *__restrict__ prev_a, T *__restrict__ u_dt, double *__restrict__ x, double *__restrict__ u,
+    unsigned int *__restrict__ status) {
   unsigned bad = 0;
 #pragma unroll
   for (int d = 0; d < 3; ++d)
     if (!isfinite(f[d])) { f[d] = 0.0; ++bad; }
   if (bad) atomicAdd(status, bad);
-  const int slot = bc_slot[i];
+  const int slot = s.slot;
   const int mask = slot >= 0 ? bc_mask[slot] : 0;
-  const double m = mdiag[i];
+  const double m = s.m;
   double vs[3];
+  const unsigned ui = (unsigned)i;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const long id = d * Ns + i;
-    if (np.write_fi) fi[id] = (T)f[d];
-    double fed = np.has_fe ? fe[id] : 0.0;
-    double a = (fed - f[d]) / m;                       // calcAccel
-    if constexpr (CONTACT) a += cf[id] / m;             //   + contact force
+    const long r = d * Ns;
+    T &fi_d = (fi + r)[ui], &v_d = (v + r)[ui], &pa_d = (prev_a + r)[ui], &ud_d = (u_dt + r)[ui];
+    double &x_d = (x + r)[ui];
+    if (np.write_fi) fi_d = (T)f[d];
+    double a = (s.fe[d] - f[d]) / m;                    // calcAccel
+    if constexpr (CONTACT) a += s.cf[d] / m;            //   + contact force
     if (mask & (1 << d)) a = 0.0;                       // ImposeBCA
-    double pa = (double)prev_a[id];
+    double pa = (double)s.pa[d];
     a = np.f * (a - np.alpha * pa);                     // UpdateCorrectionAccVel
-    double vv = (double)v[id] + np.gdt * a;
+    double vv = (double)s.v[d] + np.gdt * a;
     if (mask & (1 << d)) vv = bc_val[3L * slot + d];    // ImposeBCV
-    double ud = (double)u_dt[id] + np.bdt2 * a;         // UpdateCorrectionPos
-    x[id] = x[id] + ud;
-    if constexpr (std::is_same<T, double>::value) u[id] = u[id] + ud;
+    double ud = (double)s.ud[d] + np.bdt2 * a;          // UpdateCorrectionPos
+    x_d = s.x[d] + ud;
+    if constexpr (std::is_same<T, double>::value) (u + r)[ui] = s.u[d] + ud;
     if (np.predict) {                                   // UpdatePrediction
       ud = np.dt * (vv + np.pred_u * a);
       vv = vv + np.pred_v * a;
       if (mask & (1 << d)) vv = bc_val[3L * slot + d];  // ImposeBCV
     }
-    u_dt[id] = (T)ud;
-    prev_a[id] = (T)a;
-    v[id] = (T)vv;
+    ud_d = (T)ud;
+    pa_d = (T)a;
+    v_d = (T)vv;
     vs[d] = (double)(T)vv;
   }
   // speed of the predicted velocity, for the next step's CFL time step
@@ -1127,8 +1194,9 @@ __device__ __forceinline__ double node_update(
 
 // Internal force of node i (and with TH its thermal term in f[3]): the sum of
 // its slots' partial sums in ascending cluster order (assemblyForces,
-// Matrices.C:42-87, completed across clusters).  Slot ids are loaded four
-// at a time, then their partial sums.
+// Matrices.C:42-87, completed across clusters), four at a time: FP64 through
+// the slot ids of the node's list, FP32-mixed directly from its offset (the
+// element pass stores those sums node-major).
 template <typename T, bool TH = false>
 __device__ __forceinline__ void gather_slots(int i, const int *__restrict__ nsl_off, const int *__restrict__ nsl,
                                              const T *__restrict__ part, long S, double (&f)[TH ? 4 : 3]) {
@@ -1139,7 +1207,10 @@ __device__ __forceinline__ void gather_slots(int i, const int *__restrict__ nsl_
   for (int k = 0; k < c; k += 4) {
     int sl[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) sl[j] = (k + j < c) ? nsl[o + k + j] : -1;
+    for (int j = 0; j < 4; ++j) {
+      if constexpr (kNodeMajor<T>) sl[j] = (k + j < c) ? o + k + j : -1;
+      else sl[j] = (k + j < c) ? nsl[o + k + j] : -1;
+    }
     T q[4][R];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
@@ -1170,11 +1241,13 @@ __global__ __launch_bounds__(BS) void f4_node(
   if constexpr (CFL) set_step_length(q, *q.dtp);
   double speed = 0.0;
   if (i < N) {
+    NodeIn<T, CONTACT> st;
+    load_node<T, CONTACT>(i, N, q, cf, fe, mdiag, bc_slot, v, prev_a, u_dt, x, u, st);
     double f[TH ? 4 : 3];
     gather_slots<T, TH>(i, nsl_off, nsl, part, S, f);
-    speed = node_update<T, CFL, CONTACT>(i, N, q, *reinterpret_cast<double(*)[3]>(f), cf, fe, mdiag, bc_slot, bc_mask,
-                                         bc_val, fi, v, prev_a, u_dt, x, u, status);
-    if constexpr (TH) Tn[i] += f[3] / mdiag[i] * q.dt;
+    speed = node_update<T, CFL, CONTACT>(i, N, q, *reinterpret_cast<double(*)[3]>(f), st, bc_mask, bc_val, fi, v,
+                                         prev_a, u_dt, x, u, status);
+    if constexpr (TH) Tn[i] += f[3] / st.m * q.dt;
   }
   if constexpr (CFL) {
     if (q.predict) {
@@ -1347,7 +1420,7 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
   const auto &P = c->prm;
   const auto &st = P.stab;
   const int N = c->N, E = c->E;
-  const int gE = (E + BS - 1) / BS, gN = (N + BS - 1) / BS;
+  const int gN = (N + BS - 1) / BS;
   // nodal arrays of the fused node pass in the element real type
   T *vN, *paN, *udN, *fiN;
   if constexpr (std::is_same<T, double>::value) {
@@ -1355,17 +1428,16 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
   } else {
     vN = c->v32; paN = c->pa32; udN = c->ud32; fiN = c->fi32;
   }
+  const ClusterArgs ca{c->ms.cl_eoff, c->ms.cl_sbase, c->ms.cl_loff,  c->ms.cl_ent,
+                       c->ms.cl_snode, c->ms.cl_lslot, c->ms.spos, c->ms.nslot};
   if (phase == 1) {
     double *minl = cfl ? c->cfl_part : nullptr;
     const bool texp = c->thermal.on && c->thermal.exp_T != 0.0;
-    if (c->n == 8)
-      hipLaunchKernelGGL((f1_volume<T, 8, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol, minl,
-                         c->status, el.dte_flat, texp ? el.texp : (T *)nullptr);
-    else
-      hipLaunchKernelGGL((f1_volume<T, 4, BS>), dim3(gE), dim3(BS), 0, s, E, N, c->ms.elnod, c->nd.x, el.vol, minl,
-                         c->status, el.dte_flat, texp ? el.texp : (T *)nullptr);
+    auto kern = c->n == 8 ? f1_cluster<T, 8> : f1_cluster<T, 4>;
+    hipLaunchKernelGGL(kern, dim3(c->ms.ncl), dim3(kCluster), 0, s, E, ca, c->nd.x, N, el.vol, minl, c->status,
+                       el.dte_flat, texp ? el.texp : (T *)nullptr);
     if (cfl) {
-      hipError_t e = launch_cfl_reduce<T>(c, gE, step, s);
+      hipError_t e = launch_cfl_reduce<T>(c, c->ms.ncl, step, s);
       // calcMinEdgeLength also refreshes the element heights contact uses
       if (e == hipSuccess && c->contact.on) e = launch_elem_length(c, s);
       return e;
@@ -1400,8 +1472,6 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
     const bool rate = law_class(fp.mp.model) == LAW_RATE, th = c->thermal.on;
     auto kern = (c->n == 8) ? (rate ? f3_pick<T, 8, LAW_RATE>(th) : f3_pick<T, 8, LAW_PLAIN>(th))
                             : (rate ? f3_pick<T, 4, LAW_RATE>(th) : f3_pick<T, 4, LAW_PLAIN>(th));
-    const ClusterArgs ca{c->ms.cl_eoff, c->ms.cl_sbase, c->ms.cl_loff,  c->ms.cl_ent,
-                         c->ms.cl_snode, c->ms.cl_lslot, c->ms.nslot};
     hipLaunchKernelGGL(kern, dim3(c->ms.ncl), dim3(kCluster), 0, s, E, N, fp, ca, c->nd.x, vN,
                        c->nd.anp_ratio, el.vol0, el.rho, el.sy0, el.He, el.tau, el.eps, el.p, el.pl_strain,
                        el.sigma_y, el.part, c->thermal.on ? el.q_plheat : (T *)nullptr);

@@ -131,12 +131,14 @@ struct DevMesh {
   int *nodel_pad = nullptr; // the first 8 packed entries of every node, [8][N], -1 past its count
   // element clusters of the fused element pass (wfx_cluster.hpp): cluster
   // element and slot offsets [C+1], per-cluster slot entry offsets [S+C],
-  // element-node entries [E*n], node -> slots CSR [N+1] / [S]
+  // element-node entries [E*n], node -> slots CSR offsets [N+1]; FP64: the
+  // slots of that CSR [S]; FP32-mixed: each slot's position in it [S] (the
+  // partial sums are stored there, node-major)
   int ncl = 0;
   long nslot = 0;
   int *cl_eoff = nullptr, *cl_sbase = nullptr;
   uint16_t *cl_loff = nullptr, *cl_ent = nullptr;
-  int *nsl_off = nullptr, *nsl = nullptr;
+  int *nsl_off = nullptr, *nsl = nullptr, *spos = nullptr;
   // slot -> node [S]; each stored element's nodes as slot indices of its cluster [E][n]
   int *cl_snode = nullptr;
   uint16_t *cl_lslot = nullptr;
