@@ -118,11 +118,14 @@ __global__ __launch_bounds__(BS) void f2_nodal(int N, int nodxelem, const int *_
 #pragma unroll
     for (int j = 0; j < kChunk; ++j)
       e[j] = (k == 0) ? (pad[(long)j * N + i] >> 3) : ((k + j < c) ? (nodel_pk[o + k + j] >> 3) : -1);
+    // every load is issued (entries past the count read the chunk's first
+    // element) and only the node's own entries are added: conditional loads
+    // would each be waited for on their own
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) vv[j] = (e[j] >= 0) ? (double)vol[e[j]] : 0.0;
+    for (int j = 0; j < kChunk; ++j) vv[j] = (double)vol[e[j] >= 0 ? e[j] : e[0]];
     if (k == 0 && rho_u <= 0.0) {
 #pragma unroll
-      for (int j = 0; j < kChunk; ++j) r0[j] = (e[j] >= 0) ? (double)rho[e[j]] : 0.0;
+      for (int j = 0; j < kChunk; ++j) r0[j] = (double)rho[e[j] >= 0 ? e[j] : e[0]];
     }
 #pragma unroll
     for (int j = 0; j < kChunk; ++j)
@@ -1198,24 +1201,27 @@ __device__ __forceinline__ double node_update(
 // the slot ids of the node's list, FP32-mixed directly from its offset (the
 // element pass stores those sums node-major).
 template <typename T, bool TH = false>
-__device__ __forceinline__ void gather_slots(int i, const int *__restrict__ nsl_off, const int *__restrict__ nsl,
-                                             const T *__restrict__ part, long S, double (&f)[TH ? 4 : 3]) {
+__device__ __forceinline__ void gather_slots(int o, int c, const int *__restrict__ nsl, const T *__restrict__ part,
+                                             long S, double (&f)[TH ? 4 : 3]) {
   constexpr int R = TH ? 4 : 3;
-  const int o = nsl_off[i], c = nsl_off[i + 1] - o;
 #pragma unroll
   for (int d = 0; d < R; ++d) f[d] = 0.0;
+  // every load is issued (indices past the node's count repeat its last
+  // slot) and only the node's own entries are added: conditional loads would
+  // each be waited for on their own
   for (int k = 0; k < c; k += 4) {
     int sl[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      if constexpr (kNodeMajor<T>) sl[j] = (k + j < c) ? o + k + j : -1;
-      else sl[j] = (k + j < c) ? nsl[o + k + j] : -1;
+      const int kk = min(k + j, c - 1);
+      if constexpr (kNodeMajor<T>) sl[j] = o + kk;
+      else sl[j] = nsl[o + kk];
     }
     T q[4][R];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int d = 0; d < R; ++d) q[j][d] = (sl[j] >= 0) ? part[d * S + sl[j]] : T(0);
+      for (int d = 0; d < R; ++d) q[j][d] = part[d * S + sl[j]];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
       if (k + j < c)
@@ -1241,10 +1247,13 @@ __global__ __launch_bounds__(BS) void f4_node(
   if constexpr (CFL) set_step_length(q, *q.dtp);
   double speed = 0.0;
   if (i < N) {
+    // the node's slot range first: the partial sums wait for it, not for
+    // the state loads behind it
+    const int o = nsl_off[i], c = nsl_off[i + 1] - o;
     NodeIn<T, CONTACT> st;
     load_node<T, CONTACT>(i, N, q, cf, fe, mdiag, bc_slot, v, prev_a, u_dt, x, u, st);
     double f[TH ? 4 : 3];
-    gather_slots<T, TH>(i, nsl_off, nsl, part, S, f);
+    gather_slots<T, TH>(o, c, nsl, part, S, f);
     speed = node_update<T, CFL, CONTACT>(i, N, q, *reinterpret_cast<double(*)[3]>(f), st, bc_mask, bc_val, fi, v,
                                          prev_a, u_dt, x, u, status);
     if constexpr (TH) Tn[i] += f[3] / st.m * q.dt;
