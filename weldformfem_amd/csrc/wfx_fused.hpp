@@ -139,12 +139,15 @@ __global__ __launch_bounds__(BS) void f2_nodal(int N, int nodxelem, const int *_
   double vn = s;
   vn /= nodxelem;
   double dc = (double)c, mass = 0.0;
+  if (rho_u > 0.0) {  // every term is the same value: formed once, added c times
+    const double term = 1.0 * rho_u * vn / dc;
+    for (int k = 0; k < c; ++k) mass += term;
+    mdiag[i] = mass;
+    return;
+  }
   for (int k = 0; k < c; k += kChunk) {
     double rr[kChunk];
-    if (rho_u > 0.0) {
-#pragma unroll
-      for (int j = 0; j < kChunk; ++j) rr[j] = rho_u;
-    } else if (k == 0) {
+    if (k == 0) {
 #pragma unroll
       for (int j = 0; j < kChunk; ++j) rr[j] = r0[j];
     } else {
@@ -236,15 +239,6 @@ __device__ __forceinline__ void hexa_modes(const T (&y)[8], T (&m)[8]) {
   m[3] = d3 + d1;
   m[7] = d3 - d1;
 }
-// Only the linear modes m[1], m[2], m[4] (the Jacobian rows times 8)
-template <typename T>
-__device__ __forceinline__ void hexa_linear_modes(const T (&y)[8], T &m1, T &m2, T &m4) {
-  const T a0 = y[1] + y[0], b0 = y[1] - y[0], a1 = y[2] + y[3], b1 = y[2] - y[3];
-  const T a2 = y[5] + y[4], b2 = y[5] - y[4], a3 = y[6] + y[7], b3 = y[6] - y[7];
-  m1 = (b3 + b2) + (b1 + b0);
-  m2 = (a3 - a2) + (a1 - a0);
-  m4 = (a3 + a2) - (a1 + a0);
-}
 // The inverse: nodal values z[a] = sum_k q[k] prod_{i in k} s_i(a) for
 // mode coefficients q[1..7] (q[0] = 0), the same three stages in reverse.
 template <typename T>
@@ -264,21 +258,31 @@ __device__ __forceinline__ void hexa_synthesis(const T (&q)[8], T (&z)[8]) {
 }
 
 // Cofactors C (adj(J) = C^T, Matrix.h:694-727) and the Sarrus determinant
-// (Matrix.h:610-615) of J.
+// (Matrix.h:610-615) of J.  FP64 without contraction: the same roundings as
+// the per-step Jacobian kernel, so that an element's volume at rest equals
+// its initial volume bit for bit (the ANP ratio is then exactly 1 there).
+#define WFX_COFACTORS(J, C, detJ)                                                                         \
+  C[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]);                                                        \
+  C[0][1] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]);                                                       \
+  C[0][2] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]);                                                        \
+  C[1][0] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]);                                                       \
+  C[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]);                                                        \
+  C[1][2] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]);                                                       \
+  C[2][0] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]);                                                        \
+  C[2][1] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]);                                                       \
+  C[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]);                                                        \
+  detJ = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[1][2] * J[2][1] - J[0][1] * J[1][0] * J[2][2] +          \
+         J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] - J[0][2] * J[1][1] * J[2][0];
 template <typename T>
 __device__ __forceinline__ void cofactors(const T (&J)[3][3], T (&C)[3][3], T &detJ) {
-  C[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
-  C[0][1] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]);
-  C[0][2] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-  C[1][0] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]);
-  C[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
-  C[1][2] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]);
-  C[2][0] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
-  C[2][1] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]);
-  C[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
-  detJ = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[1][2] * J[2][1] - J[0][1] * J[1][0] * J[2][2] +
-         J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] - J[0][2] * J[1][1] * J[2][0];
+  if constexpr (std::is_same<T, double>::value) {
+#pragma clang fp contract(off)
+    WFX_COFACTORS(J, C, detJ)
+  } else {
+    WFX_COFACTORS(J, C, detJ)
+  }
 }
+#undef WFX_COFACTORS
 
 // pow(vol, 0.6666666) of the hourglass coefficient (Mechanical.f90:299): the
 // FP64 path in double, FP32-mixed with the hardware base-2 logarithm and
@@ -306,6 +310,13 @@ struct SlotLds {
         node(reinterpret_cast<int *>(base + (3 * 8 + 4 * (int)sizeof(T)) * cap)) {}
 };
 
+// Slots each thread of a cluster pass stages (slot_cap = kSlotsPerThread *
+// kCluster): the staging rows past the cluster's slot count are written with
+// a repeat of its last slot and never read, so that no load or store of the
+// staging waits behind a branch
+template <int NN>
+constexpr int kSlotsPerThread = slot_cap(NN) / kCluster;
+
 // What the first part of the element chain leaves for the second: geometry,
 // strain rate and spin, hourglass amplitudes, the nodal ANP average and,
 // with thermal coupling, the element's temperature terms.
@@ -321,26 +332,54 @@ struct ElemGeo {
   double tsum, gT0, gT1, gT2;
 };
 
-// The Jacobian of an element from its staged positions (rows J(i,d) =
-// 0.125 m_i(x_d) for the hexahedron, x_{i+1} - x_0 for the tetrahedron),
-// formed in double from the positions and narrowed to T
+// The Jacobian of an element from its staged positions, as the per-step
+// Jacobian kernel forms it (calcElemJAndDerivatives, Domain_d.C:1822-1889):
+// hexahedron rows 0.125 * (the signed sum of the nodes in the reference's
+// order), tetrahedron rows x_{i+1} - x_0; FP32-mixed from FP64 differences to
+// node 0 narrowed to float.  FP64 without contraction, so that the volume
+// at rest equals the initial volume bit for bit.
 template <typename T, int NN>
 __device__ __forceinline__ void staged_jacobian(const double *lx, int cap, const int (&ls)[NN], T (&J)[3][3]) {
+  T X[NN][3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const double *xc = lx + c * cap;
-    if constexpr (NN == 8) {
-      double y[8], m1, m2, m4;
+    if constexpr (std::is_same<T, double>::value) {
 #pragma unroll
-      for (int a = 0; a < 8; ++a) y[a] = xc[ls[a]];
-      hexa_linear_modes(y, m1, m2, m4);
-      J[0][c] = T(0.125 * m1);
-      J[1][c] = T(0.125 * m2);
-      J[2][c] = T(0.125 * m4);
+      for (int a = 0; a < NN; ++a) X[a][c] = xc[ls[a]];
     } else {
       const double x0 = xc[ls[0]];
+      X[0][c] = T(0);
 #pragma unroll
-      for (int i = 0; i < 3; ++i) J[i][c] = T(xc[ls[i + 1]] - x0);
+      for (int a = 1; a < NN; ++a) X[a][c] = T(xc[ls[a]] - x0);
+    }
+  }
+  if constexpr (std::is_same<T, double>::value) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if constexpr (NN == 8) {
+        J[0][c] = T(0.125) * (-X[0][c] + X[1][c] + X[2][c] - X[3][c] - X[4][c] + X[5][c] + X[6][c] - X[7][c]);
+        J[1][c] = T(0.125) * (-X[0][c] - X[1][c] + X[2][c] + X[3][c] - X[4][c] - X[5][c] + X[6][c] + X[7][c]);
+        J[2][c] = T(0.125) * (-X[0][c] - X[1][c] - X[2][c] - X[3][c] + X[4][c] + X[5][c] + X[6][c] + X[7][c]);
+      } else {
+        J[0][c] = X[1][c] - X[0][c];
+        J[1][c] = X[2][c] - X[0][c];
+        J[2][c] = X[3][c] - X[0][c];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if constexpr (NN == 8) {
+        J[0][c] = T(0.125) * (-X[0][c] + X[1][c] + X[2][c] - X[3][c] - X[4][c] + X[5][c] + X[6][c] - X[7][c]);
+        J[1][c] = T(0.125) * (-X[0][c] - X[1][c] + X[2][c] + X[3][c] - X[4][c] - X[5][c] + X[6][c] + X[7][c]);
+        J[2][c] = T(0.125) * (-X[0][c] - X[1][c] - X[2][c] - X[3][c] + X[4][c] + X[5][c] + X[6][c] + X[7][c]);
+      } else {
+        J[0][c] = X[1][c] - X[0][c];
+        J[1][c] = X[2][c] - X[0][c];
+        J[2][c] = X[3][c] - X[0][c];
+      }
     }
   }
 }
@@ -786,13 +825,17 @@ __global__ __launch_bounds__(kCluster) void f3_cluster(
   const uint16_t *lo = ca.loff + s0 + c;
   // the nodes of this thread's slots t and t + kCluster: first, as the
   // staging loads depend on them
-  static_assert(slot_cap(NN) <= 2 * kCluster, "two slots per thread");
-  int sn[2];
+  constexpr int kSl = kSlotsPerThread<NN>;
+  static_assert(kSl * kCluster == slot_cap(NN) && kSl <= 2, "one or two slots per thread");
+  // Every load below is issued unconditionally, threads past the cluster's
+  // elements, entries or slots repeating its last one (a cluster has at
+  // least one element and NN slots), so that none is waited for on its own;
+  // the values are used only where they are the thread's own.
+  // First the nodes of this thread's slots t and t + kCluster, which the
+  // staging loads depend on.
+  int sn[kSl];
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int j = t + k * kCluster;
-    sn[k] = j < M ? ca.snode[s0 + j] : -1;
-  }
+  for (int k = 0; k < kSl; ++k) sn[k] = ca.snode[s0 + min(t + k * kCluster, M - 1)];
   // The cluster's slot tables are loaded into registers before the element
   // chain, so that the loads complete under its arithmetic: the entries
   // (ne * NN uint16 from a 4-byte aligned start, since e0 * NN is even) go to
@@ -802,52 +845,47 @@ __global__ __launch_bounds__(kCluster) void f3_cluster(
   const int nw = ne * NN / 2;
   uint32_t ew[NN / 2];
 #pragma unroll
-  for (int k = 0; k < NN / 2; ++k) {
-    const int w = t + k * kCluster;
-    ew[k] = w < nw ? en32[w] : 0u;
-  }
-  uint32_t ow[2];  // (end << 16) | start of slots t and t + kCluster
-  int sp[2];       //   and where their sums go
+  for (int k = 0; k < NN / 2; ++k) ew[k] = en32[min(t + k * kCluster, nw - 1)];
+  uint32_t ow[kSl];  // (end << 16) | start of slots t and t + kCluster
+  int sp[kSl];       //   and where their sums go
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int q = t + k * kCluster;
-    ow[k] = q < M ? ((uint32_t)lo[q + 1] << 16) | lo[q] : 0u;
-    sp[k] = (kNodeMajor<T> && q < M) ? ca.spos[s0 + q] : s0 + q;
+  for (int k = 0; k < kSl; ++k) {
+    const int q = min(t + k * kCluster, M - 1);
+    ow[k] = ((uint32_t)lo[q + 1] << 16) | lo[q];
+    if constexpr (kNodeMajor<T>) sp[k] = ca.spos[s0 + q];
+    else sp[k] = s0 + q;
   }
   int ls[NN];
   ElemState<T> S;
-  if (t < ne) {
-    load_lslot<NN>(ca.lslot + (long)(e0 + t) * NN, ls);
-    load_elem_state<T>(e0 + t, E, tau_io, eps_io, pl_strain, sy0, He, S);
+  {
+    const int e = e0 + min(t, ne - 1);
+    load_lslot<NN>(ca.lslot + (long)e * NN, ls);
+    load_elem_state<T>(e, E, tau_io, eps_io, pl_strain, sy0, He, S);
   }
   // stage the slots: all loads first, then the LDS stores
-  double sx[2][3];
-  T sv[2][3], sr[2];
+  double sx[kSl][3];
+  T sv[kSl][3], sr[kSl];
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < kSl; ++k) {
     const int node = sn[k];
-    if (node >= 0) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        sx[k][d] = x[(long)d * N + node];
-        sv[k][d] = v[(long)d * N + node];
-      }
-      if constexpr (std::is_same<T, double>::value) sr[k] = ratio[node];
-      else sr[k] = reinterpret_cast<const float *>(ratio)[node];
+    for (int d = 0; d < 3; ++d) {
+      sx[k][d] = x[(long)d * N + node];
+      sv[k][d] = v[(long)d * N + node];
     }
+    if constexpr (std::is_same<T, double>::value) sr[k] = ratio[node];
+    else sr[k] = reinterpret_cast<const float *>(ratio)[node];
   }
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int j = t + k * kCluster;
-    if (sn[k] >= 0) {
-      L.node[j] = sn[k];
+  for (int k = 0; k < kSl; ++k) {
+    const int j = t + k * kCluster;  // < slot_cap
+    L.node[j] = sn[k];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        L.x[d * L.cap + j] = sx[k][d];
-        L.v[d * L.cap + j] = sv[k][d];
-      }
-      L.r[j] = sr[k];
+    for (int d = 0; d < 3; ++d) {
+      L.x[d * L.cap + j] = sx[k][d];
+      L.v[d * L.cap + j] = sv[k][d];
     }
+    L.r[j] = sr[k];
   }
   __syncthreads();
   ElemGeo<T, NN, TH> G;
@@ -877,7 +915,7 @@ __global__ __launch_bounds__(kCluster) void f3_cluster(
   __syncthreads();
   const uint16_t *en = reinterpret_cast<const uint16_t *>(l_ent);
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {  // M <= 2 * kCluster
+  for (int r = 0; r < kSl; ++r) {  // M <= kSl * kCluster
     const int j = t + r * kCluster;
     if (j >= M) break;
     const int k0 = ow[r] & 0xffffu, k1 = ow[r] >> 16;
@@ -900,12 +938,12 @@ __global__ __launch_bounds__(kCluster) void f3_cluster(
 
 // F1 as a cluster pass: one work-group per element cluster stages its
 // slots' positions (and node ids when thermal expansion reads them) in LDS,
-// then each thread forms its element's Jacobian from LDS: vol = detJ * w
-// (calcElemJAndDerivatives determinant + CalcElemVol), the inverted-element
-// count, with minl_part the work-group's smallest edge (per-step CFL time
-// step), with texp each element's mean of the last step's element-node
-// temperature rates at the flat indices of its node ids (calcThermalExpansion's
-// dTdt).  The same Jacobian as the element pass's.
+// then each thread forms its element's Jacobian from LDS (the same Jacobian
+// as the element pass's): vol = detJ * w (calcElemJAndDerivatives
+// determinant + CalcElemVol), the inverted-element count, with minl_part the
+// work-group's smallest edge (per-step CFL time step), with texp each
+// element's mean of the last step's element-node temperature rates at the
+// flat indices of its node ids (calcThermalExpansion's dTdt).
 template <typename T, int NN>
 __global__ __launch_bounds__(kCluster) void f1_cluster(int E, ClusterArgs ca, const double *__restrict__ x, int N,
                                                        T *__restrict__ vol, double *__restrict__ minl_part,
@@ -917,28 +955,25 @@ __global__ __launch_bounds__(kCluster) void f1_cluster(int E, ClusterArgs ca, co
   const int c = xcd_block(), t = threadIdx.x;
   const int e0 = ca.eoff[c], ne = ca.eoff[c + 1] - e0;
   const int s0 = ca.sbase[c], M = ca.sbase[c + 1] - s0;  // M <= slot_cap(NN), checked at set-up
-  int sn[2];
+  // every load is issued unconditionally (threads past the cluster's
+  // elements or slots repeat its last one), as in the element pass
+  constexpr int kSl = kSlotsPerThread<NN>;
+  int sn[kSl];
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int j = t + k * kCluster;
-    sn[k] = j < M ? ca.snode[s0 + j] : -1;
-  }
+  for (int k = 0; k < kSl; ++k) sn[k] = ca.snode[s0 + min(t + k * kCluster, M - 1)];
   int ls[NN];
-  if (t < ne) load_lslot<NN>(ca.lslot + (long)(e0 + t) * NN, ls);
-  double sx[2][3];
+  load_lslot<NN>(ca.lslot + (long)(e0 + min(t, ne - 1)) * NN, ls);
+  double sx[kSl][3];
 #pragma unroll
-  for (int k = 0; k < 2; ++k)
-    if (sn[k] >= 0)
+  for (int k = 0; k < kSl; ++k)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) sx[k][d] = x[(long)d * N + sn[k]];
+    for (int d = 0; d < 3; ++d) sx[k][d] = x[(long)d * N + sn[k]];
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int j = t + k * kCluster;
-    if (sn[k] >= 0) {
+  for (int k = 0; k < kSl; ++k) {
+    const int j = t + k * kCluster;  // < slot_cap
 #pragma unroll
-      for (int d = 0; d < 3; ++d) lx[d * cap + j] = sx[k][d];
-      if (texp) lnode[j] = sn[k];
-    }
+    for (int d = 0; d < 3; ++d) lx[d * cap + j] = sx[k][d];
+    lnode[j] = sn[k];
   }
   __syncthreads();
   double medge = kMinEdgeInit;
@@ -954,7 +989,7 @@ __global__ __launch_bounds__(kCluster) void f1_cluster(int E, ClusterArgs ca, co
     staged_jacobian<T, NN>(lx, cap, ls, J);
     cofactors(J, C, det);
     const T w = (NN == 4) ? T(1.0 / 6.0) : T(8.0);
-    vol[e] = det * w;
+    vol[e] = det * w;  // CalcElemVol's 0 + detJ * w * 1 (the same value)
     count_inverted(det, status);
     if (minl_part) {  // the first four nodes, as calcMinEdgeLength takes them
       T P[4][3];
