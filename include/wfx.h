@@ -142,6 +142,13 @@ int wfx_rigid_update(wfx_ctx *ctx, wfx_stream stream);
 /* m_contPF ("penaltyFactor", default 0.1), fricCoeffStatic, fricCoeffDynamic
  * (main.C:716-727) */
 int wfx_set_contact_params(wfx_ctx *ctx, double penalty_factor, double mu_static, double mu_dynamic);
+/* Contact heat conduction: heatCondCoeff and dieTemp of the contact block
+ * (main.C:714-719, TriMesh_d::heat_cond and T_const, defaults 0 and 20).
+ * With thermal coupling every node in contact takes
+ * q = heat_cond * node_area * (die_temp - T) (Contact.C:309), kept until its
+ * next contact, and the nodal temperature update adds q / (m cp_T)
+ * (Thermal.C:124). */
+int wfx_set_contact_heat(wfx_ctx *ctx, double heat_cond, double die_temp);
 /* TriMesh_d::node_v: new rigid-node velocities [n][3] (the reference's start-up
  * ramp of the die velocity, Solver_explicit.C:981-992, is a driver's choice) */
 int wfx_set_rigid_velocity(wfx_ctx *ctx, const double *node_vel);

@@ -84,6 +84,8 @@ class Domain_d {
   void setContactPF(double pf, double mu_static = 0.0, double mu_dynamic = 0.0) {
     check(wfx_set_contact_params(c_, pf, mu_static, mu_dynamic));
   }
+  // heatCondCoeff and dieTemp of the contact block (main.C:718-719)
+  void setContactHeat(double heat_cond, double die_temp) { check(wfx_set_contact_heat(c_, heat_cond, die_temp)); }
   double contactForce(int mesh_id = 0) { double f; check(wfx_contact_force(c_, mesh_id, &f)); return f; }
   void CalcExtFaceAreas() { check(wfx_calc_ext_face_areas(c_, s_)); }
   void CalcContactForces() { check(wfx_calc_contact_forces(c_, s_)); }

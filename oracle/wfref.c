@@ -104,6 +104,7 @@ wfref_domain *wfref_create(void) {
   wfref_set_ch_default(d);
   d->max_edot = 1.0e6; /* Domain_d.h:824 */
   d->contPF = 0.1;     /* Domain_d.h:256 */
+  d->T_const = 20.0;   /* die temperature before "dieTemp" is read (main.C:714) */
   /* m_artifvisc is left uninitialised by the C++ API drivers; zero here */
   return d;
 }
@@ -115,7 +116,7 @@ static void free_fields(wfref_domain *d) {
                    &d->pl_strain, &d->sigma_y, &d->sy0, &d->str_rate,
                    &d->rot_rate, &d->sigma, &d->tau, &d->eps,
                    &d->strain_pl_incr, &d->f_elem, &d->f_hg, &d->temp, &d->T,
-                   &d->q_plheat, &d->dTedt, &d->node_area, &d->elem_length,
+                   &d->q_plheat, &d->dTedt, &d->q_cont_conv, &d->node_area, &d->elem_length,
                    &d->contforce, &d->ut_prev, &d->cfn};
   int **ip[] = {&d->elnod, &d->nodel, &d->nodel_loc, &d->nodel_offset,
                 &d->nodel_count, &d->ext_face, &d->ext_node, &d->mesh_in_contact};
@@ -157,7 +158,7 @@ static void set_dimension(wfref_domain *d, int node_count, int elem_count) {
   RD(d->str_rate, 6 * E); RD(d->rot_rate, 6 * E); RD(d->sigma, 6 * E);
   RD(d->tau, 6 * E); RD(d->eps, 6 * E); RD(d->strain_pl_incr, 6 * E);
   RD(d->f_elem, 3 * n * E); RD(d->f_hg, 3 * n * E); RD(d->temp, E);
-  RD(d->T, N); RD(d->q_plheat, E); RD(d->dTedt, n * E);
+  RD(d->T, N); RD(d->q_plheat, E); RD(d->dTedt, n * E); RD(d->q_cont_conv, N);
   RD(d->node_area, N); RD(d->elem_length, E); RD(d->contforce, 3 * N); RD(d->ut_prev, 3 * N);
   RD(d->cfn, N); RI(d->ext_node, N); RI(d->mesh_in_contact, N);
   free(d->ext_face); d->ext_face = NULL; d->nf = 0; d->contact = 0;
@@ -989,10 +990,11 @@ void wfref_set_thermal(wfref_domain *d, int on, double k_T, double cp_T, double 
   d->plheat_frac = plheat_frac;
 }
 
-/* ThermalCalcs (Thermal.C:27-115), contact conduction absent: element
- * conduction Kt(i,j) = (sum_d dH(d,i) dH(d,j)) k_T / detJ * w on this step's
- * Jacobians, element-node rates -(Kt T)_i / m_i + 0.9 q_plheat vol / n /
- * (m_i cp_T), summed per node in CSR order, T += dTdt * dt. */
+/* ThermalCalcs (Thermal.C:27-126): element conduction Kt(i,j) =
+ * (sum_d dH(d,i) dH(d,j)) k_T / detJ * w on this step's Jacobians,
+ * element-node rates -(Kt T)_i / m_i + 0.9 q_plheat vol / n / (m_i cp_T),
+ * summed per node in CSR order, T += (dTdt + q_cont_conv / (m cp_T)) * dt
+ * with the contact conduction of the last contact step (Thermal.C:124). */
 void wfref_thermal_calcs(wfref_domain *d) {
   int n = d->nodxelem, E = d->elem_count, N = d->node_count;
   double w = (n == 4) ? 1.0 / 6.0 : 8.0;
@@ -1031,7 +1033,7 @@ void wfref_thermal_calcs(wfref_domain *d) {
       int o = d->nodel_offset[i] + k;
       dTdt += d->dTedt[d->nodel[o] * n + d->nodel_loc[o]];
     }
-    d->T[i] += dTdt * d->dt;
+    d->T[i] += (dTdt + d->q_cont_conv[i] * 1.0 / (d->mdiag[i] * d->cp_T)) * d->dt;
   }
 }
 
@@ -1292,6 +1294,8 @@ void wfref_calc_contact_forces(wfref_domain *d) {
         ut[0] = ut[1] = ut[2] = 0.0;
       }
       cfo[0] += Ft.x; cfo[1] += Ft.y; cfo[2] += Ft.z;
+      /* contact heat conduction (Contact.C:309) */
+      d->q_cont_conv[i] = d->heat_cond * d->node_area[i] * (d->T_const - d->T[i]);
       break; /* one master element per slave node */
     }
   }
@@ -1381,6 +1385,7 @@ void *wfref_field(wfref_domain *d, const char *name, long *count, int *is_int) {
       {"node_area", d->node_area, N, 0}, {"elem_length", d->elem_length, E, 0},
       {"contforce", d->contforce, 3 * N, 0}, {"ut_prev", d->ut_prev, 3 * N, 0},
       {"cfn", d->cfn, N, 0}, {"mesh_in_contact", d->mesh_in_contact, N, 1},
+      {"q_cont_conv", d->q_cont_conv, N, 0},
       {"ext_node", d->ext_node, N, 1}, {"rigid_x", d->rnode, 3 * (long)d->rn, 0},
       {"rigid_v", d->rvel, 3 * (long)d->rn, 0}, {"rigid_normal", d->rnormal, 3 * (long)d->rt, 0},
       {"rigid_pplane", d->rpplane, d->rt, 0},
@@ -1430,7 +1435,8 @@ int wfref_set_double(wfref_domain *d, const char *name, double v) {
       {"av_coeff_div", &d->av_coeff_div}, {"av_coeff_bulk", &d->av_coeff_bulk},
       {"artifvisc0", &d->artifvisc[0]}, {"artifvisc1", &d->artifvisc[1]},
       {"hg_coeff", &d->hg_coeff}, {"max_edot", &d->max_edot}, {"cfl", &d->cfl},
-      {"contPF", &d->contPF}, {"mu_sta", &d->mu_sta}, {"mu_dyn", &d->mu_dyn}};
+      {"contPF", &d->contPF}, {"mu_sta", &d->mu_sta}, {"mu_dyn", &d->mu_dyn},
+      {"heat_cond", &d->heat_cond}, {"T_const", &d->T_const}};
   for (size_t k = 0; k < sizeof(t) / sizeof(t[0]); ++k)
     if (strcmp(t[k].nm, name) == 0) { *t[k].p = v; return 0; }
   return -1;

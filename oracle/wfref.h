@@ -83,6 +83,11 @@ typedef struct {
   int *ext_node;                   /* N: 1 on an external face */
   double *node_area, *elem_length; /* N, E */
   double *contforce, *ut_prev;     /* 3N, persistent */
+  /* contact heat conduction (heatCondCoeff, dieTemp: main.C:718-719;
+   * TriMesh_d::heat_cond, T_const): q_cont_conv of the nodes in contact,
+   * kept by a node that leaves contact */
+  double heat_cond, T_const;
+  double *q_cont_conv;             /* N, persistent */
   int *mesh_in_contact;            /* N */
   double *cfn;                     /* N */
   long step_count;

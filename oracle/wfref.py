@@ -192,6 +192,11 @@ class RefDomain:
         self.set("mu_sta", mu_static)
         self.set("mu_dyn", mu_dynamic)
 
+    def setContactHeat(self, heat_cond, die_temp):
+        """contact heat conduction: heatCondCoeff and dieTemp (main.C:718-719)"""
+        self.set("heat_cond", heat_cond)
+        self.set("T_const", die_temp)
+
     def CalcExtFaceAreas(self):
         self._L.wfref_calc_ext_face_areas(self._d)
 

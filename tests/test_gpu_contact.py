@@ -5,7 +5,9 @@ tetrahedral cylinder compressed by a descending rigid plane with friction:
 per-step API and fused path over 150 steps (15 refreshes of the external
 face areas, runs split off the 10-step rhythm), with the per-step CFL time
 step, FP32-mixed, and captured graphs.  FP64 bar 1e-10 relative on u, v,
-the contact forces and the reaction; FP32-mixed 1e-3.
+the contact forces and the reaction; FP32-mixed 1e-3.  Contact heat
+conduction (a hot die heating the nodes it touches) on both paths: the
+nodal conduction terms and temperatures at 1e-10 against the oracle.
 """
 import numpy as np
 import pytest
@@ -201,3 +203,27 @@ def test_second_surface_replaces_the_first():
         e = rel(g.field(f), r.field(f))
         assert e <= 1e-10, f"{f}: {e:.3e}"
     np.testing.assert_allclose(g.field("rigid_x"), r.field("rigid_x"), rtol=1e-14)
+
+
+@pytest.mark.parametrize("path", ["api", "fused"])
+def test_contact_heat_conduction_fp64(path):
+    # a die at 900 C pressed onto a body at 20 C: the nodes it touches take
+    # heat_cond * area * (T_die - T) and warm up; conduction inside the body
+    # spreads it (Contact.C:309, Thermal.C:124)
+    say(f"case: contact heat conduction, {path} FP64 vs oracle, 100 steps")
+    g, r = _pair()
+    for d in (g, r):
+        d.setThermal(True, 50.0, 460.0, 0.9)
+        d.setContactHeat(2.0e6, 900.0)
+        d.init()
+    hg = g.run(100) if path == "api" else np.concatenate([g.runFused(41), g.runFused(59)])
+    hr = r.run(100)
+    check(g, r, hg, hr, 1e-10)
+    q = r.field("q_cont_conv")
+    assert np.sum(q > 0) > 50
+    e = rel(g.field("q_cont_conv"), q)
+    assert e <= 1e-10, f"q_cont_conv: {e:.3e}"
+    dT = r.field("T") - 20.0
+    assert np.max(dT) > 1e-3
+    e = rel(g.field("T") - 20.0, dT)
+    assert e <= 1e-10, f"T: {e:.3e}"

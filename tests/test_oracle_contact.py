@@ -6,7 +6,9 @@ against 3V/A, rigid-plane normals and offsets, and the contact force of one
 node: no force above a plane or outside a triangle, the mass-limited and
 geometry-limited penalty stiffness, critical damping ratio 0.2, stick and
 slip friction, the first-triangle rule and the persistence of a force after
-separation.  No GPU.
+separation; contact heat conduction q = h A (T_die - T) of the nodes in
+contact, its place in the nodal temperature update (the heat q dt entering
+as m cp dT) and its persistence.  No GPU.
 """
 import math
 
@@ -180,3 +182,44 @@ def test_force_persists_after_separation():
     np.testing.assert_array_equal(d.field("contforce"), before)
     assert d.field("mesh_in_contact")[k] == -1
     assert d.contactForce(0) == 0.0
+
+
+def test_contact_heat_conduction():
+    dt, h, Tdie, cp = 1e-7, 5e4, 900.0, 460.0
+    d = body(1, 1, 1, 0.1, dt=dt)
+    d.setThermal(True, 0.0, cp, 0.9)  # no conduction inside the body, no plastic heat yet
+    x = d.field("x").reshape(-1, 3)
+    top = np.where(np.isclose(x[:, 2], 0.1))[0]
+    k = top[np.argmax(x[top, 0] + x[top, 1])]
+    n, t = plane(0.1 - 1e-4, flip=True)
+    d.setRigidMesh(n, t)
+    d.setContactHeat(h, Tdie)
+    d.init()
+    xx = x.copy()
+    xx[[i for i in top if i != k], 2] = 0.1 - 2e-4
+    d.setField("x", xx.reshape(-1))
+    T0 = np.full(d.node_count, 20.0)
+    T0[k] = 35.0
+    d.setField("T", T0)
+    d.CalcContactForces()
+    q = d.field("q_cont_conv")
+    area = d.field("node_area")[k]
+    assert area > 0
+    assert q[k] == pytest.approx(h * area * (Tdie - 35.0), rel=1e-15)
+    assert np.all(np.delete(q, k) == 0.0)
+    # the nodal update adds q dt / (m cp): the heat q dt enters as m cp dT
+    m = d.field("mdiag")
+    d.ThermalCalcs()
+    dT = d.field("T") - T0
+    assert dT[k] == pytest.approx(q[k] * dt / (m[k] * cp), rel=1e-12)
+    assert np.all(np.delete(dT, k) == 0.0)
+    assert float(np.sum(m * cp * dT)) == pytest.approx(q[k] * dt, rel=1e-12)
+    # a die colder than the node cools it; the flux stays after separation
+    d.setContactHeat(h, 0.0)
+    d.CalcContactForces()
+    q2 = d.field("q_cont_conv")[k]
+    assert q2 < 0
+    xx[k, 2] = 0.0
+    d.setField("x", xx.reshape(-1))
+    d.CalcContactForces()
+    assert d.field("mesh_in_contact")[k] == -1 and d.field("q_cont_conv")[k] == q2

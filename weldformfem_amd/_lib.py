@@ -41,6 +41,7 @@ SIGNATURES = {
     "wfx_set_thermal": (_i, [_vp, _i, _d, _d, _d]),
     "wfx_set_rigid_mesh": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp]),
     "wfx_set_contact_params": (_i, [_vp, _d, _d, _d]),
+    "wfx_set_contact_heat": (_i, [_vp, _d, _d]),
     "wfx_set_rigid_velocity": (_i, [_vp, _vp]),
     "wfx_contact_force": (_i, [_vp, _i, _vp]),
     "wfx_calc_ext_face_areas": (_i, [_vp, _vp]),

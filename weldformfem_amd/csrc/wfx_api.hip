@@ -369,6 +369,7 @@ static bool find_field(wfx_ctx *c, const char *name, FieldDesc &f) {
     node3(s == "contforce" ? c->contact.contforce : c->contact.ut_prev);
   else if (c->contact.on && s == "node_area") node1(c->contact.node_area);
   else if (c->contact.on && s == "cfn") node1(c->contact.cfn);
+  else if (c->contact.on && s == "q_cont_conv") node1(c->contact.q_cont);
   else if (c->contact.on && s == "mesh_in_contact") { f.dev = c->contact.mesh_in_contact; f.cnt = N; f.is_int = 1; }
   else if (c->contact.on && s == "elem_length") { f.dev = c->contact.elem_length; f.cnt = E; f.per_elem = 1; }
   else if (c->contact.on && s == "rigid_x") { f.dev = c->contact.rnode; f.cnt = 3L * c->contact.rn; }
@@ -692,7 +693,7 @@ int wfx_set_rigid_mesh(wfx_ctx *c, int node_count, const double *nodes, const do
   dfree(c, ct.rnode); dfree(c, ct.rvel); dfree(c, ct.rnormal); dfree(c, ct.rpplane); dfree(c, ct.rtri);
   dfree(c, ct.rmesh); dfree(c, ct.ext_nodes); dfree(c, ct.afc_off); dfree(c, ct.afc_face);
   dfree(c, ct.ext_face); dfree(c, ct.node_area); dfree(c, ct.elem_length); dfree(c, ct.contforce);
-  dfree(c, ct.ut_prev); dfree(c, ct.mesh_in_contact); dfree(c, ct.cfn);
+  dfree(c, ct.ut_prev); dfree(c, ct.mesh_in_contact); dfree(c, ct.cfn); dfree(c, ct.q_cont);
   ct.rn = node_count;
   ct.rt = tri_count;
   ct.next = (int)ext.size();
@@ -712,6 +713,7 @@ int wfx_set_rigid_mesh(wfx_ctx *c, int node_count, const double *nodes, const do
   WFX_CHECK_HIP(dalloc(c, &ct.ut_prev, 3 * N));
   WFX_CHECK_HIP(dalloc(c, &ct.mesh_in_contact, N));
   WFX_CHECK_HIP(dalloc(c, &ct.cfn, N));
+  WFX_CHECK_HIP(dalloc(c, &ct.q_cont, N));
   std::vector<double> vel(3 * (size_t)node_count, 0.0);
   if (node_vel) std::copy(node_vel, node_vel + vel.size(), vel.begin());
   std::vector<int> mid((size_t)tri_count, 0);
@@ -730,6 +732,7 @@ int wfx_set_rigid_mesh(wfx_ctx *c, int node_count, const double *nodes, const do
   WFX_CHECK_HIP(hipMemset(ct.contforce, 0, 3 * N * sizeof(double)));
   WFX_CHECK_HIP(hipMemset(ct.ut_prev, 0, 3 * N * sizeof(double)));
   WFX_CHECK_HIP(hipMemset(ct.cfn, 0, N * sizeof(double)));
+  WFX_CHECK_HIP(hipMemset(ct.q_cont, 0, N * sizeof(double)));
   WFX_CHECK_HIP(hipMemset(ct.mesh_in_contact, 0xff, N * sizeof(int)));
   ct.on = 1;
   // normals and plane offsets (a zero-length move), areas of the external faces
@@ -769,6 +772,15 @@ int wfx_set_contact_params(wfx_ctx *c, double penalty_factor, double mu_static, 
   c->contact.pf = penalty_factor;
   c->contact.mu_sta = mu_static;
   c->contact.mu_dyn = mu_dynamic;
+  return WFX_OK;
+}
+
+int wfx_set_contact_heat(wfx_ctx *c, double heat_cond, double die_temp) {
+  REQUIRE_CTX(c);
+  REQUIRE(heat_cond >= 0.0 && std::isfinite(heat_cond) && std::isfinite(die_temp), WFX_ERR_ARG,
+          "the conduction coefficient must be non-negative and finite, the die temperature finite");
+  c->contact.heat_cond = heat_cond;
+  c->contact.T_const = die_temp;
   return WFX_OK;
 }
 
@@ -1547,9 +1559,9 @@ static uint64_t graph_signature(wfx_ctx *c) {
   mix(&c->thermal, sizeof(c->thermal));
   {
     const auto &ct = c->contact;
-    const double cp[3] = {ct.pf, ct.mu_sta, ct.mu_dyn};
+    const double cp[5] = {ct.pf, ct.mu_sta, ct.mu_dyn, ct.heat_cond, ct.T_const};
     const int ci[4] = {ct.on, ct.rn, ct.rt, ct.next};
-    const void *cptr[3] = {ct.rnode, ct.contforce, ct.ext_nodes};
+    const void *cptr[4] = {ct.rnode, ct.contforce, ct.ext_nodes, ct.q_cont};
     mix(cp, sizeof(cp));
     mix(ci, sizeof(ci));
     mix(cptr, sizeof(cptr));

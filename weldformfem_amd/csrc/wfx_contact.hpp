@@ -82,10 +82,15 @@ struct ContactParams {
   const int *rtri, *rmesh;
   double E, pf, mu_sta, mu_dyn, dt;
   const double *dtp;  // per-step CFL time step on the device, or null
+  // contact heat conduction (with thermal coupling; q_cont null: none)
+  double heat_cond, T_const;
+  const double *Tn;
+  double *q_cont;
 };
 
 // CalcContactForces for every external node (v: the predicted velocities,
-// stored in T); nodes out of contact keep their contact force
+// stored in T); nodes out of contact keep their contact force and their
+// contact heat conduction
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_contact(
     int next, const int *__restrict__ ext_nodes, int N, ContactParams cp, const double *__restrict__ x,
@@ -152,6 +157,8 @@ __global__ __launch_bounds__(kBlock) void k_contact(
     contforce[i] = cf.x + Ft.x;
     contforce[N + i] = cf.y + Ft.y;
     contforce[2L * N + i] = cf.z + Ft.z;
+    // contact heat conduction (Contact.C:309), kept by a node that leaves contact
+    if (cp.q_cont) cp.q_cont[i] = cp.heat_cond * node_area[i] * (cp.T_const - cp.Tn[i]);
     break;  // one master element per slave node
   }
 }

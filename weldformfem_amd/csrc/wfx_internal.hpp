@@ -104,6 +104,10 @@ struct Contact {
   int *ext_face = nullptr;                       // [nf][3], node order of the element's face table
   double *node_area = nullptr, *elem_length = nullptr;
   double *contforce = nullptr, *ut_prev = nullptr;  // [3][N], kept between steps
+  // contact heat conduction (heatCondCoeff, dieTemp: main.C:714-719): the
+  // conduction into each node in contact, kept between steps [N]
+  double heat_cond = 0.0, T_const = 20.0;
+  double *q_cont = nullptr;
   int *mesh_in_contact = nullptr;
   double *cfn = nullptr;
 };

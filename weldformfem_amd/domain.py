@@ -138,6 +138,10 @@ class Domain:
     def setContactParams(self, penalty_factor=0.1, mu_static=0.0, mu_dynamic=0.0):
         check(self._L.wfx_set_contact_params(self._h, float(penalty_factor), float(mu_static), float(mu_dynamic)))
 
+    def setContactHeat(self, heat_cond, die_temp):
+        """contact heat conduction: heatCondCoeff and dieTemp (main.C:718-719)"""
+        check(self._L.wfx_set_contact_heat(self._h, float(heat_cond), float(die_temp)))
+
     def setRigidVelocity(self, vel, node_count):
         v = np.ascontiguousarray(np.broadcast_to(np.asarray(vel, dtype=np.float64), (int(node_count), 3))).reshape(-1)
         check(self._L.wfx_set_rigid_velocity(self._h, v.ctypes.data))
