@@ -685,7 +685,7 @@ def pmc_summary(fetch_csv, write_csv, probe_log):
              "Per-dispatch values: median of each kernel's last three dispatches (past first yield).", "",
              "| kernel | reads MB | writes MB | HBM MB | model MB | HBM / model |", "|---|---:|---:|---:|---:|---:|"]
     out = {}
-    kernels = {"f1_volume": "f1_volume<", "f2_nodal": "f2_nodal<", "f3_element": "f3_cluster<",
+    kernels = {"f1_volume": "f1_cluster<", "f2_nodal": "f2_nodal<", "f3_element": "f3_cluster<",
                "f4_node": "f4_node<"}
     for name in fetch:
         short = next((k for k, tag in kernels.items() if tag in name), None)
