@@ -4,7 +4,8 @@
 //
 //   adjoint_elem_check forward   < elements.txt   element forward, one line per element
 //   adjoint_elem_check vjp NN SEED COUNT PLASTIC [LAW]  reverse vs central
-//                                    differences (LAW 0 bilinear, 1 Hollomon)
+//                                    differences (LAW 0 bilinear, 1 Hollomon,
+//                                    2 Johnson-Cook)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -30,9 +31,13 @@ static int forward_mode() {
   ElemConst k;
   int M;
   if (scanf("%lf %lf %lf %lf %lf %d %d", &k.dt, &k.G, &k.K, &k.cs0, &k.hg, &k.hourglass, &M) != 7) return 2;
-  // the law: 0 (bilinear) or 1 Kh mh eps0 eps1 sy0m (Hollomon)
+  // the law: 0 (bilinear), 1 Kh mh eps0 eps1 sy0m (Hollomon) or
+  // 2 B n C eps0 soft cap E (Johnson-Cook, A the per-element sy0)
   if (scanf("%d", &k.law) != 1) return 2;
   if (k.law == 1 && scanf("%lf %lf %lf %lf %lf", &k.Kh, &k.mh, &k.eps0, &k.eps1, &k.sy0m) != 5) return 2;
+  if (k.law == 2 && scanf("%lf %lf %lf %lf %lf %lf %lf", &k.jB, &k.jn, &k.jC, &k.jeps0, &k.jsoft, &k.jcap,
+                          &k.Emod) != 7)
+    return 2;
   for (int e = 0; e < M; ++e) {
     ElemIn<NN> in;
     for (int a = 0; a < NN; ++a)
@@ -78,6 +83,14 @@ static int vjp_mode(int count, int want_plastic, int law) {
       k.eps0 = 1.5e-3;
       k.eps1 = 1.0e-4;
       k.sy0m = 2.0e8;
+    } else if (law == 2) {  // Johnson-Cook, strain rates well below the cap
+      k.law = 2;
+      k.jn = 0.2 + 0.5 * urand();
+      k.jC = 0.01 + 0.05 * urand();
+      k.jeps0 = 1.0;
+      k.jsoft = 0.7 + 0.2 * urand();
+      k.jcap = 1.0e6;
+      k.Emod = 200.0e9;
     }
     ElemIn<NN> in;
     const double h = 0.02;
@@ -92,7 +105,8 @@ static int vjp_mode(int count, int want_plastic, int law) {
       in.r[a] = 1.0 + 2.0e-3 * sym();
     }
     for (int q = 0; q < 6; ++q) in.tau[q] = 2.0e8 * sym();
-    in.ep = 0.02 * urand();
+    // Johnson-Cook: ep^(n-2) curvature kept moderate away from ep = 0
+    in.ep = law == 2 ? 0.005 + 0.02 * urand() : 0.02 * urand();
     in.rho = 7850.0;
     in.H = 1.0e9 * (0.5 + urand());
     // yield stress well below or above the trial stress
@@ -102,6 +116,12 @@ static int vjp_mode(int count, int want_plastic, int law) {
       ElemFwd<NN> F;
       elem_forward<NN>(k, probe, F);
       in.sy0 = (want_plastic ? 0.6 : 1.6) * F.st - in.H * in.ep;
+      if (law == 2) {  // A and B ep^n in equal parts of the target yield stress
+        const double lt = 1.0 + k.jC * log(F.edot / k.jeps0);
+        const double target = (want_plastic ? 0.6 : 1.6) * F.st / (lt * k.jsoft);
+        in.sy0 = 0.5 * target;
+        k.jB = 0.5 * target / pow(in.ep, k.jn);
+      }
       if (in.sy0 <= 0.0) continue;
       // Hollomon: Kh with Kh s^mh the same fraction of the trial stress
       if (law == 1) k.Kh = (want_plastic ? 0.6 : 1.6) * F.st / pow(in.ep + k.eps0, k.mh);
@@ -120,7 +140,7 @@ static int vjp_mode(int count, int want_plastic, int law) {
     // directions: all inputs at once, then every parameter alone
     for (int dir = 0; dir < 6; ++dir) {
       ElemIn<NN> d{};
-      double dhg = 0.0, dK = 0.0, dm = 0.0;
+      double dhg = 0.0, dK = 0.0, dm = 0.0, dB = 0.0;
       if (dir < 3) {
         for (int a = 0; a < NN; ++a) {
           for (int c = 0; c < 3; ++c) {
@@ -136,11 +156,13 @@ static int vjp_mode(int count, int want_plastic, int law) {
         else d.sy0 = 1.0e8;
       } else if (dir == 4) {
         if (law == 1) dm = 0.01;
+        else if (law == 2) dB = 1.0e-3 * k.jB;
         else d.H = 1.0e9;
       } else {
         dhg = 0.05;
       }
-      double ad = B.ep * d.ep + B.sy0 * d.sy0 + B.H * d.H + B.Kh * dK + B.mh * dm + B.hg * dhg;
+      // Johnson-Cook: the B row sits in B.H
+      double ad = B.ep * d.ep + B.sy0 * d.sy0 + B.H * (law == 2 ? dB : d.H) + B.Kh * dK + B.mh * dm + B.hg * dhg;
       for (int a = 0; a < NN; ++a) {
         for (int c = 0; c < 3; ++c) ad += (B.X[a][c] + Xv[a][c]) * d.X[a][c] + B.V[a][c] * d.V[a][c];
         ad += B.r[a] * d.r[a];
@@ -168,6 +190,7 @@ static int vjp_mode(int count, int want_plastic, int law) {
         kk.hg += s * dhg;
         kk.Kh += s * dK;
         kk.mh += s * dm;
+        kk.jB += s * dB;
         return phi<NN>(kk, p, fb, tb, epb, pl);
       };
       int p1 = 0, p2 = 0;

@@ -12,8 +12,9 @@ carry an error of ~3e-5 of their own (measured with the oracle on the hexa
 case for hg_coeff: steps 1e-4 and 1e-3 agree to 1e-7, 1e-5 and 1e-6 scatter
 by 1e-5).  The recorded forward must equal the plain fused run bit for bit,
 and the gradient must not depend on the checkpoint interval.  The Hollomon
-law's gradient with respect to (K, m, hg_coeff) meets the same bar, and both
-laws' parameters are identified from a reaction history.  The identification
+law's gradient with respect to (K, m, hg_coeff) and the Johnson-Cook law's
+with respect to (A, B, hg_coeff) meet the same bar, and all three laws'
+parameters are identified from a reaction history.  The identification
 workload at its own size (30^3 hexa, 2000 steps, checkpoint every 50) meets
 the 1e-5 bar against GPU and oracle differences, with the hg_coeff step
 chosen from a sweep (see the comment above its tests).
@@ -23,7 +24,7 @@ import pytest
 
 from oracle.wfref import RefDomain
 from weldformfem_amd import Domain, WFX_FP32_MIXED, WFX_FP64, workloads
-from weldformfem_amd.adjoint import CompressionProblem, identify, reaction_history
+from weldformfem_amd.adjoint import JC_FIXED, CompressionProblem, identify, reaction_history
 
 pytestmark = pytest.mark.gpu
 
@@ -276,6 +277,78 @@ def test_hollomon_identification():
     say(f"  theta {theta} hidden {HOLL_HIDDEN} loss {loss:.3e} after {nit} iterations")
     assert loss < 1e-12 * losses[0]
     np.testing.assert_allclose(theta, HOLL_HIDDEN, rtol=1e-6)
+
+
+# ---- Johnson-Cook law: theta = (A, B, hg_coeff) at a uniform 500 K ---------
+# reference rate eps_0 = 1e-3 /s: the top face's 1 m/s over the unit cube
+# strains at about 1 /s, so the rate factor 1 + C ln(edot / eps_0) is about
+# 1.1 and its derivative through every element's strain rate carries weight
+JC = dict(JC_FIXED, eps_0=1.0e-3)
+JC_HIDDEN = np.array([330.0e6, 520.0e6, 0.05])
+JC_THETA0 = np.array([300.0e6, 450.0e6, 0.07])
+
+
+def oracle_hist_jc(theta, steps=STEPS):
+    r = RefDomain()
+    r.AddBoxLength([0, 0, 0], [1.0, 1.0, 1.0], 0.5 / CELLS)
+    bottom, top = workloads.box_layers(CELLS, CELLS, CELLS)
+    workloads.setup_compression(r, bottom, top, -1.0, hg_coeff=float(theta[2]))
+    m = workloads.STEEL
+    r.setMaterialJC(m["E"], m["nu"], m["rho"], float(theta[0]), float(theta[1]), JC["n"], JC["C"], JC["eps_0"],
+                    JC["m"], JC["T_m"], JC["T_t"])
+    r.setTemperature(JC["temp"])
+    r.init()
+    return r.run(steps)
+
+
+@pytest.fixture(scope="module")
+def jc_case():
+    pb = CompressionProblem(cells=CELLS, size=1.0, v_top=-1.0, nsteps=STEPS, every=EVERY, law="johnson_cook", jc=JC)
+    target = pb.forward(JC_HIDDEN)
+    L, g = pb.loss_and_grad(JC_THETA0, target)
+    return pb, target, L, g
+
+
+def test_johnson_cook_gradient_matches_finite_differences(jc_case):
+    pb, target, L, g = jc_case
+    say(f"case: Johnson-Cook adjoint gradient d/d(A, B, hg) {g} vs central differences (GPU and oracle)")
+    assert pb.domain.field("pl_strain").max() > 0
+    tgt = oracle_hist_jc(JC_HIDDEN)
+    assert np.max(np.abs(tgt - target)) <= 1e-9 * np.max(np.abs(tgt))
+    # A and B move the loss strongly (|theta dL/dtheta| / L = 15 and 3.6):
+    # step 1e-6.  hg_coeff moves it by 2.9e-5 of L; oracle differences at
+    # steps 3e-3, 1e-3, 3e-4, 1e-4 give -4.56909e13, -4.56849e13, -4.56845e13,
+    # -4.56840e13: truncation falls below 1e-5 at 1e-3 and the loss's
+    # rounding (~3e-14 of L over the step) reaches 1e-5 at 1e-4, so 3e-4
+    rel = [1e-6, 1e-6, 3e-4]
+    fd = central_fd(lambda t: gpu_loss(pb, t, target), JC_THETA0, rel)
+
+    def oracle_loss(t):
+        r = oracle_hist_jc(t) - tgt
+        return float(np.dot(r, r))
+
+    fdo = central_fd(oracle_loss, JC_THETA0, rel)
+    err, erro = np.abs(g - fd) / np.abs(fd), np.abs(g - fdo) / np.abs(fdo)
+    say(f"  steps {rel} GPU fd {fd} rel err {err}; oracle fd {fdo} rel err {erro}")
+    assert np.all(np.abs(g) > 0)
+    assert np.all(err <= 1e-5) and np.all(erro <= 1e-5)
+
+
+def test_johnson_cook_identification():
+    # hardening exponent n = 0.5: B ep^n rises with infinite slope at ep = 0,
+    # so each element that starts to yield one step earlier or later as theta
+    # moves leaves a cusp of order dep^n in the loss.  At n = 0.26 the cusps
+    # over 600 steps make the loss rough enough that L-BFGS-B stalls at 3e-4
+    # of the initial loss even with oracle central differences as the
+    # gradient; at n = 0.5 and 1.0 those converge to the hidden values
+    say("case: identification of (A, B, hg_coeff) from a 600-step reaction history, Johnson-Cook law, n = 0.5")
+    pb = CompressionProblem(cells=CELLS, size=1.0, v_top=-1.0, nsteps=600, every=EVERY, law="johnson_cook",
+                            jc=dict(JC, n=0.5))
+    target = pb.forward(JC_HIDDEN)
+    theta, loss, nit, losses = identify(pb, target, JC_THETA0, scale=np.array([3e8, 5e8, 0.06]), max_iter=100)
+    say(f"  theta {theta} hidden {JC_HIDDEN} loss {loss:.3e} after {nit} iterations")
+    assert loss < 1e-12 * losses[0]
+    np.testing.assert_allclose(theta, JC_HIDDEN, rtol=1e-6)
 
 
 # ---- the identification workload at its own size ---------------------------

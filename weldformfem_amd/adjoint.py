@@ -4,9 +4,11 @@ The gradient comes from the library's hand-written adjoint
 (wfx_adjoint_forward / wfx_adjoint_backward: checkpoint every K steps,
 recompute each segment, reverse step kernels).  ``ReactionHistory`` exposes
 the forward solve as a torch.autograd.Function of the parameter vector
-theta = (sigma_y0, Et, hg_coeff) for the bilinear law or (K, m, hg_coeff)
-for the Hollomon law (its plateau stress sigma_y0 held fixed), so a loss
-written in torch differentiates through the whole explicit solve.
+theta = (sigma_y0, Et, hg_coeff) for the bilinear law, (K, m, hg_coeff)
+for the Hollomon law (its plateau stress sigma_y0 held fixed) or (A, B,
+hg_coeff) for the Johnson-Cook law (n, C, eps_0, m and the uniform
+temperature held fixed), so a loss written in torch differentiates through
+the whole explicit solve.
 """
 import numpy as np
 
@@ -14,7 +16,11 @@ from . import workloads
 from .domain import WFX_BILINEAR, WFX_FP64, WFX_HOLLOMON, Domain
 
 PARAMS = ("sy0", "Et", "hg_coeff")
-LAW_PARAMS = {"bilinear": PARAMS, "hollomon": ("K", "m", "hg_coeff")}
+LAW_PARAMS = {"bilinear": PARAMS, "hollomon": ("K", "m", "hg_coeff"), "johnson_cook": ("A", "B", "hg_coeff")}
+# Johnson-Cook constants held fixed while A and B vary: hardening exponent n,
+# rate constant C at reference rate eps_0, softening exponent m between the
+# transition and melting temperatures, and the specimen's uniform temperature
+JC_FIXED = dict(n=0.26, C=0.014, eps_0=1.0, m=1.03, T_m=1793.0, T_t=293.0, temp=500.0)
 
 
 class CompressionProblem:
@@ -23,13 +29,16 @@ class CompressionProblem:
     `size` (or any mesh given as (x, elnod, nodxelem, bottom, top)), nsteps
     steps at dt = 0.8 x the critical step.  The reaction force of the top
     face is the observed history.  law "bilinear": theta = (sigma_y0, Et,
-    hg_coeff); "hollomon": theta = (K, m, hg_coeff) with sigma_y0 from mat."""
+    hg_coeff); "hollomon": theta = (K, m, hg_coeff) with sigma_y0 from mat;
+    "johnson_cook": theta = (A, B, hg_coeff) with the constants jc (default
+    JC_FIXED)."""
 
     def __init__(self, cells=30, size=1.0, v_top=-1.0, nsteps=2000, every=50, precision=WFX_FP64,
-                 mat=workloads.STEEL, stream=0, mesh=None, law="bilinear"):
+                 mat=workloads.STEEL, stream=0, mesh=None, law="bilinear", jc=None):
         if law not in LAW_PARAMS:
             raise ValueError(f"law must be one of {sorted(LAW_PARAMS)}")
         self.law, self.params = law, LAW_PARAMS[law]
+        self.jc = dict(JC_FIXED if jc is None else jc)
         d = Domain(precision, stream=stream)
         if mesh is None:
             d.AddBoxLength([0.0, 0.0, 0.0], [size, size, size], 0.5 * size / cells)
@@ -48,6 +57,10 @@ class CompressionProblem:
         d = self.domain
         if self.law == "hollomon":
             d.setMaterial(WFX_HOLLOMON, m["E"], m["nu"], m["rho"], m["sy0"], a, b)
+        elif self.law == "johnson_cook":
+            j = self.jc
+            d.setMaterialJC(m["E"], m["nu"], m["rho"], a, b, j["n"], j["C"], j["eps_0"], j["m"], j["T_m"], j["T_t"])
+            d.setTemperature(j["temp"])
         else:
             d.setMaterial(WFX_BILINEAR, m["E"], m["nu"], m["rho"], a, b, 0.0)
         d.set("hg_coeff", hg)

@@ -227,7 +227,8 @@ __global__ __launch_bounds__(kB) void a_elem1(int E, int N, adj::ElemConst K, co
     contrib[(a * 7 + 6) * Es + e] = B.r[a];
   }
   volb[e] = B.volb;
-  // rows: the law's two parameters (bilinear sy0, H; Hollomon Kh, mh), hg
+  // rows: the law's two parameters (bilinear sy0, H; Hollomon Kh, mh;
+  // Johnson-Cook A, B), hg
   grad[e] += K.law == 1 ? B.Kh : B.sy0;
   grad[Es + e] += K.law == 1 ? B.mh : B.H;
   grad[2 * Es + e] += B.hg;
@@ -349,8 +350,11 @@ template <> DevElem<float> &el_of<float>(wfx_ctx *c) { return c->e32; }
 
 static int check_supported(wfx_ctx *c) {
   const Stab &s = c->prm.stab;
-  REQUIRE(c->prm.mat.model == WFX_BILINEAR || c->prm.mat.model == WFX_HOLLOMON, WFX_ERR_STATE,
-          "the adjoint supports the bilinear and Hollomon materials");
+  REQUIRE(c->prm.mat.model == WFX_BILINEAR || c->prm.mat.model == WFX_HOLLOMON ||
+              c->prm.mat.model == WFX_JOHNSON_COOK,
+          WFX_ERR_STATE, "the adjoint supports the bilinear, Hollomon and Johnson-Cook materials");
+  REQUIRE(c->prm.mat.model != WFX_JOHNSON_COOK || !c->temp_e, WFX_ERR_STATE,
+          "the Johnson-Cook adjoint needs one uniform temperature (no per-element temperatures)");
   REQUIRE(!(c->prm.cfl > 0.0), WFX_ERR_STATE, "the adjoint needs a fixed time step (wfx_set_cfl(ctx, 0))");
   REQUIRE(!c->thermal.on, WFX_ERR_STATE, "the adjoint does not cover thermal coupling");
   REQUIRE(!c->contact.on, WFX_ERR_STATE, "the adjoint does not cover contact");
@@ -522,6 +526,16 @@ static int adjoint_backward(wfx_ctx *c, hipStream_t st) {
     K.eps0 = P.mat.eps0;
     K.eps1 = P.mat.eps1;
     K.sy0m = P.mat.sy0;
+  } else if (P.mat.model == WFX_JOHNSON_COOK) {
+    K.law = 2;
+    K.jB = P.mat.B;
+    K.jn = P.mat.n;
+    K.jC = P.mat.C;
+    K.jeps0 = P.mat.eps_0;
+    // the softening factor 1 - T_h^m of the uniform temperature (Material.cuh:379-384)
+    K.jsoft = 1.0 - pow((P.mat.temp - P.mat.T_t) / (P.mat.T_m - P.mat.T_t), P.mat.m);
+    K.jcap = P.mat.max_edot;
+    K.Emod = P.mat.E;
   }
   NodeAdj NP{P.dt, P.alpha, P.beta, P.gamma, c->has_fe, A->ng};
   for (int seg = A->nck - 1; seg >= 0; --seg) {

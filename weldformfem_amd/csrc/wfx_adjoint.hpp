@@ -24,7 +24,11 @@
 //             R_xz = txx Wxz + txy Wyz + Wxy tyz + Wxz tzz
 //   s       = dev(tau1 - p I), st = sqrt(3 J2(s))      (Mechanical.C:1710-1721)
 //   sy, H   = sy0 + H ep, H (bilinear); Kh s^mh, Kh mh s^(mh-1) with s = ep + eps0
-//             if s > eps1, else sy0 and 0 (Hollomon, Material.cuh:357-360, :391-392)
+//             if s > eps1, else sy0 and 0 (Hollomon, Material.cuh:357-360, :391-392);
+//             Johnson-Cook (Material.cuh:378-411, rate and cap Mechanical.C:1700-1748):
+//             (sy0 + B ep^n)(1 + C ln(r/eps_0)) soft with r the von Mises
+//             rate of D (1e-5 at 0), tangent n B ep^(n-1)(1 + C ln(min(r, cap)/eps_0))
+//             soft for ep > 0, else 0.1 E
 //   plastic (sy < st): dg = (st - sy)/(3G + H), tau2 = (1 - 3G dg/st) s, ep2 = ep + dg
 //   elastic:           tau2 = tau1, ep2 = ep           (Mechanical.C:1723-1762)
 //   sigma   = tau2 - p I                               (:1766)
@@ -52,6 +56,11 @@ struct ElemConst {
   // (Material.cuh:357-360, :391-392)
   int law = 0;
   double Kh = 0.0, mh = 0.0, eps0 = 0.0, eps1 = 0.0, sy0m = 0.0;
+  // law 2, Johnson-Cook at a uniform temperature (Material.cuh:378-411):
+  // sigma_y = (A_e + B ep^n)(1 + C ln(r/eps_0)) soft, r the effective strain
+  // rate (1e-5 for a zero rate); tangent n B ep^(n-1)(1 + C ln(rc/eps_0)) soft
+  // with the capped rate rc = min(r, cap) for ep > 0, 0.1 E at ep = 0
+  double jB = 0.0, jn = 0.0, jC = 0.0, jeps0 = 1.0, jsoft = 1.0, jcap = 1.0e6, Emod = 0.0;
 };
 
 template <int NN>
@@ -73,6 +82,9 @@ struct ElemFwd {
   double Ht;  // tangent modulus of the return (bilinear H; Hollomon Kh mh s^(mh-1) or 0)
   double hs;  // Hollomon: s = ep + eps0
   int plastic, hpow;  // hpow: Hollomon power branch (s > eps1)
+  // Johnson-Cook: effective rate, the rate the yield stress takes, its log
+  // term, ep^n, the capped rate and the tangent's log term
+  double edot, sr, lterm, pw, edc, ltc;
   // forces
   double hm[3][4], ch0, ch, fnet[NN][3];
 };
@@ -173,6 +185,24 @@ WFX_HD void elem_forward(const ElemConst &k, const ElemIn<NN> &in, ElemFwd<NN> &
     F.hpow = F.hs > k.eps1;
     F.sy = F.hpow ? k.Kh * pow(F.hs, k.mh) : k.sy0m;
     F.Ht = F.hpow ? k.Kh * k.mh * pow(F.hs, k.mh - 1.0) : 0.0;
+  } else if (k.law == 2) {
+    F.hs = 0.0;
+    F.hpow = 0;
+    const double *D = F.D;
+    F.edot = sqrt(0.5 * ((D[0] - D[1]) * (D[0] - D[1]) + (D[1] - D[2]) * (D[1] - D[2]) + (D[2] - D[0]) * (D[2] - D[0])) +
+                  3.0 * (D[3] * D[3] + D[4] * D[4] + D[5] * D[5]));
+    F.sr = (F.edot == 0.0) ? 1.0e-5 : F.edot;
+    F.lterm = 1.0 + k.jC * log(F.sr / k.jeps0);
+    F.pw = pow(in.ep, k.jn);
+    F.sy = (in.sy0 + k.jB * F.pw) * F.lterm * k.jsoft;
+    F.edc = (k.jcap < F.edot) ? k.jcap : F.edot;
+    if (in.ep > 0.0) {
+      F.ltc = (F.edc == F.sr) ? F.lterm : 1.0 + k.jC * log(F.edc / k.jeps0);
+      F.Ht = k.jn * k.jB * pow(in.ep, k.jn - 1.0) * F.ltc * k.jsoft;
+    } else {
+      F.ltc = 0.0;
+      F.Ht = k.Emod * 0.1;
+    }
   } else {
     F.hs = 0.0;
     F.hpow = 0;
@@ -226,7 +256,8 @@ WFX_HD void elem_forward(const ElemConst &k, const ElemIn<NN> &in, ElemFwd<NN> &
 // strain (epb).  Outputs: cotangents of X, V, the nodal ratios r, tau, ep,
 // the element volume through the hourglass coefficient (volb, the volume
 // paths through the nodal volumes are added by the caller) and of the
-// parameters: sy0 and H (bilinear) or Kh and mh (Hollomon), and hg.
+// parameters: sy0 and H (bilinear), Kh and mh (Hollomon) or A (in sy0) and B
+// (in H) (Johnson-Cook), and hg.
 template <int NN>
 struct ElemBar {
   double X[NN][3], V[NN][3], r[NN], tau[6], ep, volb, sy0, H, Kh, mh, hg;
@@ -294,6 +325,7 @@ WFX_HD void elem_reverse(const ElemConst &k, const ElemIn<NN> &in, const ElemFwd
   B.H = 0.0;
   B.Kh = 0.0;
   B.mh = 0.0;
+  double edb = 0.0;  // Johnson-Cook: cotangent of the effective strain rate
   if (F.plastic) {
     const double G3 = 3.0 * k.G, den = G3 + F.Ht;
     double sb[6], facb = 0.0;
@@ -312,6 +344,22 @@ WFX_HD void elem_reverse(const ElemConst &k, const ElemIn<NN> &in, const ElemFwd
         B.Kh = syb * pm1 * F.hs + Htb * k.mh * pm1;
         B.mh = syb * F.sy * lns + Htb * k.Kh * pm1 * (1.0 + k.mh * lns);
       }  // plateau: sy and Ht do not depend on ep, Kh or mh
+    } else if (k.law == 2) {
+      // sy = (A + B ep^n) lterm soft: rows A (sy0) and B (H)
+      const double hard = in.sy0 + k.jB * F.pw;
+      B.sy0 = syb * F.lterm * k.jsoft;
+      B.H = syb * F.pw * F.lterm * k.jsoft;
+      if (F.edot != 0.0) edb += syb * hard * k.jsoft * k.jC / F.edot;  // d lterm / d r = C / r
+      if (in.ep > 0.0) {
+        const double pnm1 = pow(in.ep, k.jn - 1.0);
+        B.ep += syb * k.jB * k.jn * pnm1 * F.lterm * k.jsoft;
+        // Ht = n B ep^(n-1) ltc soft
+        B.ep += Htb * k.jn * (k.jn - 1.0) * k.jB * pow(in.ep, k.jn - 2.0) * F.ltc * k.jsoft;
+        B.H += Htb * k.jn * pnm1 * F.ltc * k.jsoft;
+        // the capped rate follows the rate below the cap (a zero rate leaves
+        // the tangent at ln 0: no derivative)
+        if (F.edc == F.edot && F.edot != 0.0) edb += Htb * k.jn * k.jB * pnm1 * k.jsoft * k.jC / F.edot;
+      }  // ep = 0: Ht = 0.1 E
     } else {
       B.H = Htb + syb * in.ep;
       B.sy0 = syb;
@@ -338,6 +386,15 @@ WFX_HD void elem_reverse(const ElemConst &k, const ElemIn<NN> &in, const ElemFwd
     const double mb = (t1b[0] + t1b[1] + t1b[2]) / 3.0;
     for (int q = 0; q < 3; ++q) Db[q] = c2 * (t1b[q] - mb);
     for (int q = 3; q < 6; ++q) Db[q] = c2 * t1b[q];
+    if (edb != 0.0 && F.edot > 0.0) {  // r = sqrt(Q(D)), dr/dD = (dQ/dD) / (2 r)
+      const double ie = 0.5 / F.edot, *D = F.D;
+      Db[0] += edb * ie * (2.0 * D[0] - D[1] - D[2]);
+      Db[1] += edb * ie * (2.0 * D[1] - D[0] - D[2]);
+      Db[2] += edb * ie * (2.0 * D[2] - D[0] - D[1]);
+      Db[3] += edb * ie * 6.0 * D[3];
+      Db[4] += edb * ie * 6.0 * D[4];
+      Db[5] += edb * ie * 6.0 * D[5];
+    }
     const double Rxx = k.dt * t1b[0], Rxy = k.dt * t1b[3], Ryz = k.dt * t1b[4], Rxz = k.dt * t1b[5];
     // R_xx = 2(Wxy txy + Wxz txz)
     Wb[0] += 2.0 * Rxx * t[3]; tb[3] += 2.0 * Rxx * Wxy;
