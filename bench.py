@@ -63,11 +63,12 @@ def fused_bytes(n, s, slots_per_elem, clusters_per_elem, hg=True, rho_uniform=Tr
         "f2_nodal": (s + rho * s, 8 * I4 + I4 + 4 * over8_per_node + 8 + 8 + s),
         # gathers x, v, ratio
         "f3_element": (f3_elem, 3 * 8 + 3 * sn + s),
-        # [slot ids] and partial forces; offsets, mass, BC slot; v, prev_a, u_dt
-        # read and written, x read and written, [u read and written] (fi is
-        # stored on a run's last step only)
+        # [slot ids] and partial forces; offsets, mass, BC slot; v, prev_a
+        # read and written (the predicted increment is formed from them), x
+        # read and written, [u read and written] (fi and u_dt are stored on a
+        # run's last step only)
         "f4_node": (spe * ((0 if node_major else I4) + 3 * s),
-                    I4 + 8 + I4 + 3 * 2 * 3 * sn + 2 * 3 * 8 + u_rw),
+                    I4 + 8 + I4 + 2 * 2 * 3 * sn + 2 * 3 * 8 + u_rw),
     }
 
 

@@ -1134,6 +1134,8 @@ struct NodeParams {
   double pred_v;   // (1-gamma)*dt
   int predict;     // run the next step's prediction
   int write_fi;    // store the internal force (last step of a run, or its reaction is recorded)
+  int read_ud;     // read the predicted increment (else recomputed from v' and prev_a)
+  int write_ud;    // store the increment every step (else on a run's last step only)
   int has_fe;      // external forces present
   const double *dtp;  // per-step CFL time step: the step's dt on the device
   double *vmax_part;  //   and the per-block largest predicted speed
@@ -1174,7 +1176,7 @@ __device__ __forceinline__ void load_node(int i, long Ns, const NodeParams &np, 
     const long r = d * Ns;
     s.v[d] = (v + r)[ui];
     s.pa[d] = (prev_a + r)[ui];
-    s.ud[d] = (u_dt + r)[ui];
+    s.ud[d] = np.read_ud ? (u_dt + r)[ui] : T(0);
     s.x[d] = (x + r)[ui];
     s.u[d] = 0.0;
     if constexpr (std::is_same<T, double>::value) s.u[d] = (u + r)[ui];  // FP32-mixed: u from x when a run ends
@@ -1194,6 +1196,15 @@ __device__ __forceinline__ void load_node(int i, long Ns, const NodeParams &np, 
 // are stored in float (the update itself runs in double), positions stay
 // double, and u is not touched per step (recovered as x - x_start when the
 // run ends).
+// The predicted increment u_dt = dt (v + (0.5 - beta) dt a) of the previous
+// node pass is not stored between the steps of a run: it is formed again
+// from what that pass stored, v' = v + (1 - gamma) dt a (or the imposed
+// velocity, which the prediction then used itself) and prev_a = a, which
+// gives it up to the rounding of v'.  The first step of a run reads the
+// increment UpdatePrediction left (its v may differ from an imposed v'), a
+// run's last step stores the corrected increment, and with a per-step time
+// step (the previous step's length predicted it) it is read and stored
+// every step.
 template <typename T, bool CFL, bool CONTACT>
 __device__ __forceinline__ double node_update(
     int i, long Ns, const NodeParams &np, double (&f)[3], const NodeIn<T, CONTACT> &s,
@@ -1223,7 +1234,15 @@ __device__ __forceinline__ double node_update(
     a = np.f * (a - np.alpha * pa);                     // UpdateCorrectionAccVel
     double vv = (double)s.v[d] + np.gdt * a;
     if (mask & (1 << d)) vv = bc_val[3L * slot + d];    // ImposeBCV
-    double ud = (double)s.ud[d] + np.bdt2 * a;          // UpdateCorrectionPos
+    double ud;
+    if (np.read_ud) {
+      ud = (double)s.ud[d];
+    } else {  // the previous node pass's prediction from its v' and a
+      double v0 = (double)s.v[d];
+      if (!(mask & (1 << d))) v0 = v0 - np.pred_v * pa;
+      ud = np.dt * (v0 + np.pred_u * pa);
+    }
+    ud = ud + np.bdt2 * a;                              // UpdateCorrectionPos
     x_d = s.x[d] + ud;
     if constexpr (std::is_same<T, double>::value) (u + r)[ui] = s.u[d] + ud;
     if (np.predict) {                                   // UpdatePrediction
@@ -1231,7 +1250,7 @@ __device__ __forceinline__ double node_update(
       vv = vv + np.pred_v * a;
       if (mask & (1 << d)) vv = bc_val[3L * slot + d];  // ImposeBCV
     }
-    ud_d = (T)ud;
+    if (np.write_ud || !np.predict) ud_d = (T)ud;
     pa_d = (T)a;
     v_d = (T)vv;
     vs[d] = (double)(T)vv;
@@ -1441,6 +1460,8 @@ inline NodeParams node_params(wfx_ctx *c, int predict) {
   np.pred_v = (1.0 - P.gamma) * P.dt;
   np.predict = predict;
   np.write_fi = c->fused_write_fi;
+  np.read_ud = c->fused_read_ud;
+  np.write_ud = c->fused_write_ud;
   np.has_fe = c->has_fe;
   np.dtp = nullptr;
   np.vmax_part = nullptr;
@@ -1669,6 +1690,10 @@ hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t 
     // history, the element outputs after the run
     c->fused_write_fi = (k + 1 == nsteps || hist) ? 1 : 0;
     c->fused_write_out = (k + 1 == nsteps) ? 1 : 0;
+    // the increment is predicted again from v' and prev_a after the first
+    // step, unless each step has its own length
+    c->fused_read_ud = (k == 0 || c->prm.cfl > 0.0) ? 1 : 0;
+    c->fused_write_ud = c->prm.cfl > 0.0 ? 1 : 0;
     for (int ph = 1; ph <= 4; ++ph) {
       if (ev) {
         hipError_t e = hipEventRecord(ev[4 * k + ph - 1], s);
@@ -1676,11 +1701,11 @@ hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t 
       }
       hipError_t e = launch_fused_phase<T>(c, ph, k + 1 < nsteps, s, k);
       if (e != hipSuccess) {
-        c->fused_write_fi = c->fused_write_out = 1;
+        c->fused_write_fi = c->fused_write_out = c->fused_read_ud = c->fused_write_ud = 1;
         return e;
       }
     }
-    c->fused_write_fi = c->fused_write_out = 1;
+    c->fused_write_fi = c->fused_write_out = c->fused_read_ud = c->fused_write_ud = 1;
     if (ev && k + 1 == nsteps) {
       hipError_t e = hipEventRecord(ev[4 * nsteps], s);
       if (e != hipSuccess) return e;

@@ -220,6 +220,8 @@ struct wfx_ctx {
   bool fused_stale = false;  // per-step intermediates not maintained by the fused path
   int fused_write_fi = 1;    // the fused node pass stores the internal force (cleared inside runs)
   int fused_write_out = 1;   // the fused element pass stores p, sigma_y, plastic heat (cleared inside runs)
+  int fused_read_ud = 1;     // the fused node pass reads the predicted increment u_dt (cleared inside runs)
+  int fused_write_ud = 1;    // and stores it every step (cleared inside runs of a fixed time step)
   double *scalars = nullptr;      // device scratch for reductions (reaction force)
   // this context's device's address of the library's static scratch array
   // (the allocation-free entry point's reduction partials)
