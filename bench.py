@@ -44,7 +44,9 @@ def fused_bytes(n, s, slots_per_elem, clusters_per_elem, hg=True, rho_uniform=Tr
     rho = 1 if (hg and not rho_uniform) else 0
     sn = s
     spe, cpe = slots_per_elem, clusters_per_elem
-    u_rw = 2 * 3 * 8 if s == 8 else 0  # FP32-mixed recovers u from x at the end of a run
+    # FP64 accumulates the rounding error of the position updates in a float
+    # (u from it when a run ends); FP32-mixed recovers u from x alone
+    u_rw = 2 * 3 * 4 if s == 8 else 0
     node_major = s == 4
     stage = 2 * n + I4 * spe  # slot indices of the element's nodes, node id per slot
     # tau, eps read and written; plastic strain read and written; sy0;
@@ -65,8 +67,8 @@ def fused_bytes(n, s, slots_per_elem, clusters_per_elem, hg=True, rho_uniform=Tr
         "f3_element": (f3_elem, 3 * 8 + 3 * sn + s),
         # [slot ids] and partial forces; offsets, mass, BC slot; v, prev_a
         # read and written (the predicted increment is formed from them), x
-        # read and written, [u read and written] (fi and u_dt are stored on a
-        # run's last step only)
+        # read and written, [the positions' rounding error read and written]
+        # (fi, u_dt and u are stored on a run's last step only)
         "f4_node": (spe * ((0 if node_major else I4) + 3 * s),
                     I4 + 8 + I4 + 2 * 2 * 3 * sn + 2 * 3 * 8 + u_rw),
     }

@@ -459,7 +459,7 @@ static hipError_t load_ck(wfx_ctx *c, int idx, hipStream_t st) {
 // increment (the node pass forms it again from v' and prev_a after that), so
 // a segment recomputed from its checkpoint repeats the forward bit for bit.
 template <typename T>
-static hipError_t fused_step(wfx_ctx *c, int predict, bool first, double *rec, hipStream_t st) {
+static hipError_t fused_step(wfx_ctx *c, int predict, bool first, bool last, double *rec, hipStream_t st) {
   T *v, *ud, *pa, *fi;
   node_arrays<T>(c, &v, &ud, &pa, &fi);
   auto &el = el_of<T>(c);
@@ -468,11 +468,12 @@ static hipError_t fused_step(wfx_ctx *c, int predict, bool first, double *rec, h
     hipLaunchKernelGGL((k_rec_pre<T>), dim3(nb(6 * N + 7 * E)), dim3(kB), 0, st, N, E, c->nd.x, v, el.tau,
                        el.pl_strain, rec);
   }
-  c->fused_read_ud = first ? 1 : 0;
+  c->fused_read_ud = c->fused_run_first = first ? 1 : 0;
+  c->fused_run_last = last ? 1 : 0;
   c->fused_write_ud = 0;
   hipError_t e = hipSuccess;
   for (int ph = 1; ph <= 4 && e == hipSuccess; ++ph) e = launch_fused_phase<T>(c, ph, predict, st, -1);
-  c->fused_read_ud = c->fused_write_ud = 1;
+  c->fused_read_ud = c->fused_write_ud = c->fused_run_first = c->fused_run_last = 1;
   if (e != hipSuccess) return e;
   if (rec) {
     hipLaunchKernelGGL((k_rec_post<T>), dim3(nb(5 * N)), dim3(kB), 0, st, N, fi, c->nd.mdiag, c->nd.anp_ratio,
@@ -492,7 +493,7 @@ static int adjoint_forward(wfx_ctx *c, int nsteps, bool want_hist, hipStream_t s
   node_arrays<T>(c, &v, &ud, &pa, &fi);
   for (int k = 0; k < nsteps; ++k) {
     if (k % A->every == 0) LAUNCH(save_ck<T>(c, k / A->every, st));
-    LAUNCH(fused_step<T>(c, k + 1 < nsteps, k == 0, nullptr, st));
+    LAUNCH(fused_step<T>(c, k + 1 < nsteps, k == 0, k + 1 == nsteps, nullptr, st));
     if (want_hist && c->ngroups > 0) {
       double *h = A->hist + (size_t)k * A->ng;
       if constexpr (std::is_same<T, double>::value) LAUNCH(launch_reaction(c, 2, h, st));
@@ -548,7 +549,8 @@ static int adjoint_backward(wfx_ctx *c, hipStream_t st) {
     const int k0 = seg * A->every, k1 = std::min(A->nsteps, k0 + A->every);
     LAUNCH(load_ck<T>(c, seg, st));
     for (int k = k0; k < k1; ++k)
-      LAUNCH(fused_step<T>(c, k + 1 < A->nsteps, k == 0, A->slot + (size_t)(k - k0) * A->slot_stride, st));
+      LAUNCH(fused_step<T>(c, k + 1 < A->nsteps, k == 0, k + 1 == A->nsteps,
+                           A->slot + (size_t)(k - k0) * A->slot_stride, st));
     for (int k = k1 - 1; k >= k0; --k) {
       const double *rec = A->slot + (size_t)(k - k0) * A->slot_stride;
       hipLaunchKernelGGL(a_node1, dim3(nb(N)), dim3(kB), 0, st, N, NP, rec, c->nd.fe, c->bc_slot, c->bc_mask,

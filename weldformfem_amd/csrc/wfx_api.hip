@@ -74,6 +74,7 @@ static void free_all(wfx_ctx *c) {
   c->bc_tab = nullptr;
   c->v32 = c->pa32 = c->ud32 = c->fi32 = nullptr;
   c->x_start = nullptr;
+  c->x_lo = nullptr;
   c->temp_e = c->temp_buf = nullptr;
   c->contact = wfx::Contact();
   c->has_fe = 0;
@@ -182,8 +183,11 @@ static int setup_device(wfx_ctx *c, const ClusterPlan &plan) {
   c->cfl_cap_min = (int)std::max<size_t>((E + 63) / 64 + 1, (size_t)c->ms.ncl + 1);
   A(c->cfl_part, c->cfl_cap_min + (N + 63) / 64 + 1)
   if (e == hipSuccess) e = (c->precision == WFX_FP64) ? alloc_elem(c, c->e64) : alloc_elem(c, c->e32);
+  A(c->x_start, 3 * N)
   if (c->precision == WFX_FP32_MIXED) {
-    A(c->v32, 3 * N) A(c->pa32, 3 * N) A(c->ud32, 3 * N) A(c->fi32, 3 * N) A(c->x_start, 3 * N)
+    A(c->v32, 3 * N) A(c->pa32, 3 * N) A(c->ud32, 3 * N) A(c->fi32, 3 * N)
+  } else {
+    A(c->x_lo, 3 * N)
   }
 #undef A
   if (e != hipSuccess) {

@@ -1136,6 +1136,9 @@ struct NodeParams {
   int write_fi;    // store the internal force (last step of a run, or its reaction is recorded)
   int read_ud;     // read the predicted increment (else recomputed from v' and prev_a)
   int write_ud;    // store the increment every step (else on a run's last step only)
+  int run_first, run_last;  // FP64: first / last step of a run (both: u += u_dt directly)
+  double *x_start;          //   positions when the run started
+  float *x_lo;              //   rounding error of the position updates since then
   int has_fe;      // external forces present
   const double *dtp;  // per-step CFL time step: the step's dt on the device
   double *vmax_part;  //   and the per-block largest predicted speed
@@ -1158,7 +1161,8 @@ __device__ __forceinline__ void set_step_length(NodeParams &np, double dt) {
 template <typename T, bool CONTACT>
 struct NodeIn {
   T v[3], pa[3], ud[3];
-  double x[3], u[3], fe[3], cf[3], m;
+  float lo[3];
+  double x[3], u[3], xs[3], fe[3], cf[3], m;
   int slot;
 };
 template <typename T, bool CONTACT>
@@ -1176,10 +1180,24 @@ __device__ __forceinline__ void load_node(int i, long Ns, const NodeParams &np, 
     const long r = d * Ns;
     s.v[d] = (v + r)[ui];
     s.pa[d] = (prev_a + r)[ui];
-    s.ud[d] = np.read_ud ? (u_dt + r)[ui] : T(0);
+    // the predicted increment is used on a run's first step only; the other
+    // steps load the array's first entry in every lane instead (one cache
+    // line), so that no branch holds the load: a float widened inside one
+    // waits for every load issued before it
+    s.ud[d] = (u_dt + (np.read_ud ? r : 0L))[np.read_ud ? ui : 0u];
     s.x[d] = (x + r)[ui];
-    s.u[d] = 0.0;
-    if constexpr (std::is_same<T, double>::value) s.u[d] = (u + r)[ui];  // FP32-mixed: u from x when a run ends
+    s.u[d] = s.xs[d] = 0.0;
+    s.lo[d] = 0.0f;
+    if constexpr (std::is_same<T, double>::value) {  // FP32-mixed: u from x when a run ends
+      // the error term is loaded every step (a run's first ignores it): a
+      // load behind a branch would have its float widened inside the branch,
+      // which waits for every load issued before it
+      s.lo[d] = (np.x_lo + r)[ui];
+      if (np.run_last) {
+        s.u[d] = (u + r)[ui];
+        s.xs[d] = (np.x_start + r)[ui];
+      }
+    }
     s.fe[d] = np.has_fe ? (fe + r)[ui] : 0.0;
     s.cf[d] = 0.0;
     if constexpr (CONTACT) s.cf[d] = (cf + r)[ui];
@@ -1195,7 +1213,12 @@ __device__ __forceinline__ void load_node(int i, long Ns, const NodeParams &np, 
 // FP32-mixed: velocity, previous acceleration, increment and internal force
 // are stored in float (the update itself runs in double), positions stay
 // double, and u is not touched per step (recovered as x - x_start when the
-// run ends).
+// run ends).  FP64 neither: positions advance as x += u_dt, and the exact
+// rounding error of each of those additions (two-sum) is accumulated in a
+// float beside them, so that u = u_start + ((x - x_start) + error) when the
+// run ends is the sum of the increments to a few units in the last place of
+// u (x - x_start alone loses the bits of x that u does not have); a one-step
+// run adds the increment to u directly.
 // The predicted increment u_dt = dt (v + (0.5 - beta) dt a) of the previous
 // node pass is not stored between the steps of a run: it is formed again
 // from what that pass stored, v' = v + (1 - gamma) dt a (or the imposed
@@ -1234,17 +1257,24 @@ __device__ __forceinline__ double node_update(
     a = np.f * (a - np.alpha * pa);                     // UpdateCorrectionAccVel
     double vv = (double)s.v[d] + np.gdt * a;
     if (mask & (1 << d)) vv = bc_val[3L * slot + d];    // ImposeBCV
-    double ud;
-    if (np.read_ud) {
-      ud = (double)s.ud[d];
-    } else {  // the previous node pass's prediction from its v' and a
-      double v0 = (double)s.v[d];
-      if (!(mask & (1 << d))) v0 = v0 - np.pred_v * pa;
-      ud = np.dt * (v0 + np.pred_u * pa);
-    }
+    // the previous node pass's prediction, from its v' and a
+    double v0 = (double)s.v[d];
+    if (!(mask & (1 << d))) v0 = v0 - np.pred_v * pa;
+    double ud = np.read_ud ? (double)s.ud[d] : np.dt * (v0 + np.pred_u * pa);
     ud = ud + np.bdt2 * a;                              // UpdateCorrectionPos
-    x_d = s.x[d] + ud;
-    if constexpr (std::is_same<T, double>::value) (u + r)[ui] = s.u[d] + ud;
+    const double xo = s.x[d], xn = xo + ud;
+    x_d = xn;
+    if constexpr (std::is_same<T, double>::value) {
+      const double bb = xn - xo;
+      const double err = (xo - (xn - bb)) + (ud - bb);  // xo + ud - xn, exactly
+      const double lo = (np.run_first ? 0.0 : (double)s.lo[d]) + err;
+      if (np.run_last) {
+        (u + r)[ui] = s.u[d] + (np.run_first ? ud : ((xn - s.xs[d]) + lo));
+      } else {
+        if (np.run_first) (np.x_start + r)[ui] = xo;
+        (np.x_lo + r)[ui] = (float)lo;
+      }
+    }
     if (np.predict) {                                   // UpdatePrediction
       ud = np.dt * (vv + np.pred_u * a);
       vv = vv + np.pred_v * a;
@@ -1462,6 +1492,10 @@ inline NodeParams node_params(wfx_ctx *c, int predict) {
   np.write_fi = c->fused_write_fi;
   np.read_ud = c->fused_read_ud;
   np.write_ud = c->fused_write_ud;
+  np.run_first = c->fused_run_first;
+  np.run_last = c->fused_run_last;
+  np.x_start = c->x_start;
+  np.x_lo = c->x_lo;
   np.has_fe = c->has_fe;
   np.dtp = nullptr;
   np.vmax_part = nullptr;
@@ -1694,6 +1728,8 @@ hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t 
     // step, unless each step has its own length
     c->fused_read_ud = (k == 0 || c->prm.cfl > 0.0) ? 1 : 0;
     c->fused_write_ud = c->prm.cfl > 0.0 ? 1 : 0;
+    c->fused_run_first = k == 0 ? 1 : 0;
+    c->fused_run_last = k + 1 == nsteps ? 1 : 0;
     for (int ph = 1; ph <= 4; ++ph) {
       if (ev) {
         hipError_t e = hipEventRecord(ev[4 * k + ph - 1], s);
@@ -1702,10 +1738,12 @@ hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t 
       hipError_t e = launch_fused_phase<T>(c, ph, k + 1 < nsteps, s, k);
       if (e != hipSuccess) {
         c->fused_write_fi = c->fused_write_out = c->fused_read_ud = c->fused_write_ud = 1;
+        c->fused_run_first = c->fused_run_last = 1;
         return e;
       }
     }
     c->fused_write_fi = c->fused_write_out = c->fused_read_ud = c->fused_write_ud = 1;
+    c->fused_run_first = c->fused_run_last = 1;
     if (ev && k + 1 == nsteps) {
       hipError_t e = hipEventRecord(ev[4 * nsteps], s);
       if (e != hipSuccess) return e;

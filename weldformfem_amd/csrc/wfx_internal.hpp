@@ -196,7 +196,8 @@ struct wfx_ctx {
   double *bc_tab = nullptr;
   // FP32-mixed fused path: narrowed nodal state during a run, start positions
   float *v32 = nullptr, *pa32 = nullptr, *ud32 = nullptr, *fi32 = nullptr;
-  double *x_start = nullptr;
+  double *x_start = nullptr;  // (both precisions)
+  float *x_lo = nullptr;      // FP64 fused runs: rounding error of the position updates
   int has_fe = 0;            // external nodal forces set (non-zero)
   double rho_uniform = 0.0;  // > 0: every element density equals this value
   // host scan of vol0 and rho (wfx_init, wfx_set_field): elem_guard 1 when an
@@ -222,6 +223,8 @@ struct wfx_ctx {
   int fused_write_out = 1;   // the fused element pass stores p, sigma_y, plastic heat (cleared inside runs)
   int fused_read_ud = 1;     // the fused node pass reads the predicted increment u_dt (cleared inside runs)
   int fused_write_ud = 1;    // and stores it every step (cleared inside runs of a fixed time step)
+  int fused_run_first = 1;   // the fused step is the first of a run
+  int fused_run_last = 1;    // the fused step is the last of a run
   double *scalars = nullptr;      // device scratch for reductions (reaction force)
   // this context's device's address of the library's static scratch array
   // (the allocation-free entry point's reduction partials)
