@@ -302,9 +302,10 @@ int wfx_graph_launch(wfx_ctx *ctx, double *reaction_hist, wfx_stream s);
 /* ---- adjoint (no reference counterpart; parameter identification) -------- */
 /* Forward: nsteps fused steps from the current state (after wfx_init), as
  * wfx_step_fused, storing the state every checkpoint_every steps; writes the
- * reaction history [nsteps][ngroups] (may be NULL).  Bilinear, Hollomon or
- * Johnson-Cook material (one uniform temperature), fixed time step, the ANP
- * pressure without stabilisation terms, no thermal coupling or contact.
+ * reaction history [nsteps][ngroups] (may be NULL).  Bilinear, Hollomon,
+ * Johnson-Cook or GMT material (the rate laws at one uniform temperature),
+ * fixed time step, the ANP pressure without stabilisation terms, no thermal
+ * coupling or contact.
  * Backward: given dL/dR for that history, recomputes each checkpoint segment
  * and runs the hand-written reverse step over it; grad[3] receives, for the
  * bilinear law, dL/d(sigma_y0) (summed over elements: a uniform yield
@@ -312,9 +313,10 @@ int wfx_graph_launch(wfx_ctx *ctx, double *reaction_hist, wfx_stream s);
  * the Hollomon law dL/dK, dL/dm (the plateau stress sigma_y0 held fixed) and
  * dL/d(hg_coeff); for Johnson-Cook dL/dA (summed: a uniform initial yield),
  * dL/dB and dL/d(hg_coeff) (n, C, eps_0, m held fixed; the rate enters
- * through the strain rate of every step).  Optional per-element arrays
- * (length E, may be NULL): each
- * element's share of the first, second and third of these.
+ * through the strain rate of every step); for GMT dL/dC1, dL/dn2 and
+ * dL/d(hg_coeff) (the other constants and ranges held fixed).  Optional
+ * per-element arrays (length E, may be NULL): each element's share of the
+ * first, second and third of these.
  * The state after the backward call is the forward run's final state.
  * Adjoint arithmetic is FP64 in both precisions; in FP32-mixed the forward and
  * recomputation run in FP32-mixed. */

@@ -5,7 +5,7 @@
 //   adjoint_elem_check forward   < elements.txt   element forward, one line per element
 //   adjoint_elem_check vjp NN SEED COUNT PLASTIC [LAW]  reverse vs central
 //                                    differences (LAW 0 bilinear, 1 Hollomon,
-//                                    2 Johnson-Cook)
+//                                    2 Johnson-Cook, 3 GMT)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -31,12 +31,16 @@ static int forward_mode() {
   ElemConst k;
   int M;
   if (scanf("%lf %lf %lf %lf %lf %d %d", &k.dt, &k.G, &k.K, &k.cs0, &k.hg, &k.hourglass, &M) != 7) return 2;
-  // the law: 0 (bilinear), 1 Kh mh eps0 eps1 sy0m (Hollomon) or
-  // 2 B n C eps0 soft cap E (Johnson-Cook, A the per-element sy0)
+  // the law: 0 (bilinear), 1 Kh mh eps0 eps1 sy0m (Hollomon),
+  // 2 B n C eps0 soft cap E (Johnson-Cook, A the per-element sy0) or
+  // 3 ga gexp gn gm gi e_min e_max er_min er_max cap (GMT)
   if (scanf("%d", &k.law) != 1) return 2;
   if (k.law == 1 && scanf("%lf %lf %lf %lf %lf", &k.Kh, &k.mh, &k.eps0, &k.eps1, &k.sy0m) != 5) return 2;
   if (k.law == 2 && scanf("%lf %lf %lf %lf %lf %lf %lf", &k.jB, &k.jn, &k.jC, &k.jeps0, &k.jsoft, &k.jcap,
                           &k.Emod) != 7)
+    return 2;
+  if (k.law == 3 && scanf("%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", &k.ga, &k.gexp, &k.gn, &k.gm, &k.gi,
+                          &k.ge_min, &k.ge_max, &k.ger_min, &k.ger_max, &k.jcap) != 10)
     return 2;
   for (int e = 0; e < M; ++e) {
     ElemIn<NN> in;
@@ -91,6 +95,19 @@ static int vjp_mode(int count, int want_plastic, int law) {
       k.jsoft = 0.7 + 0.2 * urand();
       k.jcap = 1.0e6;
       k.Emod = 200.0e9;
+    } else if (law == 3) {  // GMT at 300 (the constants of the GPU material tests), ranges
+      // wide enough that the clamps stay open
+      k.law = 3;
+      const double Tc = 300.0;
+      k.gexp = exp(-1.5e-3 * Tc);
+      k.gn = -2e-4 * Tc + 0.15 + 0.1 * urand();
+      k.gm = 1e-4 * Tc + 0.01 + 0.02 * urand();
+      k.gi = -1e-5 * Tc + 1e-4;
+      k.ge_min = 1e-4;
+      k.ge_max = 3.0;
+      k.ger_min = 1e-3;
+      k.ger_max = 1e5;
+      k.jcap = 1.0e6;
     }
     ElemIn<NN> in;
     const double h = 0.02;
@@ -106,7 +123,7 @@ static int vjp_mode(int count, int want_plastic, int law) {
     }
     for (int q = 0; q < 6; ++q) in.tau[q] = 2.0e8 * sym();
     // Johnson-Cook: ep^(n-2) curvature kept moderate away from ep = 0
-    in.ep = law == 2 ? 0.005 + 0.02 * urand() : 0.02 * urand();
+    in.ep = law >= 2 ? 0.005 + 0.02 * urand() : 0.02 * urand();
     in.rho = 7850.0;
     in.H = 1.0e9 * (0.5 + urand());
     // yield stress well below or above the trial stress
@@ -116,6 +133,11 @@ static int vjp_mode(int count, int want_plastic, int law) {
       ElemFwd<NN> F;
       elem_forward<NN>(k, probe, F);
       in.sy0 = (want_plastic ? 0.6 : 1.6) * F.st - in.H * in.ep;
+      if (law == 3) {  // C1 from the target yield stress
+        const double target = (want_plastic ? 0.6 : 1.6) * F.st;
+        const double es = in.ep, er = F.edot;
+        k.ga = target / (pow(es, k.gn) * exp(k.gi / es) * pow(er, k.gm));
+      }
       if (law == 2) {  // A and B ep^n in equal parts of the target yield stress
         const double lt = 1.0 + k.jC * log(F.edot / k.jeps0);
         const double target = (want_plastic ? 0.6 : 1.6) * F.st / (lt * k.jsoft);
@@ -140,7 +162,7 @@ static int vjp_mode(int count, int want_plastic, int law) {
     // directions: all inputs at once, then every parameter alone
     for (int dir = 0; dir < 6; ++dir) {
       ElemIn<NN> d{};
-      double dhg = 0.0, dK = 0.0, dm = 0.0, dB = 0.0;
+      double dhg = 0.0, dK = 0.0, dm = 0.0, dB = 0.0, dC1 = 0.0, dn = 0.0;
       if (dir < 3) {
         for (int a = 0; a < NN; ++a) {
           for (int c = 0; c < 3; ++c) {
@@ -153,16 +175,20 @@ static int vjp_mode(int count, int want_plastic, int law) {
         d.ep = (dir == 2) ? 0.01 * sym() : 0.0;
       } else if (dir == 3) {
         if (law == 1) dK = 1.0e-3 * k.Kh;
+        else if (law == 3) dC1 = 1.0e-3 * k.ga / k.gexp;
         else d.sy0 = 1.0e8;
       } else if (dir == 4) {
         if (law == 1) dm = 0.01;
         else if (law == 2) dB = 1.0e-3 * k.jB;
+        else if (law == 3) dn = 0.01;
         else d.H = 1.0e9;
       } else {
         dhg = 0.05;
       }
       // Johnson-Cook: the B row sits in B.H
-      double ad = B.ep * d.ep + B.sy0 * d.sy0 + B.H * (law == 2 ? dB : d.H) + B.Kh * dK + B.mh * dm + B.hg * dhg;
+      // GMT: the C1 row sits in B.sy0, the n2 row in B.H
+      double ad = B.ep * d.ep + B.sy0 * (law == 3 ? dC1 : d.sy0) + B.H * (law == 2 ? dB : (law == 3 ? dn : d.H)) +
+                  B.Kh * dK + B.mh * dm + B.hg * dhg;
       for (int a = 0; a < NN; ++a) {
         for (int c = 0; c < 3; ++c) ad += (B.X[a][c] + Xv[a][c]) * d.X[a][c] + B.V[a][c] * d.V[a][c];
         ad += B.r[a] * d.r[a];
@@ -191,6 +217,8 @@ static int vjp_mode(int count, int want_plastic, int law) {
         kk.Kh += s * dK;
         kk.mh += s * dm;
         kk.jB += s * dB;
+        kk.ga += s * dC1 * kk.gexp;  // ga = C1 exp(C2 Tc)
+        kk.gn += s * dn;             // gn = n1 Tc + n2
         return phi<NN>(kk, p, fb, tb, epb, pl);
       };
       int p1 = 0, p2 = 0;

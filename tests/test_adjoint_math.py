@@ -33,7 +33,7 @@ def elem_check(tmp_path_factory):
     return out
 
 
-@pytest.mark.parametrize("law", [0, 1, 2])  # bilinear, Hollomon, Johnson-Cook
+@pytest.mark.parametrize("law", [0, 1, 2, 3])  # bilinear, Hollomon, Johnson-Cook, GMT
 @pytest.mark.parametrize("nn", [8, 4])
 @pytest.mark.parametrize("plastic", [0, 1])
 def test_element_reverse_matches_central_differences(elem_check, nn, plastic, law):
@@ -57,7 +57,7 @@ def _mesh(tet, seed):
     return x, el, n
 
 
-@pytest.mark.parametrize("law", ["bilinear", "hollomon", "johnson_cook"])
+@pytest.mark.parametrize("law", ["bilinear", "hollomon", "johnson_cook", "gmt"])
 @pytest.mark.parametrize("tet", [False, True])
 def test_element_forward_matches_oracle(elem_check, tet, law):
     x, el, n = _mesh(tet, seed=5)
@@ -70,12 +70,19 @@ def test_element_forward_matches_oracle(elem_check, tet, law):
     # Johnson-Cook constants: B, n, C, eps_0, m, T_m, T_t at the uniform temperature T_jc
     jc = (400e6, 0.3, 0.03, 1.0, 1.1, 1500.0, 20.0)
     T_jc = 400.0
+    # GMT constants n1 n2 C1 C2 m1 m2 I1 I2 and ranges of e, er, T at T_jc
+    # (plastic strains on both sides of e_min)
+    gmt_c = (-2e-4, 0.15, 3.0e9, -1.5e-3, 1e-4, 0.01, -1e-5, 1e-4)
+    gmt_r = (0.002, 3.0, 1e-3, 1e5, 20.0, 1200.0)
     if law == "bilinear":
         r.setMaterial(0, K_E, nu, rho, 300e6, H_t, 0.0)
     elif law == "hollomon":  # plateau below eps1 = (300e6/1e9)^(1/0.2) = 2.4e-3: some elements on each side
         r.setMaterial(1, K_E, nu, rho, 300e6, 1.0e9, 0.2)
-    else:
+    elif law == "johnson_cook":
         r.setMaterialJC(K_E, nu, rho, 300e6, *jc)
+        r.setTemperature(T_jc)
+    else:
+        r.setMaterialGMT(K_E, nu, rho, 300e6, gmt_c, gmt_r)
         r.setTemperature(T_jc)
     r.set("hg_coeff", 0.06)
     dt = 2.0e-6
@@ -112,6 +119,11 @@ def test_element_forward_matches_oracle(elem_check, tet, law):
     lines = [" ".join(repr(float(q)) for q in (dt, G, K, cs0, 0.06)) + f" {int(n == 8)} {E}"]
     if law == "bilinear":
         lines.append("0")
+    elif law == "gmt":
+        n1, n2, C1, C2, m1, m2, I1, I2 = gmt_c
+        Tc = min(max(T_jc, gmt_r[4]), gmt_r[5])
+        lines.append("3 " + " ".join(repr(float(q)) for q in (C1 * np.exp(C2 * Tc), np.exp(C2 * Tc), n1 * Tc + n2,
+                                                               m1 * Tc + m2, I1 * Tc + I2, *gmt_r[:4], 1.0e6)))
     elif law == "johnson_cook":
         soft = 1.0 - ((T_jc - jc[6]) / (jc[5] - jc[6])) ** jc[4]
         lines.append("2 " + " ".join(repr(float(q)) for q in (jc[0], jc[1], jc[2], jc[3], soft, 1.0e6, K_E)))

@@ -12,9 +12,10 @@ carry an error of ~3e-5 of their own (measured with the oracle on the hexa
 case for hg_coeff: steps 1e-4 and 1e-3 agree to 1e-7, 1e-5 and 1e-6 scatter
 by 1e-5).  The recorded forward must equal the plain fused run bit for bit,
 and the gradient must not depend on the checkpoint interval.  The Hollomon
-law's gradient with respect to (K, m, hg_coeff) and the Johnson-Cook law's
-with respect to (A, B, hg_coeff) meet the same bar, and all three laws'
-parameters are identified from a reaction history.  The identification
+law's gradient with respect to (K, m, hg_coeff), the Johnson-Cook law's
+with respect to (A, B, hg_coeff) and the GMT law's with respect to (C1, n2,
+hg_coeff) meet the same bar, and every law's parameters are identified from
+a reaction history.  The identification
 workload at its own size (30^3 hexa, 2000 steps, checkpoint every 50) meets
 the 1e-5 bar against GPU and oracle differences, with the hg_coeff step
 chosen from a sweep (see the comment above its tests).
@@ -24,7 +25,7 @@ import pytest
 
 from oracle.wfref import RefDomain
 from weldformfem_amd import Domain, WFX_FP32_MIXED, WFX_FP64, workloads
-from weldformfem_amd.adjoint import JC_FIXED, CompressionProblem, identify, reaction_history
+from weldformfem_amd.adjoint import GMT_FIXED, JC_FIXED, CompressionProblem, identify, reaction_history
 
 pytestmark = pytest.mark.gpu
 
@@ -349,6 +350,71 @@ def test_johnson_cook_identification():
     say(f"  theta {theta} hidden {JC_HIDDEN} loss {loss:.3e} after {nit} iterations")
     assert loss < 1e-12 * losses[0]
     np.testing.assert_allclose(theta, JC_HIDDEN, rtol=1e-6)
+
+
+# ---- GMT law: theta = (C1, n2, hg_coeff) at a uniform 300 K -----------------
+# GMT_FIXED: rate exponent m = 0.02 and e_min = 0.001, so that the 200 steps
+# take elements past the strain clamp.  Over 600 steps the rate term er^m
+# makes the loss rough: oracle central differences of C1 at steps 1e-5, 1e-6,
+# 1e-7 disagree in sign and by orders of magnitude, with or without the
+# exp(I/es) factor, and agree to 1e-8 with the rate term off; the
+# identification therefore runs over the 200 steps of the gradient check,
+# where the differences agree (C1 and n2 to 1e-8 between steps 1e-5 and 1e-6;
+# hg_coeff to 3e-7 between 3e-3, 1e-3 and 3e-4, 1e-5 off at 1e-4).
+GMT_HIDDEN = np.array([3.5e9, 0.15, 0.05])
+GMT_THETA0 = np.array([3.2e9, 0.13, 0.07])
+
+
+def oracle_hist_gmt(theta, steps=STEPS):
+    r = RefDomain()
+    r.AddBoxLength([0, 0, 0], [1.0, 1.0, 1.0], 0.5 / CELLS)
+    bottom, top = workloads.box_layers(CELLS, CELLS, CELLS)
+    workloads.setup_compression(r, bottom, top, -1.0, hg_coeff=float(theta[2]))
+    m, g = workloads.STEEL, GMT_FIXED
+    r.setMaterialGMT(m["E"], m["nu"], m["rho"], m["sy0"],
+                     (g["n1"], float(theta[1]), float(theta[0]), g["C2"], g["m1"], g["m2"], g["I1"], g["I2"]),
+                     (g["e_min"], g["e_max"], g["er_min"], g["er_max"], g["T_min"], g["T_max"]))
+    r.setTemperature(g["temp"])
+    r.init()
+    return r.run(steps)
+
+
+@pytest.fixture(scope="module")
+def gmt_case():
+    pb = CompressionProblem(cells=CELLS, size=1.0, v_top=-1.0, nsteps=STEPS, every=EVERY, law="gmt")
+    target = pb.forward(GMT_HIDDEN)
+    L, g = pb.loss_and_grad(GMT_THETA0, target)
+    return pb, target, L, g
+
+
+def test_gmt_gradient_matches_finite_differences(gmt_case):
+    pb, target, L, g = gmt_case
+    say(f"case: GMT adjoint gradient d/d(C1, n2, hg) {g} vs central differences (GPU and oracle)")
+    ep = pb.domain.field("pl_strain")
+    assert ep.max() > GMT_FIXED["e_min"]  # some elements past the strain clamp
+    tgt = oracle_hist_gmt(GMT_HIDDEN)
+    assert np.max(np.abs(tgt - target)) <= 1e-9 * np.max(np.abs(tgt))
+    rel = [1e-6, 1e-6, 1e-3]
+    fd = central_fd(lambda t: gpu_loss(pb, t, target), GMT_THETA0, rel)
+
+    def oracle_loss(t):
+        r = oracle_hist_gmt(t) - tgt
+        return float(np.dot(r, r))
+
+    fdo = central_fd(oracle_loss, GMT_THETA0, rel)
+    err, erro = np.abs(g - fd) / np.abs(fd), np.abs(g - fdo) / np.abs(fdo)
+    say(f"  steps {rel} GPU fd {fd} rel err {err}; oracle fd {fdo} rel err {erro}")
+    assert np.all(np.abs(g) > 0)
+    assert np.all(err <= 1e-5) and np.all(erro <= 1e-5)
+
+
+def test_gmt_identification(gmt_case):
+    pb, target, _, _ = gmt_case
+    say("case: identification of (C1, n2, hg_coeff) from the 200-step reaction history, GMT law")
+    theta, loss, nit, losses = identify(pb, target, GMT_THETA0, scale=np.array([3e9, 0.15, 0.06]), max_iter=100)
+    say(f"  theta {theta} hidden {GMT_HIDDEN} loss {loss:.3e} after {nit} iterations")
+    assert loss < 1e-12 * losses[0]
+    np.testing.assert_allclose(theta, GMT_HIDDEN, rtol=1e-6)
 
 
 # ---- the identification workload at its own size ---------------------------

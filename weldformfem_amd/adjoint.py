@@ -5,10 +5,11 @@ The gradient comes from the library's hand-written adjoint
 recompute each segment, reverse step kernels).  ``ReactionHistory`` exposes
 the forward solve as a torch.autograd.Function of the parameter vector
 theta = (sigma_y0, Et, hg_coeff) for the bilinear law, (K, m, hg_coeff)
-for the Hollomon law (its plateau stress sigma_y0 held fixed) or (A, B,
+for the Hollomon law (its plateau stress sigma_y0 held fixed), (A, B,
 hg_coeff) for the Johnson-Cook law (n, C, eps_0, m and the uniform
-temperature held fixed), so a loss written in torch differentiates through
-the whole explicit solve.
+temperature held fixed) or (C1, n2, hg_coeff) for the GMT law (its other
+constants, ranges and the uniform temperature held fixed), so a loss written
+in torch differentiates through the whole explicit solve.
 """
 import numpy as np
 
@@ -16,11 +17,17 @@ from . import workloads
 from .domain import WFX_BILINEAR, WFX_FP64, WFX_HOLLOMON, Domain
 
 PARAMS = ("sy0", "Et", "hg_coeff")
-LAW_PARAMS = {"bilinear": PARAMS, "hollomon": ("K", "m", "hg_coeff"), "johnson_cook": ("A", "B", "hg_coeff")}
+LAW_PARAMS = {"bilinear": PARAMS, "hollomon": ("K", "m", "hg_coeff"), "johnson_cook": ("A", "B", "hg_coeff"),
+              "gmt": ("C1", "n2", "hg_coeff")}
 # Johnson-Cook constants held fixed while A and B vary: hardening exponent n,
 # rate constant C at reference rate eps_0, softening exponent m between the
 # transition and melting temperatures, and the specimen's uniform temperature
 JC_FIXED = dict(n=0.26, C=0.014, eps_0=1.0, m=1.03, T_m=1793.0, T_t=293.0, temp=500.0)
+# GMT constants held fixed while C1 and n2 vary: n1, C2, m1, m2, I1, I2, the
+# ranges of plastic strain, strain rate and temperature, and the specimen's
+# uniform temperature
+GMT_FIXED = dict(n1=-2e-4, C2=-1.5e-3, m1=0.0, m2=0.02, I1=-1e-5, I2=1e-4, e_min=0.001, e_max=3.0, er_min=1e-3,
+                 er_max=1e5, T_min=20.0, T_max=1200.0, temp=300.0)
 
 
 class CompressionProblem:
@@ -31,14 +38,16 @@ class CompressionProblem:
     face is the observed history.  law "bilinear": theta = (sigma_y0, Et,
     hg_coeff); "hollomon": theta = (K, m, hg_coeff) with sigma_y0 from mat;
     "johnson_cook": theta = (A, B, hg_coeff) with the constants jc (default
-    JC_FIXED)."""
+    JC_FIXED); "gmt": theta = (C1, n2, hg_coeff) with the constants gmt
+    (default GMT_FIXED)."""
 
     def __init__(self, cells=30, size=1.0, v_top=-1.0, nsteps=2000, every=50, precision=WFX_FP64,
-                 mat=workloads.STEEL, stream=0, mesh=None, law="bilinear", jc=None):
+                 mat=workloads.STEEL, stream=0, mesh=None, law="bilinear", jc=None, gmt=None):
         if law not in LAW_PARAMS:
             raise ValueError(f"law must be one of {sorted(LAW_PARAMS)}")
         self.law, self.params = law, LAW_PARAMS[law]
         self.jc = dict(JC_FIXED if jc is None else jc)
+        self.gmt = dict(GMT_FIXED if gmt is None else gmt)
         d = Domain(precision, stream=stream)
         if mesh is None:
             d.AddBoxLength([0.0, 0.0, 0.0], [size, size, size], 0.5 * size / cells)
@@ -61,6 +70,12 @@ class CompressionProblem:
             j = self.jc
             d.setMaterialJC(m["E"], m["nu"], m["rho"], a, b, j["n"], j["C"], j["eps_0"], j["m"], j["T_m"], j["T_t"])
             d.setTemperature(j["temp"])
+        elif self.law == "gmt":
+            g = self.gmt
+            d.setMaterialGMT(m["E"], m["nu"], m["rho"], m["sy0"],
+                             (g["n1"], b, a, g["C2"], g["m1"], g["m2"], g["I1"], g["I2"]),
+                             (g["e_min"], g["e_max"], g["er_min"], g["er_max"], g["T_min"], g["T_max"]))
+            d.setTemperature(g["temp"])
         else:
             d.setMaterial(WFX_BILINEAR, m["E"], m["nu"], m["rho"], a, b, 0.0)
         d.set("hg_coeff", hg)

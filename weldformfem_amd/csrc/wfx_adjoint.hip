@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kB) void a_elem1(int E, int N, adj::ElemConst K, co
   }
   volb[e] = B.volb;
   // rows: the law's two parameters (bilinear sy0, H; Hollomon Kh, mh;
-  // Johnson-Cook A, B), hg
+  // Johnson-Cook A, B; GMT C1, n2), hg
   grad[e] += K.law == 1 ? B.Kh : B.sy0;
   grad[Es + e] += K.law == 1 ? B.mh : B.H;
   grad[2 * Es + e] += B.hg;
@@ -352,10 +352,10 @@ template <> DevElem<float> &el_of<float>(wfx_ctx *c) { return c->e32; }
 static int check_supported(wfx_ctx *c) {
   const Stab &s = c->prm.stab;
   REQUIRE(c->prm.mat.model == WFX_BILINEAR || c->prm.mat.model == WFX_HOLLOMON ||
-              c->prm.mat.model == WFX_JOHNSON_COOK,
-          WFX_ERR_STATE, "the adjoint supports the bilinear, Hollomon and Johnson-Cook materials");
-  REQUIRE(c->prm.mat.model != WFX_JOHNSON_COOK || !c->temp_e, WFX_ERR_STATE,
-          "the Johnson-Cook adjoint needs one uniform temperature (no per-element temperatures)");
+              c->prm.mat.model == WFX_JOHNSON_COOK || c->prm.mat.model == WFX_GMT,
+          WFX_ERR_STATE, "the adjoint supports the bilinear, Hollomon, Johnson-Cook and GMT materials");
+  REQUIRE((c->prm.mat.model != WFX_JOHNSON_COOK && c->prm.mat.model != WFX_GMT) || !c->temp_e, WFX_ERR_STATE,
+          "the Johnson-Cook and GMT adjoints need one uniform temperature (no per-element temperatures)");
   REQUIRE(!(c->prm.cfl > 0.0), WFX_ERR_STATE, "the adjoint needs a fixed time step (wfx_set_cfl(ctx, 0))");
   REQUIRE(!c->thermal.on, WFX_ERR_STATE, "the adjoint does not cover thermal coupling");
   REQUIRE(!c->contact.on, WFX_ERR_STATE, "the adjoint does not cover contact");
@@ -543,6 +543,20 @@ static int adjoint_backward(wfx_ctx *c, hipStream_t st) {
     K.jsoft = 1.0 - pow((P.mat.temp - P.mat.T_t) / (P.mat.T_m - P.mat.T_t), P.mat.m);
     K.jcap = P.mat.max_edot;
     K.Emod = P.mat.E;
+  } else if (P.mat.model == WFX_GMT) {
+    K.law = 3;
+    // the temperature factors at the clamped uniform temperature (Material.cuh:441-444, :465-472)
+    const double Tc = std::min(std::max(P.mat.temp, P.mat.T_min), P.mat.T_max);
+    K.gexp = exp(P.mat.C2 * Tc);
+    K.ga = P.mat.C1 * exp(P.mat.C2 * Tc);
+    K.gn = P.mat.n1 * Tc + P.mat.n2;
+    K.gm = P.mat.m1 * Tc + P.mat.m2;
+    K.gi = P.mat.I1 * Tc + P.mat.I2;
+    K.ge_min = P.mat.e_min;
+    K.ge_max = P.mat.e_max;
+    K.ger_min = P.mat.er_min;
+    K.ger_max = P.mat.er_max;
+    K.jcap = P.mat.max_edot;
   }
   NodeAdj NP{P.dt, P.alpha, P.beta, P.gamma, c->has_fe, A->ng};
   for (int seg = A->nck - 1; seg >= 0; --seg) {

@@ -28,7 +28,10 @@
 //             Johnson-Cook (Material.cuh:378-411, rate and cap Mechanical.C:1700-1748):
 //             (sy0 + B ep^n)(1 + C ln(r/eps_0)) soft with r the von Mises
 //             rate of D (1e-5 at 0), tangent n B ep^(n-1)(1 + C ln(min(r, cap)/eps_0))
-//             soft for ep > 0, else 0.1 E
+//             soft for ep > 0, else 0.1 E; GMT (Material.cuh:412-473):
+//             C1 exp(C2 T) es^n exp(I/es) er^m with es, er the clamped plastic
+//             strain and rate, tangent C1 exp(C2 T) erc^m es^(n-2) (n es - I) exp(I/es)
+//             at the capped rate
 //   plastic (sy < st): dg = (st - sy)/(3G + H), tau2 = (1 - 3G dg/st) s, ep2 = ep + dg
 //   elastic:           tau2 = tau1, ep2 = ep           (Mechanical.C:1723-1762)
 //   sigma   = tau2 - p I                               (:1766)
@@ -61,6 +64,14 @@ struct ElemConst {
   // rate (1e-5 for a zero rate); tangent n B ep^(n-1)(1 + C ln(rc/eps_0)) soft
   // with the capped rate rc = min(r, cap) for ep > 0, 0.1 E at ep = 0
   double jB = 0.0, jn = 0.0, jC = 0.0, jeps0 = 1.0, jsoft = 1.0, jcap = 1.0e6, Emod = 0.0;
+  // law 3, GMT at a uniform temperature Tc (Material.cuh:412-473):
+  // sigma_y = ga es^gn exp(gi/es) er^gm with ga = C1 exp(C2 Tc) = C1 gexp,
+  // gn = n1 Tc + n2, gm = m1 Tc + m2, gi = I1 Tc + I2, es and er the plastic
+  // strain and rate clamped to [e_min, e_max] and [er_min, er_max]; tangent
+  // ga erc^gm es^(gn-2) (gn es - gi) exp(gi/es) at the capped rate (the cap
+  // jcap)
+  double ga = 0.0, gexp = 0.0, gn = 0.0, gm = 0.0, gi = 0.0;
+  double ge_min = 0.0, ge_max = 0.0, ger_min = 0.0, ger_max = 0.0;
 };
 
 template <int NN>
@@ -85,6 +96,8 @@ struct ElemFwd {
   // Johnson-Cook: effective rate, the rate the yield stress takes, its log
   // term, ep^n, the capped rate and the tangent's log term
   double edot, sr, lterm, pw, edc, ltc;
+  // GMT: clamped strain, rate and capped rate, er^gm, erc^gm, exp(gi/es)
+  double ges, ger, gerc, gpw, gpwc, gex;
   // forces
   double hm[3][4], ch0, ch, fnet[NN][3];
 };
@@ -203,6 +216,21 @@ WFX_HD void elem_forward(const ElemConst &k, const ElemIn<NN> &in, ElemFwd<NN> &
       F.ltc = 0.0;
       F.Ht = k.Emod * 0.1;
     }
+  } else if (k.law == 3) {
+    F.hs = 0.0;
+    F.hpow = 0;
+    const double *D = F.D;
+    F.edot = sqrt(0.5 * ((D[0] - D[1]) * (D[0] - D[1]) + (D[1] - D[2]) * (D[1] - D[2]) + (D[2] - D[0]) * (D[2] - D[0])) +
+                  3.0 * (D[3] * D[3] + D[4] * D[4] + D[5] * D[5]));
+    F.ges = in.ep < k.ge_min ? k.ge_min : (in.ep > k.ge_max ? k.ge_max : in.ep);
+    F.ger = F.edot < k.ger_min ? k.ger_min : (F.edot > k.ger_max ? k.ger_max : F.edot);
+    F.gpw = pow(F.ger, k.gm);
+    F.gex = exp(k.gi / F.ges);
+    F.sy = k.ga * pow(F.ges, k.gn) * F.gex * F.gpw;
+    F.edc = (k.jcap < F.edot) ? k.jcap : F.edot;
+    F.gerc = F.edc < k.ger_min ? k.ger_min : (F.edc > k.ger_max ? k.ger_max : F.edc);
+    F.gpwc = (F.gerc == F.ger) ? F.gpw : pow(F.gerc, k.gm);
+    F.Ht = k.ga * F.gpwc * pow(F.ges, k.gn - 2.0) * (-k.gi + F.ges * k.gn) * F.gex;
   } else {
     F.hs = 0.0;
     F.hpow = 0;
@@ -360,6 +388,20 @@ WFX_HD void elem_reverse(const ElemConst &k, const ElemIn<NN> &in, const ElemFwd
         // the tangent at ln 0: no derivative)
         if (F.edc == F.edot && F.edot != 0.0) edb += Htb * k.jn * k.jB * pnm1 * k.jsoft * k.jC / F.edot;
       }  // ep = 0: Ht = 0.1 E
+    } else if (k.law == 3) {
+      // rows C1 (sy0) and n2 (H); the clamps pass nothing outside their ranges
+      const double es = F.ges, q = k.gn * es - k.gi;
+      const double pn = pow(es, k.gn), pn2 = pow(es, k.gn - 2.0), les = log(es);
+      const double hq = F.gpwc * F.gex;  // Ht = ga hq pn2 q
+      B.sy0 = syb * k.gexp * pn * F.gex * F.gpw + Htb * k.gexp * hq * pn2 * q;
+      B.H = syb * F.sy * les + Htb * k.ga * hq * pn2 * (les * q + es);
+      if (in.ep > k.ge_min && in.ep < k.ge_max) {
+        // d sy / d es = sy q / es^2;  d Ht / d es = ga hq es^(gn-3) ((gn-2) q + gn es - q gi / es)
+        B.ep += syb * F.sy * q / (es * es) +
+                Htb * k.ga * hq * pow(es, k.gn - 3.0) * ((k.gn - 2.0) * q + k.gn * es - q * k.gi / es);
+      }
+      if (F.edot > k.ger_min && F.edot < k.ger_max) edb += syb * F.sy * k.gm / F.ger;
+      if (F.edc == F.edot && F.edc > k.ger_min && F.edc < k.ger_max) edb += Htb * F.Ht * k.gm / F.gerc;
     } else {
       B.H = Htb + syb * in.ep;
       B.sy0 = syb;

@@ -347,9 +347,9 @@ class Domain:
 
     def adjointBackward(self, hist_bar, per_element=False):
         """dL/d(sigma_y0, Et, hg_coeff) (bilinear law), dL/d(K, m,
-        hg_coeff) (Hollomon) or dL/d(A, B, hg_coeff) (Johnson-Cook) for the
-        cotangent hist_bar = dL/dR of the recorded history (shape as
-        adjointForward returned)"""
+        hg_coeff) (Hollomon), dL/d(A, B, hg_coeff) (Johnson-Cook) or dL/d(C1,
+        n2, hg_coeff) (GMT) for the cotangent hist_bar = dL/dR of the recorded
+        history (shape as adjointForward returned)"""
         hb = np.ascontiguousarray(hist_bar, dtype=np.float64).reshape(-1)
         ng = max(self.reactionGroupCount(), 1)
         if hb.size != self._adj_steps * ng:
@@ -361,7 +361,8 @@ class Domain:
                                            *[(a.ctypes.data if a is not None else None) for a in el],
                                            self.stream))
         names = {WFX_HOLLOMON: ("K", "m", "hg_coeff"),
-                 WFX_JOHNSON_COOK: ("A", "B", "hg_coeff")}.get(self.law, ("sy0", "Et", "hg_coeff"))
+                 WFX_JOHNSON_COOK: ("A", "B", "hg_coeff"),
+                 WFX_GMT: ("C1", "n2", "hg_coeff")}.get(self.law, ("sy0", "Et", "hg_coeff"))
         out = dict(zip(names, g))
         if per_element:
             out.update({f"{k}_elem": a for k, a in zip(names, el)})
