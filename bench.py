@@ -620,9 +620,10 @@ def pmc_probe(args):
     """Kernels for hardware-counter runs (rocprofv3 --pmc): a plain 1 GiB
     device-to-device copy whose FETCH_SIZE/WRITE_SIZE calibrate the counters,
     then, on the benchmark mesh at the given block sizes, the headline's
-    window: --pre-steps fused steps (past first yield), then three fused
-    steps of each path (the last three dispatches of each kernel are the
-    ones summarised) and one per-step API step."""
+    window: --pre-steps fused steps (past first yield), then a run of five
+    fused steps of each path (the inner three dispatches of each kernel are
+    the ones summarised: a run's first and last node passes move extra
+    bytes) and one per-step API step."""
     import torch
 
     from weldformfem_amd import WFX_FP32_MIXED, WFX_FP64
@@ -641,7 +642,7 @@ def pmc_probe(args):
         if bs:
             d.setBlockSizes([int(q) for q in bs.split(",")])
         d.runFused(args.pre_steps, reaction=False)
-        d.runFused(3, reaction=False)
+        d.runFused(5, reaction=False)
         if prec == WFX_FP64:
             d.run(1, reaction=False)
         torch.cuda.synchronize()
@@ -660,8 +661,9 @@ def pmc_summary(fetch_csv, write_csv, probe_log):
     """Markdown table of HBM traffic per dispatch of the fused kernels from two
     rocprofv3 --pmc runs of --pmc-probe (FETCH_SIZE, WRITE_SIZE; KiB per
     dispatch): each counter scaled by what it reads for the probe's 1 GiB
-    device copy, the median over the last three dispatches of each kernel,
-    against the byte model the probe printed."""
+    device copy, the median over the inner three of each kernel's last five
+    dispatches (the probe's last run), against the byte model the probe
+    printed."""
     import csv
     import statistics
 
@@ -685,7 +687,8 @@ def pmc_summary(fetch_csv, write_csv, probe_log):
                 models[m["precision"]] = m["bytes"]
     lines = [f"Calibration: the probe's 1 GiB device copy reads {gib_kib / cf:.0f} KiB of FETCH_SIZE and "
              f"{gib_kib / cw:.0f} KiB of WRITE_SIZE: reads = {cf:.3f} x FETCH_SIZE, writes = {cw:.3f} x WRITE_SIZE.",
-             "Per-dispatch values: median of each kernel's last three dispatches (past first yield).", "",
+             "Per-dispatch values: median of the inner three of each kernel's last five dispatches (a five-step "
+             "run past first yield; its first and last node passes move extra bytes).", "",
              "| kernel | reads MB | writes MB | HBM MB | model MB | HBM / model |", "|---|---:|---:|---:|---:|---:|"]
     out = {}
     kernels = {"f1_volume": "f1_cluster<", "f2_nodal": "f2_nodal<", "f3_element": "f3_cluster<",
@@ -695,8 +698,8 @@ def pmc_summary(fetch_csv, write_csv, probe_log):
         if short is None or name not in write:
             continue
         prec = "fp64" if "<double" in name else "fp32_mixed"
-        rd = statistics.median(fetch[name][-3:]) * cf * 1024 / 1e6
-        wr = statistics.median(write[name][-3:]) * cw * 1024 / 1e6
+        rd = statistics.median(fetch[name][-4:-1]) * cf * 1024 / 1e6
+        wr = statistics.median(write[name][-4:-1]) * cw * 1024 / 1e6
         mb = models.get(prec, {}).get(short)
         ratio = (rd + wr) / (mb / 1e6) if mb else float("nan")
         out.setdefault(prec, {})[short] = (rd + wr) * 1e6
