@@ -165,6 +165,9 @@ static int setup_device(wfx_ctx *c, const ClusterPlan &plan) {
   A(c->scalars, 64) A(c->status, 16)
   c->ms.ncl = plan.clusters();
   c->ms.nslot = plan.slots();
+  // more than 12 elements per node on average (tetrahedra: about 23): the
+  // fused nodal volume sums take several lanes per node
+  c->f2_lanes = (long)n * E > 12L * (long)N ? 1 : 0;
   if (plan.max_slots > slot_cap((int)n)) {  // the element pass stages at most slot_cap slots
     set_error("element cluster with " + std::to_string(plan.max_slots) + " nodes");
     free_all(c);

@@ -166,6 +166,85 @@ __global__ __launch_bounds__(BS) void f2_nodal(int N, int nodxelem, const int *_
   mdiag[i] = mass;
 }
 
+// Nodes with many elements (tetrahedral meshes: about 23 per node) are
+// summed by kF2Lanes lanes each: lane q takes the node's CSR entries q,
+// q + kF2Lanes, ... in order, kF2Width at a time with every load issued, and
+// the lanes' partial sums are combined by a butterfly, which leaves the same
+// total in every lane.  V0_n goes through the same function, so the ANP ratio
+// is exactly 1 wherever vol equals vol0.  val(e) gives entry e's term.
+constexpr int kF2Lanes = 8, kF2Width = 4;
+template <int L, typename F>
+__device__ __forceinline__ double lane_csr_sum(int o, int c, int q, const int *__restrict__ nodel_pk, F val) {
+  constexpr int W = kF2Width;
+  double s = 0.0;
+  for (int k0 = 0; k0 < c; k0 += W * L) {
+    int e[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const int k = k0 + q + j * L;
+      e[j] = nodel_pk[o + (k < c ? k : c - 1)] >> 3;
+    }
+    double v[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) v[j] = val(e[j]);
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+      if (k0 + q + j * L < c) s += v[j];
+  }
+#pragma unroll
+  for (int m = 1; m < L; m <<= 1) s += __shfl_xor(s, m, L);
+  return s;
+}
+
+// V0_n with kF2Lanes lanes per node (grid of N * kF2Lanes threads)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void f0_vsum0_lanes(int N, const int *__restrict__ off,
+                                                         const int *__restrict__ cnt,
+                                                         const int *__restrict__ nodel_pk,
+                                                         const T *__restrict__ vol0,
+                                                         double *__restrict__ vsum0) {
+  constexpr int L = kF2Lanes;
+  const long g = (long)xcd_block() * kBlock + threadIdx.x;
+  const long n = g / L;
+  const int q = (int)(g % L), i = (int)(n < N ? n : N - 1);  // every lane reaches the butterfly
+  const int o = off[i], c = cnt[i];
+  const double s0 = lane_csr_sum<L>(o, c, q, nodel_pk, [&](int e) { return (double)vol0[e]; });
+  if (q == 0 && n < N) vsum0[i] = s0;
+}
+
+// F2 with kF2Lanes lanes per node: the nodal volume sum, ANP ratio and mass
+// of f2_nodal
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void f2_nodal_lanes(int N, int nodxelem, const int *__restrict__ off,
+                                                     const int *__restrict__ cnt,
+                                                     const int *__restrict__ nodel_pk,
+                                                     const T *__restrict__ vol, const T *__restrict__ rho,
+                                                     const double *__restrict__ vsum0,
+                                                     double *__restrict__ ratio, double *__restrict__ mdiag,
+                                                     double rho_u) {
+  constexpr int L = kF2Lanes;
+  const long g = (long)xcd_block() * BS + threadIdx.x;
+  const long n = g / L;
+  const int q = (int)(g % L), i = (int)(n < N ? n : N - 1);  // every lane reaches the butterflies
+  const int o = off[i], c = cnt[i];
+  const double v0 = vsum0[i];  // loaded with the offsets, not after the sum
+  const double s = lane_csr_sum<L>(o, c, q, nodel_pk, [&](int e) { return (double)vol[e]; });
+  double vn = s;
+  vn /= nodxelem;
+  const double dc = (double)c;
+  double mass = 0.0;
+  if (rho_u > 0.0) {  // every term is the same value: formed once, added c times
+    const double term = 1.0 * rho_u * vn / dc;
+    for (int k = 0; k < c; ++k) mass += term;
+  } else {
+    mass = lane_csr_sum<L>(o, c, q, nodel_pk, [&](int e) { return 1.0 * (double)rho[e] * vn / dc; });
+  }
+  if (q != 0 || n >= N) return;
+  if constexpr (std::is_same<T, double>::value) ratio[i] = s / v0;
+  else reinterpret_cast<float *>(ratio)[i] = (float)(s / v0 - 1.0);
+  mdiag[i] = mass;
+}
+
 struct FusedParams {
   double dt;
   MatParams mp;
@@ -1471,8 +1550,13 @@ hipError_t launch_reaction_f32(wfx_ctx *c, const float *fi32, double *dev_out, h
 
 template <typename T>
 hipError_t launch_vsum0(wfx_ctx *c, hipStream_t s) {
-  hipLaunchKernelGGL((f0_vsum0<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N, c->ms.nodel_offset,
-                     c->ms.nodel_count, c->ms.nodel_pk, elem<T>(c).vol0, c->nd.voln0_sum);
+  if (c->f2_lanes) {
+    hipLaunchKernelGGL((f0_vsum0_lanes<T>), dim3(nblocks((long)c->N * kF2Lanes)), dim3(kBlock), 0, s, c->N,
+                       c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel_pk, elem<T>(c).vol0, c->nd.voln0_sum);
+  } else {
+    hipLaunchKernelGGL((f0_vsum0<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N, c->ms.nodel_offset,
+                       c->ms.nodel_count, c->ms.nodel_pk, elem<T>(c).vol0, c->nd.voln0_sum);
+  }
   return hipGetLastError();
 }
 
@@ -1560,9 +1644,16 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
       return e;
     }
   } else if (phase == 2) {
-    hipLaunchKernelGGL((f2_nodal<T, BS>), dim3(gN), dim3(BS), 0, s, N, c->n, c->ms.nodel_offset,
-                       c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.vol, el.rho, c->nd.voln0_sum,
-                       c->nd.anp_ratio, c->nd.mdiag, c->rho_uniform);
+    if (c->f2_lanes) {
+      const int gL = (int)(((long)N * kF2Lanes + BS - 1) / BS);
+      hipLaunchKernelGGL((f2_nodal_lanes<T, BS>), dim3(gL), dim3(BS), 0, s, N, c->n, c->ms.nodel_offset,
+                         c->ms.nodel_count, c->ms.nodel_pk, el.vol, el.rho, c->nd.voln0_sum, c->nd.anp_ratio,
+                         c->nd.mdiag, c->rho_uniform);
+    } else {
+      hipLaunchKernelGGL((f2_nodal<T, BS>), dim3(gN), dim3(BS), 0, s, N, c->n, c->ms.nodel_offset,
+                         c->ms.nodel_count, c->ms.nodel_pk, c->ms.nodel_pad, el.vol, el.rho, c->nd.voln0_sum,
+                         c->nd.anp_ratio, c->nd.mdiag, c->rho_uniform);
+    }
   } else if (phase == 3) {
     FusedParams fp;
     fp.dt = P.dt;

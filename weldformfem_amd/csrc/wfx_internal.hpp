@@ -225,6 +225,7 @@ struct wfx_ctx {
   int fused_write_ud = 1;    // and stores it every step (cleared inside runs of a fixed time step)
   int fused_run_first = 1;   // the fused step is the first of a run
   int fused_run_last = 1;    // the fused step is the last of a run
+  int f2_lanes = 0;          // the nodal volume sums take several lanes per node (many elements per node)
   double *scalars = nullptr;      // device scratch for reductions (reaction force)
   // this context's device's address of the library's static scratch array
   // (the allocation-free entry point's reduction partials)
