@@ -27,14 +27,17 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 I4 = 4
 
 
-def fused_bytes(n, s, slots_per_elem, clusters_per_elem, hg=True, rho_uniform=True, over8_per_node=0.0):
+def fused_bytes(n, s, slots_per_elem, clusters_per_elem, hg=True, rho_uniform=True, over8_per_node=0.0,
+                f2_lanes=False):
     """Algorithmic bytes per element and per node of the fused kernels: each
     array a kernel touches is moved once, gathered nodal data once per node.
     s = element real size (8 FP64, 4 FP32-mixed); nodal positions are
     double, v, prev_a, u_dt, fi and the ANP ratio are s bytes.  With a uniform density (the
     benchmark's) no density is read.  slots_per_elem = (cluster, node) slots
     per element, clusters_per_elem = clusters per element; over8_per_node =
-    share of nodes with more than 8 elements (their CSR offset is read).
+    share of nodes with more than 8 elements (their CSR offset is read);
+    f2_lanes: the nodal-volume pass reads every CSR entry (meshes with more
+    than 12 elements per node) instead of the padded first-8 table.
     Both element passes stage their cluster's slots: they read each
     element's nodes as slot indices (uint16 per element-node) and each
     slot's node id (int32), and gather the nodal data once per node.  The
@@ -62,7 +65,8 @@ def fused_bytes(n, s, slots_per_elem, clusters_per_elem, hg=True, rho_uniform=Tr
         # vol gathered (V_n once per run), [rho]; padded first-8 table, count,
         # [offset], V0_n read, mass and ANP ratio written (FP32-mixed stores the
         # ratio's deviation from 1 as a float)
-        "f2_nodal": (s + rho * s, 8 * I4 + I4 + 4 * over8_per_node + 8 + 8 + s),
+        "f2_nodal": ((s + rho * s + n * I4, 2 * I4 + 8 + 8 + s) if f2_lanes else
+                     (s + rho * s, 8 * I4 + I4 + 4 * over8_per_node + 8 + 8 + s)),
         # gathers x, v, ratio
         "f3_element": (f3_elem, 3 * 8 + 3 * sn + s),
         # [slot ids] and partial forces; offsets, mass, BC slot; v, prev_a
@@ -79,7 +83,7 @@ def domain_fused_bytes(d, s, hg=True):
     N, E, n = d.sizes()
     cnt = d.field("nodel_count")
     return fused_bytes(n, s, d.get("slot_count") / E, d.get("cluster_count") / E, hg=hg,
-                       over8_per_node=float(np.mean(cnt > 8)))
+                       over8_per_node=float(np.mean(cnt > 8)), f2_lanes=n * E > 12 * N)
 
 
 def api_bytes(n, s, hg=True):
@@ -325,9 +329,11 @@ def config_legs(torch, log):
         d.init()
         N, E, _ = d.sizes()
         d.runFused(20, reaction=False)
+        bs = d.autotune(reps=5)
         t = timed(lambda: d.runFused(500, reaction=False), torch) / 500
         fb = domain_fused_bytes(d, 8 if prec == WFX_FP64 else 4, hg=False)
         out[f"cylinder_tets_{tag}"] = {"elements": E, "nodes": N, "ms_per_step": t * 1e3, "value": E / t,
+                                       "block_sizes": bs,
                                        "step_algorithmic_bytes_per_element": total(fb, E, N) / E,
                                        "step_hbm_frac": total(fb, E, N) / t / HBM_PEAK}
         log(f"tet cylinder {tag}: {t * 1e3:.3f} ms/step, {E / t:.3e} element-steps/s")
