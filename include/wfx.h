@@ -304,8 +304,8 @@ int wfx_graph_launch(wfx_ctx *ctx, double *reaction_hist, wfx_stream s);
  * wfx_step_fused, storing the state every checkpoint_every steps; writes the
  * reaction history [nsteps][ngroups] (may be NULL).  Bilinear, Hollomon,
  * Johnson-Cook or GMT material (the rate laws at one uniform temperature),
- * fixed time step, the ANP pressure without stabilisation terms, no thermal
- * coupling or contact.
+ * a fixed time step or (FP64) the per-step CFL time step, the ANP pressure
+ * without stabilisation terms, no thermal coupling or contact.
  * Backward: given dL/dR for that history, recomputes each checkpoint segment
  * and runs the hand-written reverse step over it; grad[3] receives, for the
  * bilinear law, dL/d(sigma_y0) (summed over elements: a uniform yield

@@ -159,10 +159,11 @@ static int vjp_mode(int count, int want_plastic, int law) {
     elem_reverse<NN>(k, in, F, fb, tb, epb, B);
     double Xv[NN][3];
     vol_reverse<NN>(in.X, B.volb, Xv);
-    // directions: all inputs at once, then every parameter alone
-    for (int dir = 0; dir < 6; ++dir) {
+    // directions: all inputs at once, then every parameter alone, then the
+    // step length
+    for (int dir = 0; dir < 7; ++dir) {
       ElemIn<NN> d{};
-      double dhg = 0.0, dK = 0.0, dm = 0.0, dB = 0.0, dC1 = 0.0, dn = 0.0;
+      double dhg = 0.0, dK = 0.0, dm = 0.0, dB = 0.0, dC1 = 0.0, dn = 0.0, ddt = 0.0;
       if (dir < 3) {
         for (int a = 0; a < NN; ++a) {
           for (int c = 0; c < 3; ++c) {
@@ -182,13 +183,15 @@ static int vjp_mode(int count, int want_plastic, int law) {
         else if (law == 2) dB = 1.0e-3 * k.jB;
         else if (law == 3) dn = 0.01;
         else d.H = 1.0e9;
-      } else {
+      } else if (dir == 5) {
         dhg = 0.05;
+      } else {
+        ddt = k.dt;
       }
       // Johnson-Cook: the B row sits in B.H
       // GMT: the C1 row sits in B.sy0, the n2 row in B.H
       double ad = B.ep * d.ep + B.sy0 * (law == 3 ? dC1 : d.sy0) + B.H * (law == 2 ? dB : (law == 3 ? dn : d.H)) +
-                  B.Kh * dK + B.mh * dm + B.hg * dhg;
+                  B.Kh * dK + B.mh * dm + B.hg * dhg + B.dt * ddt;
       for (int a = 0; a < NN; ++a) {
         for (int c = 0; c < 3; ++c) ad += (B.X[a][c] + Xv[a][c]) * d.X[a][c] + B.V[a][c] * d.V[a][c];
         ad += B.r[a] * d.r[a];
@@ -219,6 +222,7 @@ static int vjp_mode(int count, int want_plastic, int law) {
         kk.jB += s * dB;
         kk.ga += s * dC1 * kk.gexp;  // ga = C1 exp(C2 Tc)
         kk.gn += s * dn;             // gn = n1 Tc + n2
+        kk.dt += s * ddt;
         return phi<NN>(kk, p, fb, tb, epb, pl);
       };
       int p1 = 0, p2 = 0;

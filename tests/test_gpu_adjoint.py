@@ -417,6 +417,64 @@ def test_gmt_identification(gmt_case):
     np.testing.assert_allclose(theta, GMT_HIDDEN, rtol=1e-6)
 
 
+# ---- per-step CFL time step: dt = 0.3 min edge / (c + max |v'|) every step ---
+# dt then depends on the positions (through the smallest element edge) and on
+# the predicted velocities (through the largest speed); the adjoint carries
+# each step's dt cotangent to the edge and the node that set it.  Oracle
+# differences of the same loss agree across steps 1e-4 .. 1e-6 (sigma_y0 to
+# 1e-8, Et to 5e-7, hg_coeff to 2e-6), so fd_steps' choice holds.
+CFL_FACTOR = 0.3
+
+
+def oracle_hist_cfl(theta):
+    r = RefDomain()
+    r.AddBoxLength([0, 0, 0], [1.0, 1.0, 1.0], 0.5 / CELLS)
+    bottom, top = workloads.box_layers(CELLS, CELLS, CELLS)
+    m = dict(workloads.STEEL, sy0=float(theta[0]), Et=float(theta[1]))
+    workloads.setup_compression(r, bottom, top, -1.0, m, hg_coeff=float(theta[2]))
+    r.setCFL(CFL_FACTOR)
+    r.init()
+    return r.run(STEPS)
+
+
+@pytest.fixture(scope="module")
+def cfl_case():
+    pb = CompressionProblem(cells=CELLS, size=1.0, v_top=-1.0, nsteps=STEPS, every=EVERY, cfl=CFL_FACTOR)
+    target = pb.forward(HIDDEN)
+    L, g = pb.loss_and_grad(THETA0, target)
+    return pb, target, L, g
+
+
+def test_cfl_gradient_matches_finite_differences(cfl_case):
+    pb, target, L, g = cfl_case
+    say(f"case: adjoint gradient with the per-step CFL time step {g} vs central differences (GPU and oracle)")
+    assert pb.domain.field("pl_strain").max() > 0
+    assert pb.domain.getDT() != pb.dt  # the step length moved
+    tgt = oracle_hist_cfl(HIDDEN)
+    assert np.max(np.abs(tgt - target)) <= 1e-9 * np.max(np.abs(tgt))
+    rel = fd_steps(THETA0, g, L)
+    fd = central_fd(lambda t: gpu_loss(pb, t, target), THETA0, rel)
+
+    def oracle_loss(t):
+        r = oracle_hist_cfl(t) - tgt
+        return float(np.dot(r, r))
+
+    fdo = central_fd(oracle_loss, THETA0, rel)
+    err, erro = np.abs(g - fd) / np.abs(fd), np.abs(g - fdo) / np.abs(fdo)
+    say(f"  steps {rel} GPU fd {fd} rel err {err}; oracle fd {fdo} rel err {erro}")
+    assert np.all(err <= 1e-5) and np.all(erro <= 1e-5)
+
+
+def test_cfl_gradient_independent_of_checkpoint_interval(cfl_case):
+    pb, target, L, g = cfl_case
+    say("case: per-step CFL adjoint, checkpoint intervals 1 and 7 vs 30")
+    for every in (1, 7):
+        q = CompressionProblem(cells=CELLS, size=1.0, v_top=-1.0, nsteps=STEPS, every=every, cfl=CFL_FACTOR)
+        L2, g2 = q.loss_and_grad(THETA0, target)
+        assert L2 == L
+        np.testing.assert_array_equal(g2, g)
+
+
 # ---- the identification workload at its own size ---------------------------
 # 30^3 hexa (27,000 elements), 2000 steps, checkpoint every 50, theta0 at the
 # nominal values, target at the seeded hidden parameters.  hg_coeff moves the

@@ -156,9 +156,10 @@ def test_calc_cfl_dt_entry_point():
     assert g.getDT() == dt
 
 
-def test_adjoint_rejects_per_step_dt():
+def test_adjoint_rejects_per_step_dt_in_mixed_precision():
+    # the FP64 adjoint covers the per-step time step (tests/test_gpu_adjoint.py)
     from weldformfem_amd._lib import WfxError
 
-    g, _ = _pair(_hexa20)
-    with pytest.raises(WfxError, match="fixed time step"):
+    g, _ = _pair(_hexa20, precision=WFX_FP32_MIXED)
+    with pytest.raises(WfxError, match="needs FP64"):
         g.adjointForward(10, checkpoint_every=5)

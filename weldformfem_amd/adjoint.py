@@ -39,10 +39,11 @@ class CompressionProblem:
     hg_coeff); "hollomon": theta = (K, m, hg_coeff) with sigma_y0 from mat;
     "johnson_cook": theta = (A, B, hg_coeff) with the constants jc (default
     JC_FIXED); "gmt": theta = (C1, n2, hg_coeff) with the constants gmt
-    (default GMT_FIXED)."""
+    (default GMT_FIXED).  cfl > 0: the per-step CFL time step with that
+    factor (FP64), starting from the fixed step."""
 
     def __init__(self, cells=30, size=1.0, v_top=-1.0, nsteps=2000, every=50, precision=WFX_FP64,
-                 mat=workloads.STEEL, stream=0, mesh=None, law="bilinear", jc=None, gmt=None):
+                 mat=workloads.STEEL, stream=0, mesh=None, law="bilinear", jc=None, gmt=None, cfl=0.0):
         if law not in LAW_PARAMS:
             raise ValueError(f"law must be one of {sorted(LAW_PARAMS)}")
         self.law, self.params = law, LAW_PARAMS[law]
@@ -56,6 +57,8 @@ class CompressionProblem:
             x, elnod, n, bottom, top = mesh
             d.CreateFromMesh(x, elnod, n)
         self.dt = workloads.setup_compression(d, bottom, top, v_top, mat, hg_coeff=0.06)
+        if cfl > 0.0:
+            d.setCFL(cfl)
         self.domain, self.mat, self.nsteps, self.every = d, dict(mat), int(nsteps), int(every)
         self.x0 = d.field("x")
         self.theta = None

@@ -53,6 +53,11 @@ struct Adjoint {
   int *rgroup = nullptr;
   std::vector<std::pair<void *, size_t>> final_bufs;
   std::vector<void *> final_saved;
+  // per-step CFL time step: each step's dt (device and host), the node and
+  // element shares of its cotangent, and the arg-min edge / arg-max speed
+  bool cfl = false;
+  double *dts = nullptr, *dtb = nullptr, *cfl_loc = nullptr;
+  std::vector<double> dts_h;
 };
 
 void adjoint_free(wfx_ctx *c) {
@@ -95,16 +100,18 @@ __global__ void k_ck_load(long N, long E, const double *src, double *x, T *v, T 
   else if (i < 4 * n3 + 6 * E) tau[i - 4 * n3] = (T)src[i];
   else if (i < 4 * n3 + 7 * E) ep[i - 4 * n3 - 6 * E] = (T)src[i];
 }
-// step record [x 3N][v 3N][fi 3N][m N][r N][tau 6E][ep E]; the first part
-// before the step's kernels, the node-pass values after them
+// step record [x 3N][v 3N][fi 3N][m N][r N][tau 6E][ep E][pa 3N]; the first
+// part before the step's kernels, the node-pass values after them
 template <typename T>
-__global__ void k_rec_pre(long N, long E, const double *x, const T *v, const T *tau, const T *ep, double *dst) {
+__global__ void k_rec_pre(long N, long E, const double *x, const T *v, const T *tau, const T *ep, const T *pa,
+                          double *dst) {
   long i = tid();
   const long n3 = 3 * N, o = 11 * N;
   if (i < n3) dst[i] = x[i];
   else if (i < 2 * n3) dst[i] = (double)v[i - n3];
   else if (i < 2 * n3 + 6 * E) dst[o + i - 2 * n3] = (double)tau[i - 2 * n3];
   else if (i < 2 * n3 + 7 * E) dst[o + i - 2 * n3] = (double)ep[i - 2 * n3 - 6 * E];
+  else if (i < 3 * n3 + 7 * E) dst[o + i - 2 * n3] = (double)pa[i - 2 * n3 - 7 * E];
 }
 // (the ANP ratio as the fused node pass leaves it: double, or for FP32-mixed
 // 1 + a float deviation in the same buffer)
@@ -151,18 +158,23 @@ struct NodeAdj {
 };
 
 // node-pass reverse of step k; Z-bar of boundary k+1 in, of boundary k out
-// (partial: the element gathers are added by a_node3)
-__global__ void a_node1(int N, NodeAdj P, const double *rec, const double *fe, const int *bc_slot,
+// (partial: the element gathers are added by a_node3).  dtb (per-step CFL
+// time step, else null): node i's share of the step length's cotangent, from
+// vv = v' + gamma dt ac, ud = ud + beta dt^2 ac and the prediction
+// ud' = dt (vv + (0.5 - beta) dt ac), v'' = vv + (1 - gamma) dt ac, with ac the
+// corrected acceleration (1/(1-alpha)) (a - alpha pa)
+__global__ void a_node1(int N, int E, NodeAdj P, const double *rec, const double *fe, const int *bc_slot,
                         const int *bc_mask, const int *rgroup, const double *hbar_k, double *xb, double *vb,
-                        double *udb, double *pab, double *fib, double *mb) {
+                        double *udb, double *pab, double *fib, double *mb, double *dtb) {
   int i = (int)tid();
   if (i >= N) return;
   const long Ns = N;
   const double *fi = rec + 6 * Ns, m = rec[9 * Ns + i];
+  const double *vrec = rec + 3 * Ns, *parec = rec + 11 * Ns + 7L * E;
   const int slot = bc_slot[i];
   const int mask = slot >= 0 ? bc_mask[slot] : 0;
   const double dt = P.dt, ia = 1.0 / (1.0 - P.alpha);
-  double mbar = 0.0;
+  double mbar = 0.0, dtbar = 0.0;
   for (int d = 0; d < 3; ++d) {
     const long id = d * Ns + i;
     const bool free_d = !((mask >> d) & 1);
@@ -181,15 +193,121 @@ __global__ void a_node1(int N, NodeAdj P, const double *rec, const double *fe, c
     if (d == 2 && rgroup[i] >= 0) fb += hbar_k[rgroup[i]];
     fib[id] = fb;
     mbar -= abar * a / m;
+    if (dtb) {
+      const double ac = ia * ((free_d ? a : 0.0) - P.alpha * parec[id]);
+      const double vv = free_d ? vrec[id] + P.gamma * dt * ac : vrec[id];  // an imposed v' is its value
+      dtbar += udp * (vv + 2.0 * (0.5 - P.beta) * dt * ac) + vpm * (1.0 - P.gamma) * ac +
+               vppr * P.gamma * ac + xbar * 2.0 * P.beta * dt * ac;
+    }
   }
   mb[i] = mbar;
+  if (dtb) dtb[i] = dtbar;
+}
+
+// Per-step CFL time step dt = cfl m / (cs + v), m the smallest edge of the
+// first four nodes of every element (calcMinEdgeLength's edges, Domain_d.C:
+// 2224-2260) and v the largest speed of the predicted velocities
+// (Solver_explicit.C:576-600), both at the step's start.  One block finds an
+// element edge of length m and a node of speed v (ties: the lowest index)
+// from the step record: loc = {m, 6 e + edge, v, node}.
+constexpr int kCflB = 1024;
+__device__ const int kEdgePairs[6][2] = {{1, 0}, {2, 0}, {3, 0}, {2, 1}, {3, 1}, {3, 2}};
+__global__ __launch_bounds__(kCflB) void a_cfl_locate(int E, int N, const int *elnod, const double *rec,
+                                                       double *loc) {
+  __shared__ double sv[kCflB];
+  __shared__ long si[kCflB];
+  const long Ns = N, Es = E;
+  const double *x = rec, *v = rec + 3 * Ns;
+  const int t = threadIdx.x;
+  double m = 1.0e300;
+  long mi = -1;
+  for (int e = t; e < E; e += kCflB) {
+    double P[4][3];
+    for (int a = 0; a < 4; ++a) {
+      const int g = elnod[a * Es + e];
+      for (int c = 0; c < 3; ++c) P[a][c] = x[c * Ns + g];
+    }
+    for (int q = 0; q < 6; ++q) {
+      const int a = kEdgePairs[q][0], b = kEdgePairs[q][1];
+      const double ex = P[a][0] - P[b][0], ey = P[a][1] - P[b][1], ez = P[a][2] - P[b][2];
+      const double len = sqrt(ex * ex + ey * ey + ez * ez);
+      if (len < m) { m = len; mi = 6L * e + q; }
+    }
+  }
+  double vm = -1.0;
+  long vi = -1;
+  for (int i = t; i < N; i += kCflB) {
+    const double vx = v[i], vy = v[Ns + i], vz = v[2 * Ns + i];
+    const double sp = sqrt(vx * vx + vy * vy + vz * vz);
+    if (sp > vm) { vm = sp; vi = i; }
+  }
+  // (value, index) extremes over the block, the lower index on ties
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool mn = pass == 0;
+    sv[t] = mn ? m : vm;
+    si[t] = mn ? mi : vi;
+    __syncthreads();
+    for (int h = kCflB / 2; h > 0; h >>= 1) {
+      if (t < h) {
+        const double a = sv[t], b = sv[t + h];
+        const long ia = si[t], ib = si[t + h];
+        const bool take = ib >= 0 && (ia < 0 || (mn ? (b < a || (b == a && ib < ia)) : (b > a || (b == a && ib < ia))));
+        if (take) { sv[t] = b; si[t] = ib; }
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+      loc[mn ? 0 : 2] = sv[0];
+      loc[mn ? 1 : 3] = (double)si[0];
+    }
+    __syncthreads();
+  }
+}
+
+// The step length's cotangent (node and element shares summed in a fixed
+// order) passed on through dt = cfl m / (cs + v): dm-bar = dt-bar dt / m to the
+// two nodes of the edge found, dv-bar = -dt-bar dt^2 / (cfl m) to the node found
+// (cs + v = cfl m / dt).
+__global__ __launch_bounds__(kCflB) void a_cfl_apply(int N, int E, double dt, double cfl, const double *dtb,
+                                                      const double *loc, const int *elnod, const double *rec,
+                                                      double *xb, double *vb) {
+  __shared__ double sh[kCflB];
+  const long Ns = N, Es = E;
+  const int t = threadIdx.x;
+  double s = 0.0;
+  for (long k = t; k < Ns + Es; k += kCflB) s += dtb[k];
+  sh[t] = s;
+  __syncthreads();
+  for (int h = kCflB / 2; h > 0; h >>= 1) {
+    if (t < h) sh[t] += sh[t + h];
+    __syncthreads();
+  }
+  if (t != 0) return;
+  const double dtbar = sh[0], m = loc[0], vmax = loc[2];
+  const long mi = (long)loc[1], vi = (long)loc[3];
+  if (mi >= 0 && m > 0.0) {
+    const long e = mi / 6;
+    const int q = (int)(mi % 6);
+    const int ga = elnod[kEdgePairs[q][0] * Es + e], gb = elnod[kEdgePairs[q][1] * Es + e];
+    const double mbar = dtbar * dt / m;
+    for (int c = 0; c < 3; ++c) {
+      const double dx = rec[c * Ns + ga] - rec[c * Ns + gb];
+      xb[c * Ns + ga] += mbar * dx / m;
+      xb[c * Ns + gb] -= mbar * dx / m;
+    }
+  }
+  if (vi >= 0 && vmax > 0.0 && m > 0.0) {
+    const double vbar = -dtbar * dt * dt / (cfl * m);
+    for (int c = 0; c < 3; ++c) vb[c * Ns + vi] += vbar * rec[3 * Ns + c * Ns + vi] / vmax;
+  }
 }
 
 template <int NN>
 __global__ __launch_bounds__(kB) void a_elem1(int E, int N, adj::ElemConst K, const int *elnod,
                                               const double *rec, const double *fib, const double *sy0d,
                                               const double *Hd, const double *rhod, double *taub,
-                                              double *epb, double *contrib, double *volb, double *grad) {
+                                              double *epb, double *contrib, double *volb, double *grad,
+                                              double *dtb_e) {
   int e = (int)tid();
   if (e >= E) return;
   const long Es = E, Ns = N;
@@ -228,6 +346,7 @@ __global__ __launch_bounds__(kB) void a_elem1(int E, int N, adj::ElemConst K, co
     contrib[(a * 7 + 6) * Es + e] = B.r[a];
   }
   volb[e] = B.volb;
+  if (dtb_e) dtb_e[e] = B.dt;
   // rows: the law's two parameters (bilinear sy0, H; Hollomon Kh, mh;
   // Johnson-Cook A, B; GMT C1, n2), hg
   grad[e] += K.law == 1 ? B.Kh : B.sy0;
@@ -356,7 +475,8 @@ static int check_supported(wfx_ctx *c) {
           WFX_ERR_STATE, "the adjoint supports the bilinear, Hollomon, Johnson-Cook and GMT materials");
   REQUIRE((c->prm.mat.model != WFX_JOHNSON_COOK && c->prm.mat.model != WFX_GMT) || !c->temp_e, WFX_ERR_STATE,
           "the Johnson-Cook and GMT adjoints need one uniform temperature (no per-element temperatures)");
-  REQUIRE(!(c->prm.cfl > 0.0), WFX_ERR_STATE, "the adjoint needs a fixed time step (wfx_set_cfl(ctx, 0))");
+  REQUIRE(!(c->prm.cfl > 0.0) || c->precision == WFX_FP64, WFX_ERR_STATE,
+          "the adjoint with the per-step CFL time step needs FP64 (or a fixed time step: wfx_set_cfl(ctx, 0))");
   REQUIRE(!c->thermal.on, WFX_ERR_STATE, "the adjoint does not cover thermal coupling");
   REQUIRE(!c->contact.on, WFX_ERR_STATE, "the adjoint does not cover contact");
   REQUIRE(s.alpha_free == 0.0 && s.log_factor == 0.0 && s.J_min == 0.0 && s.hg_coeff_free == 0.0 &&
@@ -380,7 +500,7 @@ static int adj_alloc(wfx_ctx *c, int nsteps, int every) {
     A->nsteps = nsteps; A->every = every; A->nck = nck; A->ng = ng;
     A->N = (int)N; A->E = (int)E; A->n = (int)n; A->precision = c->precision;
     A->ck_stride = 12 * N + 7 * E;
-    A->slot_stride = 11 * N + 7 * E;
+    A->slot_stride = 14 * N + 7 * E;
     hipError_t err = hipSuccess;
     auto get = [&](double **p, size_t count) {
       if (err != hipSuccess) return;
@@ -396,6 +516,7 @@ static int adj_alloc(wfx_ctx *c, int nsteps, int every) {
     get(&A->taub, 6 * E); get(&A->epb, E); get(&A->fib, 3 * N); get(&A->mb, N); get(&A->w, N);
     get(&A->contrib, 7 * n * E); get(&A->volb, E); get(&A->grad, 3 * E);
     get(&A->sy0d, E); get(&A->Hd, E); get(&A->rhod, E); get(&A->rho_avg, N); get(&A->sums, 8 + E);
+    get(&A->dts, nsteps); get(&A->dtb, N + E); get(&A->cfl_loc, 8);
     if (err == hipSuccess) {
       void *q = nullptr;
       err = hipMalloc(&q, std::max<long>(N, 1) * sizeof(int));
@@ -458,21 +579,25 @@ static hipError_t load_ck(wfx_ctx *c, int idx, hipStream_t st) {
 // values.  As in a fused run, only the run's first step reads the predicted
 // increment (the node pass forms it again from v' and prev_a after that), so
 // a segment recomputed from its checkpoint repeats the forward bit for bit.
+// With the per-step CFL time step (step k >= 0) each step forms its dt on the
+// device as a fused run does, and reads and stores the increment every step.
 template <typename T>
-static hipError_t fused_step(wfx_ctx *c, int predict, bool first, bool last, double *rec, hipStream_t st) {
+static hipError_t fused_step(wfx_ctx *c, int predict, int k, bool last, double *rec, hipStream_t st) {
   T *v, *ud, *pa, *fi;
   node_arrays<T>(c, &v, &ud, &pa, &fi);
   auto &el = el_of<T>(c);
   const long N = c->N, E = c->E;
+  const bool cfl = c->adj->cfl;
   if (rec) {
-    hipLaunchKernelGGL((k_rec_pre<T>), dim3(nb(6 * N + 7 * E)), dim3(kB), 0, st, N, E, c->nd.x, v, el.tau,
-                       el.pl_strain, rec);
+    hipLaunchKernelGGL((k_rec_pre<T>), dim3(nb(9 * N + 7 * E)), dim3(kB), 0, st, N, E, c->nd.x, v, el.tau,
+                       el.pl_strain, pa, rec);
   }
-  c->fused_read_ud = c->fused_run_first = first ? 1 : 0;
+  c->fused_read_ud = (k == 0 || cfl) ? 1 : 0;
+  c->fused_run_first = k == 0 ? 1 : 0;
   c->fused_run_last = last ? 1 : 0;
-  c->fused_write_ud = 0;
+  c->fused_write_ud = cfl ? 1 : 0;
   hipError_t e = hipSuccess;
-  for (int ph = 1; ph <= 4 && e == hipSuccess; ++ph) e = launch_fused_phase<T>(c, ph, predict, st, -1);
+  for (int ph = 1; ph <= 4 && e == hipSuccess; ++ph) e = launch_fused_phase<T>(c, ph, predict, st, cfl ? k : -1);
   c->fused_read_ud = c->fused_write_ud = c->fused_run_first = c->fused_run_last = 1;
   if (e != hipSuccess) return e;
   if (rec) {
@@ -489,11 +614,14 @@ static int adjoint_forward(wfx_ctx *c, int nsteps, bool want_hist, hipStream_t s
   LAUNCH(launch_update_prediction(c, st));
   for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
   LAUNCH(launch_fused_pack<T>(c, st));
+  if constexpr (std::is_same<T, double>::value) {  // (the per-step CFL adjoint is FP64)
+    if (A->cfl) LAUNCH(launch_cfl_begin<T>(c, st));
+  }
   T *v, *ud, *pa, *fi;
   node_arrays<T>(c, &v, &ud, &pa, &fi);
   for (int k = 0; k < nsteps; ++k) {
     if (k % A->every == 0) LAUNCH(save_ck<T>(c, k / A->every, st));
-    LAUNCH(fused_step<T>(c, k + 1 < nsteps, k == 0, k + 1 == nsteps, nullptr, st));
+    LAUNCH(fused_step<T>(c, k + 1 < nsteps, k, k + 1 == nsteps, nullptr, st));
     if (want_hist && c->ngroups > 0) {
       double *h = A->hist + (size_t)k * A->ng;
       if constexpr (std::is_same<T, double>::value) LAUNCH(launch_reaction(c, 2, h, st));
@@ -562,19 +690,25 @@ static int adjoint_backward(wfx_ctx *c, hipStream_t st) {
   for (int seg = A->nck - 1; seg >= 0; --seg) {
     const int k0 = seg * A->every, k1 = std::min(A->nsteps, k0 + A->every);
     LAUNCH(load_ck<T>(c, seg, st));
+    if constexpr (std::is_same<T, double>::value) {  // the first recomputed step's largest speed
+      if (A->cfl) LAUNCH(launch_cfl_begin<T>(c, st));
+    }
     for (int k = k0; k < k1; ++k)
-      LAUNCH(fused_step<T>(c, k + 1 < A->nsteps, k == 0, k + 1 == A->nsteps,
-                           A->slot + (size_t)(k - k0) * A->slot_stride, st));
+      LAUNCH(fused_step<T>(c, k + 1 < A->nsteps, k, k + 1 == A->nsteps, A->slot + (size_t)(k - k0) * A->slot_stride,
+                           st));
     for (int k = k1 - 1; k >= k0; --k) {
       const double *rec = A->slot + (size_t)(k - k0) * A->slot_stride;
-      hipLaunchKernelGGL(a_node1, dim3(nb(N)), dim3(kB), 0, st, N, NP, rec, c->nd.fe, c->bc_slot, c->bc_mask,
-                         A->rgroup, A->hbar + (size_t)k * A->ng, A->xb, A->vb, A->udb, A->pab, A->fib, A->mb);
+      if (A->cfl) NP.dt = K.dt = A->dts_h[k];
+      double *dtb_n = A->cfl ? A->dtb : nullptr, *dtb_e = A->cfl ? A->dtb + N : nullptr;
+      hipLaunchKernelGGL(a_node1, dim3(nb(N)), dim3(kB), 0, st, N, E, NP, rec, c->nd.fe, c->bc_slot, c->bc_mask,
+                         A->rgroup, A->hbar + (size_t)k * A->ng, A->xb, A->vb, A->udb, A->pab, A->fib, A->mb,
+                         dtb_n);
       if (n == 8)
         hipLaunchKernelGGL((a_elem1<8>), dim3(nb(E)), dim3(kB), 0, st, E, N, K, c->ms.elnod, rec, A->fib,
-                           A->sy0d, A->Hd, A->rhod, A->taub, A->epb, A->contrib, A->volb, A->grad);
+                           A->sy0d, A->Hd, A->rhod, A->taub, A->epb, A->contrib, A->volb, A->grad, dtb_e);
       else
         hipLaunchKernelGGL((a_elem1<4>), dim3(nb(E)), dim3(kB), 0, st, E, N, K, c->ms.elnod, rec, A->fib,
-                           A->sy0d, A->Hd, A->rhod, A->taub, A->epb, A->contrib, A->volb, A->grad);
+                           A->sy0d, A->Hd, A->rhod, A->taub, A->epb, A->contrib, A->volb, A->grad, dtb_e);
       hipLaunchKernelGGL(a_node2, dim3(nb(N)), dim3(kB), 0, st, N, E, n, c->ms.nodel_offset, c->ms.nodel_count,
                          c->ms.nodel_pk, A->contrib, c->nd.voln0_sum, A->mb, A->rho_avg, A->w);
       if (n == 8)
@@ -585,6 +719,11 @@ static int adjoint_backward(wfx_ctx *c, hipStream_t st) {
                            A->contrib);
       hipLaunchKernelGGL(a_node3, dim3(nb(N)), dim3(kB), 0, st, N, E, c->ms.nodel_offset, c->ms.nodel_count,
                          c->ms.nodel_pk, A->contrib, A->xb, A->vb);
+      if (A->cfl) {  // the step length's dependence on x and v' at the step's start
+        hipLaunchKernelGGL(a_cfl_locate, dim3(1), dim3(kCflB), 0, st, E, N, c->ms.elnod, rec, A->cfl_loc);
+        hipLaunchKernelGGL(a_cfl_apply, dim3(1), dim3(kCflB), 0, st, N, E, A->dts_h[k], P.cfl, A->dtb, A->cfl_loc,
+                           c->ms.elnod, rec, A->xb, A->vb);
+      }
       LAUNCH(hipGetLastError());
     }
   }
@@ -608,10 +747,25 @@ int wfx_adjoint_forward(wfx_ctx *c, int nsteps, int checkpoint_every, double *re
   rc = adj_alloc(c, nsteps, checkpoint_every);
   if (rc) return rc;
   hipStream_t st = as_stream(s);
+  Adjoint *A = c->adj;
+  A->cfl = c->prm.cfl > 0.0;
+  // each step's dt goes to the adjoint's own record (the per-step CFL reduction writes dt_hist[step])
+  double *hist_keep = c->dt_hist;
+  const int cap_keep = c->dt_cap;
+  c->dt_hist = A->dts;
+  c->dt_cap = nsteps;
   rc = c->precision == WFX_FP64 ? adjoint_forward<double>(c, nsteps, reaction_hist != nullptr, st)
                                 : adjoint_forward<float>(c, nsteps, reaction_hist != nullptr, st);
+  c->dt_hist = hist_keep;
+  c->dt_cap = cap_keep;
   if (rc) return rc;
   c->fused_stale = true;
+  if (A->cfl) {  // the host's dt follows the run's last step, as after a fused run
+    A->dts_h.resize(nsteps);
+    WFX_CHECK_HIP(hipMemcpyAsync(A->dts_h.data(), A->dts, nsteps * sizeof(double), hipMemcpyDeviceToHost, st));
+    WFX_CHECK_HIP(hipStreamSynchronize(st));
+    c->prm.dt = A->dts_h.back();
+  }
   if (reaction_hist) {
     const size_t bytes = (size_t)nsteps * c->adj->ng * sizeof(double);
     if (c->ngroups > 0) WFX_CHECK_HIP(hipMemcpyAsync(reaction_hist, c->adj->hist, bytes, hipMemcpyDeviceToHost, st));
@@ -632,12 +786,21 @@ int wfx_adjoint_backward(wfx_ctx *c, const double *reaction_hist_bar, double *gr
           "mesh changed since wfx_adjoint_forward");
   REQUIRE(c->ngroups > 0, WFX_ERR_STATE, "no reaction nodes: the loss has no input");
   REQUIRE(A->ng == c->ngroups, WFX_ERR_STATE, "reaction groups changed since wfx_adjoint_forward");
+  REQUIRE(A->cfl == (c->prm.cfl > 0.0), WFX_ERR_STATE, "the time-step rule changed since wfx_adjoint_forward");
   int rc = check_supported(c);
   if (rc) return rc;
   hipStream_t st = as_stream(s);
   WFX_CHECK_HIP(hipMemcpyAsync(A->hbar, reaction_hist_bar, (size_t)A->nsteps * A->ng * sizeof(double),
                                hipMemcpyHostToDevice, st));
+  double *hist_keep = c->dt_hist;
+  const int cap_keep = c->dt_cap;
+  c->dt_hist = A->dts;
+  c->dt_cap = A->nsteps;
+  const double dt_keep = c->prm.dt;
   rc = c->precision == WFX_FP64 ? adjoint_backward<double>(c, st) : adjoint_backward<float>(c, st);
+  c->dt_hist = hist_keep;
+  c->dt_cap = cap_keep;
+  c->prm.dt = dt_keep;
   if (rc) return rc;
   WFX_CHECK_HIP(hipMemcpyAsync(grad, A->sums, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
   // per-element gradients: device rows in the stored (cluster) order, handed

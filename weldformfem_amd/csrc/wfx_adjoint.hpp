@@ -90,6 +90,7 @@ struct ElemFwd {
   double f, D[6], W[3];
   // stress
   double p, t1[6], s[6], st, sy, dg, fac, t2[6], ep2, sig[6];
+  double trate[6];  // 2G dev D + R(tau, W): tau1 = tau + dt trate
   double Ht;  // tangent modulus of the return (bilinear H; Hollomon Kh mh s^(mh-1) or 0)
   double hs;  // Hollomon: s = ep + eps0
   int plastic, hpow;  // hpow: Hollomon power branch (s > eps1)
@@ -184,7 +185,8 @@ WFX_HD void elem_forward(const ElemConst &k, const ElemIn<NN> &in, ElemFwd<NN> &
   const double trD = (F.D[0] + F.D[1] + F.D[2]) / 3.0;
   for (int q = 0; q < 6; ++q) {
     const double dd = q < 3 ? F.D[q] - trD : F.D[q];
-    F.t1[q] = t[q] + k.dt * (2.0 * k.G * dd + R[q]);
+    F.trate[q] = 2.0 * k.G * dd + R[q];
+    F.t1[q] = t[q] + k.dt * F.trate[q];
   }
   // trial state and radial return
   const double m = ((F.t1[0] - F.p) + (F.t1[1] - F.p) + (F.t1[2] - F.p)) / 3.0;
@@ -289,6 +291,7 @@ WFX_HD void elem_forward(const ElemConst &k, const ElemIn<NN> &in, ElemFwd<NN> &
 template <int NN>
 struct ElemBar {
   double X[NN][3], V[NN][3], r[NN], tau[6], ep, volb, sy0, H, Kh, mh, hg;
+  double dt;  // the step length's cotangent (through the Jaumann update)
 };
 
 template <int NN>
@@ -423,6 +426,8 @@ WFX_HD void elem_reverse(const ElemConst &k, const ElemIn<NN> &in, const ElemFwd
   const double Wxy = F.W[0], Wyz = F.W[1], Wxz = F.W[2];
   double tb[6], Db[6], Wb[3] = {0, 0, 0};
   for (int q = 0; q < 6; ++q) tb[q] = t1b[q];
+  B.dt = 0.0;
+  for (int q = 0; q < 6; ++q) B.dt += t1b[q] * F.trate[q];
   {
     const double c2 = 2.0 * k.G * k.dt;
     const double mb = (t1b[0] + t1b[1] + t1b[2]) / 3.0;
