@@ -304,7 +304,7 @@ int wfx_graph_launch(wfx_ctx *ctx, double *reaction_hist, wfx_stream s);
  * wfx_step_fused, storing the state every checkpoint_every steps; writes the
  * reaction history [nsteps][ngroups] (may be NULL).  Bilinear, Hollomon,
  * Johnson-Cook or GMT material (the rate laws at one uniform temperature),
- * a fixed time step or (FP64) the per-step CFL time step, the ANP pressure
+ * a fixed time step or the per-step CFL time step, the ANP pressure
  * without stabilisation terms, no thermal coupling or contact.
  * Backward: given dL/dR for that history, recomputes each checkpoint segment
  * and runs the hand-written reverse step over it; grad[3] receives, for the

@@ -475,6 +475,25 @@ def test_cfl_gradient_independent_of_checkpoint_interval(cfl_case):
         np.testing.assert_array_equal(g2, g)
 
 
+def test_cfl_mixed_precision_gradient(cfl_case):
+    pb, target, L, g = cfl_case
+    say("case: per-step CFL time step, FP32-mixed forward with the FP64 adjoint vs the FP64 gradient")
+    q = CompressionProblem(cells=CELLS, size=1.0, v_top=-1.0, nsteps=STEPS, every=EVERY, precision=WFX_FP32_MIXED,
+                           cfl=CFL_FACTOR)
+    L2, g2 = q.loss_and_grad(THETA0, target)
+    err = np.abs(g2 - g) / np.abs(g)
+    say(f"  mixed grad {g2} rel err {err}, loss rel err {abs(L2 - L) / L:.2e}")
+    # sigma_y0 1e-4, hg_coeff 1e-2 and the loss 1e-3 as in the fixed-step
+    # FP32-mixed check; Et 5e-4: its gradient is a quarter of the fixed-step
+    # case's (-1.76e6 against -6.91e6) with the same absolute FP32 deviation
+    # (about 300 in both, measured: 3.9e-5 relative there, 2.1e-4 here)
+    assert err[0] <= 1e-4 and err[1] <= 5e-4 and err[2] <= 1e-2
+    assert abs(L2 - L) <= 1e-3 * L
+    q2 = CompressionProblem(cells=CELLS, size=1.0, v_top=-1.0, nsteps=STEPS, every=7, precision=WFX_FP32_MIXED,
+                            cfl=CFL_FACTOR)
+    np.testing.assert_array_equal(q2.loss_and_grad(THETA0, target)[1], g2)
+
+
 # ---- the identification workload at its own size ---------------------------
 # 30^3 hexa (27,000 elements), 2000 steps, checkpoint every 50, theta0 at the
 # nominal values, target at the seeded hidden parameters.  hg_coeff moves the

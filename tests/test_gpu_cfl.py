@@ -155,11 +155,3 @@ def test_calc_cfl_dt_entry_point():
     assert dt == pytest.approx(want, rel=1e-14)
     assert g.getDT() == dt
 
-
-def test_adjoint_rejects_per_step_dt_in_mixed_precision():
-    # the FP64 adjoint covers the per-step time step (tests/test_gpu_adjoint.py)
-    from weldformfem_amd._lib import WfxError
-
-    g, _ = _pair(_hexa20, precision=WFX_FP32_MIXED)
-    with pytest.raises(WfxError, match="needs FP64"):
-        g.adjointForward(10, checkpoint_every=5)

@@ -40,7 +40,7 @@ class CompressionProblem:
     "johnson_cook": theta = (A, B, hg_coeff) with the constants jc (default
     JC_FIXED); "gmt": theta = (C1, n2, hg_coeff) with the constants gmt
     (default GMT_FIXED).  cfl > 0: the per-step CFL time step with that
-    factor (FP64), starting from the fixed step."""
+    factor, starting from the fixed step."""
 
     def __init__(self, cells=30, size=1.0, v_top=-1.0, nsteps=2000, every=50, precision=WFX_FP64,
                  mat=workloads.STEEL, stream=0, mesh=None, law="bilinear", jc=None, gmt=None, cfl=0.0):

@@ -475,8 +475,6 @@ static int check_supported(wfx_ctx *c) {
           WFX_ERR_STATE, "the adjoint supports the bilinear, Hollomon, Johnson-Cook and GMT materials");
   REQUIRE((c->prm.mat.model != WFX_JOHNSON_COOK && c->prm.mat.model != WFX_GMT) || !c->temp_e, WFX_ERR_STATE,
           "the Johnson-Cook and GMT adjoints need one uniform temperature (no per-element temperatures)");
-  REQUIRE(!(c->prm.cfl > 0.0) || c->precision == WFX_FP64, WFX_ERR_STATE,
-          "the adjoint with the per-step CFL time step needs FP64 (or a fixed time step: wfx_set_cfl(ctx, 0))");
   REQUIRE(!c->thermal.on, WFX_ERR_STATE, "the adjoint does not cover thermal coupling");
   REQUIRE(!c->contact.on, WFX_ERR_STATE, "the adjoint does not cover contact");
   REQUIRE(s.alpha_free == 0.0 && s.log_factor == 0.0 && s.J_min == 0.0 && s.hg_coeff_free == 0.0 &&
@@ -614,9 +612,7 @@ static int adjoint_forward(wfx_ctx *c, int nsteps, bool want_hist, hipStream_t s
   LAUNCH(launch_update_prediction(c, st));
   for (int d = 0; d < 3; ++d) LAUNCH(launch_impose(c, d, false, st));
   LAUNCH(launch_fused_pack<T>(c, st));
-  if constexpr (std::is_same<T, double>::value) {  // (the per-step CFL adjoint is FP64)
-    if (A->cfl) LAUNCH(launch_cfl_begin<T>(c, st));
-  }
+  if (A->cfl) LAUNCH(launch_cfl_begin<T>(c, st, true));
   T *v, *ud, *pa, *fi;
   node_arrays<T>(c, &v, &ud, &pa, &fi);
   for (int k = 0; k < nsteps; ++k) {
@@ -690,9 +686,7 @@ static int adjoint_backward(wfx_ctx *c, hipStream_t st) {
   for (int seg = A->nck - 1; seg >= 0; --seg) {
     const int k0 = seg * A->every, k1 = std::min(A->nsteps, k0 + A->every);
     LAUNCH(load_ck<T>(c, seg, st));
-    if constexpr (std::is_same<T, double>::value) {  // the first recomputed step's largest speed
-      if (A->cfl) LAUNCH(launch_cfl_begin<T>(c, st));
-    }
+    if (A->cfl) LAUNCH(launch_cfl_begin<T>(c, st, true));  // the first recomputed step's largest speed
     for (int k = k0; k < k1; ++k)
       LAUNCH(fused_step<T>(c, k + 1 < A->nsteps, k, k + 1 == A->nsteps, A->slot + (size_t)(k - k0) * A->slot_stride,
                            st));

@@ -13,5 +13,5 @@ template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipStream_t,
 template hipError_t launch_fused_steps<double>(wfx_ctx *, int, double *, hipStream_t, hipEvent_t *);
 template hipError_t launch_fused_pack<double>(wfx_ctx *, hipStream_t);
 template hipError_t launch_fused_unpack<double>(wfx_ctx *, hipStream_t);
-template hipError_t launch_cfl_begin<double>(wfx_ctx *, hipStream_t);
+template hipError_t launch_cfl_begin<double>(wfx_ctx *, hipStream_t, bool);
 }  // namespace wfx

@@ -1725,11 +1725,17 @@ hipError_t launch_fused_phase(wfx_ctx *c, int phase, int predict, hipStream_t s,
 }
 
 // Per-step CFL time step at the start of a run: the predicted velocities'
-// per-block largest speeds for the first step's dt.
+// per-block largest speeds for the first step's dt; packed: a fused run's
+// velocities as its node passes store them (FP32-mixed: the float copy, as
+// the later steps' speeds are taken)
 template <typename T>
-hipError_t launch_cfl_begin(wfx_ctx *c, hipStream_t s) {
-  hipLaunchKernelGGL(k_cfl_max_speed, dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N, c->nd.v,
-                     c->cfl_part + c->cfl_cap_min);
+hipError_t launch_cfl_begin(wfx_ctx *c, hipStream_t s, bool packed) {
+  if (packed && !std::is_same<T, double>::value)
+    hipLaunchKernelGGL((k_cfl_max_speed<float>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N, c->v32,
+                       c->cfl_part + c->cfl_cap_min);
+  else
+    hipLaunchKernelGGL((k_cfl_max_speed<double>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N, c->nd.v,
+                       c->cfl_part + c->cfl_cap_min);
   c->cfl_nvmax = nblocks(c->N);
   return hipGetLastError();
 }
@@ -1738,7 +1744,7 @@ hipError_t launch_cfl_begin(wfx_ctx *c, hipStream_t s) {
 // Jacobians, Solver_explicit.C:576-600) into the dt slot.
 template <typename T>
 hipError_t launch_cfl_step(wfx_ctx *c, int step, hipStream_t s) {
-  hipError_t e = launch_cfl_begin<T>(c, s);
+  hipError_t e = launch_cfl_begin<T>(c, s, false);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_cfl_min_edge, dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E, c->N, c->ms.elnod, c->nd.x,
                      c->cfl_part);
@@ -1803,7 +1809,7 @@ hipError_t launch_fused_steps(wfx_ctx *c, int nsteps, double *hist, hipStream_t 
     if (e != hipSuccess) return e;
   }
   if (c->prm.cfl > 0.0) {
-    hipError_t e = launch_cfl_begin<T>(c, s);
+    hipError_t e = launch_cfl_begin<T>(c, s, true);
     if (e != hipSuccess) return e;
   }
   for (int k = 0; k < nsteps; ++k) {

@@ -13,4 +13,5 @@ template hipError_t launch_thermal_expansion<float>(wfx_ctx *, hipStream_t);
 template hipError_t launch_fused_steps<float>(wfx_ctx *, int, double *, hipStream_t, hipEvent_t *);
 template hipError_t launch_fused_pack<float>(wfx_ctx *, hipStream_t);
 template hipError_t launch_fused_unpack<float>(wfx_ctx *, hipStream_t);
+template hipError_t launch_cfl_begin<float>(wfx_ctx *, hipStream_t, bool);
 }  // namespace wfx

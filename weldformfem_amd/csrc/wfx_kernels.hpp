@@ -97,12 +97,13 @@ __global__ __launch_bounds__(kBlock) void k_cfl_min_edge(int E, int N, const int
 }
 
 // the largest nodal speed |v| (norm(getVelVec(i)))
-__global__ __launch_bounds__(kBlock) void k_cfl_max_speed(int N, const double *__restrict__ v,
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cfl_max_speed(int N, const T *__restrict__ v,
                                                           double *__restrict__ part) {
   const int i = gtid();
   double sp = 0.0;
   if (i < N) {
-    const double vx = v[i], vy = v[(long)N + i], vz = v[2L * N + i];
+    const double vx = (double)v[i], vy = (double)v[(long)N + i], vz = (double)v[2L * N + i];
     sp = sqrt(vx * vx + vy * vy + vz * vz);
   }
   sp = block_extreme<kBlock, false>(sp);

@@ -60,8 +60,9 @@ extern template hipError_t launch_vsum0<float>(wfx_ctx *, hipStream_t);
 extern template hipError_t launch_fused_phase<double>(wfx_ctx *, int, int, hipStream_t, int);
 extern template hipError_t launch_fused_phase<float>(wfx_ctx *, int, int, hipStream_t, int);
 template <typename T> hipError_t launch_fused_pack(wfx_ctx *c, hipStream_t s);
-template <typename T> hipError_t launch_cfl_begin(wfx_ctx *c, hipStream_t s);
-extern template hipError_t launch_cfl_begin<double>(wfx_ctx *, hipStream_t);
+template <typename T> hipError_t launch_cfl_begin(wfx_ctx *c, hipStream_t s, bool packed);
+extern template hipError_t launch_cfl_begin<double>(wfx_ctx *, hipStream_t, bool);
+extern template hipError_t launch_cfl_begin<float>(wfx_ctx *, hipStream_t, bool);
 template <typename T> hipError_t launch_fused_unpack(wfx_ctx *c, hipStream_t s);
 extern template hipError_t launch_fused_pack<double>(wfx_ctx *, hipStream_t);
 extern template hipError_t launch_fused_pack<float>(wfx_ctx *, hipStream_t);
