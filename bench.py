@@ -332,10 +332,13 @@ def config_legs(torch, log):
         bs = d.autotune(reps=5)
         t = timed(lambda: d.runFused(500, reaction=False), torch) / 500
         fb = domain_fused_bytes(d, 8 if prec == WFX_FP64 else 4, hg=False)
+        ph = d.timeFusedPhases(50)
         out[f"cylinder_tets_{tag}"] = {"elements": E, "nodes": N, "ms_per_step": t * 1e3, "value": E / t,
                                        "block_sizes": bs,
                                        "step_algorithmic_bytes_per_element": total(fb, E, N) / E,
-                                       "step_hbm_frac": total(fb, E, N) / t / HBM_PEAK}
+                                       "step_hbm_frac": total(fb, E, N) / t / HBM_PEAK,
+                                       "per_kernel": per_kernel_table(dict(zip(fb, (u * 1e-6 for u in ph))), fb,
+                                                                      E, N)}
         log(f"tet cylinder {tag}: {t * 1e3:.3f} ms/step, {E / t:.3e} element-steps/s")
         del d
     return out
