@@ -305,7 +305,9 @@ int wfx_graph_launch(wfx_ctx *ctx, double *reaction_hist, wfx_stream s);
  * reaction history [nsteps][ngroups] (may be NULL).  Bilinear, Hollomon,
  * Johnson-Cook or GMT material (the rate laws at one uniform temperature),
  * a fixed time step or the per-step CFL time step, the ANP pressure
- * without stabilisation terms, no thermal coupling or contact.
+ * without stabilisation terms, no contact; thermal coupling only one way (a
+ * law that ignores temperature, no thermal expansion: the temperatures
+ * follow the mechanics without acting on them).
  * Backward: given dL/dR for that history, recomputes each checkpoint segment
  * and runs the hand-written reverse step over it; grad[3] receives, for the
  * bilinear law, dL/d(sigma_y0) (summed over elements: a uniform yield

@@ -25,6 +25,7 @@ import pytest
 
 from oracle.wfref import RefDomain
 from weldformfem_amd import Domain, WFX_FP32_MIXED, WFX_FP64, workloads
+from weldformfem_amd._lib import WfxError
 from weldformfem_amd.adjoint import GMT_FIXED, JC_FIXED, CompressionProblem, identify, reaction_history
 
 pytestmark = pytest.mark.gpu
@@ -492,6 +493,27 @@ def test_cfl_mixed_precision_gradient(cfl_case):
     q2 = CompressionProblem(cells=CELLS, size=1.0, v_top=-1.0, nsteps=STEPS, every=7, precision=WFX_FP32_MIXED,
                             cfl=CFL_FACTOR)
     np.testing.assert_array_equal(q2.loss_and_grad(THETA0, target)[1], g2)
+
+
+def test_thermal_one_way_gradient(fp64_case):
+    # with a law that ignores temperature and no thermal expansion the
+    # temperatures follow the mechanics without acting on them: the run with
+    # thermal coupling has the same reaction history and gradient as the one
+    # without (to the rounding of the thermal kernels' own instantiation)
+    pb, target, L, g = fp64_case
+    say("case: bilinear law with thermal coupling (one way): gradient equals the uncoupled one")
+    q = problem()
+    q.domain.setThermal(True, k_T=50.0, cp_T=460.0, plheat_frac=0.9)
+    T0 = q.domain.field("T").copy()
+    L2, g2 = q.loss_and_grad(THETA0, target)
+    assert np.max(np.abs(q.domain.field("T") - T0)) > 0  # the plastic work heated the specimen
+    assert abs(L2 - L) <= 1e-12 * L
+    np.testing.assert_allclose(g2, g, rtol=1e-10)
+    jc = CompressionProblem(cells=CELLS, size=1.0, v_top=-1.0, nsteps=STEPS, every=EVERY, law="johnson_cook",
+                            jc=JC)
+    jc.domain.setThermal(True)
+    with pytest.raises(WfxError, match="ignores temperature"):
+        jc.forward(JC_THETA0)
 
 
 # ---- the identification workload at its own size ---------------------------

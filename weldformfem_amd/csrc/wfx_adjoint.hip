@@ -475,7 +475,13 @@ static int check_supported(wfx_ctx *c) {
           WFX_ERR_STATE, "the adjoint supports the bilinear, Hollomon, Johnson-Cook and GMT materials");
   REQUIRE((c->prm.mat.model != WFX_JOHNSON_COOK && c->prm.mat.model != WFX_GMT) || !c->temp_e, WFX_ERR_STATE,
           "the Johnson-Cook and GMT adjoints need one uniform temperature (no per-element temperatures)");
-  REQUIRE(!c->thermal.on, WFX_ERR_STATE, "the adjoint does not cover thermal coupling");
+  // thermal coupling one way: the temperatures follow the mechanics, which do
+  // not read them when the law ignores temperature and nothing expands, so
+  // the mechanical adjoint is the whole one
+  REQUIRE(!c->thermal.on || ((c->prm.mat.model == WFX_BILINEAR || c->prm.mat.model == WFX_HOLLOMON) &&
+                             c->thermal.exp_T == 0.0),
+          WFX_ERR_STATE,
+          "the adjoint covers thermal coupling only with a law that ignores temperature and no thermal expansion");
   REQUIRE(!c->contact.on, WFX_ERR_STATE, "the adjoint does not cover contact");
   REQUIRE(s.alpha_free == 0.0 && s.log_factor == 0.0 && s.J_min == 0.0 && s.hg_coeff_free == 0.0 &&
               s.pspg_scale == 0.0 && s.p_pspg_bulkfac == 0.0 && s.av_coeff_div == 0.0 && s.av_coeff_bulk == 0.0 &&
