@@ -142,12 +142,16 @@ def test_fused_fp32_mixed():
     check(g, r, T0, 1e-3)
 
 
-def test_adjoint_rejects_thermal():
+def test_adjoint_rejects_two_way_thermal():
+    # the adjoint covers thermal coupling one way only (tests/test_gpu_adjoint.py):
+    # a law that reads temperature, or thermal expansion, is refused
     from weldformfem_amd._lib import WfxError
 
-    g, _, _ = _pair(_hexa16)
-    g.setMaterial(0, 200e9, 0.3, 7850.0, 300e6, 1e9, 0.0)
-    with pytest.raises(WfxError, match="thermal"):
+    g, _, _ = _pair(_hexa16)  # Johnson-Cook
+    with pytest.raises(WfxError, match="ignores temperature"):
+        g.adjointForward(10, checkpoint_every=5)
+    g, _, _ = _pair(_hexa16, exp_T=EXP_T, law="bilinear")
+    with pytest.raises(WfxError, match="thermal expansion"):
         g.adjointForward(10, checkpoint_every=5)
 
 

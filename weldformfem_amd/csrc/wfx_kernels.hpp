@@ -408,20 +408,24 @@ __global__ __launch_bounds__(kBlock) void k_elem_density(int E, const T *__restr
 }
 
 // Walk of node i's CSR entries [o, o + c) in batches of kCsrBatch: the batch's
-// entry indices, then its data loads, are issued independently of each other,
-// and acc(value) runs in CSR order (sums as in a plain loop, bit for bit).
+// entry indices, then its data loads, are all issued (positions past the
+// node's count repeat its last entry: a load behind a branch is waited for on
+// its own), and acc(value) runs over the node's own entries in CSR order (sums
+// as in a plain loop, bit for bit).
 constexpr int kCsrBatch = 8;
 template <class Load, class Acc>
 __device__ __forceinline__ void csr_walk(int o, int c, const int *__restrict__ nodel, Load load, Acc acc) {
   using V = decltype(load(0, 0));
   for (int k = 0; k < c; k += kCsrBatch) {
-    int ent[kCsrBatch];
+    int pos[kCsrBatch], ent[kCsrBatch];
 #pragma unroll
-    for (int j = 0; j < kCsrBatch; ++j) ent[j] = (k + j < c) ? nodel[o + k + j] : -1;
+    for (int j = 0; j < kCsrBatch; ++j) {
+      pos[j] = o + (k + j < c ? k + j : c - 1);
+      ent[j] = nodel[pos[j]];
+    }
     V val[kCsrBatch];
 #pragma unroll
-    for (int j = 0; j < kCsrBatch; ++j)
-      if (ent[j] >= 0) val[j] = load(ent[j], o + k + j);
+    for (int j = 0; j < kCsrBatch; ++j) val[j] = load(ent[j], pos[j]);
 #pragma unroll
     for (int j = 0; j < kCsrBatch; ++j)
       if (k + j < c) acc(val[j]);
