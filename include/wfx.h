@@ -303,7 +303,8 @@ int wfx_graph_launch(wfx_ctx *ctx, double *reaction_hist, wfx_stream s);
 /* Forward: nsteps fused steps from the current state (after wfx_init), as
  * wfx_step_fused, storing the state every checkpoint_every steps; writes the
  * reaction history [nsteps][ngroups] (may be NULL).  Bilinear, Hollomon,
- * Johnson-Cook or GMT material (the rate laws at one uniform temperature),
+ * Johnson-Cook or GMT material (the rate laws at a uniform or per-element
+ * temperature fixed during the run),
  * a fixed time step or the per-step CFL time step, the ANP pressure
  * without stabilisation terms, no contact; thermal coupling only one way (a
  * law that ignores temperature, no thermal expansion: the temperatures

@@ -336,6 +336,51 @@ def test_johnson_cook_gradient_matches_finite_differences(jc_case):
     assert np.all(err <= 1e-5) and np.all(erro <= 1e-5)
 
 
+def test_johnson_cook_per_element_temperatures():
+    # every element at its own fixed temperature, 300 K + 500 K/m times its
+    # centroid height (342..758 K): the reverse takes each element's softening
+    # factor.  n = 0.5 (at 0.26 the cusps B ep^n leaves make the differences
+    # scatter); oracle differences of A and B agree across steps 1e-5 and 1e-6
+    # to 8e-7 and 4e-8.  hg_coeff is not checked: with the temperature field
+    # its differences scatter by 20% across steps 1e-3 .. 1e-4 (kinks of
+    # elements yielding a step earlier or later).
+    say("case: Johnson-Cook at per-element temperatures, d/d(A, B) vs central differences (GPU and oracle)")
+    ref = RefDomain()
+    ref.AddBoxLength([0, 0, 0], [1.0, 1.0, 1.0], 0.5 / CELLS)
+    x, el = ref.field("x").reshape(-1, 3), ref.field("elnod").reshape(-1, 8)
+    temps = 300.0 + 500.0 * x[el, 2].mean(axis=1)
+    jc = dict(JC, n=0.5, temp=temps)
+    pb = CompressionProblem(cells=CELLS, size=1.0, v_top=-1.0, nsteps=STEPS, every=EVERY, law="johnson_cook", jc=jc)
+    target = pb.forward(JC_HIDDEN)
+    L, g = pb.loss_and_grad(JC_THETA0, target)
+
+    def ohist(t):
+        r = RefDomain()
+        r.AddBoxLength([0, 0, 0], [1.0, 1.0, 1.0], 0.5 / CELLS)
+        bottom, top = workloads.box_layers(CELLS, CELLS, CELLS)
+        workloads.setup_compression(r, bottom, top, -1.0, hg_coeff=float(t[2]))
+        m = workloads.STEEL
+        r.setMaterialJC(m["E"], m["nu"], m["rho"], float(t[0]), float(t[1]), jc["n"], jc["C"], jc["eps_0"],
+                        jc["m"], jc["T_m"], jc["T_t"])
+        r.setTemperature(temps)
+        r.init()
+        return r.run(STEPS)
+
+    tgt = ohist(JC_HIDDEN)
+    assert np.max(np.abs(tgt - target)) <= 1e-9 * np.max(np.abs(tgt))
+    rel = [1e-6, 1e-6, 0.0]
+    fd = central_fd(lambda t: gpu_loss(pb, t, target), JC_THETA0, rel, which=(0, 1))
+
+    def oracle_loss(t):
+        r = ohist(t) - tgt
+        return float(np.dot(r, r))
+
+    fdo = central_fd(oracle_loss, JC_THETA0, rel, which=(0, 1))
+    err, erro = np.abs(g[:2] - fd[:2]) / np.abs(fd[:2]), np.abs(g[:2] - fdo[:2]) / np.abs(fdo[:2])
+    say(f"  grad {g} GPU fd {fd[:2]} rel err {err}; oracle fd {fdo[:2]} rel err {erro}")
+    assert np.all(err <= 1e-5) and np.all(erro <= 1e-5)
+
+
 def test_rate_laws_mixed_precision_gradient(jc_case, gmt_case):
     say("case: Johnson-Cook and GMT, FP32-mixed forward with the FP64 adjoint vs the FP64 gradient")
     for law, case, theta0, extra in (("johnson_cook", jc_case, JC_THETA0, dict(jc=JC)),

@@ -57,6 +57,7 @@ struct Adjoint {
   // element shares of its cotangent, and the arg-min edge / arg-max speed
   bool cfl = false;
   double *dts = nullptr, *dtb = nullptr, *cfl_loc = nullptr;
+  double *lawc = nullptr;  // rate laws at per-element temperatures: their constants per element [5][E]
   std::vector<double> dts_h;
 };
 
@@ -302,14 +303,48 @@ __global__ __launch_bounds__(kCflB) void a_cfl_apply(int N, int E, double dt, do
   }
 }
 
+// The rate laws' temperature factors of each element at its own temperature
+// (per-element temperatures, fixed during a run), as the forward forms them:
+// Johnson-Cook 1 - T_h^m (Material.cuh:379-384); GMT at T clamped to
+// [T_min, T_max]: C1 exp(C2 T), exp(C2 T), n1 T + n2, m1 T + m2, I1 T + I2
+// (Material.cuh:441-444, :465-472).  lawc [5][E].
+__global__ void k_law_temps(int E, int law, const double *Te, double T_t, double T_m, double mj, double C1,
+                            double C2, double n1, double n2, double m1, double m2, double I1, double I2,
+                            double T_min, double T_max, double *lawc) {
+  int e = (int)tid();
+  if (e >= E) return;
+  const double T = Te[e];
+  if (law == 2) {
+    lawc[e] = 1.0 - pow((T - T_t) / (T_m - T_t), mj);
+  } else {
+    const double Tc = T < T_min ? T_min : (T > T_max ? T_max : T);
+    lawc[e] = C1 * exp(C2 * Tc);
+    lawc[E + e] = exp(C2 * Tc);
+    lawc[2L * E + e] = n1 * Tc + n2;
+    lawc[3L * E + e] = m1 * Tc + m2;
+    lawc[4L * E + e] = I1 * Tc + I2;
+  }
+}
+
 template <int NN>
 __global__ __launch_bounds__(kB) void a_elem1(int E, int N, adj::ElemConst K, const int *elnod,
                                               const double *rec, const double *fib, const double *sy0d,
                                               const double *Hd, const double *rhod, double *taub,
                                               double *epb, double *contrib, double *volb, double *grad,
-                                              double *dtb_e) {
+                                              double *dtb_e, const double *lawc) {
   int e = (int)tid();
   if (e >= E) return;
+  if (lawc) {  // the element's own temperature factors
+    if (K.law == 2) {
+      K.jsoft = lawc[e];
+    } else {
+      K.ga = lawc[e];
+      K.gexp = lawc[E + e];
+      K.gn = lawc[2L * E + e];
+      K.gm = lawc[3L * E + e];
+      K.gi = lawc[4L * E + e];
+    }
+  }
   const long Es = E, Ns = N;
   const double *x = rec, *v = rec + 3 * Ns, *r = rec + 10 * Ns, *tau = rec + 11 * Ns, *ep = tau + 6 * Es;
   adj::ElemIn<NN> in;
@@ -473,8 +508,6 @@ static int check_supported(wfx_ctx *c) {
   REQUIRE(c->prm.mat.model == WFX_BILINEAR || c->prm.mat.model == WFX_HOLLOMON ||
               c->prm.mat.model == WFX_JOHNSON_COOK || c->prm.mat.model == WFX_GMT,
           WFX_ERR_STATE, "the adjoint supports the bilinear, Hollomon, Johnson-Cook and GMT materials");
-  REQUIRE((c->prm.mat.model != WFX_JOHNSON_COOK && c->prm.mat.model != WFX_GMT) || !c->temp_e, WFX_ERR_STATE,
-          "the Johnson-Cook and GMT adjoints need one uniform temperature (no per-element temperatures)");
   // thermal coupling one way: the temperatures follow the mechanics, which do
   // not read them when the law ignores temperature and nothing expands, so
   // the mechanical adjoint is the whole one
@@ -520,7 +553,7 @@ static int adj_alloc(wfx_ctx *c, int nsteps, int every) {
     get(&A->taub, 6 * E); get(&A->epb, E); get(&A->fib, 3 * N); get(&A->mb, N); get(&A->w, N);
     get(&A->contrib, 7 * n * E); get(&A->volb, E); get(&A->grad, 3 * E);
     get(&A->sy0d, E); get(&A->Hd, E); get(&A->rhod, E); get(&A->rho_avg, N); get(&A->sums, 8 + E);
-    get(&A->dts, nsteps); get(&A->dtb, N + E); get(&A->cfl_loc, 8);
+    get(&A->dts, nsteps); get(&A->dtb, N + E); get(&A->cfl_loc, 8); get(&A->lawc, 5 * E);
     if (err == hipSuccess) {
       void *q = nullptr;
       err = hipMalloc(&q, std::max<long>(N, 1) * sizeof(int));
@@ -688,6 +721,15 @@ static int adjoint_backward(wfx_ctx *c, hipStream_t st) {
     K.ger_max = P.mat.er_max;
     K.jcap = P.mat.max_edot;
   }
+  // per-element temperatures: the rate laws' factors of every element
+  const double *lawc = nullptr;
+  if (c->temp_e && (K.law == 2 || K.law == 3)) {
+    const Material &M = P.mat;
+    hipLaunchKernelGGL(k_law_temps, dim3(nb(E)), dim3(kB), 0, st, E, K.law, c->temp_e, M.T_t, M.T_m, M.m, M.C1,
+                       M.C2, M.n1, M.n2, M.m1, M.m2, M.I1, M.I2, M.T_min, M.T_max, A->lawc);
+    LAUNCH(hipGetLastError());
+    lawc = A->lawc;
+  }
   NodeAdj NP{P.dt, P.alpha, P.beta, P.gamma, c->has_fe, A->ng};
   for (int seg = A->nck - 1; seg >= 0; --seg) {
     const int k0 = seg * A->every, k1 = std::min(A->nsteps, k0 + A->every);
@@ -705,10 +747,10 @@ static int adjoint_backward(wfx_ctx *c, hipStream_t st) {
                          dtb_n);
       if (n == 8)
         hipLaunchKernelGGL((a_elem1<8>), dim3(nb(E)), dim3(kB), 0, st, E, N, K, c->ms.elnod, rec, A->fib,
-                           A->sy0d, A->Hd, A->rhod, A->taub, A->epb, A->contrib, A->volb, A->grad, dtb_e);
+                           A->sy0d, A->Hd, A->rhod, A->taub, A->epb, A->contrib, A->volb, A->grad, dtb_e, lawc);
       else
         hipLaunchKernelGGL((a_elem1<4>), dim3(nb(E)), dim3(kB), 0, st, E, N, K, c->ms.elnod, rec, A->fib,
-                           A->sy0d, A->Hd, A->rhod, A->taub, A->epb, A->contrib, A->volb, A->grad, dtb_e);
+                           A->sy0d, A->Hd, A->rhod, A->taub, A->epb, A->contrib, A->volb, A->grad, dtb_e, lawc);
       hipLaunchKernelGGL(a_node2, dim3(nb(N)), dim3(kB), 0, st, N, E, n, c->ms.nodel_offset, c->ms.nodel_count,
                          c->ms.nodel_pk, A->contrib, c->nd.voln0_sum, A->mb, A->rho_avg, A->w);
       if (n == 8)
