@@ -544,6 +544,16 @@ def test_cfl_gradient_independent_of_checkpoint_interval(cfl_case):
         np.testing.assert_array_equal(g2, g)
 
 
+def test_cfl_identification():
+    say("case: identification of (sigma_y0, Et, hg_coeff) with the per-step CFL time step (200 steps)")
+    pb = CompressionProblem(cells=CELLS, size=1.0, v_top=-1.0, nsteps=STEPS, every=EVERY, cfl=CFL_FACTOR)
+    target = pb.forward(HIDDEN)
+    theta, loss, nit, losses = identify(pb, target, THETA0, scale=np.array([3e8, 1e9, 0.06]), max_iter=100)
+    say(f"  theta {theta} hidden {HIDDEN} loss {loss:.3e} after {nit} iterations")
+    assert loss < 1e-12 * losses[0]
+    np.testing.assert_allclose(theta, HIDDEN, rtol=1e-6)
+
+
 def test_cfl_mixed_precision_gradient(cfl_case):
     pb, target, L, g = cfl_case
     say("case: per-step CFL time step, FP32-mixed forward with the FP64 adjoint vs the FP64 gradient")
