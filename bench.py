@@ -719,6 +719,41 @@ def pmc_summary(fetch_csv, write_csv, probe_log):
     print("\nPMC_TRAFFIC " + json.dumps(out))
 
 
+def sq_summary(csvs):
+    """Markdown table of SQ_* counters of the fused passes from rocprofv3
+    --pmc runs of --pmc-probe: per kernel the median over the inner three of
+    its last five dispatches; SQ_INSTS_* per wavefront, the cycle counters as
+    fractions of SQ_BUSY_CYCLES or SQ_WAVE_CYCLES as named."""
+    import csv
+    import statistics
+
+    vals = {}
+    for path in csvs:
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                name = row["Kernel_Name"]
+                if not any(t in name for t in ("f1_cluster<", "f2_nodal<", "f3_cluster<", "f4_node<")):
+                    continue
+                label = name.replace("void wfx::(anonymous namespace)::", "").split(">(")[0] + ">"
+                vals.setdefault(label, {}).setdefault(row["Counter_Name"], []).append(float(row["Counter_Value"]))
+    counters = sorted({c for d in vals.values() for c in d})
+    lines = ["| kernel | " + " | ".join(counters) + " |", "|---|" + "---:|" * len(counters)]
+    for label, d in vals.items():
+        med = {c: statistics.median(v[-4:-1]) for c, v in d.items() if len(v) >= 4}
+        waves = med.get("SQ_WAVES", 0.0)
+        cells = []
+        for c in counters:
+            v = med.get(c)
+            if v is None:
+                cells.append("")
+            elif c.startswith("SQ_INSTS") and waves > 0:
+                cells.append(f"{v / waves:.1f}/wave")
+            else:
+                cells.append(f"{v:.4g}")
+        lines.append(f"| `{label}` | " + " | ".join(cells) + " |")
+    print("\n".join(lines))
+
+
 def stats_summary(stats_csv, trace_csv, rows=24, window=25):
     """Markdown table of a rocprofv3 --kernel-trace --stats run: the `rows`
     kernels with the most total time, and, per kernel, the mean duration of
@@ -775,7 +810,9 @@ def main():
                     help="print the traffic table of two --pmc-probe counter runs")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short run for counter collection: a 1 GiB device copy (calibration), "
-                         "then 3 fused steps FP64 and FP32-mixed and one per-step API step")
+                         "then a 5-step fused run FP64 and FP32-mixed and one per-step API step")
+    ap.add_argument("--sq-summary", nargs="+", metavar="PMC_CSV",
+                    help="print the per-wavefront SQ counter table of --pmc-probe counter runs")
     ap.add_argument("--stats-summary", nargs=2, metavar=("KERNEL_STATS_CSV", "KERNEL_TRACE_CSV"),
                     help="print the kernel table of a rocprofv3 --kernel-trace --stats run")
     args = ap.parse_args()
@@ -783,6 +820,8 @@ def main():
         return pmc_summary(*args.pmc_summary)
     if args.stats_summary:
         return stats_summary(*args.stats_summary)
+    if args.sq_summary:
+        return sq_summary(args.sq_summary)
     if args.pmc_probe:
         return pmc_probe(args)
     if args.gpus != 1:
