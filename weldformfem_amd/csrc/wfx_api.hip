@@ -248,6 +248,11 @@ static int setup_device(wfx_ctx *c, const ClusterPlan &plan) {
 }
 
 static int finish_mesh(wfx_ctx *c) {
+  // CSR positions are int and packed entries hold (element << 3) | local node
+  if ((int64_t)c->E * c->n > (int64_t)INT32_MAX || (int64_t)c->E >= (int64_t)1 << 28) {
+    set_error("meshes are limited to 2^28 elements and 2^31 element-node entries");
+    return WFX_ERR_ARG;
+  }
   c->nnz = c->E * c->n;
   if (build_nodel(c->N, c->E, c->n, c->elnod_h.data(), c->nodel_h, c->nodel_loc_h, c->nodel_offset_h,
                   c->nodel_count_h)) {
