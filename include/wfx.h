@@ -54,7 +54,9 @@ int wfx_destroy(wfx_ctx *ctx);
 int wfx_add_box_length(wfx_ctx *ctx, const double V[3], const double L[3], double r,
                        int red_int, int tritetra);
 /* Domain_d::CreateFromLSDyna + setNodElem (Domain_d.C:1647-1699, :1508-1611):
- * x[N][3], elnod[E][nodxelem] (nodxelem 4 or 8); builds the node->element CSR. */
+ * x[N][3], elnod[E][nodxelem] (nodxelem 4 or 8); builds the node->element CSR.
+ * Both mesh entry points refuse (WFX_ERR_ARG) meshes of 2^28 elements or more
+ * or of more than 2^31 - 1 element-node entries. */
 int wfx_set_mesh(wfx_ctx *ctx, int node_count, int elem_count, int nodxelem,
                  const double *x, const int *elnod);
 /* getNodeCount / getElemCount / m_nodxelem (Domain_d.h) */
