@@ -62,8 +62,8 @@ def oracle_steps(r, nsteps):
 
 # The 20^3 case made a GPU memory fault in its wfx_init while the per-step node
 # kernels staged their CSR slices in LDS (that variant is removed; the kernels
-# are back to the global-memory walks this test passed with).  No cause was
-# found in the staged code, so the case is kept off the GPU until one is.
+# walk the CSR from global memory, with no LDS).  No cause was found in the
+# staged code, so the case is kept off the GPU until one is.
 _FAULTED = pytest.mark.skip(reason="GPU memory fault in wfx_init with the removed LDS-staged CSR walks; "
                             "cause not found, case kept off the GPU")
 

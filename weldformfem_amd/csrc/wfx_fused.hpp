@@ -1705,7 +1705,7 @@ hipError_t launch_fused_phase_bs(wfx_ctx *c, int phase, int predict, int step, h
     // the nodal volumes no step reads, once per run: the same CSR-order sum
     // and division as CalcNodalVol, so the field equals the per-step API's
     hipLaunchKernelGGL((k_nodal_vol<T>), dim3(nblocks(N)), dim3(kBlock), 0, s, N, c->ms.nodel_offset,
-                       c->ms.nodel_count, c->ms.nodel, el.vol, c->nd.voln, c->n);
+                       c->ms.nodel_count, c->ms.nodel_pad, c->ms.nodel_pk, el.vol, c->nd.voln, c->n);
   } else {
     return hipErrorInvalidValue;
   }

@@ -407,25 +407,30 @@ __global__ __launch_bounds__(kBlock) void k_elem_density(int E, const T *__restr
   rho[e] = rho0[e] * vol0[e] / vol[e];
 }
 
-// Walk of node i's CSR entries [o, o + c) in batches of kCsrBatch: the batch's
-// entry indices, then its data loads, are all issued (positions past the
-// node's count repeat its last entry: a load behind a branch is waited for on
-// its own), and acc(value) runs over the node's own entries in CSR order (sums
-// as in a plain loop, bit for bit).
+// Walk of node i's CSR entries [o, o + c) in batches of kCsrBatch, over the
+// packed entries (element << 3) | local node: the first eight come from the
+// padded table pad[j * N + i] (unit stride across a wavefront, one or two
+// cache lines per load instead of sixteen for a strided CSR read), the rest
+// from the packed CSR.  The batch's entries, then its data loads, are all
+// issued (positions past the node's count repeat its last entry: a load
+// behind a branch is waited for on its own), and acc(value) runs over the
+// node's own entries in CSR order (sums as in a plain loop, bit for bit).
 constexpr int kCsrBatch = 8;
 template <class Load, class Acc>
-__device__ __forceinline__ void csr_walk(int o, int c, const int *__restrict__ nodel, Load load, Acc acc) {
+__device__ __forceinline__ void csr_walk(int i, int N, int o, int c, const int *__restrict__ pad,
+                                         const int *__restrict__ pk, Load load, Acc acc) {
   using V = decltype(load(0, 0));
   for (int k = 0; k < c; k += kCsrBatch) {
-    int pos[kCsrBatch], ent[kCsrBatch];
+    int ent[kCsrBatch];
 #pragma unroll
     for (int j = 0; j < kCsrBatch; ++j) {
-      pos[j] = o + (k + j < c ? k + j : c - 1);
-      ent[j] = nodel[pos[j]];
+      const int q = k + j < c ? k + j : c - 1;
+      const int *src = q < 8 ? pad + (long)q * N + i : pk + o + q;
+      ent[j] = *src;
     }
     V val[kCsrBatch];
 #pragma unroll
-    for (int j = 0; j < kCsrBatch; ++j) val[j] = load(ent[j], pos[j]);
+    for (int j = 0; j < kCsrBatch; ++j) val[j] = load(ent[j] >> 3, ent[j] & 7);
 #pragma unroll
     for (int j = 0; j < kCsrBatch; ++j)
       if (k + j < c) acc(val[j]);
@@ -436,14 +441,15 @@ __device__ __forceinline__ void csr_walk(int o, int c, const int *__restrict__ n
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_nodal_vol(int N, const int *__restrict__ off,
                                                       const int *__restrict__ cnt,
-                                                      const int *__restrict__ nodel,
+                                                      const int *__restrict__ pad,
+                                                      const int *__restrict__ pk,
                                                       const T *__restrict__ vol,
                                                       double *__restrict__ voln, int nodxelem) {
   int i = gtid();
   if (i >= N) return;
   int o = off[i], c = cnt[i];
   double s = 0.0;
-  csr_walk(o, c, nodel, [&](int e, int) { return (double)vol[e]; }, [&](double x) { s += x; });
+  csr_walk(i, N, o, c, pad, pk, [&](int e, int) { return (double)vol[e]; }, [&](double x) { s += x; });
   s /= nodxelem;
   voln[i] = s;
 }
@@ -452,7 +458,8 @@ __global__ __launch_bounds__(kBlock) void k_nodal_vol(int N, const int *__restri
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_nodal_mass(int N, const int *__restrict__ off,
                                                        const int *__restrict__ cnt,
-                                                       const int *__restrict__ nodel,
+                                                       const int *__restrict__ pad,
+                                                       const int *__restrict__ pk,
                                                        const T *__restrict__ rho,
                                                        const double *__restrict__ voln,
                                                        double *__restrict__ mdiag) {
@@ -460,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void k_nodal_mass(int N, const int *__restr
   if (i >= N) return;
   int o = off[i], c = cnt[i];
   double vn = voln[i], dc = (double)c, mass = 0.0;
-  csr_walk(o, c, nodel, [&](int e, int) { return (double)rho[e]; },
+  csr_walk(i, N, o, c, pad, pk, [&](int e, int) { return (double)rho[e]; },
            [&](double r) { mass += 1.0 * r * vn / dc; });
   mdiag[i] = mass;
 }
@@ -503,7 +510,8 @@ __global__ __launch_bounds__(kBlock) void k_strain_rates(int E, int N, const int
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pressure_nodes(int N, const int *__restrict__ off,
                                                            const int *__restrict__ cnt,
-                                                           const int *__restrict__ nodel,
+                                                           const int *__restrict__ pad,
+                                                           const int *__restrict__ pk,
                                                            const T *__restrict__ vol,
                                                            const T *__restrict__ vol0,
                                                            double *__restrict__ ratio) {
@@ -511,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_nodes(int N, const int *__r
   if (i >= N) return;
   int o = off[i], c = cnt[i];
   double s0 = 0.0, s = 0.0;
-  csr_walk(o, c, nodel, [&](int e, int) { return double2{(double)vol0[e], (double)vol[e]}; },
+  csr_walk(i, N, o, c, pad, pk, [&](int e, int) { return double2{(double)vol0[e], (double)vol[e]}; },
            [&](double2 q) {
              s0 += q.x;
              s += q.y;
@@ -1056,8 +1064,8 @@ __global__ __launch_bounds__(kBlock) void k_hourglass(int E, int N, const int *_
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_assembly(int N, int E, const int *__restrict__ off,
                                                      const int *__restrict__ cnt,
-                                                     const int *__restrict__ nodel,
-                                                     const int *__restrict__ nloc,
+                                                     const int *__restrict__ pad,
+                                                     const int *__restrict__ pk,
                                                      const T *__restrict__ fel,
                                                      const T *__restrict__ fhg,
                                                      double *__restrict__ fi,
@@ -1068,18 +1076,18 @@ __global__ __launch_bounds__(kBlock) void k_assembly(int N, int E, const int *__
   int o = off[i], c = cnt[i];
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
   auto row3 = [&](const T *__restrict__ f) {
-    return [=](int e, int k) {
-      const long b = (long)nloc[k] * 3;
+    return [=](int e, int a) {
+      const long b = (long)a * 3;
       return double3{(double)f[(b + 0) * Es + e], (double)f[(b + 1) * Es + e], (double)f[(b + 2) * Es + e]};
     };
   };
-  csr_walk(o, c, nodel, row3(fel), [&](double3 q) {
+  csr_walk(i, N, o, c, pad, pk, row3(fel), [&](double3 q) {
     f0 += q.x;
     f1 += q.y;
     f2 += q.z;
   });
   if (fhg)
-    csr_walk(o, c, nodel, row3(fhg), [&](double3 q) {
+    csr_walk(i, N, o, c, pad, pk, row3(fhg), [&](double3 q) {
       f0 -= q.x;
       f1 -= q.y;
       f2 -= q.z;
@@ -1253,15 +1261,15 @@ template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, d
   template <> hipError_t launch_nodal_vol<T>(wfx_ctx * c, hipStream_t s) {                         \
     auto &el = elem<T>(c);                                                                         \
     hipLaunchKernelGGL((k_nodal_vol<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N,            \
-                       c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel, el.vol, c->nd.voln,     \
-                       c->n);                                                                      \
+                       c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel_pad, c->ms.nodel_pk,     \
+                       el.vol, c->nd.voln, c->n);                                                  \
     return hipGetLastError();                                                                      \
   }                                                                                                \
   template <> hipError_t launch_nodal_mass<T>(wfx_ctx * c, hipStream_t s) {                        \
     auto &el = elem<T>(c);                                                                         \
     hipLaunchKernelGGL((k_nodal_mass<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N,           \
-                       c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel, el.rho, c->nd.voln,     \
-                       c->nd.mdiag);                                                               \
+                       c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel_pad, c->ms.nodel_pk,     \
+                       el.rho, c->nd.voln, c->nd.mdiag);                                           \
     return hipGetLastError();                                                                      \
   }                                                                                                \
   template <> hipError_t launch_strain_rates<T>(wfx_ctx * c, hipStream_t s) {                      \
@@ -1277,8 +1285,8 @@ template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, d
   template <> hipError_t launch_pressure<T>(wfx_ctx * c, hipStream_t s) {                          \
     auto &el = elem<T>(c);                                                                         \
     hipLaunchKernelGGL((k_pressure_nodes<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N,       \
-                       c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel, el.vol, el.vol0,        \
-                       c->nd.anp_ratio);                                                           \
+                       c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel_pad, c->ms.nodel_pk,     \
+                       el.vol, el.vol0, c->nd.anp_ratio);                                          \
     const PressParams pp = press_params(c);                                                        \
     if (c->n == 8)                                                                                 \
       hipLaunchKernelGGL((k_pressure_elem<T, 8>), dim3(nblocks(c->E)), dim3(kBlock), 0, s, c->E,   \
@@ -1331,7 +1339,7 @@ template <typename T> hipError_t launch_energies(wfx_ctx *c, double *dev_out2, d
     auto &el = elem<T>(c);                                                                         \
     const T *fhg = (c->n == 8 && c->prm.stab.hg_coeff != 0.0) ? el.f_hg : nullptr;                 \
     hipLaunchKernelGGL((k_assembly<T>), dim3(nblocks(c->N)), dim3(kBlock), 0, s, c->N, c->E,       \
-                       c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel, c->ms.nodel_loc,        \
+                       c->ms.nodel_offset, c->ms.nodel_count, c->ms.nodel_pad, c->ms.nodel_pk,     \
                        el.f_elem, fhg, c->nd.fi, c->status);                                       \
     return hipGetLastError();                                                                      \
   }                                                                                                \
